@@ -1,0 +1,213 @@
+/* libhfx — MI355X (gfx950) Hartree-Fock Fock-build / direct-minimisation kernels.
+ *
+ * Plain C ABI: caller-owned device pointers (complex data interleaved re/im,
+ * complex128 for HFX_C128 plans and complex64 for HFX_C64 plans; all real
+ * reductions, weights and moments are float64), sizes as ints, a hipStream_t
+ * passed as void*.  Every function returns 0 on success or a negative status
+ * (HFX_EINVAL bad argument, HFX_EHIP runtime error, HFX_EUNSUPPORTED shape);
+ * hfx_last_error() returns the thread-local message of the last failure.
+ * Array layout is the reference's: (nk1, nk2, nb, nb) row-major, centred
+ * meshgrid order; per-k vectors (nk1, nk2, nb); weights (nk1, nk2).
+ *
+ * Each entry point names the reference interface (src/jax_hf/...) it replaces.
+ */
+#ifndef HFX_H
+#define HFX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HFX_C128 0
+#define HFX_C64 1
+
+typedef struct hfx_plan hfx_plan;
+
+/* ABI version of this header (bumped on any signature change). */
+int hfx_abi_version(void);
+
+/* Copies the calling thread's last error message into buf (NUL-terminated). */
+int hfx_last_error(char* buf, size_t n);
+
+/* Plan for an (nk1, nk2) mesh with nb bands in the given storage precision.
+ * Uploads the FFT twiddle tables and the full / upper-triangle channel lists.
+ * Replaces the shape-keyed jit cache of _solve_jitted (solver.py:663-683). */
+int hfx_plan_create(int nk1, int nk2, int nb, int dtype, hfx_plan** out);
+int hfx_plan_destroy(hfx_plan* plan);
+
+/* Bytes of caller-provided float64 scratch needed by the energy reductions. */
+size_t hfx_energy_scratch_bytes(const hfx_plan* plan);
+
+/* Exchange self-energy on a channel list (selfenergy_fft, utils.py:66-173 and
+ * the block-spec subsets utils.py:176-311):
+ *   Sigma_c = -IFFT2[FFT2(P_c - refP_c) * VR_c],  VR phase-absorbed (problem.py:88-98).
+ * chans == NULL selects the plan's full list (herm = 0) or upper triangle
+ * (herm = 1, lower triangle written as the conjugate mirror, utils.py:140-161).
+ * ifftshift = 1 applies the final ifftshift of selfenergy_fft (utils.py:135-136).
+ * vr_full = 1 for VR of shape (nk1, nk2, nb, nb), 0 for (nk1, nk2, 1, 1).
+ * rho (optional, complex128 nb*nb) receives sum_k w_k (P - refP)_c. */
+int hfx_selfenergy(const hfx_plan* plan, const void* P, const void* refP, const void* VR, int vr_full,
+                   const int* chans, int nchan, int herm, int ifftshift, const double* w2d, void* Sigma,
+                   void* rho, void* stream);
+
+/* VR = fftn(w[..,None,None] * Vq, axes=(0,1)) (problem.py:88), times the
+ * ifftshift phase exp(2 pi i (n0 s0/N0 + n1 s1/N1)) when phase = 1
+ * (problem.py:90-98).  Vq complex in the plan precision, (nk1, nk2, nb, nb);
+ * a scalar (nk1, nk2, 1, 1) Vq uses a plan created with nb = 1. */
+int hfx_prepare_vr(const hfx_plan* plan, const void* Vq, int vr_full, const double* w2d, int phase,
+                   void* VR, void* stream);
+
+/* Weighted channel moments rho = sum_k w_k (P - refP) (fock.py:66-70, 79). */
+int hfx_density_moments(const hfx_plan* plan, const void* P, const void* refP, const double* w2d,
+                        void* rho, void* stream);
+
+/* k-independent Hartree diagonal H_a = sum_b HH_ab Re rho_bb (fock.py:66-74) and
+ * contact self-energy sum_t g_t [O_i tr(O_j rho) - O_i rho O_j] (fock.py:76-92).
+ * HH, Hdiag or (g, Oi, Oj, sigc) may be NULL to skip that term. */
+int hfx_kterms(const hfx_plan* plan, const void* rho, const double* HH, int ntc, const double* g,
+               const void* Oi, const void* Oj, double* Hdiag, void* sigc, void* stream);
+
+/* Fock epilogue of build_fock (fock.py:94) fused with hf_energy (fock.py:109-111):
+ *   F = herm(h + Sigma + sigc + diag(Hdiag)),  Sigma_out = Sigma + sigc,
+ *   E_out = sum_k w_k Re Tr[(h + (Sigma_out + diag(Hdiag))/2) P] + e_offset.
+ * Sigma, Hdiag, sigc, Sigma_out, P/E_out optional (NULL).  Sigma_out must not
+ * alias Sigma when sigc is given. scratch: hfx_energy_scratch_bytes(). */
+int hfx_fock_epilogue(const hfx_plan* plan, const void* h, const void* Sigma, const double* Hdiag,
+                      const void* sigc, const void* P, const double* w2d, void* F, void* Sigma_out,
+                      double* scratch, double* E_out, double e_offset, void* stream);
+
+/* hf_energy (fock.py:100-111) on given Sigma / H; H broadcast (1,1,nb,nb) when
+ * H_bcast = 1, full (nk1,nk2,nb,nb) when 0, absent when NULL. */
+int hfx_hf_energy(const hfx_plan* plan, const void* P, const void* h, const void* Sigma, const void* H,
+                  int H_bcast, const double* w2d, double* scratch, double* E_out, void* stream);
+
+/* ---- utilities ---------------------------------------------------------- */
+#define HFX_KIND_F64 0
+#define HFX_KIND_F32 1
+#define HFX_KIND_C128 2
+#define HFX_KIND_C64 3
+
+/* Elementwise precision conversion of n values between storage kinds
+ * (the mixed-precision plumbing of the precision form; real -> complex sets
+ * im = 0, complex -> real keeps the real part). */
+int hfx_convert(const void* src, int src_kind, void* dst, int dst_kind, size_t n, void* stream);
+
+/* out = a X + b Y elementwise (a = ar + i ai, b = br + i bi; Y may be NULL);
+ * the density mixing / DIIS extrapolation / Steihaug vector updates of
+ * reference_scf.py:275-339 and solver.py:947-989. out may alias X or Y. */
+int hfx_lincomb(size_t n, int kind, double ar, double ai, const void* X, double br, double bi, const void* Y,
+                void* out, void* stream);
+
+/* out = 1 / (1 + exp((x - mu) / (T + 1e-12))), mu = *mu or 0 (fermidirac, utils.py:51-53). */
+int hfx_fermi(const double* x, const double* mu, double T, size_t n, double* out, void* stream);
+
+/* Y = 0.5 (X + X^dagger) over nmat nb x nb matrices (fock.py:13-15, 95-96). */
+int hfx_hermitize(const void* X, void* Y, size_t nmat, int nb, int dtype, void* stream);
+
+/* out[0] = max|X|, out[1] = max|X| over channels with mask[c] != 0; the
+ * off-block check of the block-spec exchange (utils.py:263-266, 282, 295).
+ * scratch: >= 1024 doubles. */
+int hfx_absmax_masked(const void* X, size_t nmat, int nb, int dtype, const unsigned char* mask,
+                      double* scratch, double* out, void* stream);
+
+/* occupation_entropy (fock.py:114-118) of p (nk, nb) with weights w (nk);
+ * with E (device scalar) non-NULL writes free_energy E - max(T,1e-14) S
+ * (fock.py:121-129) instead.  real_kind: HFX_KIND_F64 or HFX_KIND_F32. */
+int hfx_entropy(const void* p, int real_kind, const double* w, size_t nk, int nb, const double* E, double T,
+                double* scratch, double* out, void* stream);
+
+/* ---- per-k batched kernels (nb <= 16; one wavefront per k-point) --------- */
+/* Device scalar state of the solver loops: float64[HFX_ST_LEN]. */
+#define HFX_ST_MU 0
+#define HFX_ST_BETA 1
+#define HFX_ST_LAM 2
+#define HFX_ST_GRAD 3
+#define HFX_ST_TAU0 4
+#define HFX_ST_TAU 5
+#define HFX_ST_TAUF 6
+#define HFX_ST_ACC 7
+#define HFX_ST_OMEGA0 8
+#define HFX_ST_DNORM 9
+#define HFX_ST_OMEGA 10
+#define HFX_ST_LEN 16
+
+/* Batched Hermitian eigensolver (jnp.linalg.eigh call sites, solver.py:165,
+ * 389, 623, 842; reference_scf.py:89): cyclic round-robin Jacobi, eigenvalues
+ * ascending in lam (nk, nb), eigenvectors as columns of V.  herm_in = 1
+ * hermitises A first.  dtype: HFX_C128 / HFX_C64. */
+int hfx_eigh_batched(int dtype, int nb, size_t nk, const void* A, double* lam, void* V, int herm_in,
+                     void* stream);
+
+/* P = herm(Q diag(p) Q^dagger) (solver.py:42-44 + 462, utils.py:473-474);
+ * when p is NULL, p = sigmoid((*mu - eps)/T) is formed (and stored to pout). */
+int hfx_density_from_Qp(int dtype, int nb, size_t nk, const void* Q, const double* p, const double* eps,
+                        const double* mu, double T, double* pout, void* P, void* stream);
+
+/* CG iteration pieces (solver.py:451-600); part arrays are (nval, nk) float64. */
+int hfx_cg_grad(int dtype, int nb, size_t nk, const void* Q, const void* F, const double* p, void* Ft,
+                double* eps, void* G, double* part, void* stream);
+int hfx_cg_precond(int dtype, int nb, size_t nk, const void* G, const void* Gprev, const double* eps,
+                   const double* p, const double* gp_prev, const double* st, double T, void* H, double* gp,
+                   double* hp, double* part, void* stream);
+int hfx_cg_direction(int dtype, int nb, size_t nk, const void* H, void* D, const double* hp, double* dp,
+                     const void* Ft, const double* st, void* V, double* lam_d, void* Fe, double* part,
+                     void* stream);
+int hfx_cg_trial(int dtype, int nb, size_t nk, const void* V, const double* lam_d, const void* Fe,
+                 const double* p, const double* dp, const double* eps, const double* st, int omega0,
+                 double* part, void* stream);
+/* Q <- Q V diag(c(tau)) V^dagger (solver.py:586-590); tau = st[HFX_ST_TAUF]
+ * unless use_tau; eps_new (spectral diag, Fe = V^dagger Ft V) and/or
+ * Ft_out = U^dagger Fe U (Fe = Ft, solver.py:1005-1009) optional. */
+int hfx_cg_retract(int dtype, int nb, size_t nk, void* Q, const void* V, const double* lam_d, const void* Fe,
+                   const double* st, double tau, int use_tau, double* eps_new, void* Ft_out, void* stream);
+/* Q <- Q W with (eps, W) = eigh(Q^dagger F Q)  (solver.py:622-625). */
+int hfx_rotate_eigh(int dtype, int nb, size_t nk, void* Q, const void* F, double* eps, void* stream);
+/* out = opA(A) opB(B - Bsub) [opC(C)] per k, optionally hermitised; ops bit0/1/2
+ * = conjugate-transpose A/B/C (the einsum products of solver.py:42-49, 918-924). */
+int hfx_matmul_batched(int dtype, int nb, size_t nk, const void* A, const void* B, const void* C, int ops,
+                       const void* Bsub, int herm_out, void* out, void* stream);
+
+/* SCF residuals (reference_scf.py:291-294): comm = F D - D F (stored if comm),
+ * part[0][k] = sum |comm|^2, part[1][k] = sum |Pn - D|^2 (Pn may be NULL). */
+int hfx_scf_resid(int dtype, int nb, size_t nk, const void* F, const void* D, const void* Pn, void* comm,
+                  double* part, void* stream);
+/* part[v][k] = Re sum conj(X) Y_v (mode 0, weighted inner products solver.py:205-208 and the
+ * DIIS B matrix reference_scf.py:110-111) or Re tr(X Y_v) (mode 1, reference_scf.py:319);
+ * Ys is a HOST array of nv <= 8 device pointers. */
+int hfx_batched_dots(int dtype, int nb, size_t nk, const void* X, const void* const* Ys, int nv, int mode,
+                     double* part, void* stream);
+
+/* ---- reductions and scalar steps ------------------------------------------ */
+#define HFX_OP_SUM 0
+#define HFX_OP_GRAD 1
+#define HFX_OP_BETA 2
+#define HFX_OP_TAU0 3
+#define HFX_OP_TRIAL 4
+/* Scratch (float64) for hfx_reduce / hfx_solve_mu. */
+size_t hfx_solver_scratch_doubles(void);
+/* out[v] = sum_k (w[k] if bit v of wmask) part[v][k] in a fixed order; then the
+ * scalar step `op` on st.  params: {T, denom_scale, count, max_step, shrink,
+ * tau_fallback, restart, last, omega0}. */
+int hfx_reduce(const double* part, int nval, size_t nk, const double* w, unsigned wmask, double* out,
+               double* st, int op, const double* params, double* scratch, void* stream);
+
+#define HFX_MU_NEWTON_BRACKET 0
+#define HFX_MU_BISECTION 1
+#define HFX_MU_NEWTON 2
+/* Chemical potential in one cooperative launch: Newton with bracket
+ * (solver.py:62-101), safeguarded Newton (utils.py:385-442) or bisection
+ * (utils.py:353-382), the latter twice with
+ * the level shift of reference_scf.py:87-96 when level_shift != 0.  Result to
+ * st[HFX_ST_MU] and/or mu_out; occupations to p_out (sigmoid((mu-eps)/T), or
+ * fermidirac of the shifted levels). */
+int hfx_solve_mu(const double* eps, const double* w, size_t nk, int nb, double target, double T, double mu0,
+                 int mu0_from_state, int maxiter, int mode, double level_shift, double* st, double* mu_out,
+                 double* p_out, double* scratch, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HFX_H */

@@ -1,0 +1,171 @@
+"""ctypes binding of libhfx.so (the HIP kernels) and per-shape plan cache.
+
+The library is loaded from this package directory only; a missing or stale
+library raises immediately (there is no CPU fallback in the product path).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+from pathlib import Path
+
+import torch
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libhfx.so"
+
+HFX_C128 = 0
+HFX_C64 = 1
+ABI_VERSION = 2
+
+_lib = None
+_lock = threading.Lock()
+
+vp = C.c_void_p
+i32 = C.c_int
+f64 = C.c_double
+sz = C.c_size_t
+
+# name -> argtypes (all functions return int status unless listed in _RESTYPES)
+_SIGS = {
+    "hfx_abi_version": [],
+    "hfx_last_error": [C.c_char_p, sz],
+    "hfx_plan_create": [i32, i32, i32, i32, C.POINTER(vp)],
+    "hfx_plan_destroy": [vp],
+    "hfx_energy_scratch_bytes": [vp],
+    "hfx_selfenergy": [vp, vp, vp, vp, i32, vp, i32, i32, i32, vp, vp, vp, vp],
+    "hfx_prepare_vr": [vp, vp, i32, vp, i32, vp, vp],
+    "hfx_density_moments": [vp, vp, vp, vp, vp, vp],
+    "hfx_kterms": [vp, vp, vp, i32, vp, vp, vp, vp, vp, vp],
+    "hfx_fock_epilogue": [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, f64, vp],
+    "hfx_hf_energy": [vp, vp, vp, vp, vp, i32, vp, vp, vp, vp],
+    "hfx_convert": [vp, i32, vp, i32, sz, vp],
+    "hfx_hermitize": [vp, vp, sz, i32, i32, vp],
+    "hfx_lincomb": [sz, i32, f64, f64, vp, f64, f64, vp, vp, vp],
+    "hfx_fermi": [vp, vp, f64, sz, vp, vp],
+    "hfx_absmax_masked": [vp, sz, i32, i32, vp, vp, vp, vp],
+    "hfx_entropy": [vp, i32, vp, sz, i32, vp, f64, vp, vp, vp],
+    "hfx_eigh_batched": [i32, i32, sz, vp, vp, vp, i32, vp],
+    "hfx_density_from_Qp": [i32, i32, sz, vp, vp, vp, vp, f64, vp, vp, vp],
+    "hfx_cg_grad": [i32, i32, sz, vp, vp, vp, vp, vp, vp, vp, vp],
+    "hfx_cg_precond": [i32, i32, sz, vp, vp, vp, vp, vp, vp, f64, vp, vp, vp, vp, vp],
+    "hfx_cg_direction": [i32, i32, sz, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
+    "hfx_cg_trial": [i32, i32, sz, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp],
+    "hfx_cg_retract": [i32, i32, sz, vp, vp, vp, vp, vp, f64, i32, vp, vp, vp],
+    "hfx_rotate_eigh": [i32, i32, sz, vp, vp, vp, vp],
+    "hfx_matmul_batched": [i32, i32, sz, vp, vp, vp, i32, vp, i32, vp, vp],
+    "hfx_scf_resid": [i32, i32, sz, vp, vp, vp, vp, vp, vp],
+    "hfx_batched_dots": [i32, i32, sz, vp, vp, i32, i32, vp, vp],
+    "hfx_solver_scratch_doubles": [],
+    "hfx_reduce": [vp, i32, sz, vp, C.c_uint, vp, vp, i32, vp, vp, vp],
+    "hfx_solve_mu": [vp, vp, sz, i32, f64, f64, f64, i32, i32, i32, f64, vp, vp, vp, vp, vp],
+}
+_RESTYPES = {"hfx_energy_scratch_bytes": sz, "hfx_solver_scratch_doubles": sz}
+
+
+class HfxError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded library (raises if it was never built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is missing: build the HIP library first "
+                "(python -m jax_hf_amd.build or __graft_entry__.build())")
+        L = C.CDLL(str(LIB_PATH))
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPES.get(name, i32)
+        v = L.hfx_abi_version()
+        if v != ABI_VERSION:
+            raise ImportError(f"libhfx ABI version {v} != expected {ABI_VERSION}; rebuild")
+        _lib = L
+        return L
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(1024)
+    lib().hfx_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc: int, what: str):
+    if rc == 0:
+        return
+    msg = last_error()
+    if rc == -1:
+        raise ValueError(f"{what}: {msg}")
+    raise HfxError(f"{what} failed ({rc}): {msg}")
+
+
+def call(name: str, *args):
+    rc = getattr(lib(), name)(*args)
+    check(rc, name)
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a contiguous tensor (None passes NULL)."""
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise ValueError("internal: non-contiguous tensor passed to libhfx")
+    return t.data_ptr()
+
+
+def stream(device=None) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+class Plan:
+    """Owns an hfx_plan (twiddles + channel lists) for one (nk1, nk2, nb, dtype)."""
+
+    def __init__(self, nk1: int, nk2: int, nb: int, dtype: int, device):
+        self.nk1, self.nk2, self.nb, self.dtype = nk1, nk2, nb, dtype
+        self.device = device
+        h = vp()
+        with torch.cuda.device(device):
+            call("hfx_plan_create", nk1, nk2, nb, dtype, C.byref(h))
+        self.handle = h
+        nbytes = lib().hfx_energy_scratch_bytes(h)
+        self.scratch = torch.empty(nbytes // 8, dtype=torch.float64, device=device)
+
+    def __del__(self):
+        try:
+            if self.handle and _lib is not None:
+                _lib.hfx_plan_destroy(self.handle)
+        except Exception:  # pragma: no cover
+            pass
+
+
+_plans: dict = {}
+
+
+def plan_for(nk1: int, nk2: int, nb: int, cdtype: torch.dtype, device) -> Plan:
+    dt = HFX_C128 if cdtype == torch.complex128 else HFX_C64
+    dev = torch.device(device)
+    key = (nk1, nk2, nb, dt, dev.index)
+    p = _plans.get(key)
+    if p is None:
+        p = Plan(nk1, nk2, nb, dt, dev)
+        _plans[key] = p
+    return p
+
+
+def require_gpu(device=None):
+    if not torch.cuda.is_available():
+        raise HfxError("jax_hf_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    lib()

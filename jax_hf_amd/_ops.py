@@ -1,0 +1,118 @@
+"""Thin typed wrappers over the per-k / reduction entry points of libhfx."""
+
+from __future__ import annotations
+
+import torch
+
+from . import _lib as L
+
+ST_MU, ST_BETA, ST_LAM, ST_GRAD, ST_TAU0, ST_TAU, ST_TAUF, ST_ACC = 0, 1, 2, 3, 4, 5, 6, 7
+ST_OMEGA0, ST_DNORM, ST_OMEGA, ST_E = 8, 9, 10, 11
+ST_LEN = 16
+OP_SUM, OP_GRAD, OP_BETA, OP_TAU0, OP_TRIAL = 0, 1, 2, 3, 4
+MU_NEWTON_BRACKET, MU_BISECTION, MU_NEWTON = 0, 1, 2
+
+
+def dt_code(t: torch.Tensor) -> int:
+    return L.HFX_C128 if t.dtype == torch.complex128 else L.HFX_C64
+
+
+class Workspace:
+    """Device scratch shared by the reductions / mu solver on one device."""
+
+    _cache: dict = {}
+
+    @classmethod
+    def get(cls, device, nk: int):
+        key = (torch.device(device).index, nk)
+        w = cls._cache.get(key)
+        if w is None:
+            w = cls(device, nk)
+            cls._cache[key] = w
+        return w
+
+    def __init__(self, device, nk: int):
+        n = int(L.lib().hfx_solver_scratch_doubles())
+        self.scratch = torch.zeros(n, dtype=torch.float64, device=device)
+        self.part = torch.zeros(8 * nk, dtype=torch.float64, device=device)
+        self.out = torch.zeros(16, dtype=torch.float64, device=device)
+
+
+def s(dev):
+    return L.stream(dev)
+
+
+def eigh(A, lam, V, herm_in=False):
+    nk = A.shape[0] * A.shape[1] if A.dim() == 4 else A.shape[0]
+    L.call("hfx_eigh_batched", dt_code(A), A.shape[-1], nk, L.ptr(A), L.ptr(lam), L.ptr(V), int(herm_in),
+           s(A.device))
+
+
+def density(Q, P, p=None, eps=None, mu_ptr=None, T=1.0, pout=None):
+    nk = Q.numel() // (Q.shape[-1] ** 2)
+    L.call("hfx_density_from_Qp", dt_code(Q), Q.shape[-1], nk, L.ptr(Q), L.ptr(p), L.ptr(eps), mu_ptr,
+           float(T), L.ptr(pout), L.ptr(P), s(Q.device))
+
+
+def matmul(A, B, out, C=None, ops=0, Bsub=None, herm_out=False):
+    nk = A.numel() // (A.shape[-1] ** 2)
+    L.call("hfx_matmul_batched", dt_code(A), A.shape[-1], nk, L.ptr(A), L.ptr(B), L.ptr(C), int(ops),
+           L.ptr(Bsub), int(herm_out), L.ptr(out), s(A.device))
+
+
+def hermitize(X, Y):
+    nb = X.shape[-1]
+    L.call("hfx_hermitize", L.ptr(X), L.ptr(Y), X.numel() // (nb * nb), nb, dt_code(X), s(X.device))
+
+
+def reduce(ws: Workspace, part, nval, nk, w=None, wmask=0, out=None, st=None, op=OP_SUM, params=None):
+    par = None if params is None else _params_buf(ws, params)
+    L.call("hfx_reduce", L.ptr(part), int(nval), int(nk), L.ptr(w), int(wmask), L.ptr(out), L.ptr(st), int(op),
+           None if par is None else par, L.ptr(ws.scratch), s(part.device))
+
+
+_param_bufs: dict = {}
+
+
+def _params_buf(ws, params):
+    """Host-side parameter vector for hfx_reduce (read by the launcher, not the kernel)."""
+    import ctypes as C
+    vals = list(params) + [0.0] * (9 - len(params))
+    arr = (C.c_double * 9)(*[float(v) for v in vals])
+    _param_bufs[id(ws)] = arr          # keep alive for the call
+    return C.cast(arr, C.c_void_p)
+
+
+def solve_mu(ws: Workspace, eps, w, nk, nb, target, T, mu0=0.0, mu0_from_state=False, maxiter=25,
+             mode=MU_NEWTON_BRACKET, level_shift=0.0, st=None, mu_out_ptr=None, p_out=None):
+    L.call("hfx_solve_mu", L.ptr(eps), L.ptr(w), int(nk), int(nb), float(target), float(T), float(mu0),
+           int(mu0_from_state), int(maxiter), int(mode), float(level_shift), L.ptr(st), mu_out_ptr,
+           L.ptr(p_out), L.ptr(ws.scratch), s(eps.device))
+
+
+def st_ptr(st, idx):
+    return st.data_ptr() + 8 * idx
+
+
+def scf_resid(F, D, Pn, comm, part):
+    nb = F.shape[-1]
+    nk = F.numel() // (nb * nb)
+    L.call("hfx_scf_resid", dt_code(F), nb, nk, L.ptr(F), L.ptr(D), L.ptr(Pn), L.ptr(comm), L.ptr(part),
+           s(F.device))
+
+
+def dots(X, Ys, part, mode=0):
+    import ctypes as C
+    nb = X.shape[-1]
+    nk = X.numel() // (nb * nb)
+    arr = (C.c_void_p * len(Ys))(*[y.data_ptr() for y in Ys])
+    L.call("hfx_batched_dots", dt_code(X), nb, nk, L.ptr(X), C.cast(arr, C.c_void_p), len(Ys), int(mode),
+           L.ptr(part), s(X.device))
+
+
+def lincomb(out, X, a=1.0, Y=None, b=0.0):
+    """out = a X + b Y (a, b real or complex)."""
+    from ._util import KIND
+    a, b = complex(a), complex(b)
+    L.call("hfx_lincomb", X.numel(), KIND[X.dtype], a.real, a.imag, L.ptr(X), b.real, b.imag, L.ptr(Y),
+           L.ptr(out), s(X.device))

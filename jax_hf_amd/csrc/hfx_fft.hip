@@ -1,0 +1,441 @@
+// Exchange self-energy as a fused 2D FFT convolution over the k-mesh.
+//
+//   Sigma_c(k) = -IFFT2[ FFT2(P_c - refP_c) * VR_c ](k)        (one channel c = (i,j))
+//
+// Semantics follow src/jax_hf/utils.py:140-173 with the ifftshift phase already
+// absorbed into VR (src/jax_hf/problem.py:88-98).  One workgroup owns one
+// channel's whole (n1 x n2) plane in LDS: load (strided gather from the
+// (n1, n2, nb, nb) layout, fused refP subtraction and weighted channel moment),
+// forward row + column passes, VR multiply with the -1/(n1 n2) inverse scale
+// folded in, inverse column + row passes, store (fused conjugate mirror in the
+// Hermitian-packed mode).  HBM traffic is P + VR in, Sigma out: nothing else.
+//
+// Each 1D transform is a mixed-radix Stockham autosort done in place in LDS:
+// every thread first pulls all the butterfly inputs it owns into registers,
+// the workgroup synchronises, then every thread writes its outputs.  Twiddles
+// come from per-axis tables W_N^m = exp(-2 pi i m / N) computed on the host in
+// extended precision (so the FP64 path is exact to rounding).
+#include "hfx_common.h"
+#include "hfx_internal.h"
+
+namespace hfx {
+
+// Math constants of the small DFT kernels (cos / sin of 2 pi q / R).
+#define HFX_S3 0.86602540378443864676   // sin(2 pi / 3) = sqrt(3)/2
+#define HFX_C5A 0.30901699437494742410  // cos(2 pi / 5)
+#define HFX_C5B -0.80901699437494742410 // cos(4 pi / 5)
+#define HFX_S5A 0.95105651629515357212  // sin(2 pi / 5)
+#define HFX_S5B 0.58778525229247312917  // sin(4 pi / 5)
+#define HFX_R2 0.70710678118654752440   // sqrt(1/2)
+
+// In-place R-point DFT of v[0..R), forward sign exp(-2 pi i rm/R) unless inv.
+template <typename T, int R>
+__device__ __forceinline__ void small_dft(cx<T> (&v)[R], bool inv, const cx<T>* __restrict__ tw, int N) {
+  const bool fwd = !inv;
+  if constexpr (R == 2) {
+    cx<T> a = v[0], b = v[1];
+    v[0] = a + b;
+    v[1] = a - b;
+  } else if constexpr (R == 3) {
+    const T s = (T)HFX_S3;
+    cx<T> t1 = v[1] + v[2];
+    cx<T> t2 = v[0] - (T)0.5 * t1;
+    cx<T> t3 = s * rot(v[1] - v[2], fwd);  // -i s (v1 - v2) forward
+    v[0] = v[0] + t1;
+    v[1] = t2 + t3;
+    v[2] = t2 - t3;
+  } else if constexpr (R == 4) {
+    cx<T> a = v[0] + v[2], b = v[0] - v[2];
+    cx<T> c = v[1] + v[3], d = rot(v[1] - v[3], fwd);
+    v[0] = a + c;
+    v[2] = a - c;
+    v[1] = b + d;
+    v[3] = b - d;
+  } else if constexpr (R == 5) {
+    const T c1 = (T)HFX_C5A, c2 = (T)HFX_C5B, s1 = (T)HFX_S5A, s2 = (T)HFX_S5B;
+    cx<T> a1 = v[1] + v[4], a2 = v[2] + v[3];
+    cx<T> b1 = v[1] - v[4], b2 = v[2] - v[3];
+    cx<T> x0 = v[0];
+    cx<T> r1 = x0 + c1 * a1 + c2 * a2;
+    cx<T> r2 = x0 + c2 * a1 + c1 * a2;
+    cx<T> i1 = rot(s1 * b1 + s2 * b2, fwd);
+    cx<T> i2 = rot(s2 * b1 - s1 * b2, fwd);
+    v[0] = x0 + a1 + a2;
+    v[1] = r1 + i1;
+    v[4] = r1 - i1;
+    v[2] = r2 + i2;
+    v[3] = r2 - i2;
+  } else if constexpr (R == 8) {
+    // even/odd split into two 4-point DFTs and one layer of W8 twiddles
+    cx<T> e0 = v[0] + v[4], e1 = v[0] - v[4], e2 = v[2] + v[6], e3 = rot(v[2] - v[6], fwd);
+    cx<T> E0 = e0 + e2, E2 = e0 - e2, E1 = e1 + e3, E3 = e1 - e3;
+    cx<T> o0 = v[1] + v[5], o1 = v[1] - v[5], o2 = v[3] + v[7], o3 = rot(v[3] - v[7], fwd);
+    cx<T> O0 = o0 + o2, O2 = o0 - o2, O1 = o1 + o3, O3 = o1 - o3;
+    const T r = (T)HFX_R2;
+    // W8^1 = r(1 -+ i), W8^2 = -+ i, W8^3 = r(-1 -+ i)
+    cx<T> w1 = fwd ? mk<T>(r * (O1.re + O1.im), r * (O1.im - O1.re)) : mk<T>(r * (O1.re - O1.im), r * (O1.im + O1.re));
+    cx<T> w2 = rot(O2, fwd);
+    cx<T> w3 = fwd ? mk<T>(r * (O3.im - O3.re), -r * (O3.re + O3.im)) : mk<T>(-r * (O3.re + O3.im), r * (O3.re - O3.im));
+    v[0] = E0 + O0;
+    v[4] = E0 - O0;
+    v[1] = E1 + w1;
+    v[5] = E1 - w1;
+    v[2] = E2 + w2;
+    v[6] = E2 - w2;
+    v[3] = E3 + w3;
+    v[7] = E3 - w3;
+  } else {
+    // generic odd radix: O(R^2) with W_R^q = W_N^{q N / R} from the table
+    cx<T> out[R];
+    const int step = N / R;
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      cx<T> acc = v[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cx<T> w = tw[((r * m) % R) * step];
+        if (inv) w = conj(w);
+        acc = acc + v[r] * w;
+      }
+      out[m] = acc;
+    }
+#pragma unroll
+    for (int m = 0; m < R; ++m) v[m] = out[m];
+  }
+}
+
+// One Stockham radix-R stage over `nlines` independent lines of length N.
+// Element q of line l lives at buf[l * lstride + q * estride].
+template <typename T, int R, int BLOCK, int MAXE>
+__device__ __forceinline__ void fft_stage(cx<T>* buf, int nlines, int N, int lstride, int estride,
+                                          bool lines_fast, int Ns, const cx<T>* __restrict__ tw,
+                                          bool inv) {
+  constexpr int MAXIT = (MAXE + R * BLOCK - 1) / (R * BLOCK);
+  const int NR = N / R;
+  const int total = nlines * NR;
+  const int tstep = N / (Ns * R);
+  cx<T> v[MAXIT][R];
+#pragma unroll
+  for (int m = 0; m < MAXIT; ++m) {
+    const int it = threadIdx.x + m * BLOCK;
+    if (it < total) {
+      int line, j;
+      if (lines_fast) { line = it % nlines; j = it / nlines; }
+      else { line = it / NR; j = it - line * NR; }
+      const int k = j % Ns;
+      const cx<T>* src = buf + line * lstride;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        cx<T> x = src[(j + r * NR) * estride];
+        if (r > 0 && k > 0) {
+          cx<T> w = tw[r * k * tstep];
+          x = inv ? mulc(x, w) : x * w;
+        }
+        v[m][r] = x;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < MAXIT; ++m) {
+    const int it = threadIdx.x + m * BLOCK;
+    if (it < total) {
+      int line, j;
+      if (lines_fast) { line = it % nlines; j = it / nlines; }
+      else { line = it / NR; j = it - line * NR; }
+      const int k = j % Ns;
+      small_dft<T, R>(v[m], inv, tw, N);
+      const int base = (j / Ns) * Ns * R + k;
+      cx<T>* dst = buf + line * lstride;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[(base + r * Ns) * estride] = v[m][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int BLOCK, int MAXE>
+__device__ void fft_lines(cx<T>* buf, int nlines, int N, int lstride, int estride, bool lines_fast,
+                          int nst, const int* rad, const cx<T>* __restrict__ tw, bool inv) {
+  int Ns = 1;
+  for (int s = 0; s < nst; ++s) {
+    const int R = rad[s];
+    switch (R) {
+      case 2: fft_stage<T, 2, BLOCK, MAXE>(buf, nlines, N, lstride, estride, lines_fast, Ns, tw, inv); break;
+      case 3: fft_stage<T, 3, BLOCK, MAXE>(buf, nlines, N, lstride, estride, lines_fast, Ns, tw, inv); break;
+      case 4: fft_stage<T, 4, BLOCK, MAXE>(buf, nlines, N, lstride, estride, lines_fast, Ns, tw, inv); break;
+      case 5: fft_stage<T, 5, BLOCK, MAXE>(buf, nlines, N, lstride, estride, lines_fast, Ns, tw, inv); break;
+      case 7: fft_stage<T, 7, BLOCK, MAXE>(buf, nlines, N, lstride, estride, lines_fast, Ns, tw, inv); break;
+      case 8: fft_stage<T, 8, BLOCK, MAXE>(buf, nlines, N, lstride, estride, lines_fast, Ns, tw, inv); break;
+      default: break;  // plan creation rejects other radices
+    }
+    Ns *= R;
+  }
+}
+
+template <typename T>
+struct ExchArgs {
+  const cx<T>* P;
+  const cx<T>* refP;   // may be null
+  const cx<T>* VR;     // (n1, n2, vb, vb)
+  const int* chans;    // channel list (i * nb + j)
+  const double* w2d;   // (n1, n2) weights for the channel moment, may be null
+  cx<T>* Sigma;        // (n1, n2, nb, nb)
+  cx<double>* rho;     // (nb, nb) weighted channel moments, may be null
+  int nchan, herm, vfull, xcd_remap;
+  int mode;   // 0: exchange convolution; 1: forward transform of w * V (VR preparation)
+  int shift;  // mode 0: ifftshift the result on store; mode 1: apply the ifftshift phase
+  FftGeom<T> g;
+};
+
+// LDS row pitch: odd, so lanes walking down a column (stride = pitch) and
+// lanes walking along a row (stride 1) both spread over all 64 banks.
+__host__ __device__ constexpr int lds_pitch(int n2) { return (n2 % 2 == 0) ? n2 + 1 : n2; }
+
+// ---- shared pieces of both kernel flavours --------------------------------
+template <typename T, int BLOCK>
+__device__ __forceinline__ int exch_channel(const ExchArgs<T>& a) {
+  int b = blockIdx.x;
+  if (a.xcd_remap) {  // consecutive channels on one XCD (blocks round-robin over 8 XCDs)
+    const int per = a.nchan >> 3;
+    b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  }
+  return a.chans[b];
+}
+
+template <typename T, int BLOCK>
+__device__ __forceinline__ void exch_load(const ExchArgs<T>& a, cx<T>* plane, int n1, int n2, int pitch,
+                                          int c, double* red) {
+  const int nb = a.g.nb, nb2 = nb * nb, N = n1 * n2;
+  const int ci = c / nb, cj = c - ci * nb;
+  double mre = 0.0, mim = 0.0;
+  for (int e = threadIdx.x; e < N; e += BLOCK) {
+    cx<T> v = a.P[(size_t)e * nb2 + c];
+    if (a.refP) v = v - a.refP[(size_t)e * nb2 + c];
+    const int k1 = e / n2, k2 = e - k1 * n2;
+    if (a.mode == 1) {  // VR preparation: w[q] V[q] (weight at the q index, problem.py:88)
+      plane[k1 * pitch + k2] = (T)a.w2d[e] * v;
+      continue;
+    }
+    plane[k1 * pitch + k2] = v;
+    if (a.w2d) {
+      const double w = a.w2d[e];
+      mre += w * (double)v.re;
+      mim += w * (double)v.im;
+    }
+  }
+  if (a.rho && a.mode == 0) {
+    double acc[2] = {mre, mim};
+    block_sum<2>(acc, red);
+    if (threadIdx.x == 0) {
+      a.rho[c] = cx<double>{acc[0], acc[1]};
+      if (a.herm && ci != cj) a.rho[cj * nb + ci] = cx<double>{acc[0], -acc[1]};
+    }
+  } else {
+    __syncthreads();
+  }
+}
+
+template <typename T, int BLOCK>
+__device__ __forceinline__ void exch_multiply(const ExchArgs<T>& a, cx<T>* plane, int n1, int n2, int pitch,
+                                              int c) {
+  const int nb2 = a.g.nb * a.g.nb, N = n1 * n2;
+  const int vc = a.vfull ? c : 0;
+  const int vstride = a.vfull ? nb2 : 1;
+  const T scale = (T)(-1.0 / (double)N);
+  for (int e = threadIdx.x; e < N; e += BLOCK) {
+    const int k1 = e / n2, k2 = e - k1 * n2;
+    const cx<T> vr = a.VR[(size_t)e * vstride + vc];
+    cx<T>& x = plane[k1 * pitch + k2];
+    x = scale * (x * vr);
+  }
+  __syncthreads();
+}
+
+template <typename T, int BLOCK>
+__device__ __forceinline__ void exch_store(const ExchArgs<T>& a, const cx<T>* plane, int n1, int n2, int pitch,
+                                           int c) {
+  const int nb = a.g.nb, nb2 = nb * nb, N = n1 * n2;
+  const int ci = c / nb, cj = c - ci * nb;
+  const bool mirror = a.herm && ci != cj;
+  const int cm = cj * nb + ci;
+  const int s1 = n1 / 2, s2 = n2 / 2;
+  for (int e = threadIdx.x; e < N; e += BLOCK) {
+    const int k1 = e / n2, k2 = e - k1 * n2;
+    cx<T> v = plane[k1 * pitch + k2];
+    size_t eo = (size_t)e;
+    if (a.mode == 1) {
+      if (a.shift) {  // VR * exp(2 pi i (k1 s1 / n1 + k2 s2 / n2))  (problem.py:90-98)
+        const cx<T> ph = a.g.tw1[(k1 * s1) % n1] * a.g.tw2[(k2 * s2) % n2];
+        v = mulc(v, ph);
+      }
+    } else if (a.shift) {  // ifftshift on store: out[i] = in[(i + n//2) mod n]
+      const int o1 = (k1 - s1 + n1) % n1, o2 = (k2 - s2 + n2) % n2;
+      eo = (size_t)o1 * n2 + o2;
+    }
+    a.Sigma[eo * nb2 + c] = v;
+    if (mirror) a.Sigma[eo * nb2 + cm] = conj(v);
+  }
+}
+
+// ---- generic (runtime mesh) kernel -----------------------------------------
+template <typename T, int BLOCK, int MAXE>
+__global__ __launch_bounds__(BLOCK) void exch_conv_kernel(ExchArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* plane = reinterpret_cast<cx<T>*>(smem_raw);
+  __shared__ double red[2 * (BLOCK / 64)];
+  const int n1 = a.g.n1, n2 = a.g.n2, pitch = lds_pitch(n2);
+  const int c = exch_channel<T, BLOCK>(a);
+  exch_load<T, BLOCK>(a, plane, n1, n2, pitch, c, red);
+  fft_lines<T, BLOCK, MAXE>(plane, n1, n2, pitch, 1, true, a.g.nst2, a.g.rad2, a.g.tw2, false);
+  fft_lines<T, BLOCK, MAXE>(plane, n2, n1, 1, pitch, true, a.g.nst1, a.g.rad1, a.g.tw1, false);
+  if (a.mode == 0) {
+    exch_multiply<T, BLOCK>(a, plane, n1, n2, pitch, c);
+    fft_lines<T, BLOCK, MAXE>(plane, n2, n1, 1, pitch, true, a.g.nst1, a.g.rad1, a.g.tw1, true);
+    fft_lines<T, BLOCK, MAXE>(plane, n1, n2, pitch, 1, true, a.g.nst2, a.g.rad2, a.g.tw2, true);
+  }
+  exch_store<T, BLOCK>(a, plane, n1, n2, pitch, c);
+}
+
+// ---- compile-time mesh kernels ----------------------------------------------
+// Radix sequence of a length-N axis (8s first, then 4, 2, 3, 5, 7), matching
+// the host plan's factorisation.
+constexpr int ct_rad(int N, int s) {
+  int k = 0;
+  while (N % 8 == 0) { if (k == s) return 8; N /= 8; ++k; }
+  const int rs[5] = {4, 2, 3, 5, 7};
+  for (int q = 0; q < 5; ++q)
+    while (N % rs[q] == 0) { if (k == s) return rs[q]; N /= rs[q]; ++k; }
+  return 0;
+}
+constexpr int ct_nst(int N) {
+  int k = 0;
+  while (ct_rad(N, k) != 0) ++k;
+  return k;
+}
+
+template <typename T, int R, int N, int NL, int LS, int ES, int BLOCK, int NS, bool INV>
+__device__ __forceinline__ void ct_stage(cx<T>* buf, const cx<T>* __restrict__ tw) {
+  constexpr int NR = N / R, ITEMS = NL * NR, MAXIT = (ITEMS + BLOCK - 1) / BLOCK, TSTEP = N / (NS * R);
+  cx<T> v[MAXIT][R];
+#pragma unroll
+  for (int m = 0; m < MAXIT; ++m) {
+    const int it = threadIdx.x + m * BLOCK;
+    if (ITEMS % BLOCK == 0 || it < ITEMS) {
+      const int line = it % NL, j = it / NL, k = j % NS;
+      const cx<T>* src = buf + line * LS;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        cx<T> x = src[(j + r * NR) * ES];
+        if (NS > 1 && r > 0) {
+          const cx<T> w = tw[r * k * TSTEP];
+          x = INV ? mulc(x, w) : x * w;
+        }
+        v[m][r] = x;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < MAXIT; ++m) {
+    const int it = threadIdx.x + m * BLOCK;
+    if (ITEMS % BLOCK == 0 || it < ITEMS) {
+      const int line = it % NL, j = it / NL, k = j % NS;
+      small_dft<T, R>(v[m], INV, tw, N);
+      const int base = (j / NS) * NS * R + k;
+      cx<T>* dst = buf + line * LS;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[(base + r * NS) * ES] = v[m][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int N, int NL, int LS, int ES, int BLOCK, bool INV, int S = 0, int NS = 1>
+__device__ __forceinline__ void ct_lines(cx<T>* buf, const cx<T>* __restrict__ tw) {
+  if constexpr (S < ct_nst(N)) {
+    constexpr int R = ct_rad(N, S);
+    ct_stage<T, R, N, NL, LS, ES, BLOCK, NS, INV>(buf, tw);
+    ct_lines<T, N, NL, LS, ES, BLOCK, INV, S + 1, NS * R>(buf, tw);
+  }
+}
+
+template <typename T, int N1, int N2, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void exch_ct_kernel(ExchArgs<T> a) {
+  constexpr int PITCH = lds_pitch(N2);
+  __shared__ cx<T> plane[N1 * PITCH];
+  __shared__ double red[2 * (BLOCK / 64)];
+  const int c = exch_channel<T, BLOCK>(a);
+  exch_load<T, BLOCK>(a, plane, N1, N2, PITCH, c, red);
+  ct_lines<T, N2, N1, PITCH, 1, BLOCK, false>(plane, a.g.tw2);   // rows (along k2)
+  ct_lines<T, N1, N2, 1, PITCH, BLOCK, false>(plane, a.g.tw1);   // columns (along k1)
+  if (a.mode == 0) {
+    exch_multiply<T, BLOCK>(a, plane, N1, N2, PITCH, c);
+    ct_lines<T, N1, N2, 1, PITCH, BLOCK, true>(plane, a.g.tw1);
+    ct_lines<T, N2, N1, PITCH, 1, BLOCK, true>(plane, a.g.tw2);
+  }
+  exch_store<T, BLOCK>(a, plane, N1, N2, PITCH, c);
+}
+
+constexpr int EXCH_BLOCK = 1024;
+// Largest plane (n1 * n2) held in LDS: 159 KiB, leaving room for the static
+// reduction scratch inside the 160 KiB per-workgroup limit of gfx950.
+constexpr int MAXE_F64 = 10176;
+constexpr int MAXE_F32 = 20352;
+
+template <int N>
+constexpr int ct_block() { return N * N >= 4096 ? 1024 : (N * N >= 1024 ? 512 : 256); }
+
+#define HFX_CT_SIZES(X) X(12) X(16) X(24) X(32) X(36) X(48) X(60) X(64) X(72) X(90) X(96)
+
+template <typename T>
+static int launch_exch(const ExchArgs<T>& a, hipStream_t s) {
+  constexpr int MAXE = sizeof(T) == 8 ? MAXE_F64 : MAXE_F32;
+  const int n1 = a.g.n1, n2 = a.g.n2;
+  if (n1 == n2) {
+    switch (n1) {
+#define HFX_CASE(N)                                                                        \
+  case N:                                                                                  \
+    if ((size_t)N * lds_pitch(N) * sizeof(cx<T>) <= (size_t)MAXE * sizeof(cx<T>)) {        \
+      hipLaunchKernelGGL((exch_ct_kernel<T, N, N, ct_block<N>()>), dim3(a.nchan),           \
+                         dim3(ct_block<N>()), 0, s, a);                                    \
+      return check_launch("exch_ct_kernel");                                               \
+    }                                                                                      \
+    break;
+      HFX_CT_SIZES(HFX_CASE)
+#undef HFX_CASE
+      default: break;
+    }
+  }
+  const size_t lds = (size_t)n1 * lds_pitch(n2) * sizeof(cx<T>);
+  auto kern = exch_conv_kernel<T, EXCH_BLOCK, MAXE>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    HFX_HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(MAXE * sizeof(cx<T>))));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(kern, dim3(a.nchan), dim3(EXCH_BLOCK), lds, s, a);
+  return check_launch("exch_conv_kernel");
+}
+
+int exchange_launch(const hfx_plan* plan, const void* P, const void* refP, const void* VR, int vfull,
+                    const int* chans, int nchan, int herm, const double* w2d, void* Sigma, void* rho,
+                    int mode, int shift, hipStream_t s) {
+  if (nchan <= 0) return HFX_OK;
+  if (plan->dtype == HFX_DTYPE_C128) {
+    ExchArgs<double> a{(const cx<double>*)P, (const cx<double>*)refP, (const cx<double>*)VR, chans, w2d,
+                       (cx<double>*)Sigma, (cx<double>*)rho, nchan, herm, vfull,
+                       (nchan % 8 == 0) ? 1 : 0, mode, shift, plan->g64};
+    return launch_exch<double>(a, s);
+  }
+  ExchArgs<float> a{(const cx<float>*)P, (const cx<float>*)refP, (const cx<float>*)VR, chans, w2d,
+                    (cx<float>*)Sigma, (cx<double>*)rho, nchan, herm, vfull,
+                    (nchan % 8 == 0) ? 1 : 0, mode, shift, plan->g32};
+  return launch_exch<float>(a, s);
+}
+
+int exchange_max_plane(int dtype) { return dtype == HFX_DTYPE_C128 ? MAXE_F64 : MAXE_F32; }
+int exchange_lds_elems(int n1, int n2) { return n1 * lds_pitch(n2); }
+
+}  // namespace hfx

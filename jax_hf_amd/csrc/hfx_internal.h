@@ -1,0 +1,42 @@
+// Internal (non-exported) declarations shared by the libhfx translation units.
+#pragma once
+
+#include "hfx_common.h"
+#include "../../include/hfx.h"
+
+#define HFX_DTYPE_C128 0
+#define HFX_DTYPE_C64 1
+#define HFX_MAX_STAGES 16
+
+namespace hfx {
+
+// Per-precision FFT geometry passed by value to the exchange kernel.
+template <typename T>
+struct FftGeom {
+  int n1, n2, nb;
+  int nst1, nst2;
+  int rad1[HFX_MAX_STAGES];
+  int rad2[HFX_MAX_STAGES];
+  const cx<T>* tw1;  // W_{n1}^m, m < n1 (device)
+  const cx<T>* tw2;  // W_{n2}^m, m < n2 (device)
+};
+
+}  // namespace hfx
+
+// Immutable after creation; device twiddle tables owned by the plan.
+struct hfx_plan {
+  int n1, n2, nb, dtype;
+  void* tw_dev;  // one allocation: [tw1 | tw2 | channel lists]
+  const int* chans_full;  // c = 0 .. nb*nb-1
+  const int* chans_triu;  // i * nb + j for i <= j
+  hfx::FftGeom<double> g64;
+  hfx::FftGeom<float> g32;
+};
+
+namespace hfx {
+int exchange_launch(const hfx_plan* plan, const void* P, const void* refP, const void* VR, int vfull,
+                    const int* chans, int nchan, int herm, const double* w2d, void* Sigma, void* rho,
+                    int mode, int shift, hipStream_t s);
+int exchange_max_plane(int dtype);
+int exchange_lds_elems(int n1, int n2);
+}  // namespace hfx

@@ -1,0 +1,780 @@
+// Per-k batched nb x nb complex kernels of the direct-minimisation and SCF
+// loops (src/jax_hf/solver.py:34-233, 451-600; reference_scf.py:87-96).
+//
+// Execution model: one 64-lane wavefront (= one 64-thread workgroup) owns one
+// k-point.  Its matrices are staged in LDS with an odd row pitch (nb + 1) so
+// row- and column-walking lanes hit distinct banks; lane l owns the matrix
+// elements e = l, l + 64, ... (<= 4 for nb <= 16).  Per-k real vectors
+// (occupations, eigenvalues, gradients) are float64 in both storage
+// precisions; complex matrices are in the storage precision T.  Products
+// accumulate in T (FP32 in the mixed form).  Per-k scalar partials go to a
+// (nval, Nk) float64 array that hfx_reduce sums in a fixed order.
+#include "hfx_common.h"
+#include "hfx_internal.h"
+
+namespace hfx {
+
+constexpr int PK_NBMAX = 16;
+constexpr int PK_PITCH = PK_NBMAX + 1;
+constexpr int PK_MAT = PK_NBMAX * PK_PITCH;   // LDS elements per staged matrix
+constexpr int PK_EPL = (PK_NBMAX * PK_NBMAX + 63) / 64;
+
+#define PK_FOR_ELEM(nb, ...)                                    \
+  _Pragma("unroll") for (int m_ = 0; m_ < PK_EPL; ++m_) {       \
+    const int e = threadIdx.x + 64 * m_;                        \
+    if (e < (nb) * (nb)) {                                      \
+      const int i = e / (nb), j = e - i * (nb);                 \
+      __VA_ARGS__                                               \
+    }                                                           \
+  }
+
+template <typename T>
+__device__ __forceinline__ void m_load(cx<T>* s, const cx<T>* __restrict__ g, int nb) {
+  PK_FOR_ELEM(nb, { s[i * PK_PITCH + j] = g[e]; })
+}
+template <typename T>
+__device__ __forceinline__ void m_store(cx<T>* __restrict__ g, const cx<T>* s, int nb) {
+  PK_FOR_ELEM(nb, { g[e] = s[i * PK_PITCH + j]; })
+}
+
+// C = op(A) op(B); op = conjugate transpose when the flag is set.  C must not
+// alias A or B.  Ends with a barrier.
+template <typename T, bool AH, bool BH>
+__device__ __forceinline__ void m_gemm(cx<T>* C, const cx<T>* A, const cx<T>* B, int nb) {
+  PK_FOR_ELEM(nb, {
+    cx<T> acc{0, 0};
+    for (int l = 0; l < nb; ++l) {
+      const cx<T> a = AH ? conj(A[l * PK_PITCH + i]) : A[i * PK_PITCH + l];
+      const cx<T> b = BH ? conj(B[j * PK_PITCH + l]) : B[l * PK_PITCH + j];
+      acc = acc + a * b;
+    }
+    C[i * PK_PITCH + j] = acc;
+  })
+  __syncthreads();
+}
+
+// Cayley factor c = (1 + i a) / (1 - i a), a = tau lam / 2  (solver.py:169-175)
+template <typename T>
+__device__ __forceinline__ cx<T> cayley(double lam, double tau) {
+  const T a = (T)(tau * lam * 0.5);
+  // (1 + ia)/(1 - ia) = (1 + ia)^2 / (1 + a^2)
+  const T d = (T)1 / ((T)1 + a * a);
+  return cx<T>{((T)1 - a * a) * d, (T)2 * a * d};
+}
+
+// diag(U^dagger Ft U) from the spectral form (solver.py:185-197):
+// eps_b = Re sum_j (V M)_bj conj(V_bj),  M = conj(c) (x) Ft_eig (x) c.
+// V and Fe are LDS matrices, c per column in cs[], scratch W (LDS).
+template <typename T>
+__device__ __forceinline__ void spectral_diag(const cx<T>* V, const cx<T>* Fe, const cx<T>* cs, cx<T>* W,
+                                              double* outd, int nb) {
+  PK_FOR_ELEM(nb, { W[i * PK_PITCH + j] = conj(cs[i]) * Fe[i * PK_PITCH + j] * cs[j]; })
+  __syncthreads();
+  // lane b < nb: row b of V * W, dotted with conj(V_b.)
+  if ((int)threadIdx.x < nb) {
+    const int b = threadIdx.x;
+    double acc = 0.0;
+    for (int j = 0; j < nb; ++j) {
+      cx<T> a{0, 0};
+      for (int l = 0; l < nb; ++l) a = a + V[b * PK_PITCH + l] * W[l * PK_PITCH + j];
+      const cx<T> v = V[b * PK_PITCH + j];
+      acc += (double)a.re * (double)v.re + (double)a.im * (double)v.im;   // Re(a conj(v))
+    }
+    outd[b] = acc;
+  }
+  __syncthreads();
+}
+
+// ---- batched Hermitian eigensolver: cyclic parallel (round-robin) Jacobi ----
+// A (LDS, Hermitian) is diagonalised in place; V (LDS) receives the
+// eigenvectors (columns), lam[] (LDS, double) the eigenvalues, ascending.
+// For pair (p, q) with c = A_pq = |c| e^{i phi}: zeta = (A_qq - A_pp)/(2|c|),
+// t = sign(zeta)/(|zeta| + sqrt(1 + zeta^2)), cs = 1/sqrt(1+t^2), sn = t cs,
+// J_pp = J_qq = cs, J_pq = sn e^{i phi}, J_qp = -sn e^{-i phi}; A <- J^H A J.
+template <typename T>
+__device__ void m_eigh(cx<T>* A, cx<T>* V, cx<T>* Vt, double* lam, int nb, double* red) {
+  __shared__ int partner[PK_NBMAX];
+  __shared__ T jd[PK_NBMAX];
+  __shared__ cx<T> jo[PK_NBMAX];
+  __shared__ T dnew[PK_NBMAX];
+  const T eps = sizeof(T) == 8 ? (T)2.220446049250313e-16 : (T)1.1920929e-07;
+  PK_FOR_ELEM(nb, { V[i * PK_PITCH + j] = cx<T>{i == j ? (T)1 : (T)0, (T)0}; })
+  // exact hermitian diagonal
+  if ((int)threadIdx.x < nb) A[threadIdx.x * PK_PITCH + threadIdx.x].im = (T)0;
+  __syncthreads();
+  const int m = nb + (nb & 1);    // even round-robin size (dummy index = nb when odd)
+  cx<T> newA[PK_EPL], newV[PK_EPL];
+  for (int sweep = 0; sweep < 30 && nb > 1; ++sweep) {
+    // convergence: off-diagonal Frobenius norm vs total
+    T off = 0, tot = 0;
+    PK_FOR_ELEM(nb, {
+      const cx<T> a = A[i * PK_PITCH + j];
+      const T x = a.re * a.re + a.im * a.im;
+      tot += x;
+      if (i != j) off += x;
+    })
+    const double off_s = wave_sum((double)off), tot_s = wave_sum((double)tot);
+    if (off_s <= (double)eps * (double)eps * 0.25 * tot_s || off_s == 0.0) break;
+    for (int step = 0; step < m - 1; ++step) {
+      // pair table for this step: index m-1 meets `step`; others (step+k, step-k) mod (m-1)
+      if ((int)threadIdx.x < m / 2) {
+        const int k = threadIdx.x;
+        int p, q;
+        if (k == 0) { p = step; q = m - 1; }
+        else { p = (step + k) % (m - 1); q = (step - k + (m - 1)) % (m - 1); }
+        if (p > q) { const int t = p; p = q; q = t; }
+        if (q < nb) {
+          partner[p] = q;
+          partner[q] = p;
+          const cx<T> c = A[p * PK_PITCH + q];
+          const T ac = sqrt(c.re * c.re + c.im * c.im);
+          const T app = A[p * PK_PITCH + p].re, aqq = A[q * PK_PITCH + q].re;
+          if (ac > (T)0) {
+            const T zeta = (aqq - app) / ((T)2 * ac);
+            const T t = (zeta >= (T)0 ? (T)1 : (T)-1) / (fabs(zeta) + sqrt((T)1 + zeta * zeta));
+            const T cs = (T)1 / sqrt((T)1 + t * t), sn = t * cs;
+            const cx<T> ph{c.re / ac, c.im / ac};   // e^{i phi}
+            jd[p] = cs;
+            jd[q] = cs;
+            jo[p] = cx<T>{-sn * ph.re, sn * ph.im};   // J_qp = -sn e^{-i phi}
+            jo[q] = cx<T>{sn * ph.re, sn * ph.im};    // J_pq =  sn e^{i phi}
+            dnew[p] = app - t * ac;
+            dnew[q] = aqq + t * ac;
+          } else {
+            jd[p] = jd[q] = (T)1;
+            jo[p] = jo[q] = cx<T>{0, 0};
+            dnew[p] = app;
+            dnew[q] = aqq;
+          }
+        } else if (p < nb) {   // paired with the dummy: untouched this step
+          partner[p] = p;
+          jd[p] = (T)1;
+          jo[p] = cx<T>{0, 0};
+          dnew[p] = A[p * PK_PITCH + p].re;
+        }
+      }
+      __syncthreads();
+      // A'_ij = sum_{a in {i,i'}, b in {j,j'}} conj(J_ai) A_ab J_bj ;  V' = V J
+      PK_FOR_ELEM(nb, {
+        const int ip = partner[i], jp = partner[j];
+        const T di = jd[i], dj = jd[j];
+        const cx<T> oi = jo[i], oj = jo[j];
+        cx<T> r;
+        if (i == j) {
+          r = cx<T>{dnew[i], (T)0};
+        } else if (ip == j) {
+          r = cx<T>{0, 0};
+        } else {
+          const cx<T> aij = A[i * PK_PITCH + j];
+          cx<T> x = dj * aij;
+          if (jp != j) x = x + A[i * PK_PITCH + jp] * oj;
+          cx<T> y{0, 0};
+          if (ip != i) {
+            y = dj * A[ip * PK_PITCH + j];
+            if (jp != j) y = y + A[ip * PK_PITCH + jp] * oj;
+          }
+          r = di * x + cmul(oi, y);
+        }
+        newA[m_] = r;
+        cx<T> v = dj * V[i * PK_PITCH + j];
+        if (jp != j) v = v + V[i * PK_PITCH + jp] * oj;
+        newV[m_] = v;
+      })
+      __syncthreads();
+      PK_FOR_ELEM(nb, {
+        A[i * PK_PITCH + j] = newA[m_];
+        V[i * PK_PITCH + j] = newV[m_];
+      })
+      __syncthreads();
+    }
+  }
+  // ascending order: rank_i = #{j : l_j < l_i or (l_j == l_i and j < i)}
+  __shared__ int rank[PK_NBMAX];
+  if ((int)threadIdx.x < nb) {
+    const int i = threadIdx.x;
+    const T li = A[i * PK_PITCH + i].re;
+    int r = 0;
+    for (int j = 0; j < nb; ++j) {
+      const T lj = A[j * PK_PITCH + j].re;
+      r += (lj < li) || (lj == li && j < i);
+    }
+    rank[i] = r;
+    lam[r] = (double)li;
+  }
+  __syncthreads();
+  PK_FOR_ELEM(nb, { Vt[i * PK_PITCH + rank[j]] = V[i * PK_PITCH + j]; })
+  __syncthreads();
+  PK_FOR_ELEM(nb, { V[i * PK_PITCH + j] = Vt[i * PK_PITCH + j]; })
+  __syncthreads();
+  (void)red;
+}
+
+// ============================ kernels ========================================
+
+// Batched eigh: A (nk, nb, nb) Hermitian -> lam (nk, nb) ascending, V (nk, nb, nb).
+// herm_in: hermitise the input first (reference_scf.py:89 eigh(_herm(F))).
+template <typename T>
+__global__ __launch_bounds__(64) void pk_eigh_kernel(int nb, const cx<T>* __restrict__ A, double* __restrict__ lam,
+                                                     cx<T>* __restrict__ V, int herm_in) {
+  __shared__ cx<T> sA[PK_MAT], sV[PK_MAT], sT[PK_MAT];
+  __shared__ double sl[PK_NBMAX];
+  const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  m_load(sA, A + k * nb2, nb);
+  __syncthreads();
+  if (herm_in) {
+    PK_FOR_ELEM(nb, { sT[i * PK_PITCH + j] = (T)0.5 * (sA[i * PK_PITCH + j] + conj(sA[j * PK_PITCH + i])); })
+    __syncthreads();
+    PK_FOR_ELEM(nb, { sA[i * PK_PITCH + j] = sT[i * PK_PITCH + j]; })
+    __syncthreads();
+  }
+  m_eigh(sA, sV, sT, sl, nb, nullptr);
+  if ((int)threadIdx.x < nb) lam[k * nb + threadIdx.x] = sl[threadIdx.x];
+  m_store(V + k * nb2, sV, nb);
+}
+
+// P = herm(Q diag(p) Q^dagger)  (solver.py:42-44, 462; utils.py:473-474)
+// p from pvec, or (when pvec is NULL) p = sigmoid((mu - eps)/T) and written to pout.
+template <typename T>
+__global__ __launch_bounds__(64) void pk_density_kernel(int nb, const cx<T>* __restrict__ Q,
+                                                        const double* __restrict__ pvec,
+                                                        const double* __restrict__ eps, const double* __restrict__ mu,
+                                                        double T_, double* __restrict__ pout,
+                                                        cx<T>* __restrict__ P) {
+  __shared__ cx<T> sQ[PK_MAT], sX[PK_MAT], sP[PK_MAT];
+  __shared__ double sp[PK_NBMAX];
+  const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  if ((int)threadIdx.x < nb) {
+    double pv;
+    if (pvec) pv = pvec[k * nb + threadIdx.x];
+    else {
+      pv = 1.0 / (1.0 + exp(-((*mu) - eps[k * nb + threadIdx.x]) / T_));
+      if (pout) pout[k * nb + threadIdx.x] = pv;
+    }
+    sp[threadIdx.x] = pv;
+  }
+  m_load(sQ, Q + k * nb2, nb);
+  __syncthreads();
+  PK_FOR_ELEM(nb, { sX[i * PK_PITCH + j] = (T)sp[j] * sQ[i * PK_PITCH + j]; })
+  __syncthreads();
+  m_gemm<T, false, true>(sP, sX, sQ, nb);
+  PK_FOR_ELEM(nb, { P[k * nb2 + e] = (T)0.5 * (sP[i * PK_PITCH + j] + conj(sP[j * PK_PITCH + i])); })
+}
+
+// CG step 1 (solver.py:472-491): Ft = Q^dagger F Q, eps = Re diag Ft,
+// G = skew((p_j - p_i) Ft) offdiag.  part[0][k] = sum |G|^2, part[1][k] = sum eps^2.
+template <typename T>
+__global__ __launch_bounds__(64) void pk_cg_grad_kernel(int nb, size_t Nk, const cx<T>* __restrict__ Q,
+                                                        const cx<T>* __restrict__ F, const double* __restrict__ p,
+                                                        cx<T>* __restrict__ Ft, double* __restrict__ eps,
+                                                        cx<T>* __restrict__ G, double* __restrict__ part) {
+  __shared__ cx<T> sQ[PK_MAT], sF[PK_MAT], sX[PK_MAT], sT[PK_MAT];
+  __shared__ double sp[PK_NBMAX];
+  const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  m_load(sQ, Q + k * nb2, nb);
+  m_load(sF, F + k * nb2, nb);
+  if ((int)threadIdx.x < nb) sp[threadIdx.x] = p[k * nb + threadIdx.x];
+  __syncthreads();
+  m_gemm<T, false, false>(sX, sF, sQ, nb);
+  m_gemm<T, true, false>(sT, sQ, sX, nb);
+  double g2 = 0.0, e2 = 0.0;
+  PK_FOR_ELEM(nb, {
+    const cx<T> fij = sT[i * PK_PITCH + j];
+    Ft[k * nb2 + e] = fij;
+    cx<T> g{0, 0};
+    if (i != j) {
+      const T dij = (T)(sp[j] - sp[i]), dji = (T)(sp[i] - sp[j]);
+      const cx<T> a = dij * fij, b = dji * sT[j * PK_PITCH + i];
+      g = (T)0.5 * (a - conj(b));
+    }
+    G[k * nb2 + e] = g;
+    g2 += (double)g.re * g.re + (double)g.im * g.im;
+    if (i == j) {
+      eps[k * nb + i] = (double)fij.re;
+      e2 += (double)fij.re * (double)fij.re;
+    }
+  })
+  g2 = wave_sum(g2);
+  e2 = wave_sum(e2);
+  if (threadIdx.x == 0) {
+    part[k] = g2;
+    part[Nk + k] = e2;
+  }
+}
+
+// CG step 2 (solver.py:486, 493-519): preconditioned gradients and the six
+// Polak-Ribiere inner products per k (weights applied in the reduction).
+// st: device scalar state (mu, lam).  Outputs H, gp, hp; part[0..5][k].
+template <typename T>
+__global__ __launch_bounds__(64) void pk_cg_precond_kernel(int nb, size_t Nk, const cx<T>* __restrict__ G,
+                                                           const cx<T>* __restrict__ Gprev,
+                                                           const double* __restrict__ eps,
+                                                           const double* __restrict__ p,
+                                                           const double* __restrict__ gp_prev,
+                                                           const double* __restrict__ st, double T_,
+                                                           cx<T>* __restrict__ H, double* __restrict__ gp,
+                                                           double* __restrict__ hp, double* __restrict__ part) {
+  const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  const double lam = st[HFX_ST_LAM], mu = st[HFX_ST_MU];
+  double s_gh = 0, s_ph = 0, s_pp = 0;
+  PK_FOR_ELEM(nb, {
+    const double gap = eps[k * nb + i] - eps[k * nb + j];
+    const T den = (T)sqrt(gap * gap + lam * lam);
+    const cx<T> g = G[k * nb2 + e], gpv = Gprev[k * nb2 + e];
+    const cx<T> hq{g.re / den, g.im / den};
+    H[k * nb2 + e] = hq;
+    s_gh += (double)g.re * hq.re + (double)g.im * hq.im;
+    s_ph += (double)gpv.re * hq.re + (double)gpv.im * hq.im;
+    s_pp += (double)gpv.re * gpv.re + (double)gpv.im * gpv.im;
+  })
+  double v_gh = 0, v_ph = 0, v_pp = 0;
+  if ((int)threadIdx.x < nb) {
+    const size_t q = k * nb + threadIdx.x;
+    const double pv = p[q];
+    const double pl = fmin(fmax(pv, 1e-6), 1.0 - 1e-6);
+    const double g = eps[q] + T_ * (log(pl) - log1p(-pl)) - mu;   // solver.py:52-55, 486
+    const double ps = fmin(fmax(pv, 1e-8), 1.0 - 1e-8);
+    const double h = g * (ps * (1.0 - ps) / T_);
+    gp[q] = g;
+    hp[q] = h;
+    const double gq = gp_prev[q];
+    v_gh = g * h;
+    v_ph = gq * h;
+    v_pp = gq * gq;
+  }
+  s_gh = wave_sum(s_gh);
+  s_ph = wave_sum(s_ph);
+  s_pp = wave_sum(s_pp);
+  v_gh = wave_sum(v_gh);
+  v_ph = wave_sum(v_ph);
+  v_pp = wave_sum(v_pp);
+  if (threadIdx.x == 0) {
+    part[0 * Nk + k] = s_gh;
+    part[1 * Nk + k] = s_ph;
+    part[2 * Nk + k] = v_gh;
+    part[3 * Nk + k] = v_ph;
+    part[4 * Nk + k] = s_pp;
+    part[5 * Nk + k] = v_pp;
+  }
+}
+
+// CG step 3 (solver.py:531-546): d = H + beta d_prev (in place in D),
+// dp likewise, eigh(herm(i d)) -> V, lam_d, Ft_eig = V^dagger Ft V,
+// part[k] = sum |d|^2.
+template <typename T>
+__global__ __launch_bounds__(64) void pk_cg_direction_kernel(int nb, const cx<T>* __restrict__ H,
+                                                             cx<T>* __restrict__ D, const double* __restrict__ hp,
+                                                             double* __restrict__ dp, const cx<T>* __restrict__ Ft,
+                                                             const double* __restrict__ st, cx<T>* __restrict__ Vout,
+                                                             double* __restrict__ lam_d, cx<T>* __restrict__ Fe,
+                                                             double* __restrict__ part) {
+  __shared__ cx<T> sA[PK_MAT], sV[PK_MAT], sT[PK_MAT], sF[PK_MAT];
+  __shared__ double sl[PK_NBMAX];
+  const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  const T beta = (T)st[HFX_ST_BETA];
+  double dn = 0.0;
+  PK_FOR_ELEM(nb, {
+    const cx<T> hq = H[k * nb2 + e];
+    const cx<T> dv = D[k * nb2 + e];
+    const cx<T> d = hq + beta * dv;
+    D[k * nb2 + e] = d;
+    sT[i * PK_PITCH + j] = d;
+    dn += (double)d.re * d.re + (double)d.im * d.im;
+  })
+  if ((int)threadIdx.x < nb) {
+    const size_t q = k * nb + threadIdx.x;
+    dp[q] = hp[q] + (double)beta * dp[q];
+  }
+  m_load(sF, Ft + k * nb2, nb);
+  __syncthreads();
+  // herm(i d): x = i d_ij, y = i d_ji, 0.5 (x + conj(y))  (solver.py:163-164)
+  PK_FOR_ELEM(nb, {
+    const cx<T> a = sT[i * PK_PITCH + j], b = sT[j * PK_PITCH + i];
+    const cx<T> x{-a.im, a.re}, y{-b.im, b.re};
+    sA[i * PK_PITCH + j] = (T)0.5 * (x + conj(y));
+  })
+  __syncthreads();
+  m_eigh(sA, sV, sT, sl, nb, nullptr);
+  if ((int)threadIdx.x < nb) lam_d[k * nb + threadIdx.x] = sl[threadIdx.x];
+  m_store(Vout + k * nb2, sV, nb);
+  m_gemm<T, false, false>(sT, sF, sV, nb);
+  m_gemm<T, true, false>(sA, sV, sT, nb);
+  m_store(Fe + k * nb2, sA, nb);
+  dn = wave_sum(dn);
+  if (threadIdx.x == 0) part[k] = dn;
+}
+
+// CG line-search trial (solver.py:549-553; tau from st[HFX_ST_TAU]):
+// part[0][k] = sum_b p_t eps_t, part[1][k] = sum_b [p_t ln p_t + (1-p_t) ln(1-p_t)].
+// With omega0 != 0 also the tau = 0 point (solver.py:559-561) into part[2], part[3].
+// Skipped (no writes) once st[HFX_ST_ACC] is set.
+template <typename T>
+__global__ __launch_bounds__(64) void pk_cg_trial_kernel(int nb, size_t Nk, const cx<T>* __restrict__ V,
+                                                         const double* __restrict__ lam_d,
+                                                         const cx<T>* __restrict__ Fe, const double* __restrict__ p,
+                                                         const double* __restrict__ dp,
+                                                         const double* __restrict__ eps, const double* __restrict__ st,
+                                                         int omega0, double* __restrict__ part) {
+  if (st[HFX_ST_ACC] != 0.0) return;
+  __shared__ cx<T> sV[PK_MAT], sF[PK_MAT], sW[PK_MAT];
+  __shared__ cx<T> cs[PK_NBMAX];
+  __shared__ double et[PK_NBMAX];
+  const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  const double tau = st[HFX_ST_TAU];
+  m_load(sV, V + k * nb2, nb);
+  m_load(sF, Fe + k * nb2, nb);
+  if ((int)threadIdx.x < nb) cs[threadIdx.x] = cayley<T>(lam_d[k * nb + threadIdx.x], tau);
+  __syncthreads();
+  spectral_diag(sV, sF, cs, sW, et, nb);
+  double ef = 0.0, sf = 0.0, e0 = 0.0, s0 = 0.0;
+  if ((int)threadIdx.x < nb) {
+    const size_t q = k * nb + threadIdx.x;
+    double pt = fmin(fmax(p[q] - tau * dp[q], 1e-8), 1.0 - 1e-8);
+    ef = pt * et[threadIdx.x];
+    double x = fmin(fmax(pt, 1e-14), 1.0 - 1e-14);
+    sf = x * log(x) + (1.0 - x) * log1p(-x);
+    if (omega0) {
+      const double p0 = fmin(fmax(p[q], 1e-8), 1.0 - 1e-8);
+      e0 = p0 * eps[q];
+      x = fmin(fmax(p0, 1e-14), 1.0 - 1e-14);
+      s0 = x * log(x) + (1.0 - x) * log1p(-x);
+    }
+  }
+  ef = wave_sum(ef);
+  sf = wave_sum(sf);
+  e0 = wave_sum(e0);
+  s0 = wave_sum(s0);
+  if (threadIdx.x == 0) {
+    part[k] = ef;
+    part[Nk + k] = sf;
+    if (omega0) {
+      part[2 * Nk + k] = e0;
+      part[3 * Nk + k] = s0;
+    }
+  }
+}
+
+// CG retraction (solver.py:586-590): U = V diag(c) V^dagger, Q <- Q U,
+// eps_new = diag(U^dagger Ft U) (spectral), tau from st[HFX_ST_TAUF].
+template <typename T>
+__global__ __launch_bounds__(64) void pk_cg_retract_kernel(int nb, cx<T>* __restrict__ Q, const cx<T>* __restrict__ V,
+                                                           const double* __restrict__ lam_d,
+                                                           const cx<T>* __restrict__ Fe, const double* __restrict__ st,
+                                                           double tau_override, int use_override,
+                                                           double* __restrict__ eps_new, cx<T>* __restrict__ Ft_out) {
+  __shared__ cx<T> sQ[PK_MAT], sV[PK_MAT], sX[PK_MAT], sU[PK_MAT];
+  __shared__ cx<T> cs[PK_NBMAX];
+  __shared__ double et[PK_NBMAX];
+  const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  const double tau = use_override ? tau_override : st[HFX_ST_TAUF];
+  m_load(sQ, Q + k * nb2, nb);
+  m_load(sV, V + k * nb2, nb);
+  if ((int)threadIdx.x < nb) cs[threadIdx.x] = cayley<T>(lam_d[k * nb + threadIdx.x], tau);
+  __syncthreads();
+  PK_FOR_ELEM(nb, { sX[i * PK_PITCH + j] = sV[i * PK_PITCH + j] * cs[j]; })
+  __syncthreads();
+  m_gemm<T, false, true>(sU, sX, sV, nb);     // U
+  m_gemm<T, false, false>(sX, sQ, sU, nb);    // Q U
+  m_store(Q + k * nb2, sX, nb);
+  if (eps_new) {
+    m_load(sQ, Fe + k * nb2, nb);             // reuse sQ for Ft_eig
+    __syncthreads();
+    spectral_diag(sV, sQ, cs, sX, et, nb);
+    if ((int)threadIdx.x < nb) eps_new[k * nb + threadIdx.x] = et[threadIdx.x];
+  }
+  if (Ft_out) {   // Ft_out = U^dagger Ft U with Ft given in Fe (Newton trial, solver.py:1008)
+    m_load(sQ, Fe + k * nb2, nb);
+    __syncthreads();
+    m_gemm<T, false, false>(sX, sQ, sU, nb);
+    m_gemm<T, true, false>(sV, sU, sX, nb);
+    m_store(Ft_out + k * nb2, sV, nb);
+  }
+}
+
+// Ft = Q^dagger F Q, eigh(Ft) -> (eps, W), Q <- Q W  (solver.py:622-625)
+template <typename T>
+__global__ __launch_bounds__(64) void pk_rotate_eigh_kernel(int nb, cx<T>* __restrict__ Q, const cx<T>* __restrict__ F,
+                                                            double* __restrict__ eps) {
+  __shared__ cx<T> sQ[PK_MAT], sF[PK_MAT], sX[PK_MAT], sT[PK_MAT];
+  __shared__ double sl[PK_NBMAX];
+  const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  m_load(sQ, Q + k * nb2, nb);
+  m_load(sF, F + k * nb2, nb);
+  __syncthreads();
+  m_gemm<T, false, false>(sX, sF, sQ, nb);
+  m_gemm<T, true, false>(sT, sQ, sX, nb);     // Ft
+  m_eigh(sT, sX, sF, sl, nb, nullptr);        // W in sX
+  m_gemm<T, false, false>(sF, sQ, sX, nb);    // Q W
+  m_store(Q + k * nb2, sF, nb);
+  if ((int)threadIdx.x < nb) eps[k * nb + threadIdx.x] = sl[threadIdx.x];
+}
+
+// C = op(A) op(B) op(C0)?  Generic batched products: out = opA(A) opB(B) [opC(Cm)]
+template <typename T, bool AH, bool BH, bool CH>
+__global__ __launch_bounds__(64) void pk_mm_kernel(int nb, const cx<T>* __restrict__ A, const cx<T>* __restrict__ B,
+                                                   const cx<T>* __restrict__ Cm, cx<T>* __restrict__ out,
+                                                   const cx<T>* __restrict__ Bsub, int herm_out) {
+  __shared__ cx<T> sA[PK_MAT], sB[PK_MAT], sC[PK_MAT], sX[PK_MAT];
+  const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  m_load(sA, A + k * nb2, nb);
+  m_load(sB, B + k * nb2, nb);
+  if (Cm) m_load(sC, Cm + k * nb2, nb);
+  __syncthreads();
+  if (Bsub) {   // B <- B - Bsub   (e.g. F[dP] - h)
+    PK_FOR_ELEM(nb, { sB[i * PK_PITCH + j] = sB[i * PK_PITCH + j] - Bsub[k * nb2 + e]; })
+    __syncthreads();
+  }
+  m_gemm<T, AH, BH>(sX, sA, sB, nb);
+  const cx<T>* R = sX;
+  if (Cm) {
+    m_gemm<T, false, CH>(sA, sX, sC, nb);
+    R = sA;
+  }
+  if (herm_out) {
+    PK_FOR_ELEM(nb, { out[k * nb2 + e] = (T)0.5 * (R[i * PK_PITCH + j] + conj(R[j * PK_PITCH + i])); })
+  } else {
+    m_store(out + k * nb2, R, nb);
+  }
+}
+
+// SCF residuals (reference_scf.py:291-294): comm = F D - D F (stored when
+// comm != NULL), part[0][k] = sum |comm|^2, part[1][k] = sum |Pn - D|^2.
+template <typename T>
+__global__ __launch_bounds__(64) void pk_scf_resid_kernel(int nb, size_t Nk, const cx<T>* __restrict__ F,
+                                                          const cx<T>* __restrict__ D, const cx<T>* __restrict__ Pn,
+                                                          cx<T>* __restrict__ comm, double* __restrict__ part) {
+  __shared__ cx<T> sF[PK_MAT], sD[PK_MAT], sX[PK_MAT], sY[PK_MAT];
+  const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  m_load(sF, F + k * nb2, nb);
+  m_load(sD, D + k * nb2, nb);
+  __syncthreads();
+  m_gemm<T, false, false>(sX, sF, sD, nb);
+  m_gemm<T, false, false>(sY, sD, sF, nb);
+  double c2 = 0.0, d2 = 0.0;
+  PK_FOR_ELEM(nb, {
+    const cx<T> c = sX[i * PK_PITCH + j] - sY[i * PK_PITCH + j];
+    if (comm) comm[k * nb2 + e] = c;
+    c2 += (double)c.re * c.re + (double)c.im * c.im;
+    if (Pn) {
+      const cx<T> d = Pn[k * nb2 + e] - sD[i * PK_PITCH + j];
+      d2 += (double)d.re * d.re + (double)d.im * d.im;
+    }
+  })
+  c2 = wave_sum(c2);
+  d2 = wave_sum(d2);
+  if (threadIdx.x == 0) {
+    part[k] = c2;
+    part[Nk + k] = d2;
+  }
+}
+
+// Per-k inner products of X with up to 8 matrices Y_v:
+//   mode 0: part[v][k] = Re sum_ij conj(X_ij) Y_v,ij   (Frobenius, solver.py:205-208, reference_scf.py:110-111)
+//   mode 1: part[v][k] = Re sum_ij X_ij Y_v,ji          (trace of the product, reference_scf.py:319)
+struct DotList {
+  const void* y[8];
+};
+template <typename T>
+__global__ __launch_bounds__(64) void pk_dots_kernel(int nb, size_t Nk, const cx<T>* __restrict__ X, DotList L,
+                                                     int nv, int mode, double* __restrict__ part) {
+  const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  for (int v = 0; v < nv; ++v) {
+    const cx<T>* Y = (const cx<T>*)L.y[v];
+    double acc = 0.0;
+    PK_FOR_ELEM(nb, {
+      const cx<T> x = X[k * nb2 + e];
+      const cx<T> y = mode == 0 ? Y[k * nb2 + e] : Y[k * nb2 + (size_t)j * nb + i];
+      acc += mode == 0 ? ((double)x.re * y.re + (double)x.im * y.im) : ((double)x.re * y.re - (double)x.im * y.im);
+    })
+    acc = wave_sum(acc);
+    if (threadIdx.x == 0) part[(size_t)v * Nk + k] = acc;
+  }
+}
+
+// ============================ launchers ======================================
+
+#define PK_DISPATCH(dtype, K, grid, ...)                                                        \
+  do {                                                                                         \
+    if ((dtype) == HFX_DTYPE_C128)                                                             \
+      hipLaunchKernelGGL(K<double>, dim3(grid), dim3(64), 0, s, __VA_ARGS__);                   \
+    else                                                                                       \
+      hipLaunchKernelGGL(K<float>, dim3(grid), dim3(64), 0, s, __VA_ARGS__);                    \
+  } while (0)
+
+template <typename T> using C_ = cx<T>;
+
+int pk_check_nb(int nb) {
+  if (nb < 1 || nb > PK_NBMAX) return set_error(HFX_EUNSUPPORTED, "per-k kernels support 1 <= nb <= %d (got %d)",
+                                                PK_NBMAX, nb);
+  return HFX_OK;
+}
+
+}  // namespace hfx
+
+using namespace hfx;
+
+#define PKC(T, p) ((const cx<T>*)(p))
+#define PKM(T, p) ((cx<T>*)(p))
+
+extern "C" {
+
+int hfx_eigh_batched(int dtype, int nb, size_t nk, const void* A, double* lam, void* V, int herm_in, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  if (!A || !lam || !V) return set_error(HFX_EINVAL, "hfx_eigh_batched: NULL argument");
+  if (nk == 0) return HFX_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(pk_eigh_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKC(double, A), lam, PKM(double, V), herm_in);
+  else
+    hipLaunchKernelGGL(pk_eigh_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKC(float, A), lam, PKM(float, V), herm_in);
+  return check_launch("pk_eigh_kernel");
+}
+
+int hfx_density_from_Qp(int dtype, int nb, size_t nk, const void* Q, const double* p, const double* eps,
+                        const double* mu, double T, double* pout, void* P, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  if (!Q || !P || (!p && (!eps || !mu))) return set_error(HFX_EINVAL, "hfx_density_from_Qp: NULL argument");
+  if (nk == 0) return HFX_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(pk_density_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKC(double, Q), p, eps, mu, T, pout,
+                       PKM(double, P));
+  else
+    hipLaunchKernelGGL(pk_density_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKC(float, Q), p, eps, mu, T, pout,
+                       PKM(float, P));
+  return check_launch("pk_density_kernel");
+}
+
+int hfx_cg_grad(int dtype, int nb, size_t nk, const void* Q, const void* F, const double* p, void* Ft, double* eps,
+                void* G, double* part, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(pk_cg_grad_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(double, Q), PKC(double, F), p,
+                       PKM(double, Ft), eps, PKM(double, G), part);
+  else
+    hipLaunchKernelGGL(pk_cg_grad_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(float, Q), PKC(float, F), p,
+                       PKM(float, Ft), eps, PKM(float, G), part);
+  return check_launch("pk_cg_grad_kernel");
+}
+
+int hfx_cg_precond(int dtype, int nb, size_t nk, const void* G, const void* Gprev, const double* eps, const double* p,
+                   const double* gp_prev, const double* st, double T, void* H, double* gp, double* hp, double* part,
+                   void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(pk_cg_precond_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(double, G),
+                       PKC(double, Gprev), eps, p, gp_prev, st, T, PKM(double, H), gp, hp, part);
+  else
+    hipLaunchKernelGGL(pk_cg_precond_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(float, G), PKC(float, Gprev),
+                       eps, p, gp_prev, st, T, PKM(float, H), gp, hp, part);
+  return check_launch("pk_cg_precond_kernel");
+}
+
+int hfx_cg_direction(int dtype, int nb, size_t nk, const void* H, void* D, const double* hp, double* dp, const void* Ft,
+                     const double* st, void* V, double* lam_d, void* Fe, double* part, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(pk_cg_direction_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKC(double, H), PKM(double, D),
+                       hp, dp, PKC(double, Ft), st, PKM(double, V), lam_d, PKM(double, Fe), part);
+  else
+    hipLaunchKernelGGL(pk_cg_direction_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKC(float, H), PKM(float, D), hp,
+                       dp, PKC(float, Ft), st, PKM(float, V), lam_d, PKM(float, Fe), part);
+  return check_launch("pk_cg_direction_kernel");
+}
+
+int hfx_cg_trial(int dtype, int nb, size_t nk, const void* V, const double* lam_d, const void* Fe, const double* p,
+                 const double* dp, const double* eps, const double* st, int omega0, double* part, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(pk_cg_trial_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(double, V), lam_d,
+                       PKC(double, Fe), p, dp, eps, st, omega0, part);
+  else
+    hipLaunchKernelGGL(pk_cg_trial_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(float, V), lam_d,
+                       PKC(float, Fe), p, dp, eps, st, omega0, part);
+  return check_launch("pk_cg_trial_kernel");
+}
+
+int hfx_cg_retract(int dtype, int nb, size_t nk, void* Q, const void* V, const double* lam_d, const void* Fe,
+                   const double* st, double tau, int use_tau, double* eps_new, void* Ft_out, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(pk_cg_retract_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKM(double, Q), PKC(double, V),
+                       lam_d, PKC(double, Fe), st, tau, use_tau, eps_new, PKM(double, Ft_out));
+  else
+    hipLaunchKernelGGL(pk_cg_retract_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKM(float, Q), PKC(float, V), lam_d,
+                       PKC(float, Fe), st, tau, use_tau, eps_new, PKM(float, Ft_out));
+  return check_launch("pk_cg_retract_kernel");
+}
+
+int hfx_rotate_eigh(int dtype, int nb, size_t nk, void* Q, const void* F, double* eps, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(pk_rotate_eigh_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKM(double, Q), PKC(double, F), eps);
+  else
+    hipLaunchKernelGGL(pk_rotate_eigh_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKM(float, Q), PKC(float, F), eps);
+  return check_launch("pk_rotate_eigh_kernel");
+}
+
+// out = opA(A) opB(B - Bsub) [opC(Cm)], optionally hermitised.  ops: bit0 A^H, bit1 B^H, bit2 C^H.
+int hfx_matmul_batched(int dtype, int nb, size_t nk, const void* A, const void* B, const void* Cm, int ops,
+                       const void* Bsub, int herm_out, void* out, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  if (!A || !B || !out) return set_error(HFX_EINVAL, "hfx_matmul_batched: NULL argument");
+  hipStream_t s = (hipStream_t)stream;
+#define MM_CASE(o, a, b, c)                                                                                  \
+  case o:                                                                                                   \
+    if (dtype == HFX_DTYPE_C128)                                                                            \
+      hipLaunchKernelGGL((pk_mm_kernel<double, a, b, c>), dim3(nk), dim3(64), 0, s, nb, PKC(double, A),        \
+                         PKC(double, B), PKC(double, Cm), PKM(double, out), PKC(double, Bsub), herm_out);        \
+    else                                                                                                    \
+      hipLaunchKernelGGL((pk_mm_kernel<float, a, b, c>), dim3(nk), dim3(64), 0, s, nb, PKC(float, A),          \
+                         PKC(float, B), PKC(float, Cm), PKM(float, out), PKC(float, Bsub), herm_out);            \
+    break;
+  switch (ops & 7) {
+    MM_CASE(0, false, false, false)
+    MM_CASE(1, true, false, false)
+    MM_CASE(2, false, true, false)
+    MM_CASE(3, true, true, false)
+    MM_CASE(4, false, false, true)
+    MM_CASE(5, true, false, true)
+    MM_CASE(6, false, true, true)
+    MM_CASE(7, true, true, true)
+  }
+#undef MM_CASE
+  return check_launch("pk_mm_kernel");
+}
+
+int hfx_scf_resid(int dtype, int nb, size_t nk, const void* F, const void* D, const void* Pn, void* comm,
+                  double* part, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  if (!F || !D || !part) return set_error(HFX_EINVAL, "hfx_scf_resid: NULL argument");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(pk_scf_resid_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(double, F), PKC(double, D),
+                       PKC(double, Pn), PKM(double, comm), part);
+  else
+    hipLaunchKernelGGL(pk_scf_resid_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(float, F), PKC(float, D),
+                       PKC(float, Pn), PKM(float, comm), part);
+  return check_launch("pk_scf_resid_kernel");
+}
+
+int hfx_batched_dots(int dtype, int nb, size_t nk, const void* X, const void* const* Ys, int nv, int mode,
+                     double* part, void* stream) {
+  if (nb < 1 || !X || !Ys || !part || nv < 1 || nv > 8 || (mode != 0 && mode != 1))
+    return set_error(HFX_EINVAL, "hfx_batched_dots: bad argument");
+  DotList L{};
+  for (int v = 0; v < nv; ++v) L.y[v] = Ys[v];
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(pk_dots_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(double, X), L, nv, mode, part);
+  else
+    hipLaunchKernelGGL(pk_dots_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(float, X), L, nv, mode, part);
+  return check_launch("pk_dots_kernel");
+}
+
+}  // extern "C"

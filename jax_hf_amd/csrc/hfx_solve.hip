@@ -1,0 +1,411 @@
+// Deterministic reductions with fused scalar post-ops, and the chemical-
+// potential solvers, for the GPU-resident solver loops.
+//
+// Reductions: per-k partial values (nval, Nk) float64 -> nval sums in a fixed
+// two-level order (bitwise reproducible), optionally weighted by w[k]; thread
+// 0 of the final block then applies the scalar step of the algorithm
+// (gradient norm / preconditioner scale, Polak-Ribiere beta, tau0, line-search
+// acceptance) directly into the device scalar state, so a CG iteration needs
+// no host round trip before its convergence test.
+//
+// Chemical potential: one cooperative launch runs all iterations of
+//   * the Newton-with-bracket solver of solver.py:62-101, or
+//   * the pure bisection of utils.py:353-382 (optionally twice, with the
+//     level shift of reference_scf.py:87-96),
+// each iteration being a grid-wide (count, slope) reduction followed by a
+// grid barrier; every block then evaluates the same fixed-order sum, so all
+// blocks step mu identically.
+#include <hip/hip_cooperative_groups.h>
+
+#include "hfx_common.h"
+#include "hfx_internal.h"
+
+namespace cg = cooperative_groups;
+
+namespace hfx {
+
+constexpr int RED_BLOCK = 256;
+constexpr int RED_MAX_BLOCKS = 256;
+constexpr int RED_MAX_VALS = 16;
+
+__global__ __launch_bounds__(RED_BLOCK) void reduce_stage1(const double* __restrict__ part, int nval, size_t nk,
+                                                           const double* __restrict__ w, unsigned wmask,
+                                                           double* __restrict__ s1) {
+  __shared__ double sh[RED_MAX_VALS * (RED_BLOCK / 64)];
+  const size_t chunk = (nk + gridDim.x - 1) / gridDim.x;
+  const size_t k0 = (size_t)blockIdx.x * chunk, k1 = k0 + chunk < nk ? k0 + chunk : nk;
+  double acc[RED_MAX_VALS];
+#pragma unroll
+  for (int v = 0; v < RED_MAX_VALS; ++v) acc[v] = 0.0;
+  for (size_t k = k0 + threadIdx.x; k < k1; k += RED_BLOCK) {
+    const double wk = w ? w[k] : 1.0;
+#pragma unroll
+    for (int v = 0; v < RED_MAX_VALS; ++v)
+      if (v < nval) acc[v] += ((wmask >> v) & 1u ? wk : 1.0) * part[(size_t)v * nk + k];
+  }
+  block_sum<RED_MAX_VALS>(acc, sh);
+  if (threadIdx.x == 0)
+    for (int v = 0; v < nval; ++v) s1[(size_t)v * gridDim.x + blockIdx.x] = acc[v];
+}
+
+struct PostArgs {
+  int op;
+  double T, denom_scale, count, max_step, shrink, tau_fallback;
+  int restart, last, omega0;
+};
+
+__global__ __launch_bounds__(RED_BLOCK) void reduce_stage2(const double* __restrict__ s1, int nval, int nblk,
+                                                           double* __restrict__ out, double* __restrict__ st,
+                                                           PostArgs pa) {
+  __shared__ double sh[RED_MAX_VALS * (RED_BLOCK / 64)];
+  if (pa.op == HFX_OP_TRIAL && st[HFX_ST_ACC] != 0.0) return;
+  double acc[RED_MAX_VALS];
+#pragma unroll
+  for (int v = 0; v < RED_MAX_VALS; ++v) acc[v] = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += RED_BLOCK)
+#pragma unroll
+    for (int v = 0; v < RED_MAX_VALS; ++v)
+      if (v < nval) acc[v] += s1[(size_t)v * nblk + b];
+  block_sum<RED_MAX_VALS>(acc, sh);
+  if (threadIdx.x != 0) return;
+  if (out)
+    for (int v = 0; v < nval; ++v) out[v] = acc[v];
+  const double tiny = 1e-30;
+  switch (pa.op) {
+    case HFX_OP_GRAD: {   // solver.py:491, 496-497
+      st[HFX_ST_GRAD] = sqrt(fmax(0.0, acc[0]));
+      const double eps_scale = sqrt(acc[1] / pa.count + tiny);
+      st[HFX_ST_LAM] = fmax(pa.T, pa.denom_scale * eps_scale);
+    } break;
+    case HFX_OP_BETA: {   // solver.py:517-526
+      const double num = acc[0] - acc[1] + acc[2] - acc[3];
+      const double den = acc[4] + acc[5];
+      double beta = den > tiny ? fmax(0.0, num / den) : 0.0;
+      beta = fmin(beta, 5.0);
+      if (pa.restart) beta = 0.0;
+      st[HFX_ST_BETA] = beta;
+    } break;
+    case HFX_OP_TAU0: {   // solver.py:546-547
+      const double dn = sqrt(fmax(0.0, acc[0]));
+      const double tau0 = fmin(1.0, pa.max_step / fmax(dn, tiny));
+      st[HFX_ST_DNORM] = dn;
+      st[HFX_ST_TAU0] = tau0;
+      st[HFX_ST_TAU] = tau0;
+      st[HFX_ST_ACC] = 0.0;
+      st[HFX_ST_TAUF] = tau0 * pa.tau_fallback;   // used when no trial runs (bt_max = 0)
+    } break;
+    case HFX_OP_TRIAL: {  // solver.py:549-579 (free_energy = E - T S, S = -sum w s)
+      const double Tv = fmax(pa.T, 1e-14);
+      if (pa.omega0) st[HFX_ST_OMEGA0] = acc[2] + Tv * acc[3];
+      const double om = acc[0] + Tv * acc[1];
+      st[HFX_ST_OMEGA] = om;
+      if (om < st[HFX_ST_OMEGA0]) {
+        st[HFX_ST_ACC] = 1.0;
+        st[HFX_ST_TAUF] = st[HFX_ST_TAU];
+      } else {
+        st[HFX_ST_TAU] = st[HFX_ST_TAU] * pa.shrink;
+        if (pa.last) st[HFX_ST_TAUF] = st[HFX_ST_TAU0] * pa.tau_fallback;
+      }
+    } break;
+    default:
+      break;
+  }
+}
+
+int reduce_blocks(size_t nk) {
+  size_t g = (nk + RED_BLOCK - 1) / RED_BLOCK;
+  if (g > RED_MAX_BLOCKS) g = RED_MAX_BLOCKS;
+  return (int)(g < 1 ? 1 : g);
+}
+
+// ---------------------------------------------------------------------------
+// chemical potential
+// ---------------------------------------------------------------------------
+constexpr int MU_BLOCK = 256;
+
+struct MuArgs {
+  const double* eps;   // (nk, nb)
+  const double* w;     // (nk)
+  size_t nk;
+  int nb;
+  double target, T, mu0, level_shift;
+  int maxiter, mode, mu0_from_state;
+  double* st;          // reads st[HFX_ST_MU] as mu0 when mu0_from_state; writes st[HFX_ST_MU]
+  double* mu_out;      // optional extra output
+  double* p_out;       // optional occupations (nk, nb)
+  double* scratch;     // >= 4 * gridDim.x doubles
+};
+
+__device__ __forceinline__ double sigm(double x) { return 1.0 / (1.0 + exp(-x)); }
+
+// grid-wide deterministic sum of two values; all threads receive the totals
+__device__ void grid_sum2(cg::grid_group& grid, double a, double b, double* scratch, int slot, double* sh,
+                          double& ta, double& tb) {
+  double acc[2] = {a, b};
+  block_sum<2>(acc, sh);
+  if (threadIdx.x == 0) {
+    scratch[(size_t)slot * 2 * gridDim.x + 2 * blockIdx.x] = acc[0];
+    scratch[(size_t)slot * 2 * gridDim.x + 2 * blockIdx.x + 1] = acc[1];
+  }
+  grid.sync();
+  __shared__ double tot[2];
+  if (threadIdx.x == 0) {
+    double x = 0.0, y = 0.0;
+    for (unsigned b = 0; b < gridDim.x; ++b) {
+      x += scratch[(size_t)slot * 2 * gridDim.x + 2 * b];
+      y += scratch[(size_t)slot * 2 * gridDim.x + 2 * b + 1];
+    }
+    tot[0] = x;
+    tot[1] = y;
+  }
+  __syncthreads();
+  ta = tot[0];
+  tb = tot[1];
+  __syncthreads();
+}
+
+__device__ void grid_minmax(cg::grid_group& grid, const double* v, size_t n, double* scratch, double* sh,
+                            double& mn, double& mx) {
+  double lo = INFINITY, hi = -INFINITY;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+    lo = fmin(lo, v[e]);
+    hi = fmax(hi, v[e]);
+  }
+  lo = wave_min(lo);
+  hi = wave_max(hi);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane == 0) { sh[wid] = lo; sh[nw + wid] = hi; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < nw; ++w) { lo = fmin(lo, sh[w]); hi = fmax(hi, sh[nw + w]); }
+    scratch[2 * blockIdx.x] = lo;
+    scratch[2 * blockIdx.x + 1] = hi;
+  }
+  grid.sync();
+  __shared__ double tot[2];
+  if (threadIdx.x == 0) {
+    double a = INFINITY, b = -INFINITY;
+    for (unsigned q = 0; q < gridDim.x; ++q) { a = fmin(a, scratch[2 * q]); b = fmax(b, scratch[2 * q + 1]); }
+    tot[0] = a;
+    tot[1] = b;
+  }
+  __syncthreads();
+  mn = tot[0];
+  mx = tot[1];
+  __syncthreads();
+  grid.sync();   // scratch is reused right after
+}
+
+// value of element e after the optional level shift (reference_scf.py:91-92)
+__device__ __forceinline__ double shifted(const MuArgs& a, size_t e, double mu_raw, bool use_shift) {
+  const double x = a.eps[e];
+  if (!use_shift) return x;
+  const double f = sigm(-(x - mu_raw) / (a.T + 1e-12));
+  return x + a.level_shift * (1.0 - f);
+}
+
+__device__ double bisect(cg::grid_group& grid, const MuArgs& a, double mu_raw, bool use_shift, double* sh) {
+  const size_t n = a.nk * a.nb, stride = (size_t)gridDim.x * blockDim.x;
+  double mn = INFINITY, mx = -INFINITY;
+  {
+    double lo = INFINITY, hi = -INFINITY;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+      const double v = shifted(a, e, mu_raw, use_shift);
+      lo = fmin(lo, v);
+      hi = fmax(hi, v);
+    }
+    // reuse grid_minmax machinery on the fly
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0) { sh[wid] = lo; sh[nw + wid] = hi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 0; w < nw; ++w) { lo = fmin(lo, sh[w]); hi = fmax(hi, sh[nw + w]); }
+      a.scratch[2 * blockIdx.x] = lo;
+      a.scratch[2 * blockIdx.x + 1] = hi;
+    }
+    grid.sync();
+    __shared__ double tot[2];
+    if (threadIdx.x == 0) {
+      double p = INFINITY, q = -INFINITY;
+      for (unsigned b = 0; b < gridDim.x; ++b) { p = fmin(p, a.scratch[2 * b]); q = fmax(q, a.scratch[2 * b + 1]); }
+      tot[0] = p;
+      tot[1] = q;
+    }
+    __syncthreads();
+    mn = tot[0];
+    mx = tot[1];
+    __syncthreads();
+    grid.sync();
+  }
+  const double span = mx - mn + 10.0 * fmax(a.T, 1e-6);
+  double lo = mn - span, hi = mx + span;
+  for (int it = 0; it < a.maxiter; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    double c = 0.0;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+      const double v = shifted(a, e, mu_raw, use_shift);
+      c += a.w[e / a.nb] * sigm(-(v - mid) / (a.T + 1e-12));
+    }
+    double cnt, dummy;
+    grid_sum2(grid, c, 0.0, a.scratch, 1 + (it & 1), sh, cnt, dummy);
+    if (cnt > a.target) hi = mid;
+    else lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+__global__ __launch_bounds__(MU_BLOCK) void mu_kernel(MuArgs a) {
+  cg::grid_group grid = cg::this_grid();
+  __shared__ double sh[2 * (MU_BLOCK / 64)];
+  const size_t n = a.nk * a.nb, stride = (size_t)gridDim.x * blockDim.x;
+  double mu;
+  bool use_shift = false;
+  double mu_raw = 0.0;
+  if (a.mode == HFX_MU_NEWTON_BRACKET) {   // solver.py:62-101
+    const double Tj = fmax(a.T, 1e-12);
+    double mn, mx;
+    grid_minmax(grid, a.eps, n, a.scratch, sh, mn, mx);
+    double lo = mn - 50.0 * Tj, hi = mx + 50.0 * Tj;
+    mu = a.mu0_from_state ? a.st[HFX_ST_MU] : a.mu0;
+    mu = fmin(fmax(mu, lo), hi);
+    for (int it = 0; it < a.maxiter; ++it) {
+      double Np = 0.0, Zp = 0.0;
+      for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+        const double pv = sigm((mu - a.eps[e]) / Tj);
+        const double wk = a.w[e / a.nb];
+        Np += wk * pv;
+        Zp += wk * pv * (1.0 - pv) / Tj;
+      }
+      double N, Z;
+      grid_sum2(grid, Np, Zp, a.scratch, 1 + (it & 1), sh, N, Z);
+      const double g = N - a.target;
+      if (g < 0) lo = mu;
+      if (g > 0) hi = mu;
+      double mu_new = mu - g / fmax(Z, 1e-18);
+      const double mu_bis = 0.5 * (lo + hi);
+      if (mu_new <= lo || mu_new >= hi) mu_new = mu_bis;
+      mu_new = fmin(fmax(mu_new, lo), hi);
+      if (!isfinite(mu_new)) mu_new = mu_bis;
+      mu = mu_new;
+    }
+  } else if (a.mode == HFX_MU_NEWTON) {   // utils.py:385-442
+    const double Tj = fmax(a.T, 1e-12);
+    double mn, mx;
+    grid_minmax(grid, a.eps, n, a.scratch, sh, mn, mx);
+    const double span = mx - mn + 10.0 * fmax(Tj, 1e-6);
+    double lo = mn - span, hi = mx + span;
+    mu = 0.5 * (lo + hi);
+    double N = 0.0, Z = 0.0;
+    for (int it = 0; it <= a.maxiter; ++it) {
+      double Np = 0.0, Zp = 0.0;
+      for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+        const double pv = sigm((mu - a.eps[e]) / Tj);
+        const double wk = a.w[e / a.nb];
+        Np += wk * pv;
+        Zp += wk * pv * (1.0 - pv) / Tj;
+      }
+      grid_sum2(grid, Np, Zp, a.scratch, 1 + (it & 1), sh, N, Z);
+      if (it == a.maxiter) break;   // final residual check below
+      const double g = N - a.target;
+      if (g < 0) lo = mu;
+      if (g > 0) hi = mu;
+      double mu_new = mu - g / fmax(Z, 1e-18);
+      const double mu_bis = 0.5 * (lo + hi);
+      if (mu_new <= lo || mu_new >= hi) mu_new = mu_bis;
+      mu_new = fmin(fmax(mu_new, lo), hi);
+      if (!isfinite(mu_new)) mu_new = mu_bis;
+      mu = mu_new;
+    }
+    if (fabs(N - a.target) > 1e-12) mu = 0.5 * (lo + hi);
+  } else {   // HFX_MU_BISECTION (utils.py:353-382) with optional level shift
+    mu = bisect(grid, a, 0.0, false, sh);
+    if (a.level_shift != 0.0) {
+      mu_raw = mu;
+      use_shift = true;
+      mu = bisect(grid, a, mu_raw, true, sh);
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (a.st) a.st[HFX_ST_MU] = mu;
+    if (a.mu_out) *a.mu_out = mu;
+  }
+  if (a.p_out) {
+    const double Tj = fmax(a.T, 1e-12);
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+      if (a.mode != HFX_MU_BISECTION) a.p_out[e] = sigm((mu - a.eps[e]) / Tj);   // solver.py:394, 592
+      else a.p_out[e] = sigm(-(shifted(a, e, mu_raw, use_shift) - mu) / (a.T + 1e-12));  // fermidirac
+    }
+  }
+}
+
+static int g_mu_blocks = 0;
+
+int mu_blocks() {
+  if (g_mu_blocks) return g_mu_blocks;
+  int dev = 0, ncu = 0, occ = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mu_kernel, MU_BLOCK, 0);
+  int b = ncu * (occ > 0 ? (occ < 2 ? occ : 2) : 1);
+  if (b > 512) b = 512;
+  if (b < 1) b = 1;
+  g_mu_blocks = b;
+  return b;
+}
+
+}  // namespace hfx
+
+using namespace hfx;
+
+extern "C" {
+
+size_t hfx_solver_scratch_doubles(void) {
+  return (size_t)RED_MAX_VALS * RED_MAX_BLOCKS + 8 * 512;
+}
+
+int hfx_reduce(const double* part, int nval, size_t nk, const double* w, unsigned wmask, double* out, double* st,
+               int op, const double* params, double* scratch, void* stream) {
+  if (!part || !scratch || nval < 1 || nval > RED_MAX_VALS) return set_error(HFX_EINVAL, "hfx_reduce: bad argument");
+  if (op != HFX_OP_SUM && !st) return set_error(HFX_EINVAL, "hfx_reduce: op needs the state vector");
+  hipStream_t s = (hipStream_t)stream;
+  const int g = reduce_blocks(nk);
+  PostArgs pa{};
+  pa.op = op;
+  if (params) {
+    pa.T = params[0];
+    pa.denom_scale = params[1];
+    pa.count = params[2];
+    pa.max_step = params[3];
+    pa.shrink = params[4];
+    pa.tau_fallback = params[5];
+    pa.restart = (int)params[6];
+    pa.last = (int)params[7];
+    pa.omega0 = (int)params[8];
+  }
+  hipLaunchKernelGGL(reduce_stage1, dim3(g), dim3(RED_BLOCK), 0, s, part, nval, nk, w, wmask, scratch);
+  int rc = check_launch("reduce_stage1");
+  if (rc) return rc;
+  hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(RED_BLOCK), 0, s, scratch, nval, g, out, st, pa);
+  return check_launch("reduce_stage2");
+}
+
+int hfx_solve_mu(const double* eps, const double* w, size_t nk, int nb, double target, double T, double mu0,
+                 int mu0_from_state, int maxiter, int mode, double level_shift, double* st, double* mu_out,
+                 double* p_out, double* scratch, void* stream) {
+  if (!eps || !w || !scratch || nb < 1 || nk == 0) return set_error(HFX_EINVAL, "hfx_solve_mu: bad argument");
+  if (mode != HFX_MU_NEWTON_BRACKET && mode != HFX_MU_BISECTION && mode != HFX_MU_NEWTON)
+    return set_error(HFX_EINVAL, "hfx_solve_mu: unknown mode %d", mode);
+  if (mu0_from_state && !st) return set_error(HFX_EINVAL, "hfx_solve_mu: mu0_from_state needs st");
+  MuArgs a{eps, w, nk, nb, target, T, mu0, level_shift, maxiter, mode, mu0_from_state, st, mu_out, p_out, scratch};
+  const int blocks = mu_blocks();
+  void* args[] = {&a};
+  hipError_t e = hipLaunchCooperativeKernel((const void*)mu_kernel, dim3(blocks), dim3(MU_BLOCK), args, 0,
+                                            (hipStream_t)stream);
+  if (e != hipSuccess) return set_error(HFX_EHIP, "mu_kernel cooperative launch: %s", hipGetErrorString(e));
+  return check_launch("mu_kernel");
+}
+
+}  // extern "C"

@@ -1,0 +1,217 @@
+// Small elementwise / reduction kernels around the Fock build:
+//   * precision conversion between the storage kinds (mixed-precision plumbing),
+//   * hermitisation 0.5 (X + X^dagger) (fock.py:13-15, used after project_fn),
+//   * max |P| and max |P| over an off-block channel mask (utils.py:263-266),
+//   * occupation entropy / free energy (fock.py:114-129).
+#include "hfx_common.h"
+#include "hfx_internal.h"
+
+namespace hfx {
+
+constexpr int UTIL_BLOCK = 256;
+constexpr int UTIL_MAX_BLOCKS = 1024;
+
+enum Kind { K_F64 = 0, K_F32 = 1, K_C128 = 2, K_C64 = 3 };
+
+__device__ __forceinline__ void load_any(const void* p, int kind, size_t i, double& re, double& im) {
+  switch (kind) {
+    case K_F64: re = ((const double*)p)[i]; im = 0.0; break;
+    case K_F32: re = ((const float*)p)[i]; im = 0.0; break;
+    case K_C128: { cx<double> v = ((const cx<double>*)p)[i]; re = v.re; im = v.im; } break;
+    default: { cx<float> v = ((const cx<float>*)p)[i]; re = v.re; im = v.im; } break;
+  }
+}
+__device__ __forceinline__ void store_any(void* p, int kind, size_t i, double re, double im) {
+  switch (kind) {
+    case K_F64: ((double*)p)[i] = re; break;
+    case K_F32: ((float*)p)[i] = (float)re; break;
+    case K_C128: ((cx<double>*)p)[i] = cx<double>{re, im}; break;
+    default: ((cx<float>*)p)[i] = cx<float>{(float)re, (float)im}; break;
+  }
+}
+
+__global__ void convert_kernel(const void* src, int sk, void* dst, int dk, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * UTIL_BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * UTIL_BLOCK) {
+    double re, im;
+    load_any(src, sk, i, re, im);
+    store_any(dst, dk, i, re, im);
+  }
+}
+
+template <typename T>
+__global__ void hermitize_kernel(const cx<T>* X, cx<T>* Y, size_t nmat, int nb) {
+  const size_t nb2 = (size_t)nb * nb, total = nmat * nb2;
+  for (size_t e = (size_t)blockIdx.x * UTIL_BLOCK + threadIdx.x; e < total; e += (size_t)gridDim.x * UTIL_BLOCK) {
+    const size_t k = e / nb2;
+    const int c = (int)(e - k * nb2), i = c / nb, j = c - i * nb;
+    const cx<T> a = X[e], b = X[k * nb2 + (size_t)j * nb + i];
+    Y[e] = (T)0.5 * (a + conj(b));
+  }
+}
+
+// out[0] = max |X|, out[1] = max |X| over channels with mask[c] != 0 (per block, then reduced)
+template <typename T>
+__global__ void absmax_kernel(const cx<T>* X, size_t nmat, int nb2, const unsigned char* mask, double* part) {
+  __shared__ double sh[2 * (UTIL_BLOCK / 64)];
+  double m0 = 0.0, m1 = 0.0;
+  const size_t total = nmat * nb2;
+  for (size_t e = (size_t)blockIdx.x * UTIL_BLOCK + threadIdx.x; e < total; e += (size_t)gridDim.x * UTIL_BLOCK) {
+    const cx<T> v = X[e];
+    const double a = sqrt((double)v.re * v.re + (double)v.im * v.im);
+    m0 = fmax(m0, a);
+    if (mask && mask[e % nb2]) m1 = fmax(m1, a);
+  }
+  m0 = wave_max(m0);
+  m1 = wave_max(m1);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { sh[wid] = m0; sh[UTIL_BLOCK / 64 + wid] = m1; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < UTIL_BLOCK / 64; ++w) { m0 = fmax(m0, sh[w]); m1 = fmax(m1, sh[UTIL_BLOCK / 64 + w]); }
+    m0 = fmax(m0, sh[0]);
+    m1 = fmax(m1, sh[UTIL_BLOCK / 64]);
+    part[2 * blockIdx.x] = m0;
+    part[2 * blockIdx.x + 1] = m1;
+  }
+}
+
+__global__ void absmax_final_kernel(const double* part, int n, double* out) {
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int i = 0; i < n; ++i) { a = fmax(a, part[2 * i]); b = fmax(b, part[2 * i + 1]); }
+    out[0] = a;
+    out[1] = b;
+  }
+}
+
+// S = -sum_k w_k sum_i [p ln p + (1-p) ln(1-p)], p clipped to [1e-14, 1-1e-14]
+template <typename R>
+__global__ void entropy_kernel(const R* p, const double* w, size_t nk, int nb, double* part) {
+  __shared__ double sh[UTIL_BLOCK / 64];
+  double s = 0.0;
+  const size_t total = nk * nb;
+  for (size_t e = (size_t)blockIdx.x * UTIL_BLOCK + threadIdx.x; e < total; e += (size_t)gridDim.x * UTIL_BLOCK) {
+    double x = (double)p[e];
+    x = fmin(fmax(x, 1e-14), 1.0 - 1e-14);
+    s += w[e / nb] * (x * log(x) + (1.0 - x) * log1p(-x));
+  }
+  double acc[1] = {s};
+  block_sum<1>(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = -acc[0];
+}
+
+__global__ void entropy_final_kernel(const double* part, int n, const double* E, double T, double* out) {
+  __shared__ double sh[1024 / 64];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += part[i];
+  double acc[1] = {s};
+  block_sum<1>(acc, sh);
+  if (threadIdx.x == 0) *out = E ? (*E - fmax(T, 1e-14) * acc[0]) : acc[0];
+}
+
+// out = a X + b Y (complex a, b; Y may be NULL), elementwise over n values
+__global__ void lincomb_kernel(size_t n, int kind, double ar, double ai, const void* X, double br, double bi,
+                               const void* Y, void* out) {
+  for (size_t i = (size_t)blockIdx.x * UTIL_BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * UTIL_BLOCK) {
+    double xr, xi, yr = 0.0, yi = 0.0;
+    load_any(X, kind, i, xr, xi);
+    if (Y) load_any(Y, kind, i, yr, yi);
+    const double r = ar * xr - ai * xi + br * yr - bi * yi;
+    const double m = ar * xi + ai * xr + br * yi + bi * yr;
+    store_any(out, kind, i, r, m);
+  }
+}
+
+// out = expit(-(x - mu) / (T + 1e-12))  (utils.py:51-53)
+__global__ void fermi_kernel(const double* x, const double* mu, double T, size_t n, double* out) {
+  const double m = mu ? *mu : 0.0;
+  for (size_t i = (size_t)blockIdx.x * UTIL_BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * UTIL_BLOCK)
+    out[i] = 1.0 / (1.0 + exp((x[i] - m) / (T + 1e-12)));
+}
+
+static int grid_for(size_t n) {
+  size_t g = (n + UTIL_BLOCK - 1) / UTIL_BLOCK;
+  if (g > UTIL_MAX_BLOCKS) g = UTIL_MAX_BLOCKS;
+  return (int)(g < 1 ? 1 : g);
+}
+
+}  // namespace hfx
+
+using namespace hfx;
+
+extern "C" {
+
+int hfx_convert(const void* src, int src_kind, void* dst, int dst_kind, size_t n, void* stream) {
+  if (!src || !dst || src_kind < 0 || src_kind > 3 || dst_kind < 0 || dst_kind > 3)
+    return set_error(HFX_EINVAL, "hfx_convert: bad argument");
+  if (n == 0) return HFX_OK;
+  hipLaunchKernelGGL(convert_kernel, dim3(grid_for(n)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream, src, src_kind,
+                     dst, dst_kind, n);
+  return check_launch("convert_kernel");
+}
+
+int hfx_lincomb(size_t n, int kind, double ar, double ai, const void* X, double br, double bi, const void* Y,
+                void* out, void* stream) {
+  if (!X || !out || kind < 0 || kind > 3) return set_error(HFX_EINVAL, "hfx_lincomb: bad argument");
+  if (n == 0) return HFX_OK;
+  hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(n)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream, n, kind, ar, ai,
+                     X, br, bi, Y, out);
+  return check_launch("lincomb_kernel");
+}
+
+int hfx_fermi(const double* x, const double* mu, double T, size_t n, double* out, void* stream) {
+  if (!x || !out) return set_error(HFX_EINVAL, "hfx_fermi: bad argument");
+  if (n == 0) return HFX_OK;
+  hipLaunchKernelGGL(fermi_kernel, dim3(grid_for(n)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream, x, mu, T, n, out);
+  return check_launch("fermi_kernel");
+}
+
+int hfx_hermitize(const void* X, void* Y, size_t nmat, int nb, int dtype, void* stream) {
+  if (!X || !Y || nb <= 0) return set_error(HFX_EINVAL, "hfx_hermitize: bad argument");
+  if (X == Y) return set_error(HFX_EINVAL, "hfx_hermitize: in-place not supported");
+  const size_t n = nmat * nb * nb;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(hermitize_kernel<double>, dim3(grid_for(n)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream,
+                       (const cx<double>*)X, (cx<double>*)Y, nmat, nb);
+  else
+    hipLaunchKernelGGL(hermitize_kernel<float>, dim3(grid_for(n)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream,
+                       (const cx<float>*)X, (cx<float>*)Y, nmat, nb);
+  return check_launch("hermitize_kernel");
+}
+
+int hfx_absmax_masked(const void* X, size_t nmat, int nb, int dtype, const unsigned char* mask, double* scratch,
+                      double* out, void* stream) {
+  if (!X || !scratch || !out || nb <= 0) return set_error(HFX_EINVAL, "hfx_absmax_masked: bad argument");
+  const size_t n = nmat * nb * nb;
+  const int g = grid_for(n) > 512 ? 512 : grid_for(n);
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(absmax_kernel<double>, dim3(g), dim3(UTIL_BLOCK), 0, (hipStream_t)stream,
+                       (const cx<double>*)X, nmat, nb * nb, mask, scratch);
+  else
+    hipLaunchKernelGGL(absmax_kernel<float>, dim3(g), dim3(UTIL_BLOCK), 0, (hipStream_t)stream,
+                       (const cx<float>*)X, nmat, nb * nb, mask, scratch);
+  int rc = check_launch("absmax_kernel");
+  if (rc) return rc;
+  hipLaunchKernelGGL(absmax_final_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, scratch, g, out);
+  return check_launch("absmax_final_kernel");
+}
+
+int hfx_entropy(const void* p, int real_kind, const double* w, size_t nk, int nb, const double* E, double T,
+                double* scratch, double* out, void* stream) {
+  if (!p || !w || !scratch || !out || nb <= 0 || (real_kind != 0 && real_kind != 1))
+    return set_error(HFX_EINVAL, "hfx_entropy: bad argument");
+  const size_t n = nk * nb;
+  const int g = grid_for(n);
+  if (real_kind == 0)
+    hipLaunchKernelGGL(entropy_kernel<double>, dim3(g), dim3(UTIL_BLOCK), 0, (hipStream_t)stream,
+                       (const double*)p, w, nk, nb, scratch);
+  else
+    hipLaunchKernelGGL(entropy_kernel<float>, dim3(g), dim3(UTIL_BLOCK), 0, (hipStream_t)stream,
+                       (const float*)p, w, nk, nb, scratch);
+  int rc = check_launch("entropy_kernel");
+  if (rc) return rc;
+  hipLaunchKernelGGL(entropy_final_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, scratch, g, E, T, out);
+  return check_launch("entropy_final_kernel");
+}
+
+}  // extern "C"

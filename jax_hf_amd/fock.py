@@ -1,0 +1,379 @@
+"""Fock-matrix construction and energy evaluation on the GPU (drop-in for fock.py).
+
+Mirrors ``src/jax_hf/fock.py:18-129`` (``build_fock``, ``hf_energy``,
+``occupation_entropy``, ``free_energy``) with the same keyword signatures.
+Arrays are torch device tensors (numpy inputs are uploaded); every arithmetic
+step runs in libhfx HIP kernels:
+
+* exchange: fused LDS-resident 2D FFT convolution per (i, j) channel
+  (``hfx_selfenergy``), optionally on the Hermitian upper-triangle channel set
+  or on block-diagonal channel subsets;
+* Hartree / contact: weighted channel moments (a by-product of the exchange
+  kernel) -> ``hfx_kterms``;
+* ``F = herm(h + Sigma + H)`` and the energy trace: ``hfx_fock_epilogue``.
+"""
+
+from __future__ import annotations
+
+from typing import Any
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from ._util import KIND, convert, host_array, real_dtype_of, to_tensor
+
+__all__ = ["build_fock", "hf_energy", "occupation_entropy", "free_energy", "FockEngine",
+           "normalize_block_specs"]
+
+
+# ---------------------------------------------------------------------------
+# block specs (linalg.py:19-90)
+# ---------------------------------------------------------------------------
+
+def normalize_block_specs(block_specs: Any):
+    """Canonical (("sizes", ...) | ("indices", ...), ...) form (linalg.py:19-90)."""
+    if block_specs is None:
+        return None
+    items = (block_specs,) if isinstance(block_specs, dict) else block_specs
+    if not isinstance(items, (list, tuple)):
+        raise TypeError("block_specs must be a sequence of specs or a single spec dict; "
+                        f"got {type(block_specs)}.")
+    out = []
+    for spec in items:
+        if spec is None:
+            raise TypeError("block_specs entries must be non-None.")
+        if isinstance(spec, tuple) and len(spec) == 2 and isinstance(spec[0], str) \
+                and spec[0].strip().lower() in ("sizes", "indices"):
+            kind = spec[0].strip().lower()
+            if kind == "sizes":
+                out.append(("sizes", tuple(int(x) for x in spec[1])))
+            else:
+                out.append(("indices", tuple(tuple(int(i) for i in b) for b in spec[1])))
+        elif isinstance(spec, dict):
+            if "block_sizes" in spec:
+                out.append(("sizes", tuple(int(x) for x in spec["block_sizes"])))
+            elif "block_indices" in spec:
+                out.append(("indices", tuple(tuple(int(i) for i in b) for b in spec["block_indices"])))
+            else:
+                raise TypeError("Spec dict must contain 'block_sizes' or 'block_indices'.")
+        else:
+            seq = list(spec)
+            if len(seq) == 0:
+                raise TypeError("Spec sequences must be non-empty.")
+            if all(isinstance(x, (int, np.integer)) for x in seq):
+                out.append(("sizes", tuple(int(x) for x in seq)))
+            else:
+                out.append(("indices", tuple(tuple(int(i) for i in b) for b in seq)))
+    return tuple(out)
+
+
+def _block_sets(kind, data, nb):
+    if kind == "sizes":
+        if sum(data) != nb:
+            raise ValueError(f"block_sizes must sum to {nb}.")
+        sets, s = [], 0
+        for size in data:
+            sets.append(tuple(range(s, s + size)))
+            s += size
+        return sets
+    if kind == "indices":
+        return [tuple(b) for b in data]
+    raise ValueError("block_specs kind must be 'sizes' or 'indices'.")
+
+
+# ---------------------------------------------------------------------------
+# Engine: one Fock build path with preallocated device buffers
+# ---------------------------------------------------------------------------
+
+class FockEngine:
+    """Fock build + energy for fixed (h, VR, refP, HH, contact) on one device.
+
+    Holds the plan and the small reduction buffers so repeated builds inside a
+    solver allocate only their outputs.  ``herm`` selects the Hermitian
+    upper-triangle channel set (valid for Hermitian P and real symmetric V).
+    """
+
+    def __init__(self, *, h, VR, refP, HH, w2d, include_exchange, include_hartree, herm,
+                 contact_g=None, contact_Oi=None, contact_Oj=None, e_offset=0.0):
+        self.h = h
+        self.dev = h.device
+        self.cdt = h.dtype
+        nk1, nk2, nb = h.shape[0], h.shape[1], h.shape[-1]
+        self.shape = (nk1, nk2, nb)
+        self.nb = nb
+        self.plan = L.plan_for(nk1, nk2, nb, self.cdt, self.dev)
+        self.VR = VR
+        self.vfull = 1 if VR is not None and VR.shape[-1] != 1 else 0
+        if herm and VR is not None and self.vfull and VR.shape[-1] != nb:
+            raise ValueError("VR orbital shape mismatch")
+        self.refP = refP
+        self.w2d = to_tensor(w2d, self.dev, torch.float64)
+        self.include_exchange = bool(include_exchange)
+        self.include_hartree = bool(include_hartree)
+        self.herm = bool(herm)
+        self.HH = to_tensor(HH, self.dev, torch.float64) if include_hartree else None
+        self.e_offset = float(e_offset)
+        g = None if contact_g is None else host_array(contact_g).astype(np.float64).reshape(-1)
+        self.contact_on = (g is not None and contact_Oi is not None and contact_Oj is not None
+                           and bool(np.any(g != 0.0)))
+        if self.contact_on:
+            self.cg = to_tensor(g, self.dev, torch.float64)
+            self.cOi = to_tensor(contact_Oi, self.dev, torch.complex128)
+            self.cOj = to_tensor(contact_Oj, self.dev, torch.complex128)
+            self.ntc = int(g.shape[0])
+        self.need_rho = self.include_hartree or self.contact_on
+        f64 = dict(dtype=torch.float64, device=self.dev)
+        self.rho = torch.zeros((nb, nb), dtype=torch.complex128, device=self.dev)
+        self.Hdiag = torch.zeros(nb, **f64) if self.include_hartree else None
+        self.sigc = torch.zeros((nb, nb), dtype=torch.complex128, device=self.dev) if self.contact_on else None
+        self.E = torch.zeros((), **f64)
+        self._sig_ws = None
+
+    # -- pieces -------------------------------------------------------------
+    def selfenergy(self, P, out, refP=None, chans=None, nchan=0, herm=None, rho=True, ifftshift=False):
+        herm = self.herm if herm is None else herm
+        L.call("hfx_selfenergy", self.plan.handle, L.ptr(P), L.ptr(refP), L.ptr(self.VR), self.vfull,
+               L.ptr(chans), int(nchan), int(herm), int(ifftshift),
+               L.ptr(self.w2d) if rho else None, L.ptr(out), L.ptr(self.rho) if rho else None,
+               L.stream(self.dev))
+
+    def kterms(self):
+        L.call("hfx_kterms", self.plan.handle, L.ptr(self.rho), L.ptr(self.HH),
+               self.ntc if self.contact_on else 0,
+               L.ptr(self.cg) if self.contact_on else None,
+               L.ptr(self.cOi) if self.contact_on else None,
+               L.ptr(self.cOj) if self.contact_on else None,
+               L.ptr(self.Hdiag), L.ptr(self.sigc), L.stream(self.dev))
+
+    def sigma_workspace(self, P):
+        if self._sig_ws is None or self._sig_ws.shape != P.shape:
+            self._sig_ws = torch.empty_like(P)
+        return self._sig_ws
+
+    def build(self, P, F, *, Sigma_out=None, energy=False, refP="default", chans=None, nchan=0,
+              zero_sigma=False, moments_separately=False, E_ptr=None, sub_h=False):
+        """F (and optionally Sigma_out, E) at density P.  Returns the device scalar E or None.
+
+        ``E_ptr`` redirects the energy to another device float64 address;
+        ``sub_h`` returns the interaction response F - h instead of F.
+        """
+        refP = self.refP if isinstance(refP, str) else refP
+        st = L.stream(self.dev)
+        Sig = None
+        if self.include_exchange:
+            Sig = Sigma_out if (Sigma_out is not None and not self.contact_on) else self.sigma_workspace(P)
+            if zero_sigma:
+                Sig.zero_()
+            self.selfenergy(P, Sig, refP=refP, chans=chans, nchan=nchan,
+                            rho=self.need_rho and not moments_separately)
+        if self.need_rho and (not self.include_exchange or moments_separately):
+            L.call("hfx_density_moments", self.plan.handle, L.ptr(P), L.ptr(refP), L.ptr(self.w2d),
+                   L.ptr(self.rho), st)
+        if self.need_rho:
+            self.kterms()
+        sig_out = Sigma_out if (Sigma_out is not None and Sigma_out is not Sig) else None
+        E_dst = (E_ptr if E_ptr is not None else L.ptr(self.E)) if energy else None
+        L.call("hfx_fock_epilogue", self.plan.handle, L.ptr(self.h), L.ptr(Sig), L.ptr(self.Hdiag),
+               L.ptr(self.sigc), L.ptr(P) if energy else None, L.ptr(self.w2d), L.ptr(F),
+               L.ptr(sig_out), L.ptr(self.plan.scratch), E_dst, self.e_offset, st)
+        if sub_h:
+            _sub_inplace(F, self.h)
+        if Sigma_out is not None and Sig is None:
+            # no exchange: Sigma_out = contact term broadcast (or zeros)
+            Sigma_out.zero_()
+            if self.contact_on:
+                L.call("hfx_fock_epilogue", self.plan.handle, L.ptr(_zeros_like_cached(P)), None, None,
+                       L.ptr(self.sigc), None, L.ptr(self.w2d), L.ptr(self.sigma_workspace(P)),
+                       L.ptr(Sigma_out), L.ptr(self.plan.scratch), None, 0.0, st)
+        return self.E if energy else None
+
+    def hartree_matrix_out(self):
+        """H of build_fock: (1, 1, nb, nb) real (fock.py:66-74) or zeros_like(h)."""
+        nb = self.nb
+        rdt = real_dtype_of(self.cdt)
+        if not self.include_hartree:
+            return torch.zeros((1, 1, nb, nb), dtype=self.cdt, device=self.dev).expand(self.h.shape)
+        Hm = torch.zeros((1, 1, nb, nb), dtype=torch.float64, device=self.dev)
+        Hm.view(nb * nb)[:: nb + 1].copy_(self.Hdiag)
+        return convert(Hm, rdt)
+
+
+def _sub_inplace(X, Y):
+    """X <- X - Y on the device (libhfx lincomb)."""
+    L.call("hfx_lincomb", X.numel(), KIND[X.dtype], 1.0, 0.0, L.ptr(X), -1.0, 0.0, L.ptr(Y), L.ptr(X),
+           L.stream(X.device))
+
+
+_zero_cache: dict = {}
+
+
+def _zeros_like_cached(P):
+    key = (tuple(P.shape), P.dtype, P.device)
+    z = _zero_cache.get(key)
+    if z is None:
+        z = torch.zeros_like(P)
+        _zero_cache[key] = z
+    return z
+
+
+# ---------------------------------------------------------------------------
+# public API (fock.py:18-129)
+# ---------------------------------------------------------------------------
+
+def build_fock(P, *, h, VR, refP, HH, w2d, include_exchange: bool, include_hartree: bool,
+               exchange_hermitian_channel_packing: bool, contact_g=None, contact_Oi=None,
+               contact_Oj=None, exchange_block_specs=None, exchange_check_offdiag=None,
+               exchange_offdiag_atol: float = 1e-12, exchange_offdiag_rtol: float = 0.0,
+               project_fn=None):
+    """Build the Fock matrix and return ``(Sigma, H, F)`` (fock.py:18-97).
+
+    ``F = herm(h + Sigma[P-refP] + H[P-refP] + sigma_contact)``, optionally
+    projected.  ``Sigma`` includes the contact self-energy; ``H`` is
+    ``(1, 1, nb, nb)`` (real) with Hartree on, ``zeros_like(h)`` otherwise.
+    """
+    P = to_tensor(P)
+    dev = P.device
+    cdt = P.dtype if P.is_complex() else (torch.complex128 if P.dtype == torch.float64 else torch.complex64)
+    P = convert(P, cdt)
+    h = to_tensor(h, dev, cdt)
+    if h.shape != P.shape or h.dim() != 4 or h.shape[-1] != h.shape[-2]:
+        raise ValueError(f"P and h must both have shape (nk1, nk2, nb, nb); got {tuple(P.shape)}, "
+                         f"{tuple(h.shape)}")
+    nk1, nk2, nb = P.shape[0], P.shape[1], P.shape[-1]
+    VRt = to_tensor(VR, dev, cdt) if include_exchange else None
+    if VRt is not None:
+        if VRt.shape[:2] != (nk1, nk2) or VRt.shape[2:] not in ((1, 1), (nb, nb)):
+            raise ValueError(f"VR must have shape (nk1, nk2, 1, 1) or (nk1, nk2, nb, nb); got {tuple(VRt.shape)}")
+        if exchange_hermitian_channel_packing and VRt.shape[-2:] != (1, 1):
+            raise ValueError("hermitian_channel_packing requires a scalar interaction kernel "
+                             "with shape (..., 1, 1).")
+    refPt = None if refP is None else to_tensor(refP, dev, cdt)
+    if refPt is not None and refPt.shape != P.shape:
+        raise ValueError(f"refP must have shape {tuple(P.shape)}")
+    HHt = HH if include_hartree else None
+    eng = FockEngine(h=h, VR=VRt, refP=refPt, HH=HHt, w2d=w2d, include_exchange=include_exchange,
+                     include_hartree=include_hartree, herm=exchange_hermitian_channel_packing,
+                     contact_g=contact_g, contact_Oi=contact_Oi, contact_Oj=contact_Oj)
+    chans, nchan, zero_sigma, sep = None, 0, False, False
+    if include_exchange and exchange_block_specs is not None:
+        dP = P if refPt is None else _sub(P, refPt)
+        sel = block_channels(dP, exchange_block_specs, exchange_check_offdiag, exchange_offdiag_atol,
+                             exchange_offdiag_rtol, exchange_hermitian_channel_packing, eng.plan)
+        if sel is not None:
+            chans, nchan = sel
+            zero_sigma = True
+            sep = True
+    Sigma = torch.empty_like(P)
+    F = torch.empty_like(P)
+    eng.build(P, F, Sigma_out=Sigma, chans=chans, nchan=nchan, zero_sigma=zero_sigma,
+              moments_separately=sep)
+    if eng.include_exchange and not eng.contact_on:
+        Sigma = eng._sig_ws if eng._sig_ws is not None and Sigma is None else Sigma
+    H = eng.hartree_matrix_out()
+    if project_fn is not None:
+        Fp = to_tensor(project_fn(F), dev, cdt)
+        F = torch.empty_like(Fp)
+        L.call("hfx_hermitize", L.ptr(Fp), L.ptr(F), nk1 * nk2, nb, eng.plan.dtype, L.stream(dev))
+    return Sigma, H, F
+
+
+def _sub(A, B):
+    """A - B on the device (used only for the block-structure check)."""
+    out = torch.empty_like(A)
+    L.call("hfx_lincomb", A.numel(), KIND[A.dtype], 1.0, 0.0, L.ptr(A), -1.0, 0.0, L.ptr(B), L.ptr(out),
+           L.stream(A.device))
+    return out
+
+
+def block_channels(P, block_specs, check_offdiag, atol, rtol, herm, plan):
+    """Channel list of the first compatible block spec (utils.py:252-311), or None for full.
+
+    The off-block magnitude test max|P_offblock| <= atol + rtol max|P| runs on
+    the device (hfx_absmax_masked); the chosen spec's diagonal-block channels
+    (upper triangle only with Hermitian packing) are returned as a device list.
+    """
+    specs = normalize_block_specs(block_specs)
+    if not specs:
+        return None
+    nb = P.shape[-1]
+    check = bool(check_offdiag) if check_offdiag is not None else True
+    for kind, data in specs:
+        sets = _block_sets(str(kind).strip().lower(), data, nb)
+        mask = np.ones((nb, nb), np.uint8)
+        for s_ in sets:
+            idx = np.asarray(s_)
+            mask[np.ix_(idx, idx)] = 0
+        if check:
+            mx = _absmax(P, mask, plan)
+            ok = mx[1] <= atol + rtol * mx[0]
+        else:
+            ok = True
+        if ok:
+            ch = [a * nb + b for s_ in sets for a in s_ for b in s_ if not (herm and b < a)]
+            return torch.tensor(ch, dtype=torch.int32, device=P.device), len(ch)
+    return None
+
+
+def _absmax(X, mask_np, plan):
+    dev = X.device
+    mask = torch.from_numpy(mask_np.reshape(-1)).to(dev)
+    out = torch.zeros(2, dtype=torch.float64, device=dev)
+    L.call("hfx_absmax_masked", L.ptr(X), X.shape[0] * X.shape[1], X.shape[-1], plan.dtype, L.ptr(mask),
+           L.ptr(plan.scratch), L.ptr(out), L.stream(dev))
+    return out.cpu().numpy()
+
+
+def hf_energy(P, *, h, Sigma, H, weights_b):
+    """E = sum_k w_k Re Tr[(h + (Sigma + H)/2) P] (fock.py:100-111); FP64 device scalar."""
+    P = to_tensor(P)
+    dev = P.device
+    cdt = P.dtype
+    h = to_tensor(h, dev, cdt)
+    nk1, nk2, nb = P.shape[0], P.shape[1], P.shape[-1]
+    S = None if Sigma is None else to_tensor(Sigma, dev, cdt)
+    Hb, bc = None, 0
+    if H is not None:
+        Ht = H if isinstance(H, torch.Tensor) else to_tensor(H, dev)
+        if Ht.dim() == 4 and (Ht.shape[:2] == (1, 1) or (Ht.stride(0) == 0 and Ht.stride(1) == 0)):
+            Hb, bc = to_tensor(Ht[0, 0].contiguous(), dev, cdt), 1
+        else:
+            Hb, bc = to_tensor(Ht, dev, cdt), 0
+    wb = weights_b if isinstance(weights_b, torch.Tensor) else to_tensor(weights_b, dev)
+    w = to_tensor(wb.reshape(nk1, nk2), dev, torch.float64)
+    plan = L.plan_for(nk1, nk2, nb, cdt, dev)
+    E = torch.zeros((), dtype=torch.float64, device=dev)
+    L.call("hfx_hf_energy", plan.handle, L.ptr(P), L.ptr(h), L.ptr(S), L.ptr(Hb), bc, L.ptr(w),
+           L.ptr(plan.scratch), L.ptr(E), L.stream(dev))
+    return E
+
+
+def _entropy_call(p, w_norm, E, T):
+    p = to_tensor(p)
+    dev = p.device
+    if p.dtype not in (torch.float64, torch.float32):
+        p = convert(p, torch.float64)
+    nb = p.shape[-1]
+    nk = p.numel() // nb
+    w = to_tensor(w_norm, dev, torch.float64).reshape(-1)
+    if w.numel() != nk:
+        raise ValueError("w_norm must have one weight per k-point")
+    scratch = torch.empty(1024, dtype=torch.float64, device=dev)
+    out = torch.zeros((), dtype=torch.float64, device=dev)
+    Et = None
+    if E is not None:
+        Et = E if isinstance(E, torch.Tensor) else torch.tensor(float(E), dtype=torch.float64)
+        Et = to_tensor(Et.reshape(()), dev, torch.float64)
+    L.call("hfx_entropy", L.ptr(p.contiguous()), KIND[p.dtype], L.ptr(w), nk, nb, L.ptr(Et),
+           float(T), L.ptr(scratch), L.ptr(out), L.stream(dev))
+    return out
+
+
+def occupation_entropy(p, w_norm):
+    """S = -sum_k w_k sum_i [p ln p + (1-p) ln(1-p)] (fock.py:114-118)."""
+    return _entropy_call(p, w_norm, None, 0.0)
+
+
+def free_energy(E, p, w_norm, T):
+    """Omega = E - max(T, 1e-14) S(p) (fock.py:121-129)."""
+    return _entropy_call(p, w_norm, E, T)

@@ -1,0 +1,340 @@
+"""Direct-minimisation HF solver on the GPU (drop-in for solver.py).
+
+Mirrors ``src/jax_hf/solver.py``: ``SolverConfig`` (same fields/defaults plus
+``precision``), ``SolveResult``, ``solve`` / ``solve_direct_minimization``
+with the CG (``_solve_impl``, solver.py:307-660) and trust-region Newton
+(``_solve_newton_impl``, solver.py:686-1078) algorithms.
+
+The loop body is a fixed sequence of libhfx launches on one stream; all
+scalar steps (gradient norm, preconditioner scale, Polak-Ribiere beta, tau0,
+line-search acceptance, chemical potential) run on the device and live in a
+16-double state vector, so an iteration needs exactly one host readback (the
+energy / gradient pair that the stopping rule of solver.py:422-449 needs).
+
+Precision: ``precision="fp64"`` runs everything in complex128;
+``"mixed"`` runs the loop in complex64 storage with FP32 FFT / products and
+FP64 reductions, mu and entropy, then re-converges in FP64 from the mixed
+iterate (warm start) so the final Fock builds and the reported energy are
+FP64.  ``"auto"`` follows the kernel dtype.
+"""
+
+from __future__ import annotations
+
+import warnings
+from dataclasses import dataclass
+from typing import Any, NamedTuple
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import _ops as K
+from ._util import convert, host_array, to_tensor
+
+__all__ = ["SolverConfig", "SolveResult", "solve", "solve_direct_minimization"]
+
+
+@dataclass(frozen=True)
+class SolverConfig:
+    """Solver controls (solver.py:240-286), plus ``precision``."""
+    max_iter: int = 200
+    tol_E: float = 1e-7
+    tol_grad: float = 0.0
+    denom_scale: float = 1e-3
+    max_step: float = 0.6
+    cg_restart: int = 10
+    bt_shrink: float = 0.5
+    bt_max: int = 8
+    mu_maxiter: int = 25
+    plateau_window: int = 5
+    optimizer: str = "cg"
+    tr_delta0: float = 0.5
+    tr_cg_max: int = 20
+    deflation_targets: Any = None
+    deflation_sigma: float = 0.0
+    deflation_length: float = 1.0
+    block_sizes: tuple | None = None
+    project_fn: Any = None
+    precision: str = "auto"
+    mixed_tol_E: float = 1e-6
+
+
+class SolveResult(NamedTuple):
+    """solver.py:289-298."""
+    Q: torch.Tensor
+    p: torch.Tensor
+    mu: torch.Tensor
+    density: torch.Tensor
+    fock: torch.Tensor
+    energy: torch.Tensor
+    n_iter: int
+    converged: bool
+    history: dict
+
+
+# ---------------------------------------------------------------------------
+# shared helpers
+# ---------------------------------------------------------------------------
+
+class _Fock:
+    """Fock build adaptor: the libhfx engine, or a user ``fock_build_fn`` (plugin seam)."""
+
+    def __init__(self, kernel, cdt, fock_build_fn=None, block_sizes=None):
+        self.kernel = kernel
+        self.cdt = cdt
+        self.fn = fock_build_fn
+        self.block_specs = None if block_sizes is None else (("sizes", tuple(int(s) for s in block_sizes)),)
+        self.eng = kernel.fock_engine(dtype=cdt)
+        self.eng_nohs = None
+        if self.block_specs is not None or fock_build_fn is not None:
+            self._args = kernel.as_args()
+
+    def build(self, P, F, E_ptr=None, project_fn=None, refP="kernel", sub_h=False):
+        eng = self.eng
+        if self.fn is None and self.block_specs is None and project_fn is None:
+            ref = "default" if isinstance(refP, str) else refP
+            eng.build(P, F, energy=E_ptr is not None, E_ptr=E_ptr, refP=ref, sub_h=sub_h)
+            return
+        a = self._args
+        fn = self.fn
+        if fn is None:
+            from .fock import build_fock as fn
+        ref = a["refP"] if isinstance(refP, str) else refP
+        h = a["h"] if self.cdt == self.kernel.dtype else convert(a["h"], self.cdt)
+        VR = a["VR"] if self.cdt == self.kernel.dtype else convert(a["VR"], self.cdt)
+        S, H, Fo = fn(P, h=h, VR=VR, refP=ref, HH=a["HH"], w2d=self.kernel.w2d,
+                      include_exchange=a["include_exchange"], include_hartree=a["include_hartree"],
+                      exchange_hermitian_channel_packing=a["exchange_hermitian_channel_packing"],
+                      contact_g=a["contact_g"], contact_Oi=a["contact_Oi"], contact_Oj=a["contact_Oj"],
+                      exchange_block_specs=self.block_specs, project_fn=project_fn)
+        Fo = to_tensor(Fo, P.device, self.cdt)
+        if sub_h:
+            L.call("hfx_lincomb", Fo.numel(), 2 if self.cdt == torch.complex128 else 3, 1.0, 0.0, L.ptr(Fo),
+                   -1.0, 0.0, L.ptr(h), L.ptr(F), L.stream(P.device))
+        else:
+            F.copy_(Fo)
+        if E_ptr is not None:
+            from .fock import hf_energy
+            E = hf_energy(P, h=h, Sigma=to_tensor(S, P.device, self.cdt), H=H, weights_b=self.kernel._w64)
+            L.call("hfx_convert", L.ptr(E), 0, E_ptr, 0, 1, L.stream(P.device))
+
+
+def _project_herm(P, project_fn, tmp):
+    """P <- herm(project_fn(P))  (solver.py:462)."""
+    if project_fn is None:
+        return P
+    X = to_tensor(project_fn(P), P.device, P.dtype)
+    K.hermitize(X, tmp)
+    return tmp
+
+
+# ---------------------------------------------------------------------------
+# CG (solver.py:307-660)
+# ---------------------------------------------------------------------------
+
+class _CG:
+    def __init__(self, kernel, cdt, cfg: SolverConfig, fock_build_fn=None):
+        self.kernel = kernel
+        self.cfg = cfg
+        self.cdt = cdt
+        dev = kernel.device
+        self.dev = dev
+        nk1, nk2, nb = kernel.h.shape[0], kernel.h.shape[1], kernel.h.shape[-1]
+        self.shape4 = (nk1, nk2, nb, nb)
+        self.nk, self.nb = nk1 * nk2, nb
+        self.fock = _Fock(kernel, cdt, fock_build_fn, cfg.block_sizes)
+        ws = max(kernel.weight_sum, 1e-30)
+        self.w_norm = (kernel._w64 / ws).contiguous()
+        self.T = max(kernel.T, 1e-12)
+        self.ws = K.Workspace.get(dev, self.nk)
+        c = dict(dtype=cdt, device=dev)
+        r = dict(dtype=torch.float64, device=dev)
+        z = lambda: torch.zeros(self.shape4, **c)  # noqa: E731
+        self.Q, self.P, self.F, self.Ft, self.G, self.Gp = z(), z(), z(), z(), z(), z()
+        self.H, self.D, self.V, self.Fe, self.tmp = z(), z(), z(), z(), z()
+        v = lambda: torch.zeros((nk1, nk2, nb), **r)  # noqa: E731
+        self.p, self.eps, self.gp, self.gpp, self.hp, self.dp, self.lam = v(), v(), v(), v(), v(), v(), v()
+        self.st = torch.zeros(K.ST_LEN, **r)
+
+    # -- pieces ---------------------------------------------------------------
+    def density(self):
+        K.density(self.Q, self.P, p=self.p)
+        return _project_herm(self.P, self.cfg.project_fn, self.tmp)
+
+    def mu(self, eps, from_state, mu0=0.0):
+        n_norm = self.n_norm
+        K.solve_mu(self.ws, eps, self.w_norm, self.nk, self.nb, n_norm, self.T, mu0=mu0,
+                   mu0_from_state=from_state, maxiter=self.cfg.mu_maxiter, mode=K.MU_NEWTON_BRACKET,
+                   st=self.st, p_out=self.p)
+
+    def init_from(self, P0):
+        """solver.py:380-394: eigh(F[herm(P0)]) -> (Q, eps), mu, p."""
+        P0h = torch.empty(self.shape4, dtype=self.cdt, device=self.dev)
+        K.hermitize(to_tensor(P0, self.dev, self.cdt), P0h)
+        self.fock.build(P0h, self.F, project_fn=self.cfg.project_fn)
+        K.eigh(self.F, self.eps, self.Q)
+        self.mu(self.eps, False, 0.0)
+
+    def warm_from(self, Q, p, mu):
+        self.Q.copy_(convert(Q, self.cdt))
+        self.p.copy_(p)
+        self.st[K.ST_MU] = float(mu)
+
+    def run(self, n_electrons, tol_E, tol_grad, max_iter, state0=None):
+        cfg = self.cfg
+        self.n_norm = float(n_electrons) / max(self.kernel.weight_sum, 1e-30)
+        nk, nb = self.nk, self.nb
+        dt = K.dt_code(self.Q)
+        st, ws, part = self.st, self.ws, self.ws.part
+        Esp = K.st_ptr(st, K.ST_E)
+        sm = L.stream(self.dev)
+        self.G.zero_()
+        self.Gp.zero_()
+        self.D.zero_()
+        self.gp.zero_()
+        self.gpp.zero_()
+        self.dp.zero_()
+        hist_E, hist_G = [], []
+        k = 0
+        grad_norm, dE, E_prev = np.inf, np.inf, np.inf
+        shrink_pow = cfg.bt_shrink ** cfg.bt_max
+        while True:
+            # stopping rule (solver.py:422-449)
+            if cfg.plateau_window > 0:
+                e_now = hist_E[max(k - 1, 0)] if hist_E else 0.0
+                e_past = hist_E[max(k - 1 - cfg.plateau_window, 0)] if hist_E else 0.0
+                enc = (e_past - e_now) > tol_E if k > cfg.plateau_window else dE > tol_E
+            else:
+                enc = dE > tol_E
+            nc = grad_norm > tol_grad if tol_grad > 0.0 else enc
+            if not (k < max_iter and nc):
+                break
+            P = self.density()                                                  # :462
+            self.fock.build(P, self.F, E_ptr=Esp)                               # :463-471
+            L.call("hfx_cg_grad", dt, nb, nk, L.ptr(self.Q), L.ptr(self.F), L.ptr(self.p), L.ptr(self.Ft),
+                   L.ptr(self.eps), L.ptr(self.G), L.ptr(part), sm)             # :472-478
+            K.reduce(ws, part, 2, nk, w=self.w_norm, wmask=0b01, st=st, op=K.OP_GRAD,
+                     params=[self.T, cfg.denom_scale, nk * nb])                 # :491-497
+            L.call("hfx_cg_precond", dt, nb, nk, L.ptr(self.G), L.ptr(self.Gp), L.ptr(self.eps), L.ptr(self.p),
+                   L.ptr(self.gpp), L.ptr(st), self.T, L.ptr(self.H), L.ptr(self.gp), L.ptr(self.hp),
+                   L.ptr(part), sm)                                             # :486, 499-504
+            restart = 1.0 if (k == 0 or k % cfg.cg_restart == 0) else 0.0
+            K.reduce(ws, part, 6, nk, w=self.w_norm, wmask=0b111111, st=st, op=K.OP_BETA,
+                     params=[0, 0, 0, 0, 0, 0, restart])                        # :517-526
+            L.call("hfx_cg_direction", dt, nb, nk, L.ptr(self.H), L.ptr(self.D), L.ptr(self.hp), L.ptr(self.dp),
+                   L.ptr(self.Ft), L.ptr(st), L.ptr(self.V), L.ptr(self.lam), L.ptr(self.Fe), L.ptr(part), sm)
+            K.reduce(ws, part, 1, nk, w=self.w_norm, wmask=1, st=st, op=K.OP_TAU0,
+                     params=[0, 0, 0, cfg.max_step, 0, shrink_pow])             # :543-547
+            for i in range(cfg.bt_max):                                         # :563-579
+                L.call("hfx_cg_trial", dt, nb, nk, L.ptr(self.V), L.ptr(self.lam), L.ptr(self.Fe), L.ptr(self.p),
+                       L.ptr(self.dp), L.ptr(self.eps), L.ptr(st), int(i == 0), L.ptr(part), sm)
+                K.reduce(ws, part, 4 if i == 0 else 2, nk, w=self.w_norm, wmask=0b1111, st=st, op=K.OP_TRIAL,
+                         params=[self.T, 0, 0, 0, cfg.bt_shrink, shrink_pow, 0, int(i == cfg.bt_max - 1),
+                                 int(i == 0)])
+            L.call("hfx_cg_retract", dt, nb, nk, L.ptr(self.Q), L.ptr(self.V), L.ptr(self.lam), L.ptr(self.Fe),
+                   L.ptr(st), 0.0, 0, L.ptr(self.eps), None, sm)                # :586-590
+            self.mu(self.eps, True)                                             # :591-592
+            self.G, self.Gp = self.Gp, self.G
+            self.gp, self.gpp = self.gpp, self.gp
+            sv = st.cpu().numpy()                                               # one readback
+            E = float(sv[K.ST_E])
+            grad_norm = float(sv[K.ST_GRAD])
+            hist_E.append(E)
+            hist_G.append(grad_norm)
+            dE = abs(E - E_prev)
+            E_prev = E
+            k += 1
+        self.G, self.Gp = self.Gp, self.G   # last computed gradient back in self.G
+        return k, grad_norm, hist_E, hist_G
+
+    def finalize(self):
+        """solver.py:606-638: Fock, eigh canonicalisation, mu, p, Fock + E."""
+        P = self.density()
+        self.fock.build(P, self.F)
+        L.call("hfx_rotate_eigh", K.dt_code(self.Q), self.nb, self.nk, L.ptr(self.Q), L.ptr(self.F),
+               L.ptr(self.eps), L.stream(self.dev))
+        self.mu(self.eps, True)
+        P = self.density()
+        Pout = P.clone()
+        F = torch.empty_like(Pout)
+        self.fock.build(Pout, F, E_ptr=K.st_ptr(self.st, K.ST_E))
+        return Pout, F
+
+
+def _solve_cg(kernel, P0, n_electrons, cfg: SolverConfig, fock_build_fn):
+    prec = _precision(kernel, cfg)
+    dev = kernel.device
+    if prec == "mixed":
+        lo = _CG(kernel, torch.complex64, cfg, fock_build_fn)
+        lo.init_from(P0)
+        k1, g1, hE1, hG1 = lo.run(n_electrons, max(cfg.tol_E, cfg.mixed_tol_E), cfg.tol_grad, cfg.max_iter)
+        hi = _CG(kernel, torch.complex128, cfg, fock_build_fn)
+        hi.n_norm = float(n_electrons) / max(kernel.weight_sum, 1e-30)
+        hi.warm_from(lo.Q, lo.p, float(lo.st[K.ST_MU]))
+        k2, grad, hE2, hG2 = hi.run(n_electrons, cfg.tol_E, cfg.tol_grad, max(cfg.max_iter - k1, 0))
+        k, hist_E, hist_G = k1 + k2, hE1 + hE2, hG1 + hG2
+        runner = hi
+        del lo
+    else:
+        cdt = torch.complex128 if prec == "fp64" else torch.complex64
+        runner = _CG(kernel, cdt, cfg, fock_build_fn)
+        runner.init_from(P0)
+        k, grad, hist_E, hist_G = runner.run(n_electrons, cfg.tol_E, cfg.tol_grad, cfg.max_iter)
+    Pfin, Ffin = runner.finalize()
+    converged = (grad <= cfg.tol_grad) if cfg.tol_grad > 0.0 else (k < cfg.max_iter)
+    sv = runner.st.cpu()
+    return _result(runner, Pfin, Ffin, sv, k, converged, hist_E, hist_G, cfg.max_iter, dev)
+
+
+def _result(runner, Pfin, Ffin, sv, k, converged, hist_E, hist_G, max_iter, dev):
+    hE = torch.zeros(max_iter, dtype=torch.float64)
+    hG = torch.zeros(max_iter, dtype=torch.float64)
+    n = min(len(hist_E), max_iter)
+    if n:
+        hE[:n] = torch.tensor(hist_E[:n], dtype=torch.float64)
+        hG[:n] = torch.tensor(hist_G[:n], dtype=torch.float64)
+    return SolveResult(Q=runner.Q, p=runner.p, mu=sv[K.ST_MU].to(dev), density=Pfin, fock=Ffin,
+                       energy=sv[K.ST_E].to(dev), n_iter=int(k), converged=bool(converged),
+                       history=dict(E=hE.to(dev), grad_norm=hG.to(dev)))
+
+
+def _precision(kernel, cfg):
+    p = str(cfg.precision).lower()
+    if p == "auto":
+        return "fp64" if kernel.dtype == torch.complex128 else "fp32"
+    if p not in ("fp64", "mixed", "fp32"):
+        raise ValueError(f"precision must be 'auto', 'fp64', 'mixed' or 'fp32'; got {cfg.precision!r}")
+    return p
+
+
+# ---------------------------------------------------------------------------
+# public dispatcher (solver.py:1098-1251)
+# ---------------------------------------------------------------------------
+
+def solve_direct_minimization(kernel, P0, n_electrons: float, *, config: SolverConfig | None = None,
+                              fock_build_fn=None) -> SolveResult:
+    """Direct-minimisation HF solver (solver.py:1098-1247): CG or trust-region Newton."""
+    from .utils import validate_electron_count
+    if config is None:
+        config = SolverConfig()
+    validate_electron_count(host_array(kernel._w64), kernel.h.shape[-1], n_electrons, context="n_electrons")
+    optimizer = str(config.optimizer).lower()
+    if optimizer not in ("cg", "newton"):
+        raise ValueError(f"Unknown optimizer {config.optimizer!r}; expected 'cg' or 'newton'.")
+    deflation_on = config.deflation_targets is not None and float(config.deflation_sigma) > 0.0
+    if deflation_on and optimizer != "newton":
+        raise ValueError("deflation is only supported with optimizer='newton' "
+                         f"(got optimizer={config.optimizer!r}).")
+    if kernel.h.shape[-1] > 16:
+        raise ValueError("the GPU solvers support nb <= 16 bands")
+    if optimizer == "newton":
+        from .newton import solve_newton
+        if kernel.dtype != torch.complex128 and _precision(kernel, config) != "fp64":
+            warnings.warn("optimizer='newton' needs float64 precision to converge reliably; the kernel is "
+                          "float32. Build the kernel in complex128 (or pass precision='fp64'). The CG "
+                          "optimizer works in float32.", stacklevel=2)
+        return solve_newton(kernel, P0, n_electrons, config, fock_build_fn, deflation_on)
+    return _solve_cg(kernel, P0, n_electrons, config, fock_build_fn)
+
+
+solve = solve_direct_minimization
