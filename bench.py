@@ -43,6 +43,12 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--extra", action="store_true", help="also time the packed and complex64 forms")
+    ap.add_argument("--solve", action="store_true",
+                    help="also time converged CG solves (FP64 and mixed) at the same mesh")
+    ap.add_argument("--solve-tol", type=float, default=1e-9)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of the exchange kernel's byte count "
+                         "(hfx_lincomb out = X), to scale FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
 
 
@@ -89,13 +95,22 @@ def main():
 
     dev = torch.device("cuda:0")
     nk, nb = args.nk, args.nb
+    if args.copy_calibration:
+        n = nk * nk * nb * nb * 3 // 2          # complex128 values: read n*16 B, write n*16 B
+        X = torch.ones(n, dtype=torch.complex128, device=dev)
+        Y = torch.empty_like(X)
+        for _ in range(5):
+            L.call("hfx_lincomb", n, 2, 1.0, 0.0, L.ptr(X), 0.0, 0.0, None, L.ptr(Y), L.stream(dev))
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": n * 16, "copy_calibration_bytes_written": n * 16}))
+        return
     inp = syn.config1(nk=nk, nb=nb, seed=1)
     cdt = torch.complex128 if args.dtype == "c128" else torch.complex64
     npd = np.complex128 if args.dtype == "c128" else np.complex64
     G = J.HartreeFockKernel(inp["weights"], inp["hamiltonian"].astype(npd), inp["coulomb_q"], inp["T"],
                             device=dev)
     eng = G.fock_engine(herm=False)        # reference semantics: all nb^2 channels
-    P = torch.from_numpy(inp["P"].astype(npd)).to(dev)
+    P = torch.from_numpy(np.ascontiguousarray(inp["P"], dtype=npd)).to(dev)
     F = torch.empty_like(P)
     Sig = torch.empty_like(P)
     E = eng.E
@@ -152,6 +167,8 @@ def main():
     }
     if args.extra:
         out["extra"] = extra_forms(J, L, syn, inp, dev, nk, nb, args)
+    if args.solve:
+        out["solves"] = solve_forms(J, syn, inp, dev, nk, nb, args)
     if not args.no_cpu_baseline:
         cb, E_cpu, F_cpu = cpu_baseline(inp, args.cpu_seconds)
         out["cpu_baseline"] = cb
@@ -171,7 +188,7 @@ def extra_forms(J, L, syn, inp, dev, nk, nb, args):
         G = J.HartreeFockKernel(inp["weights"], inp["hamiltonian"].astype(npd), inp["coulomb_q"],
                                 inp["T"], device=dev)
         eng = G.fock_engine(herm=herm)
-        P = torch.from_numpy(inp["P"].astype(npd)).to(dev)
+        P = torch.from_numpy(np.ascontiguousarray(inp["P"], dtype=npd)).to(dev)
         F = torch.empty_like(P)
         for _ in range(10):
             eng.build(P, F, energy=True, refP=None)
@@ -182,6 +199,35 @@ def extra_forms(J, L, syn, inp, dev, nk, nb, args):
             eng.build(P, F, energy=True, refP=None)
         torch.cuda.synchronize()
         res[name] = {"ms_per_build": (time.perf_counter() - t0) * 1e3 / n}
+    return res
+
+
+def solve_forms(J, syn, inp, dev, nk, nb, args):
+    """Converged CG solves from P0 = 0 at half filling, FP64 vs mixed precision.
+
+    Same h / layer-resolved V as the Fock-build workload, T = 0.1,
+    n_electrons = nb/2 per k-point, tol_E = args.solve_tol (windowed test).
+    |dE| of the mixed solve is against the FP64 GPU solve at the same tolerance.
+    """
+    import torch
+    res = {}
+    G = J.HartreeFockKernel(inp["weights"], inp["hamiltonian"], inp["coulomb_q"], 0.1, device=dev)
+    P0 = np.zeros_like(inp["hamiltonian"])
+    ne = nb / 2.0
+    E_ref = None
+    for prec in ("fp64", "mixed"):
+        cfg = J.SolverConfig(max_iter=500, tol_E=args.solve_tol, precision=prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = J.solve(G, P0, ne, config=cfg)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        E = float(r.energy)
+        if E_ref is None:
+            E_ref = E
+        res[prec] = {"solves_per_s": 1.0 / dt, "seconds": dt, "n_iter": r.n_iter, "converged": r.converged,
+                     "energy": E, "abs_dE_vs_fp64": abs(E - E_ref)}
+    res["speedup_mixed_vs_fp64"] = res["fp64"]["seconds"] / res["mixed"]["seconds"]
     return res
 
 

@@ -59,6 +59,17 @@ int hfx_selfenergy(const hfx_plan* plan, const void* P, const void* refP, const 
 int hfx_prepare_vr(const hfx_plan* plan, const void* Vq, int vr_full, const double* w2d, int phase,
                    void* VR, void* stream);
 
+/* Adjoint of the exchange map (backward of build_fock / hf_energy, no reference
+ * counterpart: jax autodiff differentiates fock.py:51-64): for Sigma = A[P],
+ * A_c = IFFT2 diag(-VR_c) FFT2, writes A^H[G] = -IFFT2[conj(VR_c) FFT2(G_c)]
+ * on all nb^2 channels -- the cross-correlation, recomputed by the same fused kernel. */
+int hfx_selfenergy_adjoint(const hfx_plan* plan, const void* G, const void* VR, int vr_full, void* out,
+                           void* stream);
+
+/* out_c = FFT2(w * X_c) over all nb^2 channels (w float64 (nk1, nk2)); the forward
+ * transforms of the VR gradient dE/dVR = -conj(FFT2 P) FFT2(G) / N. */
+int hfx_fft2_weighted(const hfx_plan* plan, const void* X, const double* w, void* out, void* stream);
+
 /* Weighted channel moments rho = sum_k w_k (P - refP) (fock.py:66-70, 79). */
 int hfx_density_moments(const hfx_plan* plan, const void* P, const void* refP, const double* w2d,
                         void* rho, void* stream);
@@ -102,6 +113,15 @@ int hfx_lincomb(size_t n, int kind, double ar, double ai, const void* X, double 
 
 /* out = 1 / (1 + exp((x - mu) / (T + 1e-12))), mu = *mu or 0 (fermidirac, utils.py:51-53). */
 int hfx_fermi(const double* x, const double* mu, double T, size_t n, double* out, void* stream);
+
+/* out[k,c] = a conj(X[k,c]) Y[k,c] (sum_ch = 0) or out[k] = a sum_c conj(X[k,c]) Y[k,c]
+ * (sum_ch = 1, nch channels per k): the pointwise products of the VR gradient. */
+int hfx_cmul(int dtype, size_t nk, int nch, const void* X, const void* Y, double ar, double ai, int sum_ch,
+             void* out, void* stream);
+
+/* out[k, c] = alpha w[k] X[k, c] (w may be NULL): the k-weighting of adjoint seeds. */
+int hfx_scale_k(size_t nk, int nch, int kind, const void* X, const double* w, double alpha, void* out,
+                void* stream);
 
 /* Y = 0.5 (X + X^dagger) over nmat nb x nb matrices (fock.py:13-15, 95-96). */
 int hfx_hermitize(const void* X, void* Y, size_t nmat, int nb, int dtype, void* stream);
@@ -173,11 +193,29 @@ int hfx_matmul_batched(int dtype, int nb, size_t nk, const void* A, const void* 
  * part[0][k] = sum |comm|^2, part[1][k] = sum |Pn - D|^2 (Pn may be NULL). */
 int hfx_scf_resid(int dtype, int nb, size_t nk, const void* F, const void* D, const void* Pn, void* comm,
                   double* part, void* stream);
-/* part[v][k] = Re sum conj(X) Y_v (mode 0, weighted inner products solver.py:205-208 and the
- * DIIS B matrix reference_scf.py:110-111) or Re tr(X Y_v) (mode 1, reference_scf.py:319);
- * Ys is a HOST array of nv <= 8 device pointers. */
-int hfx_batched_dots(int dtype, int nb, size_t nk, const void* X, const void* const* Ys, int nv, int mode,
+/* part[v][k] = Re sum conj(X_v) Y_v (mode 0, the inner products of solver.py:205-208 and the
+ * DIIS B matrix reference_scf.py:110-111) or Re tr(X_v Y_v) (mode 1, reference_scf.py:319);
+ * Xs, Ys are HOST arrays of nv <= 8 device pointers. */
+int hfx_batched_dots(int dtype, int nb, size_t nk, const void* const* Xs, const void* const* Ys, int nv, int mode,
                      double* part, void* stream);
+/* part[v][k] = sum_b x_v[k,b] y_v[k,b] for per-k real vectors (solver.py:211-213). */
+int hfx_vec_dots(const double* const* xs, const double* const* ys, int nv, size_t nk, int nb, double* part,
+                 void* stream);
+/* x[i] -= scale * (*s), s a device scalar (the projection of solver.py:224-233). */
+int hfx_vec_shift(double* x, size_t n, const double* s, double scale, void* stream);
+
+/* Trust-region Newton HVP pieces (solver.py:916-930):
+ *   hfx_newton_dP:  dP = herm(Q (X (x) diff_p + diag(dp)) Q^dagger)
+ *   hfx_newton_hvp: with R = F[dP] - h: HX = skew(diff_dp (x) Ft + diff_p (x) A) offdiag,
+ *                   Hp = Re diag(A) + T/(pc(1-pc)) dp, A = [Ft, X] + Q^dagger R Q; part[k] = sum_b Hp
+ *   hfx_newton_prec: ZX = RX / sqrt(gap^2 + st[LAM]^2), zp = rp pc(1-pc)/T; part[k] = sum_b zp
+ *                    (solver.py:905-913, projection applied by the caller). */
+int hfx_newton_dP(int dtype, int nb, size_t nk, const void* Q, const void* X, const double* p, const double* dp,
+                  void* dP, void* stream);
+int hfx_newton_hvp(int dtype, int nb, size_t nk, const void* Q, const void* R, const void* Ft, const void* X,
+                   const double* p, const double* dp, double T, void* HX, double* Hp, double* part, void* stream);
+int hfx_newton_prec(int dtype, int nb, size_t nk, const void* RX, const double* rp, const double* eps,
+                    const double* p, const double* st, double T, void* ZX, double* zp, double* part, void* stream);
 
 /* ---- reductions and scalar steps ------------------------------------------ */
 #define HFX_OP_SUM 0

@@ -37,6 +37,10 @@ _SIGS = {
     "hfx_energy_scratch_bytes": [vp],
     "hfx_selfenergy": [vp, vp, vp, vp, i32, vp, i32, i32, i32, vp, vp, vp, vp],
     "hfx_prepare_vr": [vp, vp, i32, vp, i32, vp, vp],
+    "hfx_selfenergy_adjoint": [vp, vp, vp, i32, vp, vp],
+    "hfx_fft2_weighted": [vp, vp, vp, vp, vp],
+    "hfx_cmul": [i32, sz, i32, vp, vp, f64, f64, i32, vp, vp],
+    "hfx_scale_k": [sz, i32, i32, vp, vp, f64, vp, vp],
     "hfx_density_moments": [vp, vp, vp, vp, vp, vp],
     "hfx_kterms": [vp, vp, vp, i32, vp, vp, vp, vp, vp, vp],
     "hfx_fock_epilogue": [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, f64, vp],
@@ -58,6 +62,11 @@ _SIGS = {
     "hfx_matmul_batched": [i32, i32, sz, vp, vp, vp, i32, vp, i32, vp, vp],
     "hfx_scf_resid": [i32, i32, sz, vp, vp, vp, vp, vp, vp],
     "hfx_batched_dots": [i32, i32, sz, vp, vp, i32, i32, vp, vp],
+    "hfx_vec_dots": [vp, vp, i32, sz, i32, vp, vp],
+    "hfx_vec_shift": [vp, sz, vp, f64, vp],
+    "hfx_newton_dP": [i32, i32, sz, vp, vp, vp, vp, vp, vp],
+    "hfx_newton_hvp": [i32, i32, sz, vp, vp, vp, vp, vp, vp, f64, vp, vp, vp, vp],
+    "hfx_newton_prec": [i32, i32, sz, vp, vp, vp, vp, vp, f64, vp, vp, vp, vp],
     "hfx_solver_scratch_doubles": [],
     "hfx_reduce": [vp, i32, sz, vp, C.c_uint, vp, vp, i32, vp, vp, vp],
     "hfx_solve_mu": [vp, vp, sz, i32, f64, f64, f64, i32, i32, i32, f64, vp, vp, vp, vp, vp],

@@ -102,12 +102,25 @@ def scf_resid(F, D, Pn, comm, part):
 
 
 def dots(X, Ys, part, mode=0):
+    """part[v][k] = per-k <X, Y_v> (X may be one matrix or a list paired with Ys)."""
     import ctypes as C
-    nb = X.shape[-1]
-    nk = X.numel() // (nb * nb)
-    arr = (C.c_void_p * len(Ys))(*[y.data_ptr() for y in Ys])
-    L.call("hfx_batched_dots", dt_code(X), nb, nk, L.ptr(X), C.cast(arr, C.c_void_p), len(Ys), int(mode),
-           L.ptr(part), s(X.device))
+    Xs = X if isinstance(X, (list, tuple)) else [X] * len(Ys)
+    nb = Xs[0].shape[-1]
+    nk = Xs[0].numel() // (nb * nb)
+    xa = (C.c_void_p * len(Ys))(*[x.data_ptr() for x in Xs])
+    ya = (C.c_void_p * len(Ys))(*[y.data_ptr() for y in Ys])
+    L.call("hfx_batched_dots", dt_code(Xs[0]), nb, nk, C.cast(xa, C.c_void_p), C.cast(ya, C.c_void_p),
+           len(Ys), int(mode), L.ptr(part), s(Xs[0].device))
+
+
+def vdots(xs, ys, part):
+    import ctypes as C
+    nb = xs[0].shape[-1]
+    nk = xs[0].numel() // nb
+    xa = (C.c_void_p * len(xs))(*[x.data_ptr() for x in xs])
+    ya = (C.c_void_p * len(ys))(*[y.data_ptr() for y in ys])
+    L.call("hfx_vec_dots", C.cast(xa, C.c_void_p), C.cast(ya, C.c_void_p), len(xs), nk, nb, L.ptr(part),
+           s(xs[0].device))
 
 
 def lincomb(out, X, a=1.0, Y=None, b=0.0):

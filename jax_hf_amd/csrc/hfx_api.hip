@@ -186,7 +186,20 @@ int hfx_selfenergy(const hfx_plan* plan, const void* P, const void* refP, const 
   }
   HFX_REQUIRE(nchan >= 0 && nchan <= nb2, "hfx_selfenergy: nchan=%d out of range", nchan);
   return exchange_launch(plan, P, refP, VR, vr_full ? 1 : 0, chans, nchan, herm ? 1 : 0, w2d, Sigma, rho,
-                         0, ifftshift ? 1 : 0, (hipStream_t)stream);
+                         0, ifftshift ? 1 : 0, 0, (hipStream_t)stream);
+}
+
+int hfx_selfenergy_adjoint(const hfx_plan* plan, const void* G, const void* VR, int vr_full, void* out,
+                           void* stream) {
+  HFX_REQUIRE(plan && G && VR && out, "hfx_selfenergy_adjoint: NULL argument");
+  return exchange_launch(plan, G, nullptr, VR, vr_full ? 1 : 0, plan->chans_full, plan->nb * plan->nb, 0,
+                         nullptr, out, nullptr, 0, 0, 1, (hipStream_t)stream);
+}
+
+int hfx_fft2_weighted(const hfx_plan* plan, const void* X, const double* w, void* out, void* stream) {
+  HFX_REQUIRE(plan && X && w && out, "hfx_fft2_weighted: NULL argument");
+  return exchange_launch(plan, X, nullptr, X, 1, plan->chans_full, plan->nb * plan->nb, 0, w, out, nullptr, 1, 0,
+                         0, (hipStream_t)stream);
 }
 
 int hfx_prepare_vr(const hfx_plan* plan, const void* Vq, int vr_full, const double* w2d, int phase,
@@ -198,7 +211,7 @@ int hfx_prepare_vr(const hfx_plan* plan, const void* Vq, int vr_full, const doub
     // a scalar V is a single channel with nb = 1 strides: use a 1-band view
     return set_error(HFX_EUNSUPPORTED, "hfx_prepare_vr: use a plan with nb = 1 for scalar V");
   }
-  return exchange_launch(plan, Vq, nullptr, Vq, 1, chans, nchan, 0, w2d, VR, nullptr, 1, phase ? 1 : 0,
+  return exchange_launch(plan, Vq, nullptr, Vq, 1, chans, nchan, 0, w2d, VR, nullptr, 1, phase ? 1 : 0, 0,
                          (hipStream_t)stream);
 }
 

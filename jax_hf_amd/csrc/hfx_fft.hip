@@ -184,6 +184,7 @@ struct ExchArgs {
   cx<double>* rho;     // (nb, nb) weighted channel moments, may be null
   int nchan, herm, vfull, xcd_remap;
   int mode;   // 0: exchange convolution; 1: forward transform of w * V (VR preparation)
+  int conjv;  // mode 0: multiply by conj(VR) (the adjoint / cross-correlation)
   int shift;  // mode 0: ifftshift the result on store; mode 1: apply the ifftshift phase
   FftGeom<T> g;
 };
@@ -247,7 +248,7 @@ __device__ __forceinline__ void exch_multiply(const ExchArgs<T>& a, cx<T>* plane
     const int k1 = e / n2, k2 = e - k1 * n2;
     const cx<T> vr = a.VR[(size_t)e * vstride + vc];
     cx<T>& x = plane[k1 * pitch + k2];
-    x = scale * (x * vr);
+    x = scale * (a.conjv ? mulc(x, vr) : x * vr);
   }
   __syncthreads();
 }
@@ -421,17 +422,17 @@ static int launch_exch(const ExchArgs<T>& a, hipStream_t s) {
 
 int exchange_launch(const hfx_plan* plan, const void* P, const void* refP, const void* VR, int vfull,
                     const int* chans, int nchan, int herm, const double* w2d, void* Sigma, void* rho,
-                    int mode, int shift, hipStream_t s) {
+                    int mode, int shift, int conjv, hipStream_t s) {
   if (nchan <= 0) return HFX_OK;
   if (plan->dtype == HFX_DTYPE_C128) {
     ExchArgs<double> a{(const cx<double>*)P, (const cx<double>*)refP, (const cx<double>*)VR, chans, w2d,
                        (cx<double>*)Sigma, (cx<double>*)rho, nchan, herm, vfull,
-                       (nchan % 8 == 0) ? 1 : 0, mode, shift, plan->g64};
+                       (nchan % 8 == 0) ? 1 : 0, mode, shift, conjv, plan->g64};
     return launch_exch<double>(a, s);
   }
   ExchArgs<float> a{(const cx<float>*)P, (const cx<float>*)refP, (const cx<float>*)VR, chans, w2d,
                     (cx<float>*)Sigma, (cx<double>*)rho, nchan, herm, vfull,
-                    (nchan % 8 == 0) ? 1 : 0, mode, shift, plan->g32};
+                    (nchan % 8 == 0) ? 1 : 0, mode, shift, conjv, plan->g32};
   return launch_exch<float>(a, s);
 }
 

@@ -36,7 +36,7 @@ struct hfx_plan {
 namespace hfx {
 int exchange_launch(const hfx_plan* plan, const void* P, const void* refP, const void* VR, int vfull,
                     const int* chans, int nchan, int herm, const double* w2d, void* Sigma, void* rho,
-                    int mode, int shift, hipStream_t s);
+                    int mode, int shift, int conjv, hipStream_t s);
 int exchange_max_plane(int dtype);
 int exchange_lds_elems(int n1, int n2);
 }  // namespace hfx

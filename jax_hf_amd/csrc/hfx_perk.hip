@@ -114,7 +114,9 @@ __device__ void m_eigh(cx<T>* A, cx<T>* V, cx<T>* Vt, double* lam, int nb, doubl
       if (i != j) off += x;
     })
     const double off_s = wave_sum((double)off), tot_s = wave_sum((double)tot);
-    if (off_s <= (double)eps * (double)eps * 0.25 * tot_s || off_s == 0.0) break;
+    // FP64: (eps/2)^2 relative; FP32 stalls near its own rounding, so (2 eps)^2
+    const double thr = sizeof(T) == 8 ? 0.25 : 4.0;
+    if (off_s <= (double)eps * (double)eps * thr * tot_s || off_s == 0.0) break;
     for (int step = 0; step < m - 1; ++step) {
       // pair table for this step: index m-1 meets `step`; others (step+k, step-k) mod (m-1)
       if ((int)threadIdx.x < m / 2) {
@@ -571,13 +573,15 @@ __global__ __launch_bounds__(64) void pk_scf_resid_kernel(int nb, size_t Nk, con
 //   mode 0: part[v][k] = Re sum_ij conj(X_ij) Y_v,ij   (Frobenius, solver.py:205-208, reference_scf.py:110-111)
 //   mode 1: part[v][k] = Re sum_ij X_ij Y_v,ji          (trace of the product, reference_scf.py:319)
 struct DotList {
+  const void* x[8];
   const void* y[8];
 };
 template <typename T>
-__global__ __launch_bounds__(64) void pk_dots_kernel(int nb, size_t Nk, const cx<T>* __restrict__ X, DotList L,
-                                                     int nv, int mode, double* __restrict__ part) {
+__global__ __launch_bounds__(64) void pk_dots_kernel(int nb, size_t Nk, DotList L, int nv, int mode,
+                                                     double* __restrict__ part) {
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
   for (int v = 0; v < nv; ++v) {
+    const cx<T>* X = (const cx<T>*)L.x[v];
     const cx<T>* Y = (const cx<T>*)L.y[v];
     double acc = 0.0;
     PK_FOR_ELEM(nb, {
@@ -588,6 +592,113 @@ __global__ __launch_bounds__(64) void pk_dots_kernel(int nb, size_t Nk, const cx
     acc = wave_sum(acc);
     if (threadIdx.x == 0) part[(size_t)v * Nk + k] = acc;
   }
+}
+
+// Newton HVP, density part (solver.py:917-919):
+//   M = X (x) diff_p + diag(dp), diff_p[i][j] = p_j - p_i;  dP = herm(Q M Q^dagger).
+template <typename T>
+__global__ __launch_bounds__(64) void pk_newton_dP_kernel(int nb, const cx<T>* __restrict__ Q, const cx<T>* __restrict__ X,
+                                                          const double* __restrict__ p, const double* __restrict__ dp,
+                                                          cx<T>* __restrict__ dP) {
+  __shared__ cx<T> sQ[PK_MAT], sM[PK_MAT], sX[PK_MAT], sP[PK_MAT];
+  __shared__ double sp[PK_NBMAX], sd[PK_NBMAX];
+  const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  m_load(sQ, Q + k * nb2, nb);
+  if ((int)threadIdx.x < nb) {
+    sp[threadIdx.x] = p[k * nb + threadIdx.x];
+    sd[threadIdx.x] = dp[k * nb + threadIdx.x];
+  }
+  __syncthreads();
+  PK_FOR_ELEM(nb, {
+    cx<T> m = (T)(sp[j] - sp[i]) * X[k * nb2 + e];
+    if (i == j) m.re += (T)sd[i];
+    sM[i * PK_PITCH + j] = m;
+  })
+  __syncthreads();
+  m_gemm<T, false, false>(sX, sQ, sM, nb);
+  m_gemm<T, false, true>(sP, sX, sQ, nb);
+  PK_FOR_ELEM(nb, { dP[k * nb2 + e] = (T)0.5 * (sP[i * PK_PITCH + j] + conj(sP[j * PK_PITCH + i])); })
+}
+
+// Newton HVP, response part (solver.py:923-929) with R = F[dP] - h (+ penalty):
+//   St = Q^dagger R Q, A = Ft X - X Ft + St,
+//   HX = skew(diff_dp (x) Ft + diff_p (x) A) offdiag,  Hp = Re diag(A) + T/(pc(1-pc)) dp
+// (Hp before the particle-conserving projection); part[k] = sum_b Hp.
+template <typename T>
+__global__ __launch_bounds__(64) void pk_newton_hvp_kernel(int nb, const cx<T>* __restrict__ Q,
+                                                           const cx<T>* __restrict__ R, const cx<T>* __restrict__ Ft,
+                                                           const cx<T>* __restrict__ X, const double* __restrict__ p,
+                                                           const double* __restrict__ dp, double T_,
+                                                           cx<T>* __restrict__ HX, double* __restrict__ Hp,
+                                                           double* __restrict__ part) {
+  __shared__ cx<T> sQ[PK_MAT], sR[PK_MAT], sF[PK_MAT], sX[PK_MAT], sY[PK_MAT], sA[PK_MAT];
+  __shared__ double sp[PK_NBMAX], sd[PK_NBMAX];
+  const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  m_load(sQ, Q + k * nb2, nb);
+  m_load(sR, R + k * nb2, nb);
+  m_load(sF, Ft + k * nb2, nb);
+  m_load(sX, X + k * nb2, nb);
+  if ((int)threadIdx.x < nb) {
+    sp[threadIdx.x] = p[k * nb + threadIdx.x];
+    sd[threadIdx.x] = dp[k * nb + threadIdx.x];
+  }
+  __syncthreads();
+  m_gemm<T, false, false>(sY, sR, sQ, nb);    // R Q
+  m_gemm<T, true, false>(sA, sQ, sY, nb);     // St
+  m_gemm<T, false, false>(sY, sF, sX, nb);    // Ft X
+  PK_FOR_ELEM(nb, { sA[i * PK_PITCH + j] = sA[i * PK_PITCH + j] + sY[i * PK_PITCH + j]; })
+  __syncthreads();
+  m_gemm<T, false, false>(sY, sX, sF, nb);    // X Ft
+  PK_FOR_ELEM(nb, { sR[i * PK_PITCH + j] = sA[i * PK_PITCH + j] - sY[i * PK_PITCH + j]; })   // A
+  __syncthreads();
+  PK_FOR_ELEM(nb, {
+    // C = diff_dp (x) Ft + diff_p (x) A ; HX = 0.5 (C - C^H) offdiag
+    cx<T> h{0, 0};
+    if (i != j) {
+      const cx<T> cij = (T)(sd[j] - sd[i]) * sF[i * PK_PITCH + j] + (T)(sp[j] - sp[i]) * sR[i * PK_PITCH + j];
+      const cx<T> cji = (T)(sd[i] - sd[j]) * sF[j * PK_PITCH + i] + (T)(sp[i] - sp[j]) * sR[j * PK_PITCH + i];
+      h = (T)0.5 * (cij - conj(cji));
+    }
+    HX[k * nb2 + e] = h;
+  })
+  double acc = 0.0;
+  if ((int)threadIdx.x < nb) {
+    const int b = threadIdx.x;
+    const double pc = fmin(fmax(sp[b], 1e-8), 1.0 - 1e-8);
+    const double v = (double)sR[b * PK_PITCH + b].re + (T_ / (pc * (1.0 - pc))) * sd[b];
+    Hp[k * nb + b] = v;
+    acc = v;
+  }
+  acc = wave_sum(acc);
+  if (threadIdx.x == 0) part[k] = acc;
+}
+
+// Newton preconditioner (solver.py:905-913): ZX = RX / sqrt(gap^2 + lam^2)
+// (lam = st[HFX_ST_LAM]), zp = rp pc(1-pc)/T before projection; part[k] = sum_b zp.
+template <typename T>
+__global__ __launch_bounds__(64) void pk_newton_prec_kernel(int nb, const cx<T>* __restrict__ RX,
+                                                            const double* __restrict__ rp, const double* __restrict__ eps,
+                                                            const double* __restrict__ p, const double* __restrict__ st,
+                                                            double T_, cx<T>* __restrict__ ZX, double* __restrict__ zp,
+                                                            double* __restrict__ part) {
+  const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  const double lam = st[HFX_ST_LAM];
+  PK_FOR_ELEM(nb, {
+    const double gap = eps[k * nb + i] - eps[k * nb + j];
+    const T den = (T)sqrt(gap * gap + lam * lam);
+    const cx<T> r = RX[k * nb2 + e];
+    ZX[k * nb2 + e] = cx<T>{r.re / den, r.im / den};
+  })
+  double acc = 0.0;
+  if ((int)threadIdx.x < nb) {
+    const size_t q = k * nb + threadIdx.x;
+    const double pc = fmin(fmax(p[q], 1e-8), 1.0 - 1e-8);
+    const double v = rp[q] * (pc * (1.0 - pc) / T_);
+    zp[q] = v;
+    acc = v;
+  }
+  acc = wave_sum(acc);
+  if (threadIdx.x == 0) part[k] = acc;
 }
 
 // ============================ launchers ======================================
@@ -763,18 +874,60 @@ int hfx_scf_resid(int dtype, int nb, size_t nk, const void* F, const void* D, co
   return check_launch("pk_scf_resid_kernel");
 }
 
-int hfx_batched_dots(int dtype, int nb, size_t nk, const void* X, const void* const* Ys, int nv, int mode,
+int hfx_batched_dots(int dtype, int nb, size_t nk, const void* const* Xs, const void* const* Ys, int nv, int mode,
                      double* part, void* stream) {
-  if (nb < 1 || !X || !Ys || !part || nv < 1 || nv > 8 || (mode != 0 && mode != 1))
+  if (nb < 1 || !Xs || !Ys || !part || nv < 1 || nv > 8 || (mode != 0 && mode != 1))
     return set_error(HFX_EINVAL, "hfx_batched_dots: bad argument");
   DotList L{};
-  for (int v = 0; v < nv; ++v) L.y[v] = Ys[v];
+  for (int v = 0; v < nv; ++v) {
+    L.x[v] = Xs[v];
+    L.y[v] = Ys[v];
+  }
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_dots_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(double, X), L, nv, mode, part);
+    hipLaunchKernelGGL(pk_dots_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nk, L, nv, mode, part);
   else
-    hipLaunchKernelGGL(pk_dots_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(float, X), L, nv, mode, part);
+    hipLaunchKernelGGL(pk_dots_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nk, L, nv, mode, part);
   return check_launch("pk_dots_kernel");
+}
+
+int hfx_newton_dP(int dtype, int nb, size_t nk, const void* Q, const void* X, const double* p, const double* dp,
+                  void* dP, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(pk_newton_dP_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKC(double, Q), PKC(double, X), p,
+                       dp, PKM(double, dP));
+  else
+    hipLaunchKernelGGL(pk_newton_dP_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKC(float, Q), PKC(float, X), p, dp,
+                       PKM(float, dP));
+  return check_launch("pk_newton_dP_kernel");
+}
+
+int hfx_newton_hvp(int dtype, int nb, size_t nk, const void* Q, const void* R, const void* Ft, const void* X,
+                   const double* p, const double* dp, double T, void* HX, double* Hp, double* part, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(pk_newton_hvp_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKC(double, Q), PKC(double, R),
+                       PKC(double, Ft), PKC(double, X), p, dp, T, PKM(double, HX), Hp, part);
+  else
+    hipLaunchKernelGGL(pk_newton_hvp_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKC(float, Q), PKC(float, R),
+                       PKC(float, Ft), PKC(float, X), p, dp, T, PKM(float, HX), Hp, part);
+  return check_launch("pk_newton_hvp_kernel");
+}
+
+int hfx_newton_prec(int dtype, int nb, size_t nk, const void* RX, const double* rp, const double* eps,
+                    const double* p, const double* st, double T, void* ZX, double* zp, double* part, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(pk_newton_prec_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKC(double, RX), rp, eps, p, st, T,
+                       PKM(double, ZX), zp, part);
+  else
+    hipLaunchKernelGGL(pk_newton_prec_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKC(float, RX), rp, eps, p, st, T,
+                       PKM(float, ZX), zp, part);
+  return check_launch("pk_newton_prec_kernel");
 }
 
 }  // extern "C"

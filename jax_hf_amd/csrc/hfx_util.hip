@@ -129,6 +129,54 @@ __global__ void fermi_kernel(const double* x, const double* mu, double T, size_t
     out[i] = 1.0 / (1.0 + exp((x[i] - m) / (T + 1e-12)));
 }
 
+// x[i] -= scale * (*s) : the particle-conserving shift of solver.py:224-233
+__global__ void vec_shift_kernel(double* x, size_t n, const double* s, double scale) {
+  const double d = (*s) * scale;
+  for (size_t i = (size_t)blockIdx.x * UTIL_BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * UTIL_BLOCK) x[i] -= d;
+}
+
+// part[v][k] = sum_b x_v[k, b] y_v[k, b]
+struct VecPairs {
+  const double* x[8];
+  const double* y[8];
+};
+__global__ void vec_dots_kernel(VecPairs P, int nv, size_t nk, int nb, double* part) {
+  for (size_t k = (size_t)blockIdx.x * UTIL_BLOCK + threadIdx.x; k < nk; k += (size_t)gridDim.x * UTIL_BLOCK)
+    for (int v = 0; v < nv; ++v) {
+      double acc = 0.0;
+      for (int b = 0; b < nb; ++b) acc += P.x[v][k * nb + b] * P.y[v][k * nb + b];
+      part[(size_t)v * nk + k] = acc;
+    }
+}
+
+// out[k, c] = a conj(X[k, c]) Y[k, c], or summed over the nch channels per k
+template <typename T>
+__global__ void cmul_kernel(size_t nk, int nch, const cx<T>* X, const cx<T>* Y, double ar, double ai, int sum_ch,
+                            cx<T>* out) {
+  const size_t total = sum_ch ? nk : nk * nch;
+  for (size_t i = (size_t)blockIdx.x * UTIL_BLOCK + threadIdx.x; i < total; i += (size_t)gridDim.x * UTIL_BLOCK) {
+    double re = 0.0, im = 0.0;
+    const size_t c0 = sum_ch ? i * nch : i, c1 = sum_ch ? c0 + nch : i + 1;
+    for (size_t c = c0; c < c1; ++c) {
+      const cx<T> x = X[c], y = Y[c];
+      re += (double)x.re * y.re + (double)x.im * y.im;   // conj(x) y
+      im += (double)x.re * y.im - (double)x.im * y.re;
+    }
+    out[i] = cx<T>{(T)(ar * re - ai * im), (T)(ar * im + ai * re)};
+  }
+}
+
+// out[k, c] = alpha w[k] X[k, c]  (per-k weight scaling, complex storage kinds)
+__global__ void scale_k_kernel(size_t nk, int nch, int kind, const void* X, const double* w, double alpha, void* out) {
+  const size_t total = nk * nch;
+  for (size_t i = (size_t)blockIdx.x * UTIL_BLOCK + threadIdx.x; i < total; i += (size_t)gridDim.x * UTIL_BLOCK) {
+    double re, im;
+    load_any(X, kind, i, re, im);
+    const double s = alpha * (w ? w[i / nch] : 1.0);
+    store_any(out, kind, i, s * re, s * im);
+  }
+}
+
 static int grid_for(size_t n) {
   size_t g = (n + UTIL_BLOCK - 1) / UTIL_BLOCK;
   if (g > UTIL_MAX_BLOCKS) g = UTIL_MAX_BLOCKS;
@@ -164,6 +212,46 @@ int hfx_fermi(const double* x, const double* mu, double T, size_t n, double* out
   if (n == 0) return HFX_OK;
   hipLaunchKernelGGL(fermi_kernel, dim3(grid_for(n)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream, x, mu, T, n, out);
   return check_launch("fermi_kernel");
+}
+
+int hfx_vec_shift(double* x, size_t n, const double* s, double scale, void* stream) {
+  if (!x || !s) return set_error(HFX_EINVAL, "hfx_vec_shift: bad argument");
+  if (n == 0) return HFX_OK;
+  hipLaunchKernelGGL(vec_shift_kernel, dim3(grid_for(n)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream, x, n, s, scale);
+  return check_launch("vec_shift_kernel");
+}
+
+int hfx_vec_dots(const double* const* xs, const double* const* ys, int nv, size_t nk, int nb, double* part,
+                 void* stream) {
+  if (!xs || !ys || !part || nv < 1 || nv > 8 || nb < 1) return set_error(HFX_EINVAL, "hfx_vec_dots: bad argument");
+  VecPairs P{};
+  for (int v = 0; v < nv; ++v) {
+    P.x[v] = xs[v];
+    P.y[v] = ys[v];
+  }
+  hipLaunchKernelGGL(vec_dots_kernel, dim3(grid_for(nk)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream, P, nv, nk, nb, part);
+  return check_launch("vec_dots_kernel");
+}
+
+int hfx_cmul(int dtype, size_t nk, int nch, const void* X, const void* Y, double ar, double ai, int sum_ch,
+             void* out, void* stream) {
+  if (!X || !Y || !out || nch < 1) return set_error(HFX_EINVAL, "hfx_cmul: bad argument");
+  const size_t n = sum_ch ? nk : nk * nch;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(cmul_kernel<double>, dim3(grid_for(n)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream, nk, nch,
+                       (const cx<double>*)X, (const cx<double>*)Y, ar, ai, sum_ch, (cx<double>*)out);
+  else
+    hipLaunchKernelGGL(cmul_kernel<float>, dim3(grid_for(n)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream, nk, nch,
+                       (const cx<float>*)X, (const cx<float>*)Y, ar, ai, sum_ch, (cx<float>*)out);
+  return check_launch("cmul_kernel");
+}
+
+int hfx_scale_k(size_t nk, int nch, int kind, const void* X, const double* w, double alpha, void* out,
+                void* stream) {
+  if (!X || !out || nch < 1 || kind < 0 || kind > 3) return set_error(HFX_EINVAL, "hfx_scale_k: bad argument");
+  hipLaunchKernelGGL(scale_k_kernel, dim3(grid_for(nk * nch)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream, nk, nch, kind,
+                     X, w, alpha, out);
+  return check_launch("scale_k_kernel");
 }
 
 int hfx_hermitize(const void* X, void* Y, size_t nmat, int nb, int dtype, void* stream) {

@@ -56,7 +56,7 @@ class SolverConfig:
     block_sizes: tuple | None = None
     project_fn: Any = None
     precision: str = "auto"
-    mixed_tol_E: float = 1e-6
+    mixed_tol_E: float = 1e-7   # relative windowed-energy stop of the FP32 phase (x max(1, |E|))
 
 
 class SolveResult(NamedTuple):
@@ -167,6 +167,9 @@ class _CG:
                    mu0_from_state=from_state, maxiter=self.cfg.mu_maxiter, mode=K.MU_NEWTON_BRACKET,
                    st=self.st, p_out=self.p)
 
+    def set_target(self, n_electrons):
+        self.n_norm = float(n_electrons) / max(self.kernel.weight_sum, 1e-30)
+
     def init_from(self, P0):
         """solver.py:380-394: eigh(F[herm(P0)]) -> (Q, eps), mu, p."""
         P0h = torch.empty(self.shape4, dtype=self.cdt, device=self.dev)
@@ -180,9 +183,9 @@ class _CG:
         self.p.copy_(p)
         self.st[K.ST_MU] = float(mu)
 
-    def run(self, n_electrons, tol_E, tol_grad, max_iter, state0=None):
+    def run(self, n_electrons, tol_E, tol_grad, max_iter, rel_tol=False):
         cfg = self.cfg
-        self.n_norm = float(n_electrons) / max(self.kernel.weight_sum, 1e-30)
+        self.set_target(n_electrons)
         nk, nb = self.nk, self.nb
         dt = K.dt_code(self.Q)
         st, ws, part = self.st, self.ws, self.ws.part
@@ -198,8 +201,11 @@ class _CG:
         k = 0
         grad_norm, dE, E_prev = np.inf, np.inf, np.inf
         shrink_pow = cfg.bt_shrink ** cfg.bt_max
+        tol_E_rel = tol_E
         while True:
-            # stopping rule (solver.py:422-449)
+            # stopping rule (solver.py:422-449); rel_tol scales tol_E by |E| (FP32 phase)
+            if rel_tol and hist_E:
+                tol_E = tol_E_rel * max(1.0, abs(hist_E[-1]))
             if cfg.plateau_window > 0:
                 e_now = hist_E[max(k - 1, 0)] if hist_E else 0.0
                 e_past = hist_E[max(k - 1 - cfg.plateau_window, 0)] if hist_E else 0.0
@@ -266,18 +272,23 @@ def _solve_cg(kernel, P0, n_electrons, cfg: SolverConfig, fock_build_fn):
     dev = kernel.device
     if prec == "mixed":
         lo = _CG(kernel, torch.complex64, cfg, fock_build_fn)
+        lo.set_target(n_electrons)
         lo.init_from(P0)
-        k1, g1, hE1, hG1 = lo.run(n_electrons, max(cfg.tol_E, cfg.mixed_tol_E), cfg.tol_grad, cfg.max_iter)
+        k1, g1, hE1, hG1 = lo.run(n_electrons, cfg.mixed_tol_E, 0.0, cfg.max_iter, rel_tol=True)
+        # FP64 refinement restarts from the mixed density: eigh of the FP64 Fock
+        # matrix gives exactly unitary orbitals (an FP32 Q is unitary only to ~1e-7)
+        P_lo = lo.density()
         hi = _CG(kernel, torch.complex128, cfg, fock_build_fn)
-        hi.n_norm = float(n_electrons) / max(kernel.weight_sum, 1e-30)
-        hi.warm_from(lo.Q, lo.p, float(lo.st[K.ST_MU]))
+        hi.set_target(n_electrons)
+        hi.init_from(convert(P_lo, torch.complex128))
+        del lo
         k2, grad, hE2, hG2 = hi.run(n_electrons, cfg.tol_E, cfg.tol_grad, max(cfg.max_iter - k1, 0))
         k, hist_E, hist_G = k1 + k2, hE1 + hE2, hG1 + hG2
         runner = hi
-        del lo
     else:
         cdt = torch.complex128 if prec == "fp64" else torch.complex64
         runner = _CG(kernel, cdt, cfg, fock_build_fn)
+        runner.set_target(n_electrons)
         runner.init_from(P0)
         k, grad, hist_E, hist_G = runner.run(n_electrons, cfg.tol_E, cfg.tol_grad, cfg.max_iter)
     Pfin, Ffin = runner.finalize()

@@ -63,7 +63,7 @@ def tight_binding(n1: int, n2: int, nb: int, seed: int, rng=None) -> np.ndarray:
     for R, t in zip(HOPS, ts):
         ph = np.exp(1j * (k @ R))
         M += ph[..., None, None] * t[None, None]
-    return M + np.conj(np.swapaxes(M, -1, -2))
+    return np.ascontiguousarray(M + np.conj(np.swapaxes(M, -1, -2)))
 
 
 def coulomb_scalar(n1: int, n2: int, U0: float = 5.0, q0: float = 0.3) -> np.ndarray:
@@ -94,7 +94,7 @@ def haar_density(n1: int, n2: int, nb: int, rng) -> np.ndarray:
     f = rng.uniform(0.0, 1.0, size=(n1 * n2, nb))
     P = np.einsum("kin,kn,kjn->kij", Q, f, np.conj(Q))
     P = 0.5 * (P + np.conj(np.swapaxes(P, -1, -2)))
-    return P.reshape(n1, n2, nb, nb)
+    return np.ascontiguousarray(P.reshape(n1, n2, nb, nb))
 
 
 def uniform_weights(n1: int, n2: int) -> np.ndarray:

@@ -124,11 +124,12 @@ def test_block_specs_gpu(dev):
     VR = rng.normal(size=(nk, nk, nb, nb)) + 1j * rng.normal(size=(nk, nk, nb, nb))
     w = np.ones((nk, nk))
     h = np.zeros_like(P)
-    kw = dict(h=h, VR=VR, refP=None, HH=np.zeros((nb, nb)), w2d=w, include_exchange=True,
+    kw = dict(h=h, VR=VR, HH=np.zeros((nb, nb)), w2d=w, include_exchange=True,
               include_hartree=False, exchange_hermitian_channel_packing=False)
     for Pc, spec in ((P, (("sizes", (2, 2)),)), (P + 0.1, (("sizes", (2, 2)),))):
-        S0, _, _ = O.build_fock(Pc, exchange_block_specs=spec, exchange_offdiag_atol=1e-6, **kw)
-        S1, _, _ = J.build_fock(Pc, exchange_block_specs=spec, exchange_offdiag_atol=1e-6, **kw)
+        S0, _, _ = O.build_fock(Pc, refP=np.zeros_like(Pc), exchange_block_specs=spec,
+                                exchange_offdiag_atol=1e-6, **kw)
+        S1, _, _ = J.build_fock(Pc, refP=None, exchange_block_specs=spec, exchange_offdiag_atol=1e-6, **kw)
         assert _rel(_np(S1), S0) < 1e-12
 
 

@@ -1,0 +1,315 @@
+"""GPU parity of the per-k kernels and the solvers against the FP64 oracle.
+
+Bars: batched eigh -- eigenvalues within 1e-12 (FP64) / 2e-5 (FP32) relative to
+LAPACK, reconstruction and orthonormality; mu -- 1e-12; CG / SCF -- converged
+energy within 1e-10 per k-point (weights sum to 1, FP64) of the oracle run with
+the same configuration, density within 1e-6; the reference fixture's
+converged state within its complex64 precision.
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from jax_hf_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _herm(X):
+    return 0.5 * (X + np.conj(np.swapaxes(X, -1, -2)))
+
+
+@pytest.mark.parametrize("nb", [1, 2, 3, 4, 5, 8, 13, 16])
+@pytest.mark.parametrize("dt", ["c128", "c64"])
+def test_eigh_batched(dev, nb, dt):
+    import torch
+    from jax_hf_amd import _ops as K
+    rng = np.random.default_rng(nb)
+    A = _herm(rng.standard_normal((5, 7, nb, nb)) + 1j * rng.standard_normal((5, 7, nb, nb)))
+    A[0, 0] = np.diag(np.arange(nb, dtype=float))           # already diagonal
+    A[0, 1] = np.eye(nb)                                     # fully degenerate
+    if nb >= 2:
+        A[0, 2] = np.diag(np.r_[np.ones(nb - 1), 2.0])       # degenerate block
+    cdt = torch.complex128 if dt == "c128" else torch.complex64
+    At = torch.from_numpy(A).to(dev).to(cdt)
+    lam = torch.empty((5, 7, nb), dtype=torch.float64, device=dev)
+    V = torch.empty_like(At)
+    K.eigh(At, lam, V)
+    w_ref = np.linalg.eigvalsh(A)
+    lam_h, V_h = _np(lam), _np(V).astype(np.complex128)
+    tol = 1e-12 if dt == "c128" else 2e-5
+    scale = np.abs(w_ref).max()
+    assert np.abs(lam_h - w_ref).max() <= tol * scale * 10
+    assert np.all(np.diff(lam_h, axis=-1) >= -tol * scale)
+    rec = V_h @ (lam_h[..., None] * np.conj(np.swapaxes(V_h, -1, -2)))
+    assert np.abs(rec - A).max() <= 50 * tol * scale
+    eye = np.conj(np.swapaxes(V_h, -1, -2)) @ V_h
+    assert np.abs(eye - np.eye(nb)).max() <= 50 * tol
+
+
+def test_density_and_matmul(dev):
+    import torch
+    from jax_hf_amd import _ops as K
+    rng = np.random.default_rng(0)
+    nb = 6
+    Q = np.linalg.qr(rng.standard_normal((3, 4, nb, nb)) + 1j * rng.standard_normal((3, 4, nb, nb)))[0]
+    p = rng.uniform(0, 1, (3, 4, nb))
+    Qt = torch.from_numpy(Q).to(dev)
+    P = torch.empty_like(Qt)
+    K.density(Qt, P, p=torch.from_numpy(p).to(dev))
+    ref = _herm(np.einsum("...in,...n,...jn->...ij", Q, p, np.conj(Q)))
+    assert np.abs(_np(P) - ref).max() < 1e-14
+    F = _herm(rng.standard_normal((3, 4, nb, nb)) + 1j * rng.standard_normal((3, 4, nb, nb)))
+    Ft = torch.from_numpy(F).to(dev)
+    out = torch.empty_like(Qt)
+    K.matmul(Qt, Ft, out, C=Qt, ops=1)       # Q^H F Q
+    ref = np.conj(np.swapaxes(Q, -1, -2)) @ F @ Q
+    assert np.abs(_np(out) - ref).max() < 1e-13
+
+
+@pytest.mark.parametrize("T", [0.5, 0.05, 0.005])
+def test_mu_solvers_match_oracle(dev, T):
+    import torch
+    import jax_hf_amd.utils as U
+    from jax_hf_amd import _ops as K
+    rng = np.random.default_rng(3)
+    eps = rng.standard_normal((20, 20, 6))
+    w = rng.uniform(0.5, 1.5, (20, 20))
+    wn = w / w.sum()
+    target = 2.3
+    mu0 = O.solve_mu(eps, wn, target, 0.1, T, 25)
+    et = torch.from_numpy(eps).to(dev)
+    ws = K.Workspace.get(dev, 400)
+    st = torch.zeros(16, dtype=torch.float64, device=dev)
+    p = torch.empty_like(et)
+    K.solve_mu(ws, et, torch.from_numpy(wn).to(dev), 400, 6, target, T, mu0=0.1, maxiter=25,
+               mode=K.MU_NEWTON_BRACKET, st=st, p_out=p)
+    assert abs(float(st[0]) - mu0) < 1e-12
+    assert np.abs(_np(p) - 1 / (1 + np.exp(-(mu0 - eps) / T))).max() < 1e-12
+    n = target * w.sum()
+    for method in ("bisection", "newton"):
+        m_ref = O.find_chemical_potential(eps, w, n, T, method=method)
+        m_gpu = float(U.find_chemical_potential(torch.from_numpy(eps).to(dev), w, n, T, method=method))
+        assert abs(m_gpu - m_ref) < 1e-12
+
+
+def test_density_matrix_from_fock_fixture(dev, meshgrid_case):
+    import jax_hf_amd.utils as U
+    d = meshgrid_case
+    P, mu = U.density_matrix_from_fock(d["F_expected"].astype(np.complex128), d["weights"],
+                                       float(d["electrondensity0"]), float(d["T"]))
+    P0, mu0 = O.density_matrix_from_fock(d["F_expected"], d["weights"].astype(float),
+                                         float(d["electrondensity0"]), float(d["T"]))
+    assert abs(float(mu) - mu0) < 1e-12
+    assert np.abs(_np(P) - P0).max() < 1e-12
+    assert np.abs(_np(P) - d["P_expected"]).max() < 5e-6
+
+
+def _fixture_kernels(d):
+    import jax_hf_amd as J
+    kw = dict(include_hartree=True, reference_density=d["reference_density"].astype(np.complex128),
+              hartree_matrix=d["hartree_matrix"])
+    h = d["hamiltonian"].astype(np.complex128)
+    return (O.OracleKernel(d["weights"], h, d["coulomb_q"], float(d["T"]), **kw),
+            J.HartreeFockKernel(d["weights"], h, d["coulomb_q"], float(d["T"]), **kw))
+
+
+def test_cg_fixture_fixed_point(dev, meshgrid_case):
+    import jax_hf_amd as J
+    d = meshgrid_case
+    K0, G = _fixture_kernels(d)
+    n = float(d["electrondensity0"])
+    cfg = J.SolverConfig(max_iter=300, tol_E=1e-12)
+    r = J.solve(G, np.zeros_like(d["hamiltonian"], dtype=np.complex128), n, config=cfg)
+    r0 = O.solve(K0, np.zeros_like(K0.h), n, config=O.SolverConfig(max_iter=300, tol_E=1e-12))
+    assert r.converged
+    assert abs(float(r.energy) - float(d["E_expected"])) < 1e-6
+    assert np.abs(_np(r.density) - d["P_expected"]).max() < 1e-5
+    assert abs(float(r.energy) - r0.energy) < 1e-10
+    assert np.abs(_np(r.density) - r0.density).max() < 1e-6
+    # the first iterations follow the oracle trajectory
+    hE = _np(r.history["E"])
+    assert np.abs(hE[:5] - r0.history["E"][:5]).max() < 1e-10
+
+
+def test_cg_config0_fp64_matches_oracle(dev):
+    import jax_hf_amd as J
+    c = syn.config0(nk=24, nb=4, seed=0)
+    G = J.HartreeFockKernel(c["weights"], c["hamiltonian"], c["coulomb_q"], c["T"])
+    K0 = O.OracleKernel(c["weights"], c["hamiltonian"], c["coulomb_q"], c["T"])
+    cfg_kw = dict(max_iter=400, tol_E=1e-12)
+    r = J.solve(G, c["P0"], c["n_electrons"], config=J.SolverConfig(**cfg_kw))
+    r0 = O.solve(K0, c["P0"], c["n_electrons"], config=O.SolverConfig(**cfg_kw))
+    assert r.converged and r0.converged
+    assert abs(float(r.energy) - r0.energy) < 1e-10          # weights sum to 1: per k-point
+    assert np.abs(_np(r.density) - r0.density).max() < 1e-6
+    def comm_rms(F, P):
+        c = F @ P - P @ F
+        return np.sqrt(np.mean(np.sum(np.abs(c) ** 2, axis=(-2, -1))))
+    c_gpu, c_ref = comm_rms(_np(r.fock), _np(r.density)), comm_rms(r0.fock, r0.density)
+    assert abs(c_gpu - c_ref) < 0.1 * c_ref + 1e-8
+
+
+def test_cg_mixed_precision_energy_error(dev):
+    import jax_hf_amd as J
+    c = syn.config0(nk=24, nb=4, seed=0)
+    G = J.HartreeFockKernel(c["weights"], c["hamiltonian"], c["coulomb_q"], c["T"])
+    K0 = O.OracleKernel(c["weights"], c["hamiltonian"], c["coulomb_q"], c["T"])
+    r0 = O.solve(K0, c["P0"], c["n_electrons"], config=O.SolverConfig(max_iter=400, tol_E=1e-12))
+    r = J.solve(G, c["P0"], c["n_electrons"],
+                config=J.SolverConfig(max_iter=400, tol_E=1e-12, precision="mixed"))
+    assert r.converged
+    assert abs(float(r.energy) - r0.energy) < 1e-8            # |dE| < 1e-8 per k-point
+
+
+def _contact_kernels(nk=2, g=0.3, T=0.05):
+    import jax_hf_amd as J
+    h = np.zeros((nk, nk, 2, 2), np.complex128)
+    h[..., 0, 0], h[..., 1, 1] = -1.0, 1.0
+    for i in range(nk):
+        for j in range(nk):
+            h[i, j, 0, 1] = h[i, j, 1, 0] = 0.4 + 0.1 * (i + j)
+    sz = np.diag([1.0, -1.0]).astype(complex)
+    args = (np.ones((nk, nk)), h, np.full((nk, nk, 1, 1), 0.1), T)
+    return (O.OracleKernel(*args, contact_terms=[(g, sz, sz)]),
+            J.HartreeFockKernel(*args, contact_terms=[(g, sz, sz)]))
+
+
+def test_cg_and_scf_agree_with_contact(dev):
+    import jax_hf_amd as J
+    K0, G = _contact_kernels()
+    P0 = np.zeros((2, 2, 2, 2), np.complex128)
+    dm = J.solve(G, P0, 4.0, config=J.SolverConfig(max_iter=200, tol_E=1e-9))
+    scf = J.solve_scf(G, P0, 4.0, config=J.SCFConfig(max_iter=400, mixing=0.3, density_tol=1e-7,
+                                                     comm_tol=1e-6))
+    assert dm.converged and scf.converged
+    assert abs(float(dm.energy) - float(scf.energy)) < 1e-5
+    assert np.linalg.norm(_np(dm.density) - _np(scf.density_matrix)) < 1e-4
+    n = dm.n_iter
+    assert abs(float(_np(dm.history["E"])[n - 1]) - float(dm.energy)) < 1e-5
+
+
+@pytest.mark.parametrize("acc", ["linear", "diis", "oda"])
+def test_scf_matches_oracle(dev, meshgrid_case, acc):
+    import jax_hf_amd as J
+    d = meshgrid_case
+    K0, G = _fixture_kernels(d)
+    n = float(d["electrondensity0"])
+    kw = dict(max_iter=300, acceleration=acc, density_tol=1e-10, comm_tol=1e-9)
+    r = J.solve_scf(G, np.zeros_like(K0.h), n, config=J.SCFConfig(**kw))
+    r0 = O.solve_scf(K0, np.zeros_like(K0.h), n, config=O.SCFConfig(**kw))
+    assert r.converged == r0.converged
+    assert abs(float(r.energy) - r0.energy) < 1e-10
+    assert np.abs(_np(r.density_matrix) - r0.density_matrix).max() < 1e-8
+    assert abs(r.iterations - r0.iterations) <= 2
+    assert abs(float(r.chemical_potential) - r0.chemical_potential) < 1e-9
+
+
+def test_scf_level_shift_and_trust_radius(dev, meshgrid_case):
+    import jax_hf_amd as J
+    d = meshgrid_case
+    K0, G = _fixture_kernels(d)
+    n = float(d["electrondensity0"])
+    kw = dict(max_iter=300, level_shift=0.2, trust_radius=0.05, density_tol=1e-9, comm_tol=1e-8)
+    r = J.solve_scf(G, np.zeros_like(K0.h), n, config=J.SCFConfig(**kw))
+    r0 = O.solve_scf(K0, np.zeros_like(K0.h), n, config=O.SCFConfig(**kw))
+    assert r.converged and r0.converged
+    assert abs(float(r.energy) - r0.energy) < 1e-9
+    np.testing.assert_allclose(_np(r.history["E"])[:5], r0.history["E"][:5], atol=1e-10)
+
+
+def test_gradient_stop_and_unreachable_count(dev):
+    import jax_hf_amd as J
+    h = np.zeros((2, 2, 2, 2), np.complex128)
+    h[..., 0, 0], h[..., 1, 1] = -0.5, 0.5
+    h[..., 0, 1] = h[..., 1, 0] = 0.2
+    G = J.HartreeFockKernel(np.ones((2, 2)), h, np.full((2, 2, 1, 1), 0.5), 0.1)
+    P0 = np.zeros_like(h)
+    P0[..., 0, 0], P0[..., 1, 1] = 0.6, 0.15
+    P0[..., 0, 1] = P0[..., 1, 0] = 0.25
+    tol = 1e-4
+    r = J.solve(G, P0, 3.0, config=J.SolverConfig(max_iter=300, tol_grad=tol))
+    assert r.converged and 1 < r.n_iter < 300
+    hG = _np(r.history["grad_norm"])[:r.n_iter]
+    assert hG[-1] <= tol and np.all(hG[:-1] > tol)
+    r3 = J.solve(G, P0, 3.0, config=J.SolverConfig(max_iter=3, tol_grad=1e-14))
+    assert not r3.converged and r3.n_iter == 3
+    with pytest.raises(ValueError, match="physically reachable range"):
+        J.solve(G, P0, 9.0)
+    with pytest.raises(ValueError, match="optimizer"):
+        J.solve(G, P0, 3.0, config=J.SolverConfig(optimizer="bfgs"))
+
+
+# ----------------------------------------------------------------- Newton
+
+def _multiband(nk, nb, T, ex, seed):
+    import jax_hf_amd as J
+    rng = np.random.default_rng(seed)
+    diag = np.linspace(-1.0, 1.0, nb)
+    A = (rng.standard_normal((nb, nb)) + 1j * rng.standard_normal((nb, nb))) * 0.35
+    A = 0.5 * (A + A.conj().T)
+    h = np.zeros((nk, nk, nb, nb), np.complex128)
+    kx = np.arange(nk) / nk * 2 * np.pi
+    for i in range(nk):
+        for j in range(nk):
+            hk = np.diag(diag).astype(np.complex128) + A * (np.cos(kx[i]) + np.cos(kx[j]))
+            h[i, j] = 0.5 * (hk + hk.conj().T)
+    args = (np.ones((nk, nk)), h, np.full((nk, nk, 1, 1), ex), T)
+    return O.OracleKernel(*args), J.HartreeFockKernel(*args)
+
+
+def test_newton_matches_oracle_and_cg(dev):
+    import jax_hf_amd as J
+    K0, G = _contact_kernels()
+    P0 = np.zeros((2, 2, 2, 2), np.complex128)
+    nw = J.solve(G, P0, 4.0, config=J.SolverConfig(max_iter=200, tol_grad=1e-7, optimizer="newton"))
+    nw0 = O.solve(K0, P0, 4.0, config=O.SolverConfig(max_iter=200, tol_grad=1e-7, optimizer="newton"))
+    assert nw.converged and nw0.converged
+    assert abs(float(nw.energy) - nw0.energy) < 1e-10
+    assert abs(nw.n_iter - nw0.n_iter) <= 1
+
+
+@pytest.mark.parametrize("nk,nb,T,ex,seed", [(4, 4, 0.05, 0.3, 7), (5, 5, 0.05, 0.4, 4)])
+def test_newton_multiband_stationary(dev, nk, nb, T, ex, seed):
+    import jax_hf_amd as J
+    K0, G = _multiband(nk, nb, T, ex, seed)
+    P0 = np.zeros((nk, nk, nb, nb), np.complex128)
+    ne = float(nk * nk * nb * 0.5)
+    r = J.solve(G, P0, ne, config=J.SolverConfig(max_iter=200, tol_grad=1e-6, optimizer="newton"))
+    r0 = O.solve(K0, P0, ne, config=O.SolverConfig(max_iter=200, tol_grad=1e-6, optimizer="newton"))
+    assert r.converged
+    Fh, Ph = _np(r.fock), _np(r.density)
+    comm = Fh @ Ph - Ph @ Fh
+    assert np.sqrt(np.mean(np.sum(np.abs(comm) ** 2, axis=(-2, -1)))) < 1e-4
+    assert abs(float(np.sum(np.ones((nk, nk))[..., None] * _np(r.p))) - ne) < 1e-6
+    assert abs(float(r.energy) - r0.energy) < 1e-9 * nk * nk
+
+
+def test_newton_cold_start_relaxation(dev):
+    import jax_hf_amd as J
+    h = np.zeros((2, 2, 2, 2), np.complex128)
+    h[..., 0, 0], h[..., 1, 1] = -0.5, 0.5
+    args = (np.ones((2, 2)), h, np.full((2, 2, 1, 1), 0.25), 0.1)
+    G = J.HartreeFockKernel(*args)
+    P0 = np.zeros_like(h)
+    cg = J.solve(G, P0, 4.0, config=J.SolverConfig(max_iter=400, tol_E=1e-12, tol_grad=1e-9))
+    nw = J.solve(G, P0, 4.0, config=J.SolverConfig(max_iter=200, tol_grad=1e-8, optimizer="newton"))
+    np.testing.assert_allclose(float(nw.energy), float(cg.energy), atol=1e-8, rtol=1e-9)
+
+
+def test_newton_deflation_matches_oracle(dev):
+    import jax_hf_amd as J
+    K0, G = _contact_kernels()
+    P0 = np.zeros((2, 2, 2, 2), np.complex128)
+    base = O.solve(K0, P0, 4.0, config=O.SolverConfig(optimizer="newton", tol_grad=1e-8))
+    kw = dict(optimizer="newton", tol_grad=1e-8, max_iter=100, deflation_targets=base.density[None],
+              deflation_sigma=0.05, deflation_length=0.5)
+    r = J.solve(G, P0, 4.0, config=J.SolverConfig(**kw))
+    r0 = O.solve(K0, P0, 4.0, config=O.SolverConfig(**kw))
+    assert abs(float(r.energy) - r0.energy) < 1e-9
+    assert r.n_iter == r0.n_iter
