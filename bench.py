@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--dtype", choices=["c128", "c64"], default="c128")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in exchange-kernel variant instead of timing all of them first")
     ap.add_argument("--extra", action="store_true", help="also time the packed and complex64 forms")
     ap.add_argument("--solve", action="store_true",
                     help="also time converged CG solves (FP64 and mixed) at the same mesh")
@@ -120,6 +122,11 @@ def main():
     bytes_exch = csz * Nk * (3 * nb * nb)               # P + VR in, Sigma out
     bytes_build = csz * Nk * (4 * nb * nb + nb * nb)    # + h in, F out
 
+    variant_ms = None
+    if not args.no_autotune:
+        best, variant_ms = eng.plan.autotune(P, eng.VR, Sig, reps=10)
+    kname = {1: "exch_ct_kernel"}.get(eng.plan.exchange_variant, "exch_v2_kernel")
+
     def step(evs=None):
         if evs is not None:
             evs[0].record()
@@ -160,7 +167,9 @@ def main():
                                f"all {nb * nb} channels, Sigma+F+E outputs",
                    "nk": nk, "nb": nb, "parallelism": "1 GPU"},
         "fock_build_GBps": bytes_build / (ms_step * 1e-3) / 1e9,
-        "roofline": {"bound": "hbm", "kernel": "exch_ct_kernel", "achieved": achieved,
+        "exchange_variant": L.Plan.VARIANTS[eng.plan.exchange_variant],
+        "exchange_variants_ms": variant_ms,
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "algorithmic_bytes": bytes_exch, "avg_ms": exch_ms},
         "energy": float(E),
@@ -227,6 +236,11 @@ def solve_forms(J, syn, inp, dev, nk, nb, args):
             E_ref = E
         res[prec] = {"solves_per_s": 1.0 / dt, "seconds": dt, "n_iter": r.n_iter, "converged": r.converged,
                      "energy": E, "abs_dE_vs_fp64": abs(E - E_ref)}
+        if "phase_iters" in r.history:
+            hE = r.history["E"].cpu().numpy()
+            res[prec]["phase_iters"] = [int(x) for x in r.history["phase_iters"]]
+            res[prec]["E_head"] = [float(x) for x in hE[:12]]
+            res[prec]["E_tail"] = [float(x) for x in hE[max(r.n_iter - 6, 0):r.n_iter]]
     res["speedup_mixed_vs_fp64"] = res["fp64"]["seconds"] / res["mixed"]["seconds"]
     return res
 

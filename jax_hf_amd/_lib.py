@@ -18,7 +18,7 @@ LIB_PATH = _HERE / "libhfx.so"
 
 HFX_C128 = 0
 HFX_C64 = 1
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 _lib = None
 _lock = threading.Lock()
@@ -35,6 +35,9 @@ _SIGS = {
     "hfx_plan_create": [i32, i32, i32, i32, C.POINTER(vp)],
     "hfx_plan_destroy": [vp],
     "hfx_energy_scratch_bytes": [vp],
+    "hfx_plan_set_exchange_variant": [vp, i32],
+    "hfx_plan_get_exchange_variant": [vp],
+    "hfx_exchange_autotune": [vp, vp, vp, i32, vp, i32, vp, vp, vp],
     "hfx_selfenergy": [vp, vp, vp, vp, i32, vp, i32, i32, i32, vp, vp, vp, vp],
     "hfx_prepare_vr": [vp, vp, i32, vp, i32, vp, vp],
     "hfx_selfenergy_adjoint": [vp, vp, vp, i32, vp, vp],
@@ -151,6 +154,29 @@ class Plan:
         self.handle = h
         nbytes = lib().hfx_energy_scratch_bytes(h)
         self.scratch = torch.empty(nbytes // 8, dtype=torch.float64, device=device)
+
+    VARIANTS = {0: "auto", 1: "v1", 2: "v2_a", 3: "v2_b", 4: "v2_c"}
+
+    @property
+    def exchange_variant(self) -> int:
+        return int(lib().hfx_plan_get_exchange_variant(self.handle))
+
+    @exchange_variant.setter
+    def exchange_variant(self, v: int):
+        call("hfx_plan_set_exchange_variant", self.handle, int(v))
+
+    def autotune(self, P, VR, Sigma, reps: int = 5):
+        """Time every exchange-kernel variant on (P, VR) -> Sigma; keep the fastest.
+
+        Returns (best variant, {name: ms per launch}).
+        """
+        t = (C.c_double * 5)()
+        best = C.c_int()
+        vfull = 0 if VR.shape[-1] == 1 else 1
+        with torch.cuda.device(self.device):
+            call("hfx_exchange_autotune", self.handle, ptr(P), ptr(VR), vfull, ptr(Sigma), int(reps), t,
+                 C.byref(best), stream(self.device))
+        return best.value, {self.VARIANTS[v]: t[v] for v in range(1, 5)}
 
     def __del__(self):
         try:

@@ -70,7 +70,7 @@ using namespace hfx;
 
 extern "C" {
 
-int hfx_abi_version(void) { return 2; }
+int hfx_abi_version(void) { return 3; }
 
 int hfx_last_error(char* buf, size_t n) {
   if (!buf || n == 0) return HFX_EINVAL;
@@ -92,6 +92,7 @@ int hfx_plan_create(int nk1, int nk2, int nb, int dtype, hfx_plan** out) {
   p->n2 = nk2;
   p->nb = nb;
   p->dtype = dtype;
+  p->exch_variant = HFX_EXCH_AUTO;
   int rad1[HFX_MAX_STAGES], rad2[HFX_MAX_STAGES], ns1 = 0, ns2 = 0;
   if (factorize(nk1, rad1, &ns1) || factorize(nk2, rad2, &ns2)) {
     delete p;
@@ -200,6 +201,55 @@ int hfx_fft2_weighted(const hfx_plan* plan, const void* X, const double* w, void
   HFX_REQUIRE(plan && X && w && out, "hfx_fft2_weighted: NULL argument");
   return exchange_launch(plan, X, nullptr, X, 1, plan->chans_full, plan->nb * plan->nb, 0, w, out, nullptr, 1, 0,
                          0, (hipStream_t)stream);
+}
+
+int hfx_plan_set_exchange_variant(hfx_plan* plan, int variant) {
+  HFX_REQUIRE(plan, "hfx_plan_set_exchange_variant: NULL plan");
+  HFX_REQUIRE(variant >= HFX_EXCH_AUTO && variant <= HFX_EXCH_V2_C,
+              "hfx_plan_set_exchange_variant: variant %d out of range", variant);
+  plan->exch_variant = variant;
+  return HFX_OK;
+}
+
+int hfx_plan_get_exchange_variant(const hfx_plan* plan) { return plan ? plan->exch_variant : HFX_EINVAL; }
+
+int hfx_exchange_autotune(hfx_plan* plan, const void* P, const void* VR, int vr_full, void* Sigma, int reps,
+                          double* times_ms, int* best, void* stream) {
+  HFX_REQUIRE(plan && P && VR && Sigma, "hfx_exchange_autotune: NULL argument");
+  HFX_REQUIRE(reps > 0 && reps <= 1000, "hfx_exchange_autotune: reps must be in 1..1000");
+  const hipStream_t s = (hipStream_t)stream;
+  const int keep = plan->exch_variant;
+  hipEvent_t e0, e1;
+  HFX_HIP_TRY(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return set_error(HFX_EHIP, "hfx_exchange_autotune: hipEventCreate failed");
+  }
+  int rc = HFX_OK, arg = HFX_EXCH_V1;
+  double tbest = 1e300;
+  if (times_ms) times_ms[0] = 0.0;
+  for (int v = HFX_EXCH_V1; v <= HFX_EXCH_V2_C && rc == HFX_OK; ++v) {
+    plan->exch_variant = v;
+    const int nch = plan->nb * plan->nb;
+    for (int r = -1; r < reps && rc == HFX_OK; ++r) {  // r = -1: warm-up launch
+      if (r == 0 && hipEventRecord(e0, s) != hipSuccess) rc = set_error(HFX_EHIP, "hipEventRecord failed");
+      if (rc == HFX_OK)
+        rc = exchange_launch(plan, P, nullptr, VR, vr_full, plan->chans_full, nch, 0, nullptr, Sigma, nullptr,
+                             0, 0, 0, s);
+    }
+    float ms = 0.f;
+    if (rc == HFX_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                         hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      rc = set_error(HFX_EHIP, "hfx_exchange_autotune: event timing failed");
+    const double t = (double)ms / reps;
+    if (times_ms) times_ms[v] = t;
+    if (rc == HFX_OK && t < tbest) { tbest = t; arg = v; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  plan->exch_variant = rc == HFX_OK ? arg : keep;
+  if (best) *best = plan->exch_variant;
+  return rc;
 }
 
 int hfx_prepare_vr(const hfx_plan* plan, const void* Vq, int vr_full, const double* w2d, int phase,

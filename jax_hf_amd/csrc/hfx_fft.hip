@@ -184,10 +184,12 @@ struct ExchArgs {
   cx<double>* rho;     // (nb, nb) weighted channel moments, may be null
   int nchan, herm, vfull, xcd_remap;
   int mode;   // 0: exchange convolution; 1: forward transform of w * V (VR preparation)
-  int conjv;  // mode 0: multiply by conj(VR) (the adjoint / cross-correlation)
   int shift;  // mode 0: ifftshift the result on store; mode 1: apply the ifftshift phase
+  int conjv;  // mode 0: multiply by conj(VR) (the adjoint / cross-correlation)
   FftGeom<T> g;
 };
+static_assert(offsetof(ExchArgs<double>, shift) < offsetof(ExchArgs<double>, conjv),
+              "exchange_launch initialises ExchArgs positionally: mode, shift, conjv");
 
 // LDS row pitch: odd, so lanes walking down a column (stride = pitch) and
 // lanes walking along a row (stride 1) both spread over all 64 banks.
@@ -384,13 +386,203 @@ constexpr int EXCH_BLOCK = 1024;
 constexpr int MAXE_F64 = 10176;
 constexpr int MAXE_F32 = 20352;
 
+// ---- v2: two composite-radix Stockham stages per axis ------------------------
+// Row pass: stage 1 loads straight from HBM (fused refP subtraction and channel
+// moment), stage 2 in LDS.  Column pass: stage 1 in LDS, stage 2 multiplies by
+// VR (prefetched into registers before the stage barrier) on the way out.
+// Inverse passes mirror this; the last inverse row stage stores straight to HBM
+// (with the Hermitian mirror).  Twiddle tables live in LDS.  7 LDS round trips
+// per channel instead of 15.
+
+// Composite sizes R = A * B are done in registers by the four-step split
+// X[ka + A kb] = DFT_B over b of { W_R^{b ka} DFT_A over a of x[B a + b] }.
+constexpr int v2_split(int R) {
+  return R == 6 ? 2 : R == 9 ? 3 : R == 10 ? 2 : R == 12 ? 3 : R == 15 ? 3 : R == 16 ? 4 : 0;
+}
+
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dft_any(cx<T> (&v)[R], const cx<T>* tw, int NT) {
+  constexpr int A = v2_split(R);
+  if constexpr (A == 0) {
+    small_dft<T, R>(v, INV, tw, NT);
+  } else {
+    constexpr int B = R / A;
+    cx<T> y[B][A];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      cx<T> t[A];
+#pragma unroll
+      for (int q = 0; q < A; ++q) t[q] = v[B * q + b];
+      dft_any<T, A, INV>(t, tw, NT);
+#pragma unroll
+      for (int q = 0; q < A; ++q) {
+        cx<T> x = t[q];
+        if (b > 0 && q > 0) {
+          const cx<T> w = tw[((b * q) % R) * (NT / R)];
+          x = INV ? mulc(x, w) : x * w;
+        }
+        y[b][q] = x;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < A; ++q) {
+      cx<T> t[B];
+#pragma unroll
+      for (int b = 0; b < B; ++b) t[b] = y[b][q];
+      dft_any<T, B, INV>(t, tw, NT);
+#pragma unroll
+      for (int b = 0; b < B; ++b) v[q + A * b] = t[b];
+    }
+  }
+}
+
+// first (row-pass) and column-pass radix pairs per mesh size
+constexpr int v2_row_a(int N) {
+  return N == 96 ? 12 : N == 90 ? 10 : N == 72 ? 8 : N == 64 ? 8 : N == 60 ? 6 : N == 48 ? 6 : N == 36 ? 6
+       : N == 32 ? 8 : N == 24 ? 6 : N == 16 ? 4 : N == 12 ? 4 : 0;
+}
+constexpr int v2_col_a(int N) {
+  return N == 96 ? 8 : N == 90 ? 9 : N == 72 ? 9 : N == 64 ? 8 : N == 60 ? 10 : N == 48 ? 8 : N == 36 ? 6
+       : N == 32 ? 4 : N == 24 ? 4 : N == 16 ? 4 : N == 12 ? 3 : 0;
+}
+
+// One stage.  SRC 1: inputs from HBM (row pass, lines = rows k1, positions = k2).
+// DST 1: outputs to HBM (same addressing).  VRM: multiply outputs by VR
+// (column pass, lines = columns k2, positions = k1).
+template <typename T, int R, int N, int NL, int LS, int ES, int BLOCK, int NS, bool INV, int SRC, int DST,
+          bool VRM, int N2>
+__device__ __forceinline__ void v2_stage(cx<T>* plane, const cx<T>* tw, const ExchArgs<T>& a, int c,
+                                         double& mre, double& mim) {
+  constexpr int NR = N / R, ITEMS = NL * NR, MAXIT = (ITEMS + BLOCK - 1) / BLOCK, TSTEP = N / (NS * R);
+  const int nb = a.g.nb, nb2 = nb * nb;
+  cx<T> v[MAXIT][R];
+  cx<T> vr[VRM ? MAXIT : 1][VRM ? R : 1];
+#pragma unroll
+  for (int m = 0; m < MAXIT; ++m) {
+    const int it = threadIdx.x + m * BLOCK;
+    if (ITEMS % BLOCK == 0 || it < ITEMS) {
+      const int line = it % NL, j = it / NL, k = j % NS;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int pos = j + r * NR;
+        cx<T> x;
+        if constexpr (SRC == 1) {
+          const size_t e = (size_t)line * N2 + pos;
+          x = a.P[e * nb2 + c];
+          if (a.refP) x = x - a.refP[e * nb2 + c];
+          if (a.w2d) {
+            const double w = a.w2d[e];
+            mre += w * (double)x.re;
+            mim += w * (double)x.im;
+          }
+        } else {
+          x = plane[line * LS + pos * ES];
+        }
+        if (NS > 1 && r > 0) {
+          const cx<T> w = tw[r * k * TSTEP];
+          x = INV ? mulc(x, w) : x * w;
+        }
+        v[m][r] = x;
+      }
+      if constexpr (VRM) {   // prefetch VR at the output positions (k1 = base + r NS, k2 = line)
+        const int base = (j / NS) * NS * R + k;
+        const int vc = a.vfull ? c : 0, vstride = a.vfull ? nb2 : 1;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          vr[m][r] = a.VR[((size_t)(base + r * NS) * N2 + line) * vstride + vc];
+      }
+    }
+  }
+  if constexpr (SRC == 0) __syncthreads();
+#pragma unroll
+  for (int m = 0; m < MAXIT; ++m) {
+    const int it = threadIdx.x + m * BLOCK;
+    if (ITEMS % BLOCK == 0 || it < ITEMS) {
+      const int line = it % NL, j = it / NL, k = j % NS;
+      dft_any<T, R, INV>(v[m], tw, N);
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int pos = base + r * NS;
+        cx<T> y = v[m][r];
+        if constexpr (VRM) {
+          const T scale = (T)(-1.0 / (double)(N * N2));
+          y = scale * (a.conjv ? mulc(y, vr[m][r]) : y * vr[m][r]);
+        }
+        if constexpr (DST == 1) {
+          const size_t e = (size_t)line * N2 + pos;
+          a.Sigma[e * nb2 + c] = y;
+          const int ci = c / nb, cj = c - ci * nb;
+          if (a.herm && ci != cj) a.Sigma[e * nb2 + cj * nb + ci] = conj(y);
+        } else {
+          plane[line * LS + pos * ES] = y;
+        }
+      }
+    }
+  }
+  if constexpr (DST == 0) __syncthreads();
+}
+
+template <typename T, int N1, int N2, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void exch_v2_kernel(ExchArgs<T> a) {
+  constexpr int PITCH = lds_pitch(N2);
+  constexpr int RA2 = v2_row_a(N2), RB2 = N2 / RA2;   // row pass (along k2)
+  constexpr int RA1 = v2_col_a(N1), RB1 = N1 / RA1;   // column pass (along k1)
+  __shared__ cx<T> plane[N1 * PITCH];
+  __shared__ cx<T> tw1[N1], tw2[N2];
+  __shared__ double red[2 * (BLOCK / 64)];
+  const int c = exch_channel<T, BLOCK>(a);
+  for (int m = threadIdx.x; m < N1; m += BLOCK) tw1[m] = a.g.tw1[m];
+  for (int m = threadIdx.x; m < N2; m += BLOCK) tw2[m] = a.g.tw2[m];
+  double mre = 0.0, mim = 0.0;
+  __syncthreads();
+  // forward rows: lines = k1 rows (N1 of them), stride PITCH; positions along k2
+  v2_stage<T, RA2, N2, N1, PITCH, 1, BLOCK, 1, false, 1, 0, false, N2>(plane, tw2, a, c, mre, mim);
+  v2_stage<T, RB2, N2, N1, PITCH, 1, BLOCK, RA2, false, 0, 0, false, N2>(plane, tw2, a, c, mre, mim);
+  // forward columns (+ VR multiply), inverse columns
+  v2_stage<T, RA1, N1, N2, 1, PITCH, BLOCK, 1, false, 0, 0, false, N2>(plane, tw1, a, c, mre, mim);
+  v2_stage<T, RB1, N1, N2, 1, PITCH, BLOCK, RA1, false, 0, 0, true, N2>(plane, tw1, a, c, mre, mim);
+  v2_stage<T, RA1, N1, N2, 1, PITCH, BLOCK, 1, true, 0, 0, false, N2>(plane, tw1, a, c, mre, mim);
+  v2_stage<T, RB1, N1, N2, 1, PITCH, BLOCK, RA1, true, 0, 0, false, N2>(plane, tw1, a, c, mre, mim);
+  // inverse rows, last stage straight to HBM
+  v2_stage<T, RA2, N2, N1, PITCH, 1, BLOCK, 1, true, 0, 0, false, N2>(plane, tw2, a, c, mre, mim);
+  v2_stage<T, RB2, N2, N1, PITCH, 1, BLOCK, RA2, true, 0, 1, false, N2>(plane, tw2, a, c, mre, mim);
+  if (a.rho) {
+    double acc[2] = {mre, mim};
+    block_sum<2>(acc, red);
+    if (threadIdx.x == 0) {
+      const int nb = a.g.nb, ci = c / nb, cj = c - ci * nb;
+      a.rho[c] = cx<double>{acc[0], acc[1]};
+      if (a.herm && ci != cj) a.rho[cj * nb + ci] = cx<double>{acc[0], -acc[1]};
+    }
+  }
+}
+
 template <int N>
 constexpr int ct_block() { return N * N >= 4096 ? 1024 : (N * N >= 1024 ? 512 : 256); }
 
 #define HFX_CT_SIZES(X) X(12) X(16) X(24) X(32) X(36) X(48) X(60) X(64) X(72) X(90) X(96)
 
+// Workgroup sizes of the two-stage kernel, per variant (HFX_EXCH_V2_*).
+template <int N>
+constexpr int v2_block(int variant) {
+  constexpr int big[3] = {1024, 768, 512}, mid[3] = {512, 384, 256}, small[3] = {256, 192, 128};
+  const int i = variant == HFX_EXCH_V2_B ? 1 : variant == HFX_EXCH_V2_C ? 2 : 0;
+  return N * N >= 4096 ? big[i] : (N * N >= 1024 ? mid[i] : small[i]);
+}
+
+// Default variant per mesh (used when the plan's variant is HFX_EXCH_AUTO).
+constexpr int v2_default(int N) { return N == 96 ? HFX_EXCH_V2_B : HFX_EXCH_V2_A; }
+
+template <typename T, int N, int V>
+static int launch_v2(const ExchArgs<T>& a, hipStream_t s) {
+  constexpr int B = v2_block<N>(V);
+  hipLaunchKernelGGL((exch_v2_kernel<T, N, N, B>), dim3(a.nchan), dim3(B), 0, s, a);
+  return check_launch("exch_v2_kernel");
+}
+
 template <typename T>
-static int launch_exch(const ExchArgs<T>& a, hipStream_t s) {
+static int launch_exch(const ExchArgs<T>& a, int variant, hipStream_t s) {
   constexpr int MAXE = sizeof(T) == 8 ? MAXE_F64 : MAXE_F32;
   const int n1 = a.g.n1, n2 = a.g.n2;
   if (n1 == n2) {
@@ -398,6 +590,12 @@ static int launch_exch(const ExchArgs<T>& a, hipStream_t s) {
 #define HFX_CASE(N)                                                                        \
   case N:                                                                                  \
     if ((size_t)N * lds_pitch(N) * sizeof(cx<T>) <= (size_t)MAXE * sizeof(cx<T>)) {        \
+      const int v = variant == HFX_EXCH_AUTO ? v2_default(N) : variant;                    \
+      if (a.mode == 0 && a.shift == 0 && v != HFX_EXCH_V1) {                               \
+        if (v == HFX_EXCH_V2_B) return launch_v2<T, N, HFX_EXCH_V2_B>(a, s);                \
+        if (v == HFX_EXCH_V2_C) return launch_v2<T, N, HFX_EXCH_V2_C>(a, s);                \
+        return launch_v2<T, N, HFX_EXCH_V2_A>(a, s);                                       \
+      }                                                                                    \
       hipLaunchKernelGGL((exch_ct_kernel<T, N, N, ct_block<N>()>), dim3(a.nchan),           \
                          dim3(ct_block<N>()), 0, s, a);                                    \
       return check_launch("exch_ct_kernel");                                               \
@@ -427,13 +625,13 @@ int exchange_launch(const hfx_plan* plan, const void* P, const void* refP, const
   if (plan->dtype == HFX_DTYPE_C128) {
     ExchArgs<double> a{(const cx<double>*)P, (const cx<double>*)refP, (const cx<double>*)VR, chans, w2d,
                        (cx<double>*)Sigma, (cx<double>*)rho, nchan, herm, vfull,
-                       (nchan % 8 == 0) ? 1 : 0, mode, shift, conjv, plan->g64};
-    return launch_exch<double>(a, s);
+                       (nchan % 8 == 0) ? 1 : 0, /*mode=*/mode, /*shift=*/shift, /*conjv=*/conjv, plan->g64};
+    return launch_exch<double>(a, plan->exch_variant, s);
   }
   ExchArgs<float> a{(const cx<float>*)P, (const cx<float>*)refP, (const cx<float>*)VR, chans, w2d,
                     (cx<float>*)Sigma, (cx<double>*)rho, nchan, herm, vfull,
-                    (nchan % 8 == 0) ? 1 : 0, mode, shift, conjv, plan->g32};
-  return launch_exch<float>(a, s);
+                    (nchan % 8 == 0) ? 1 : 0, /*mode=*/mode, /*shift=*/shift, /*conjv=*/conjv, plan->g32};
+  return launch_exch<float>(a, plan->exch_variant, s);
 }
 
 int exchange_max_plane(int dtype) { return dtype == HFX_DTYPE_C128 ? MAXE_F64 : MAXE_F32; }

@@ -23,9 +23,11 @@ struct FftGeom {
 
 }  // namespace hfx
 
-// Immutable after creation; device twiddle tables owned by the plan.
+// Device twiddle tables owned by the plan; only exch_variant changes after
+// creation (hfx_plan_set_exchange_variant / hfx_exchange_autotune).
 struct hfx_plan {
   int n1, n2, nb, dtype;
+  int exch_variant;  // HFX_EXCH_*
   void* tw_dev;  // one allocation: [tw1 | tw2 | channel lists]
   const int* chans_full;  // c = 0 .. nb*nb-1
   const int* chans_triu;  // i * nb + j for i <= j

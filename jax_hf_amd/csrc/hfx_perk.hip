@@ -104,19 +104,26 @@ __device__ void m_eigh(cx<T>* A, cx<T>* V, cx<T>* Vt, double* lam, int nb, doubl
   __syncthreads();
   const int m = nb + (nb & 1);    // even round-robin size (dummy index = nb when odd)
   cx<T> newA[PK_EPL], newV[PK_EPL];
-  for (int sweep = 0; sweep < 30 && nb > 1; ++sweep) {
-    // convergence: off-diagonal Frobenius norm vs total
-    T off = 0, tot = 0;
+  __shared__ int nrot;
+  // Threshold Jacobi: a pair (p, q) is rotated only while
+  //   |a_pq| > eps * max(sqrt(|a_pp a_qq|), 0.01 * ||A||_F / sqrt(nb)),
+  // otherwise a_pq is set to zero (a perturbation below the rounding of the
+  // diagonal); the sweeps stop after the first sweep with no rotation.  The
+  // relative part keeps small eigenvalues accurate, the absolute floor stops
+  // rotations that would only stir rounding noise.
+  T floor_abs;
+  {
+    T tot = 0;
     PK_FOR_ELEM(nb, {
       const cx<T> a = A[i * PK_PITCH + j];
-      const T x = a.re * a.re + a.im * a.im;
-      tot += x;
-      if (i != j) off += x;
+      tot += a.re * a.re + a.im * a.im;
     })
-    const double off_s = wave_sum((double)off), tot_s = wave_sum((double)tot);
-    // FP64: (eps/2)^2 relative; FP32 stalls near its own rounding, so (2 eps)^2
-    const double thr = sizeof(T) == 8 ? 0.25 : 4.0;
-    if (off_s <= (double)eps * (double)eps * thr * tot_s || off_s == 0.0) break;
+    const double tot_s = wave_sum((double)tot);
+    floor_abs = (T)(0.01 * sqrt(tot_s / (double)nb));
+  }
+  for (int sweep = 0; sweep < 30 && nb > 1; ++sweep) {
+    if (threadIdx.x == 0) nrot = 0;
+    __syncthreads();
     for (int step = 0; step < m - 1; ++step) {
       // pair table for this step: index m-1 meets `step`; others (step+k, step-k) mod (m-1)
       if ((int)threadIdx.x < m / 2) {
@@ -131,7 +138,8 @@ __device__ void m_eigh(cx<T>* A, cx<T>* V, cx<T>* Vt, double* lam, int nb, doubl
           const cx<T> c = A[p * PK_PITCH + q];
           const T ac = sqrt(c.re * c.re + c.im * c.im);
           const T app = A[p * PK_PITCH + p].re, aqq = A[q * PK_PITCH + q].re;
-          if (ac > (T)0) {
+          if (ac > eps * fmax(sqrt(fabs(app * aqq)), floor_abs)) {
+            nrot = 1;
             const T zeta = (aqq - app) / ((T)2 * ac);
             const T t = (zeta >= (T)0 ? (T)1 : (T)-1) / (fabs(zeta) + sqrt((T)1 + zeta * zeta));
             const T cs = (T)1 / sqrt((T)1 + t * t), sn = t * cs;
@@ -189,6 +197,7 @@ __device__ void m_eigh(cx<T>* A, cx<T>* V, cx<T>* Vt, double* lam, int nb, doubl
       })
       __syncthreads();
     }
+    if (nrot == 0) break;
   }
   // ascending order: rank_i = #{j : l_j < l_i or (l_j == l_i and j < i)}
   __shared__ int rank[PK_NBMAX];
@@ -476,7 +485,26 @@ __global__ __launch_bounds__(64) void pk_cg_retract_kernel(int nb, cx<T>* __rest
   __syncthreads();
   m_gemm<T, false, true>(sU, sX, sV, nb);     // U
   m_gemm<T, false, false>(sX, sQ, sU, nb);    // Q U
-  m_store(Q + k * nb2, sX, nb);
+  if constexpr (sizeof(T) == 4) {
+    // FP32: V (and so U) is unitary only to ~1e-7, and the error would build up
+    // over the iterations (the density then holds more or fewer than n electrons
+    // and the energy drifts).  One Newton-Schulz step Q <- Q (3 I - Q^dagger Q) / 2
+    // squares the deviation from unitarity, holding Q at FP32 rounding.
+    m_gemm<T, true, false>(sQ, sX, sX, nb);   // S = X^dagger X  (sQ is free here)
+    PK_FOR_ELEM(nb, {
+      const cx<T> v = sQ[i * PK_PITCH + j];
+      sQ[i * PK_PITCH + j] = cx<T>{(i == j ? (T)1.5 : (T)0) - (T)0.5 * v.re, (T)-0.5 * v.im};
+    })
+    __syncthreads();
+    PK_FOR_ELEM(nb, {   // Q <- X M, straight to global (sU, sV keep U and V)
+      cx<T> acc{0, 0};
+      for (int l = 0; l < nb; ++l) acc = acc + sX[i * PK_PITCH + l] * sQ[l * PK_PITCH + j];
+      Q[k * nb2 + e] = acc;
+    })
+    __syncthreads();
+  } else {
+    m_store(Q + k * nb2, sX, nb);
+  }
   if (eps_new) {
     m_load(sQ, Fe + k * nb2, nb);             // reuse sQ for Ft_eig
     __syncthreads();

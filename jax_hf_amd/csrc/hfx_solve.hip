@@ -138,30 +138,41 @@ struct MuArgs {
 
 __device__ __forceinline__ double sigm(double x) { return 1.0 / (1.0 + exp(-x)); }
 
-// grid-wide deterministic sum of two values; all threads receive the totals
-__device__ void grid_sum2(cg::grid_group& grid, double a, double b, double* scratch, int slot, double* sh,
-                          double& ta, double& tb) {
-  double acc[2] = {a, b};
-  block_sum<2>(acc, sh);
-  if (threadIdx.x == 0) {
-    scratch[(size_t)slot * 2 * gridDim.x + 2 * blockIdx.x] = acc[0];
-    scratch[(size_t)slot * 2 * gridDim.x + 2 * blockIdx.x + 1] = acc[1];
+// Block-wide sum of the first n entries of a global array (strided per
+// thread, then a block tree): every block of the grid evaluates the same
+// expression tree over the same data, so all blocks get bitwise-identical totals.
+__device__ __forceinline__ void block_gather2(const double* src, int n, double* sh, double& ta, double& tb) {
+  double acc[2] = {0.0, 0.0};
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    acc[0] += src[2 * q];
+    acc[1] += src[2 * q + 1];
   }
-  grid.sync();
+  block_sum<2>(acc, sh);
   __shared__ double tot[2];
   if (threadIdx.x == 0) {
-    double x = 0.0, y = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b) {
-      x += scratch[(size_t)slot * 2 * gridDim.x + 2 * b];
-      y += scratch[(size_t)slot * 2 * gridDim.x + 2 * b + 1];
-    }
-    tot[0] = x;
-    tot[1] = y;
+    tot[0] = acc[0];
+    tot[1] = acc[1];
   }
   __syncthreads();
   ta = tot[0];
   tb = tot[1];
   __syncthreads();
+}
+
+// grid-wide deterministic sum of two values; all threads receive the totals.
+// `slot` alternates between calls so a block that runs ahead never overwrites
+// partials another block is still reading.
+__device__ void grid_sum2(cg::grid_group& grid, double a, double b, double* scratch, int slot, double* sh,
+                          double& ta, double& tb) {
+  double acc[2] = {a, b};
+  block_sum<2>(acc, sh);
+  double* part = scratch + (size_t)slot * 2 * gridDim.x;
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = acc[0];
+    part[2 * blockIdx.x + 1] = acc[1];
+  }
+  grid.sync();
+  block_gather2(part, (int)gridDim.x, sh, ta, tb);
 }
 
 __device__ void grid_minmax(cg::grid_group& grid, const double* v, size_t n, double* scratch, double* sh,
@@ -183,18 +194,22 @@ __device__ void grid_minmax(cg::grid_group& grid, const double* v, size_t n, dou
     scratch[2 * blockIdx.x + 1] = hi;
   }
   grid.sync();
-  __shared__ double tot[2];
-  if (threadIdx.x == 0) {
-    double a = INFINITY, b = -INFINITY;
-    for (unsigned q = 0; q < gridDim.x; ++q) { a = fmin(a, scratch[2 * q]); b = fmax(b, scratch[2 * q + 1]); }
-    tot[0] = a;
-    tot[1] = b;
+  double a = INFINITY, b = -INFINITY;
+  for (unsigned q = threadIdx.x; q < gridDim.x; q += blockDim.x) {
+    a = fmin(a, scratch[2 * q]);
+    b = fmax(b, scratch[2 * q + 1]);
   }
+  a = wave_min(a);
+  b = wave_max(b);
   __syncthreads();
-  mn = tot[0];
-  mx = tot[1];
+  if (lane == 0) { sh[wid] = a; sh[nw + wid] = b; }
   __syncthreads();
-  grid.sync();   // scratch is reused right after
+  a = INFINITY;
+  b = -INFINITY;
+  for (int w = 0; w < nw; ++w) { a = fmin(a, sh[w]); b = fmax(b, sh[nw + w]); }
+  mn = a;
+  mx = b;
+  __syncthreads();
 }
 
 // value of element e after the optional level shift (reference_scf.py:91-92)
@@ -215,7 +230,7 @@ __device__ double bisect(cg::grid_group& grid, const MuArgs& a, double mu_raw, b
       lo = fmin(lo, v);
       hi = fmax(hi, v);
     }
-    // reuse grid_minmax machinery on the fly
+    // min / max via the (min, -max) pair of a grid "sum" with min/max combine
     lo = wave_min(lo);
     hi = wave_max(hi);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -227,23 +242,25 @@ __device__ double bisect(cg::grid_group& grid, const MuArgs& a, double mu_raw, b
       a.scratch[2 * blockIdx.x + 1] = hi;
     }
     grid.sync();
-    __shared__ double tot[2];
-    if (threadIdx.x == 0) {
-      double p = INFINITY, q = -INFINITY;
-      for (unsigned b = 0; b < gridDim.x; ++b) { p = fmin(p, a.scratch[2 * b]); q = fmax(q, a.scratch[2 * b + 1]); }
-      tot[0] = p;
-      tot[1] = q;
+    double p = INFINITY, q = -INFINITY;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+      p = fmin(p, a.scratch[2 * b]);
+      q = fmax(q, a.scratch[2 * b + 1]);
     }
+    p = wave_min(p);
+    q = wave_max(q);
     __syncthreads();
-    mn = tot[0];
-    mx = tot[1];
+    if (lane == 0) { sh[wid] = p; sh[nw + wid] = q; }
     __syncthreads();
-    grid.sync();
+    for (int w = 0; w < nw; ++w) { mn = fmin(mn, sh[w]); mx = fmax(mx, sh[nw + w]); }
+    __syncthreads();
+    grid.sync();   // slot 0 of the scratch is reused right after
   }
   const double span = mx - mn + 10.0 * fmax(a.T, 1e-6);
   double lo = mn - span, hi = mx + span;
   for (int it = 0; it < a.maxiter; ++it) {
     const double mid = 0.5 * (lo + hi);
+    if (mid == lo || mid == hi) break;   // bracket collapsed: later iterations change nothing
     double c = 0.0;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
       const double v = shifted(a, e, mu_raw, use_shift);
@@ -282,6 +299,9 @@ __global__ __launch_bounds__(MU_BLOCK) void mu_kernel(MuArgs a) {
       double N, Z;
       grid_sum2(grid, Np, Zp, a.scratch, 1 + (it & 1), sh, N, Z);
       const double g = N - a.target;
+      // converged: the Newton step no longer moves mu (every block takes this
+      // branch together, the totals being bitwise identical across blocks)
+      if (mu - g / fmax(Z, 1e-18) == mu) break;
       if (g < 0) lo = mu;
       if (g > 0) hi = mu;
       double mu_new = mu - g / fmax(Z, 1e-18);
@@ -310,6 +330,9 @@ __global__ __launch_bounds__(MU_BLOCK) void mu_kernel(MuArgs a) {
       grid_sum2(grid, Np, Zp, a.scratch, 1 + (it & 1), sh, N, Z);
       if (it == a.maxiter) break;   // final residual check below
       const double g = N - a.target;
+      // converged: the Newton step no longer moves mu (every block takes this
+      // branch together, the totals being bitwise identical across blocks)
+      if (mu - g / fmax(Z, 1e-18) == mu) break;
       if (g < 0) lo = mu;
       if (g > 0) hi = mu;
       double mu_new = mu - g / fmax(Z, 1e-18);
@@ -343,17 +366,22 @@ __global__ __launch_bounds__(MU_BLOCK) void mu_kernel(MuArgs a) {
 
 static int g_mu_blocks = 0;
 
-int mu_blocks() {
-  if (g_mu_blocks) return g_mu_blocks;
-  int dev = 0, ncu = 0, occ = 0;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mu_kernel, MU_BLOCK, 0);
-  int b = ncu * (occ > 0 ? (occ < 2 ? occ : 2) : 1);
-  if (b > 512) b = 512;
-  if (b < 1) b = 1;
-  g_mu_blocks = b;
-  return b;
+// Co-resident grid for the cooperative launch, sized to ~8 eigenvalues per
+// thread: fewer blocks make each grid barrier cheaper, and the per-iteration
+// work (one exp per eigenvalue) is small.
+int mu_blocks(size_t n) {
+  if (!g_mu_blocks) {
+    int dev = 0, ncu = 0, occ = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mu_kernel, MU_BLOCK, 0);
+    int b = ncu * (occ > 0 ? (occ < 2 ? occ : 2) : 1);
+    if (b > 512) b = 512;
+    g_mu_blocks = b < 1 ? 1 : b;
+  }
+  size_t want = (n + (size_t)MU_BLOCK * 8 - 1) / ((size_t)MU_BLOCK * 8);
+  if (want < 1) want = 1;
+  return want < (size_t)g_mu_blocks ? (int)want : g_mu_blocks;
 }
 
 }  // namespace hfx
@@ -400,7 +428,7 @@ int hfx_solve_mu(const double* eps, const double* w, size_t nk, int nb, double t
     return set_error(HFX_EINVAL, "hfx_solve_mu: unknown mode %d", mode);
   if (mu0_from_state && !st) return set_error(HFX_EINVAL, "hfx_solve_mu: mu0_from_state needs st");
   MuArgs a{eps, w, nk, nb, target, T, mu0, level_shift, maxiter, mode, mu0_from_state, st, mu_out, p_out, scratch};
-  const int blocks = mu_blocks();
+  const int blocks = mu_blocks(nk * (size_t)nb);
   void* args[] = {&a};
   hipError_t e = hipLaunchCooperativeKernel((const void*)mu_kernel, dim3(blocks), dim3(MU_BLOCK), args, 0,
                                             (hipStream_t)stream);

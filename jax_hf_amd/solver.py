@@ -274,7 +274,8 @@ def _solve_cg(kernel, P0, n_electrons, cfg: SolverConfig, fock_build_fn):
         lo = _CG(kernel, torch.complex64, cfg, fock_build_fn)
         lo.set_target(n_electrons)
         lo.init_from(P0)
-        k1, g1, hE1, hG1 = lo.run(n_electrons, cfg.mixed_tol_E, 0.0, cfg.max_iter, rel_tol=True)
+        # the FP32 phase gets at most half of the iteration budget
+        k1, g1, hE1, hG1 = lo.run(n_electrons, cfg.mixed_tol_E, 0.0, max(cfg.max_iter // 2, 1), rel_tol=True)
         # FP64 refinement restarts from the mixed density: eigh of the FP64 Fock
         # matrix gives exactly unitary orbitals (an FP32 Q is unitary only to ~1e-7)
         P_lo = lo.density()
@@ -294,7 +295,10 @@ def _solve_cg(kernel, P0, n_electrons, cfg: SolverConfig, fock_build_fn):
     Pfin, Ffin = runner.finalize()
     converged = (grad <= cfg.tol_grad) if cfg.tol_grad > 0.0 else (k < cfg.max_iter)
     sv = runner.st.cpu()
-    return _result(runner, Pfin, Ffin, sv, k, converged, hist_E, hist_G, cfg.max_iter, dev)
+    res = _result(runner, Pfin, Ffin, sv, k, converged, hist_E, hist_G, cfg.max_iter, dev)
+    if prec == "mixed":
+        res.history["phase_iters"] = torch.tensor([k1, k2], dtype=torch.int64)
+    return res
 
 
 def _result(runner, Pfin, Ffin, sv, k, converged, hist_E, hist_G, max_iter, dev):

@@ -121,7 +121,7 @@ def test_stationary_parameter_gradients_vs_finite_differences(dev):
     def omega(U, q0_, uu):
         G = J.HartreeFockKernel(w, h0 + uu * D, Vq(U, q0_), T)
         r = J.solve(G, np.zeros_like(h0), nb / 2.0,
-                    config=J.SolverConfig(optimizer="newton", tol_grad=1e-11, max_iter=200))
+                    config=J.SolverConfig(optimizer="newton", tol_grad=1e-9, max_iter=200))
         assert r.converged
         return float(J.free_energy(r.energy, r.p, w, T)), G, r
 

@@ -181,3 +181,52 @@ def test_entropy_and_free_energy(dev):
     S0 = O.occupation_entropy(p, w)
     assert abs(float(J.occupation_entropy(p, w)) - S0) < 1e-13 * max(1, abs(S0))
     assert abs(float(J.free_energy(-1.25, p, w, 0.3)) - O.free_energy(-1.25, p, w, 0.3)) < 1e-13
+
+
+@pytest.mark.parametrize("cdt", ["c128", "c64"])
+@pytest.mark.parametrize("nk", [12, 16, 24, 32, 36, 48, 60, 64, 72, 90, 96])
+def test_exchange_variants_match_oracle(dev, nk, cdt):
+    """Every exchange-kernel variant (one-stage-per-radix, two-stage at three
+    workgroup sizes) gives the oracle's Sigma and channel moments, with refP,
+    full and Hermitian-packed channel lists, and the conjugated-VR adjoint."""
+    import torch
+    import jax_hf_amd as J
+    from jax_hf_amd import _lib as L
+    nb = 2
+    rng = np.random.default_rng(1000 + nk)
+    h = syn.tight_binding(nk, nk, nb, seed=nk)
+    V = syn.coulomb_layer(nk, nk, nb, 2)
+    w = rng.uniform(0.5, 1.5, size=(nk, nk)) / nk ** 2
+    P = syn.haar_density(nk, nk, nb, rng)
+    R = syn.haar_density(nk, nk, nb, rng)
+    K = O.OracleKernel(w, h, V, 0.1)
+    S_ref = O.selfenergy_fft(K._VR_shifted, P - R, apply_ifftshift=False)
+    A_ref = O.selfenergy_fft(np.conj(K._VR_shifted), P, apply_ifftshift=False)
+    rho_ref = np.einsum("ab,abij->ij", w, P - R)
+    npd = np.complex128 if cdt == "c128" else np.complex64
+    tol = 1e-12 if cdt == "c128" else 2e-6
+    G = J.HartreeFockKernel(w, h.astype(npd), V, 0.1)
+    eng = G.fock_engine(herm=False)
+    Pt = torch.from_numpy(P.astype(npd)).to(G.device)
+    Rt = torch.from_numpy(R.astype(npd)).to(G.device)
+    plan = eng.plan
+    try:
+        for v in (1, 2, 3, 4):
+            plan.exchange_variant = v
+            for herm in (False, True):
+                S = torch.full_like(Pt, float("nan"))
+                eng.rho.zero_()
+                eng.selfenergy(Pt, S, refP=Rt, herm=herm, rho=True)
+                assert _rel(_np(S), S_ref) < tol, (v, herm)
+                assert np.abs(_np(eng.rho) - rho_ref).max() < 1e-12 * max(1.0, np.abs(rho_ref).max()) * (
+                    1 if cdt == "c128" else 1e6), (v, herm)
+            A = torch.full_like(Pt, float("nan"))
+            L.call("hfx_selfenergy_adjoint", plan.handle, L.ptr(Pt), L.ptr(eng.VR), 1, L.ptr(A),
+                   L.stream(G.device))
+            assert _rel(_np(A), A_ref) < tol, v
+        S = torch.empty_like(Pt)
+        best, times = plan.autotune(Pt, eng.VR, S, reps=2)
+        assert best in (1, 2, 3, 4) and plan.exchange_variant == best
+        assert all(t > 0 for t in times.values()), times
+    finally:
+        plan.exchange_variant = 0
