@@ -1,20 +1,28 @@
 #!/usr/bin/env python
-"""Benchmark: isolated build_fock + hf_energy at nk1 = nk2 = 96, nb = 16.
+"""Benchmark: converged HF solves/s and Fock-build GB/s at nk1 = nk2 = 96, nb = 16.
 
-Workload (one "step"): one Fock build in the reference's API semantics on
-seeded synthetic inputs resident in HBM -- layer-resolved real V (layer =
-band mod 2, U0 = 5, q0 = 0.3, d = 0.5), Haar-random Hermitian density, all
-256 (i, j) exchange channels, outputs Sigma, F and the HF energy:
+Inputs (seeded, synthetic, resident in HBM before timing): the tight-binding
+h(k) of the triangular lattice at 16x16, layer-resolved real V_ll'(q)
+(layer = band mod 2, U0 = 5, q0 = 0.3, d = 0.5), uniform weights.
 
-    exchange kernel   P, VR -> Sigma          (fused 2D FFT convolution)
-    epilogue kernel   h, Sigma, P -> F, E     (herm + energy trace, FP64 sums)
+``value`` -- converged solves/s of the mixed-precision form: FP32 CG
+direct minimisation (complex64 FFT exchange and per-k algebra, FP64 mu /
+energy / entropy sums) from P0 = 0, T = 0.1, half filling, then an FP64 DIIS
+SCF polish to the fixed point (residuals <= 1e-8).  One "step" is one such
+solve.  Its energy is compared with the FP64 fixed point (FP64 DIIS SCF to
+1e-10 residuals) and reported beside the FP64 forms: the reference-semantics
+FP64 CG (windowed tol_E = 1e-9 stop) and FP64 CG + the same polish.
 
-``value`` = Fock builds per second on one GPU.  ``roofline`` is for the
-exchange kernel (the dominant one): algorithmic bytes per launch
-c * Nk * (2 nb^2 + nb^2) (P and VR read, Sigma written) over its average
-HIP-event duration inside the timed region.  ``cpu_baseline`` times the CPU
-oracle (NumPy/SciPy FP64 restatement of the reference algorithm) on the same
-inputs for a bounded number of builds.
+``fock`` / ``roofline`` -- isolated build_fock + hf_energy in the API
+semantics (all 256 channels, Sigma + F + E) on a Haar-random density: Fock
+builds/s, and for the dominant kernel (the fused FFT exchange) the
+algorithmic bytes per launch c * Nk * 3 nb^2 (P and VR read, Sigma written)
+over its mean HIP-event duration on the launch stream.
+
+``cpu_baseline`` -- the CPU oracle (NumPy/SciPy FP64 restatement of the
+reference algorithm) on the host cores: a bounded number of its CG
+iterations on the same inputs, scaled to solves/s with the FP64 GPU form's
+iteration count; its Fock builds/s is reported beside it.
 """
 
 from __future__ import annotations
@@ -35,31 +43,161 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=5, help="timed mixed-precision solves")
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nk", type=int, default=96)
     ap.add_argument("--nb", type=int, default=16)
-    ap.add_argument("--dtype", choices=["c128", "c64"], default="c128")
+    ap.add_argument("--fock-steps", type=int, default=200, help="timed Fock builds for the GB/s figure")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the built-in exchange-kernel variant instead of timing all of them first")
-    ap.add_argument("--extra", action="store_true", help="also time the packed and complex64 forms")
-    ap.add_argument("--solve", action="store_true",
-                    help="also time converged CG solves (FP64 and mixed) at the same mesh")
-    ap.add_argument("--solve-tol", type=float, default=1e-9)
+    ap.add_argument("--fock-only", action="store_true", help="only the Fock-build measurement")
+    ap.add_argument("--extra", action="store_true", help="also time the packed and complex64 Fock builds")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of the exchange kernel's byte count "
                          "(hfx_lincomb out = X), to scale FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
 
 
-def cpu_baseline(inp, budget_s):
-    """Oracle build_fock + hf_energy on the host cores, bounded to ~budget_s."""
+# ---------------------------------------------------------------------------
+# Fock build
+# ---------------------------------------------------------------------------
+
+def fock_bench(J, L, syn, inp, dev, nk, nb, args):
+    import torch
+    G = J.HartreeFockKernel(inp["weights"], inp["hamiltonian"], inp["coulomb_q"], inp["T"], device=dev)
+    eng = G.fock_engine(herm=False)        # reference semantics: all nb^2 channels
+    P = torch.from_numpy(np.ascontiguousarray(inp["P"])).to(dev)
+    F = torch.empty_like(P)
+    Sig = torch.empty_like(P)
+    E = eng.E
+    st = L.stream(dev)
+    variant_ms = None
+    if not args.no_autotune:
+        _, variant_ms = eng.plan.autotune(P, eng.VR, Sig, reps=10)
+    kname = {1: "exch_ct_kernel"}.get(eng.plan.exchange_variant, "exch_v2_kernel")
+    Nk = nk * nk
+    bytes_exch = 16 * Nk * (3 * nb * nb)               # P + VR in, Sigma out
+    bytes_build = 16 * Nk * (4 * nb * nb + nb * nb)    # + h in, F out
+
+    def step(evs=None):
+        if evs is not None:
+            evs[0].record()
+        eng.selfenergy(P, Sig, refP=None, rho=False)
+        if evs is not None:
+            evs[1].record()
+        L.call("hfx_fock_epilogue", eng.plan.handle, L.ptr(eng.h), L.ptr(Sig), None, None, L.ptr(P),
+               L.ptr(eng.w2d), L.ptr(F), None, L.ptr(eng.plan.scratch), L.ptr(E), 0.0, st)
+
+    for _ in range(20):
+        step()
+    torch.cuda.synchronize()
+    n = args.fock_steps
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(n):
+        step(evs[i])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    exch_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    achieved = bytes_exch / (exch_ms * 1e-3) / 1e9
+    out = {"builds_per_s": n / dt, "ms_per_build": dt * 1e3 / n,
+           "GBps": bytes_build / (dt / n) / 1e9, "algorithmic_bytes_build": bytes_build,
+           "energy": float(E), "exchange_variant": L.Plan.VARIANTS[eng.plan.exchange_variant],
+           "exchange_variants_ms": variant_ms}
+    roof = {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": bytes_exch, "avg_ms": exch_ms}
+    return out, roof, F
+
+
+def extra_forms(J, L, syn, inp, dev, nk, nb, args):
+    """Packed (upper-triangle) and complex64 Fock builds, wall time per build."""
+    import torch
+    res = {}
+    for name, npd, herm in (("c128_packed", np.complex128, True), ("c64_full", np.complex64, False),
+                            ("c64_packed", np.complex64, True)):
+        G = J.HartreeFockKernel(inp["weights"], inp["hamiltonian"].astype(npd), inp["coulomb_q"],
+                                inp["T"], device=dev)
+        eng = G.fock_engine(herm=herm)
+        P = torch.from_numpy(np.ascontiguousarray(inp["P"], dtype=npd)).to(dev)
+        F = torch.empty_like(P)
+        for _ in range(10):
+            eng.build(P, F, energy=True, refP=None)
+        torch.cuda.synchronize()
+        n = max(20, args.fock_steps // 2)
+        t0 = time.perf_counter()
+        for _ in range(n):
+            eng.build(P, F, energy=True, refP=None)
+        torch.cuda.synchronize()
+        res[name] = {"ms_per_build": (time.perf_counter() - t0) * 1e3 / n}
+    return res
+
+
+# ---------------------------------------------------------------------------
+# solves
+# ---------------------------------------------------------------------------
+
+SOLVE_T = 0.1
+
+
+def _timed_solves(J, G, P0, ne, cfg, n):
+    import torch
+    times, r = [], None
+    for _ in range(n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = J.solve(G, P0, ne, config=cfg)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    return r, times
+
+
+def solve_bench(J, inp, dev, nk, nb, args):
+    G = J.HartreeFockKernel(inp["weights"], inp["hamiltonian"], inp["coulomb_q"], SOLVE_T, device=dev)
+    P0 = np.zeros_like(inp["hamiltonian"])
+    ne = nb / 2.0
+    # FP64 fixed point (the accuracy reference): DIIS SCF to 1e-10 residuals
+    ref = J.solve_scf(G, P0, ne, config=J.SCFConfig(max_iter=200, acceleration="diis",
+                                                    density_tol=1e-10, comm_tol=1e-10))
+    E_fp = float(ref.energy)
+    forms = {
+        "fp64_cg": J.SolverConfig(max_iter=500, tol_E=1e-9, precision="fp64"),
+        "fp64_polished": J.SolverConfig(max_iter=500, tol_E=1e-9, precision="fp64", polish="diis",
+                                        polish_tol=1e-8),
+        "mixed": J.SolverConfig(max_iter=500, precision="mixed", polish_tol=1e-8),
+    }
+    res = {"fixed_point": {"energy": E_fp, "scf_iterations": int(ref.iterations),
+                           "converged": bool(ref.converged)}}
+    for name, cfg in forms.items():
+        _timed_solves(J, G, P0, ne, cfg, args.warmup)                      # allocation / first use
+        n = args.steps if name == "mixed" else max(2, min(args.steps, 3))
+        r, times = _timed_solves(J, G, P0, ne, cfg, n)
+        E = float(r.energy)
+        ent = {"seconds_median": float(np.median(times)), "seconds": [float(t) for t in times],
+               "solves_per_s": 1.0 / float(np.median(times)), "n_iter": int(r.n_iter),
+               "converged": bool(r.converged), "energy": E, "abs_dE_vs_fp64_fixed_point": abs(E - E_fp)}
+        if "phase_iters" in r.history:
+            ent["phase_iters"] = [int(x) for x in r.history["phase_iters"]]
+        res[name] = ent
+    res["speedup_mixed_vs_fp64_polished"] = (res["fp64_polished"]["seconds_median"]
+                                            / res["mixed"]["seconds_median"])
+    res["speedup_mixed_vs_fp64_cg"] = res["fp64_cg"]["seconds_median"] / res["mixed"]["seconds_median"]
+    return res
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline (oracle on the host cores)
+# ---------------------------------------------------------------------------
+
+def cpu_baseline(inp, budget_s, n_iter_solve):
     import oracle as O
     from oracle import hf_oracle as OH
     ncores = len(os.sched_getaffinity(0))
     OH.FFT_WORKERS = ncores
+    nk, nb = inp["P"].shape[0], inp["P"].shape[-1]
+    # Fock builds on the Haar density
     K = O.OracleKernel(inp["weights"], inp["hamiltonian"], inp["coulomb_q"], inp["T"])
     a = K.as_args()
     P = inp["P"]
@@ -72,17 +210,31 @@ def cpu_baseline(inp, budget_s):
         return O.hf_energy(P, h=a["h"], Sigma=S, H=H, weights_b=K.weights_b), F
 
     one()
-    n = 0
-    t0 = time.perf_counter()
+    n, t0 = 0, time.perf_counter()
     while True:
         E, F = one()
         n += 1
-        if time.perf_counter() - t0 >= budget_s:
+        if time.perf_counter() - t0 >= 0.4 * budget_s:
             break
-    dt = time.perf_counter() - t0
-    return dict(value=n / dt, unit="Fock builds/s", cores=ncores, kind="port",
-                sample=f"{n} FP64 build_fock+hf_energy calls of the NumPy/SciPy oracle at "
-                       f"{P.shape[0]}x{P.shape[1]}, nb={P.shape[-1]} (scipy.fft workers={ncores})"), E, F
+    fock_rate = n / (time.perf_counter() - t0)
+    # CG iterations of the oracle solve from P0 = 0: the cost of one more
+    # iteration is the difference of a 2-iteration and a 1-iteration solve
+    Ks = O.OracleKernel(inp["weights"], inp["hamiltonian"], inp["coulomb_q"], SOLVE_T)
+    P0 = np.zeros_like(inp["hamiltonian"])
+    t0 = time.perf_counter()
+    O.solve(Ks, P0, nb / 2.0, config=O.SolverConfig(max_iter=1, tol_E=0.0))
+    t1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.solve(Ks, P0, nb / 2.0, config=O.SolverConfig(max_iter=2, tol_E=0.0))
+    t2 = time.perf_counter() - t0
+    t_iter = max(t2 - t1, 1e-9)
+    solve_rate = 1.0 / (t_iter * max(n_iter_solve, 1))
+    return {"value": solve_rate, "unit": "solves/s", "cores": ncores, "kind": "port",
+            "sample": (f"oracle (NumPy/SciPy) CG solves of 1 and 2 iterations at {nk}x{nk}, nb={nb}: "
+                       f"{t_iter:.2f} s per iteration (scipy.fft workers={ncores}), scaled by the FP64 GPU "
+                       f"form's {n_iter_solve} iterations per solve; plus {n} oracle build_fock+hf_energy "
+                       f"calls ({fock_rate:.2f} Fock builds/s)"),
+            "fock_builds_per_s": fock_rate}, E, F
 
 
 def main():
@@ -107,142 +259,38 @@ def main():
         print(json.dumps({"copy_calibration_bytes_read": n * 16, "copy_calibration_bytes_written": n * 16}))
         return
     inp = syn.config1(nk=nk, nb=nb, seed=1)
-    cdt = torch.complex128 if args.dtype == "c128" else torch.complex64
-    npd = np.complex128 if args.dtype == "c128" else np.complex64
-    G = J.HartreeFockKernel(inp["weights"], inp["hamiltonian"].astype(npd), inp["coulomb_q"], inp["T"],
-                            device=dev)
-    eng = G.fock_engine(herm=False)        # reference semantics: all nb^2 channels
-    P = torch.from_numpy(np.ascontiguousarray(inp["P"], dtype=npd)).to(dev)
-    F = torch.empty_like(P)
-    Sig = torch.empty_like(P)
-    E = eng.E
-    st = L.stream(dev)
-    csz = 16 if args.dtype == "c128" else 8
-    Nk = nk * nk
-    bytes_exch = csz * Nk * (3 * nb * nb)               # P + VR in, Sigma out
-    bytes_build = csz * Nk * (4 * nb * nb + nb * nb)    # + h in, F out
-
-    variant_ms = None
-    if not args.no_autotune:
-        best, variant_ms = eng.plan.autotune(P, eng.VR, Sig, reps=10)
-    kname = {1: "exch_ct_kernel"}.get(eng.plan.exchange_variant, "exch_v2_kernel")
-
-    def step(evs=None):
-        if evs is not None:
-            evs[0].record()
-        eng.selfenergy(P, Sig, refP=None, rho=False)
-        if evs is not None:
-            evs[1].record()
-        L.call("hfx_fock_epilogue", eng.plan.handle, L.ptr(eng.h), L.ptr(Sig), None, None, L.ptr(P),
-               L.ptr(eng.w2d), L.ptr(F), None, L.ptr(eng.plan.scratch), L.ptr(E), 0.0, st)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step(evs[i])
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    exch_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
-    ms_step = dt * 1e3 / args.steps
-    achieved = bytes_exch / (exch_ms * 1e-3) / 1e9
-    out = {
-        "metric": METRIC,
-        "value": args.steps / dt,
-        "unit": "Fock builds/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": args.dtype,
-        "data": "synthetic (seeded tight-binding h, layer-resolved V, Haar density)",
-        "config": {"workload": f"build_fock+hf_energy nk={nk}x{nk} nb={nb} layer-resolved V, "
-                               f"all {nb * nb} channels, Sigma+F+E outputs",
-                   "nk": nk, "nb": nb, "parallelism": "1 GPU"},
-        "fock_build_GBps": bytes_build / (ms_step * 1e-3) / 1e9,
-        "exchange_variant": L.Plan.VARIANTS[eng.plan.exchange_variant],
-        "exchange_variants_ms": variant_ms,
-        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": bytes_exch, "avg_ms": exch_ms},
-        "energy": float(E),
-    }
+    fock, roof, F_gpu = fock_bench(J, L, syn, inp, dev, nk, nb, args)
+    out = {"metric": METRIC, "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "c64+f64", "data": "synthetic (seeded tight-binding h, layer-resolved V; Haar density "
+                                       "for the Fock build)"}
+    if args.fock_only:
+        out.update({"value": fock["builds_per_s"], "unit": "Fock builds/s", "steps": args.fock_steps,
+                    "warmup": 20, "ms_per_step": fock["ms_per_build"], "dtype": "c128",
+                    "config": {"workload": f"build_fock+hf_energy nk={nk}x{nk} nb={nb} layer-resolved V, "
+                                           f"all {nb * nb} channels, Sigma+F+E outputs"}})
+    else:
+        solves = solve_bench(J, inp, dev, nk, nb, args)
+        mixed = solves["mixed"]
+        out.update({"value": mixed["solves_per_s"], "unit": "solves/s", "steps": args.steps,
+                    "warmup": args.warmup, "ms_per_step": mixed["seconds_median"] * 1e3,
+                    "config": {"workload": (f"converged CG solve, mixed precision (FP32 CG + FP64 DIIS "
+                                            f"polish to 1e-8), nk={nk}x{nk} nb={nb}, layer-resolved V, "
+                                            f"T={SOLVE_T}, half filling, P0=0"),
+                               "nk": nk, "nb": nb, "parallelism": "1 GPU"},
+                    "solves": solves})
+    out["fock"] = fock
+    out["fock_build_GBps"] = fock["GBps"]
+    out["roofline"] = roof
     if args.extra:
         out["extra"] = extra_forms(J, L, syn, inp, dev, nk, nb, args)
-    if args.solve:
-        out["solves"] = solve_forms(J, syn, inp, dev, nk, nb, args)
     if not args.no_cpu_baseline:
-        cb, E_cpu, F_cpu = cpu_baseline(inp, args.cpu_seconds)
+        n_it = out.get("solves", {}).get("fp64_cg", {}).get("n_iter", 1)
+        cb, E_cpu, F_cpu = cpu_baseline(inp, args.cpu_seconds, n_it)
         out["cpu_baseline"] = cb
-        if args.dtype == "c128":
-            Fg = F.cpu().numpy()
-            out["parity"] = {"E_abs_diff": abs(float(E) - E_cpu),
-                             "F_max_rel": float(np.abs(Fg - F_cpu).max() / np.abs(F_cpu).max())}
+        Fg = F_gpu.cpu().numpy()
+        out["parity"] = {"fock_E_abs_diff": abs(fock["energy"] - E_cpu),
+                         "fock_F_max_rel": float(np.abs(Fg - F_cpu).max() / np.abs(F_cpu).max())}
     print(json.dumps(out))
-
-
-def extra_forms(J, L, syn, inp, dev, nk, nb, args):
-    """Packed (upper-triangle) and complex64 variants, same timing method."""
-    import torch
-    res = {}
-    for name, npd, herm in (("c128_packed", np.complex128, True), ("c64_full", np.complex64, False),
-                            ("c64_packed", np.complex64, True)):
-        G = J.HartreeFockKernel(inp["weights"], inp["hamiltonian"].astype(npd), inp["coulomb_q"],
-                                inp["T"], device=dev)
-        eng = G.fock_engine(herm=herm)
-        P = torch.from_numpy(np.ascontiguousarray(inp["P"], dtype=npd)).to(dev)
-        F = torch.empty_like(P)
-        for _ in range(10):
-            eng.build(P, F, energy=True, refP=None)
-        torch.cuda.synchronize()
-        n = max(20, args.steps // 2)
-        t0 = time.perf_counter()
-        for _ in range(n):
-            eng.build(P, F, energy=True, refP=None)
-        torch.cuda.synchronize()
-        res[name] = {"ms_per_build": (time.perf_counter() - t0) * 1e3 / n}
-    return res
-
-
-def solve_forms(J, syn, inp, dev, nk, nb, args):
-    """Converged CG solves from P0 = 0 at half filling, FP64 vs mixed precision.
-
-    Same h / layer-resolved V as the Fock-build workload, T = 0.1,
-    n_electrons = nb/2 per k-point, tol_E = args.solve_tol (windowed test).
-    |dE| of the mixed solve is against the FP64 GPU solve at the same tolerance.
-    """
-    import torch
-    res = {}
-    G = J.HartreeFockKernel(inp["weights"], inp["hamiltonian"], inp["coulomb_q"], 0.1, device=dev)
-    P0 = np.zeros_like(inp["hamiltonian"])
-    ne = nb / 2.0
-    E_ref = None
-    for prec in ("fp64", "mixed"):
-        cfg = J.SolverConfig(max_iter=500, tol_E=args.solve_tol, precision=prec)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        r = J.solve(G, P0, ne, config=cfg)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        E = float(r.energy)
-        if E_ref is None:
-            E_ref = E
-        res[prec] = {"solves_per_s": 1.0 / dt, "seconds": dt, "n_iter": r.n_iter, "converged": r.converged,
-                     "energy": E, "abs_dE_vs_fp64": abs(E - E_ref)}
-        if "phase_iters" in r.history:
-            hE = r.history["E"].cpu().numpy()
-            res[prec]["phase_iters"] = [int(x) for x in r.history["phase_iters"]]
-            res[prec]["E_head"] = [float(x) for x in hE[:12]]
-            res[prec]["E_tail"] = [float(x) for x in hE[max(r.n_iter - 6, 0):r.n_iter]]
-    res["speedup_mixed_vs_fp64"] = res["fp64"]["seconds"] / res["mixed"]["seconds"]
-    return res
 
 
 if __name__ == "__main__":

@@ -175,6 +175,11 @@ int hfx_entropy(const void* p, int real_kind, const double* w, size_t nk, int nb
 #define HFX_ST_OMEGA0 8
 #define HFX_ST_DNORM 9
 #define HFX_ST_OMEGA 10
+#define HFX_ST_E 11
+#define HFX_ST_ITER 12   /* iteration counter advanced by the HFX_OP_BETA step */
+#define HFX_ST_MULO 13   /* mu bracket and done flag of hfx_solve_mu_steps */
+#define HFX_ST_MUHI 14
+#define HFX_ST_MUDONE 15
 #define HFX_ST_LEN 16
 
 /* Batched Hermitian eigensolver (jnp.linalg.eigh call sites, solver.py:165,
@@ -206,6 +211,24 @@ int hfx_cg_trial(int dtype, int nb, size_t nk, const void* V, const double* lam_
  * Ft_out = U^dagger Fe U (Fe = Ft, solver.py:1005-1009) optional. */
 int hfx_cg_retract(int dtype, int nb, size_t nk, void* Q, const void* V, const double* lam_d, const void* Fe,
                    const double* st, double tau, int use_tau, double* eps_new, void* Ft_out, void* stream);
+/* The CG direction / trial / retraction without the spectral setup: the
+ * Cayley factor U(tau) = (I + (tau/2) d)^{-1} (I - (tau/2) d), d = skew(D), is
+ * formed per trial by Gauss-Jordan elimination (the LU form of the same
+ * unitary, solver.py:140-197 and linalg.py Cayley retraction), so no eigh runs
+ * inside the iteration.  direction: D <- H + beta D, dp <- hp + beta dp,
+ * part = |D|^2; trial: as hfx_cg_trial from D and Ft; retract: Q <- Q U(st
+ * [HFX_ST_TAUF]), eps_new = diag(U^dagger Ft U)  (solver.py:528-592). */
+int hfx_cg_direction_gj(int dtype, int nb, size_t nk, const void* H, void* D, const double* hp, double* dp,
+                        const double* st, double* part, void* stream);
+int hfx_cg_trial_gj(int dtype, int nb, size_t nk, const void* D, const void* Ft, const double* p,
+                    const double* dp, const double* eps, const double* st, int omega0, double* part,
+                    void* U_keep, double* eps_keep, void* stream);
+/* U_keep / eps_keep (optional, both or neither): the last evaluated trial's U
+ * and diag(U^dagger Ft U); the retraction reuses them when a trial was
+ * accepted (st[HFX_ST_ACC] set, so tau_final is that trial's tau). */
+int hfx_cg_retract_gj(int dtype, int nb, size_t nk, void* Q, const void* D, const void* Ft, const double* st,
+                      double* eps_new, const void* U_keep, const double* eps_keep, void* stream);
+
 /* Q <- Q W with (eps, W) = eigh(Q^dagger F Q)  (solver.py:622-625). */
 int hfx_rotate_eigh(int dtype, int nb, size_t nk, void* Q, const void* F, double* eps, void* stream);
 /* out = opA(A) opB(B - Bsub) [opC(C)] per k, optionally hermitised; ops bit0/1/2
@@ -250,8 +273,11 @@ int hfx_newton_prec(int dtype, int nb, size_t nk, const void* RX, const double* 
 /* Scratch (float64) for hfx_reduce / hfx_solve_mu. */
 size_t hfx_solver_scratch_doubles(void);
 /* out[v] = sum_k (w[k] if bit v of wmask) part[v][k] in a fixed order; then the
- * scalar step `op` on st.  params: {T, denom_scale, count, max_step, shrink,
- * tau_fallback, restart, last, omega0}. */
+ * scalar step `op` on st.  params (10 doubles): {T, denom_scale, count,
+ * max_step, shrink, tau_fallback, restart, last, omega0, restart_every}; the
+ * HFX_OP_BETA step restarts (beta = 0) when `restart` is set or when
+ * restart_every > 0 and st[HFX_ST_ITER] % restart_every == 0 (solver.py:517),
+ * then advances st[HFX_ST_ITER]. */
 int hfx_reduce(const double* part, int nval, size_t nk, const double* w, unsigned wmask, double* out,
                double* st, int op, const double* params, double* scratch, void* stream);
 
@@ -267,6 +293,17 @@ int hfx_reduce(const double* part, int nval, size_t nk, const double* w, unsigne
 int hfx_solve_mu(const double* eps, const double* w, size_t nk, int nb, double target, double T, double mu0,
                  int mu0_from_state, int maxiter, int mode, double level_shift, double* st, double* mu_out,
                  double* p_out, double* scratch, void* stream);
+
+/* The Newton-with-bracket mu solver of solver.py:62-101 as a fixed sequence
+ * of ordinary launches (bracket, maxiter steps, occupations) that can be
+ * captured in a HIP graph: each step is a grid reduction whose last block
+ * updates st[HFX_ST_MU] / [MULO] / [MUHI]; once a Newton step no longer moves
+ * mu, st[HFX_ST_MUDONE] is set and the remaining steps return at once.
+ * Occupations sigmoid((mu - eps) / T) to p_out.  Same result as
+ * hfx_solve_mu(HFX_MU_NEWTON_BRACKET) up to the order of the sums. */
+int hfx_solve_mu_steps(const double* eps, const double* w, size_t nk, int nb, double target, double T,
+                       double mu0, int mu0_from_state, int maxiter, double* st, double* p_out, double* scratch,
+                       void* stream);
 
 #ifdef __cplusplus
 }

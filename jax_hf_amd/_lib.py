@@ -60,6 +60,9 @@ _SIGS = {
     "hfx_cg_precond": [i32, i32, sz, vp, vp, vp, vp, vp, vp, f64, vp, vp, vp, vp, vp],
     "hfx_cg_direction": [i32, i32, sz, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
     "hfx_cg_trial": [i32, i32, sz, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp],
+    "hfx_cg_direction_gj": [i32, i32, sz, vp, vp, vp, vp, vp, vp, vp],
+    "hfx_cg_trial_gj": [i32, i32, sz, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp],
+    "hfx_cg_retract_gj": [i32, i32, sz, vp, vp, vp, vp, vp, vp, vp, vp],
     "hfx_cg_retract": [i32, i32, sz, vp, vp, vp, vp, vp, f64, i32, vp, vp, vp],
     "hfx_rotate_eigh": [i32, i32, sz, vp, vp, vp, vp],
     "hfx_matmul_batched": [i32, i32, sz, vp, vp, vp, i32, vp, i32, vp, vp],
@@ -72,6 +75,7 @@ _SIGS = {
     "hfx_newton_prec": [i32, i32, sz, vp, vp, vp, vp, vp, f64, vp, vp, vp, vp],
     "hfx_solver_scratch_doubles": [],
     "hfx_reduce": [vp, i32, sz, vp, C.c_uint, vp, vp, i32, vp, vp, vp],
+    "hfx_solve_mu_steps": [vp, vp, sz, i32, f64, f64, f64, i32, i32, vp, vp, vp, vp],
     "hfx_solve_mu": [vp, vp, sz, i32, f64, f64, f64, i32, i32, i32, f64, vp, vp, vp, vp, vp],
 }
 _RESTYPES = {"hfx_energy_scratch_bytes": sz, "hfx_solver_scratch_doubles": sz}

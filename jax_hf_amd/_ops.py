@@ -8,6 +8,7 @@ from . import _lib as L
 
 ST_MU, ST_BETA, ST_LAM, ST_GRAD, ST_TAU0, ST_TAU, ST_TAUF, ST_ACC = 0, 1, 2, 3, 4, 5, 6, 7
 ST_OMEGA0, ST_DNORM, ST_OMEGA, ST_E = 8, 9, 10, 11
+ST_ITER, ST_MULO, ST_MUHI, ST_MUDONE = 12, 13, 14, 15
 ST_LEN = 16
 OP_SUM, OP_GRAD, OP_BETA, OP_TAU0, OP_TRIAL = 0, 1, 2, 3, 4
 MU_NEWTON_BRACKET, MU_BISECTION, MU_NEWTON = 0, 1, 2
@@ -77,8 +78,8 @@ _param_bufs: dict = {}
 def _params_buf(ws, params):
     """Host-side parameter vector for hfx_reduce (read by the launcher, not the kernel)."""
     import ctypes as C
-    vals = list(params) + [0.0] * (9 - len(params))
-    arr = (C.c_double * 9)(*[float(v) for v in vals])
+    vals = list(params) + [0.0] * (10 - len(params))
+    arr = (C.c_double * 10)(*[float(v) for v in vals])
     _param_bufs[id(ws)] = arr          # keep alive for the call
     return C.cast(arr, C.c_void_p)
 
@@ -88,6 +89,12 @@ def solve_mu(ws: Workspace, eps, w, nk, nb, target, T, mu0=0.0, mu0_from_state=F
     L.call("hfx_solve_mu", L.ptr(eps), L.ptr(w), int(nk), int(nb), float(target), float(T), float(mu0),
            int(mu0_from_state), int(maxiter), int(mode), float(level_shift), L.ptr(st), mu_out_ptr,
            L.ptr(p_out), L.ptr(ws.scratch), s(eps.device))
+
+
+def solve_mu_steps(ws: Workspace, eps, w, nk, nb, target, T, mu0=0.0, mu0_from_state=False, maxiter=25,
+                   st=None, p_out=None):
+    L.call("hfx_solve_mu_steps", L.ptr(eps), L.ptr(w), int(nk), int(nb), float(target), float(T), float(mu0),
+           int(mu0_from_state), int(maxiter), L.ptr(st), L.ptr(p_out), L.ptr(ws.scratch), s(eps.device))
 
 
 def st_ptr(st, idx):

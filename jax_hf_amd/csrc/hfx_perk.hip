@@ -520,6 +520,185 @@ __global__ __launch_bounds__(64) void pk_cg_retract_kernel(int nb, cx<T>* __rest
   }
 }
 
+// ---- Cayley retraction by Gauss-Jordan (no eigh) -----------------------------
+// U(tau) = (I + (tau/2) d)^{-1} (I - (tau/2) d), d = (D - D^dagger)/2 the
+// skew-Hermitian part of the direction -- the same unitary as the spectral
+// form V diag((1 + i tau lam/2)/(1 - i tau lam/2)) V^dagger of solver.py:140-197
+// (i d = V diag(lam) V^dagger), obtained without diagonalising i d.  The
+// elimination needs no pivoting: I + (tau/2) d has Hermitian part I, so every
+// Schur complement keeps a positive definite Hermitian part and every pivot a
+// real part >= 1.  On entry sX holds D; on exit sU holds U (sM, sX clobbered).
+template <typename T>
+__device__ void m_cayley_gj(cx<T>* sM, cx<T>* sU, const cx<T>* sX, double tau, int nb) {
+  const T h = (T)(0.5 * tau);
+  PK_FOR_ELEM(nb, {
+    const cx<T> a = sX[i * PK_PITCH + j], b = sX[j * PK_PITCH + i];
+    const cx<T> d{(T)0.5 * (a.re - b.re), (T)0.5 * (a.im + b.im)};   // (D - D^dagger)/2
+    const T one = i == j ? (T)1 : (T)0;
+    sM[i * PK_PITCH + j] = cx<T>{one + h * d.re, h * d.im};
+    sU[i * PK_PITCH + j] = cx<T>{one - h * d.re, -h * d.im};
+  })
+  __syncthreads();
+  for (int pv = 0; pv < nb; ++pv) {
+    const cx<T> mp = sM[pv * PK_PITCH + pv];
+    const T n2 = mp.re * mp.re + mp.im * mp.im;
+    const cx<T> ip{mp.re / n2, -mp.im / n2};   // 1 / M_pp
+    cx<T> nm[PK_EPL], nu[PK_EPL];
+    PK_FOR_ELEM(nb, {
+      const cx<T> mpj = sM[pv * PK_PITCH + j] * ip, upj = sU[pv * PK_PITCH + j] * ip;
+      if (i == pv) {
+        nm[m_] = mpj;
+        nu[m_] = upj;
+      } else {
+        const cx<T> f = sM[i * PK_PITCH + pv];
+        nm[m_] = sM[i * PK_PITCH + j] - f * mpj;
+        nu[m_] = sU[i * PK_PITCH + j] - f * upj;
+      }
+    })
+    __syncthreads();
+    PK_FOR_ELEM(nb, {
+      sM[i * PK_PITCH + j] = nm[m_];
+      sU[i * PK_PITCH + j] = nu[m_];
+    })
+    __syncthreads();
+  }
+}
+
+// eps_b = Re sum_i conj(U_ib) (Ft U)_ib into et[b] (Y: LDS scratch)
+template <typename T>
+__device__ __forceinline__ void m_diag_UFU(const cx<T>* sF, const cx<T>* sU, cx<T>* sY, double* et, int nb) {
+  m_gemm<T, false, false>(sY, sF, sU, nb);
+  if ((int)threadIdx.x < nb) {
+    const int b = threadIdx.x;
+    double acc = 0.0;
+    for (int i = 0; i < nb; ++i) {
+      const cx<T> u = sU[i * PK_PITCH + b], y = sY[i * PK_PITCH + b];
+      acc += (double)u.re * (double)y.re + (double)u.im * (double)y.im;
+    }
+    et[b] = acc;
+  }
+  __syncthreads();
+}
+
+// Direction without the spectral setup: D <- H + beta D, dp <- hp + beta dp,
+// part[k] = |D_k|^2 (solver.py:530-531, 545).
+template <typename T>
+__global__ __launch_bounds__(64) void pk_cg_dir_kernel(int nb, const cx<T>* __restrict__ H, cx<T>* __restrict__ D,
+                                                       const double* __restrict__ hp, double* __restrict__ dp,
+                                                       const double* __restrict__ st, double* __restrict__ part) {
+  const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  const T beta = (T)st[HFX_ST_BETA];
+  double dn = 0.0;
+  PK_FOR_ELEM(nb, {
+    const cx<T> d = H[k * nb2 + e] + beta * D[k * nb2 + e];
+    D[k * nb2 + e] = d;
+    dn += (double)d.re * d.re + (double)d.im * d.im;
+  })
+  if ((int)threadIdx.x < nb) {
+    const size_t q = k * nb + threadIdx.x;
+    dp[q] = hp[q] + (double)beta * dp[q];
+  }
+  dn = wave_sum(dn);
+  if (threadIdx.x == 0) part[k] = dn;
+}
+
+// Line-search trial with the Gauss-Jordan Cayley factor; same outputs as
+// pk_cg_trial_kernel.  Skipped once st[HFX_ST_ACC] is set.
+template <typename T>
+__global__ __launch_bounds__(64) void pk_cg_trial_gj_kernel(int nb, size_t Nk, const cx<T>* __restrict__ D,
+                                                            const cx<T>* __restrict__ Ft,
+                                                            const double* __restrict__ p,
+                                                            const double* __restrict__ dp,
+                                                            const double* __restrict__ eps,
+                                                            const double* __restrict__ st, int omega0,
+                                                            double* __restrict__ part, cx<T>* __restrict__ U_out,
+                                                            double* __restrict__ eps_out) {
+  if (st[HFX_ST_ACC] != 0.0) return;
+  __shared__ cx<T> sX[PK_MAT], sM[PK_MAT], sU[PK_MAT], sF[PK_MAT];
+  __shared__ double et[PK_NBMAX];
+  const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  const double tau = st[HFX_ST_TAU];
+  m_load(sX, D + k * nb2, nb);
+  m_load(sF, Ft + k * nb2, nb);
+  __syncthreads();
+  m_cayley_gj(sM, sU, sX, tau, nb);
+  m_diag_UFU(sF, sU, sX, et, nb);
+  if (U_out) {   // kept for the retraction when this trial is the accepted one
+    m_store(U_out + k * nb2, sU, nb);
+    if ((int)threadIdx.x < nb) eps_out[k * nb + threadIdx.x] = et[threadIdx.x];
+  }
+  double ef = 0.0, sf = 0.0, e0 = 0.0, s0 = 0.0;
+  if ((int)threadIdx.x < nb) {
+    const size_t q = k * nb + threadIdx.x;
+    double pt = fmin(fmax(p[q] - tau * dp[q], 1e-8), 1.0 - 1e-8);
+    ef = pt * et[threadIdx.x];
+    double x = fmin(fmax(pt, 1e-14), 1.0 - 1e-14);
+    sf = x * log(x) + (1.0 - x) * log1p(-x);
+    if (omega0) {
+      const double p0 = fmin(fmax(p[q], 1e-8), 1.0 - 1e-8);
+      e0 = p0 * eps[q];
+      x = fmin(fmax(p0, 1e-14), 1.0 - 1e-14);
+      s0 = x * log(x) + (1.0 - x) * log1p(-x);
+    }
+  }
+  ef = wave_sum(ef);
+  sf = wave_sum(sf);
+  e0 = wave_sum(e0);
+  s0 = wave_sum(s0);
+  if (threadIdx.x == 0) {
+    part[k] = ef;
+    part[Nk + k] = sf;
+    if (omega0) {
+      part[2 * Nk + k] = e0;
+      part[3 * Nk + k] = s0;
+    }
+  }
+}
+
+// Retraction with the Gauss-Jordan Cayley factor (tau = st[HFX_ST_TAUF]):
+// Q <- Q U (FP32: plus one Newton-Schulz re-orthonormalisation step),
+// eps_new = diag(U^dagger Ft U)  (solver.py:586-592).
+template <typename T>
+__global__ __launch_bounds__(64) void pk_cg_retract_gj_kernel(int nb, cx<T>* __restrict__ Q,
+                                                              const cx<T>* __restrict__ D,
+                                                              const cx<T>* __restrict__ Ft,
+                                                              const double* __restrict__ st,
+                                                              double* __restrict__ eps_new,
+                                                              const cx<T>* __restrict__ U_acc,
+                                                              const double* __restrict__ eps_acc) {
+  __shared__ cx<T> sX[PK_MAT], sM[PK_MAT], sU[PK_MAT], sF[PK_MAT];
+  __shared__ double et[PK_NBMAX];
+  const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  if (U_acc && st[HFX_ST_ACC] != 0.0) {
+    // a trial was accepted: tau_final is its tau, and its U / eps were kept
+    m_load(sU, U_acc + k * nb2, nb);
+    if ((int)threadIdx.x < nb) eps_new[k * nb + threadIdx.x] = eps_acc[k * nb + threadIdx.x];
+  } else {
+    const double tau = st[HFX_ST_TAUF];
+    m_load(sX, D + k * nb2, nb);
+    m_load(sF, Ft + k * nb2, nb);
+    __syncthreads();
+    m_cayley_gj(sM, sU, sX, tau, nb);
+    m_diag_UFU(sF, sU, sX, et, nb);
+    if ((int)threadIdx.x < nb) eps_new[k * nb + threadIdx.x] = et[threadIdx.x];
+  }
+  m_load(sF, Q + k * nb2, nb);
+  __syncthreads();
+  m_gemm<T, false, false>(sX, sF, sU, nb);    // X = Q U
+  if constexpr (sizeof(T) == 4) {
+    m_gemm<T, true, false>(sM, sX, sX, nb);   // S = X^dagger X
+    PK_FOR_ELEM(nb, {
+      const cx<T> v = sM[i * PK_PITCH + j];
+      sM[i * PK_PITCH + j] = cx<T>{(i == j ? (T)1.5 : (T)0) - (T)0.5 * v.re, (T)-0.5 * v.im};
+    })
+    __syncthreads();
+    m_gemm<T, false, false>(sU, sX, sM, nb);  // Q <- X (3 I - S) / 2
+    m_store(Q + k * nb2, sU, nb);
+  } else {
+    m_store(Q + k * nb2, sX, nb);
+  }
+}
+
 // Ft = Q^dagger F Q, eigh(Ft) -> (eps, W), Q <- Q W  (solver.py:622-625)
 template <typename T>
 __global__ __launch_bounds__(64) void pk_rotate_eigh_kernel(int nb, cx<T>* __restrict__ Q, const cx<T>* __restrict__ F,
@@ -847,6 +1026,49 @@ int hfx_cg_retract(int dtype, int nb, size_t nk, void* Q, const void* V, const d
     hipLaunchKernelGGL(pk_cg_retract_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKM(float, Q), PKC(float, V), lam_d,
                        PKC(float, Fe), st, tau, use_tau, eps_new, PKM(float, Ft_out));
   return check_launch("pk_cg_retract_kernel");
+}
+
+int hfx_cg_direction_gj(int dtype, int nb, size_t nk, const void* H, void* D, const double* hp, double* dp,
+                        const double* st, double* part, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(pk_cg_dir_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKC(double, H), PKM(double, D), hp,
+                       dp, st, part);
+  else
+    hipLaunchKernelGGL(pk_cg_dir_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKC(float, H), PKM(float, D), hp, dp,
+                       st, part);
+  return check_launch("pk_cg_dir_kernel");
+}
+
+int hfx_cg_trial_gj(int dtype, int nb, size_t nk, const void* D, const void* Ft, const double* p, const double* dp,
+                    const double* eps, const double* st, int omega0, double* part, void* U_keep,
+                    double* eps_keep, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  if (!U_keep != !eps_keep) return set_error(HFX_EINVAL, "hfx_cg_trial_gj: U_keep and eps_keep go together");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(pk_cg_trial_gj_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(double, D),
+                       PKC(double, Ft), p, dp, eps, st, omega0, part, PKM(double, U_keep), eps_keep);
+  else
+    hipLaunchKernelGGL(pk_cg_trial_gj_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(float, D),
+                       PKC(float, Ft), p, dp, eps, st, omega0, part, PKM(float, U_keep), eps_keep);
+  return check_launch("pk_cg_trial_gj_kernel");
+}
+
+int hfx_cg_retract_gj(int dtype, int nb, size_t nk, void* Q, const void* D, const void* Ft, const double* st,
+                      double* eps_new, const void* U_keep, const double* eps_keep, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  if (!eps_new) return set_error(HFX_EINVAL, "hfx_cg_retract_gj: eps_new is required");
+  if (!U_keep != !eps_keep) return set_error(HFX_EINVAL, "hfx_cg_retract_gj: U_keep and eps_keep go together");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(pk_cg_retract_gj_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKM(double, Q),
+                       PKC(double, D), PKC(double, Ft), st, eps_new, PKC(double, U_keep), eps_keep);
+  else
+    hipLaunchKernelGGL(pk_cg_retract_gj_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKM(float, Q),
+                       PKC(float, D), PKC(float, Ft), st, eps_new, PKC(float, U_keep), eps_keep);
+  return check_launch("pk_cg_retract_gj_kernel");
 }
 
 int hfx_rotate_eigh(int dtype, int nb, size_t nk, void* Q, const void* F, double* eps, void* stream) {

@@ -28,10 +28,9 @@ constexpr int RED_BLOCK = 256;
 constexpr int RED_MAX_BLOCKS = 256;
 constexpr int RED_MAX_VALS = 16;
 
-__global__ __launch_bounds__(RED_BLOCK) void reduce_stage1(const double* __restrict__ part, int nval, size_t nk,
-                                                           const double* __restrict__ w, unsigned wmask,
-                                                           double* __restrict__ s1) {
-  __shared__ double sh[RED_MAX_VALS * (RED_BLOCK / 64)];
+__device__ __forceinline__ void stage1_body(const double* __restrict__ part, int nval, size_t nk,
+                                            const double* __restrict__ w, unsigned wmask, double* __restrict__ s1,
+                                            double* sh) {
   const size_t chunk = (nk + gridDim.x - 1) / gridDim.x;
   const size_t k0 = (size_t)blockIdx.x * chunk, k1 = k0 + chunk < nk ? k0 + chunk : nk;
   double acc[RED_MAX_VALS];
@@ -51,14 +50,12 @@ __global__ __launch_bounds__(RED_BLOCK) void reduce_stage1(const double* __restr
 struct PostArgs {
   int op;
   double T, denom_scale, count, max_step, shrink, tau_fallback;
-  int restart, last, omega0;
+  int restart, last, omega0, restart_every;
 };
 
-__global__ __launch_bounds__(RED_BLOCK) void reduce_stage2(const double* __restrict__ s1, int nval, int nblk,
-                                                           double* __restrict__ out, double* __restrict__ st,
-                                                           PostArgs pa) {
-  __shared__ double sh[RED_MAX_VALS * (RED_BLOCK / 64)];
-  if (pa.op == HFX_OP_TRIAL && st[HFX_ST_ACC] != 0.0) return;
+__device__ __forceinline__ void stage2_body(const double* __restrict__ s1, int nval, int nblk,
+                                            double* __restrict__ out, double* __restrict__ st, const PostArgs& pa,
+                                            double* sh) {
   double acc[RED_MAX_VALS];
 #pragma unroll
   for (int v = 0; v < RED_MAX_VALS; ++v) acc[v] = 0.0;
@@ -82,8 +79,10 @@ __global__ __launch_bounds__(RED_BLOCK) void reduce_stage2(const double* __restr
       const double den = acc[4] + acc[5];
       double beta = den > tiny ? fmax(0.0, num / den) : 0.0;
       beta = fmin(beta, 5.0);
-      if (pa.restart) beta = 0.0;
+      const int it = (int)st[HFX_ST_ITER];
+      if (pa.restart || (pa.restart_every > 0 && it % pa.restart_every == 0)) beta = 0.0;
       st[HFX_ST_BETA] = beta;
+      st[HFX_ST_ITER] = (double)(it + 1);
     } break;
     case HFX_OP_TAU0: {   // solver.py:546-547
       const double dn = sqrt(fmax(0.0, acc[0]));
@@ -110,6 +109,28 @@ __global__ __launch_bounds__(RED_BLOCK) void reduce_stage2(const double* __restr
     default:
       break;
   }
+}
+
+// One launch: every block reduces its chunk; the block that finishes last
+// (completion ticket) sums the block partials in the fixed order and applies
+// the scalar step.  Same order, so same bits, as the former two-launch form.
+__global__ __launch_bounds__(RED_BLOCK) void reduce_fused(const double* __restrict__ part, int nval, size_t nk,
+                                                          const double* __restrict__ w, unsigned wmask,
+                                                          double* __restrict__ s1, double* __restrict__ out,
+                                                          double* __restrict__ st, PostArgs pa,
+                                                          unsigned* __restrict__ ticket) {
+  __shared__ double sh[RED_MAX_VALS * (RED_BLOCK / 64)];
+  if (pa.op == HFX_OP_TRIAL && st[HFX_ST_ACC] != 0.0) return;   // line search already done
+  stage1_body(part, nval, nk, w, wmask, s1, sh);
+  __shared__ bool last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (threadIdx.x == 0) *ticket = 0u;
+  stage2_body(s1, nval, (int)gridDim.x, out, st, pa, sh);
 }
 
 int reduce_blocks(size_t nk) {
@@ -299,9 +320,9 @@ __global__ __launch_bounds__(MU_BLOCK) void mu_kernel(MuArgs a) {
       double N, Z;
       grid_sum2(grid, Np, Zp, a.scratch, 1 + (it & 1), sh, N, Z);
       const double g = N - a.target;
-      // converged: the Newton step no longer moves mu (every block takes this
-      // branch together, the totals being bitwise identical across blocks)
-      if (mu - g / fmax(Z, 1e-18) == mu) break;
+      // converged: the Newton step is below 4 ulp of max(|mu|, T) (every block
+      // takes this branch together, the totals being bitwise identical)
+      if (fabs(g / fmax(Z, 1e-18)) <= 8.9e-16 * fmax(fabs(mu), Tj)) break;
       if (g < 0) lo = mu;
       if (g > 0) hi = mu;
       double mu_new = mu - g / fmax(Z, 1e-18);
@@ -330,9 +351,9 @@ __global__ __launch_bounds__(MU_BLOCK) void mu_kernel(MuArgs a) {
       grid_sum2(grid, Np, Zp, a.scratch, 1 + (it & 1), sh, N, Z);
       if (it == a.maxiter) break;   // final residual check below
       const double g = N - a.target;
-      // converged: the Newton step no longer moves mu (every block takes this
-      // branch together, the totals being bitwise identical across blocks)
-      if (mu - g / fmax(Z, 1e-18) == mu) break;
+      // converged: the Newton step is below 4 ulp of max(|mu|, T) (every block
+      // takes this branch together, the totals being bitwise identical)
+      if (fabs(g / fmax(Z, 1e-18)) <= 8.9e-16 * fmax(fabs(mu), Tj)) break;
       if (g < 0) lo = mu;
       if (g > 0) hi = mu;
       double mu_new = mu - g / fmax(Z, 1e-18);
@@ -364,6 +385,119 @@ __global__ __launch_bounds__(MU_BLOCK) void mu_kernel(MuArgs a) {
   }
 }
 
+// ---- graph-capturable mu (Newton with bracket, one launch per step) --------
+constexpr int MUS_BLOCK = 256;
+constexpr size_t MU_PART_OFF = (size_t)RED_MAX_VALS * RED_MAX_BLOCKS;   // mu partials
+constexpr size_t MU_TICKET_OFF = MU_PART_OFF + 8 * 512;                 // completion tickets
+constexpr size_t RED_TICKET_OFF = MU_TICKET_OFF + 1;
+
+struct MuStepArgs {
+  const double* eps;
+  const double* w;
+  size_t nk;
+  int nb;
+  double target, T, mu0;
+  int mu0_from_state;
+  double* st;
+  double* part;        // 2 * gridDim.x doubles
+  unsigned* ticket;    // zero between launches (reset by the last block)
+  double* p_out;
+};
+
+// Returns true in the block that finished last (all partials visible to it).
+__device__ __forceinline__ bool last_block_done(unsigned* ticket) {
+  __shared__ bool last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (last) __threadfence();
+  return last;
+}
+
+__global__ __launch_bounds__(MUS_BLOCK) void mu_bracket_kernel(MuStepArgs a) {
+  __shared__ double sh[2 * (MUS_BLOCK / 64)];
+  const size_t n = a.nk * a.nb, stride = (size_t)gridDim.x * MUS_BLOCK;
+  double lo = INFINITY, hi = -INFINITY;
+  for (size_t e = (size_t)blockIdx.x * MUS_BLOCK + threadIdx.x; e < n; e += stride) {
+    lo = fmin(lo, a.eps[e]);
+    hi = fmax(hi, a.eps[e]);
+  }
+  lo = wave_min(lo);
+  hi = wave_max(hi);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = MUS_BLOCK / 64;
+  if (lane == 0) { sh[wid] = lo; sh[nw + wid] = hi; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < nw; ++q) { lo = fmin(lo, sh[q]); hi = fmax(hi, sh[nw + q]); }
+    a.part[2 * blockIdx.x] = lo;
+    a.part[2 * blockIdx.x + 1] = hi;
+  }
+  if (!last_block_done(a.ticket)) return;
+  if (threadIdx.x == 0) {
+    double mn = INFINITY, mx = -INFINITY;
+    for (unsigned b = 0; b < gridDim.x; ++b) { mn = fmin(mn, a.part[2 * b]); mx = fmax(mx, a.part[2 * b + 1]); }
+    const double Tj = fmax(a.T, 1e-12);
+    const double blo = mn - 50.0 * Tj, bhi = mx + 50.0 * Tj;   // solver.py:70-72
+    double mu = a.mu0_from_state ? a.st[HFX_ST_MU] : a.mu0;
+    mu = fmin(fmax(mu, blo), bhi);
+    a.st[HFX_ST_MU] = mu;
+    a.st[HFX_ST_MULO] = blo;
+    a.st[HFX_ST_MUHI] = bhi;
+    a.st[HFX_ST_MUDONE] = 0.0;
+    *a.ticket = 0u;
+  }
+}
+
+__global__ __launch_bounds__(MUS_BLOCK) void mu_step_kernel(MuStepArgs a) {
+  if (a.st[HFX_ST_MUDONE] != 0.0) return;
+  __shared__ double sh[2 * (MUS_BLOCK / 64)];
+  const size_t n = a.nk * a.nb, stride = (size_t)gridDim.x * MUS_BLOCK;
+  const double Tj = fmax(a.T, 1e-12);
+  const double mu = a.st[HFX_ST_MU];
+  double acc[2] = {0.0, 0.0};
+  for (size_t e = (size_t)blockIdx.x * MUS_BLOCK + threadIdx.x; e < n; e += stride) {
+    const double pv = sigm((mu - a.eps[e]) / Tj);
+    const double wk = a.w[e / a.nb];
+    acc[0] += wk * pv;
+    acc[1] += wk * pv * (1.0 - pv) / Tj;
+  }
+  block_sum<2>(acc, sh);
+  if (threadIdx.x == 0) {
+    a.part[2 * blockIdx.x] = acc[0];
+    a.part[2 * blockIdx.x + 1] = acc[1];
+  }
+  if (!last_block_done(a.ticket)) return;
+  double N, Z;
+  block_gather2(a.part, (int)gridDim.x, sh, N, Z);
+  if (threadIdx.x == 0) {
+    const double g = N - a.target;   // solver.py:80-99
+    if (fabs(g / fmax(Z, 1e-18)) <= 8.9e-16 * fmax(fabs(mu), Tj)) {   // step below 4 ulp: converged
+      a.st[HFX_ST_MUDONE] = 1.0;
+    } else {
+      double lo = a.st[HFX_ST_MULO], hi = a.st[HFX_ST_MUHI];
+      if (g < 0) lo = mu;
+      if (g > 0) hi = mu;
+      double mu_new = mu - g / fmax(Z, 1e-18);
+      const double mu_bis = 0.5 * (lo + hi);
+      if (mu_new <= lo || mu_new >= hi) mu_new = mu_bis;
+      mu_new = fmin(fmax(mu_new, lo), hi);
+      if (!isfinite(mu_new)) mu_new = mu_bis;
+      a.st[HFX_ST_MU] = mu_new;
+      a.st[HFX_ST_MULO] = lo;
+      a.st[HFX_ST_MUHI] = hi;
+    }
+    *a.ticket = 0u;
+  }
+}
+
+__global__ __launch_bounds__(MUS_BLOCK) void mu_occ_kernel(MuStepArgs a) {
+  const size_t n = a.nk * a.nb, stride = (size_t)gridDim.x * MUS_BLOCK;
+  const double Tj = fmax(a.T, 1e-12), mu = a.st[HFX_ST_MU];
+  for (size_t e = (size_t)blockIdx.x * MUS_BLOCK + threadIdx.x; e < n; e += stride)
+    a.p_out[e] = sigm((mu - a.eps[e]) / Tj);   // solver.py:394, 592
+}
+
 static int g_mu_blocks = 0;
 
 // Co-resident grid for the cooperative launch, sized to ~8 eigenvalues per
@@ -391,7 +525,7 @@ using namespace hfx;
 extern "C" {
 
 size_t hfx_solver_scratch_doubles(void) {
-  return (size_t)RED_MAX_VALS * RED_MAX_BLOCKS + 8 * 512;
+  return (size_t)MU_TICKET_OFF + 8;
 }
 
 int hfx_reduce(const double* part, int nval, size_t nk, const double* w, unsigned wmask, double* out, double* st,
@@ -412,12 +546,36 @@ int hfx_reduce(const double* part, int nval, size_t nk, const double* w, unsigne
     pa.restart = (int)params[6];
     pa.last = (int)params[7];
     pa.omega0 = (int)params[8];
+    pa.restart_every = (int)params[9];
   }
-  hipLaunchKernelGGL(reduce_stage1, dim3(g), dim3(RED_BLOCK), 0, s, part, nval, nk, w, wmask, scratch);
-  int rc = check_launch("reduce_stage1");
-  if (rc) return rc;
-  hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(RED_BLOCK), 0, s, scratch, nval, g, out, st, pa);
-  return check_launch("reduce_stage2");
+  hipLaunchKernelGGL(reduce_fused, dim3(g), dim3(RED_BLOCK), 0, s, part, nval, nk, w, wmask, scratch, out, st, pa,
+                     reinterpret_cast<unsigned*>(scratch + RED_TICKET_OFF));
+  return check_launch("reduce_fused");
+}
+
+int hfx_solve_mu_steps(const double* eps, const double* w, size_t nk, int nb, double target, double T,
+                       double mu0, int mu0_from_state, int maxiter, double* st, double* p_out, double* scratch,
+                       void* stream) {
+  if (!eps || !w || !scratch || !st || nb < 1 || nk == 0 || maxiter < 0)
+    return set_error(HFX_EINVAL, "hfx_solve_mu_steps: bad argument");
+  const size_t n = nk * (size_t)nb;
+  size_t nbk = (n + (size_t)MUS_BLOCK * 4 - 1) / ((size_t)MUS_BLOCK * 4);
+  if (nbk > 512) nbk = 512;
+  if (nbk < 1) nbk = 1;
+  MuStepArgs a{eps, w, nk, nb, target, T, mu0, mu0_from_state, st, scratch + MU_PART_OFF,
+               reinterpret_cast<unsigned*>(scratch + MU_TICKET_OFF), p_out};
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(mu_bracket_kernel, dim3((unsigned)nbk), dim3(MUS_BLOCK), 0, s, a);
+  if (int rc = check_launch("mu_bracket_kernel")) return rc;
+  for (int it = 0; it < maxiter; ++it) {
+    hipLaunchKernelGGL(mu_step_kernel, dim3((unsigned)nbk), dim3(MUS_BLOCK), 0, s, a);
+    if (int rc = check_launch("mu_step_kernel")) return rc;
+  }
+  if (p_out) {
+    hipLaunchKernelGGL(mu_occ_kernel, dim3((unsigned)nbk), dim3(MUS_BLOCK), 0, s, a);
+    return check_launch("mu_occ_kernel");
+  }
+  return HFX_OK;
 }
 
 int hfx_solve_mu(const double* eps, const double* w, size_t nk, int nb, double target, double T, double mu0,

@@ -82,12 +82,12 @@ def diis_solve(B: np.ndarray, active: np.ndarray):
 
 
 class _SCF:
-    def __init__(self, kernel, cfg: SCFConfig, fock_build_fn=None):
+    def __init__(self, kernel, cfg: SCFConfig, fock_build_fn=None, dtype=None):
         from .solver import _Fock
         self.kernel = kernel
         self.cfg = cfg
         self.dev = kernel.device
-        self.cdt = kernel.dtype
+        self.cdt = dtype or kernel.dtype
         self.single = self.cdt == torch.complex64
         nk1, nk2, nb = kernel.h.shape[0], kernel.h.shape[1], kernel.h.shape[-1]
         self.shape4 = (nk1, nk2, nb, nb)

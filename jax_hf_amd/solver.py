@@ -36,7 +36,10 @@ __all__ = ["SolverConfig", "SolveResult", "solve", "solve_direct_minimization"]
 
 @dataclass(frozen=True)
 class SolverConfig:
-    """Solver controls (solver.py:240-286), plus ``precision``."""
+    """Solver controls (solver.py:240-286), plus ``precision``, ``mixed_tol_E`` and
+    ``use_graph`` (replay the CG iteration as a captured HIP graph; the launches
+    are identical to the eager loop) and ``cayley`` (how the CG line search
+    forms U(tau): per trial by Gauss-Jordan, or from one eigh of i d)."""
     max_iter: int = 200
     tol_E: float = 1e-7
     tol_grad: float = 0.0
@@ -57,6 +60,11 @@ class SolverConfig:
     project_fn: Any = None
     precision: str = "auto"
     mixed_tol_E: float = 1e-7   # relative windowed-energy stop of the FP32 phase (x max(1, |E|))
+    use_graph: bool = True
+    cayley: str = "gauss_jordan"   # or "spectral" (the reference's eigh-based form)
+    polish: str | None = None      # "diis": FP64 DIIS SCF to the fixed point after CG (default for mixed)
+    polish_tol: float = 1e-8       # density and commutator residual of the polish
+    polish_max_iter: int = 60
 
 
 class SolveResult(NamedTuple):
@@ -88,6 +96,10 @@ class _Fock:
         self.eng_nohs = None
         if self.block_specs is not None or fock_build_fn is not None:
             self._args = kernel.as_args()
+
+    def graph_safe(self, project_fn=None):
+        """True when a build is device launches only (no user callable, no host check)."""
+        return self.fn is None and self.block_specs is None and project_fn is None
 
     def build(self, P, F, E_ptr=None, project_fn=None, refP="kernel", sub_h=False):
         eng = self.eng
@@ -155,6 +167,10 @@ class _CG:
         v = lambda: torch.zeros((nk1, nk2, nb), **r)  # noqa: E731
         self.p, self.eps, self.gp, self.gpp, self.hp, self.dp, self.lam = v(), v(), v(), v(), v(), v(), v()
         self.st = torch.zeros(K.ST_LEN, **r)
+        form = str(cfg.cayley).lower()
+        if form not in ("gauss_jordan", "spectral"):
+            raise ValueError(f"cayley must be 'gauss_jordan' or 'spectral'; got {cfg.cayley!r}")
+        self.gauss_jordan = form == "gauss_jordan"
 
     # -- pieces ---------------------------------------------------------------
     def density(self):
@@ -162,10 +178,9 @@ class _CG:
         return _project_herm(self.P, self.cfg.project_fn, self.tmp)
 
     def mu(self, eps, from_state, mu0=0.0):
-        n_norm = self.n_norm
-        K.solve_mu(self.ws, eps, self.w_norm, self.nk, self.nb, n_norm, self.T, mu0=mu0,
-                   mu0_from_state=from_state, maxiter=self.cfg.mu_maxiter, mode=K.MU_NEWTON_BRACKET,
-                   st=self.st, p_out=self.p)
+        """_solve_mu (solver.py:62-101): Newton with bracket, graph-capturable launches."""
+        K.solve_mu_steps(self.ws, eps, self.w_norm, self.nk, self.nb, self.n_norm, self.T, mu0=mu0,
+                         mu0_from_state=from_state, maxiter=self.cfg.mu_maxiter, st=self.st, p_out=self.p)
 
     def set_target(self, n_electrons):
         self.n_norm = float(n_electrons) / max(self.kernel.weight_sum, 1e-30)
@@ -183,14 +198,10 @@ class _CG:
         self.p.copy_(p)
         self.st[K.ST_MU] = float(mu)
 
-    def run(self, n_electrons, tol_E, tol_grad, max_iter, rel_tol=False):
+    def run(self, n_electrons, tol_E, tol_grad, max_iter, rel_tol=False, abs_window=False):
         cfg = self.cfg
         self.set_target(n_electrons)
-        nk, nb = self.nk, self.nb
-        dt = K.dt_code(self.Q)
-        st, ws, part = self.st, self.ws, self.ws.part
-        Esp = K.st_ptr(st, K.ST_E)
-        sm = L.stream(self.dev)
+        st = self.st
         self.G.zero_()
         self.Gp.zero_()
         self.D.zero_()
@@ -202,6 +213,9 @@ class _CG:
         grad_norm, dE, E_prev = np.inf, np.inf, np.inf
         shrink_pow = cfg.bt_shrink ** cfg.bt_max
         tol_E_rel = tol_E
+        st[K.ST_ITER] = 0.0                 # device iteration counter (CG restart schedule)
+        use_graph = bool(cfg.use_graph) and self.fock.graph_safe(cfg.project_fn)
+        graphs, g_base = None, 0
         while True:
             # stopping rule (solver.py:422-449); rel_tol scales tol_E by |E| (FP32 phase)
             if rel_tol and hist_E:
@@ -209,39 +223,24 @@ class _CG:
             if cfg.plateau_window > 0:
                 e_now = hist_E[max(k - 1, 0)] if hist_E else 0.0
                 e_past = hist_E[max(k - 1 - cfg.plateau_window, 0)] if hist_E else 0.0
-                enc = (e_past - e_now) > tol_E if k > cfg.plateau_window else dE > tol_E
+                # abs_window (refinement phase of the mixed form): |change| over the
+                # window, since E may rise while the free energy falls when the
+                # loop restarts from a nearby, not fully consistent, state
+                win = abs(e_past - e_now) if abs_window else (e_past - e_now)
+                enc = win > tol_E if k > cfg.plateau_window else dE > tol_E
             else:
                 enc = dE > tol_E
             nc = grad_norm > tol_grad if tol_grad > 0.0 else enc
             if not (k < max_iter and nc):
                 break
-            P = self.density()                                                  # :462
-            self.fock.build(P, self.F, E_ptr=Esp)                               # :463-471
-            L.call("hfx_cg_grad", dt, nb, nk, L.ptr(self.Q), L.ptr(self.F), L.ptr(self.p), L.ptr(self.Ft),
-                   L.ptr(self.eps), L.ptr(self.G), L.ptr(part), sm)             # :472-478
-            K.reduce(ws, part, 2, nk, w=self.w_norm, wmask=0b01, st=st, op=K.OP_GRAD,
-                     params=[self.T, cfg.denom_scale, nk * nb])                 # :491-497
-            L.call("hfx_cg_precond", dt, nb, nk, L.ptr(self.G), L.ptr(self.Gp), L.ptr(self.eps), L.ptr(self.p),
-                   L.ptr(self.gpp), L.ptr(st), self.T, L.ptr(self.H), L.ptr(self.gp), L.ptr(self.hp),
-                   L.ptr(part), sm)                                             # :486, 499-504
-            restart = 1.0 if (k == 0 or k % cfg.cg_restart == 0) else 0.0
-            K.reduce(ws, part, 6, nk, w=self.w_norm, wmask=0b111111, st=st, op=K.OP_BETA,
-                     params=[0, 0, 0, 0, 0, 0, restart])                        # :517-526
-            L.call("hfx_cg_direction", dt, nb, nk, L.ptr(self.H), L.ptr(self.D), L.ptr(self.hp), L.ptr(self.dp),
-                   L.ptr(self.Ft), L.ptr(st), L.ptr(self.V), L.ptr(self.lam), L.ptr(self.Fe), L.ptr(part), sm)
-            K.reduce(ws, part, 1, nk, w=self.w_norm, wmask=1, st=st, op=K.OP_TAU0,
-                     params=[0, 0, 0, cfg.max_step, 0, shrink_pow])             # :543-547
-            for i in range(cfg.bt_max):                                         # :563-579
-                L.call("hfx_cg_trial", dt, nb, nk, L.ptr(self.V), L.ptr(self.lam), L.ptr(self.Fe), L.ptr(self.p),
-                       L.ptr(self.dp), L.ptr(self.eps), L.ptr(st), int(i == 0), L.ptr(part), sm)
-                K.reduce(ws, part, 4 if i == 0 else 2, nk, w=self.w_norm, wmask=0b1111, st=st, op=K.OP_TRIAL,
-                         params=[self.T, 0, 0, 0, cfg.bt_shrink, shrink_pow, 0, int(i == cfg.bt_max - 1),
-                                 int(i == 0)])
-            L.call("hfx_cg_retract", dt, nb, nk, L.ptr(self.Q), L.ptr(self.V), L.ptr(self.lam), L.ptr(self.Fe),
-                   L.ptr(st), 0.0, 0, L.ptr(self.eps), None, sm)                # :586-590
-            self.mu(self.eps, True)                                             # :591-592
-            self.G, self.Gp = self.Gp, self.G
-            self.gp, self.gpp = self.gpp, self.gp
+            if graphs is None and use_graph and k >= 1:
+                graphs = self._capture(shrink_pow)
+                g_base = k
+            if graphs is not None:
+                graphs[(k - g_base) % 2].replay()
+                self._swap()
+            else:
+                self._body(shrink_pow)
             sv = st.cpu().numpy()                                               # one readback
             E = float(sv[K.ST_E])
             grad_norm = float(sv[K.ST_GRAD])
@@ -252,6 +251,76 @@ class _CG:
             k += 1
         self.G, self.Gp = self.Gp, self.G   # last computed gradient back in self.G
         return k, grad_norm, hist_E, hist_G
+
+    def _swap(self):
+        self.G, self.Gp = self.Gp, self.G
+        self.gp, self.gpp = self.gpp, self.gp
+
+    def _body(self, shrink_pow):
+        """One CG iteration (solver.py:451-604) as device launches only: no host
+        read, no allocation, so the same sequence can be captured in a graph."""
+        cfg = self.cfg
+        nk, nb = self.nk, self.nb
+        dt = K.dt_code(self.Q)
+        st, ws, part = self.st, self.ws, self.ws.part
+        sm = L.stream(self.dev)
+        P = self.density()                                                      # :462
+        self.fock.build(P, self.F, E_ptr=K.st_ptr(st, K.ST_E))                  # :463-471
+        L.call("hfx_cg_grad", dt, nb, nk, L.ptr(self.Q), L.ptr(self.F), L.ptr(self.p), L.ptr(self.Ft),
+               L.ptr(self.eps), L.ptr(self.G), L.ptr(part), sm)                 # :472-478
+        K.reduce(ws, part, 2, nk, w=self.w_norm, wmask=0b01, st=st, op=K.OP_GRAD,
+                 params=[self.T, cfg.denom_scale, nk * nb])                     # :491-497
+        L.call("hfx_cg_precond", dt, nb, nk, L.ptr(self.G), L.ptr(self.Gp), L.ptr(self.eps), L.ptr(self.p),
+               L.ptr(self.gpp), L.ptr(st), self.T, L.ptr(self.H), L.ptr(self.gp), L.ptr(self.hp),
+               L.ptr(part), sm)                                                 # :486, 499-504
+        # restart when st[ITER] % cg_restart == 0 (k == 0 included), :522-526
+        K.reduce(ws, part, 6, nk, w=self.w_norm, wmask=0b111111, st=st, op=K.OP_BETA,
+                 params=[0, 0, 0, 0, 0, 0, 0, 0, 0, max(int(cfg.cg_restart), 1)])   # :517-526
+        gj = self.gauss_jordan
+        if gj:    # Cayley factor per trial by Gauss-Jordan: no eigh in the loop
+            L.call("hfx_cg_direction_gj", dt, nb, nk, L.ptr(self.H), L.ptr(self.D), L.ptr(self.hp),
+                   L.ptr(self.dp), L.ptr(st), L.ptr(part), sm)
+        else:     # spectral setup: eigh(i d) once, V^dagger Ft V  (:538-539)
+            L.call("hfx_cg_direction", dt, nb, nk, L.ptr(self.H), L.ptr(self.D), L.ptr(self.hp), L.ptr(self.dp),
+                   L.ptr(self.Ft), L.ptr(st), L.ptr(self.V), L.ptr(self.lam), L.ptr(self.Fe), L.ptr(part), sm)
+        K.reduce(ws, part, 1, nk, w=self.w_norm, wmask=1, st=st, op=K.OP_TAU0,
+                 params=[0, 0, 0, cfg.max_step, 0, shrink_pow])                 # :543-547
+        for i in range(cfg.bt_max):                                             # :563-579
+            if gj:
+                L.call("hfx_cg_trial_gj", dt, nb, nk, L.ptr(self.D), L.ptr(self.Ft), L.ptr(self.p),
+                       L.ptr(self.dp), L.ptr(self.eps), L.ptr(st), int(i == 0), L.ptr(part),
+                       L.ptr(self.V), L.ptr(self.lam), sm)
+            else:
+                L.call("hfx_cg_trial", dt, nb, nk, L.ptr(self.V), L.ptr(self.lam), L.ptr(self.Fe), L.ptr(self.p),
+                       L.ptr(self.dp), L.ptr(self.eps), L.ptr(st), int(i == 0), L.ptr(part), sm)
+            K.reduce(ws, part, 4 if i == 0 else 2, nk, w=self.w_norm, wmask=0b1111, st=st, op=K.OP_TRIAL,
+                     params=[self.T, 0, 0, 0, cfg.bt_shrink, shrink_pow, 0, int(i == cfg.bt_max - 1),
+                             int(i == 0)])
+        if gj:
+            L.call("hfx_cg_retract_gj", dt, nb, nk, L.ptr(self.Q), L.ptr(self.D), L.ptr(self.Ft), L.ptr(st),
+                   L.ptr(self.eps), L.ptr(self.V), L.ptr(self.lam), sm)         # :586-590
+        else:
+            L.call("hfx_cg_retract", dt, nb, nk, L.ptr(self.Q), L.ptr(self.V), L.ptr(self.lam), L.ptr(self.Fe),
+                   L.ptr(st), 0.0, 0, L.ptr(self.eps), None, sm)                # :586-590
+        self.mu(self.eps, True)                                                 # :591-592
+        self._swap()
+
+    def _capture(self, shrink_pow):
+        """Two HIP graphs of the iteration body, one per parity of the G/Gp swap."""
+        graphs = []
+        cur = torch.cuda.current_stream(self.dev)
+        cap = torch.cuda.Stream(self.dev)          # default priority
+        cap.wait_stream(cur)
+        with torch.cuda.stream(cap):
+            for _ in range(2):
+                g = torch.cuda.CUDAGraph()
+                g.capture_begin()
+                self._body(shrink_pow)
+                g.capture_end()
+                graphs.append(g)
+        cur.wait_stream(cap)
+        # each capture swapped the Python references once: two swaps restore them
+        return graphs
 
     def finalize(self):
         """solver.py:606-638: Fock, eigh canonicalisation, mu, p, Fock + E."""
@@ -270,35 +339,62 @@ class _CG:
 def _solve_cg(kernel, P0, n_electrons, cfg: SolverConfig, fock_build_fn):
     prec = _precision(kernel, cfg)
     dev = kernel.device
+    polish = cfg.polish if cfg.polish is not None else ("diis" if prec == "mixed" else None)
+    if polish not in (None, "diis"):
+        raise ValueError(f"polish must be None or 'diis'; got {cfg.polish!r}")
+    cdt = torch.complex128 if prec == "fp64" else torch.complex64
+    runner = _CG(kernel, cdt, cfg, fock_build_fn)
+    runner.set_target(n_electrons)
+    runner.init_from(P0)
     if prec == "mixed":
-        lo = _CG(kernel, torch.complex64, cfg, fock_build_fn)
-        lo.set_target(n_electrons)
-        lo.init_from(P0)
-        # the FP32 phase gets at most half of the iteration budget
-        k1, g1, hE1, hG1 = lo.run(n_electrons, cfg.mixed_tol_E, 0.0, max(cfg.max_iter // 2, 1), rel_tol=True)
-        # FP64 refinement restarts from the mixed density: eigh of the FP64 Fock
-        # matrix gives exactly unitary orbitals (an FP32 Q is unitary only to ~1e-7)
-        P_lo = lo.density()
-        hi = _CG(kernel, torch.complex128, cfg, fock_build_fn)
-        hi.set_target(n_electrons)
-        hi.init_from(convert(P_lo, torch.complex128))
-        del lo
-        k2, grad, hE2, hG2 = hi.run(n_electrons, cfg.tol_E, cfg.tol_grad, max(cfg.max_iter - k1, 0))
-        k, hist_E, hist_G = k1 + k2, hE1 + hE2, hG1 + hG2
-        runner = hi
+        # FP32 phase: relative windowed stop, at most half of the iteration budget
+        k, grad, hist_E, hist_G = runner.run(n_electrons, cfg.mixed_tol_E, 0.0, max(cfg.max_iter // 2, 1),
+                                             rel_tol=True)
     else:
-        cdt = torch.complex128 if prec == "fp64" else torch.complex64
-        runner = _CG(kernel, cdt, cfg, fock_build_fn)
-        runner.set_target(n_electrons)
-        runner.init_from(P0)
         k, grad, hist_E, hist_G = runner.run(n_electrons, cfg.tol_E, cfg.tol_grad, cfg.max_iter)
+    if polish == "diis":
+        return _polish_diis(kernel, runner, n_electrons, cfg, fock_build_fn, k, hist_E, hist_G)
     Pfin, Ffin = runner.finalize()
     converged = (grad <= cfg.tol_grad) if cfg.tol_grad > 0.0 else (k < cfg.max_iter)
     sv = runner.st.cpu()
-    res = _result(runner, Pfin, Ffin, sv, k, converged, hist_E, hist_G, cfg.max_iter, dev)
-    if prec == "mixed":
-        res.history["phase_iters"] = torch.tensor([k1, k2], dtype=torch.int64)
-    return res
+    return _result(runner, Pfin, Ffin, sv, k, converged, hist_E, hist_G, cfg.max_iter, dev)
+
+
+def _polish_diis(kernel, runner, n_electrons, cfg, fock_build_fn, k_cg, hist_E, hist_G):
+    """FP64 Pulay-DIIS SCF from the CG density to the fixed point [F, P] = 0.
+
+    The CG loop's windowed energy stop (solver.py:422-449) can end well before
+    stationarity (E keeps drifting while the free energy still falls); the
+    mixed form therefore finishes with the reference's own DIIS SCF
+    (reference_scf.py:298-314) in FP64, to residuals <= polish_tol, so its
+    energy is the FP64 fixed point's, independent of the FP32 trajectory.
+    """
+    from .reference_scf import SCFConfig, _SCF
+    dev = kernel.device
+    P_lo = runner.density()
+    P64 = convert(P_lo, torch.complex128) if P_lo.dtype != torch.complex128 else P_lo.clone()
+    scfg = SCFConfig(max_iter=cfg.polish_max_iter, acceleration="diis", density_tol=cfg.polish_tol,
+                     comm_tol=cfg.polish_tol, project_fn=cfg.project_fn)
+    scf = _SCF(kernel, scfg, fock_build_fn, dtype=torch.complex128)
+    density, F, E, mu, n2, conv, hE2, hD2, hC2 = scf.run(P64, n_electrons)
+    # orbitals / occupations of the returned state (solver.py:606-638 form)
+    eps = torch.empty((F.shape[0], F.shape[1], F.shape[-1]), dtype=torch.float64, device=dev)
+    Q = torch.empty_like(F)
+    K.eigh(F, eps, Q, herm_in=True)
+    p = torch.empty_like(eps)
+    L.call("hfx_fermi", L.ptr(eps), L.ptr(mu), float(kernel.T), eps.numel(), L.ptr(p),
+           L.stream(dev))
+    k = k_cg + n2
+    hE = list(hist_E) + list(hE2)
+    hG = list(hist_G) + list(hC2)
+    hEt = torch.zeros(max(cfg.max_iter, k), dtype=torch.float64)
+    hGt = torch.zeros_like(hEt)
+    hEt[:k] = torch.tensor(hE[:k], dtype=torch.float64)
+    hGt[:k] = torch.tensor(hG[:k], dtype=torch.float64)
+    hist = dict(E=hEt.to(dev), grad_norm=hGt.to(dev),
+                phase_iters=torch.tensor([k_cg, n2], dtype=torch.int64))
+    return SolveResult(Q=Q, p=p, mu=mu.to(dev), density=density, fock=F, energy=E, n_iter=int(k),
+                       converged=bool(conv), history=hist)
 
 
 def _result(runner, Pfin, Ffin, sv, k, converged, hist_E, hist_G, max_iter, dev):

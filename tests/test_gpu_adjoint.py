@@ -17,6 +17,10 @@ from jax_hf_amd import synthetic as syn
 pytestmark = pytest.mark.gpu
 
 
+def _np(t):
+    return t.detach().cpu().numpy() if hasattr(t, "detach") else np.asarray(t)
+
+
 def _torch_energy_cpu(P, h, VR, w, refP=None):
     """CPU torch restatement of fock.py:51-64, 94, 109-111 (exchange only)."""
     import torch
@@ -103,12 +107,8 @@ def test_adjoint_hartree_term(dev):
     assert abs(fd - an) <= 1e-9 * max(1.0, abs(an))
 
 
-def test_stationary_parameter_gradients_vs_finite_differences(dev):
-    import jax_hf_amd as J
-    from jax_hf_amd.adjoint import free_energy_gradients
-    nk, nb, T = 12, 4, 0.1
-    U0, q0, d_, u = 5.0, 0.3, 0.5, 0.2
-    h0 = syn.tight_binding(nk, nk, nb, 4)
+def _layer_setup(nk, nb, U0, q0, d_, seed):
+    h0 = syn.tight_binding(nk, nk, nb, seed)
     D = syn.displacement(nb, 1.0)
     w = syn.uniform_weights(nk, nk)
     q = syn.qnorm_min_image(nk, nk)[..., None, None]
@@ -118,16 +118,59 @@ def test_stationary_parameter_gradients_vs_finite_differences(dev):
     def Vq(U, q0_):
         return U * np.exp(-q * d_ * dl) / (q + q0_)
 
+    return h0, D, w, q, Vq
+
+
+def test_explicit_parameter_gradients_at_fixed_density(dev):
+    """d E(P; theta)/d theta at fixed P (U0, q0 through V(q); u through h) against
+    central differences of the GPU energy: the adjoint's parameter part alone."""
+    import jax_hf_amd as J
+    from jax_hf_amd.adjoint import free_energy_gradients
+    nk, nb, T = 12, 4, 0.1
+    U0, q0, d_, u = 5.0, 0.3, 0.5, 0.2
+    h0, D, w, q, Vq = _layer_setup(nk, nb, U0, q0, d_, 4)
+    P = syn.haar_density(nk, nk, nb, np.random.default_rng(9))
+
+    def energy(U, q0_, uu):
+        G = J.HartreeFockKernel(w, h0 + uu * D, Vq(U, q0_), T)
+        a = G.as_args()
+        S, H, F = J.build_fock(P, h=a["h"], VR=a["VR"], refP=a["refP"], HH=a["HH"], w2d=G.w2d,
+                               include_exchange=True, include_hartree=False,
+                               exchange_hermitian_channel_packing=a["exchange_hermitian_channel_packing"])
+        return float(J.hf_energy(P, h=G.h, Sigma=S, H=H, weights_b=G.weights_b)), G
+
+    _, G = energy(U0, q0, u)
+    grads = free_energy_gradients(G, P, dVq={"U0": Vq(U0, q0) / U0, "q0": -Vq(U0, q0) / (q + q0)},
+                                  dh={"u": D})
+    step = 1e-5
+    fds = {"U0": (energy(U0 + step, q0, u)[0] - energy(U0 - step, q0, u)[0]) / (2 * step),
+           "q0": (energy(U0, q0 + step, u)[0] - energy(U0, q0 - step, u)[0]) / (2 * step),
+           "u": (energy(U0, q0, u + step)[0] - energy(U0, q0, u - step)[0]) / (2 * step)}
+    for k, fd in fds.items():
+        assert abs(grads[k] - fd) <= 1e-8 * max(abs(fd), 1e-3), (k, grads[k], fd)
+
+
+def test_stationary_parameter_gradients_vs_finite_differences(dev):
+    """dOmega/dtheta of the converged free energy (Hellmann-Feynman at an SCF
+    fixed point, [F, P] ~ 1e-13) against central differences of re-converged
+    solves, 1e-6 relative."""
+    import jax_hf_amd as J
+    from jax_hf_amd.adjoint import free_energy_gradients
+    nk, nb, T = 12, 4, 0.1
+    U0, q0, d_, u = 5.0, 0.3, 0.5, 0.2
+    h0, D, w, q, Vq = _layer_setup(nk, nb, U0, q0, d_, 4)
+    cfg = J.SCFConfig(max_iter=300, acceleration="diis", density_tol=1e-12, comm_tol=1e-11)
+
     def omega(U, q0_, uu):
         G = J.HartreeFockKernel(w, h0 + uu * D, Vq(U, q0_), T)
-        r = J.solve(G, np.zeros_like(h0), nb / 2.0,
-                    config=J.SolverConfig(optimizer="newton", tol_grad=1e-9, max_iter=200))
+        r = J.solve_scf(G, np.zeros_like(h0), nb / 2.0, config=cfg)
         assert r.converged
-        return float(J.free_energy(r.energy, r.p, w, T)), G, r
+        P = _np(r.density_matrix)
+        p = np.linalg.eigvalsh(0.5 * (P + np.conj(np.swapaxes(P, -1, -2))))
+        return float(J.free_energy(float(r.energy), p, w, T)), G, P
 
-    _, G, r = omega(U0, q0, u)
-    grads = free_energy_gradients(G, r.density,
-                                  dVq={"U0": Vq(U0, q0) / U0, "q0": -Vq(U0, q0) / (q + q0)},
+    _, G, P = omega(U0, q0, u)
+    grads = free_energy_gradients(G, P, dVq={"U0": Vq(U0, q0) / U0, "q0": -Vq(U0, q0) / (q + q0)},
                                   dh={"u": D})
     step = 1e-5
     fds = {
