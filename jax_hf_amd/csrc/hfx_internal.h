@@ -40,5 +40,6 @@ int exchange_launch(const hfx_plan* plan, const void* P, const void* refP, const
                     const int* chans, int nchan, int herm, const double* w2d, void* Sigma, void* rho,
                     int mode, int shift, int conjv, hipStream_t s);
 int exchange_max_plane(int dtype);
+int eigh16_launch(int dtype, int nb, size_t nk, const void* A, double* lam, void* V, int herm_in, hipStream_t s);
 int exchange_lds_elems(int n1, int n2);
 }  // namespace hfx

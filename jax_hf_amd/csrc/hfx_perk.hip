@@ -939,12 +939,7 @@ int hfx_eigh_batched(int dtype, int nb, size_t nk, const void* A, double* lam, v
   if (int rc = pk_check_nb(nb)) return rc;
   if (!A || !lam || !V) return set_error(HFX_EINVAL, "hfx_eigh_batched: NULL argument");
   if (nk == 0) return HFX_OK;
-  hipStream_t s = (hipStream_t)stream;
-  if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_eigh_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKC(double, A), lam, PKM(double, V), herm_in);
-  else
-    hipLaunchKernelGGL(pk_eigh_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKC(float, A), lam, PKM(float, V), herm_in);
-  return check_launch("pk_eigh_kernel");
+  return eigh16_launch(dtype, nb, nk, A, lam, V, herm_in, (hipStream_t)stream);
 }
 
 int hfx_density_from_Qp(int dtype, int nb, size_t nk, const void* Q, const double* p, const double* eps,

@@ -326,8 +326,11 @@ class _CG:
         """solver.py:606-638: Fock, eigh canonicalisation, mu, p, Fock + E."""
         P = self.density()
         self.fock.build(P, self.F)
-        L.call("hfx_rotate_eigh", K.dt_code(self.Q), self.nb, self.nk, L.ptr(self.Q), L.ptr(self.F),
-               L.ptr(self.eps), L.stream(self.dev))
+        # Q <- Q W, (eps, W) = eigh(Q^dagger F Q)  (:622-625), with the register eigensolver
+        K.matmul(self.Q, self.F, self.Ft, C=self.Q, ops=0b001)              # Ft = Q^H F Q
+        K.eigh(self.Ft, self.eps, self.V, herm_in=True)
+        K.matmul(self.Q, self.V, self.H)                                      # Q W
+        self.Q, self.H = self.H, self.Q
         self.mu(self.eps, True)
         P = self.density()
         Pout = P.clone()
