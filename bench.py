@@ -64,6 +64,22 @@ def parse():
 # Fock build
 # ---------------------------------------------------------------------------
 
+PMC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                           "exchange_traffic_96x96_nb16_pmc.md")
+
+
+def _pmc_traffic(kname, nk, nb):
+    """HBM bytes per exchange launch from the committed FETCH_SIZE / WRITE_SIZE
+    summary (counters scaled by the copy-kernel calibration), for the 96x96,
+    nb=16 workload only; None otherwise."""
+    if (nk, nb) != (96, 16) or not os.path.exists(PMC_SUMMARY):
+        return None
+    for line in open(PMC_SUMMARY):
+        if line.startswith(f"traffic_bytes_per_launch {kname}:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def fock_bench(J, L, syn, inp, dev, nk, nb, args):
     import torch
     G = J.HartreeFockKernel(inp["weights"], inp["hamiltonian"], inp["coulomb_q"], inp["T"], device=dev)
@@ -108,7 +124,8 @@ def fock_bench(J, L, syn, inp, dev, nk, nb, args):
            "energy": float(E), "exchange_variant": L.Plan.VARIANTS[eng.plan.exchange_variant],
            "exchange_variants_ms": variant_ms}
     roof = {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": bytes_exch, "avg_ms": exch_ms}
+            "frac": achieved / HBM_PEAK_GBS, "traffic": _pmc_traffic(kname, nk, nb),
+            "algorithmic_bytes": bytes_exch, "avg_ms": exch_ms}
     return out, roof, F
 
 

@@ -498,181 +498,6 @@ __global__ __launch_bounds__(MUS_BLOCK) void mu_occ_kernel(MuStepArgs a) {
     a.p_out[e] = sigm((mu - a.eps[e]) / Tj);   // solver.py:394, 592
 }
 
-// ---- bisection (utils.py:353-382) by multisection passes ----------------------
-// Each launch advances BIS_LEVELS bisection steps at once: every thread counts
-// the electrons at all 2^L - 1 midpoints of the next L levels of the bisection
-// tree of [lo, hi] (each node's midpoint is formed by the same 0.5 (l + h)
-// arithmetic as the sequential loop), and the last block walks the tree with
-// the totals.  The resulting bracket is the sequential bisection's, and once
-// the midpoint equals an end point the bracket no longer moves, which ends the
-// remaining passes.  Optional level shift (reference_scf.py:87-96): a second
-// bisection on x + shift (1 - f(x; mu_raw)), mu_raw from the first.
-constexpr int BIS_LEVELS = 5;
-constexpr int BIS_NODES = (1 << BIS_LEVELS) - 1;
-constexpr int BIS_MAX_BLOCKS = 128;
-constexpr size_t BIS_TICKET_OFF = MU_TICKET_OFF + 2;
-constexpr size_t BIS_STATE_OFF = MU_TICKET_OFF + 4;   // lo, hi, done, mu_raw
-
-struct BisArgs {
-  const double* eps;
-  const double* w;
-  size_t nk;
-  int nb;
-  double target, T, level_shift;
-  int use_shift, levels;
-  double* state;       // [lo, hi, done, mu_raw]
-  double* part;        // BIS_NODES * gridDim.x
-  unsigned* ticket;
-  double* mu_out;
-  double* p_out;
-};
-
-__device__ __forceinline__ double bis_value(const BisArgs& a, size_t e, double mu_raw) {
-  const double x = a.eps[e];
-  if (!a.use_shift) return x;
-  const double f = 1.0 / (1.0 + exp((x - mu_raw) / (a.T + 1e-12)));   // fermidirac(x - mu_raw)
-  return x + a.level_shift * (1.0 - f);
-}
-
-__global__ __launch_bounds__(MUS_BLOCK) void bis_bracket_kernel(BisArgs a) {
-  __shared__ double sh[2 * (MUS_BLOCK / 64)];
-  const size_t n = a.nk * a.nb, stride = (size_t)gridDim.x * MUS_BLOCK;
-  const double mu_raw = a.state[3];
-  double lo = INFINITY, hi = -INFINITY;
-  for (size_t e = (size_t)blockIdx.x * MUS_BLOCK + threadIdx.x; e < n; e += stride) {
-    const double v = bis_value(a, e, mu_raw);
-    lo = fmin(lo, v);
-    hi = fmax(hi, v);
-  }
-  lo = wave_min(lo);
-  hi = wave_max(hi);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = MUS_BLOCK / 64;
-  if (lane == 0) { sh[wid] = lo; sh[nw + wid] = hi; }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int q = 0; q < nw; ++q) { lo = fmin(lo, sh[q]); hi = fmax(hi, sh[nw + q]); }
-    a.part[2 * blockIdx.x] = lo;
-    a.part[2 * blockIdx.x + 1] = hi;
-  }
-  if (!last_block_done(a.ticket)) return;
-  if (threadIdx.x == 0) {
-    double mn = INFINITY, mx = -INFINITY;
-    for (unsigned b = 0; b < gridDim.x; ++b) { mn = fmin(mn, a.part[2 * b]); mx = fmax(mx, a.part[2 * b + 1]); }
-    const double span = mx - mn + 10.0 * fmax(a.T, 1e-6);   // utils.py:360-363
-    a.state[0] = mn - span;
-    a.state[1] = mx + span;
-    a.state[2] = 0.0;
-    *a.ticket = 0u;
-  }
-}
-
-__global__ __launch_bounds__(MUS_BLOCK) void bis_pass_kernel(BisArgs a) {
-  if (a.state[2] != 0.0) return;
-  __shared__ double sh[BIS_NODES * (MUS_BLOCK / 64)];
-  const size_t n = a.nk * a.nb, stride = (size_t)gridDim.x * MUS_BLOCK;
-  const double mu_raw = a.state[3];
-  const int L = a.levels, nodes = (1 << L) - 1;
-  // midpoints of the tree nodes (heap order, node 1 = the current bracket)
-  double mid[BIS_NODES];
-  {
-    double lo_[BIS_NODES + 1], hi_[BIS_NODES + 1];
-    lo_[1] = a.state[0];
-    hi_[1] = a.state[1];
-#pragma unroll
-    for (int q = 1; q <= BIS_NODES; ++q) {
-      const double m = 0.5 * (lo_[q] + hi_[q]);
-      mid[q - 1] = m;
-      if (2 * q + 1 <= BIS_NODES) {
-        lo_[2 * q] = lo_[q];
-        hi_[2 * q] = m;
-        lo_[2 * q + 1] = m;
-        hi_[2 * q + 1] = hi_[q];
-      }
-    }
-  }
-  double cnt[BIS_NODES];
-#pragma unroll
-  for (int q = 0; q < BIS_NODES; ++q) cnt[q] = 0.0;
-  const double Tj = a.T + 1e-12;
-  for (size_t e = (size_t)blockIdx.x * MUS_BLOCK + threadIdx.x; e < n; e += stride) {
-    const double v = bis_value(a, e, mu_raw), wk = a.w[e / a.nb];
-#pragma unroll
-    for (int q = 0; q < BIS_NODES; ++q)
-      if (q < nodes) cnt[q] += wk * (1.0 / (1.0 + exp((v - mid[q]) / Tj)));   // fermidirac(v - mid)
-  }
-  block_sum<BIS_NODES>(cnt, sh);
-  if (threadIdx.x == 0)
-    for (int q = 0; q < nodes; ++q) a.part[(size_t)q * gridDim.x + blockIdx.x] = cnt[q];
-  if (!last_block_done(a.ticket)) return;
-  // totals per node in a fixed order, then the walk down the tree
-  double tot[BIS_NODES];
-#pragma unroll
-  for (int q = 0; q < BIS_NODES; ++q) tot[q] = 0.0;
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += MUS_BLOCK)
-#pragma unroll
-    for (int q = 0; q < BIS_NODES; ++q)
-      if (q < nodes) tot[q] += a.part[(size_t)q * gridDim.x + b];
-  block_sum<BIS_NODES>(tot, sh);
-  if (threadIdx.x == 0) {
-    double lo = a.state[0], hi = a.state[1];
-    int node = 1;
-    for (int l = 0; l < L; ++l) {
-      const double m = 0.5 * (lo + hi);
-      if (m == lo || m == hi) {   // collapsed: later steps would change nothing
-        a.state[2] = 1.0;
-        break;
-      }
-      double c = 0.0;   // tot[node - 1], selected without a dynamic register index
-#pragma unroll
-      for (int q = 0; q < BIS_NODES; ++q) c = (q == node - 1) ? tot[q] : c;
-      if (c > a.target) { hi = m; node = 2 * node; }
-      else { lo = m; node = 2 * node + 1; }
-    }
-    a.state[0] = lo;
-    a.state[1] = hi;
-    *a.ticket = 0u;
-  }
-}
-
-__global__ __launch_bounds__(MUS_BLOCK) void bis_finish_kernel(BisArgs a, int final_pass) {
-  const double mu = 0.5 * (a.state[0] + a.state[1]);
-  if (!final_pass) {   // first of two bisections: keep mu_raw for the shifted levels
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.state[3] = mu;
-    return;
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0 && a.mu_out) *a.mu_out = mu;
-  if (!a.p_out) return;
-  const size_t n = a.nk * a.nb, stride = (size_t)gridDim.x * MUS_BLOCK;
-  const double mu_raw = a.state[3];
-  for (size_t e = (size_t)blockIdx.x * MUS_BLOCK + threadIdx.x; e < n; e += stride)
-    a.p_out[e] = 1.0 / (1.0 + exp((bis_value(a, e, mu_raw) - mu) / (a.T + 1e-12)));
-}
-
-static int bisect_steps(const double* eps, const double* w, size_t nk, int nb, double target, double T,
-                        int maxiter, double level_shift, double* mu_out, double* p_out, double* scratch,
-                        hipStream_t s) {
-  const size_t n = nk * (size_t)nb;
-  size_t nbk = (n + (size_t)MUS_BLOCK * 4 - 1) / ((size_t)MUS_BLOCK * 4);
-  if (nbk > BIS_MAX_BLOCKS) nbk = BIS_MAX_BLOCKS;
-  if (nbk < 1) nbk = 1;
-  BisArgs a{eps, w, nk, nb, target, T, level_shift, 0, BIS_LEVELS, scratch + BIS_STATE_OFF, scratch + MU_PART_OFF,
-            reinterpret_cast<unsigned*>(scratch + BIS_TICKET_OFF), mu_out, p_out};
-  const int rounds = level_shift != 0.0 ? 2 : 1;
-  for (int r = 0; r < rounds; ++r) {
-    a.use_shift = r;
-    hipLaunchKernelGGL(bis_bracket_kernel, dim3((unsigned)nbk), dim3(MUS_BLOCK), 0, s, a);
-    if (int rc = check_launch("bis_bracket_kernel")) return rc;
-    for (int done = 0; done < maxiter; done += BIS_LEVELS) {
-      a.levels = maxiter - done < BIS_LEVELS ? maxiter - done : BIS_LEVELS;
-      hipLaunchKernelGGL(bis_pass_kernel, dim3((unsigned)nbk), dim3(MUS_BLOCK), 0, s, a);
-      if (int rc = check_launch("bis_pass_kernel")) return rc;
-    }
-    hipLaunchKernelGGL(bis_finish_kernel, dim3((unsigned)nbk), dim3(MUS_BLOCK), 0, s, a, r == rounds - 1 ? 1 : 0);
-    if (int rc = check_launch("bis_finish_kernel")) return rc;
-  }
-  return HFX_OK;
-}
-
 static int g_mu_blocks = 0;
 
 // Co-resident grid for the cooperative launch, sized to ~8 eigenvalues per
@@ -760,15 +585,6 @@ int hfx_solve_mu(const double* eps, const double* w, size_t nk, int nb, double t
   if (mode != HFX_MU_NEWTON_BRACKET && mode != HFX_MU_BISECTION && mode != HFX_MU_NEWTON)
     return set_error(HFX_EINVAL, "hfx_solve_mu: unknown mode %d", mode);
   if (mu0_from_state && !st) return set_error(HFX_EINVAL, "hfx_solve_mu: mu0_from_state needs st");
-  if (mode == HFX_MU_BISECTION) {
-    // result to st[HFX_ST_MU] as well, when given
-    double* dst = mu_out ? mu_out : (st ? st + HFX_ST_MU : nullptr);
-    int rc = bisect_steps(eps, w, nk, nb, target, T, maxiter, level_shift, dst, p_out, scratch, (hipStream_t)stream);
-    if (rc || !(mu_out && st)) return rc;
-    return hipMemcpyAsync(st + HFX_ST_MU, mu_out, sizeof(double), hipMemcpyDeviceToDevice,
-                          (hipStream_t)stream) == hipSuccess ? HFX_OK
-                                                            : set_error(HFX_EHIP, "hfx_solve_mu: copy failed");
-  }
   MuArgs a{eps, w, nk, nb, target, T, mu0, level_shift, maxiter, mode, mu0_from_state, st, mu_out, p_out, scratch};
   const int blocks = mu_blocks(nk * (size_t)nb);
   void* args[] = {&a};

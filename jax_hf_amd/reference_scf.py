@@ -82,8 +82,10 @@ def diis_solve(B: np.ndarray, active: np.ndarray):
 
 
 class _SCF:
-    def __init__(self, kernel, cfg: SCFConfig, fock_build_fn=None, dtype=None):
+    def __init__(self, kernel, cfg: SCFConfig, fock_build_fn=None, dtype=None, mu_newton=False):
         from .solver import _Fock
+        self.mu_newton = bool(mu_newton)
+        self.mu_warm = False
         self.kernel = kernel
         self.cfg = cfg
         self.dev = kernel.device
@@ -114,11 +116,23 @@ class _SCF:
         return out
 
     def density_from_fock(self, F, P_out, mu_slot):
-        """reference_scf.py:87-96: eigh(herm F), two bisections with level shift, P."""
+        """reference_scf.py:87-96: eigh(herm F), two bisections with level shift, P.
+
+        With ``mu_newton`` (the FP64 polish of the mixed CG form, level shift 0)
+        mu comes from the safeguarded Newton solver warm-started at the previous
+        mu instead: the same root to rounding, in a few steps instead of 54.
+        """
         K.eigh(F, self.eps, self.U, herm_in=True)
-        K.solve_mu(self.wsp, self.eps, self.w, self.nk, self.nb, self.n_e, self.kernel.T,
-                   maxiter=30 if self.single else 54, mode=K.MU_BISECTION,
-                   level_shift=self.cfg.level_shift, mu_out_ptr=K.st_ptr(self.st, mu_slot), p_out=self.occ)
+        if self.mu_newton and self.cfg.level_shift == 0.0:
+            K.solve_mu_steps(self.wsp, self.eps, self.w, self.nk, self.nb, self.n_e, self.kernel.T,
+                             mu0_from_state=self.mu_warm, maxiter=54, st=self.st, p_out=self.occ)
+            self.mu_warm = True
+            if mu_slot != K.ST_MU:
+                self.st[mu_slot] = self.st[K.ST_MU]
+        else:
+            K.solve_mu(self.wsp, self.eps, self.w, self.nk, self.nb, self.n_e, self.kernel.T,
+                       maxiter=30 if self.single else 54, mode=K.MU_BISECTION,
+                       level_shift=self.cfg.level_shift, mu_out_ptr=K.st_ptr(self.st, mu_slot), p_out=self.occ)
         K.density(self.U, self.tmp, p=self.occ)
         return self.herm_project(self.tmp, P_out)
 

@@ -378,7 +378,7 @@ def _polish_diis(kernel, runner, n_electrons, cfg, fock_build_fn, k_cg, hist_E, 
     P64 = convert(P_lo, torch.complex128) if P_lo.dtype != torch.complex128 else P_lo.clone()
     scfg = SCFConfig(max_iter=cfg.polish_max_iter, acceleration="diis", density_tol=cfg.polish_tol,
                      comm_tol=cfg.polish_tol, project_fn=cfg.project_fn)
-    scf = _SCF(kernel, scfg, fock_build_fn, dtype=torch.complex128)
+    scf = _SCF(kernel, scfg, fock_build_fn, dtype=torch.complex128, mu_newton=True)
     density, F, E, mu, n2, conv, hE2, hD2, hC2 = scf.run(P64, n_electrons)
     # orbitals / occupations of the returned state (solver.py:606-638 form)
     eps = torch.empty((F.shape[0], F.shape[1], F.shape[-1]), dtype=torch.float64, device=dev)
