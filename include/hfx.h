@@ -180,7 +180,8 @@ int hfx_entropy(const void* p, int real_kind, const double* w, size_t nk, int nb
 #define HFX_ST_MULO 13   /* mu bracket and done flag of hfx_solve_mu_steps */
 #define HFX_ST_MUHI 14
 #define HFX_ST_MUDONE 15
-#define HFX_ST_LEN 16
+#define HFX_ST_FROZEN 16 /* batched solves: a converged solve's kernels skip their updates */
+#define HFX_ST_LEN 32
 
 /* Batched Hermitian eigensolver (jnp.linalg.eigh call sites, solver.py:165,
  * 389, 623, 842; reference_scf.py:89): cyclic round-robin Jacobi, eigenvalues
@@ -199,7 +200,7 @@ int hfx_cg_grad(int dtype, int nb, size_t nk, const void* Q, const void* F, cons
                 double* eps, void* G, double* part, void* stream);
 int hfx_cg_precond(int dtype, int nb, size_t nk, const void* G, const void* Gprev, const double* eps,
                    const double* p, const double* gp_prev, const double* st, double T, void* H, double* gp,
-                   double* hp, double* part, void* stream);
+                   double* hp, double* part, int seg_len, void* stream);
 int hfx_cg_direction(int dtype, int nb, size_t nk, const void* H, void* D, const double* hp, double* dp,
                      const void* Ft, const double* st, void* V, double* lam_d, void* Fe, double* part,
                      void* stream);
@@ -219,15 +220,18 @@ int hfx_cg_retract(int dtype, int nb, size_t nk, void* Q, const void* V, const d
  * part = |D|^2; trial: as hfx_cg_trial from D and Ft; retract: Q <- Q U(st
  * [HFX_ST_TAUF]), eps_new = diag(U^dagger Ft U)  (solver.py:528-592). */
 int hfx_cg_direction_gj(int dtype, int nb, size_t nk, const void* H, void* D, const double* hp, double* dp,
-                        const double* st, double* part, void* stream);
+                        const double* st, double* part, int seg_len, void* stream);
 int hfx_cg_trial_gj(int dtype, int nb, size_t nk, const void* D, const void* Ft, const double* p,
                     const double* dp, const double* eps, const double* st, int omega0, double* part,
-                    void* U_keep, double* eps_keep, void* stream);
+                    void* U_keep, double* eps_keep, int seg_len, void* stream);
 /* U_keep / eps_keep (optional, both or neither): the last evaluated trial's U
  * and diag(U^dagger Ft U); the retraction reuses them when a trial was
  * accepted (st[HFX_ST_ACC] set, so tau_final is that trial's tau). */
 int hfx_cg_retract_gj(int dtype, int nb, size_t nk, void* Q, const void* D, const void* Ft, const double* st,
-                      double* eps_new, const void* U_keep, const double* eps_keep, void* stream);
+                      double* eps_new, const void* U_keep, const double* eps_keep, int seg_len, void* stream);
+/* seg_len (hfx_cg_precond and the _gj kernels): 0 for one solve; otherwise the
+ * nk k-points are consecutive solves of seg_len k-points each, and k-point k
+ * reads the state vector st + HFX_ST_LEN * (k / seg_len) (batched scans). */
 
 /* Q <- Q W with (eps, W) = eigh(Q^dagger F Q)  (solver.py:622-625). */
 int hfx_rotate_eigh(int dtype, int nb, size_t nk, void* Q, const void* F, double* eps, void* stream);
@@ -304,6 +308,34 @@ int hfx_solve_mu(const double* eps, const double* w, size_t nk, int nb, double t
 int hfx_solve_mu_steps(const double* eps, const double* w, size_t nk, int nb, double target, double T,
                        double mu0, int mu0_from_state, int maxiter, double* st, double* p_out, double* scratch,
                        void* stream);
+
+/* ---- batched solves (a density / field scan, no counterpart in the reference,
+ * whose scan is a serial loop of solve calls: examples/multilayer_graphene_
+ * density_scan.py:117-137).  nsolve solves of one mesh and band count are
+ * stored back to back ((nsolve, nk1, nk2, nb, nb)), each with its own
+ * HFX_ST_LEN state vector (st + HFX_ST_LEN * s); a solve whose
+ * st[HFX_ST_FROZEN] is set is skipped by the reductions, mu, epilogue and the
+ * _gj line-search kernels. ---- */
+
+/* Exchange self-energy of nsolve stacked densities with one shared VR (one
+ * launch; all nb^2 channels, or the upper triangle + mirror with herm). */
+int hfx_selfenergy_batched(const hfx_plan* plan, int nsolve, const void* P, const void* VR, int vr_full, int herm,
+                           void* Sigma, void* stream);
+/* F = herm(h + Sigma) and E_s = sum_k w_k Re Tr[(h + Sigma/2) P] into
+ * st[s][HFX_ST_E] (exchange-only Fock build of fock.py:76-97, 109-111);
+ * w2d (nsolve, nk1, nk2); scratch >= 64 * nsolve doubles. */
+int hfx_fock_epilogue_batched(const hfx_plan* plan, int nsolve, const void* h, const void* Sigma, const void* P,
+                              const double* w2d, void* F, double* st, double* scratch, void* stream);
+/* Scratch (float64) for hfx_reduce_batched / hfx_solve_mu_batched. */
+size_t hfx_batch_scratch_doubles(int nseg);
+/* hfx_reduce per solve (segments of seg_len k-points); with op == HFX_OP_SUM
+ * and out_slot >= 0 the sums go to st[s][out_slot ...]. */
+int hfx_reduce_batched(const double* part, int nval, size_t nk_total, int seg_len, const double* w, unsigned wmask,
+                       double* st, int op, const double* params, int out_slot, double* scratch, void* stream);
+/* hfx_solve_mu_steps per solve (targets[s] electrons, mu warm-started from
+ * st[s][HFX_ST_MU]). */
+int hfx_solve_mu_batched(const double* eps, const double* w, const double* targets, size_t nk_total, int seg_len,
+                         int nb, double T, int maxiter, double* st, double* p_out, double* scratch, void* stream);
 
 #ifdef __cplusplus
 }

@@ -42,4 +42,5 @@ def _extend(mod_names):
 _extend([
     ("solver", ["SolverConfig", "SolveResult", "solve", "solve_direct_minimization"]),
     ("reference_scf", ["SCFConfig", "SCFResult", "solve_scf"]),
+    ("batch", ["solve_batch"]),
 ])

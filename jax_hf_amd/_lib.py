@@ -18,7 +18,7 @@ LIB_PATH = _HERE / "libhfx.so"
 
 HFX_C128 = 0
 HFX_C64 = 1
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 _lib = None
 _lock = threading.Lock()
@@ -57,12 +57,17 @@ _SIGS = {
     "hfx_eigh_batched": [i32, i32, sz, vp, vp, vp, i32, vp],
     "hfx_density_from_Qp": [i32, i32, sz, vp, vp, vp, vp, f64, vp, vp, vp],
     "hfx_cg_grad": [i32, i32, sz, vp, vp, vp, vp, vp, vp, vp, vp],
-    "hfx_cg_precond": [i32, i32, sz, vp, vp, vp, vp, vp, vp, f64, vp, vp, vp, vp, vp],
+    "hfx_cg_precond": [i32, i32, sz, vp, vp, vp, vp, vp, vp, f64, vp, vp, vp, vp, i32, vp],
     "hfx_cg_direction": [i32, i32, sz, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
     "hfx_cg_trial": [i32, i32, sz, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp],
-    "hfx_cg_direction_gj": [i32, i32, sz, vp, vp, vp, vp, vp, vp, vp],
-    "hfx_cg_trial_gj": [i32, i32, sz, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp],
-    "hfx_cg_retract_gj": [i32, i32, sz, vp, vp, vp, vp, vp, vp, vp, vp],
+    "hfx_cg_direction_gj": [i32, i32, sz, vp, vp, vp, vp, vp, vp, i32, vp],
+    "hfx_cg_trial_gj": [i32, i32, sz, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, i32, vp],
+    "hfx_cg_retract_gj": [i32, i32, sz, vp, vp, vp, vp, vp, vp, vp, i32, vp],
+    "hfx_selfenergy_batched": [vp, i32, vp, vp, i32, i32, vp, vp],
+    "hfx_fock_epilogue_batched": [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp],
+    "hfx_batch_scratch_doubles": [i32],
+    "hfx_reduce_batched": [vp, i32, sz, i32, vp, C.c_uint, vp, i32, vp, i32, vp, vp],
+    "hfx_solve_mu_batched": [vp, vp, vp, sz, i32, i32, f64, i32, vp, vp, vp, vp],
     "hfx_cg_retract": [i32, i32, sz, vp, vp, vp, vp, vp, f64, i32, vp, vp, vp],
     "hfx_rotate_eigh": [i32, i32, sz, vp, vp, vp, vp],
     "hfx_matmul_batched": [i32, i32, sz, vp, vp, vp, i32, vp, i32, vp, vp],
@@ -78,7 +83,7 @@ _SIGS = {
     "hfx_solve_mu_steps": [vp, vp, sz, i32, f64, f64, f64, i32, i32, vp, vp, vp, vp],
     "hfx_solve_mu": [vp, vp, sz, i32, f64, f64, f64, i32, i32, i32, f64, vp, vp, vp, vp, vp],
 }
-_RESTYPES = {"hfx_energy_scratch_bytes": sz, "hfx_solver_scratch_doubles": sz}
+_RESTYPES = {"hfx_energy_scratch_bytes": sz, "hfx_solver_scratch_doubles": sz, "hfx_batch_scratch_doubles": sz}
 
 
 class HfxError(RuntimeError):

@@ -9,7 +9,8 @@ from . import _lib as L
 ST_MU, ST_BETA, ST_LAM, ST_GRAD, ST_TAU0, ST_TAU, ST_TAUF, ST_ACC = 0, 1, 2, 3, 4, 5, 6, 7
 ST_OMEGA0, ST_DNORM, ST_OMEGA, ST_E = 8, 9, 10, 11
 ST_ITER, ST_MULO, ST_MUHI, ST_MUDONE = 12, 13, 14, 15
-ST_LEN = 16
+ST_FROZEN = 16
+ST_LEN = 32
 OP_SUM, OP_GRAD, OP_BETA, OP_TAU0, OP_TRIAL = 0, 1, 2, 3, 4
 MU_NEWTON_BRACKET, MU_BISECTION, MU_NEWTON = 0, 1, 2
 
@@ -44,7 +45,7 @@ def s(dev):
 
 
 def eigh(A, lam, V, herm_in=False):
-    nk = A.shape[0] * A.shape[1] if A.dim() == 4 else A.shape[0]
+    nk = A.numel() // (A.shape[-1] * A.shape[-1])
     L.call("hfx_eigh_batched", dt_code(A), A.shape[-1], nk, L.ptr(A), L.ptr(lam), L.ptr(V), int(herm_in),
            s(A.device))
 

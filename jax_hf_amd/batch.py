@@ -1,0 +1,327 @@
+"""Batched independent solves: a density / field scan in shared launches.
+
+The reference runs a scan as a serial Python loop of ``solve`` calls
+(``examples/multilayer_graphene_density_scan.py:117-137``); each call on a
+small mesh leaves most of the GPU idle (a 36x36 mesh is 1296 per-k
+wavefronts and 256 exchange workgroups).  ``solve_batch`` stacks the solves
+that share a mesh, band count, temperature and interaction: one launch per
+kernel serves all of them --
+
+* the per-k CG kernels run over the stacked k-points, each k-point reading
+  the state vector of its own solve (``seg_len``);
+* the exchange runs all solves' channel planes in one launch with the shared
+  VR (``hfx_selfenergy_batched``), the epilogue writes one energy per solve;
+* reductions, the line-search scalar steps and the mu solver are segmented
+  per solve (``hfx_reduce_batched``, ``hfx_solve_mu_batched``);
+* a solve that meets its stopping rule is frozen (its state vector's flag):
+  its kernels skip their updates while the others continue;
+* the iteration is captured once as HIP graphs and replayed, one readback
+  of all state vectors per iteration.
+
+The algorithm is ``solve``'s CG (optimizer "cg", Gauss-Jordan Cayley line
+search) per solve; with ``precision="mixed"`` each solve is then polished by
+the FP64 DIIS SCF (serially).  Solves that cannot be stacked (Hartree,
+contact terms, a reference density, a user ``fock_build_fn``, block specs,
+projection) run through ``solve`` one by one.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import _ops as K
+from ._util import KIND, convert, to_tensor
+from .solver import SolveResult, SolverConfig, _polish_diis, _precision
+
+__all__ = ["solve_batch"]
+
+
+def _batchable(kernel, cfg):
+    g = kernel._contact_host[0]
+    contact = g is not None and bool(np.any(np.asarray(g) != 0.0))
+    return (not kernel.include_hartree and kernel.include_exchange and kernel._refP is None and not contact
+            and cfg.block_sizes is None and cfg.project_fn is None
+            and str(cfg.optimizer).lower() == "cg" and str(cfg.cayley).lower() == "gauss_jordan"
+            and kernel.h.shape[-1] <= 16)
+
+
+def _group_key(kernel):
+    return (tuple(kernel.h.shape), kernel.dtype, float(kernel.T), kernel._v_real_symmetric,
+            tuple(kernel._VR_shifted.shape))
+
+
+class _BatchCG:
+    """Stacked CG state of S solves on one (nk1, nk2, nb) mesh."""
+
+    def __init__(self, kernels, cdt, cfg):
+        k0 = kernels[0]
+        self.kernels = kernels
+        self.cfg = cfg
+        self.cdt = cdt
+        self.dev = dev = k0.device
+        nk1, nk2, nb = k0.h.shape[0], k0.h.shape[1], k0.h.shape[-1]
+        self.S = S = len(kernels)
+        self.Nk, self.nb = nk1 * nk2, nb
+        self.nk_tot = S * self.Nk
+        self.shape4 = (S, nk1, nk2, nb, nb)
+        self.plan = L.plan_for(nk1, nk2, nb, cdt, dev)
+        self.herm = bool(k0._v_real_symmetric)
+        VR = k0._VR_shifted
+        self.VR = VR if VR.dtype == cdt else convert(VR, cdt)
+        self.vfull = 0 if VR.shape[-1] == 1 else 1
+        self.h = torch.stack([k.h if k.h.dtype == cdt else convert(k.h, cdt) for k in kernels]).contiguous()
+        self.w_raw = torch.stack([k._w64 for k in kernels]).contiguous()
+        self.w_norm = torch.stack([k._w64 / max(k.weight_sum, 1e-30) for k in kernels]).contiguous()
+        self.T = max(float(k0.T), 1e-12)
+        c = dict(dtype=cdt, device=dev)
+        r = dict(dtype=torch.float64, device=dev)
+        z = lambda: torch.zeros(self.shape4, **c)  # noqa: E731
+        self.Q, self.P, self.F, self.Ft, self.G, self.Gp = z(), z(), z(), z(), z(), z()
+        self.H, self.D, self.V, self.Sig = z(), z(), z(), z()
+        v = lambda: torch.zeros((S, nk1, nk2, nb), **r)  # noqa: E731
+        self.p, self.eps, self.gp, self.gpp, self.hp, self.dp, self.lam = v(), v(), v(), v(), v(), v(), v()
+        self.st = torch.zeros((S, K.ST_LEN), **r)
+        self.part = torch.zeros((8, self.nk_tot), **r)
+        self.scratch = torch.zeros(int(L.lib().hfx_batch_scratch_doubles(S)), **r)
+        self.epi = torch.zeros(64 * S, **r)
+        self.targets = torch.zeros(S, **r)
+
+    # -- pieces ---------------------------------------------------------------
+    def set_targets(self, n_electrons):
+        t = [float(n) / max(k.weight_sum, 1e-30) for n, k in zip(n_electrons, self.kernels)]
+        self.targets.copy_(torch.tensor(t, dtype=torch.float64))
+
+    def density(self):
+        K.density(self.Q, self.P, p=self.p)
+        return self.P
+
+    def fock(self, P, F):
+        sm = L.stream(self.dev)
+        L.call("hfx_selfenergy_batched", self.plan.handle, self.S, L.ptr(P), L.ptr(self.VR), self.vfull,
+               int(self.herm), L.ptr(self.Sig), sm)
+        L.call("hfx_fock_epilogue_batched", self.plan.handle, self.S, L.ptr(self.h), L.ptr(self.Sig), L.ptr(P),
+               L.ptr(self.w_raw), L.ptr(F), L.ptr(self.st), L.ptr(self.epi), sm)
+
+    def mu(self):
+        L.call("hfx_solve_mu_batched", L.ptr(self.eps), L.ptr(self.w_norm), L.ptr(self.targets), self.nk_tot,
+               self.Nk, self.nb, self.T, int(self.cfg.mu_maxiter), L.ptr(self.st), L.ptr(self.p),
+               L.ptr(self.scratch), L.stream(self.dev))
+
+    def reduce(self, nval, wmask, op, params, out_slot=-1):
+        vals = list(params) + [0.0] * (10 - len(params))
+        import ctypes as C
+        arr = (C.c_double * 10)(*[float(x) for x in vals])
+        L.call("hfx_reduce_batched", L.ptr(self.part), int(nval), self.nk_tot, self.Nk, L.ptr(self.w_norm),
+               int(wmask), L.ptr(self.st), int(op), C.cast(arr, C.c_void_p), int(out_slot), L.ptr(self.scratch),
+               L.stream(self.dev))
+
+    def init_from(self, P0s):
+        """solver.py:380-394 per solve: eigh(F[herm(P0)]) -> (Q, eps), mu, p."""
+        P0 = torch.stack([to_tensor(P, self.dev, self.cdt) for P in P0s]).contiguous()
+        K.hermitize(P0, self.P)
+        self.fock(self.P, self.F)
+        K.eigh(self.F, self.eps, self.Q, herm_in=True)
+        self.st.zero_()
+        self.mu()
+
+    def _swap(self):
+        self.G, self.Gp = self.Gp, self.G
+        self.gp, self.gpp = self.gpp, self.gp
+
+    def _body(self, shrink_pow):
+        cfg = self.cfg
+        nk, nb, seg = self.nk_tot, self.nb, self.Nk
+        dt = K.dt_code(self.Q)
+        st, part = self.st, self.part
+        sm = L.stream(self.dev)
+        P = self.density()                                                      # solver.py:462
+        self.fock(P, self.F)                                                    # :463-471
+        L.call("hfx_cg_grad", dt, nb, nk, L.ptr(self.Q), L.ptr(self.F), L.ptr(self.p), L.ptr(self.Ft),
+               L.ptr(self.eps), L.ptr(self.G), L.ptr(part), sm)                 # :472-478
+        self.reduce(2, 0b01, K.OP_GRAD, [self.T, cfg.denom_scale, seg * nb])    # :491-497
+        L.call("hfx_cg_precond", dt, nb, nk, L.ptr(self.G), L.ptr(self.Gp), L.ptr(self.eps), L.ptr(self.p),
+               L.ptr(self.gpp), L.ptr(st), self.T, L.ptr(self.H), L.ptr(self.gp), L.ptr(self.hp),
+               L.ptr(part), seg, sm)                                            # :486, 499-504
+        self.reduce(6, 0b111111, K.OP_BETA, [0, 0, 0, 0, 0, 0, 0, 0, 0, max(int(cfg.cg_restart), 1)])
+        L.call("hfx_cg_direction_gj", dt, nb, nk, L.ptr(self.H), L.ptr(self.D), L.ptr(self.hp),
+               L.ptr(self.dp), L.ptr(st), L.ptr(part), seg, sm)
+        self.reduce(1, 1, K.OP_TAU0, [0, 0, 0, cfg.max_step, 0, shrink_pow])     # :543-547
+        for i in range(cfg.bt_max):                                             # :563-579
+            L.call("hfx_cg_trial_gj", dt, nb, nk, L.ptr(self.D), L.ptr(self.Ft), L.ptr(self.p),
+                   L.ptr(self.dp), L.ptr(self.eps), L.ptr(st), int(i == 0), L.ptr(part),
+                   L.ptr(self.V), L.ptr(self.lam), seg, sm)
+            self.reduce(4 if i == 0 else 2, 0b1111, K.OP_TRIAL,
+                        [self.T, 0, 0, 0, cfg.bt_shrink, shrink_pow, 0, int(i == cfg.bt_max - 1), int(i == 0)])
+        L.call("hfx_cg_retract_gj", dt, nb, nk, L.ptr(self.Q), L.ptr(self.D), L.ptr(self.Ft), L.ptr(st),
+               L.ptr(self.eps), L.ptr(self.V), L.ptr(self.lam), seg, sm)      # :586-590
+        self.mu()                                                               # :591-592
+        self._swap()
+
+    def _capture(self, shrink_pow):
+        graphs = []
+        cur = torch.cuda.current_stream(self.dev)
+        cap = torch.cuda.Stream(self.dev)
+        cap.wait_stream(cur)
+        with torch.cuda.stream(cap):
+            for _ in range(2):
+                g = torch.cuda.CUDAGraph()
+                g.capture_begin()
+                self._body(shrink_pow)
+                g.capture_end()
+                graphs.append(g)
+        cur.wait_stream(cap)
+        return graphs
+
+    def run(self, tol_E, tol_grad, max_iter, rel_tol=False):
+        """solver.py:422-449 per solve; a solve meeting its rule is frozen."""
+        cfg = self.cfg
+        S = self.S
+        self.G.zero_()
+        self.Gp.zero_()
+        self.D.zero_()
+        self.gp.zero_()
+        self.gpp.zero_()
+        self.dp.zero_()
+        self.st[:, K.ST_ITER] = 0.0
+        self.st[:, K.ST_FROZEN] = 0.0
+        shrink_pow = cfg.bt_shrink ** cfg.bt_max
+        hist_E = [[] for _ in range(S)]
+        hist_G = [[] for _ in range(S)]
+        k_of = [0] * S
+        grad = [np.inf] * S
+        dE = [np.inf] * S
+        E_prev = [np.inf] * S
+        active = [True] * S
+        graphs, g_base, k = None, 0, 0
+        W = cfg.plateau_window
+        while True:
+            newly = []
+            for s in range(S):
+                if not active[s]:
+                    continue
+                hE = hist_E[s]
+                tE = tol_E * max(1.0, abs(hE[-1])) if (rel_tol and hE) else tol_E
+                ks = k_of[s]
+                if W > 0:
+                    e_now = hE[max(ks - 1, 0)] if hE else 0.0
+                    e_past = hE[max(ks - 1 - W, 0)] if hE else 0.0
+                    enc = (e_past - e_now) > tE if ks > W else dE[s] > tE
+                else:
+                    enc = dE[s] > tE
+                nc = grad[s] > tol_grad if tol_grad > 0.0 else enc
+                if not (ks < max_iter and nc):
+                    active[s] = False
+                    newly.append(s)
+            if newly:
+                self.st[newly, K.ST_FROZEN] = 1.0
+            if not any(active):
+                break
+            if graphs is None and bool(cfg.use_graph) and k >= 1:
+                graphs = self._capture(shrink_pow)
+                g_base = k
+            if graphs is not None:
+                graphs[(k - g_base) % 2].replay()
+                self._swap()
+            else:
+                self._body(shrink_pow)
+            sv = self.st.cpu().numpy()
+            for s in range(S):
+                if not active[s]:
+                    continue
+                E = float(sv[s, K.ST_E])
+                grad[s] = float(sv[s, K.ST_GRAD])
+                hist_E[s].append(E)
+                hist_G[s].append(grad[s])
+                dE[s] = abs(E - E_prev[s])
+                E_prev[s] = E
+                k_of[s] += 1
+            k += 1
+        self.G, self.Gp = self.Gp, self.G
+        self.st[:, K.ST_FROZEN] = 0.0
+        return k_of, grad, hist_E, hist_G
+
+    def finalize(self):
+        """solver.py:606-638 for every solve: Fock, eigh canonicalisation, mu, p, Fock + E."""
+        P = self.density()
+        self.fock(P, self.F)
+        K.matmul(self.Q, self.F, self.Ft, C=self.Q, ops=0b001)
+        K.eigh(self.Ft, self.eps, self.V, herm_in=True)
+        K.matmul(self.Q, self.V, self.H)
+        self.Q, self.H = self.H, self.Q
+        self.mu()
+        P = self.density()
+        Pout = P.clone()
+        F = torch.empty_like(Pout)
+        self.fock(Pout, F)
+        return Pout, F
+
+
+def solve_batch(kernels, P0s, n_electrons, *, config: SolverConfig | None = None):
+    """Independent CG solves of a scan, stacked into shared launches.
+
+    ``kernels``, ``P0s`` and ``n_electrons`` are sequences of equal length
+    (``P0s`` / ``n_electrons`` may also be a single density / count used for
+    all).  Returns one ``SolveResult`` per solve, in order.  Solves are
+    stacked per (mesh, bands, dtype, T, interaction); the rest go through
+    ``solve``.
+    """
+    from .solver import solve
+    from .utils import validate_electron_count
+    from ._util import host_array
+    cfg = config or SolverConfig()
+    n = len(kernels)
+    if not isinstance(P0s, (list, tuple)):
+        P0s = [P0s] * n
+    if np.ndim(n_electrons) == 0:
+        n_electrons = [float(n_electrons)] * n
+    if len(P0s) != n or len(n_electrons) != n:
+        raise ValueError("kernels, P0s and n_electrons must have the same length")
+    for kern, ne in zip(kernels, n_electrons):
+        validate_electron_count(host_array(kern._w64), kern.h.shape[-1], ne, context="n_electrons")
+    results = [None] * n
+    groups: dict = {}
+    for i, kern in enumerate(kernels):
+        if not _batchable(kern, cfg):
+            results[i] = solve(kern, P0s[i], n_electrons[i], config=cfg)
+            continue
+        key = _group_key(kern)
+        placed = False
+        for gkey, members in groups.items():
+            if gkey[0] == key and torch.equal(kernels[members[0]]._VR_shifted, kern._VR_shifted):
+                members.append(i)
+                placed = True
+                break
+        if not placed:
+            groups[(key, i)] = [i]
+    for members in groups.values():
+        ks = [kernels[i] for i in members]
+        prec = _precision(ks[0], cfg)
+        cdt = torch.complex128 if prec == "fp64" else torch.complex64
+        b = _BatchCG(ks, cdt, cfg)
+        b.set_targets([n_electrons[i] for i in members])
+        b.init_from([P0s[i] for i in members])
+        if prec == "mixed":
+            k_of, grad, hE, hG = b.run(cfg.mixed_tol_E, 0.0, max(cfg.max_iter // 2, 1), rel_tol=True)
+            P_lo = b.density()
+            for j, i in enumerate(members):
+                results[i] = _polish_diis(kernels[i], None, n_electrons[i], cfg, None, k_of[j], hE[j], hG[j],
+                                          P_lo=P_lo[j])
+            continue
+        k_of, grad, hE, hG = b.run(cfg.tol_E, cfg.tol_grad, cfg.max_iter)
+        Pfin, Ffin = b.finalize()
+        sv = b.st.cpu()
+        for j, i in enumerate(members):
+            kk = k_of[j]
+            conv = (grad[j] <= cfg.tol_grad) if cfg.tol_grad > 0.0 else (kk < cfg.max_iter)
+            m = max(cfg.max_iter, kk)
+            hEt = torch.zeros(m, dtype=torch.float64)
+            hGt = torch.zeros(m, dtype=torch.float64)
+            hEt[:kk] = torch.tensor(hE[j][:kk], dtype=torch.float64)
+            hGt[:kk] = torch.tensor(hG[j][:kk], dtype=torch.float64)
+            results[i] = SolveResult(Q=b.Q[j].clone(), p=b.p[j].clone(), mu=sv[j, K.ST_MU].to(b.dev),
+                                     density=Pfin[j].clone(), fock=Ffin[j].clone(),
+                                     energy=sv[j, K.ST_E].to(b.dev), n_iter=int(kk), converged=bool(conv),
+                                     history=dict(E=hEt.to(b.dev), grad_norm=hGt.to(b.dev)))
+    return results

@@ -70,7 +70,7 @@ using namespace hfx;
 
 extern "C" {
 
-int hfx_abi_version(void) { return 3; }
+int hfx_abi_version(void) { return 4; }
 
 int hfx_last_error(char* buf, size_t n) {
   if (!buf || n == 0) return HFX_EINVAL;
@@ -190,6 +190,17 @@ int hfx_selfenergy(const hfx_plan* plan, const void* P, const void* refP, const 
                          0, ifftshift ? 1 : 0, 0, (hipStream_t)stream);
 }
 
+int hfx_selfenergy_batched(const hfx_plan* plan, int nsolve, const void* P, const void* VR, int vr_full, int herm,
+                           void* Sigma, void* stream) {
+  HFX_REQUIRE(plan && P && VR && Sigma, "hfx_selfenergy_batched: NULL argument");
+  HFX_REQUIRE(nsolve >= 1 && (long long)nsolve * plan->nb * plan->nb <= (1LL << 30),
+              "hfx_selfenergy_batched: nsolve=%d out of range", nsolve);
+  const int* chans = herm ? plan->chans_triu : plan->chans_full;
+  const int nchan = herm ? plan->nb * (plan->nb + 1) / 2 : plan->nb * plan->nb;
+  return exchange_launch(plan, P, nullptr, VR, vr_full ? 1 : 0, chans, nchan, herm ? 1 : 0, nullptr, Sigma, nullptr,
+                         0, 0, 0, (hipStream_t)stream, nsolve);
+}
+
 int hfx_selfenergy_adjoint(const hfx_plan* plan, const void* G, const void* VR, int vr_full, void* out,
                            void* stream) {
   HFX_REQUIRE(plan && G && VR && out, "hfx_selfenergy_adjoint: NULL argument");
@@ -277,6 +288,13 @@ int hfx_kterms(const hfx_plan* plan, const void* rho, const double* HH, int ntc,
   HFX_REQUIRE(!Hdiag || HH, "hfx_kterms: Hdiag requires HH");
   HFX_REQUIRE(!sigc || (ntc >= 0 && (ntc == 0 || (g && Oi && Oj))), "hfx_kterms: contact arrays missing");
   return kterms_launch(plan, rho, HH, sigc ? ntc : 0, g, Oi, Oj, Hdiag, sigc, (hipStream_t)stream);
+}
+
+int hfx_fock_epilogue_batched(const hfx_plan* plan, int nsolve, const void* h, const void* Sigma, const void* P,
+                              const double* w2d, void* F, double* st, double* scratch, void* stream) {
+  HFX_REQUIRE(plan && h && Sigma && P && w2d && F && st && scratch, "hfx_fock_epilogue_batched: NULL argument");
+  HFX_REQUIRE(nsolve >= 1, "hfx_fock_epilogue_batched: nsolve=%d", nsolve);
+  return epilogue_seg_launch(plan, nsolve, h, Sigma, P, w2d, F, st, scratch, (hipStream_t)stream);
 }
 
 int hfx_fock_epilogue(const hfx_plan* plan, const void* h, const void* Sigma, const double* Hdiag,

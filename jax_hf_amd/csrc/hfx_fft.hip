@@ -183,6 +183,8 @@ struct ExchArgs {
   cx<T>* Sigma;        // (n1, n2, nb, nb)
   cx<double>* rho;     // (nb, nb) weighted channel moments, may be null
   int nchan, herm, vfull, xcd_remap;
+  int nsolve;          // planes = nsolve * nchan; solve s reads / writes at s * plane_stride
+  size_t plane_stride; // elements between consecutive solves' P / refP / Sigma (Nk * nb^2)
   int mode;   // 0: exchange convolution; 1: forward transform of w * V (VR preparation)
   int shift;  // mode 0: ifftshift the result on store; mode 1: apply the ifftshift phase
   int conjv;  // mode 0: multiply by conj(VR) (the adjoint / cross-correlation)
@@ -196,14 +198,24 @@ static_assert(offsetof(ExchArgs<double>, shift) < offsetof(ExchArgs<double>, con
 __host__ __device__ constexpr int lds_pitch(int n2) { return (n2 % 2 == 0) ? n2 + 1 : n2; }
 
 // ---- shared pieces of both kernel flavours --------------------------------
+// Channel of this workgroup; for a batch of solves also moves the P / refP /
+// Sigma pointers to this workgroup's solve (VR is shared by the batch).
 template <typename T, int BLOCK>
-__device__ __forceinline__ int exch_channel(const ExchArgs<T>& a) {
+__device__ __forceinline__ int exch_channel(ExchArgs<T>& a) {
   int b = blockIdx.x;
-  if (a.xcd_remap) {  // consecutive channels on one XCD (blocks round-robin over 8 XCDs)
-    const int per = a.nchan >> 3;
+  const int planes = a.nchan * a.nsolve;
+  if (a.xcd_remap) {  // consecutive planes on one XCD (blocks round-robin over 8 XCDs)
+    const int per = planes >> 3;
     b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
   }
-  return a.chans[b];
+  const int s = b / a.nchan;
+  if (s > 0) {
+    const size_t off = (size_t)s * a.plane_stride;
+    a.P += off;
+    if (a.refP) a.refP += off;
+    a.Sigma += off;
+  }
+  return a.chans[b - s * a.nchan];
 }
 
 template <typename T, int BLOCK>
@@ -577,7 +589,7 @@ constexpr int v2_default(int N) { return N == 96 ? HFX_EXCH_V2_B : HFX_EXCH_V2_A
 template <typename T, int N, int V>
 static int launch_v2(const ExchArgs<T>& a, hipStream_t s) {
   constexpr int B = v2_block<N>(V);
-  hipLaunchKernelGGL((exch_v2_kernel<T, N, N, B>), dim3(a.nchan), dim3(B), 0, s, a);
+  hipLaunchKernelGGL((exch_v2_kernel<T, N, N, B>), dim3(a.nchan * a.nsolve), dim3(B), 0, s, a);
   return check_launch("exch_v2_kernel");
 }
 
@@ -596,7 +608,7 @@ static int launch_exch(const ExchArgs<T>& a, int variant, hipStream_t s) {
         if (v == HFX_EXCH_V2_C) return launch_v2<T, N, HFX_EXCH_V2_C>(a, s);                \
         return launch_v2<T, N, HFX_EXCH_V2_A>(a, s);                                       \
       }                                                                                    \
-      hipLaunchKernelGGL((exch_ct_kernel<T, N, N, ct_block<N>()>), dim3(a.nchan),           \
+      hipLaunchKernelGGL((exch_ct_kernel<T, N, N, ct_block<N>()>), dim3(a.nchan * a.nsolve),           \
                          dim3(ct_block<N>()), 0, s, a);                                    \
       return check_launch("exch_ct_kernel");                                               \
     }                                                                                      \
@@ -614,23 +626,25 @@ static int launch_exch(const ExchArgs<T>& a, int variant, hipStream_t s) {
                                     (int)(MAXE * sizeof(cx<T>))));
     attr_set = true;
   }
-  hipLaunchKernelGGL(kern, dim3(a.nchan), dim3(EXCH_BLOCK), lds, s, a);
+  hipLaunchKernelGGL(kern, dim3(a.nchan * a.nsolve), dim3(EXCH_BLOCK), lds, s, a);
   return check_launch("exch_conv_kernel");
 }
 
 int exchange_launch(const hfx_plan* plan, const void* P, const void* refP, const void* VR, int vfull,
                     const int* chans, int nchan, int herm, const double* w2d, void* Sigma, void* rho,
-                    int mode, int shift, int conjv, hipStream_t s) {
-  if (nchan <= 0) return HFX_OK;
+                    int mode, int shift, int conjv, hipStream_t s, int nsolve) {
+  if (nchan <= 0 || nsolve <= 0) return HFX_OK;
+  const int remap = ((nchan * nsolve) % 8 == 0) ? 1 : 0;
+  const size_t stride = (size_t)plan->n1 * plan->n2 * plan->nb * plan->nb;
   if (plan->dtype == HFX_DTYPE_C128) {
     ExchArgs<double> a{(const cx<double>*)P, (const cx<double>*)refP, (const cx<double>*)VR, chans, w2d,
-                       (cx<double>*)Sigma, (cx<double>*)rho, nchan, herm, vfull,
-                       (nchan % 8 == 0) ? 1 : 0, /*mode=*/mode, /*shift=*/shift, /*conjv=*/conjv, plan->g64};
+                       (cx<double>*)Sigma, (cx<double>*)rho, nchan, herm, vfull, remap, nsolve, stride,
+                       /*mode=*/mode, /*shift=*/shift, /*conjv=*/conjv, plan->g64};
     return launch_exch<double>(a, plan->exch_variant, s);
   }
   ExchArgs<float> a{(const cx<float>*)P, (const cx<float>*)refP, (const cx<float>*)VR, chans, w2d,
-                    (cx<float>*)Sigma, (cx<double>*)rho, nchan, herm, vfull,
-                    (nchan % 8 == 0) ? 1 : 0, /*mode=*/mode, /*shift=*/shift, /*conjv=*/conjv, plan->g32};
+                    (cx<float>*)Sigma, (cx<double>*)rho, nchan, herm, vfull, remap, nsolve, stride,
+                    /*mode=*/mode, /*shift=*/shift, /*conjv=*/conjv, plan->g32};
   return launch_exch<float>(a, plan->exch_variant, s);
 }
 

@@ -198,6 +198,66 @@ int epilogue_launch(const hfx_plan* p, const void* h, const void* Sig, const dou
   return check_launch("reduce_partials_kernel");
 }
 
+// Batched epilogue: nsolve consecutive solves of Nk k-points; blockIdx.y is
+// the solve, energy partials per (solve, block); frozen solves are skipped.
+template <typename T>
+__global__ __launch_bounds__(EPI_BLOCK) void fock_epilogue_seg_kernel(
+    int Nk, int nb, const cx<T>* __restrict__ h, const cx<T>* __restrict__ Sig, const cx<T>* __restrict__ P,
+    const double* __restrict__ w2d, cx<T>* __restrict__ F, const double* __restrict__ st, double* __restrict__ part) {
+  __shared__ double red[EPI_BLOCK / 64];
+  const int sidx = blockIdx.y;
+  const int nb2 = nb * nb;
+  const size_t total = (size_t)Nk * nb2, base = (size_t)sidx * total;
+  double eacc = 0.0;
+  if (st[(size_t)sidx * HFX_ST_LEN + HFX_ST_FROZEN] == 0.0) {
+    for (size_t e0 = (size_t)blockIdx.x * EPI_BLOCK + threadIdx.x; e0 < total; e0 += (size_t)gridDim.x * EPI_BLOCK) {
+      const size_t e = base + e0;
+      const size_t k = e0 / nb2;
+      const int c = (int)(e0 - k * nb2);
+      const int i = c / nb, j = c - i * nb;
+      const size_t et = base + k * nb2 + (size_t)j * nb + i;
+      const cx<T> a_ij = h[e] + Sig[e], a_ji = h[et] + Sig[et];
+      F[e] = (T)0.5 * (a_ij + conj(a_ji));
+      const cx<T> m = h[e] + (T)0.5 * Sig[e];
+      const cx<T> pji = P[et];
+      eacc += w2d[(size_t)sidx * Nk + k] * ((double)m.re * (double)pji.re - (double)m.im * (double)pji.im);
+    }
+  }
+  double acc[1] = {eacc};
+  block_sum<1>(acc, red);
+  if (threadIdx.x == 0) part[(size_t)sidx * gridDim.x + blockIdx.x] = acc[0];
+}
+
+__global__ __launch_bounds__(256) void energy_seg_kernel(const double* __restrict__ part, int nblk, double* st) {
+  __shared__ double red[4];
+  const int sidx = blockIdx.x;
+  double* sts = st + (size_t)sidx * HFX_ST_LEN;
+  if (sts[HFX_ST_FROZEN] != 0.0) return;
+  double acc[1] = {0.0};
+  for (int b = threadIdx.x; b < nblk; b += 256) acc[0] += part[(size_t)sidx * nblk + b];
+  block_sum<1>(acc, red);
+  if (threadIdx.x == 0) sts[HFX_ST_E] = acc[0];
+}
+
+int epilogue_seg_launch(const hfx_plan* p, int nsolve, const void* h, const void* Sig, const void* P, const double* w2d,
+                        void* F, double* st, double* part, hipStream_t s) {
+  const int Nk = p->n1 * p->n2;
+  int gx = epi_grid((size_t)Nk * p->nb * p->nb);
+  if (gx > 64) gx = 64;
+  const dim3 grid((unsigned)gx, (unsigned)nsolve);
+  if (p->dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(fock_epilogue_seg_kernel<double>, grid, dim3(EPI_BLOCK), 0, s, Nk, p->nb,
+                       (const cx<double>*)h, (const cx<double>*)Sig, (const cx<double>*)P, w2d, (cx<double>*)F, st,
+                       part);
+  else
+    hipLaunchKernelGGL(fock_epilogue_seg_kernel<float>, grid, dim3(EPI_BLOCK), 0, s, Nk, p->nb,
+                       (const cx<float>*)h, (const cx<float>*)Sig, (const cx<float>*)P, w2d, (cx<float>*)F, st, part);
+  int rc = check_launch("fock_epilogue_seg_kernel");
+  if (rc) return rc;
+  hipLaunchKernelGGL(energy_seg_kernel, dim3((unsigned)nsolve), dim3(256), 0, s, part, gx, st);
+  return check_launch("energy_seg_kernel");
+}
+
 int hf_energy_launch(const hfx_plan* p, const void* P, const void* h, const void* Sig, const void* H,
                      int H_bcast, const double* w2d, double* partials, double* E_out, hipStream_t s) {
   const int Nk = p->n1 * p->n2;

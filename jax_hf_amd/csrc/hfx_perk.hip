@@ -16,7 +16,12 @@ namespace hfx {
 
 constexpr int PK_NBMAX = 16;
 constexpr int PK_PITCH = PK_NBMAX + 1;
-constexpr int PK_MAT = PK_NBMAX * PK_PITCH;   // LDS elements per staged matrix
+constexpr int PK_MAT = PK_NBMAX * PK_PITCH;
+
+// state vector of the solve that k-point k belongs to (seg_len = 0: one solve)
+__device__ __forceinline__ const double* seg_state(const double* st, size_t k, int seg_len) {
+  return seg_len > 0 ? st + (k / (size_t)seg_len) * HFX_ST_LEN : st;
+}   // LDS elements per staged matrix
 constexpr int PK_EPL = (PK_NBMAX * PK_NBMAX + 63) / 64;
 
 #define PK_FOR_ELEM(nb, ...)                                    \
@@ -321,10 +326,12 @@ __global__ __launch_bounds__(64) void pk_cg_precond_kernel(int nb, size_t Nk, co
                                                            const double* __restrict__ eps,
                                                            const double* __restrict__ p,
                                                            const double* __restrict__ gp_prev,
-                                                           const double* __restrict__ st, double T_,
+                                                           const double* __restrict__ st_all, double T_,
                                                            cx<T>* __restrict__ H, double* __restrict__ gp,
-                                                           double* __restrict__ hp, double* __restrict__ part) {
+                                                           double* __restrict__ hp, double* __restrict__ part,
+                                                           int seg_len) {
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  const double* st = seg_state(st_all, k, seg_len);
   const double lam = st[HFX_ST_LAM], mu = st[HFX_ST_MU];
   double s_gh = 0, s_ph = 0, s_pp = 0;
   PK_FOR_ELEM(nb, {
@@ -585,8 +592,10 @@ __device__ __forceinline__ void m_diag_UFU(const cx<T>* sF, const cx<T>* sU, cx<
 template <typename T>
 __global__ __launch_bounds__(64) void pk_cg_dir_kernel(int nb, const cx<T>* __restrict__ H, cx<T>* __restrict__ D,
                                                        const double* __restrict__ hp, double* __restrict__ dp,
-                                                       const double* __restrict__ st, double* __restrict__ part) {
+                                                       const double* __restrict__ st_all, double* __restrict__ part,
+                                                       int seg_len) {
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  const double* st = seg_state(st_all, k, seg_len);
   const T beta = (T)st[HFX_ST_BETA];
   double dn = 0.0;
   PK_FOR_ELEM(nb, {
@@ -610,10 +619,11 @@ __global__ __launch_bounds__(64) void pk_cg_trial_gj_kernel(int nb, size_t Nk, c
                                                             const double* __restrict__ p,
                                                             const double* __restrict__ dp,
                                                             const double* __restrict__ eps,
-                                                            const double* __restrict__ st, int omega0,
+                                                            const double* __restrict__ st_all, int omega0,
                                                             double* __restrict__ part, cx<T>* __restrict__ U_out,
-                                                            double* __restrict__ eps_out) {
-  if (st[HFX_ST_ACC] != 0.0) return;
+                                                            double* __restrict__ eps_out, int seg_len) {
+  const double* st = seg_state(st_all, blockIdx.x, seg_len);
+  if (st[HFX_ST_ACC] != 0.0 || st[HFX_ST_FROZEN] != 0.0) return;
   __shared__ cx<T> sX[PK_MAT], sM[PK_MAT], sU[PK_MAT], sF[PK_MAT];
   __shared__ double et[PK_NBMAX];
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
@@ -662,13 +672,15 @@ template <typename T>
 __global__ __launch_bounds__(64) void pk_cg_retract_gj_kernel(int nb, cx<T>* __restrict__ Q,
                                                               const cx<T>* __restrict__ D,
                                                               const cx<T>* __restrict__ Ft,
-                                                              const double* __restrict__ st,
+                                                              const double* __restrict__ st_all,
                                                               double* __restrict__ eps_new,
                                                               const cx<T>* __restrict__ U_acc,
-                                                              const double* __restrict__ eps_acc) {
+                                                              const double* __restrict__ eps_acc, int seg_len) {
   __shared__ cx<T> sX[PK_MAT], sM[PK_MAT], sU[PK_MAT], sF[PK_MAT];
   __shared__ double et[PK_NBMAX];
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  const double* st = seg_state(st_all, k, seg_len);
+  if (st[HFX_ST_FROZEN] != 0.0) return;
   if (U_acc && st[HFX_ST_ACC] != 0.0) {
     // a trial was accepted: tau_final is its tau, and its U / eps were kept
     m_load(sU, U_acc + k * nb2, nb);
@@ -972,15 +984,15 @@ int hfx_cg_grad(int dtype, int nb, size_t nk, const void* Q, const void* F, cons
 
 int hfx_cg_precond(int dtype, int nb, size_t nk, const void* G, const void* Gprev, const double* eps, const double* p,
                    const double* gp_prev, const double* st, double T, void* H, double* gp, double* hp, double* part,
-                   void* stream) {
+                   int seg_len, void* stream) {
   if (int rc = pk_check_nb(nb)) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
     hipLaunchKernelGGL(pk_cg_precond_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(double, G),
-                       PKC(double, Gprev), eps, p, gp_prev, st, T, PKM(double, H), gp, hp, part);
+                       PKC(double, Gprev), eps, p, gp_prev, st, T, PKM(double, H), gp, hp, part, seg_len);
   else
     hipLaunchKernelGGL(pk_cg_precond_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(float, G), PKC(float, Gprev),
-                       eps, p, gp_prev, st, T, PKM(float, H), gp, hp, part);
+                       eps, p, gp_prev, st, T, PKM(float, H), gp, hp, part, seg_len);
   return check_launch("pk_cg_precond_kernel");
 }
 
@@ -1024,45 +1036,45 @@ int hfx_cg_retract(int dtype, int nb, size_t nk, void* Q, const void* V, const d
 }
 
 int hfx_cg_direction_gj(int dtype, int nb, size_t nk, const void* H, void* D, const double* hp, double* dp,
-                        const double* st, double* part, void* stream) {
+                        const double* st, double* part, int seg_len, void* stream) {
   if (int rc = pk_check_nb(nb)) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
     hipLaunchKernelGGL(pk_cg_dir_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKC(double, H), PKM(double, D), hp,
-                       dp, st, part);
+                       dp, st, part, seg_len);
   else
     hipLaunchKernelGGL(pk_cg_dir_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKC(float, H), PKM(float, D), hp, dp,
-                       st, part);
+                       st, part, seg_len);
   return check_launch("pk_cg_dir_kernel");
 }
 
 int hfx_cg_trial_gj(int dtype, int nb, size_t nk, const void* D, const void* Ft, const double* p, const double* dp,
                     const double* eps, const double* st, int omega0, double* part, void* U_keep,
-                    double* eps_keep, void* stream) {
+                    double* eps_keep, int seg_len, void* stream) {
   if (int rc = pk_check_nb(nb)) return rc;
   if (!U_keep != !eps_keep) return set_error(HFX_EINVAL, "hfx_cg_trial_gj: U_keep and eps_keep go together");
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
     hipLaunchKernelGGL(pk_cg_trial_gj_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(double, D),
-                       PKC(double, Ft), p, dp, eps, st, omega0, part, PKM(double, U_keep), eps_keep);
+                       PKC(double, Ft), p, dp, eps, st, omega0, part, PKM(double, U_keep), eps_keep, seg_len);
   else
     hipLaunchKernelGGL(pk_cg_trial_gj_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(float, D),
-                       PKC(float, Ft), p, dp, eps, st, omega0, part, PKM(float, U_keep), eps_keep);
+                       PKC(float, Ft), p, dp, eps, st, omega0, part, PKM(float, U_keep), eps_keep, seg_len);
   return check_launch("pk_cg_trial_gj_kernel");
 }
 
 int hfx_cg_retract_gj(int dtype, int nb, size_t nk, void* Q, const void* D, const void* Ft, const double* st,
-                      double* eps_new, const void* U_keep, const double* eps_keep, void* stream) {
+                      double* eps_new, const void* U_keep, const double* eps_keep, int seg_len, void* stream) {
   if (int rc = pk_check_nb(nb)) return rc;
   if (!eps_new) return set_error(HFX_EINVAL, "hfx_cg_retract_gj: eps_new is required");
   if (!U_keep != !eps_keep) return set_error(HFX_EINVAL, "hfx_cg_retract_gj: U_keep and eps_keep go together");
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
     hipLaunchKernelGGL(pk_cg_retract_gj_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKM(double, Q),
-                       PKC(double, D), PKC(double, Ft), st, eps_new, PKC(double, U_keep), eps_keep);
+                       PKC(double, D), PKC(double, Ft), st, eps_new, PKC(double, U_keep), eps_keep, seg_len);
   else
     hipLaunchKernelGGL(pk_cg_retract_gj_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKM(float, Q),
-                       PKC(float, D), PKC(float, Ft), st, eps_new, PKC(float, U_keep), eps_keep);
+                       PKC(float, D), PKC(float, Ft), st, eps_new, PKC(float, U_keep), eps_keep, seg_len);
   return check_launch("pk_cg_retract_gj_kernel");
 }
 

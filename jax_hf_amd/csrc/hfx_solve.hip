@@ -498,6 +498,188 @@ __global__ __launch_bounds__(MUS_BLOCK) void mu_occ_kernel(MuStepArgs a) {
     a.p_out[e] = sigm((mu - a.eps[e]) / Tj);   // solver.py:394, 592
 }
 
+// ---- batched solves: segmented reductions and mu -------------------------------
+// nseg consecutive solves of seg_len k-points; blockIdx.y is the solve, each
+// solve has its own 32-double state vector (st + HFX_ST_LEN * s), block
+// partials and completion ticket; frozen solves skip their launches.
+constexpr int SEG_BLOCKS = 32;   // blocks per solve (k-points per solve <= 8192 at 256 threads each)
+
+struct SegScratch {
+  double* s1;          // [nseg][RED_MAX_VALS][SEG_BLOCKS]
+  double* mpart;       // [nseg][2][SEG_BLOCKS]
+  unsigned* tickets;   // [nseg][2]
+};
+
+__host__ __device__ inline SegScratch seg_scratch(double* base, int nseg) {
+  SegScratch t;
+  t.s1 = base;
+  t.mpart = base + (size_t)nseg * RED_MAX_VALS * SEG_BLOCKS;
+  t.tickets = reinterpret_cast<unsigned*>(t.mpart + (size_t)nseg * 2 * SEG_BLOCKS);
+  return t;
+}
+
+__device__ __forceinline__ bool seg_last_block(unsigned* ticket, unsigned nblk) {
+  __shared__ bool last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == nblk - 1;
+  __syncthreads();
+  if (last) __threadfence();
+  return last;
+}
+
+__global__ __launch_bounds__(RED_BLOCK) void reduce_seg_kernel(const double* __restrict__ part, int nval,
+                                                               size_t nk_total, int seg_len,
+                                                               const double* __restrict__ w, unsigned wmask,
+                                                               double* __restrict__ st_all, PostArgs pa,
+                                                               int out_slot, SegScratch scr) {
+  __shared__ double sh[RED_MAX_VALS * (RED_BLOCK / 64)];
+  const int sidx = blockIdx.y;
+  double* st = st_all + (size_t)sidx * HFX_ST_LEN;
+  if (st[HFX_ST_FROZEN] != 0.0) return;
+  if (pa.op == HFX_OP_TRIAL && st[HFX_ST_ACC] != 0.0) return;
+  const size_t chunk = ((size_t)seg_len + gridDim.x - 1) / gridDim.x;
+  const size_t k0 = (size_t)sidx * seg_len + (size_t)blockIdx.x * chunk;
+  size_t k1 = k0 + chunk;
+  const size_t kend = (size_t)(sidx + 1) * seg_len;
+  if (k1 > kend) k1 = kend;
+  double acc[RED_MAX_VALS];
+#pragma unroll
+  for (int v = 0; v < RED_MAX_VALS; ++v) acc[v] = 0.0;
+  for (size_t k = k0 + threadIdx.x; k < k1; k += RED_BLOCK) {
+    const double wk = w ? w[k] : 1.0;
+#pragma unroll
+    for (int v = 0; v < RED_MAX_VALS; ++v)
+      if (v < nval) acc[v] += ((wmask >> v) & 1u ? wk : 1.0) * part[(size_t)v * nk_total + k];
+  }
+  block_sum<RED_MAX_VALS>(acc, sh);
+  double* s1 = scr.s1 + (size_t)sidx * RED_MAX_VALS * SEG_BLOCKS;
+  if (threadIdx.x == 0)
+    for (int v = 0; v < nval; ++v) s1[(size_t)v * gridDim.x + blockIdx.x] = acc[v];
+  unsigned* ticket = scr.tickets + 2 * sidx;
+  if (!seg_last_block(ticket, gridDim.x)) return;
+  if (threadIdx.x == 0) *ticket = 0u;
+  if (pa.op == HFX_OP_SUM && out_slot >= 0) {
+    double tot[RED_MAX_VALS];
+#pragma unroll
+    for (int v = 0; v < RED_MAX_VALS; ++v) tot[v] = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += RED_BLOCK)
+#pragma unroll
+      for (int v = 0; v < RED_MAX_VALS; ++v)
+        if (v < nval) tot[v] += s1[(size_t)v * gridDim.x + b];
+    block_sum<RED_MAX_VALS>(tot, sh);
+    if (threadIdx.x == 0)
+      for (int v = 0; v < nval; ++v) st[out_slot + v] = tot[v];
+    return;
+  }
+  stage2_body(s1, nval, (int)gridDim.x, nullptr, st, pa, sh);
+}
+
+struct MuSegArgs {
+  const double* eps;
+  const double* w;
+  const double* targets;   // electrons per solve (normalised weights)
+  int seg_len, nb;
+  double T;
+  double* st_all;
+  double* p_out;
+  SegScratch scr;
+};
+
+__global__ __launch_bounds__(MUS_BLOCK) void mu_bracket_seg_kernel(MuSegArgs a) {
+  __shared__ double sh[2 * (MUS_BLOCK / 64)];
+  const int sidx = blockIdx.y;
+  double* st = a.st_all + (size_t)sidx * HFX_ST_LEN;
+  if (st[HFX_ST_FROZEN] != 0.0) return;
+  const size_t n = (size_t)a.seg_len * a.nb, base = (size_t)sidx * n, stride = (size_t)gridDim.x * MUS_BLOCK;
+  double lo = INFINITY, hi = -INFINITY;
+  for (size_t e = (size_t)blockIdx.x * MUS_BLOCK + threadIdx.x; e < n; e += stride) {
+    lo = fmin(lo, a.eps[base + e]);
+    hi = fmax(hi, a.eps[base + e]);
+  }
+  lo = wave_min(lo);
+  hi = wave_max(hi);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = MUS_BLOCK / 64;
+  if (lane == 0) { sh[wid] = lo; sh[nw + wid] = hi; }
+  __syncthreads();
+  double* mp = a.scr.mpart + (size_t)sidx * 2 * SEG_BLOCKS;
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < nw; ++q) { lo = fmin(lo, sh[q]); hi = fmax(hi, sh[nw + q]); }
+    mp[2 * blockIdx.x] = lo;
+    mp[2 * blockIdx.x + 1] = hi;
+  }
+  unsigned* ticket = a.scr.tickets + 2 * sidx + 1;
+  if (!seg_last_block(ticket, gridDim.x)) return;
+  if (threadIdx.x == 0) {
+    double mn = INFINITY, mx = -INFINITY;
+    for (unsigned b = 0; b < gridDim.x; ++b) { mn = fmin(mn, mp[2 * b]); mx = fmax(mx, mp[2 * b + 1]); }
+    const double Tj = fmax(a.T, 1e-12);
+    const double blo = mn - 50.0 * Tj, bhi = mx + 50.0 * Tj;   // solver.py:70-72
+    st[HFX_ST_MU] = fmin(fmax(st[HFX_ST_MU], blo), bhi);       // warm start from the state
+    st[HFX_ST_MULO] = blo;
+    st[HFX_ST_MUHI] = bhi;
+    st[HFX_ST_MUDONE] = 0.0;
+    *ticket = 0u;
+  }
+}
+
+__global__ __launch_bounds__(MUS_BLOCK) void mu_step_seg_kernel(MuSegArgs a) {
+  const int sidx = blockIdx.y;
+  double* st = a.st_all + (size_t)sidx * HFX_ST_LEN;
+  if (st[HFX_ST_FROZEN] != 0.0 || st[HFX_ST_MUDONE] != 0.0) return;
+  __shared__ double sh[2 * (MUS_BLOCK / 64)];
+  const size_t n = (size_t)a.seg_len * a.nb, base = (size_t)sidx * n, stride = (size_t)gridDim.x * MUS_BLOCK;
+  const double Tj = fmax(a.T, 1e-12);
+  const double mu = st[HFX_ST_MU];
+  const size_t kbase = (size_t)sidx * a.seg_len;
+  double acc[2] = {0.0, 0.0};
+  for (size_t e = (size_t)blockIdx.x * MUS_BLOCK + threadIdx.x; e < n; e += stride) {
+    const double pv = sigm((mu - a.eps[base + e]) / Tj);
+    const double wk = a.w[kbase + e / a.nb];
+    acc[0] += wk * pv;
+    acc[1] += wk * pv * (1.0 - pv) / Tj;
+  }
+  block_sum<2>(acc, sh);
+  double* mp = a.scr.mpart + (size_t)sidx * 2 * SEG_BLOCKS;
+  if (threadIdx.x == 0) {
+    mp[2 * blockIdx.x] = acc[0];
+    mp[2 * blockIdx.x + 1] = acc[1];
+  }
+  unsigned* ticket = a.scr.tickets + 2 * sidx + 1;
+  if (!seg_last_block(ticket, gridDim.x)) return;
+  double N, Z;
+  block_gather2(mp, (int)gridDim.x, sh, N, Z);
+  if (threadIdx.x == 0) {
+    const double g = N - a.targets[sidx];   // solver.py:80-99
+    if (fabs(g / fmax(Z, 1e-18)) <= 8.9e-16 * fmax(fabs(mu), Tj)) {
+      st[HFX_ST_MUDONE] = 1.0;
+    } else {
+      double lo = st[HFX_ST_MULO], hi = st[HFX_ST_MUHI];
+      if (g < 0) lo = mu;
+      if (g > 0) hi = mu;
+      double mu_new = mu - g / fmax(Z, 1e-18);
+      const double mu_bis = 0.5 * (lo + hi);
+      if (mu_new <= lo || mu_new >= hi) mu_new = mu_bis;
+      mu_new = fmin(fmax(mu_new, lo), hi);
+      if (!isfinite(mu_new)) mu_new = mu_bis;
+      st[HFX_ST_MU] = mu_new;
+      st[HFX_ST_MULO] = lo;
+      st[HFX_ST_MUHI] = hi;
+    }
+    *ticket = 0u;
+  }
+}
+
+__global__ __launch_bounds__(MUS_BLOCK) void mu_occ_seg_kernel(MuSegArgs a) {
+  const int sidx = blockIdx.y;
+  const double* st = a.st_all + (size_t)sidx * HFX_ST_LEN;
+  if (st[HFX_ST_FROZEN] != 0.0) return;
+  const size_t n = (size_t)a.seg_len * a.nb, base = (size_t)sidx * n, stride = (size_t)gridDim.x * MUS_BLOCK;
+  const double Tj = fmax(a.T, 1e-12), mu = st[HFX_ST_MU];
+  for (size_t e = (size_t)blockIdx.x * MUS_BLOCK + threadIdx.x; e < n; e += stride)
+    a.p_out[base + e] = sigm((mu - a.eps[base + e]) / Tj);
+}
+
 static int g_mu_blocks = 0;
 
 // Co-resident grid for the cooperative launch, sized to ~8 eigenvalues per
@@ -576,6 +758,63 @@ int hfx_solve_mu_steps(const double* eps, const double* w, size_t nk, int nb, do
     return check_launch("mu_occ_kernel");
   }
   return HFX_OK;
+}
+
+size_t hfx_batch_scratch_doubles(int nseg) {
+  if (nseg < 1) nseg = 1;
+  return (size_t)nseg * (RED_MAX_VALS * SEG_BLOCKS + 2 * SEG_BLOCKS + 1);
+}
+
+int hfx_reduce_batched(const double* part, int nval, size_t nk_total, int seg_len, const double* w, unsigned wmask,
+                       double* st, int op, const double* params, int out_slot, double* scratch, void* stream) {
+  if (!part || !st || !scratch || nval < 1 || nval > RED_MAX_VALS || seg_len < 1 || nk_total % (size_t)seg_len)
+    return set_error(HFX_EINVAL, "hfx_reduce_batched: bad argument");
+  if (out_slot >= 0 && (op != HFX_OP_SUM || out_slot + nval > HFX_ST_LEN))
+    return set_error(HFX_EINVAL, "hfx_reduce_batched: out_slot only with HFX_OP_SUM, inside the state vector");
+  const int nseg = (int)(nk_total / seg_len);
+  PostArgs pa{};
+  pa.op = op;
+  if (params) {
+    pa.T = params[0];
+    pa.denom_scale = params[1];
+    pa.count = params[2];
+    pa.max_step = params[3];
+    pa.shrink = params[4];
+    pa.tau_fallback = params[5];
+    pa.restart = (int)params[6];
+    pa.last = (int)params[7];
+    pa.omega0 = (int)params[8];
+    pa.restart_every = (int)params[9];
+  }
+  int g = (seg_len + RED_BLOCK - 1) / RED_BLOCK;
+  if (g > SEG_BLOCKS) g = SEG_BLOCKS;
+  hipLaunchKernelGGL(reduce_seg_kernel, dim3((unsigned)g, (unsigned)nseg), dim3(RED_BLOCK), 0, (hipStream_t)stream,
+                     part, nval, nk_total, seg_len, w, wmask, st, pa, out_slot, seg_scratch(scratch, nseg));
+  return check_launch("reduce_seg_kernel");
+}
+
+int hfx_solve_mu_batched(const double* eps, const double* w, const double* targets, size_t nk_total, int seg_len,
+                         int nb, double T, int maxiter, double* st, double* p_out, double* scratch, void* stream) {
+  if (!eps || !w || !targets || !st || !scratch || nb < 1 || seg_len < 1 || nk_total % (size_t)seg_len ||
+      maxiter < 0)
+    return set_error(HFX_EINVAL, "hfx_solve_mu_batched: bad argument");
+  const int nseg = (int)(nk_total / seg_len);
+  const size_t n = (size_t)seg_len * nb;
+  size_t g = (n + (size_t)MUS_BLOCK * 4 - 1) / ((size_t)MUS_BLOCK * 4);
+  if (g > SEG_BLOCKS) g = SEG_BLOCKS;
+  if (g < 1) g = 1;
+  MuSegArgs a{eps, w, targets, seg_len, nb, T, st, p_out, seg_scratch(scratch, nseg)};
+  const dim3 grid((unsigned)g, (unsigned)nseg);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(mu_bracket_seg_kernel, grid, dim3(MUS_BLOCK), 0, s, a);
+  if (int rc = check_launch("mu_bracket_seg_kernel")) return rc;
+  for (int it = 0; it < maxiter; ++it) {
+    hipLaunchKernelGGL(mu_step_seg_kernel, grid, dim3(MUS_BLOCK), 0, s, a);
+    if (int rc = check_launch("mu_step_seg_kernel")) return rc;
+  }
+  if (!p_out) return HFX_OK;
+  hipLaunchKernelGGL(mu_occ_seg_kernel, grid, dim3(MUS_BLOCK), 0, s, a);
+  return check_launch("mu_occ_seg_kernel");
 }
 
 int hfx_solve_mu(const double* eps, const double* w, size_t nk, int nb, double target, double T, double mu0,

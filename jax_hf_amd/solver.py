@@ -272,14 +272,14 @@ class _CG:
                  params=[self.T, cfg.denom_scale, nk * nb])                     # :491-497
         L.call("hfx_cg_precond", dt, nb, nk, L.ptr(self.G), L.ptr(self.Gp), L.ptr(self.eps), L.ptr(self.p),
                L.ptr(self.gpp), L.ptr(st), self.T, L.ptr(self.H), L.ptr(self.gp), L.ptr(self.hp),
-               L.ptr(part), sm)                                                 # :486, 499-504
+               L.ptr(part), 0, sm)                                              # :486, 499-504
         # restart when st[ITER] % cg_restart == 0 (k == 0 included), :522-526
         K.reduce(ws, part, 6, nk, w=self.w_norm, wmask=0b111111, st=st, op=K.OP_BETA,
                  params=[0, 0, 0, 0, 0, 0, 0, 0, 0, max(int(cfg.cg_restart), 1)])   # :517-526
         gj = self.gauss_jordan
         if gj:    # Cayley factor per trial by Gauss-Jordan: no eigh in the loop
             L.call("hfx_cg_direction_gj", dt, nb, nk, L.ptr(self.H), L.ptr(self.D), L.ptr(self.hp),
-                   L.ptr(self.dp), L.ptr(st), L.ptr(part), sm)
+                   L.ptr(self.dp), L.ptr(st), L.ptr(part), 0, sm)
         else:     # spectral setup: eigh(i d) once, V^dagger Ft V  (:538-539)
             L.call("hfx_cg_direction", dt, nb, nk, L.ptr(self.H), L.ptr(self.D), L.ptr(self.hp), L.ptr(self.dp),
                    L.ptr(self.Ft), L.ptr(st), L.ptr(self.V), L.ptr(self.lam), L.ptr(self.Fe), L.ptr(part), sm)
@@ -289,7 +289,7 @@ class _CG:
             if gj:
                 L.call("hfx_cg_trial_gj", dt, nb, nk, L.ptr(self.D), L.ptr(self.Ft), L.ptr(self.p),
                        L.ptr(self.dp), L.ptr(self.eps), L.ptr(st), int(i == 0), L.ptr(part),
-                       L.ptr(self.V), L.ptr(self.lam), sm)
+                       L.ptr(self.V), L.ptr(self.lam), 0, sm)
             else:
                 L.call("hfx_cg_trial", dt, nb, nk, L.ptr(self.V), L.ptr(self.lam), L.ptr(self.Fe), L.ptr(self.p),
                        L.ptr(self.dp), L.ptr(self.eps), L.ptr(st), int(i == 0), L.ptr(part), sm)
@@ -298,7 +298,7 @@ class _CG:
                              int(i == 0)])
         if gj:
             L.call("hfx_cg_retract_gj", dt, nb, nk, L.ptr(self.Q), L.ptr(self.D), L.ptr(self.Ft), L.ptr(st),
-                   L.ptr(self.eps), L.ptr(self.V), L.ptr(self.lam), sm)         # :586-590
+                   L.ptr(self.eps), L.ptr(self.V), L.ptr(self.lam), 0, sm)      # :586-590
         else:
             L.call("hfx_cg_retract", dt, nb, nk, L.ptr(self.Q), L.ptr(self.V), L.ptr(self.lam), L.ptr(self.Fe),
                    L.ptr(st), 0.0, 0, L.ptr(self.eps), None, sm)                # :586-590
@@ -363,7 +363,7 @@ def _solve_cg(kernel, P0, n_electrons, cfg: SolverConfig, fock_build_fn):
     return _result(runner, Pfin, Ffin, sv, k, converged, hist_E, hist_G, cfg.max_iter, dev)
 
 
-def _polish_diis(kernel, runner, n_electrons, cfg, fock_build_fn, k_cg, hist_E, hist_G):
+def _polish_diis(kernel, runner, n_electrons, cfg, fock_build_fn, k_cg, hist_E, hist_G, P_lo=None):
     """FP64 Pulay-DIIS SCF from the CG density to the fixed point [F, P] = 0.
 
     The CG loop's windowed energy stop (solver.py:422-449) can end well before
@@ -374,7 +374,8 @@ def _polish_diis(kernel, runner, n_electrons, cfg, fock_build_fn, k_cg, hist_E, 
     """
     from .reference_scf import SCFConfig, _SCF
     dev = kernel.device
-    P_lo = runner.density()
+    if P_lo is None:
+        P_lo = runner.density()
     P64 = convert(P_lo, torch.complex128) if P_lo.dtype != torch.complex128 else P_lo.clone()
     scfg = SCFConfig(max_iter=cfg.polish_max_iter, acceleration="diis", density_tol=cfg.polish_tol,
                      comm_tol=cfg.polish_tol, project_fn=cfg.project_fn)
