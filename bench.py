@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the built-in exchange-kernel variant instead of timing all of them first")
     ap.add_argument("--fock-only", action="store_true", help="only the Fock-build measurement")
+    ap.add_argument("--no-scan", action="store_true", help="skip the 64-solve density scan")
+    ap.add_argument("--scan-solves", type=int, default=64)
     ap.add_argument("--extra", action="store_true", help="also time the packed and complex64 Fock builds")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of the exchange kernel's byte count "
@@ -205,6 +207,57 @@ def solve_bench(J, inp, dev, nk, nb, args):
 
 
 # ---------------------------------------------------------------------------
+# density scan: many independent solves, one by one vs stacked
+# ---------------------------------------------------------------------------
+
+def scan_bench(J, syn, dev, args):
+    """64 independent solves (nk ~ U{36, 48, 60, 72, 90}, filling nb/2 + nu,
+    nu ~ U(-0.5, 0.5), displacement u ~ U(0, 0.4), nb = 16, seed 3; h and V
+    by the Fock-build recipes), each solved by ``solve`` one at a time and by
+    ``solve_batch`` (stacked per mesh).  Forms: FP64 CG and mixed
+    (FP32 CG + FP64 DIIS polish)."""
+    import torch
+    nb = 16
+    specs = syn.config3_specs(nsolves=args.scan_solves, nb=nb, seed=3)
+    t0 = time.perf_counter()
+    kernels, P0s, ne = [], [], []
+    cache = {}
+    for sp in specs:
+        nk = sp["nk"]
+        if nk not in cache:
+            cache[nk] = (syn.tight_binding(nk, nk, nb, seed=1), syn.coulomb_layer(nk, nk, nb, 2),
+                         syn.uniform_weights(nk, nk))
+        h0, V, w = cache[nk]
+        kernels.append(J.HartreeFockKernel(w, np.ascontiguousarray(h0 + syn.displacement(nb, sp["u"])), V,
+                                           SOLVE_T, device=dev))
+        P0s.append(np.zeros((nk, nk, nb, nb), np.complex128))
+        ne.append(sp["n_electrons"])
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    out = {"n_solves": len(specs), "nk_counts": {str(k): sum(1 for s_ in specs if s_["nk"] == k)
+                                                 for k in sorted(cache)},
+           "kernel_setup_s": setup}
+    for name, cfg in (("fp64_cg", J.SolverConfig(max_iter=300, tol_E=1e-9, precision="fp64")),
+                      ("mixed", J.SolverConfig(max_iter=300, precision="mixed", polish_tol=1e-8))):
+        J.solve_batch(kernels[:2], P0s[:2], ne[:2], config=cfg)          # first use of the code paths
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        serial = [J.solve(k, P, n, config=cfg) for k, P, n in zip(kernels, P0s, ne)]
+        torch.cuda.synchronize()
+        ts = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        batched = J.solve_batch(kernels, P0s, ne, config=cfg)
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - t0
+        dE = max(abs(float(a.energy) - float(b.energy)) for a, b in zip(serial, batched))
+        out[name] = {"serial_s": ts, "batched_s": tb, "serial_solves_per_s": len(specs) / ts,
+                     "batched_solves_per_s": len(specs) / tb, "speedup": ts / tb,
+                     "max_abs_dE_batched_vs_serial": dE,
+                     "all_converged": all(bool(r.converged) for r in batched)}
+    return out
+
+
+# ---------------------------------------------------------------------------
 # CPU baseline (oracle on the host cores)
 # ---------------------------------------------------------------------------
 
@@ -295,6 +348,8 @@ def main():
                                             f"T={SOLVE_T}, half filling, P0=0"),
                                "nk": nk, "nb": nb, "parallelism": "1 GPU"},
                     "solves": solves})
+        if not args.no_scan:
+            out["scan"] = scan_bench(J, syn, dev, args)
     out["fock"] = fock
     out["fock_build_GBps"] = fock["GBps"]
     out["roofline"] = roof
