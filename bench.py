@@ -92,21 +92,42 @@ def fock_bench(J, L, syn, inp, dev, nk, nb, args):
     E = eng.E
     st = L.stream(dev)
     variant_ms = None
+    planar = J.fock.PLANAR
+    nch = nb * nb
+    Pt, St = eng.planar_workspace(nch)
+    VRc = eng.vr_planar()
+    h = eng.plan.handle
+    if planar:
+        L.call("hfx_planar_pack", h, 1, L.ptr(P), None, None, 0, 0, L.ptr(Pt), st)
     if not args.no_autotune:
-        _, variant_ms = eng.plan.autotune(P, eng.VR, Sig, reps=10)
+        if planar:
+            _, variant_ms = eng.plan.autotune(Pt, VRc, St, reps=10, planar=True)
+        else:
+            _, variant_ms = eng.plan.autotune(P, eng.VR, Sig, reps=10)
     kname = {1: "exch_ct_kernel"}.get(eng.plan.exchange_variant, "exch_v2_kernel")
     Nk = nk * nk
     bytes_exch = 16 * Nk * (3 * nb * nb)               # P + VR in, Sigma out
     bytes_build = 16 * Nk * (4 * nb * nb + nb * nb)    # + h in, F out
 
     def step(evs=None):
+        # the steps of FockEngine.build, spelled out so the exchange launch can be timed
+        if planar:
+            L.call("hfx_planar_pack", h, 1, L.ptr(P), None, None, 0, 0, L.ptr(Pt), st)
         if evs is not None:
             evs[0].record()
-        eng.selfenergy(P, Sig, refP=None, rho=False)
+        if planar:
+            L.call("hfx_selfenergy_planar", h, 1, L.ptr(Pt), L.ptr(VRc), eng.vfull, None, 0, 0, 0, None,
+                   L.ptr(St), None, st)
+        else:
+            eng.selfenergy(P, Sig, refP=None, rho=False)
         if evs is not None:
             evs[1].record()
-        L.call("hfx_fock_epilogue", eng.plan.handle, L.ptr(eng.h), L.ptr(Sig), None, None, L.ptr(P),
-               L.ptr(eng.w2d), L.ptr(F), None, L.ptr(eng.plan.scratch), L.ptr(E), 0.0, st)
+        if planar:
+            L.call("hfx_fock_epilogue_planar", h, L.ptr(eng.h), L.ptr(St), None, 0, 0, None, None, L.ptr(P),
+                   L.ptr(eng.w2d), L.ptr(F), None, L.ptr(eng.plan.scratch), L.ptr(E), 0.0, st)
+        else:
+            L.call("hfx_fock_epilogue", h, L.ptr(eng.h), L.ptr(Sig), None, None, L.ptr(P),
+                   L.ptr(eng.w2d), L.ptr(F), None, L.ptr(eng.plan.scratch), L.ptr(E), 0.0, st)
 
     for _ in range(20):
         step()
@@ -124,6 +145,7 @@ def fock_bench(J, L, syn, inp, dev, nk, nb, args):
     out = {"builds_per_s": n / dt, "ms_per_build": dt * 1e3 / n,
            "GBps": bytes_build / (dt / n) / 1e9, "algorithmic_bytes_build": bytes_build,
            "energy": float(E), "exchange_variant": L.Plan.VARIANTS[eng.plan.exchange_variant],
+           "layout": "slot planes" if planar else "(k, i, j)",
            "exchange_variants_ms": variant_ms}
     roof = {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": _pmc_traffic(kname, nk, nb),

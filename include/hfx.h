@@ -55,13 +55,15 @@ int hfx_plan_get_exchange_variant(const hfx_plan* plan);
 
 /* Times every exchange variant (reps launches each, all nb*nb channels, on the
  * caller's buffers; Sigma is overwritten) and keeps the fastest on the plan.
+ * planar: P / Sigma are slot planes and VR channel-major (hfx_selfenergy_planar).
  * times_ms (optional, 5 doubles) receives the mean time per launch of variants
  * 0 .. 4 (0 = AUTO's choice is not timed and is reported as 0).  Synchronises
  * the stream.  No counterpart in the reference (XLA autotunes its own FFTs). */
-int hfx_exchange_autotune(hfx_plan* plan, const void* P, const void* VR, int vr_full, void* Sigma, int reps,
-                          double* times_ms, int* best, void* stream);
+int hfx_exchange_autotune(hfx_plan* plan, const void* P, const void* VR, int vr_full, int planar, void* Sigma,
+                          int reps, double* times_ms, int* best, void* stream);
 
-/* Bytes of caller-provided float64 scratch needed by the energy reductions. */
+/* Bytes of caller-provided float64 scratch needed by the energy reductions
+ * (calls sharing one scratch must not overlap). */
 size_t hfx_energy_scratch_bytes(const hfx_plan* plan);
 
 /* Exchange self-energy on a channel list (selfenergy_fft, utils.py:66-173 and
@@ -112,6 +114,29 @@ int hfx_kterms(const hfx_plan* plan, const void* rho, const double* HH, int ntc,
 int hfx_fock_epilogue(const hfx_plan* plan, const void* h, const void* Sigma, const double* Hdiag,
                       const void* sigc, const void* P, const double* w2d, void* F, void* Sigma_out,
                       double* scratch, double* E_out, double e_offset, void* stream);
+
+/* ---- planar (channel-major) Fock build --------------------------------------
+ * The same build as hfx_selfenergy + hfx_fock_epilogue (fock.py:76-111), with
+ * the exchange operands staged as slot planes Xt[s][t][k] = X[s][k][chans[t]]
+ * so that every exchange load and store is coalesced.  chans NULL selects the
+ * upper triangle (herm) or all nb^2 channels.  Replaces the same reference
+ * calls as the (k, i, j)-layout entry points above. */
+/* X (nsolve, nk1, nk2, nb, nb) [- refX (nk1, nk2, nb, nb)] -> Xt (nsolve, nchan, nk1, nk2).
+ * Also makes the channel-major VR (herm 0, chans NULL) that hfx_selfenergy_planar reads. */
+int hfx_planar_pack(const hfx_plan* plan, int nsolve, const void* X, const void* refX, const int* chans, int nchan,
+                    int herm, void* Xt, void* stream);
+/* Sigma_t = exchange of the slot planes Pt (nsolve, nchan, nk1, nk2); VRc the
+ * channel-major VR (nb^2, nk1, nk2) when vr_full, else (nk1, nk2); rho (nb, nb)
+ * weighted moments of Pt (nsolve == 1 only, mirrored with herm). */
+int hfx_selfenergy_planar(const hfx_plan* plan, int nsolve, const void* Pt, const void* VRc, int vr_full,
+                          const int* chans, int nchan, int herm, int ifftshift, const double* w2d, void* Sigt,
+                          void* rho, void* stream);
+/* hfx_fock_epilogue reading Sigma from slot planes (mirrored with herm;
+ * channels outside a given chans list are zero). */
+int hfx_fock_epilogue_planar(const hfx_plan* plan, const void* h, const void* Sigt, const int* chans, int nchan,
+                             int herm, const double* Hdiag, const void* sigc, const void* P, const double* w2d,
+                             void* F, void* Sigma_out, double* scratch, double* E_out, double e_offset,
+                             void* stream);
 
 /* hf_energy (fock.py:100-111) on given Sigma / H; H broadcast (1,1,nb,nb) when
  * H_bcast = 1, full (nk1,nk2,nb,nb) when 0, absent when NULL. */
@@ -326,6 +351,12 @@ int hfx_selfenergy_batched(const hfx_plan* plan, int nsolve, const void* P, cons
  * w2d (nsolve, nk1, nk2); scratch >= 64 * nsolve doubles. */
 int hfx_fock_epilogue_batched(const hfx_plan* plan, int nsolve, const void* h, const void* Sigma, const void* P,
                               const double* w2d, void* F, double* st, double* scratch, void* stream);
+/* hfx_fock_epilogue_batched reading Sigma from the slot planes of
+ * hfx_selfenergy_planar (default channel set for herm); scratch >= 256 * nsolve
+ * doubles. */
+int hfx_fock_epilogue_planar_batched(const hfx_plan* plan, int nsolve, const void* h, const void* Sigt, int herm,
+                                     const void* P, const double* w2d, void* F, double* st, double* scratch,
+                                     void* stream);
 /* Scratch (float64) for hfx_reduce_batched / hfx_solve_mu_batched. */
 size_t hfx_batch_scratch_doubles(int nseg);
 /* hfx_reduce per solve (segments of seg_len k-points); with op == HFX_OP_SUM

@@ -18,7 +18,7 @@ LIB_PATH = _HERE / "libhfx.so"
 
 HFX_C128 = 0
 HFX_C64 = 1
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 _lib = None
 _lock = threading.Lock()
@@ -37,7 +37,7 @@ _SIGS = {
     "hfx_energy_scratch_bytes": [vp],
     "hfx_plan_set_exchange_variant": [vp, i32],
     "hfx_plan_get_exchange_variant": [vp],
-    "hfx_exchange_autotune": [vp, vp, vp, i32, vp, i32, vp, vp, vp],
+    "hfx_exchange_autotune": [vp, vp, vp, i32, i32, vp, i32, vp, vp, vp],
     "hfx_selfenergy": [vp, vp, vp, vp, i32, vp, i32, i32, i32, vp, vp, vp, vp],
     "hfx_prepare_vr": [vp, vp, i32, vp, i32, vp, vp],
     "hfx_selfenergy_adjoint": [vp, vp, vp, i32, vp, vp],
@@ -66,6 +66,10 @@ _SIGS = {
     "hfx_selfenergy_batched": [vp, i32, vp, vp, i32, i32, vp, vp],
     "hfx_fock_epilogue_batched": [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp],
     "hfx_batch_scratch_doubles": [i32],
+    "hfx_planar_pack": [vp, i32, vp, vp, vp, i32, i32, vp, vp],
+    "hfx_selfenergy_planar": [vp, i32, vp, vp, i32, vp, i32, i32, i32, vp, vp, vp, vp],
+    "hfx_fock_epilogue_planar": [vp, vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, f64, vp],
+    "hfx_fock_epilogue_planar_batched": [vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp],
     "hfx_reduce_batched": [vp, i32, sz, i32, vp, C.c_uint, vp, i32, vp, i32, vp, vp],
     "hfx_solve_mu_batched": [vp, vp, vp, sz, i32, i32, f64, i32, vp, vp, vp, vp],
     "hfx_cg_retract": [i32, i32, sz, vp, vp, vp, vp, vp, f64, i32, vp, vp, vp],
@@ -162,7 +166,7 @@ class Plan:
             call("hfx_plan_create", nk1, nk2, nb, dtype, C.byref(h))
         self.handle = h
         nbytes = lib().hfx_energy_scratch_bytes(h)
-        self.scratch = torch.empty(nbytes // 8, dtype=torch.float64, device=device)
+        self.scratch = torch.zeros(nbytes // 8, dtype=torch.float64, device=device)
 
     VARIANTS = {0: "auto", 1: "v1", 2: "v2_a", 3: "v2_b", 4: "v2_c"}
 
@@ -174,16 +178,20 @@ class Plan:
     def exchange_variant(self, v: int):
         call("hfx_plan_set_exchange_variant", self.handle, int(v))
 
-    def autotune(self, P, VR, Sigma, reps: int = 5):
+    def autotune(self, P, VR, Sigma, reps: int = 5, planar: bool = False):
         """Time every exchange-kernel variant on (P, VR) -> Sigma; keep the fastest.
 
-        Returns (best variant, {name: ms per launch}).
+        ``planar``: P / Sigma are nb^2 slot planes and VR the channel-major VR
+        (or the scalar VR).  Returns (best variant, {name: ms per launch}).
         """
         t = (C.c_double * 5)()
         best = C.c_int()
-        vfull = 0 if VR.shape[-1] == 1 else 1
+        if planar:
+            vfull = 1 if VR.numel() > P.numel() // (self.nb * self.nb) else 0
+        else:
+            vfull = 0 if VR.shape[-1] == 1 else 1
         with torch.cuda.device(self.device):
-            call("hfx_exchange_autotune", self.handle, ptr(P), ptr(VR), vfull, ptr(Sigma), int(reps), t,
+            call("hfx_exchange_autotune", self.handle, ptr(P), ptr(VR), vfull, int(planar), ptr(Sigma), int(reps), t,
                  C.byref(best), stream(self.device))
         return best.value, {self.VARIANTS[v]: t[v] for v in range(1, 5)}
 

@@ -10,7 +10,9 @@ kernel serves all of them --
 * the per-k CG kernels run over the stacked k-points, each k-point reading
   the state vector of its own solve (``seg_len``);
 * the exchange runs all solves' channel planes in one launch with the shared
-  VR (``hfx_selfenergy_batched``), the epilogue writes one energy per solve;
+  VR (``hfx_planar_pack`` + ``hfx_selfenergy_planar``, or
+  ``hfx_selfenergy_batched`` on the (k, i, j) layout), the epilogue writes
+  one energy per solve;
 * reductions, the line-search scalar steps and the mu solver are segmented
   per solve (``hfx_reduce_batched``, ``hfx_solve_mu_batched``);
 * a solve that meets its stopping rule is frozen (its state vector's flag):
@@ -32,6 +34,7 @@ import torch
 
 from . import _lib as L
 from . import _ops as K
+from . import fock as fock_mod
 from ._util import KIND, convert, to_tensor
 from .solver import SolveResult, SolverConfig, _polish_diis, _precision
 
@@ -79,13 +82,26 @@ class _BatchCG:
         r = dict(dtype=torch.float64, device=dev)
         z = lambda: torch.zeros(self.shape4, **c)  # noqa: E731
         self.Q, self.P, self.F, self.Ft, self.G, self.Gp = z(), z(), z(), z(), z(), z()
-        self.H, self.D, self.V, self.Sig = z(), z(), z(), z()
+        self.H, self.D, self.V = z(), z(), z()
+        self.planar = fock_mod.PLANAR
+        if self.planar:
+            nch = nb * (nb + 1) // 2 if self.herm else nb * nb
+            self.Pt = torch.empty((S, nch, nk1, nk2), **c)
+            self.St = torch.empty((S, nch, nk1, nk2), **c)
+            if self.vfull:
+                self.VRc = torch.empty((nb * nb, nk1, nk2), **c)
+                L.call("hfx_planar_pack", self.plan.handle, 1, L.ptr(self.VR.contiguous()), None, None, 0, 0,
+                       L.ptr(self.VRc), L.stream(dev))
+            else:
+                self.VRc = self.VR.contiguous()
+        else:
+            self.Sig = z()
         v = lambda: torch.zeros((S, nk1, nk2, nb), **r)  # noqa: E731
         self.p, self.eps, self.gp, self.gpp, self.hp, self.dp, self.lam = v(), v(), v(), v(), v(), v(), v()
         self.st = torch.zeros((S, K.ST_LEN), **r)
         self.part = torch.zeros((8, self.nk_tot), **r)
         self.scratch = torch.zeros(int(L.lib().hfx_batch_scratch_doubles(S)), **r)
-        self.epi = torch.zeros(64 * S, **r)
+        self.epi = torch.zeros(256 * S, **r)   # energy partials, 256 per solve
         self.targets = torch.zeros(S, **r)
 
     # -- pieces ---------------------------------------------------------------
@@ -99,9 +115,18 @@ class _BatchCG:
 
     def fock(self, P, F):
         sm = L.stream(self.dev)
-        L.call("hfx_selfenergy_batched", self.plan.handle, self.S, L.ptr(P), L.ptr(self.VR), self.vfull,
+        h = self.plan.handle
+        if self.planar:
+            herm = int(self.herm)
+            L.call("hfx_planar_pack", h, self.S, L.ptr(P), None, None, 0, herm, L.ptr(self.Pt), sm)
+            L.call("hfx_selfenergy_planar", h, self.S, L.ptr(self.Pt), L.ptr(self.VRc), self.vfull, None, 0, herm, 0,
+                   None, L.ptr(self.St), None, sm)
+            L.call("hfx_fock_epilogue_planar_batched", h, self.S, L.ptr(self.h), L.ptr(self.St), herm, L.ptr(P),
+                   L.ptr(self.w_raw), L.ptr(F), L.ptr(self.st), L.ptr(self.epi), sm)
+            return
+        L.call("hfx_selfenergy_batched", h, self.S, L.ptr(P), L.ptr(self.VR), self.vfull,
                int(self.herm), L.ptr(self.Sig), sm)
-        L.call("hfx_fock_epilogue_batched", self.plan.handle, self.S, L.ptr(self.h), L.ptr(self.Sig), L.ptr(P),
+        L.call("hfx_fock_epilogue_batched", h, self.S, L.ptr(self.h), L.ptr(self.Sig), L.ptr(P),
                L.ptr(self.w_raw), L.ptr(F), L.ptr(self.st), L.ptr(self.epi), sm)
 
     def mu(self):

@@ -70,7 +70,7 @@ using namespace hfx;
 
 extern "C" {
 
-int hfx_abi_version(void) { return 4; }
+int hfx_abi_version(void) { return 5; }
 
 int hfx_last_error(char* buf, size_t n) {
   if (!buf || n == 0) return HFX_EINVAL;
@@ -166,7 +166,8 @@ int hfx_plan_destroy(hfx_plan* plan) {
 
 size_t hfx_energy_scratch_bytes(const hfx_plan* plan) {
   (void)plan;
-  return (size_t)energy_partials_max() * sizeof(double);
+  const int n = energy_partials_max() > planar_epi_max_blocks() ? energy_partials_max() : planar_epi_max_blocks();
+  return (size_t)n * sizeof(double);
 }
 
 #define HFX_REQUIRE(cond, ...) \
@@ -201,6 +202,40 @@ int hfx_selfenergy_batched(const hfx_plan* plan, int nsolve, const void* P, cons
                          0, 0, 0, (hipStream_t)stream, nsolve);
 }
 
+// Default channel set when chans is NULL: upper triangle (herm) or all nb^2.
+// *cover: the set (with the Hermitian mirror) reaches every channel.
+static void resolve_chans(const hfx_plan* plan, const int*& chans, int& nchan, int herm, int* cover) {
+  if (!chans) {
+    chans = herm ? plan->chans_triu : plan->chans_full;
+    nchan = herm ? plan->nb * (plan->nb + 1) / 2 : plan->nb * plan->nb;
+    if (cover) *cover = 1;
+  } else if (cover) {
+    *cover = 0;
+  }
+}
+
+int hfx_planar_pack(const hfx_plan* plan, int nsolve, const void* X, const void* refX, const int* chans, int nchan,
+                    int herm, void* Xt, void* stream) {
+  HFX_REQUIRE(plan && X && Xt, "hfx_planar_pack: NULL argument");
+  HFX_REQUIRE(nsolve >= 1, "hfx_planar_pack: nsolve=%d", nsolve);
+  resolve_chans(plan, chans, nchan, herm, nullptr);
+  HFX_REQUIRE(nchan >= 1 && nchan <= plan->nb * plan->nb, "hfx_planar_pack: nchan=%d out of range", nchan);
+  return pack_planar_launch(plan, nsolve, X, refX, chans, nchan, Xt, (hipStream_t)stream);
+}
+
+int hfx_selfenergy_planar(const hfx_plan* plan, int nsolve, const void* Pt, const void* VRc, int vr_full,
+                          const int* chans, int nchan, int herm, int ifftshift, const double* w2d, void* Sigt,
+                          void* rho, void* stream) {
+  HFX_REQUIRE(plan && Pt && VRc && Sigt, "hfx_selfenergy_planar: NULL argument");
+  HFX_REQUIRE(!rho || (w2d && nsolve == 1), "hfx_selfenergy_planar: rho needs w2d and one solve");
+  HFX_REQUIRE(nsolve >= 1 && (long long)nsolve * plan->nb * plan->nb <= (1LL << 30),
+              "hfx_selfenergy_planar: nsolve=%d out of range", nsolve);
+  resolve_chans(plan, chans, nchan, herm, nullptr);
+  HFX_REQUIRE(nchan >= 0 && nchan <= plan->nb * plan->nb, "hfx_selfenergy_planar: nchan=%d out of range", nchan);
+  return exchange_launch(plan, Pt, nullptr, VRc, vr_full ? 1 : 0, chans, nchan, herm ? 1 : 0, w2d, Sigt, rho, 0,
+                         ifftshift ? 1 : 0, 0, (hipStream_t)stream, nsolve, /*planar=*/1);
+}
+
 int hfx_selfenergy_adjoint(const hfx_plan* plan, const void* G, const void* VR, int vr_full, void* out,
                            void* stream) {
   HFX_REQUIRE(plan && G && VR && out, "hfx_selfenergy_adjoint: NULL argument");
@@ -224,8 +259,8 @@ int hfx_plan_set_exchange_variant(hfx_plan* plan, int variant) {
 
 int hfx_plan_get_exchange_variant(const hfx_plan* plan) { return plan ? plan->exch_variant : HFX_EINVAL; }
 
-int hfx_exchange_autotune(hfx_plan* plan, const void* P, const void* VR, int vr_full, void* Sigma, int reps,
-                          double* times_ms, int* best, void* stream) {
+int hfx_exchange_autotune(hfx_plan* plan, const void* P, const void* VR, int vr_full, int planar, void* Sigma,
+                          int reps, double* times_ms, int* best, void* stream) {
   HFX_REQUIRE(plan && P && VR && Sigma, "hfx_exchange_autotune: NULL argument");
   HFX_REQUIRE(reps > 0 && reps <= 1000, "hfx_exchange_autotune: reps must be in 1..1000");
   const hipStream_t s = (hipStream_t)stream;
@@ -246,7 +281,7 @@ int hfx_exchange_autotune(hfx_plan* plan, const void* P, const void* VR, int vr_
       if (r == 0 && hipEventRecord(e0, s) != hipSuccess) rc = set_error(HFX_EHIP, "hipEventRecord failed");
       if (rc == HFX_OK)
         rc = exchange_launch(plan, P, nullptr, VR, vr_full, plan->chans_full, nch, 0, nullptr, Sigma, nullptr,
-                             0, 0, 0, s);
+                             0, 0, 0, s, 1, planar ? 1 : 0);
     }
     float ms = 0.f;
     if (rc == HFX_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
@@ -305,6 +340,34 @@ int hfx_fock_epilogue(const hfx_plan* plan, const void* h, const void* Sigma, co
   HFX_REQUIRE(!E_out || (P && w2d && scratch), "hfx_fock_epilogue: energy needs P, w2d and scratch");
   return epilogue_launch(plan, h, Sigma, Hdiag, sigc, P, w2d, F, Sigma_out, scratch, E_out, e_offset,
                          (hipStream_t)stream);
+}
+
+int hfx_fock_epilogue_planar(const hfx_plan* plan, const void* h, const void* Sigt, const int* chans, int nchan,
+                             int herm, const double* Hdiag, const void* sigc, const void* P, const double* w2d,
+                             void* F, void* Sigma_out, double* scratch, double* E_out, double e_offset,
+                             void* stream) {
+  HFX_REQUIRE(plan && h && Sigt && F, "hfx_fock_epilogue_planar: h, Sigt and F are required");
+  HFX_REQUIRE(!E_out || (P && w2d && scratch), "hfx_fock_epilogue_planar: energy needs P, w2d and scratch");
+  int cover = 0;
+  resolve_chans(plan, chans, nchan, herm, &cover);
+  HFX_REQUIRE(nchan >= 1 && nchan <= plan->nb * plan->nb, "hfx_fock_epilogue_planar: nchan=%d out of range", nchan);
+  return epilogue_planar_launch(plan, 1, planar_epi_max_blocks(), h, Sigt, chans, nchan, herm ? 1 : 0, cover, Hdiag,
+                                sigc, E_out ? P : nullptr, w2d, 0, F, Sigma_out, E_out ? scratch : nullptr, E_out,
+                                e_offset, nullptr, (hipStream_t)stream);
+}
+
+int hfx_fock_epilogue_planar_batched(const hfx_plan* plan, int nsolve, const void* h, const void* Sigt, int herm,
+                                     const void* P, const double* w2d, void* F, double* st, double* scratch,
+                                     void* stream) {
+  HFX_REQUIRE(plan && h && Sigt && P && w2d && F && st && scratch,
+              "hfx_fock_epilogue_planar_batched: NULL argument");
+  HFX_REQUIRE(nsolve >= 1, "hfx_fock_epilogue_planar_batched: nsolve=%d", nsolve);
+  const int* chans = nullptr;
+  int nchan = 0, cover = 0;
+  resolve_chans(plan, chans, nchan, herm, &cover);
+  return epilogue_planar_launch(plan, nsolve, 256, h, Sigt, chans, nchan, herm ? 1 : 0, cover, nullptr, nullptr, P,
+                                w2d, (size_t)plan->n1 * plan->n2, F, nullptr, scratch, nullptr, 0.0, st,
+                                (hipStream_t)stream);
 }
 
 int hfx_hf_energy(const hfx_plan* plan, const void* P, const void* h, const void* Sigma, const void* H,

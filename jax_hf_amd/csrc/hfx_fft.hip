@@ -185,6 +185,8 @@ struct ExchArgs {
   int nchan, herm, vfull, xcd_remap;
   int nsolve;          // planes = nsolve * nchan; solve s reads / writes at s * plane_stride
   size_t plane_stride; // elements between consecutive solves' P / refP / Sigma (Nk * nb^2)
+  int planar;          // 1: P / refP / Sigma are slot planes (nsolve * nchan, n1, n2), plane s * nchan + t
+                       //    holding channel chans[t]; VR (vfull) is channel-major (nb^2, n1, n2)
   int mode;   // 0: exchange convolution; 1: forward transform of w * V (VR preparation)
   int shift;  // mode 0: ifftshift the result on store; mode 1: apply the ifftshift phase
   int conjv;  // mode 0: multiply by conj(VR) (the adjoint / cross-correlation)
@@ -199,7 +201,9 @@ __host__ __device__ constexpr int lds_pitch(int n2) { return (n2 % 2 == 0) ? n2 
 
 // ---- shared pieces of both kernel flavours --------------------------------
 // Channel of this workgroup; for a batch of solves also moves the P / refP /
-// Sigma pointers to this workgroup's solve (VR is shared by the batch).
+// Sigma pointers to this workgroup's solve (VR is shared by the batch).  In
+// the planar layout the pointers move to this workgroup's planes, after which
+// k-point e of the channel is element e (exch_ix / exch_vix).
 template <typename T, int BLOCK>
 __device__ __forceinline__ int exch_channel(ExchArgs<T>& a) {
   int b = blockIdx.x;
@@ -209,13 +213,30 @@ __device__ __forceinline__ int exch_channel(ExchArgs<T>& a) {
     b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
   }
   const int s = b / a.nchan;
-  if (s > 0) {
+  const int c = a.chans[b - s * a.nchan];
+  if (a.planar) {
+    const size_t nk = (size_t)a.g.n1 * a.g.n2, off = (size_t)b * nk;
+    a.P += off;
+    if (a.refP) a.refP += off;
+    a.Sigma += off;
+    if (a.vfull) a.VR += (size_t)c * nk;
+  } else if (s > 0) {
     const size_t off = (size_t)s * a.plane_stride;
     a.P += off;
     if (a.refP) a.refP += off;
     a.Sigma += off;
   }
-  return a.chans[b - s * a.nchan];
+  return c;
+}
+
+// Index of channel c at k-point e in P / refP / Sigma, and in VR.
+template <typename T>
+__device__ __forceinline__ size_t exch_ix(const ExchArgs<T>& a, size_t e, int c, int nb2) {
+  return a.planar ? e : e * nb2 + c;
+}
+template <typename T>
+__device__ __forceinline__ size_t exch_vix(const ExchArgs<T>& a, size_t e, int c, int nb2) {
+  return (a.planar || !a.vfull) ? e : e * nb2 + c;
 }
 
 template <typename T, int BLOCK>
@@ -225,8 +246,9 @@ __device__ __forceinline__ void exch_load(const ExchArgs<T>& a, cx<T>* plane, in
   const int ci = c / nb, cj = c - ci * nb;
   double mre = 0.0, mim = 0.0;
   for (int e = threadIdx.x; e < N; e += BLOCK) {
-    cx<T> v = a.P[(size_t)e * nb2 + c];
-    if (a.refP) v = v - a.refP[(size_t)e * nb2 + c];
+    const size_t ix = exch_ix(a, (size_t)e, c, nb2);
+    cx<T> v = a.P[ix];
+    if (a.refP) v = v - a.refP[ix];
     const int k1 = e / n2, k2 = e - k1 * n2;
     if (a.mode == 1) {  // VR preparation: w[q] V[q] (weight at the q index, problem.py:88)
       plane[k1 * pitch + k2] = (T)a.w2d[e] * v;
@@ -255,12 +277,10 @@ template <typename T, int BLOCK>
 __device__ __forceinline__ void exch_multiply(const ExchArgs<T>& a, cx<T>* plane, int n1, int n2, int pitch,
                                               int c) {
   const int nb2 = a.g.nb * a.g.nb, N = n1 * n2;
-  const int vc = a.vfull ? c : 0;
-  const int vstride = a.vfull ? nb2 : 1;
   const T scale = (T)(-1.0 / (double)N);
   for (int e = threadIdx.x; e < N; e += BLOCK) {
     const int k1 = e / n2, k2 = e - k1 * n2;
-    const cx<T> vr = a.VR[(size_t)e * vstride + vc];
+    const cx<T> vr = a.VR[exch_vix(a, (size_t)e, c, nb2)];
     cx<T>& x = plane[k1 * pitch + k2];
     x = scale * (a.conjv ? mulc(x, vr) : x * vr);
   }
@@ -272,7 +292,7 @@ __device__ __forceinline__ void exch_store(const ExchArgs<T>& a, const cx<T>* pl
                                            int c) {
   const int nb = a.g.nb, nb2 = nb * nb, N = n1 * n2;
   const int ci = c / nb, cj = c - ci * nb;
-  const bool mirror = a.herm && ci != cj;
+  const bool mirror = a.herm && ci != cj && !a.planar;
   const int cm = cj * nb + ci;
   const int s1 = n1 / 2, s2 = n2 / 2;
   for (int e = threadIdx.x; e < N; e += BLOCK) {
@@ -288,7 +308,7 @@ __device__ __forceinline__ void exch_store(const ExchArgs<T>& a, const cx<T>* pl
       const int o1 = (k1 - s1 + n1) % n1, o2 = (k2 - s2 + n2) % n2;
       eo = (size_t)o1 * n2 + o2;
     }
-    a.Sigma[eo * nb2 + c] = v;
+    a.Sigma[exch_ix(a, eo, c, nb2)] = v;
     if (mirror) a.Sigma[eo * nb2 + cm] = conj(v);
   }
 }
@@ -462,7 +482,7 @@ constexpr int v2_col_a(int N) {
 // DST 1: outputs to HBM (same addressing).  VRM: multiply outputs by VR
 // (column pass, lines = columns k2, positions = k1).
 template <typename T, int R, int N, int NL, int LS, int ES, int BLOCK, int NS, bool INV, int SRC, int DST,
-          bool VRM, int N2>
+          bool VRM, int N2, bool PL>
 __device__ __forceinline__ void v2_stage(cx<T>* plane, const cx<T>* tw, const ExchArgs<T>& a, int c,
                                          double& mre, double& mim) {
   constexpr int NR = N / R, ITEMS = NL * NR, MAXIT = (ITEMS + BLOCK - 1) / BLOCK, TSTEP = N / (NS * R);
@@ -473,15 +493,18 @@ __device__ __forceinline__ void v2_stage(cx<T>* plane, const cx<T>* tw, const Ex
   for (int m = 0; m < MAXIT; ++m) {
     const int it = threadIdx.x + m * BLOCK;
     if (ITEMS % BLOCK == 0 || it < ITEMS) {
-      const int line = it % NL, j = it / NL, k = j % NS;
+      // planar HBM stages: lanes walk along the row (coalesced); otherwise across lines
+      constexpr bool JF = PL && (SRC == 1 || DST == 1);
+      const int line = JF ? it / NR : it % NL, j = JF ? it % NR : it / NL, k = j % NS;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int pos = j + r * NR;
         cx<T> x;
         if constexpr (SRC == 1) {
           const size_t e = (size_t)line * N2 + pos;
-          x = a.P[e * nb2 + c];
-          if (a.refP) x = x - a.refP[e * nb2 + c];
+          const size_t ix = PL ? e : e * nb2 + c;
+          x = a.P[ix];
+          if (a.refP) x = x - a.refP[ix];
           if (a.w2d) {
             const double w = a.w2d[e];
             mre += w * (double)x.re;
@@ -500,8 +523,10 @@ __device__ __forceinline__ void v2_stage(cx<T>* plane, const cx<T>* tw, const Ex
         const int base = (j / NS) * NS * R + k;
         const int vc = a.vfull ? c : 0, vstride = a.vfull ? nb2 : 1;
 #pragma unroll
-        for (int r = 0; r < R; ++r)
-          vr[m][r] = a.VR[((size_t)(base + r * NS) * N2 + line) * vstride + vc];
+        for (int r = 0; r < R; ++r) {
+          const size_t e = (size_t)(base + r * NS) * N2 + line;
+          vr[m][r] = a.VR[PL ? e : e * vstride + vc];
+        }
       }
     }
   }
@@ -510,7 +535,8 @@ __device__ __forceinline__ void v2_stage(cx<T>* plane, const cx<T>* tw, const Ex
   for (int m = 0; m < MAXIT; ++m) {
     const int it = threadIdx.x + m * BLOCK;
     if (ITEMS % BLOCK == 0 || it < ITEMS) {
-      const int line = it % NL, j = it / NL, k = j % NS;
+      constexpr bool JF = PL && (SRC == 1 || DST == 1);
+      const int line = JF ? it / NR : it % NL, j = JF ? it % NR : it / NL, k = j % NS;
       dft_any<T, R, INV>(v[m], tw, N);
       const int base = (j / NS) * NS * R + k;
 #pragma unroll
@@ -523,6 +549,10 @@ __device__ __forceinline__ void v2_stage(cx<T>* plane, const cx<T>* tw, const Ex
         }
         if constexpr (DST == 1) {
           const size_t e = (size_t)line * N2 + pos;
+          if constexpr (PL) {
+            a.Sigma[e] = y;
+            continue;
+          }
           a.Sigma[e * nb2 + c] = y;
           const int ci = c / nb, cj = c - ci * nb;
           if (a.herm && ci != cj) a.Sigma[e * nb2 + cj * nb + ci] = conj(y);
@@ -535,7 +565,7 @@ __device__ __forceinline__ void v2_stage(cx<T>* plane, const cx<T>* tw, const Ex
   if constexpr (DST == 0) __syncthreads();
 }
 
-template <typename T, int N1, int N2, int BLOCK>
+template <typename T, int N1, int N2, int BLOCK, bool PL>
 __global__ __launch_bounds__(BLOCK) void exch_v2_kernel(ExchArgs<T> a) {
   constexpr int PITCH = lds_pitch(N2);
   constexpr int RA2 = v2_row_a(N2), RB2 = N2 / RA2;   // row pass (along k2)
@@ -549,16 +579,16 @@ __global__ __launch_bounds__(BLOCK) void exch_v2_kernel(ExchArgs<T> a) {
   double mre = 0.0, mim = 0.0;
   __syncthreads();
   // forward rows: lines = k1 rows (N1 of them), stride PITCH; positions along k2
-  v2_stage<T, RA2, N2, N1, PITCH, 1, BLOCK, 1, false, 1, 0, false, N2>(plane, tw2, a, c, mre, mim);
-  v2_stage<T, RB2, N2, N1, PITCH, 1, BLOCK, RA2, false, 0, 0, false, N2>(plane, tw2, a, c, mre, mim);
+  v2_stage<T, RA2, N2, N1, PITCH, 1, BLOCK, 1, false, 1, 0, false, N2, PL>(plane, tw2, a, c, mre, mim);
+  v2_stage<T, RB2, N2, N1, PITCH, 1, BLOCK, RA2, false, 0, 0, false, N2, PL>(plane, tw2, a, c, mre, mim);
   // forward columns (+ VR multiply), inverse columns
-  v2_stage<T, RA1, N1, N2, 1, PITCH, BLOCK, 1, false, 0, 0, false, N2>(plane, tw1, a, c, mre, mim);
-  v2_stage<T, RB1, N1, N2, 1, PITCH, BLOCK, RA1, false, 0, 0, true, N2>(plane, tw1, a, c, mre, mim);
-  v2_stage<T, RA1, N1, N2, 1, PITCH, BLOCK, 1, true, 0, 0, false, N2>(plane, tw1, a, c, mre, mim);
-  v2_stage<T, RB1, N1, N2, 1, PITCH, BLOCK, RA1, true, 0, 0, false, N2>(plane, tw1, a, c, mre, mim);
+  v2_stage<T, RA1, N1, N2, 1, PITCH, BLOCK, 1, false, 0, 0, false, N2, PL>(plane, tw1, a, c, mre, mim);
+  v2_stage<T, RB1, N1, N2, 1, PITCH, BLOCK, RA1, false, 0, 0, true, N2, PL>(plane, tw1, a, c, mre, mim);
+  v2_stage<T, RA1, N1, N2, 1, PITCH, BLOCK, 1, true, 0, 0, false, N2, PL>(plane, tw1, a, c, mre, mim);
+  v2_stage<T, RB1, N1, N2, 1, PITCH, BLOCK, RA1, true, 0, 0, false, N2, PL>(plane, tw1, a, c, mre, mim);
   // inverse rows, last stage straight to HBM
-  v2_stage<T, RA2, N2, N1, PITCH, 1, BLOCK, 1, true, 0, 0, false, N2>(plane, tw2, a, c, mre, mim);
-  v2_stage<T, RB2, N2, N1, PITCH, 1, BLOCK, RA2, true, 0, 1, false, N2>(plane, tw2, a, c, mre, mim);
+  v2_stage<T, RA2, N2, N1, PITCH, 1, BLOCK, 1, true, 0, 0, false, N2, PL>(plane, tw2, a, c, mre, mim);
+  v2_stage<T, RB2, N2, N1, PITCH, 1, BLOCK, RA2, true, 0, 1, false, N2, PL>(plane, tw2, a, c, mre, mim);
   if (a.rho) {
     double acc[2] = {mre, mim};
     block_sum<2>(acc, red);
@@ -589,7 +619,10 @@ constexpr int v2_default(int N) { return N == 96 ? HFX_EXCH_V2_B : HFX_EXCH_V2_A
 template <typename T, int N, int V>
 static int launch_v2(const ExchArgs<T>& a, hipStream_t s) {
   constexpr int B = v2_block<N>(V);
-  hipLaunchKernelGGL((exch_v2_kernel<T, N, N, B>), dim3(a.nchan * a.nsolve), dim3(B), 0, s, a);
+  if (a.planar)
+    hipLaunchKernelGGL((exch_v2_kernel<T, N, N, B, true>), dim3(a.nchan * a.nsolve), dim3(B), 0, s, a);
+  else
+    hipLaunchKernelGGL((exch_v2_kernel<T, N, N, B, false>), dim3(a.nchan * a.nsolve), dim3(B), 0, s, a);
   return check_launch("exch_v2_kernel");
 }
 
@@ -632,18 +665,18 @@ static int launch_exch(const ExchArgs<T>& a, int variant, hipStream_t s) {
 
 int exchange_launch(const hfx_plan* plan, const void* P, const void* refP, const void* VR, int vfull,
                     const int* chans, int nchan, int herm, const double* w2d, void* Sigma, void* rho,
-                    int mode, int shift, int conjv, hipStream_t s, int nsolve) {
+                    int mode, int shift, int conjv, hipStream_t s, int nsolve, int planar) {
   if (nchan <= 0 || nsolve <= 0) return HFX_OK;
   const int remap = ((nchan * nsolve) % 8 == 0) ? 1 : 0;
   const size_t stride = (size_t)plan->n1 * plan->n2 * plan->nb * plan->nb;
   if (plan->dtype == HFX_DTYPE_C128) {
     ExchArgs<double> a{(const cx<double>*)P, (const cx<double>*)refP, (const cx<double>*)VR, chans, w2d,
-                       (cx<double>*)Sigma, (cx<double>*)rho, nchan, herm, vfull, remap, nsolve, stride,
+                       (cx<double>*)Sigma, (cx<double>*)rho, nchan, herm, vfull, remap, nsolve, stride, planar,
                        /*mode=*/mode, /*shift=*/shift, /*conjv=*/conjv, plan->g64};
     return launch_exch<double>(a, plan->exch_variant, s);
   }
   ExchArgs<float> a{(const cx<float>*)P, (const cx<float>*)refP, (const cx<float>*)VR, chans, w2d,
-                    (cx<float>*)Sigma, (cx<double>*)rho, nchan, herm, vfull, remap, nsolve, stride,
+                    (cx<float>*)Sigma, (cx<double>*)rho, nchan, herm, vfull, remap, nsolve, stride, planar,
                     /*mode=*/mode, /*shift=*/shift, /*conjv=*/conjv, plan->g32};
   return launch_exch<float>(a, plan->exch_variant, s);
 }

@@ -38,10 +38,17 @@ struct hfx_plan {
 namespace hfx {
 int exchange_launch(const hfx_plan* plan, const void* P, const void* refP, const void* VR, int vfull,
                     const int* chans, int nchan, int herm, const double* w2d, void* Sigma, void* rho,
-                    int mode, int shift, int conjv, hipStream_t s, int nsolve = 1);
+                    int mode, int shift, int conjv, hipStream_t s, int nsolve = 1, int planar = 0);
 int exchange_max_plane(int dtype);
 int epilogue_seg_launch(const hfx_plan* p, int nsolve, const void* h, const void* Sig, const void* P, const double* w2d,
                         void* F, double* st, double* part, hipStream_t s);
 int eigh16_launch(int dtype, int nb, size_t nk, const void* A, double* lam, void* V, int herm_in, hipStream_t s);
 int exchange_lds_elems(int n1, int n2);
+int pack_planar_launch(const hfx_plan* p, int nsolve, const void* X, const void* refX, const int* chans, int nchan,
+                       void* Xt, hipStream_t s);
+int epilogue_planar_launch(const hfx_plan* p, int nsolve, int max_blocks, const void* h, const void* St,
+                           const int* chans, int nchan, int herm, int cover, const double* Hdiag, const void* sigc,
+                           const void* P, const double* w2d, size_t w_stride, void* F, void* Sig_out,
+                           double* part, double* E_out, double e_offset, double* st, hipStream_t s);
+int planar_epi_max_blocks();
 }  // namespace hfx

@@ -5,17 +5,23 @@ Mirrors ``src/jax_hf/fock.py:18-129`` (``build_fock``, ``hf_energy``,
 Arrays are torch device tensors (numpy inputs are uploaded); every arithmetic
 step runs in libhfx HIP kernels:
 
-* exchange: fused LDS-resident 2D FFT convolution per (i, j) channel
-  (``hfx_selfenergy``), optionally on the Hermitian upper-triangle channel set
-  or on block-diagonal channel subsets;
+* exchange: fused LDS-resident 2D FFT convolution per (i, j) channel,
+  optionally on the Hermitian upper-triangle channel set or on block-diagonal
+  channel subsets.  The Fock build stages the channels as slot planes
+  (``hfx_planar_pack``), runs the exchange on them (``hfx_selfenergy_planar``)
+  and reads Sigma back in the epilogue (``hfx_fock_epilogue_planar``), so
+  every exchange access is coalesced; ``PLANAR = False`` selects the
+  (k, i, j)-layout kernels (``hfx_selfenergy`` + ``hfx_fock_epilogue``);
 * Hartree / contact: weighted channel moments (a by-product of the exchange
   kernel) -> ``hfx_kterms``;
-* ``F = herm(h + Sigma + H)`` and the energy trace: ``hfx_fock_epilogue``.
+* ``F = herm(h + Sigma + H)`` and the energy trace in the epilogue.
 """
 
 from __future__ import annotations
 
 from typing import Any
+
+import os
 
 import numpy as np
 import torch
@@ -26,6 +32,10 @@ from ._util import KIND, convert, host_array, real_dtype_of, to_tensor
 __all__ = ["build_fock", "hf_energy", "occupation_entropy", "free_energy", "FockEngine",
            "normalize_block_specs"]
 
+
+# Fock builds stage the exchange channels as slot planes (set HFX_PLANAR=0 or
+# assign False to use the (k, i, j)-layout exchange kernels).
+PLANAR = os.environ.get("HFX_PLANAR", "1") != "0"
 
 # ---------------------------------------------------------------------------
 # block specs (linalg.py:19-90)
@@ -129,6 +139,8 @@ class FockEngine:
         self.sigc = torch.zeros((nb, nb), dtype=torch.complex128, device=self.dev) if self.contact_on else None
         self.E = torch.zeros((), **f64)
         self._sig_ws = None
+        self._vrc = None
+        self._pl_ws = None
 
     # -- pieces -------------------------------------------------------------
     def selfenergy(self, P, out, refP=None, chans=None, nchan=0, herm=None, rho=True, ifftshift=False):
@@ -146,6 +158,25 @@ class FockEngine:
                L.ptr(self.cOj) if self.contact_on else None,
                L.ptr(self.Hdiag), L.ptr(self.sigc), L.stream(self.dev))
 
+    def vr_planar(self):
+        """Channel-major VR (nb^2, nk1, nk2) for a full VR; the VR itself when scalar."""
+        if self._vrc is None:
+            if not self.vfull:
+                self._vrc = self.VR.contiguous()
+            else:
+                nk1, nk2, nb = self.shape
+                self._vrc = torch.empty((nb * nb, nk1, nk2), dtype=self.VR.dtype, device=self.dev)
+                L.call("hfx_planar_pack", self.plan.handle, 1, L.ptr(self.VR.contiguous()), None, None, 0, 0,
+                       L.ptr(self._vrc), L.stream(self.dev))
+        return self._vrc
+
+    def planar_workspace(self, nchan):
+        nk1, nk2, _ = self.shape
+        if self._pl_ws is None or self._pl_ws[0].shape[0] < nchan:
+            mk = lambda: torch.empty((nchan, nk1, nk2), dtype=self.cdt, device=self.dev)  # noqa: E731
+            self._pl_ws = (mk(), mk())
+        return self._pl_ws
+
     def sigma_workspace(self, P):
         if self._sig_ws is None or self._sig_ws.shape != P.shape:
             self._sig_ws = torch.empty_like(P)
@@ -160,6 +191,9 @@ class FockEngine:
         """
         refP = self.refP if isinstance(refP, str) else refP
         st = L.stream(self.dev)
+        if self.include_exchange and PLANAR and self.nb <= 64:
+            return self._build_planar(P, F, Sigma_out, energy, refP, chans, nchan, moments_separately, E_ptr,
+                                      sub_h, st)
         Sig = None
         if self.include_exchange:
             Sig = Sigma_out if (Sigma_out is not None and not self.contact_on) else self.sigma_workspace(P)
@@ -186,6 +220,28 @@ class FockEngine:
                 L.call("hfx_fock_epilogue", self.plan.handle, L.ptr(_zeros_like_cached(P)), None, None,
                        L.ptr(self.sigc), None, L.ptr(self.w2d), L.ptr(self.sigma_workspace(P)),
                        L.ptr(Sigma_out), L.ptr(self.plan.scratch), None, 0.0, st)
+        return self.E if energy else None
+
+    def _build_planar(self, P, F, Sigma_out, energy, refP, chans, nchan, moments_separately, E_ptr, sub_h, st):
+        h = self.plan.handle
+        nch = nchan if chans is not None else (self.nb * (self.nb + 1) // 2 if self.herm else self.nb * self.nb)
+        Pt, St = self.planar_workspace(nch)
+        herm = int(self.herm)
+        L.call("hfx_planar_pack", h, 1, L.ptr(P), L.ptr(refP), L.ptr(chans), int(nchan), herm, L.ptr(Pt), st)
+        rho_ex = self.need_rho and not moments_separately
+        L.call("hfx_selfenergy_planar", h, 1, L.ptr(Pt), L.ptr(self.vr_planar()), self.vfull, L.ptr(chans),
+               int(nchan), herm, 0, L.ptr(self.w2d) if rho_ex else None, L.ptr(St),
+               L.ptr(self.rho) if rho_ex else None, st)
+        if self.need_rho and moments_separately:
+            L.call("hfx_density_moments", h, L.ptr(P), L.ptr(refP), L.ptr(self.w2d), L.ptr(self.rho), st)
+        if self.need_rho:
+            self.kterms()
+        E_dst = (E_ptr if E_ptr is not None else L.ptr(self.E)) if energy else None
+        L.call("hfx_fock_epilogue_planar", h, L.ptr(self.h), L.ptr(St), L.ptr(chans), int(nchan), herm,
+               L.ptr(self.Hdiag), L.ptr(self.sigc), L.ptr(P) if energy else None, L.ptr(self.w2d), L.ptr(F),
+               L.ptr(Sigma_out), L.ptr(self.plan.scratch), E_dst, self.e_offset, st)
+        if sub_h:
+            _sub_inplace(F, self.h)
         return self.E if energy else None
 
     def hartree_matrix_out(self):

@@ -46,11 +46,14 @@ def test_library_loads_on_gpu(dev):
     _lib.require_gpu()
 
 
+@pytest.mark.parametrize("planar", [True, False])
 @pytest.mark.parametrize("nk,nb,layer", [(1, 2, False), (2, 2, False), (5, 3, False), (12, 4, False),
                                          (24, 4, False), (24, 4, True), (36, 8, True), (48, 16, True),
                                          (9, 2, True), (7, 3, False)])
-def test_vr_and_fock_fp64_match_oracle(dev, nk, nb, layer):
+def test_vr_and_fock_fp64_match_oracle(dev, nk, nb, layer, planar, monkeypatch):
     import jax_hf_amd as J
+    from jax_hf_amd import fock as fock_mod
+    monkeypatch.setattr(fock_mod, "PLANAR", planar)
     rng = np.random.default_rng(nk * 100 + nb)
     h = syn.tight_binding(nk, nk, nb, seed=nk)
     V = syn.coulomb_layer(nk, nk, nb, 2) if layer else syn.coulomb_scalar(nk, nk)
@@ -70,7 +73,10 @@ def test_vr_and_fock_fp64_match_oracle(dev, nk, nb, layer):
     assert abs(E1 - E0) <= 1e-11 * max(1.0, abs(E0))
 
 
-def test_fixture_fock_hartree_refP(dev, meshgrid_case):
+@pytest.mark.parametrize("planar", [True, False])
+def test_fixture_fock_hartree_refP(dev, meshgrid_case, planar, monkeypatch):
+    from jax_hf_amd import fock as fock_mod
+    monkeypatch.setattr(fock_mod, "PLANAR", planar)
     """Hartree + refP + scalar exchange on the reference's converged state."""
     import jax_hf_amd as J
     d = meshgrid_case
@@ -89,7 +95,10 @@ def test_fixture_fock_hartree_refP(dev, meshgrid_case):
     assert abs(E1 - float(d["E_expected"])) < 1e-6
 
 
-def test_contact_terms_and_no_exchange(dev):
+@pytest.mark.parametrize("planar", [True, False])
+def test_contact_terms_and_no_exchange(dev, planar, monkeypatch):
+    from jax_hf_amd import fock as fock_mod
+    monkeypatch.setattr(fock_mod, "PLANAR", planar)
     import jax_hf_amd as J
     rng = np.random.default_rng(5)
     nk, nb = 4, 2
@@ -111,7 +120,10 @@ def test_contact_terms_and_no_exchange(dev):
         assert _rel(_np(S1), S0) < 1e-12
 
 
-def test_block_specs_gpu(dev):
+@pytest.mark.parametrize("planar", [True, False])
+def test_block_specs_gpu(dev, planar, monkeypatch):
+    from jax_hf_amd import fock as fock_mod
+    monkeypatch.setattr(fock_mod, "PLANAR", planar)
     import jax_hf_amd as J
     rng = np.random.default_rng(0)
     nk, n0 = 4, 2
@@ -230,3 +242,89 @@ def test_exchange_variants_match_oracle(dev, nk, cdt):
         assert all(t > 0 for t in times.values()), times
     finally:
         plan.exchange_variant = 0
+
+
+@pytest.mark.parametrize("cdt", ["c128", "c64"])
+@pytest.mark.parametrize("nk", [7, 12, 20, 36, 48, 96])
+def test_planar_exchange_matches_oracle(dev, nk, cdt):
+    """Slot-plane exchange: pack is an exact gather; Sigma planes and moments
+    match the oracle for every kernel variant, full / Hermitian / explicit
+    channel lists, scalar and full VR, with refP and ifftshift; the planar
+    epilogue equals the (k, i, j)-layout epilogue."""
+    import torch
+    import jax_hf_amd as J
+    from jax_hf_amd import _lib as L
+    nb = 3
+    rng = np.random.default_rng(2000 + nk)
+    h = syn.tight_binding(nk, nk, nb, seed=nk)
+    w = rng.uniform(0.5, 1.5, size=(nk, nk)) / nk ** 2
+    P = syn.haar_density(nk, nk, nb, rng)
+    R = syn.haar_density(nk, nk, nb, rng)
+    npd = np.complex128 if cdt == "c128" else np.complex64
+    tol = 1e-12 if cdt == "c128" else 2e-6
+    st = None
+    for layer in (True, False):
+        V = syn.coulomb_layer(nk, nk, nb, 2) if layer else syn.coulomb_scalar(nk, nk)
+        K = O.OracleKernel(w, h, V, 0.1)
+        G = J.HartreeFockKernel(w, h.astype(npd), V, 0.1)
+        eng = G.fock_engine(herm=False)
+        plan = eng.plan
+        st = L.stream(G.device)
+        dev_ = G.device
+        Pt = torch.from_numpy(P.astype(npd)).to(dev_)
+        Rt = torch.from_numpy(R.astype(npd)).to(dev_)
+        VRc = eng.vr_planar()
+        if eng.vfull:
+            assert torch.equal(VRc, eng.VR.permute(2, 3, 0, 1).reshape(nb * nb, nk, nk))
+        rho_ref = np.einsum("ab,abij->ij", w, P - R)
+        for shift in (0, 1):
+            S_ref = O.selfenergy_fft(K._VR_shifted, P - R, apply_ifftshift=bool(shift))
+            for herm, chans in ((0, None), (1, None), (0, [4, 0, 7]), (1, [1, 5])):
+                if chans is None:
+                    cl = list(range(nb * nb)) if not herm else [i * nb + j for i in range(nb) for j in range(i, nb)]
+                else:
+                    cl = chans
+                ct = None if chans is None else torch.tensor(chans, dtype=torch.int32, device=dev_)
+                nch = len(cl)
+                X = torch.full((nch, nk, nk), float("nan"), dtype=Pt.dtype, device=dev_)
+                L.call("hfx_planar_pack", plan.handle, 1, L.ptr(Pt), L.ptr(Rt), L.ptr(ct), nch, herm, L.ptr(X), st)
+                D = (Pt - Rt).reshape(nk, nk, nb * nb)
+                assert torch.equal(X, D[..., cl].permute(2, 0, 1)), (herm, chans)
+                for v in (1, 2, 3, 4):
+                    plan.exchange_variant = v
+                    S = torch.full_like(X, float("nan"))
+                    eng.rho.zero_()
+                    L.call("hfx_selfenergy_planar", plan.handle, 1, L.ptr(X), L.ptr(VRc), eng.vfull, L.ptr(ct), nch,
+                           herm, shift, L.ptr(eng.w2d), L.ptr(S), L.ptr(eng.rho), st)
+                    Sn = _np(S)
+                    for t, c in enumerate(cl):
+                        i, j = divmod(c, nb)
+                        assert _rel(Sn[t], S_ref[..., i, j]) < tol, (v, herm, chans, shift, c)
+                        r = _np(eng.rho)
+                        assert abs(r[i, j] - rho_ref[i, j]) < 1e-12 * (1 if cdt == "c128" else 1e6) * max(
+                            1.0, np.abs(rho_ref).max()), (v, herm, c)
+                plan.exchange_variant = 0
+                if shift:
+                    continue
+                # epilogue on the slot planes == epilogue on the (k, i, j) Sigma
+                Sa = torch.zeros_like(Pt)
+                Sn = torch.from_numpy(_np(S)).to(dev_)
+                for t, c in enumerate(cl):
+                    i, j = divmod(c, nb)
+                    Sa[..., i, j] = Sn[t]
+                    if herm and i != j:
+                        Sa[..., j, i] = Sn[t].conj()
+                hd = torch.tensor(rng.normal(size=nb), dtype=torch.float64, device=dev_)
+                sigc = torch.from_numpy(rng.normal(size=(nb, nb)) + 1j * rng.normal(size=(nb, nb))).to(dev_)
+                F0, F1 = torch.empty_like(Pt), torch.empty_like(Pt)
+                O0, O1 = torch.empty_like(Pt), torch.empty_like(Pt)
+                E = torch.zeros(2, dtype=torch.float64, device=dev_)
+                L.call("hfx_fock_epilogue", plan.handle, L.ptr(eng.h), L.ptr(Sa), L.ptr(hd), L.ptr(sigc), L.ptr(Pt),
+                       L.ptr(eng.w2d), L.ptr(F0), L.ptr(O0), L.ptr(plan.scratch), L.ptr(E), 0.25, st)
+                L.call("hfx_fock_epilogue_planar", plan.handle, L.ptr(eng.h), L.ptr(S), L.ptr(ct), nch, herm,
+                       L.ptr(hd), L.ptr(sigc), L.ptr(Pt), L.ptr(eng.w2d), L.ptr(F1), L.ptr(O1),
+                       L.ptr(plan.scratch), L.ptr(E) + 8, 0.25, st)
+                assert torch.equal(F0, F1) and torch.equal(O0, O1), (herm, chans)
+                e = _np(E)
+                etol = 1e-13 if cdt == "c128" else 1e-5
+                assert abs(e[0] - e[1]) <= etol * max(1.0, abs(e[0])), (e, herm, chans)
