@@ -160,6 +160,13 @@ int hfx_convert(const void* src, int src_kind, void* dst, int dst_kind, size_t n
 int hfx_lincomb(size_t n, int kind, double ar, double ai, const void* X, double br, double bi, const void* Y,
                 void* out, void* stream);
 
+/* out = sum_v coef[v] X[v] over nmat nb x nb matrices (1 <= nin <= 8, real
+ * coefficients), Hermitised per matrix when herm: the DIIS extrapolation
+ * F = herm(sum_i c_i F_i) of reference_scf.py:298-314 in one pass. out may
+ * alias an input only without herm. */
+int hfx_lincomb_multi(size_t nmat, int nb, int kind, int nin, const void* const* X, const double* coef, int herm,
+                      void* out, void* stream);
+
 /* out = 1 / (1 + exp((x - mu) / (T + 1e-12))), mu = *mu or 0 (fermidirac, utils.py:51-53). */
 int hfx_fermi(const double* x, const double* mu, double T, size_t n, double* out, void* stream);
 

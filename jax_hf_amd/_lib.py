@@ -52,6 +52,7 @@ _SIGS = {
     "hfx_hermitize": [vp, vp, sz, i32, i32, vp],
     "hfx_lincomb": [sz, i32, f64, f64, vp, f64, f64, vp, vp, vp],
     "hfx_fermi": [vp, vp, f64, sz, vp, vp],
+    "hfx_lincomb_multi": [sz, i32, i32, i32, vp, vp, i32, vp, vp],
     "hfx_absmax_masked": [vp, sz, i32, i32, vp, vp, vp, vp],
     "hfx_entropy": [vp, i32, vp, sz, i32, vp, f64, vp, vp, vp],
     "hfx_eigh_batched": [i32, i32, sz, vp, vp, vp, i32, vp],

@@ -131,6 +131,17 @@ def vdots(xs, ys, part):
            s(xs[0].device))
 
 
+def lincomb_multi(out, Xs, coefs, herm=False):
+    """out = sum_v coefs[v] Xs[v] (real coefficients, <= 8 inputs), Hermitised per matrix with herm."""
+    import ctypes as C
+    from ._util import KIND
+    nb = out.shape[-1]
+    xa = (C.c_void_p * len(Xs))(*[x.data_ptr() for x in Xs])
+    ca = (C.c_double * len(Xs))(*[float(c) for c in coefs])
+    L.call("hfx_lincomb_multi", out.numel() // (nb * nb), nb, KIND[out.dtype], len(Xs), C.cast(xa, C.c_void_p),
+           C.cast(ca, C.c_void_p), int(herm), L.ptr(out), s(out.device))
+
+
 def lincomb(out, X, a=1.0, Y=None, b=0.0):
     """out = a X + b Y (a, b real or complex)."""
     from ._util import KIND

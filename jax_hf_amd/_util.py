@@ -14,6 +14,39 @@ _REAL = {torch.complex128: torch.float64, torch.complex64: torch.float32}
 KIND = {torch.float64: 0, torch.float32: 1, torch.complex128: 2, torch.complex64: 3}
 
 
+def capture_graphs(dev, fn, n=1):
+    """Record n successive calls of fn() (launches only) as HIP graphs.
+
+    Captured on a side stream of default priority.  The cyclic garbage
+    collector runs first and is held off until the captures end: a collection
+    inside a capture could destroy another graph or free the device memory
+    of an unreachable solver state while the stream is capturing.
+    """
+    import gc
+    gc.collect()
+    was = gc.isenabled()
+    gc.disable()
+    graphs = []
+    try:
+        cur = torch.cuda.current_stream(dev)
+        cap = torch.cuda.Stream(dev)
+        cap.wait_stream(cur)
+        with torch.cuda.stream(cap):
+            for _ in range(n):
+                g = torch.cuda.CUDAGraph()
+                g.capture_begin()
+                try:
+                    fn()
+                finally:
+                    g.capture_end()
+                graphs.append(g)
+        cur.wait_stream(cap)
+    finally:
+        if was:
+            gc.enable()
+    return graphs
+
+
 def default_device():
     if torch.cuda.is_available():
         return torch.device("cuda", torch.cuda.current_device())

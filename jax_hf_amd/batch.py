@@ -35,7 +35,7 @@ import torch
 from . import _lib as L
 from . import _ops as K
 from . import fock as fock_mod
-from ._util import KIND, convert, to_tensor
+from ._util import KIND, capture_graphs, convert, to_tensor
 from .solver import SolveResult, SolverConfig, _polish_diis, _precision
 
 __all__ = ["solve_batch"]
@@ -185,19 +185,8 @@ class _BatchCG:
         self._swap()
 
     def _capture(self, shrink_pow):
-        graphs = []
-        cur = torch.cuda.current_stream(self.dev)
-        cap = torch.cuda.Stream(self.dev)
-        cap.wait_stream(cur)
-        with torch.cuda.stream(cap):
-            for _ in range(2):
-                g = torch.cuda.CUDAGraph()
-                g.capture_begin()
-                self._body(shrink_pow)
-                g.capture_end()
-                graphs.append(g)
-        cur.wait_stream(cap)
-        return graphs
+        """Two graphs of the body, one per parity of the buffer swap it makes."""
+        return capture_graphs(self.dev, lambda: self._body(shrink_pow), n=2)
 
     def run(self, tol_E, tol_grad, max_iter, rel_tol=False):
         """solver.py:422-449 per solve; a solve meeting its rule is frozen."""
@@ -332,7 +321,7 @@ def solve_batch(kernels, P0s, n_electrons, *, config: SolverConfig | None = None
             P_lo = b.density()
             for j, i in enumerate(members):
                 results[i] = _polish_diis(kernels[i], None, n_electrons[i], cfg, None, k_of[j], hE[j], hG[j],
-                                          P_lo=P_lo[j])
+                                          P_lo=P_lo[j], mu0=b.st[j, K.ST_MU])
             continue
         k_of, grad, hE, hG = b.run(cfg.tol_E, cfg.tol_grad, cfg.max_iter)
         Pfin, Ffin = b.finalize()

@@ -25,7 +25,14 @@ constexpr int PL_BLOCK = 256;
 constexpr int PL_MAX_LDS = 40 * 1024;   // four epilogue workgroups per CU
 constexpr int PL_EPI_MAX_BLOCKS = 4096;
 constexpr int PL_TILE_LIMIT = 150 * 1024;   // largest epilogue tile (nb <= 96 in complex128)
-constexpr int PL_U = 4;   // loads each thread issues before the first use (memory-level parallelism)
+#ifndef HFX_PL_U
+#define HFX_PL_U 4
+#endif
+#ifndef HFX_PL_EPI_BLOCK
+#define HFX_PL_EPI_BLOCK 512
+#endif
+constexpr int PL_U = HFX_PL_U;   // loads each thread issues before the first use (memory-level parallelism)
+constexpr int EPI_BLK = HFX_PL_EPI_BLOCK;
 
 // Bytes of one slot's k-segment in the epilogue tile (one cache line).
 #ifndef HFX_PL_SEG
@@ -99,7 +106,7 @@ __global__ __launch_bounds__(PL_BLOCK) void pack_planar_kernel(int Nk, int nb2, 
 // kernel would need a device-scope release per block, which writes back the
 // block's XCD L2 full of dirty F lines: 4x slower than the second launch.)
 template <typename T>
-__global__ __launch_bounds__(PL_BLOCK) void epilogue_planar_kernel(
+__global__ __launch_bounds__(EPI_BLK) void epilogue_planar_kernel(
     int Nk, int nb, int ktl, const cx<T>* __restrict__ h, const cx<T>* __restrict__ St,
     const int* __restrict__ chans, int nchan, int herm, int cover, const double* __restrict__ Hdiag,
     const cx<double>* __restrict__ sigc, const cx<T>* __restrict__ P, const double* __restrict__ w2d,
@@ -107,7 +114,7 @@ __global__ __launch_bounds__(PL_BLOCK) void epilogue_planar_kernel(
     const double* __restrict__ st) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* tile = reinterpret_cast<cx<T>*>(smem_raw);
-  __shared__ double red[PL_BLOCK / 64];
+  __shared__ double red[EPI_BLK / 64];
   const int nb2 = nb * nb, KT = 1 << ktl, pitch = nb2 + 1;
   const int s = blockIdx.y;
   double eacc = 0.0;
@@ -120,21 +127,21 @@ __global__ __launch_bounds__(PL_BLOCK) void epilogue_planar_kernel(
       const int e0 = b << ktl;
       const int kt = min(KT, Nk - e0);
       if (!cover) {
-        for (int i = threadIdx.x; i < kt * pitch; i += PL_BLOCK) tile[i] = cx<T>{0, 0};
+        for (int i = threadIdx.x; i < kt * pitch; i += EPI_BLK) tile[i] = cx<T>{0, 0};
         __syncthreads();
       }
       const int ns = nchan * KT;
-      for (int i0 = 0; i0 < ns; i0 += PL_U * PL_BLOCK) {
+      for (int i0 = 0; i0 < ns; i0 += PL_U * EPI_BLK) {
         cx<T> v[PL_U];
 #pragma unroll
         for (int u = 0; u < PL_U; ++u) {
-          const int i = i0 + u * PL_BLOCK + threadIdx.x;
+          const int i = i0 + u * EPI_BLK + threadIdx.x;
           const int t = i >> ktl, kk = i & (KT - 1);
           if (i < ns && kk < kt) v[u] = Ss[(size_t)t * Nk + e0 + kk];
         }
 #pragma unroll
         for (int u = 0; u < PL_U; ++u) {
-          const int i = i0 + u * PL_BLOCK + threadIdx.x;
+          const int i = i0 + u * EPI_BLK + threadIdx.x;
           const int t = i >> ktl, kk = i & (KT - 1);
           if (i < ns && kk < kt) {
             const int c = chans[t], ci = c / nb, cj = c - ci * nb;
@@ -146,11 +153,11 @@ __global__ __launch_bounds__(PL_BLOCK) void epilogue_planar_kernel(
       __syncthreads();
       const int n = kt * nb2;
       const size_t eb = base + (size_t)e0 * nb2;
-      for (int i0 = 0; i0 < n; i0 += PL_U * PL_BLOCK) {
+      for (int i0 = 0; i0 < n; i0 += PL_U * EPI_BLK) {
         cx<T> hij[PL_U], hji[PL_U], pji[PL_U];
 #pragma unroll
         for (int u = 0; u < PL_U; ++u) {
-          const int i = i0 + u * PL_BLOCK + threadIdx.x;
+          const int i = i0 + u * EPI_BLK + threadIdx.x;
           if (i < n) {
             const int kk = i / nb2, c = i - kk * nb2;
             const int ci = c / nb, cj = c - ci * nb;
@@ -162,7 +169,7 @@ __global__ __launch_bounds__(PL_BLOCK) void epilogue_planar_kernel(
         }
 #pragma unroll
         for (int u = 0; u < PL_U; ++u) {
-          const int i = i0 + u * PL_BLOCK + threadIdx.x;
+          const int i = i0 + u * EPI_BLK + threadIdx.x;
           if (i < n) {
             const int kk = i / nb2, c = i - kk * nb2;
             const int ci = c / nb, cj = c - ci * nb, ct = cj * nb + ci;
@@ -253,14 +260,14 @@ int epilogue_planar_launch(const hfx_plan* p, int nsolve, int max_blocks, const 
     auto kern = epilogue_planar_kernel<double>;
     static int limit = 64 * 1024;
     if (int rc = set_lds((const void*)kern, limit, lds)) return rc;
-    hipLaunchKernelGGL(kern, grid, dim3(PL_BLOCK), lds, s, Nk, p->nb, ktl, (const cx<double>*)h,
+    hipLaunchKernelGGL(kern, grid, dim3(EPI_BLK), lds, s, Nk, p->nb, ktl, (const cx<double>*)h,
                        (const cx<double>*)St, chans, nchan, herm, cover, Hdiag, (const cx<double>*)sigc,
                        (const cx<double>*)P, w2d, w_stride, (cx<double>*)F, (cx<double>*)Sig_out, pe, st);
   } else {
     auto kern = epilogue_planar_kernel<float>;
     static int limit = 64 * 1024;
     if (int rc = set_lds((const void*)kern, limit, lds)) return rc;
-    hipLaunchKernelGGL(kern, grid, dim3(PL_BLOCK), lds, s, Nk, p->nb, ktl, (const cx<float>*)h,
+    hipLaunchKernelGGL(kern, grid, dim3(EPI_BLK), lds, s, Nk, p->nb, ktl, (const cx<float>*)h,
                        (const cx<float>*)St, chans, nchan, herm, cover, Hdiag, (const cx<double>*)sigc,
                        (const cx<float>*)P, w2d, w_stride, (cx<float>*)F, (cx<float>*)Sig_out, pe, st);
   }

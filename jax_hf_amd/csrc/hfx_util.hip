@@ -122,6 +122,42 @@ __global__ void lincomb_kernel(size_t n, int kind, double ar, double ai, const v
   }
 }
 
+// out = sum_v c_v X_v over nmat nb x nb matrices, optionally Hermitised per
+// matrix: out_ij = (Y_ij + conj(Y_ji)) / 2 with Y = sum_v c_v X_v.
+constexpr int LCM_MAX = 8;
+struct LcmInputs {
+  const void* x[LCM_MAX];
+  double c[LCM_MAX];
+};
+
+__global__ void lincomb_multi_kernel(size_t nmat, int nb, int kind, int nin, LcmInputs in, int herm, void* out) {
+  const size_t nb2 = (size_t)nb * nb, total = nmat * nb2;
+  for (size_t i = (size_t)blockIdx.x * UTIL_BLOCK + threadIdx.x; i < total; i += (size_t)gridDim.x * UTIL_BLOCK) {
+    double r = 0.0, m = 0.0, rt = 0.0, mt = 0.0;
+    size_t it = i;
+    if (herm) {
+      const size_t k = i / nb2, c = i - k * nb2, a = c / nb, b = c - a * nb;
+      it = k * nb2 + b * nb + a;
+    }
+    for (int v = 0; v < nin; ++v) {
+      double xr, xi;
+      load_any(in.x[v], kind, i, xr, xi);
+      r += in.c[v] * xr;
+      m += in.c[v] * xi;
+      if (herm) {
+        load_any(in.x[v], kind, it, xr, xi);
+        rt += in.c[v] * xr;
+        mt += in.c[v] * xi;
+      }
+    }
+    if (herm) {
+      r = 0.5 * (r + rt);
+      m = 0.5 * (m - mt);
+    }
+    store_any(out, kind, i, r, m);
+  }
+}
+
 // out = expit(-(x - mu) / (T + 1e-12))  (utils.py:51-53)
 __global__ void fermi_kernel(const double* x, const double* mu, double T, size_t n, double* out) {
   const double m = mu ? *mu : 0.0;
@@ -205,6 +241,24 @@ int hfx_lincomb(size_t n, int kind, double ar, double ai, const void* X, double 
   hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(n)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream, n, kind, ar, ai,
                      X, br, bi, Y, out);
   return check_launch("lincomb_kernel");
+}
+
+int hfx_lincomb_multi(size_t nmat, int nb, int kind, int nin, const void* const* X, const double* coef, int herm,
+                      void* out, void* stream) {
+  if (!X || !coef || !out || nin < 1 || nin > LCM_MAX || nb < 1 || kind < 0 || kind > 3)
+    return set_error(HFX_EINVAL, "hfx_lincomb_multi: bad argument (nin=%d)", nin);
+  LcmInputs in{};
+  for (int v = 0; v < nin; ++v) {
+    if (!X[v]) return set_error(HFX_EINVAL, "hfx_lincomb_multi: input %d is NULL", v);
+    if (herm && X[v] == out) return set_error(HFX_EINVAL, "hfx_lincomb_multi: out aliases an input with herm");
+    in.x[v] = X[v];
+    in.c[v] = coef[v];
+  }
+  const size_t n = nmat * (size_t)nb * nb;
+  if (n == 0) return HFX_OK;
+  hipLaunchKernelGGL(lincomb_multi_kernel, dim3(grid_for(n)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream, nmat, nb, kind,
+                     nin, in, herm, out);
+  return check_launch("lincomb_multi_kernel");
 }
 
 int hfx_fermi(const double* x, const double* mu, double T, size_t n, double* out, void* stream) {

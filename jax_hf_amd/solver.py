@@ -21,6 +21,7 @@ FP64.  ``"auto"`` follows the kernel dtype.
 from __future__ import annotations
 
 import warnings
+import weakref
 from dataclasses import dataclass
 from typing import Any, NamedTuple
 
@@ -29,7 +30,8 @@ import torch
 
 from . import _lib as L
 from . import _ops as K
-from ._util import convert, host_array, to_tensor
+from ._util import capture_graphs, convert, host_array, to_tensor
+from .reference_scf import diis_solve
 
 __all__ = ["SolverConfig", "SolveResult", "solve", "solve_direct_minimization"]
 
@@ -171,6 +173,7 @@ class _CG:
         if form not in ("gauss_jordan", "spectral"):
             raise ValueError(f"cayley must be 'gauss_jordan' or 'spectral'; got {cfg.cayley!r}")
         self.gauss_jordan = form == "gauss_jordan"
+        self.graph = None        # captured iteration body, kept while the runner is cached
 
     # -- pieces ---------------------------------------------------------------
     def density(self):
@@ -215,7 +218,6 @@ class _CG:
         tol_E_rel = tol_E
         st[K.ST_ITER] = 0.0                 # device iteration counter (CG restart schedule)
         use_graph = bool(cfg.use_graph) and self.fock.graph_safe(cfg.project_fn)
-        graphs, g_base = None, 0
         while True:
             # stopping rule (solver.py:422-449); rel_tol scales tol_E by |E| (FP32 phase)
             if rel_tol and hist_E:
@@ -233,12 +235,10 @@ class _CG:
             nc = grad_norm > tol_grad if tol_grad > 0.0 else enc
             if not (k < max_iter and nc):
                 break
-            if graphs is None and use_graph and k >= 1:
-                graphs = self._capture(shrink_pow)
-                g_base = k
-            if graphs is not None:
-                graphs[(k - g_base) % 2].replay()
-                self._swap()
+            if self.graph is None and use_graph and k >= 1:   # k = 0 ran eagerly: buffers exist
+                self.graph = self._capture(shrink_pow)
+            if use_graph and self.graph is not None:
+                self.graph.replay()
             else:
                 self._body(shrink_pow)
             sv = st.cpu().numpy()                                               # one readback
@@ -249,12 +249,7 @@ class _CG:
             dE = abs(E - E_prev)
             E_prev = E
             k += 1
-        self.G, self.Gp = self.Gp, self.G   # last computed gradient back in self.G
         return k, grad_norm, hist_E, hist_G
-
-    def _swap(self):
-        self.G, self.Gp = self.Gp, self.G
-        self.gp, self.gpp = self.gpp, self.gp
 
     def _body(self, shrink_pow):
         """One CG iteration (solver.py:451-604) as device launches only: no host
@@ -303,24 +298,14 @@ class _CG:
             L.call("hfx_cg_retract", dt, nb, nk, L.ptr(self.Q), L.ptr(self.V), L.ptr(self.lam), L.ptr(self.Fe),
                    L.ptr(st), 0.0, 0, L.ptr(self.eps), None, sm)                # :586-590
         self.mu(self.eps, True)                                                 # :591-592
-        self._swap()
+        # this iteration's gradient / preconditioned norms become the previous ones
+        # (copies, not reference swaps, so one captured graph serves every iteration)
+        K.lincomb(self.Gp, self.G, 1.0)
+        K.lincomb(self.gpp, self.gp, 1.0)
 
     def _capture(self, shrink_pow):
-        """Two HIP graphs of the iteration body, one per parity of the G/Gp swap."""
-        graphs = []
-        cur = torch.cuda.current_stream(self.dev)
-        cap = torch.cuda.Stream(self.dev)          # default priority
-        cap.wait_stream(cur)
-        with torch.cuda.stream(cap):
-            for _ in range(2):
-                g = torch.cuda.CUDAGraph()
-                g.capture_begin()
-                self._body(shrink_pow)
-                g.capture_end()
-                graphs.append(g)
-        cur.wait_stream(cap)
-        # each capture swapped the Python references once: two swaps restore them
-        return graphs
+        """HIP graph of the iteration body (identical launches to the eager body)."""
+        return capture_graphs(self.dev, lambda: self._body(shrink_pow))[0]
 
     def finalize(self):
         """solver.py:606-638: Fock, eigh canonicalisation, mu, p, Fock + E."""
@@ -330,13 +315,38 @@ class _CG:
         K.matmul(self.Q, self.F, self.Ft, C=self.Q, ops=0b001)              # Ft = Q^H F Q
         K.eigh(self.Ft, self.eps, self.V, herm_in=True)
         K.matmul(self.Q, self.V, self.H)                                      # Q W
-        self.Q, self.H = self.H, self.Q
+        K.lincomb(self.Q, self.H, 1.0)       # copy back: the captured graph holds this buffer as Q
         self.mu(self.eps, True)
         P = self.density()
         Pout = P.clone()
         F = torch.empty_like(Pout)
         self.fock.build(Pout, F, E_ptr=K.st_ptr(self.st, K.ST_E))
         return Pout, F
+
+
+def _cg_runner(kernel, cdt, cfg, fock_build_fn, n_electrons):
+    """The CG state for (kernel, precision, loop controls, electron count).
+
+    Kept on the kernel between solves when the iteration is graph-captured:
+    the buffers and the captured graph (which bakes in these controls and the
+    electron count) are reused, so a repeated solve allocates and captures
+    nothing.  At most four runners are kept per kernel.
+    """
+    if fock_build_fn is not None or cfg.project_fn is not None or cfg.block_sizes is not None \
+            or not cfg.use_graph:
+        return _CG(kernel, cdt, cfg, fock_build_fn)
+    key = (cdt, float(cfg.denom_scale), float(cfg.max_step), int(cfg.cg_restart), float(cfg.bt_shrink),
+           int(cfg.bt_max), int(cfg.mu_maxiter), str(cfg.cayley).lower(), float(n_electrons))
+    cache = kernel.__dict__.setdefault("_hfx_cg", {})
+    r = cache.get(key)
+    if r is None:
+        while len(cache) >= 4:
+            cache.pop(next(iter(cache)))
+        r = _CG(kernel, cdt, cfg, fock_build_fn)
+        r.kernel = r.fock.kernel = weakref.proxy(kernel)    # the kernel owns the runner, not both ways
+        cache[key] = r
+    r.cfg = cfg
+    return r
 
 
 def _solve_cg(kernel, P0, n_electrons, cfg: SolverConfig, fock_build_fn):
@@ -346,7 +356,7 @@ def _solve_cg(kernel, P0, n_electrons, cfg: SolverConfig, fock_build_fn):
     if polish not in (None, "diis"):
         raise ValueError(f"polish must be None or 'diis'; got {cfg.polish!r}")
     cdt = torch.complex128 if prec == "fp64" else torch.complex64
-    runner = _CG(kernel, cdt, cfg, fock_build_fn)
+    runner = _cg_runner(kernel, cdt, cfg, fock_build_fn, n_electrons)
     runner.set_target(n_electrons)
     runner.init_from(P0)
     if prec == "mixed":
@@ -363,7 +373,134 @@ def _solve_cg(kernel, P0, n_electrons, cfg: SolverConfig, fock_build_fn):
     return _result(runner, Pfin, Ffin, sv, k, converged, hist_E, hist_G, cfg.max_iter, dev)
 
 
-def _polish_diis(kernel, runner, n_electrons, cfg, fock_build_fn, k_cg, hist_E, hist_G, P_lo=None):
+class _DIISPolish:
+    """FP64 Pulay C-DIIS from the CG density to the fixed point [F, P] = 0.
+
+    One eigh per iteration: F_i = F[P_i] and the DIIS error e_i = F_i P_i -
+    P_i F_i (reference_scf.py:292-314) give P_{k+1} = density(herm(sum_i c_i
+    F_i)).  The reference loop also forms density(F_k) for its density
+    residual, a second eigh per iteration; here the density residual is the
+    step |P_k - P_{k-1}| the iteration itself takes.  The density update
+    (eigh of the Hermitised extrapolated Fock matrix, Newton mu warm-started
+    at the previous mu, occupations, density) is one replayed HIP graph, and
+    the object (buffers + graph) is kept on the kernel between solves.
+    """
+
+    def __init__(self, kernel, n_electrons, nh=6):
+        cdt = torch.complex128
+        self.kernel = kernel
+        self.dev = dev = kernel.device
+        nk1, nk2, nb = kernel.h.shape[0], kernel.h.shape[1], kernel.h.shape[-1]
+        self.shape4 = (nk1, nk2, nb, nb)
+        self.nk, self.nb = nk1 * nk2, nb
+        self.n_e = float(n_electrons)
+        self.fock = _Fock(kernel, cdt, None, None)
+        self.w = kernel._w64.reshape(-1).contiguous()
+        self.ws_sum = max(kernel.weight_sum, 1e-30)
+        self.wsp = K.Workspace.get(dev, self.nk)
+        self.nh = nh
+        c = dict(dtype=cdt, device=dev)
+        z = lambda: torch.zeros(self.shape4, **c)  # noqa: E731
+        self.P_prev, self.P_cur, self.P_new, self.Fx, self.U, self.tmp = z(), z(), z(), z(), z(), z()
+        self.F_hist = [z() for _ in range(nh)]
+        self.R_hist = [z() for _ in range(nh)]
+        self.eps = torch.zeros((nk1, nk2, nb), dtype=torch.float64, device=dev)
+        self.occ = torch.zeros_like(self.eps)
+        self.st = torch.zeros(K.ST_LEN, dtype=torch.float64, device=dev)
+        self.outs = torch.zeros(64, dtype=torch.float64, device=dev)  # [0:2] residuals, [2] E, [8:] B row
+        self.graph = None
+
+    def _mu(self):
+        K.solve_mu_steps(self.wsp, self.eps, self.w, self.nk, self.nb, self.n_e, self.kernel.T,
+                         mu0_from_state=True, maxiter=54, st=self.st, p_out=self.occ)
+
+    def _update(self):
+        """Fx (Hermitian) -> P_new: launches only, captured once."""
+        K.eigh(self.Fx, self.eps, self.U, herm_in=True)
+        self._mu()
+        K.density(self.U, self.tmp, p=self.occ)
+        K.hermitize(self.tmp, self.P_new)
+
+    def _capture(self):
+        return capture_graphs(self.dev, self._update)[0]
+
+    def run(self, P_lo, mu0, tol, max_iter):
+        nh, nk = self.nh, self.nk
+        K.hermitize(convert(P_lo, torch.complex128) if P_lo.dtype != torch.complex128 else P_lo, self.P_cur)
+        self.st[K.ST_MU] = mu0
+        part, outs = self.wsp.part, self.outs
+        o_res, o_B = outs[0:2], outs[8:8 + nh]
+        E_ptr = outs.data_ptr() + 8 * 2
+        B = np.zeros((nh, nh))
+        hE, hC = [], []
+        k, converged, E, idx = 0, False, float("nan"), 0
+        while k < max_iter:
+            idx = k % nh
+            Fk, Rk = self.F_hist[idx], self.R_hist[idx]
+            self.fock.build(self.P_cur, Fk, E_ptr=E_ptr)
+            K.scf_resid(Fk, self.P_cur, self.P_prev if k > 0 else None, Rk, part)
+            K.reduce(self.wsp, part, 2, nk, w=self.w, wmask=0b11, out=o_res)
+            for j0 in range(0, nh, 8):                                    # B row (reference_scf.py:110-111)
+                Ys = self.R_hist[j0:j0 + 8]
+                K.dots(Rk, Ys, part, mode=0)
+                K.reduce(self.wsp, part, len(Ys), nk, w=self.w, wmask=(1 << len(Ys)) - 1,
+                         out=o_B[j0:j0 + len(Ys)])
+            sv = outs.cpu().numpy()                                       # one readback per iteration
+            c_res = float(np.sqrt(sv[0] / self.ws_sum))
+            d_res = float(np.sqrt(sv[1] / self.ws_sum)) if k > 0 else float("inf")
+            E = float(sv[2])
+            hE.append(E)
+            hC.append(c_res)
+            if c_res <= tol and d_res <= tol:
+                converged = True
+                break
+            row = sv[8:8 + nh]
+            B[idx, :] = row
+            B[:, idx] = row
+            count = min(k + 1, nh)
+            active = np.arange(nh) < count
+            coeffs, ok = diis_solve(B * np.outer(active, active), active)
+            if count >= 2 and ok:
+                sel = [i for i in range(nh) if active[i] and coeffs[i] != 0.0]
+                K.lincomb_multi(self.Fx, [self.F_hist[i] for i in sel], [float(coeffs[i]) for i in sel],
+                                herm=True)
+            else:
+                K.lincomb_multi(self.Fx, [Fk], [1.0], herm=True)
+            if self.graph is None:
+                self._update()
+                self.graph = self._capture() if k > 0 else None
+            else:
+                self.graph.replay()
+            self.P_prev, self.P_cur = self.P_cur, self.P_prev
+            K.lincomb(self.P_cur, self.P_new, 1.0)
+            k += 1
+        n_it = len(hE)
+        Ffin = self.F_hist[idx]
+        if not converged and k > 0:     # budget spent after an update: evaluate the returned density
+            Ffin = self.Fx
+            self.fock.build(self.P_cur, Ffin, E_ptr=E_ptr)
+            E = float(outs[2].item())
+        # orbitals / occupations / mu of the returned state (solver.py:606-638 form)
+        Q = torch.empty_like(Ffin)
+        K.eigh(Ffin, self.eps, Q, herm_in=True)
+        self._mu()
+        return (self.P_cur.clone(), Ffin.clone(), Q, self.occ.clone(), self.st[K.ST_MU].clone(), E, n_it,
+                converged, hE, hC)
+
+
+def _polish_state(kernel, n_electrons):
+    cache = kernel.__dict__.setdefault("_hfx_polish", {})
+    key = float(n_electrons)
+    pol = cache.get(key)
+    if pol is None:
+        cache.clear()                 # one polish state per kernel (the last electron count)
+        pol = _DIISPolish(kernel, n_electrons)
+        pol.kernel = pol.fock.kernel = weakref.proxy(kernel)
+        cache[key] = pol
+    return pol
+
+
+def _polish_diis(kernel, runner, n_electrons, cfg, fock_build_fn, k_cg, hist_E, hist_G, P_lo=None, mu0=None):
     """FP64 Pulay-DIIS SCF from the CG density to the fixed point [F, P] = 0.
 
     The CG loop's windowed energy stop (solver.py:422-449) can end well before
@@ -376,6 +513,21 @@ def _polish_diis(kernel, runner, n_electrons, cfg, fock_build_fn, k_cg, hist_E, 
     dev = kernel.device
     if P_lo is None:
         P_lo = runner.density()
+    if fock_build_fn is None and cfg.project_fn is None and cfg.use_graph:
+        pol = _polish_state(kernel, n_electrons)
+        mu_start = runner.st[K.ST_MU] if mu0 is None else mu0
+        density, F, Q, p, mu, E, n2, conv, hE2, hC2 = pol.run(P_lo, mu_start, cfg.polish_tol, cfg.polish_max_iter)
+        k = k_cg + n2
+        hE = list(hist_E) + list(hE2)
+        hG = list(hist_G) + list(hC2)
+        hEt = torch.zeros(max(cfg.max_iter, k), dtype=torch.float64)
+        hGt = torch.zeros_like(hEt)
+        hEt[:k] = torch.tensor(hE[:k], dtype=torch.float64)
+        hGt[:k] = torch.tensor(hG[:k], dtype=torch.float64)
+        hist = dict(E=hEt.to(dev), grad_norm=hGt.to(dev), phase_iters=torch.tensor([k_cg, n2], dtype=torch.int64))
+        return SolveResult(Q=Q, p=p, mu=mu, density=density, fock=F,
+                           energy=torch.tensor(E, dtype=torch.float64, device=dev), n_iter=int(k),
+                           converged=bool(conv), history=hist)
     P64 = convert(P_lo, torch.complex128) if P_lo.dtype != torch.complex128 else P_lo.clone()
     scfg = SCFConfig(max_iter=cfg.polish_max_iter, acceleration="diis", density_tol=cfg.polish_tol,
                      comm_tol=cfg.polish_tol, project_fn=cfg.project_fn)
@@ -408,7 +560,7 @@ def _result(runner, Pfin, Ffin, sv, k, converged, hist_E, hist_G, max_iter, dev)
     if n:
         hE[:n] = torch.tensor(hist_E[:n], dtype=torch.float64)
         hG[:n] = torch.tensor(hist_G[:n], dtype=torch.float64)
-    return SolveResult(Q=runner.Q, p=runner.p, mu=sv[K.ST_MU].to(dev), density=Pfin, fock=Ffin,
+    return SolveResult(Q=runner.Q.clone(), p=runner.p.clone(), mu=sv[K.ST_MU].to(dev), density=Pfin, fock=Ffin,
                        energy=sv[K.ST_E].to(dev), n_iter=int(k), converged=bool(converged),
                        history=dict(E=hE.to(dev), grad_norm=hG.to(dev)))
 

@@ -366,3 +366,40 @@ def test_cg_gauss_jordan_cayley_matches_spectral(dev, nb):
     assert a.converged and b.converged
     assert abs(float(a.energy) - float(b.energy)) < 1e-10
     assert np.abs(_np(a.density) - _np(b.density)).max() < 1e-5
+
+
+def test_repeated_solves_reuse_cached_state(dev):
+    """A repeated solve on the same kernel reuses the kept CG runner, polish
+    state and their captured graphs: the second result equals the first
+    bitwise, the first result's tensors are not overwritten by the second
+    solve, and a solve at another electron count gets its own state."""
+    import jax_hf_amd as J
+    c = syn.config0(nk=24, nb=4, seed=0)
+    G = J.HartreeFockKernel(c["weights"], c["hamiltonian"], c["coulomb_q"], c["T"])
+    cfg = J.SolverConfig(max_iter=400, precision="mixed")
+    r1 = J.solve(G, c["P0"], c["n_electrons"], config=cfg)
+    snap = {k: _np(getattr(r1, k)).copy() for k in ("Q", "p", "density", "fock")}
+    E1 = float(r1.energy)
+    r2 = J.solve(G, c["P0"], c["n_electrons"], config=cfg)
+    assert float(r2.energy) == E1 and r2.n_iter == r1.n_iter
+    for k, v in snap.items():
+        np.testing.assert_array_equal(_np(getattr(r1, k)), v)
+        np.testing.assert_array_equal(_np(getattr(r2, k)), v)
+    r3 = J.solve(G, c["P0"], c["n_electrons"] + 1.0, config=cfg)
+    K0 = O.OracleKernel(c["weights"], c["hamiltonian"], c["coulomb_q"], c["T"])
+    r0 = O.solve_scf(K0, c["P0"], c["n_electrons"] + 1.0,
+                     config=O.SCFConfig(max_iter=300, acceleration="diis", density_tol=1e-11, comm_tol=1e-11))
+    assert r3.converged and r0.converged and abs(float(r3.energy) - r0.energy) < 1e-8
+    # the kept state refers back to its kernel weakly: dropping the kernel frees
+    # it (and the state, graphs included) at once, without the cyclic collector
+    import gc
+    import weakref
+    wr = weakref.ref(G)
+    was = gc.isenabled()
+    gc.disable()
+    try:
+        del G
+        assert wr() is None
+    finally:
+        if was:
+            gc.enable()
