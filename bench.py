@@ -197,7 +197,8 @@ def _timed_solves(J, G, P0, ne, cfg, n):
 
 def solve_bench(J, inp, dev, nk, nb, args):
     G = J.HartreeFockKernel(inp["weights"], inp["hamiltonian"], inp["coulomb_q"], SOLVE_T, device=dev)
-    P0 = np.zeros_like(inp["hamiltonian"])
+    import torch
+    P0 = torch.zeros(inp["hamiltonian"].shape, dtype=torch.complex128, device=dev)   # resident in HBM
     ne = nb / 2.0
     # FP64 fixed point (the accuracy reference): DIIS SCF to 1e-10 residuals
     ref = J.solve_scf(G, P0, ne, config=J.SCFConfig(max_iter=200, acceleration="diis",
@@ -252,7 +253,7 @@ def scan_bench(J, syn, dev, args):
         h0, V, w = cache[nk]
         kernels.append(J.HartreeFockKernel(w, np.ascontiguousarray(h0 + syn.displacement(nb, sp["u"])), V,
                                            SOLVE_T, device=dev))
-        P0s.append(np.zeros((nk, nk, nb, nb), np.complex128))
+        P0s.append(torch.zeros((nk, nk, nb, nb), dtype=torch.complex128, device=dev))
         ne.append(sp["n_electrons"])
     torch.cuda.synchronize()
     setup = time.perf_counter() - t0

@@ -18,12 +18,11 @@ def capture_graphs(dev, fn, n=1):
     """Record n successive calls of fn() (launches only) as HIP graphs.
 
     Captured on a side stream of default priority.  The cyclic garbage
-    collector runs first and is held off until the captures end: a collection
-    inside a capture could destroy another graph or free the device memory
-    of an unreachable solver state while the stream is capturing.
+    collector is held off until the captures end: a collection inside a
+    capture could destroy another graph or free the device memory of an
+    unreachable solver state while the stream is capturing.
     """
     import gc
-    gc.collect()
     was = gc.isenabled()
     gc.disable()
     graphs = []

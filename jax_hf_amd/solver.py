@@ -384,6 +384,12 @@ class _DIISPolish:
     (eigh of the Hermitised extrapolated Fock matrix, Newton mu warm-started
     at the previous mu, occupations, density) is one replayed HIP graph, and
     the object (buffers + graph) is kept on the kernel between solves.
+
+    After the first update of a run the eigh starts from the previous basis U:
+    U^H F U is nearly diagonal once the iteration settles, so the Jacobi
+    sweeps stop after one or two, and V = U W (the CG finish does the same,
+    solver.py:622-625).  Each run begins with a cold eigh, so rounding in the
+    accumulated basis stays bounded by one run's updates.
     """
 
     def __init__(self, kernel, n_electrons, nh=6):
@@ -402,6 +408,7 @@ class _DIISPolish:
         c = dict(dtype=cdt, device=dev)
         z = lambda: torch.zeros(self.shape4, **c)  # noqa: E731
         self.P_prev, self.P_cur, self.P_new, self.Fx, self.U, self.tmp = z(), z(), z(), z(), z(), z()
+        self.Ft, self.W = z(), z()
         self.F_hist = [z() for _ in range(nh)]
         self.R_hist = [z() for _ in range(nh)]
         self.eps = torch.zeros((nk1, nk2, nb), dtype=torch.float64, device=dev)
@@ -414,9 +421,19 @@ class _DIISPolish:
         K.solve_mu_steps(self.wsp, self.eps, self.w, self.nk, self.nb, self.n_e, self.kernel.T,
                          mu0_from_state=True, maxiter=54, st=self.st, p_out=self.occ)
 
-    def _update(self):
-        """Fx (Hermitian) -> P_new: launches only, captured once."""
-        K.eigh(self.Fx, self.eps, self.U, herm_in=True)
+    def _eig_warm(self, A):
+        """(eps, U) <- eigh(A) from the current basis U: U^H A U = W eps W^H, U <- U W."""
+        K.matmul(self.U, A, self.Ft, C=self.U, ops=0b001, herm_out=True)
+        K.eigh(self.Ft, self.eps, self.W, herm_in=True)
+        K.matmul(self.U, self.W, self.tmp)
+        K.lincomb(self.U, self.tmp, 1.0)
+
+    def _update(self, warm=True):
+        """Fx (Hermitian) -> P_new: launches only (the warm form is captured once)."""
+        if warm:
+            self._eig_warm(self.Fx)
+        else:
+            K.eigh(self.Fx, self.eps, self.U, herm_in=True)
         self._mu()
         K.density(self.U, self.tmp, p=self.occ)
         K.hermitize(self.tmp, self.P_new)
@@ -466,9 +483,11 @@ class _DIISPolish:
                                 herm=True)
             else:
                 K.lincomb_multi(self.Fx, [Fk], [1.0], herm=True)
-            if self.graph is None:
+            if k == 0:
+                self._update(warm=False)
+            elif self.graph is None:
                 self._update()
-                self.graph = self._capture() if k > 0 else None
+                self.graph = self._capture()
             else:
                 self.graph.replay()
             self.P_prev, self.P_cur = self.P_cur, self.P_prev
@@ -481,8 +500,11 @@ class _DIISPolish:
             self.fock.build(self.P_cur, Ffin, E_ptr=E_ptr)
             E = float(outs[2].item())
         # orbitals / occupations / mu of the returned state (solver.py:606-638 form)
-        Q = torch.empty_like(Ffin)
-        K.eigh(Ffin, self.eps, Q, herm_in=True)
+        if k > 0:
+            self._eig_warm(Ffin)
+        else:
+            K.eigh(Ffin, self.eps, self.U, herm_in=True)
+        Q = self.U.clone()
         self._mu()
         return (self.P_cur.clone(), Ffin.clone(), Q, self.occ.clone(), self.st[K.ST_MU].clone(), E, n_it,
                 converged, hE, hC)
