@@ -159,6 +159,17 @@ struct MuArgs {
 
 __device__ __forceinline__ double sigm(double x) { return 1.0 / (1.0 + exp(-x)); }
 
+// Newton mu is converged when the step is below 4 ulp of max(|mu|, T), when
+// N(mu) matches the target to the rounding of its own sum (64 eps N: past
+// that the sign of g is noise and further steps only shrink the bracket), or
+// when the bracket has closed to 4 ulp.
+__device__ __forceinline__ bool mu_converged(double g, double Z, double N, double mu, double Tj, double lo,
+                                             double hi) {
+  const double tol_mu = 8.9e-16 * fmax(fabs(mu), Tj);
+  return fabs(g / fmax(Z, 1e-18)) <= tol_mu || fabs(g) <= 64.0 * 2.220446049250313e-16 * fabs(N) ||
+         hi - lo <= tol_mu;
+}
+
 // Block-wide sum of the first n entries of a global array (strided per
 // thread, then a block tree): every block of the grid evaluates the same
 // expression tree over the same data, so all blocks get bitwise-identical totals.
@@ -313,16 +324,15 @@ __global__ __launch_bounds__(MU_BLOCK) void mu_kernel(MuArgs a) {
       double Np = 0.0, Zp = 0.0;
       for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
         const double pv = sigm((mu - a.eps[e]) / Tj);
-        const double wk = a.w[e / a.nb];
+        const double wk = a.w[(unsigned)e / (unsigned)a.nb];   // n < 2^32
         Np += wk * pv;
         Zp += wk * pv * (1.0 - pv) / Tj;
       }
       double N, Z;
       grid_sum2(grid, Np, Zp, a.scratch, 1 + (it & 1), sh, N, Z);
       const double g = N - a.target;
-      // converged: the Newton step is below 4 ulp of max(|mu|, T) (every block
-      // takes this branch together, the totals being bitwise identical)
-      if (fabs(g / fmax(Z, 1e-18)) <= 8.9e-16 * fmax(fabs(mu), Tj)) break;
+      // every block takes this branch together, the totals being bitwise identical
+      if (mu_converged(g, Z, N, mu, Tj, lo, hi)) break;
       if (g < 0) lo = mu;
       if (g > 0) hi = mu;
       double mu_new = mu - g / fmax(Z, 1e-18);
@@ -344,7 +354,7 @@ __global__ __launch_bounds__(MU_BLOCK) void mu_kernel(MuArgs a) {
       double Np = 0.0, Zp = 0.0;
       for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
         const double pv = sigm((mu - a.eps[e]) / Tj);
-        const double wk = a.w[e / a.nb];
+        const double wk = a.w[(unsigned)e / (unsigned)a.nb];   // n < 2^32
         Np += wk * pv;
         Zp += wk * pv * (1.0 - pv) / Tj;
       }
@@ -352,7 +362,8 @@ __global__ __launch_bounds__(MU_BLOCK) void mu_kernel(MuArgs a) {
       if (it == a.maxiter) break;   // final residual check below
       const double g = N - a.target;
       // converged: the Newton step is below 4 ulp of max(|mu|, T) (every block
-      // takes this branch together, the totals being bitwise identical)
+      // takes this branch together, the totals being bitwise identical); no
+      // noise-floor stop here: the final check below wants |g| <= 1e-12
       if (fabs(g / fmax(Z, 1e-18)) <= 8.9e-16 * fmax(fabs(mu), Tj)) break;
       if (g < 0) lo = mu;
       if (g > 0) hi = mu;
@@ -434,9 +445,19 @@ __global__ __launch_bounds__(MUS_BLOCK) void mu_bracket_kernel(MuStepArgs a) {
     a.part[2 * blockIdx.x + 1] = hi;
   }
   if (!last_block_done(a.ticket)) return;
+  // all threads of the last block gather the block extremes (min / max are order-free)
+  double mn = INFINITY, mx = -INFINITY;
+  for (unsigned b = threadIdx.x; b < gridDim.x; b += MUS_BLOCK) {
+    mn = fmin(mn, a.part[2 * b]);
+    mx = fmax(mx, a.part[2 * b + 1]);
+  }
+  mn = wave_min(mn);
+  mx = wave_max(mx);
+  __syncthreads();
+  if (lane == 0) { sh[wid] = mn; sh[nw + wid] = mx; }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    double mn = INFINITY, mx = -INFINITY;
-    for (unsigned b = 0; b < gridDim.x; ++b) { mn = fmin(mn, a.part[2 * b]); mx = fmax(mx, a.part[2 * b + 1]); }
+    for (int q = 0; q < nw; ++q) { mn = fmin(mn, sh[q]); mx = fmax(mx, sh[nw + q]); }
     const double Tj = fmax(a.T, 1e-12);
     const double blo = mn - 50.0 * Tj, bhi = mx + 50.0 * Tj;   // solver.py:70-72
     double mu = a.mu0_from_state ? a.st[HFX_ST_MU] : a.mu0;
@@ -458,7 +479,7 @@ __global__ __launch_bounds__(MUS_BLOCK) void mu_step_kernel(MuStepArgs a) {
   double acc[2] = {0.0, 0.0};
   for (size_t e = (size_t)blockIdx.x * MUS_BLOCK + threadIdx.x; e < n; e += stride) {
     const double pv = sigm((mu - a.eps[e]) / Tj);
-    const double wk = a.w[e / a.nb];
+    const double wk = a.w[(unsigned)e / (unsigned)a.nb];   // n < 2^32
     acc[0] += wk * pv;
     acc[1] += wk * pv * (1.0 - pv) / Tj;
   }
@@ -472,10 +493,10 @@ __global__ __launch_bounds__(MUS_BLOCK) void mu_step_kernel(MuStepArgs a) {
   block_gather2(a.part, (int)gridDim.x, sh, N, Z);
   if (threadIdx.x == 0) {
     const double g = N - a.target;   // solver.py:80-99
-    if (fabs(g / fmax(Z, 1e-18)) <= 8.9e-16 * fmax(fabs(mu), Tj)) {   // step below 4 ulp: converged
+    double lo = a.st[HFX_ST_MULO], hi = a.st[HFX_ST_MUHI];
+    if (mu_converged(g, Z, N, mu, Tj, lo, hi)) {
       a.st[HFX_ST_MUDONE] = 1.0;
     } else {
-      double lo = a.st[HFX_ST_MULO], hi = a.st[HFX_ST_MUHI];
       if (g < 0) lo = mu;
       if (g > 0) hi = mu;
       double mu_new = mu - g / fmax(Z, 1e-18);
@@ -651,10 +672,10 @@ __global__ __launch_bounds__(MUS_BLOCK) void mu_step_seg_kernel(MuSegArgs a) {
   block_gather2(mp, (int)gridDim.x, sh, N, Z);
   if (threadIdx.x == 0) {
     const double g = N - a.targets[sidx];   // solver.py:80-99
-    if (fabs(g / fmax(Z, 1e-18)) <= 8.9e-16 * fmax(fabs(mu), Tj)) {
+    double lo = st[HFX_ST_MULO], hi = st[HFX_ST_MUHI];
+    if (mu_converged(g, Z, N, mu, Tj, lo, hi)) {
       st[HFX_ST_MUDONE] = 1.0;
     } else {
-      double lo = st[HFX_ST_MULO], hi = st[HFX_ST_MUHI];
       if (g < 0) lo = mu;
       if (g > 0) hi = mu;
       double mu_new = mu - g / fmax(Z, 1e-18);
@@ -738,7 +759,7 @@ int hfx_reduce(const double* part, int nval, size_t nk, const double* w, unsigne
 int hfx_solve_mu_steps(const double* eps, const double* w, size_t nk, int nb, double target, double T,
                        double mu0, int mu0_from_state, int maxiter, double* st, double* p_out, double* scratch,
                        void* stream) {
-  if (!eps || !w || !scratch || !st || nb < 1 || nk == 0 || maxiter < 0)
+  if (!eps || !w || !scratch || !st || nb < 1 || nk == 0 || maxiter < 0 || nk * (size_t)nb >= 0xFFFFFFFFull)
     return set_error(HFX_EINVAL, "hfx_solve_mu_steps: bad argument");
   const size_t n = nk * (size_t)nb;
   size_t nbk = (n + (size_t)MUS_BLOCK * 4 - 1) / ((size_t)MUS_BLOCK * 4);
@@ -820,7 +841,8 @@ int hfx_solve_mu_batched(const double* eps, const double* w, const double* targe
 int hfx_solve_mu(const double* eps, const double* w, size_t nk, int nb, double target, double T, double mu0,
                  int mu0_from_state, int maxiter, int mode, double level_shift, double* st, double* mu_out,
                  double* p_out, double* scratch, void* stream) {
-  if (!eps || !w || !scratch || nb < 1 || nk == 0) return set_error(HFX_EINVAL, "hfx_solve_mu: bad argument");
+  if (!eps || !w || !scratch || nb < 1 || nk == 0 || nk * (size_t)nb >= 0xFFFFFFFFull)
+    return set_error(HFX_EINVAL, "hfx_solve_mu: bad argument");
   if (mode != HFX_MU_NEWTON_BRACKET && mode != HFX_MU_BISECTION && mode != HFX_MU_NEWTON)
     return set_error(HFX_EINVAL, "hfx_solve_mu: unknown mode %d", mode);
   if (mu0_from_state && !st) return set_error(HFX_EINVAL, "hfx_solve_mu: mu0_from_state needs st");

@@ -180,10 +180,16 @@ class _CG:
         K.density(self.Q, self.P, p=self.p)
         return _project_herm(self.P, self.cfg.project_fn, self.tmp)
 
-    def mu(self, eps, from_state, mu0=0.0):
+    # Newton steps per mu solve inside the iteration (a step after convergence
+    # returns at once but still costs a dispatch); from the previous mu a few
+    # converge, and a solve that needs more is finished on the readback
+    MU_LOOP_STEPS = 8
+
+    def mu(self, eps, from_state, mu0=0.0, steps=None):
         """_solve_mu (solver.py:62-101): Newton with bracket, graph-capturable launches."""
+        n = self.cfg.mu_maxiter if steps is None else min(steps, self.cfg.mu_maxiter)
         K.solve_mu_steps(self.ws, eps, self.w_norm, self.nk, self.nb, self.n_norm, self.T, mu0=mu0,
-                         mu0_from_state=from_state, maxiter=self.cfg.mu_maxiter, st=self.st, p_out=self.p)
+                         mu0_from_state=from_state, maxiter=n, st=self.st, p_out=self.p)
 
     def set_target(self, n_electrons):
         self.n_norm = float(n_electrons) / max(self.kernel.weight_sum, 1e-30)
@@ -242,6 +248,8 @@ class _CG:
             else:
                 self._body(shrink_pow)
             sv = st.cpu().numpy()                                               # one readback
+            if sv[K.ST_MUDONE] == 0.0:       # mu not converged in the loop's steps: finish it
+                self.mu(self.eps, True)
             E = float(sv[K.ST_E])
             grad_norm = float(sv[K.ST_GRAD])
             hist_E.append(E)
@@ -297,7 +305,7 @@ class _CG:
         else:
             L.call("hfx_cg_retract", dt, nb, nk, L.ptr(self.Q), L.ptr(self.V), L.ptr(self.lam), L.ptr(self.Fe),
                    L.ptr(st), 0.0, 0, L.ptr(self.eps), None, sm)                # :586-590
-        self.mu(self.eps, True)                                                 # :591-592
+        self.mu(self.eps, True, steps=self.MU_LOOP_STEPS)                      # :591-592
         # this iteration's gradient / preconditioned norms become the previous ones
         # (copies, not reference swaps, so one captured graph serves every iteration)
         K.lincomb(self.Gp, self.G, 1.0)
@@ -413,13 +421,18 @@ class _DIISPolish:
         self.R_hist = [z() for _ in range(nh)]
         self.eps = torch.zeros((nk1, nk2, nb), dtype=torch.float64, device=dev)
         self.occ = torch.zeros_like(self.eps)
-        self.st = torch.zeros(K.ST_LEN, dtype=torch.float64, device=dev)
-        self.outs = torch.zeros(64, dtype=torch.float64, device=dev)  # [0:2] residuals, [2] E, [8:] B row
+        # one readback buffer: outs [0:2] residuals, [2] E, [8:] B row; then the state vector
+        self.io = torch.zeros(64 + K.ST_LEN, dtype=torch.float64, device=dev)
+        self.outs, self.st = self.io[:64], self.io[64:]
         self.graph = None
 
-    def _mu(self):
+    # Newton steps per mu solve inside the update graph: from the previous mu a
+    # handful converge; a solve that needs more is completed on the next readback
+    MU_GRAPH_STEPS = 6
+
+    def _mu(self, maxiter=54):
         K.solve_mu_steps(self.wsp, self.eps, self.w, self.nk, self.nb, self.n_e, self.kernel.T,
-                         mu0_from_state=True, maxiter=54, st=self.st, p_out=self.occ)
+                         mu0_from_state=True, maxiter=maxiter, st=self.st, p_out=self.occ)
 
     def _eig_warm(self, A):
         """(eps, U) <- eigh(A) from the current basis U: U^H A U = W eps W^H, U <- U W."""
@@ -434,7 +447,7 @@ class _DIISPolish:
             self._eig_warm(self.Fx)
         else:
             K.eigh(self.Fx, self.eps, self.U, herm_in=True)
-        self._mu()
+        self._mu(self.MU_GRAPH_STEPS)
         K.density(self.U, self.tmp, p=self.occ)
         K.hermitize(self.tmp, self.P_new)
 
@@ -450,7 +463,7 @@ class _DIISPolish:
         E_ptr = outs.data_ptr() + 8 * 2
         B = np.zeros((nh, nh))
         hE, hC = [], []
-        k, converged, E, idx = 0, False, float("nan"), 0
+        k, converged, E, idx, redo = 0, False, float("nan"), 0, False
         while k < max_iter:
             idx = k % nh
             Fk, Rk = self.F_hist[idx], self.R_hist[idx]
@@ -462,7 +475,17 @@ class _DIISPolish:
                 K.dots(Rk, Ys, part, mode=0)
                 K.reduce(self.wsp, part, len(Ys), nk, w=self.w, wmask=(1 << len(Ys)) - 1,
                          out=o_B[j0:j0 + len(Ys)])
-            sv = outs.cpu().numpy()                                       # one readback per iteration
+            sv_all = self.io.cpu().numpy()                                # one readback per iteration
+            if k > 0 and sv_all[64 + K.ST_MUDONE] == 0.0 and not redo:
+                # the last update's mu needed more than the graph's steps: finish
+                # it, rebuild that density and evaluate this iteration again
+                self._mu()
+                K.density(self.U, self.tmp, p=self.occ)
+                K.hermitize(self.tmp, self.P_cur)
+                redo = True
+                continue
+            redo = False
+            sv = sv_all[:64]
             c_res = float(np.sqrt(sv[0] / self.ws_sum))
             d_res = float(np.sqrt(sv[1] / self.ws_sum)) if k > 0 else float("inf")
             E = float(sv[2])
@@ -505,7 +528,9 @@ class _DIISPolish:
         else:
             K.eigh(Ffin, self.eps, self.U, herm_in=True)
         Q = self.U.clone()
-        self._mu()
+        self._mu(self.MU_GRAPH_STEPS)
+        if float(self.st[K.ST_MUDONE].item()) == 0.0:
+            self._mu()
         return (self.P_cur.clone(), Ffin.clone(), Q, self.occ.clone(), self.st[K.ST_MU].clone(), E, n_it,
                 converged, hE, hC)
 
