@@ -53,6 +53,9 @@ struct PostArgs {
   int restart, last, omega0, restart_every;
 };
 
+__device__ __forceinline__ void post_op(const double (&acc)[RED_MAX_VALS], int nval, double* __restrict__ out,
+                                        double* __restrict__ st, const PostArgs& pa);
+
 __device__ __forceinline__ void stage2_body(const double* __restrict__ s1, int nval, int nblk,
                                             double* __restrict__ out, double* __restrict__ st, const PostArgs& pa,
                                             double* sh) {
@@ -64,7 +67,12 @@ __device__ __forceinline__ void stage2_body(const double* __restrict__ s1, int n
     for (int v = 0; v < RED_MAX_VALS; ++v)
       if (v < nval) acc[v] += s1[(size_t)v * nblk + b];
   block_sum<RED_MAX_VALS>(acc, sh);
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x == 0) post_op(acc, nval, out, st, pa);
+}
+
+// The scalar step applied to the weighted sums (thread 0).
+__device__ __forceinline__ void post_op(const double (&acc)[RED_MAX_VALS], int nval, double* __restrict__ out,
+                                        double* __restrict__ st, const PostArgs& pa) {
   if (out)
     for (int v = 0; v < nval; ++v) out[v] = acc[v];
   const double tiny = 1e-30;
@@ -131,6 +139,31 @@ __global__ __launch_bounds__(RED_BLOCK) void reduce_fused(const double* __restri
   __threadfence();
   if (threadIdx.x == 0) *ticket = 0u;
   stage2_body(s1, nval, (int)gridDim.x, out, st, pa, sh);
+}
+
+// Small reductions in one workgroup: no block partials, no completion ticket and
+// so no device-scope fence (which writes back the XCD's L2 and costs more than
+// the sum itself at these sizes).
+constexpr int RED1_BLOCK = 1024;
+constexpr size_t RED1_MAX_NK = 2048;   // above this one CU reads the partials too slowly
+
+__global__ __launch_bounds__(RED1_BLOCK) void reduce_single(const double* __restrict__ part, int nval, size_t nk,
+                                                            const double* __restrict__ w, unsigned wmask,
+                                                            double* __restrict__ out, double* __restrict__ st,
+                                                            PostArgs pa) {
+  __shared__ double sh[RED_MAX_VALS * (RED1_BLOCK / 64)];
+  if (pa.op == HFX_OP_TRIAL && st[HFX_ST_ACC] != 0.0) return;   // line search already done
+  double acc[RED_MAX_VALS];
+#pragma unroll
+  for (int v = 0; v < RED_MAX_VALS; ++v) acc[v] = 0.0;
+  for (size_t k = threadIdx.x; k < nk; k += RED1_BLOCK) {
+    const double wk = w ? w[k] : 1.0;
+#pragma unroll
+    for (int v = 0; v < RED_MAX_VALS; ++v)
+      if (v < nval) acc[v] += ((wmask >> v) & 1u ? wk : 1.0) * part[(size_t)v * nk + k];
+  }
+  block_sum<RED_MAX_VALS>(acc, sh);
+  if (threadIdx.x == 0) post_op(acc, nval, out, st, pa);
 }
 
 int reduce_blocks(size_t nk) {
@@ -750,6 +783,10 @@ int hfx_reduce(const double* part, int nval, size_t nk, const double* w, unsigne
     pa.last = (int)params[7];
     pa.omega0 = (int)params[8];
     pa.restart_every = (int)params[9];
+  }
+  if (nk <= RED1_MAX_NK) {
+    hipLaunchKernelGGL(reduce_single, dim3(1), dim3(RED1_BLOCK), 0, s, part, nval, nk, w, wmask, out, st, pa);
+    return check_launch("reduce_single");
   }
   hipLaunchKernelGGL(reduce_fused, dim3(g), dim3(RED_BLOCK), 0, s, part, nval, nk, w, wmask, scratch, out, st, pa,
                      reinterpret_cast<unsigned*>(scratch + RED_TICKET_OFF));
