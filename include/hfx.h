@@ -272,6 +272,16 @@ int hfx_rotate_eigh(int dtype, int nb, size_t nk, void* Q, const void* F, double
 int hfx_matmul_batched(int dtype, int nb, size_t nk, const void* A, const void* B, const void* C, int ops,
                        const void* Bsub, int herm_out, void* out, void* stream);
 
+/* Ft = U^dagger herm(sum_v coef[v] X[v]) U per k (Hermitised; 1 <= nin <= 8):
+ * the DIIS-extrapolated Fock matrix (reference_scf.py:298-314) expressed in the
+ * basis U, ready for an eigh that starts from the previous orbitals. */
+int hfx_extrap_rotate(int dtype, int nb, size_t nk, int nin, const void* const* X, const double* coef, const void* U,
+                      void* Ft, void* stream);
+/* U <- U W (in place) and P = herm(U diag(occ) U^dagger) per k with the new U:
+ * eigenvectors back from the rotated basis and the density of solver.py:42-44. */
+int hfx_rotate_density(int dtype, int nb, size_t nk, void* U, const void* W, const double* occ, void* P,
+                       void* stream);
+
 /* SCF residuals (reference_scf.py:291-294): comm = F D - D F (stored if comm),
  * part[0][k] = sum |comm|^2, part[1][k] = sum |Pn - D|^2 (Pn may be NULL). */
 int hfx_scf_resid(int dtype, int nb, size_t nk, const void* F, const void* D, const void* Pn, void* comm,

@@ -76,6 +76,8 @@ _SIGS = {
     "hfx_cg_retract": [i32, i32, sz, vp, vp, vp, vp, vp, f64, i32, vp, vp, vp],
     "hfx_rotate_eigh": [i32, i32, sz, vp, vp, vp, vp],
     "hfx_matmul_batched": [i32, i32, sz, vp, vp, vp, i32, vp, i32, vp, vp],
+    "hfx_extrap_rotate": [i32, i32, sz, i32, vp, vp, vp, vp, vp],
+    "hfx_rotate_density": [i32, i32, sz, vp, vp, vp, vp, vp],
     "hfx_scf_resid": [i32, i32, sz, vp, vp, vp, vp, vp, vp],
     "hfx_batched_dots": [i32, i32, sz, vp, vp, i32, i32, vp, vp],
     "hfx_vec_dots": [vp, vp, i32, sz, i32, vp, vp],

@@ -142,6 +142,23 @@ def lincomb_multi(out, Xs, coefs, herm=False):
            C.cast(ca, C.c_void_p), int(herm), L.ptr(out), s(out.device))
 
 
+def extrap_rotate(Ft, U, Xs, coefs):
+    """Ft = U^H herm(sum_v coefs[v] Xs[v]) U per k (<= 8 inputs)."""
+    import ctypes as C
+    nb = U.shape[-1]
+    xa = (C.c_void_p * len(Xs))(*[x.data_ptr() for x in Xs])
+    ca = (C.c_double * len(Xs))(*[float(c) for c in coefs])
+    L.call("hfx_extrap_rotate", dt_code(U), nb, U.numel() // (nb * nb), len(Xs), C.cast(xa, C.c_void_p),
+           C.cast(ca, C.c_void_p), L.ptr(U), L.ptr(Ft), s(U.device))
+
+
+def rotate_density(U, W, occ, P):
+    """U <- U W per k, P = herm(U diag(occ) U^H) with the new U."""
+    nb = U.shape[-1]
+    L.call("hfx_rotate_density", dt_code(U), nb, U.numel() // (nb * nb), L.ptr(U), L.ptr(W), L.ptr(occ), L.ptr(P),
+           s(U.device))
+
+
 def lincomb(out, X, a=1.0, Y=None, b=0.0):
     """out = a X + b Y (a, b real or complex)."""
     from ._util import KIND

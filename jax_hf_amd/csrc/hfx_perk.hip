@@ -757,6 +757,58 @@ __global__ __launch_bounds__(64) void pk_mm_kernel(int nb, const cx<T>* __restri
   }
 }
 
+// DIIS extrapolation in the current basis (reference_scf.py:298-314, then the
+// eigh of reference_scf.py:87-96 started from the basis U):
+//   Ft = U^dagger herm(sum_v c_v X_v) U   (Hermitised), one wave per k.
+constexpr int PK_XMAX = 8;
+struct PkInputs {
+  const void* x[PK_XMAX];
+  double c[PK_XMAX];
+};
+
+template <typename T>
+__global__ __launch_bounds__(64) void pk_extrap_rotate_kernel(int nb, int nin, PkInputs in,
+                                                              const cx<T>* __restrict__ U, cx<T>* __restrict__ Ft) {
+  __shared__ cx<T> sF[PK_MAT], sU[PK_MAT], sX[PK_MAT];
+  const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  PK_FOR_ELEM(nb, {
+    cx<T> a{0, 0};
+    for (int v = 0; v < nin; ++v) a = a + (T)in.c[v] * reinterpret_cast<const cx<T>*>(in.x[v])[k * nb2 + e];
+    sX[i * PK_PITCH + j] = a;
+  })
+  m_load(sU, U + k * nb2, nb);
+  __syncthreads();
+  PK_FOR_ELEM(nb, { sF[i * PK_PITCH + j] = (T)0.5 * (sX[i * PK_PITCH + j] + conj(sX[j * PK_PITCH + i])); })
+  __syncthreads();
+  m_gemm<T, true, false>(sX, sU, sF, nb);     // U^H F
+  m_gemm<T, false, false>(sF, sX, sU, nb);    // (U^H F) U
+  PK_FOR_ELEM(nb, { Ft[k * nb2 + e] = (T)0.5 * (sF[i * PK_PITCH + j] + conj(sF[j * PK_PITCH + i])); })
+}
+
+// Back from the rotated basis: U <- U W (in place, per k), and the density
+// P = herm(U diag(occ) U^dagger) of the new orbitals (solver.py:42-44).
+template <typename T>
+__global__ __launch_bounds__(64) void pk_rotate_density_kernel(int nb, cx<T>* __restrict__ U,
+                                                               const cx<T>* __restrict__ W,
+                                                               const double* __restrict__ occ,
+                                                               cx<T>* __restrict__ P) {
+  __shared__ cx<T> sU[PK_MAT], sW[PK_MAT], sX[PK_MAT];
+  __shared__ double sp[PK_NBMAX];
+  const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  m_load(sU, U + k * nb2, nb);
+  m_load(sW, W + k * nb2, nb);
+  if ((int)threadIdx.x < nb) sp[threadIdx.x] = occ[k * nb + threadIdx.x];
+  __syncthreads();
+  m_gemm<T, false, false>(sX, sU, sW, nb);    // new orbitals U W
+  PK_FOR_ELEM(nb, {
+    U[k * nb2 + e] = sX[i * PK_PITCH + j];
+    sW[i * PK_PITCH + j] = (T)sp[j] * sX[i * PK_PITCH + j];
+  })
+  __syncthreads();
+  m_gemm<T, false, true>(sU, sW, sX, nb);     // (U W) diag(occ) (U W)^H
+  PK_FOR_ELEM(nb, { P[k * nb2 + e] = (T)0.5 * (sU[i * PK_PITCH + j] + conj(sU[j * PK_PITCH + i])); })
+}
+
 // SCF residuals (reference_scf.py:291-294): comm = F D - D F (stored when
 // comm != NULL), part[0][k] = sum |comm|^2, part[1][k] = sum |Pn - D|^2.
 template <typename T>
@@ -1115,6 +1167,43 @@ int hfx_matmul_batched(int dtype, int nb, size_t nk, const void* A, const void* 
   }
 #undef MM_CASE
   return check_launch("pk_mm_kernel");
+}
+
+int hfx_extrap_rotate(int dtype, int nb, size_t nk, int nin, const void* const* X, const double* coef, const void* U,
+                      void* Ft, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  if (!X || !coef || !U || !Ft || nin < 1 || nin > PK_XMAX)
+    return set_error(HFX_EINVAL, "hfx_extrap_rotate: bad argument (nin=%d)", nin);
+  PkInputs in{};
+  for (int v = 0; v < nin; ++v) {
+    if (!X[v]) return set_error(HFX_EINVAL, "hfx_extrap_rotate: input %d is NULL", v);
+    in.x[v] = X[v];
+    in.c[v] = coef[v];
+  }
+  if (nk == 0) return HFX_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(pk_extrap_rotate_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nin, in, PKC(double, U),
+                       PKM(double, Ft));
+  else
+    hipLaunchKernelGGL(pk_extrap_rotate_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nin, in, PKC(float, U),
+                       PKM(float, Ft));
+  return check_launch("pk_extrap_rotate_kernel");
+}
+
+int hfx_rotate_density(int dtype, int nb, size_t nk, void* U, const void* W, const double* occ, void* P,
+                       void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  if (!U || !W || !occ || !P || U == W || U == P) return set_error(HFX_EINVAL, "hfx_rotate_density: bad argument");
+  if (nk == 0) return HFX_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(pk_rotate_density_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKM(double, U),
+                       PKC(double, W), occ, PKM(double, P));
+  else
+    hipLaunchKernelGGL(pk_rotate_density_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKM(float, U),
+                       PKC(float, W), occ, PKM(float, P));
+  return check_launch("pk_rotate_density_kernel");
 }
 
 int hfx_scf_resid(int dtype, int nb, size_t nk, const void* F, const void* D, const void* Pn, void* comm,

@@ -434,22 +434,25 @@ class _DIISPolish:
         K.solve_mu_steps(self.wsp, self.eps, self.w, self.nk, self.nb, self.n_e, self.kernel.T,
                          mu0_from_state=True, maxiter=maxiter, st=self.st, p_out=self.occ)
 
-    def _eig_warm(self, A):
-        """(eps, U) <- eigh(A) from the current basis U: U^H A U = W eps W^H, U <- U W."""
-        K.matmul(self.U, A, self.Ft, C=self.U, ops=0b001, herm_out=True)
-        K.eigh(self.Ft, self.eps, self.W, herm_in=True)
-        K.matmul(self.U, self.W, self.tmp)
-        K.lincomb(self.U, self.tmp, 1.0)
+    def _extrapolate(self, Xs, cs, warm):
+        """herm(sum c X): in the basis U (Ft, warm) or as is (Fx)."""
+        if warm:
+            K.extrap_rotate(self.Ft, self.U, Xs, cs)
+        else:
+            K.lincomb_multi(self.Fx, Xs, cs, herm=True)
 
     def _update(self, warm=True):
-        """Fx (Hermitian) -> P_new: launches only (the warm form is captured once)."""
+        """-> P_new: launches only (the warm form is captured once).  Warm: eigh of
+        Ft = U^H F U (eps, W), mu, then U <- U W and P_new from (U, occ) in one pass."""
         if warm:
-            self._eig_warm(self.Fx)
+            K.eigh(self.Ft, self.eps, self.W, herm_in=True)
+            self._mu(self.MU_GRAPH_STEPS)
+            K.rotate_density(self.U, self.W, self.occ, self.P_new)
         else:
             K.eigh(self.Fx, self.eps, self.U, herm_in=True)
-        self._mu(self.MU_GRAPH_STEPS)
-        K.density(self.U, self.tmp, p=self.occ)
-        K.hermitize(self.tmp, self.P_new)
+            self._mu(self.MU_GRAPH_STEPS)
+            K.density(self.U, self.tmp, p=self.occ)
+            K.hermitize(self.tmp, self.P_new)
 
     def _capture(self):
         return capture_graphs(self.dev, self._update)[0]
@@ -502,10 +505,9 @@ class _DIISPolish:
             coeffs, ok = diis_solve(B * np.outer(active, active), active)
             if count >= 2 and ok:
                 sel = [i for i in range(nh) if active[i] and coeffs[i] != 0.0]
-                K.lincomb_multi(self.Fx, [self.F_hist[i] for i in sel], [float(coeffs[i]) for i in sel],
-                                herm=True)
+                self._extrapolate([self.F_hist[i] for i in sel], [float(coeffs[i]) for i in sel], k > 0)
             else:
-                K.lincomb_multi(self.Fx, [Fk], [1.0], herm=True)
+                self._extrapolate([Fk], [1.0], k > 0)
             if k == 0:
                 self._update(warm=False)
             elif self.graph is None:
@@ -524,13 +526,16 @@ class _DIISPolish:
             E = float(outs[2].item())
         # orbitals / occupations / mu of the returned state (solver.py:606-638 form)
         if k > 0:
-            self._eig_warm(Ffin)
+            K.extrap_rotate(self.Ft, self.U, [Ffin], [1.0])
+            K.eigh(self.Ft, self.eps, self.W, herm_in=True)
         else:
             K.eigh(Ffin, self.eps, self.U, herm_in=True)
-        Q = self.U.clone()
         self._mu(self.MU_GRAPH_STEPS)
         if float(self.st[K.ST_MUDONE].item()) == 0.0:
             self._mu()
+        if k > 0:
+            K.rotate_density(self.U, self.W, self.occ, self.tmp)
+        Q = self.U.clone()
         return (self.P_cur.clone(), Ffin.clone(), Q, self.occ.clone(), self.st[K.ST_MU].clone(), E, n_it,
                 converged, hE, hC)
 

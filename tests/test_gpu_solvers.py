@@ -403,3 +403,33 @@ def test_repeated_solves_reuse_cached_state(dev):
     finally:
         if was:
             gc.enable()
+
+
+@pytest.mark.parametrize("dt", ["c128", "c64"])
+@pytest.mark.parametrize("nb", [3, 16])
+def test_extrap_rotate_and_rotate_density(dev, dt, nb):
+    """The fused per-k kernels of the DIIS polish against NumPy: Ft =
+    U^H herm(sum_v c_v X_v) U, and U <- U W with P = herm(U diag(occ) U^H)."""
+    import torch
+    from jax_hf_amd import _ops as K
+    rng = np.random.default_rng(nb)
+    nk = 37
+    cdt = torch.complex128 if dt == "c128" else torch.complex64
+    tol = 1e-13 if dt == "c128" else 2e-5
+    Xs = [rng.standard_normal((nk, nb, nb)) + 1j * rng.standard_normal((nk, nb, nb)) for _ in range(5)]
+    cs = rng.standard_normal(5)
+    U = np.linalg.qr(rng.standard_normal((nk, nb, nb)) + 1j * rng.standard_normal((nk, nb, nb)))[0]
+    W = np.linalg.qr(rng.standard_normal((nk, nb, nb)) + 1j * rng.standard_normal((nk, nb, nb)))[0]
+    occ = rng.uniform(0, 1, (nk, nb))
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev).to(cdt)   # noqa: E731
+    Ut, Ft = t(U), torch.empty((nk, nb, nb), dtype=cdt, device=dev)
+    K.extrap_rotate(Ft, Ut, [t(x) for x in Xs], cs)
+    S = sum(c * x for c, x in zip(cs, Xs))
+    ref = np.conj(np.swapaxes(U, -1, -2)) @ _herm(S) @ U
+    assert np.abs(_np(Ft) - ref).max() <= tol * np.abs(ref).max()
+    P = torch.empty_like(Ft)
+    K.rotate_density(Ut, t(W), torch.from_numpy(occ).to(dev), P)
+    Un = U @ W
+    assert np.abs(_np(Ut) - Un).max() <= tol * 10
+    Pr = np.einsum("kin,kn,kjn->kij", Un, occ, np.conj(Un))
+    assert np.abs(_np(P) - Pr).max() <= tol * 10
