@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--fock-only", action="store_true", help="only the Fock-build measurement")
     ap.add_argument("--no-scan", action="store_true", help="skip the 64-solve density scan")
     ap.add_argument("--scan-solves", type=int, default=64)
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the Newton (64x64, nb=8) and adjoint (48x48, nb=16) sections")
     ap.add_argument("--extra", action="store_true", help="also time the packed and complex64 Fock builds")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of the exchange kernel's byte count "
@@ -124,7 +126,7 @@ def fock_bench(J, L, syn, inp, dev, nk, nb, args):
             evs[1].record()
         if planar:
             L.call("hfx_fock_epilogue_planar", h, L.ptr(eng.h), L.ptr(St), None, 0, 0, None, None, L.ptr(P),
-                   L.ptr(eng.w2d), L.ptr(F), None, L.ptr(eng.plan.scratch), L.ptr(E), 0.0, st)
+                   L.ptr(eng.w2d), L.ptr(F), L.ptr(Sig), L.ptr(eng.plan.scratch), L.ptr(E), 0.0, st)
         else:
             L.call("hfx_fock_epilogue", h, L.ptr(eng.h), L.ptr(Sig), None, None, L.ptr(P),
                    L.ptr(eng.w2d), L.ptr(F), None, L.ptr(eng.plan.scratch), L.ptr(E), 0.0, st)
@@ -179,6 +181,92 @@ def extra_forms(J, L, syn, inp, dev, nk, nb, args):
 # ---------------------------------------------------------------------------
 # solves
 # ---------------------------------------------------------------------------
+
+def _median_time(fn, n):
+    import torch
+    ts, r = [], None
+    for _ in range(n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    return r, float(np.median(ts))
+
+
+def newton_bench(J, syn, dev, args):
+    """Precision form at nk = 64x64, nb = 8 (layer = band mod 4, T = 0.05, quarter
+    filling + 0.1): trust-region Newton to tol_grad 1e-6 in FP64 and mixed
+    (complex64 Hessian-vector Fock builds), and solve_scf with DIIS; |dE|
+    against the FP64 fixed point (DIIS SCF to 1e-10 residuals)."""
+    import torch
+    c = syn.config2()
+    G = J.HartreeFockKernel(c["weights"], c["hamiltonian"], c["coulomb_q"], c["T"], device=dev)
+    P0 = torch.zeros(c["hamiltonian"].shape, dtype=torch.complex128, device=dev)
+    ne = c["n_electrons"]
+    ref = J.solve_scf(G, P0, ne, config=J.SCFConfig(max_iter=400, acceleration="diis",
+                                                    density_tol=1e-10, comm_tol=1e-10))
+    E_fp = float(ref.energy)
+    forms = {
+        "newton_fp64": lambda: J.solve(G, P0, ne, config=J.SolverConfig(optimizer="newton", tol_grad=1e-6,
+                                                                        precision="fp64", max_iter=200)),
+        "newton_mixed": lambda: J.solve(G, P0, ne, config=J.SolverConfig(optimizer="newton", tol_grad=1e-6,
+                                                                         precision="mixed", max_iter=200)),
+        "scf_diis": lambda: J.solve_scf(G, P0, ne, config=J.SCFConfig(acceleration="diis")),
+    }
+    out = {"fixed_point_energy": E_fp, "fixed_point_converged": bool(ref.converged)}
+    for name, fn in forms.items():
+        fn()                                                         # first use
+        r, t = _median_time(fn, 2)
+        it = int(getattr(r, "n_iter", getattr(r, "iterations", 0)))
+        out[name] = {"seconds_median": t, "iterations": it, "converged": bool(r.converged),
+                     "abs_dE_vs_fp64_fixed_point": abs(float(r.energy) - E_fp)}
+    out["speedup_newton_mixed_vs_fp64"] = out["newton_fp64"]["seconds_median"] / out["newton_mixed"]["seconds_median"]
+    out["note"] = ("the trust-region Newton stalls above tol_grad 1e-6 on this input (gradient ~5e-5, trust "
+                   "region shrinking), as the oracle restatement of the reference does at 16x16 with the same "
+                   "iterates; its time is for the max_iter budget")
+    return out
+
+
+def adjoint_bench(J, syn, dev, args):
+    """Differentiation form at nk = 48x48, nb = 16 (seed 4): converge an FP64
+    solve, then the free-energy gradients w.r.t. U0, q0 (through V(q)), the
+    displacement u and h(k) from the hand-written adjoint at the stationary
+    point; backward time (the adjoint of build_fock + hf_energy) against one
+    forward Fock build."""
+    import torch
+    from jax_hf_amd.adjoint import HFEnergy, free_energy_gradients
+    nk, nb = 48, 16
+    c = syn.config1(nk=nk, nb=nb, seed=4)
+    U0, q0, d_ = 5.0, 0.3, 0.5
+    q = syn.qnorm_min_image(nk, nk)[..., None, None]
+    V = c["coulomb_q"]
+    G = J.HartreeFockKernel(c["weights"], c["hamiltonian"], V, c["T"], device=dev)
+    P0 = torch.zeros(c["hamiltonian"].shape, dtype=torch.complex128, device=dev)
+    r = J.solve_scf(G, P0, nb / 2.0, config=J.SCFConfig(max_iter=300, acceleration="diis",
+                                                       density_tol=1e-10, comm_tol=1e-10))
+    P = r.density_matrix
+    eng = G.fock_engine(herm=False)
+    F = torch.empty_like(P)
+    _, t_fwd = _median_time(lambda: [eng.build(P, F, energy=True) for _ in range(20)], 3)
+    spec = dict(w2d=G._w64, refP=G._refP, HH=None)
+    h = G.h.clone().requires_grad_(True)
+    VR = G._VR_shifted.clone().requires_grad_(True)
+
+    def backward():
+        E = HFEnergy.apply(P, h, VR, spec)
+        return torch.autograd.grad(E, (h, VR))
+    backward()
+    _, t_fb = _median_time(lambda: [backward() for _ in range(10)], 3)
+    D = syn.displacement(nb, 1.0)
+    grads = free_energy_gradients(G, P, dVq={"U0": V / U0, "q0": -V / (q + q0)}, dh={"u": D})
+    return {"scf_iterations": int(r.iterations), "converged": bool(r.converged),
+            "forward_fock_build_ms": t_fwd / 20 * 1e3,
+            "forward_plus_backward_ms": t_fb / 10 * 1e3,
+            "backward_over_forward": (t_fb / 10) / (t_fwd / 20) - 1.0,
+            "dOmega_dU0": grads["U0"], "dOmega_dq0": grads["q0"], "dOmega_du": grads["u"],
+            "grad_h_norm": float(torch.linalg.vector_norm(grads["h"]).item())}
+
 
 SOLVE_T = 0.1
 POLISH_TOL = 1e-6
@@ -371,12 +459,15 @@ def main():
         out.update({"value": mixed["solves_per_s"], "unit": "solves/s", "steps": args.steps,
                     "warmup": args.warmup, "ms_per_step": mixed["seconds_median"] * 1e3,
                     "config": {"workload": (f"converged CG solve, mixed precision (FP32 CG + FP64 DIIS "
-                                            f"polish to 1e-8), nk={nk}x{nk} nb={nb}, layer-resolved V, "
+                                            f"polish to {POLISH_TOL:g}), nk={nk}x{nk} nb={nb}, layer-resolved V, "
                                             f"T={SOLVE_T}, half filling, P0=0"),
                                "nk": nk, "nb": nb, "parallelism": "1 GPU"},
                     "solves": solves})
         if not args.no_scan:
             out["scan"] = scan_bench(J, syn, dev, args)
+        if not args.no_forms:
+            out["newton_64x64_nb8"] = newton_bench(J, syn, dev, args)
+            out["adjoint_48x48_nb16"] = adjoint_bench(J, syn, dev, args)
     out["fock"] = fock
     out["fock_build_GBps"] = fock["GBps"]
     out["roofline"] = roof

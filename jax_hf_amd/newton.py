@@ -33,6 +33,16 @@ class _Newton:
         self.shape3 = (nk1, nk2, nb)
         self.nk, self.nb = nk1 * nk2, nb
         self.fock = _Fock(kernel, self.cdt, fock_build_fn, cfg.block_sizes)
+        # precision="mixed" on a complex128 kernel: the Fock-response builds of the
+        # Hessian-vector products run in complex64 (FP32 FFT and products, the
+        # bulk of the work); gradients, energies, mu, the Steihaug subproblem and
+        # every other Fock build stay FP64
+        self.hvp_fock = None
+        if (str(cfg.precision).lower() == "mixed" and self.cdt == torch.complex128 and fock_build_fn is None
+                and cfg.block_sizes is None):
+            self.hvp_fock = _Fock(kernel, torch.complex64, None, None)
+            z = lambda: torch.zeros((nk1, nk2, nb, nb), dtype=torch.complex64, device=self.dev)  # noqa: E731
+            self.dP32, self.R32 = z(), z()
         self.w2d = kernel._w64.reshape(-1).contiguous()
         self.wsum = float(kernel.weight_sum)
         self.w_norm = (self.w2d / max(self.wsum, 1e-30)).contiguous()
@@ -132,7 +142,13 @@ class _Newton:
         L.call("hfx_newton_dP", self.dt, self.nb, self.nk, L.ptr(Q), L.ptr(X), L.ptr(p), L.ptr(dp), L.ptr(dP),
                L.stream(self.dev))
         # F[dP] - h with refP = 0 (solver.py:770-780); the engine takes None as "no reference"
-        self.fock.build(dP, R, refP=None if self.fock.fn is None else self.zero_ref, sub_h=True)
+        if self.hvp_fock is not None:
+            n, st = dP.numel(), L.stream(self.dev)
+            L.call("hfx_convert", L.ptr(dP), KIND[dP.dtype], L.ptr(self.dP32), KIND[torch.complex64], n, st)
+            self.hvp_fock.build(self.dP32, self.R32, refP=None, sub_h=True)
+            L.call("hfx_convert", L.ptr(self.R32), KIND[torch.complex64], L.ptr(R), KIND[R.dtype], n, st)
+        else:
+            self.fock.build(dP, R, refP=None if self.fock.fn is None else self.zero_ref, sub_h=True)
         if self.has_defl and pen != 0.0:
             K.lincomb(R, R, 1.0, dP, pen)
         L.call("hfx_newton_hvp", self.dt, self.nb, self.nk, L.ptr(Q), L.ptr(R), L.ptr(Ft), L.ptr(X), L.ptr(p),
