@@ -277,6 +277,10 @@ int hfx_matmul_batched(int dtype, int nb, size_t nk, const void* A, const void* 
  * basis U, ready for an eigh that starts from the previous orbitals. */
 int hfx_extrap_rotate(int dtype, int nb, size_t nk, int nin, const void* const* X, const double* coef, const void* U,
                       void* Ft, void* stream);
+/* hfx_extrap_rotate over a stack of solves of seg_len k-points each, with
+ * per-solve coefficients on the device: coef_dev[s * 8 + v] for solve s. */
+int hfx_extrap_rotate_batched(int dtype, int nb, size_t nk, int seg_len, int nin, const void* const* X,
+                              const double* coef_dev, const void* U, void* Ft, void* stream);
 /* U <- U W (in place) and P = herm(U diag(occ) U^dagger) per k with the new U:
  * eigenvectors back from the rotated basis and the density of solver.py:42-44. */
 int hfx_rotate_density(int dtype, int nb, size_t nk, void* U, const void* W, const double* occ, void* P,

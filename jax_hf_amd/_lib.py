@@ -77,6 +77,7 @@ _SIGS = {
     "hfx_rotate_eigh": [i32, i32, sz, vp, vp, vp, vp],
     "hfx_matmul_batched": [i32, i32, sz, vp, vp, vp, i32, vp, i32, vp, vp],
     "hfx_extrap_rotate": [i32, i32, sz, i32, vp, vp, vp, vp, vp],
+    "hfx_extrap_rotate_batched": [i32, i32, sz, i32, i32, vp, vp, vp, vp, vp],
     "hfx_rotate_density": [i32, i32, sz, vp, vp, vp, vp, vp],
     "hfx_scf_resid": [i32, i32, sz, vp, vp, vp, vp, vp, vp],
     "hfx_batched_dots": [i32, i32, sz, vp, vp, i32, i32, vp, vp],

@@ -36,7 +36,8 @@ from . import _lib as L
 from . import _ops as K
 from . import fock as fock_mod
 from ._util import KIND, capture_graphs, convert, to_tensor
-from .solver import SolveResult, SolverConfig, _polish_diis, _precision
+from .reference_scf import diis_solve
+from .solver import SolveResult, SolverConfig, _precision
 
 __all__ = ["solve_batch"]
 
@@ -55,6 +56,55 @@ def _group_key(kernel):
             tuple(kernel._VR_shifted.shape))
 
 
+class _BatchFock:
+    """Exchange-only Fock build + energy of S stacked solves sharing mesh and VR
+    (fock.py:76-111 per solve); the energies go to st[s][HFX_ST_E]."""
+
+    def __init__(self, kernels, cdt):
+        k0 = kernels[0]
+        self.dev = dev = k0.device
+        nk1, nk2, nb = k0.h.shape[0], k0.h.shape[1], k0.h.shape[-1]
+        self.S = S = len(kernels)
+        self.plan = L.plan_for(nk1, nk2, nb, cdt, dev)
+        self.herm = bool(k0._v_real_symmetric)
+        VR = k0._VR_shifted
+        self.VR = VR if VR.dtype == cdt else convert(VR, cdt)
+        self.vfull = 0 if VR.shape[-1] == 1 else 1
+        self.h = torch.stack([k.h if k.h.dtype == cdt else convert(k.h, cdt) for k in kernels]).contiguous()
+        self.w_raw = torch.stack([k._w64 for k in kernels]).contiguous()
+        c = dict(dtype=cdt, device=dev)
+        self.planar = fock_mod.PLANAR
+        if self.planar:
+            nch = nb * (nb + 1) // 2 if self.herm else nb * nb
+            self.Pt = torch.empty((S, nch, nk1, nk2), **c)
+            self.St = torch.empty((S, nch, nk1, nk2), **c)
+            if self.vfull:
+                self.VRc = torch.empty((nb * nb, nk1, nk2), **c)
+                L.call("hfx_planar_pack", self.plan.handle, 1, L.ptr(self.VR.contiguous()), None, None, 0, 0,
+                       L.ptr(self.VRc), L.stream(dev))
+            else:
+                self.VRc = self.VR.contiguous()
+        else:
+            self.Sig = torch.zeros((S, nk1, nk2, nb, nb), **c)
+        self.epi = torch.zeros(256 * S, dtype=torch.float64, device=dev)   # energy partials, 256 per solve
+
+    def __call__(self, P, F, st):
+        sm = L.stream(self.dev)
+        h = self.plan.handle
+        if self.planar:
+            herm = int(self.herm)
+            L.call("hfx_planar_pack", h, self.S, L.ptr(P), None, None, 0, herm, L.ptr(self.Pt), sm)
+            L.call("hfx_selfenergy_planar", h, self.S, L.ptr(self.Pt), L.ptr(self.VRc), self.vfull, None, 0, herm, 0,
+                   None, L.ptr(self.St), None, sm)
+            L.call("hfx_fock_epilogue_planar_batched", h, self.S, L.ptr(self.h), L.ptr(self.St), herm, L.ptr(P),
+                   L.ptr(self.w_raw), L.ptr(F), L.ptr(st), L.ptr(self.epi), sm)
+            return
+        L.call("hfx_selfenergy_batched", h, self.S, L.ptr(P), L.ptr(self.VR), self.vfull,
+               int(self.herm), L.ptr(self.Sig), sm)
+        L.call("hfx_fock_epilogue_batched", h, self.S, L.ptr(self.h), L.ptr(self.Sig), L.ptr(P),
+               L.ptr(self.w_raw), L.ptr(F), L.ptr(st), L.ptr(self.epi), sm)
+
+
 class _BatchCG:
     """Stacked CG state of S solves on one (nk1, nk2, nb) mesh."""
 
@@ -69,13 +119,7 @@ class _BatchCG:
         self.Nk, self.nb = nk1 * nk2, nb
         self.nk_tot = S * self.Nk
         self.shape4 = (S, nk1, nk2, nb, nb)
-        self.plan = L.plan_for(nk1, nk2, nb, cdt, dev)
-        self.herm = bool(k0._v_real_symmetric)
-        VR = k0._VR_shifted
-        self.VR = VR if VR.dtype == cdt else convert(VR, cdt)
-        self.vfull = 0 if VR.shape[-1] == 1 else 1
-        self.h = torch.stack([k.h if k.h.dtype == cdt else convert(k.h, cdt) for k in kernels]).contiguous()
-        self.w_raw = torch.stack([k._w64 for k in kernels]).contiguous()
+        self.fk = _BatchFock(kernels, cdt)
         self.w_norm = torch.stack([k._w64 / max(k.weight_sum, 1e-30) for k in kernels]).contiguous()
         self.T = max(float(k0.T), 1e-12)
         c = dict(dtype=cdt, device=dev)
@@ -83,25 +127,11 @@ class _BatchCG:
         z = lambda: torch.zeros(self.shape4, **c)  # noqa: E731
         self.Q, self.P, self.F, self.Ft, self.G, self.Gp = z(), z(), z(), z(), z(), z()
         self.H, self.D, self.V = z(), z(), z()
-        self.planar = fock_mod.PLANAR
-        if self.planar:
-            nch = nb * (nb + 1) // 2 if self.herm else nb * nb
-            self.Pt = torch.empty((S, nch, nk1, nk2), **c)
-            self.St = torch.empty((S, nch, nk1, nk2), **c)
-            if self.vfull:
-                self.VRc = torch.empty((nb * nb, nk1, nk2), **c)
-                L.call("hfx_planar_pack", self.plan.handle, 1, L.ptr(self.VR.contiguous()), None, None, 0, 0,
-                       L.ptr(self.VRc), L.stream(dev))
-            else:
-                self.VRc = self.VR.contiguous()
-        else:
-            self.Sig = z()
         v = lambda: torch.zeros((S, nk1, nk2, nb), **r)  # noqa: E731
         self.p, self.eps, self.gp, self.gpp, self.hp, self.dp, self.lam = v(), v(), v(), v(), v(), v(), v()
         self.st = torch.zeros((S, K.ST_LEN), **r)
         self.part = torch.zeros((8, self.nk_tot), **r)
         self.scratch = torch.zeros(int(L.lib().hfx_batch_scratch_doubles(S)), **r)
-        self.epi = torch.zeros(256 * S, **r)   # energy partials, 256 per solve
         self.targets = torch.zeros(S, **r)
 
     # -- pieces ---------------------------------------------------------------
@@ -114,20 +144,7 @@ class _BatchCG:
         return self.P
 
     def fock(self, P, F):
-        sm = L.stream(self.dev)
-        h = self.plan.handle
-        if self.planar:
-            herm = int(self.herm)
-            L.call("hfx_planar_pack", h, self.S, L.ptr(P), None, None, 0, herm, L.ptr(self.Pt), sm)
-            L.call("hfx_selfenergy_planar", h, self.S, L.ptr(self.Pt), L.ptr(self.VRc), self.vfull, None, 0, herm, 0,
-                   None, L.ptr(self.St), None, sm)
-            L.call("hfx_fock_epilogue_planar_batched", h, self.S, L.ptr(self.h), L.ptr(self.St), herm, L.ptr(P),
-                   L.ptr(self.w_raw), L.ptr(F), L.ptr(self.st), L.ptr(self.epi), sm)
-            return
-        L.call("hfx_selfenergy_batched", h, self.S, L.ptr(P), L.ptr(self.VR), self.vfull,
-               int(self.herm), L.ptr(self.Sig), sm)
-        L.call("hfx_fock_epilogue_batched", h, self.S, L.ptr(self.h), L.ptr(self.Sig), L.ptr(P),
-               L.ptr(self.w_raw), L.ptr(F), L.ptr(self.st), L.ptr(self.epi), sm)
+        self.fk(P, F, self.st)
 
     def mu(self):
         L.call("hfx_solve_mu_batched", L.ptr(self.eps), L.ptr(self.w_norm), L.ptr(self.targets), self.nk_tot,
@@ -272,6 +289,138 @@ class _BatchCG:
         return Pout, F
 
 
+# state-vector slots the batched polish reduces into (unused by the CG layout)
+_SL_RES = 20    # [20] sum w |FP - PF|^2, [21] sum w |P - P_prev|^2
+_SL_B = 22      # [22 ...] the new row of the DIIS B matrix
+
+
+class _BatchPolish:
+    """FP64 C-DIIS polish of S stacked solves: the batched form of the mixed
+    solve's polish (one eigh per iteration, extrapolation in the previous
+    basis, reference_scf.py:292-314 per solve).  Each solve keeps its own B
+    matrix, coefficients (passed to the extrapolation kernel per solve) and
+    stopping test; a converged solve's state is copied out and it continues
+    with plain steps at its fixed point while the others finish."""
+
+    def __init__(self, kernels, n_electrons, nh=6):
+        k0 = kernels[0]
+        cdt = torch.complex128
+        self.kernels = kernels
+        self.dev = dev = k0.device
+        nk1, nk2, nb = k0.h.shape[0], k0.h.shape[1], k0.h.shape[-1]
+        self.S = S = len(kernels)
+        self.Nk, self.nb, self.nk_tot = nk1 * nk2, nb, S * nk1 * nk2
+        self.shape4 = (S, nk1, nk2, nb, nb)
+        self.fk = _BatchFock(kernels, cdt)
+        self.nh = nh
+        c = dict(dtype=cdt, device=dev)
+        r = dict(dtype=torch.float64, device=dev)
+        z = lambda: torch.zeros(self.shape4, **c)  # noqa: E731
+        self.P_prev, self.P_cur, self.P_new, self.Ft, self.W, self.U, self.tmp = z(), z(), z(), z(), z(), z(), z()
+        self.Pfin, self.Ffin = z(), z()
+        self.F_hist = [z() for _ in range(nh)]
+        self.R_hist = [z() for _ in range(nh)]
+        self.eps = torch.zeros((S, nk1, nk2, nb), **r)
+        self.occ = torch.zeros_like(self.eps)
+        self.st = torch.zeros((S, K.ST_LEN), **r)
+        self.part = torch.zeros((8, self.nk_tot), **r)
+        self.scratch = torch.zeros(int(L.lib().hfx_batch_scratch_doubles(S)), **r)
+        self.w_norm = torch.stack([k._w64 / max(k.weight_sum, 1e-30) for k in kernels]).contiguous()
+        self.targets = torch.tensor([float(n) / max(k.weight_sum, 1e-30) for n, k in zip(n_electrons, kernels)],
+                                    dtype=torch.float64, device=dev)
+        self.T = max(float(k0.T), 1e-12)
+        self.coef = torch.zeros((S, 8), **r)
+
+    def _reduce(self, nval, slot):
+        L.call("hfx_reduce_batched", L.ptr(self.part), int(nval), self.nk_tot, self.Nk, L.ptr(self.w_norm),
+               (1 << nval) - 1, L.ptr(self.st), K.OP_SUM, None, int(slot), L.ptr(self.scratch), L.stream(self.dev))
+
+    def _mu(self, maxiter=54):
+        L.call("hfx_solve_mu_batched", L.ptr(self.eps), L.ptr(self.w_norm), L.ptr(self.targets), self.nk_tot,
+               self.Nk, self.nb, self.T, int(maxiter), L.ptr(self.st), L.ptr(self.occ), L.ptr(self.scratch),
+               L.stream(self.dev))
+
+    def run(self, P_lo, mu0, tol, max_iter):
+        import ctypes as C
+        S, nh = self.S, self.nh
+        K.hermitize(convert(P_lo, torch.complex128) if P_lo.dtype != torch.complex128 else P_lo, self.P_cur)
+        self.st.zero_()
+        self.st[:, K.ST_MU] = mu0
+        B = np.zeros((S, nh, nh))
+        done = [False] * S
+        hE = [[] for _ in range(S)]
+        hC = [[] for _ in range(S)]
+        E_out = [float("nan")] * S
+        xa = (C.c_void_p * nh)(*[f.data_ptr() for f in self.F_hist])
+        dt = K.dt_code(self.U)
+        k = 0
+        while k < max_iter:
+            idx = k % nh
+            Fk, Rk = self.F_hist[idx], self.R_hist[idx]
+            self.fk(self.P_cur, Fk, self.st)
+            K.scf_resid(Fk, self.P_cur, self.P_prev if k > 0 else None, Rk, self.part)
+            self._reduce(2, _SL_RES)
+            K.dots(Rk, self.R_hist, self.part, mode=0)                   # B row (reference_scf.py:110-111)
+            self._reduce(nh, _SL_B)
+            sv = self.st.cpu().numpy()                                    # one readback per iteration
+            coefs = np.zeros((S, 8))
+            for s in range(S):
+                coefs[s, idx] = 1.0                                       # plain step unless DIIS engages
+                if done[s]:
+                    continue
+                c_res = float(np.sqrt(max(sv[s, _SL_RES], 0.0)))
+                d_res = float(np.sqrt(max(sv[s, _SL_RES + 1], 0.0))) if k > 0 else float("inf")
+                E_out[s] = float(sv[s, K.ST_E])
+                hE[s].append(E_out[s])
+                hC[s].append(c_res)
+                if c_res <= tol and d_res <= tol:                        # this iterate is the result
+                    done[s] = True
+                    self.Pfin[s].copy_(self.P_cur[s])
+                    self.Ffin[s].copy_(Fk[s])
+                    continue
+                row = sv[s, _SL_B:_SL_B + nh]
+                B[s, idx, :] = row
+                B[s, :, idx] = row
+                count = min(k + 1, nh)
+                active = np.arange(nh) < count
+                cs, ok = diis_solve(B[s] * np.outer(active, active), active)
+                if count >= 2 and ok:
+                    coefs[s, :nh] = np.where(active, cs, 0.0)
+                    coefs[s, nh:] = 0.0
+            if all(done):
+                break
+            if k == 0:                    # cold first update: eigh of herm(F_0)
+                K.lincomb_multi(self.Ft, [Fk], [1.0], herm=True)
+                K.eigh(self.Ft, self.eps, self.U, herm_in=True)
+                self._mu()
+                K.density(self.U, self.tmp, p=self.occ)
+                K.hermitize(self.tmp, self.P_new)
+            else:                         # extrapolation in the previous basis, per-solve coefficients
+                self.coef.copy_(torch.from_numpy(coefs))
+                L.call("hfx_extrap_rotate_batched", dt, self.nb, self.nk_tot, self.Nk, nh, C.cast(xa, C.c_void_p),
+                       L.ptr(self.coef), L.ptr(self.U), L.ptr(self.Ft), L.stream(self.dev))
+                K.eigh(self.Ft, self.eps, self.W, herm_in=True)
+                self._mu()
+                K.rotate_density(self.U, self.W, self.occ, self.P_new)
+            self.P_prev, self.P_cur = self.P_cur, self.P_prev
+            K.lincomb(self.P_cur, self.P_new, 1.0)
+            k += 1
+        if not all(done):                 # budget spent: evaluate the returned densities
+            self.fk(self.P_cur, self.tmp, self.st)
+            sv = self.st.cpu().numpy()
+            for s in range(S):
+                if not done[s]:
+                    self.Pfin[s].copy_(self.P_cur[s])
+                    self.Ffin[s].copy_(self.tmp[s])
+                    E_out[s] = float(sv[s, K.ST_E])
+        # orbitals / occupations / mu of the returned states (solver.py:606-638 form)
+        K.extrap_rotate(self.Ft, self.U, [self.Ffin], [1.0])
+        K.eigh(self.Ft, self.eps, self.W, herm_in=True)
+        self._mu()
+        K.rotate_density(self.U, self.W, self.occ, self.tmp)
+        return E_out, done, hE, hC
+
+
 def solve_batch(kernels, P0s, n_electrons, *, config: SolverConfig | None = None):
     """Independent CG solves of a scan, stacked into shared launches.
 
@@ -319,9 +468,27 @@ def solve_batch(kernels, P0s, n_electrons, *, config: SolverConfig | None = None
         if prec == "mixed":
             k_of, grad, hE, hG = b.run(cfg.mixed_tol_E, 0.0, max(cfg.max_iter // 2, 1), rel_tol=True)
             P_lo = b.density()
+            mu0 = b.st[:, K.ST_MU].clone()
+            del b                                            # free the FP32 stack before the FP64 one
+            pol = _BatchPolish(ks, [n_electrons[i] for i in members])
+            E2, conv2, hE2, hC2 = pol.run(P_lo, mu0, cfg.polish_tol, cfg.polish_max_iter)
+            sv = pol.st.cpu()
             for j, i in enumerate(members):
-                results[i] = _polish_diis(kernels[i], None, n_electrons[i], cfg, None, k_of[j], hE[j], hG[j],
-                                          P_lo=P_lo[j], mu0=b.st[j, K.ST_MU])
+                n2 = len(hE2[j])
+                kk = k_of[j] + n2
+                hEj = list(hE[j]) + list(hE2[j])
+                hGj = list(hG[j]) + list(hC2[j])
+                m = max(cfg.max_iter, kk)
+                hEt = torch.zeros(m, dtype=torch.float64)
+                hGt = torch.zeros(m, dtype=torch.float64)
+                hEt[:kk] = torch.tensor(hEj[:kk], dtype=torch.float64)
+                hGt[:kk] = torch.tensor(hGj[:kk], dtype=torch.float64)
+                results[i] = SolveResult(Q=pol.U[j].clone(), p=pol.occ[j].clone(), mu=sv[j, K.ST_MU].to(pol.dev),
+                                         density=pol.Pfin[j].clone(), fock=pol.Ffin[j].clone(),
+                                         energy=torch.tensor(E2[j], dtype=torch.float64, device=pol.dev),
+                                         n_iter=int(kk), converged=bool(conv2[j]),
+                                         history=dict(E=hEt.to(pol.dev), grad_norm=hGt.to(pol.dev),
+                                                      phase_iters=torch.tensor([k_of[j], n2], dtype=torch.int64)))
             continue
         k_of, grad, hE, hG = b.run(cfg.tol_E, cfg.tol_grad, cfg.max_iter)
         Pfin, Ffin = b.finalize()

@@ -766,14 +766,21 @@ struct PkInputs {
   double c[PK_XMAX];
 };
 
+// coef_dev (optional): per-solve coefficients, coef_dev[s * PK_XMAX + v] for the
+// solve s = k / seg_len of a stack of solves (the batched polish).
 template <typename T>
 __global__ __launch_bounds__(64) void pk_extrap_rotate_kernel(int nb, int nin, PkInputs in,
+                                                              const double* __restrict__ coef_dev, int seg_len,
                                                               const cx<T>* __restrict__ U, cx<T>* __restrict__ Ft) {
   __shared__ cx<T> sF[PK_MAT], sU[PK_MAT], sX[PK_MAT];
+  __shared__ double sc[PK_XMAX];
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  if ((int)threadIdx.x < nin)
+    sc[threadIdx.x] = coef_dev ? coef_dev[(k / (size_t)seg_len) * PK_XMAX + threadIdx.x] : in.c[threadIdx.x];
+  __syncthreads();
   PK_FOR_ELEM(nb, {
     cx<T> a{0, 0};
-    for (int v = 0; v < nin; ++v) a = a + (T)in.c[v] * reinterpret_cast<const cx<T>*>(in.x[v])[k * nb2 + e];
+    for (int v = 0; v < nin; ++v) a = a + (T)sc[v] * reinterpret_cast<const cx<T>*>(in.x[v])[k * nb2 + e];
     sX[i * PK_PITCH + j] = a;
   })
   m_load(sU, U + k * nb2, nb);
@@ -1169,26 +1176,38 @@ int hfx_matmul_batched(int dtype, int nb, size_t nk, const void* A, const void* 
   return check_launch("pk_mm_kernel");
 }
 
-int hfx_extrap_rotate(int dtype, int nb, size_t nk, int nin, const void* const* X, const double* coef, const void* U,
-                      void* Ft, void* stream) {
+static int extrap_rotate_launch(int dtype, int nb, size_t nk, int nin, const void* const* X, const double* coef,
+                                const double* coef_dev, int seg_len, const void* U, void* Ft, void* stream) {
   if (int rc = pk_check_nb(nb)) return rc;
-  if (!X || !coef || !U || !Ft || nin < 1 || nin > PK_XMAX)
+  if (!X || (!coef && !coef_dev) || !U || !Ft || nin < 1 || nin > PK_XMAX || (coef_dev && seg_len < 1))
     return set_error(HFX_EINVAL, "hfx_extrap_rotate: bad argument (nin=%d)", nin);
   PkInputs in{};
   for (int v = 0; v < nin; ++v) {
     if (!X[v]) return set_error(HFX_EINVAL, "hfx_extrap_rotate: input %d is NULL", v);
     in.x[v] = X[v];
-    in.c[v] = coef[v];
+    in.c[v] = coef ? coef[v] : 0.0;
   }
   if (nk == 0) return HFX_OK;
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_extrap_rotate_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nin, in, PKC(double, U),
-                       PKM(double, Ft));
+    hipLaunchKernelGGL(pk_extrap_rotate_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nin, in, coef_dev, seg_len,
+                       PKC(double, U), PKM(double, Ft));
   else
-    hipLaunchKernelGGL(pk_extrap_rotate_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nin, in, PKC(float, U),
-                       PKM(float, Ft));
+    hipLaunchKernelGGL(pk_extrap_rotate_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nin, in, coef_dev, seg_len,
+                       PKC(float, U), PKM(float, Ft));
   return check_launch("pk_extrap_rotate_kernel");
+}
+
+int hfx_extrap_rotate(int dtype, int nb, size_t nk, int nin, const void* const* X, const double* coef, const void* U,
+                      void* Ft, void* stream) {
+  return extrap_rotate_launch(dtype, nb, nk, nin, X, coef, nullptr, 1, U, Ft, stream);
+}
+
+int hfx_extrap_rotate_batched(int dtype, int nb, size_t nk, int seg_len, int nin, const void* const* X,
+                              const double* coef_dev, const void* U, void* Ft, void* stream) {
+  if (!coef_dev || seg_len < 1 || nk % (size_t)seg_len)
+    return set_error(HFX_EINVAL, "hfx_extrap_rotate_batched: bad argument");
+  return extrap_rotate_launch(dtype, nb, nk, nin, X, nullptr, coef_dev, seg_len, U, Ft, stream);
 }
 
 int hfx_rotate_density(int dtype, int nb, size_t nk, void* U, const void* W, const double* occ, void* P,
