@@ -42,10 +42,80 @@ __device__ __forceinline__ void m_store(cx<T>* __restrict__ g, const cx<T>* s, i
   PK_FOR_ELEM(nb, { g[e] = s[i * PK_PITCH + j]; })
 }
 
+// ---- 16 x 16 complex products on the matrix cores ---------------------------
+// One wave computes C = op(A) op(B) with the 16x16x4 MFMA of its precision
+// (v_mfma_f64_16x16x4_f64 / v_mfma_f32_16x16x4_f32): K = 16 in four slices,
+// four real products per slice (re*re - im*im, re*im + im*re).  Operand lanes:
+// A[i = l % 16][k = 4 s + l / 16], B[k = 4 s + l / 16][j = l % 16].  The
+// accumulator element r of lane l is C[row][l % 16] with row = l / 16 + 4 r
+// (f64) or 4 (l / 16) + r (f32) -- the two instructions lay rows out
+// differently.  Each operand element is read from LDS once (the scalar form
+// reads it nb times).
+typedef double pk_v4d __attribute__((ext_vector_type(4)));
+typedef float pk_v4f __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef pk_v4d v4;
+  static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int l, int r) { return (l >> 4) + 4 * r; }
+};
+template <> struct Mfma<float> {
+  typedef pk_v4f v4;
+  static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+
+template <typename T>
+struct MAcc {
+  typename Mfma<T>::v4 re, im;
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) { re[r] = (T)0; im[r] = (T)0; }
+  }
+};
+
+// acc += sgn * op(A) op(B) for 16 x 16 LDS matrices (pitch PK_PITCH); all 64 lanes.
+template <typename T, bool AH, bool BH>
+__device__ __forceinline__ void mf_gemm16(MAcc<T>& acc, const cx<T>* A, const cx<T>* B, T sgn = (T)1) {
+  const int l = threadIdx.x & 63, x = l & 15, kq = l >> 4;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int kk = 4 * s + kq;
+    cx<T> a = AH ? conj(A[kk * PK_PITCH + x]) : A[x * PK_PITCH + kk];
+    const cx<T> b = BH ? conj(B[x * PK_PITCH + kk]) : B[kk * PK_PITCH + x];
+    a.re *= sgn;
+    a.im *= sgn;
+    acc.re = Mfma<T>::mma(a.re, b.re, acc.re);
+    acc.re = Mfma<T>::mma(-a.im, b.im, acc.re);
+    acc.im = Mfma<T>::mma(a.re, b.im, acc.im);
+    acc.im = Mfma<T>::mma(a.im, b.re, acc.im);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void mf_store16(cx<T>* C, const MAcc<T>& acc) {
+  const int l = threadIdx.x & 63;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) C[Mfma<T>::row(l, r) * PK_PITCH + (l & 15)] = cx<T>{acc.re[r], acc.im[r]};
+}
+
 // C = op(A) op(B); op = conjugate transpose when the flag is set.  C must not
 // alias A or B.  Ends with a barrier.
 template <typename T, bool AH, bool BH>
 __device__ __forceinline__ void m_gemm(cx<T>* C, const cx<T>* A, const cx<T>* B, int nb) {
+  if (nb == PK_NBMAX) {
+    MAcc<T> acc;
+    acc.zero();
+    mf_gemm16<T, AH, BH>(acc, A, B);
+    mf_store16(C, acc);
+    __syncthreads();
+    return;
+  }
   PK_FOR_ELEM(nb, {
     cx<T> acc{0, 0};
     for (int l = 0; l < nb; ++l) {

@@ -72,6 +72,35 @@ def test_density_and_matmul(dev):
     assert np.abs(_np(out) - ref).max() < 1e-13
 
 
+@pytest.mark.parametrize("dt", ["c128", "c64"])
+@pytest.mark.parametrize("nb", [6, 16])
+def test_matmul_operand_forms(dev, nb, dt):
+    """Every op(A) op(B) [op(C)] form of the batched product against NumPy.
+    nb = 16 runs on the matrix cores (16x16x4 MFMA), nb = 6 on the LDS loop."""
+    import torch
+    from jax_hf_amd import _ops as K
+    rng = np.random.default_rng(nb)
+    cdt = np.complex128 if dt == "c128" else np.complex64
+    tdt = torch.complex128 if dt == "c128" else torch.complex64
+    tol = 1e-13 if dt == "c128" else 2e-5
+    shp = (5, 7, nb, nb)
+    A, B, C = [(rng.standard_normal(shp) + 1j * rng.standard_normal(shp)).astype(cdt) for _ in range(3)]
+    H = lambda X: np.conj(np.swapaxes(X, -1, -2))  # noqa: E731
+    At, Bt, Ct = (torch.from_numpy(X).to(dev) for X in (A, B, C))
+    out = torch.empty_like(At)
+    for ops in range(8):                     # bit 0: A^H, bit 1: B^H, bit 2: C^H
+        a = H(A) if ops & 1 else A
+        b = H(B) if ops & 2 else B
+        c = H(C) if ops & 4 else C
+        for use_c in (False, True):
+            K.matmul(At, Bt, out, C=Ct if use_c else None, ops=ops)
+            ref = a.astype(np.complex128) @ b
+            if use_c:
+                ref = ref @ c
+            scale = np.abs(ref).max()
+            assert np.abs(_np(out) - ref).max() <= tol * scale, (ops, use_c)
+
+
 @pytest.mark.parametrize("T", [0.5, 0.05, 0.005])
 def test_mu_solvers_match_oracle(dev, T):
     import torch
