@@ -286,10 +286,22 @@ int hfx_extrap_rotate_batched(int dtype, int nb, size_t nk, int seg_len, int nin
 int hfx_rotate_density(int dtype, int nb, size_t nk, void* U, const void* W, const double* occ, void* P,
                        void* stream);
 
+/* Newton-Schulz re-orthonormalisation of nk nb x nb bases in place, steps in
+ * [1, 4] times U <- U (3 I - U^dagger U) / 2 (each step squares the deviation
+ * from unitarity; the CG orbitals handed to the FP64 polish, solver.py:42-44). */
+int hfx_orthonormalize(int dtype, int nb, size_t nk, void* U, int steps, void* stream);
+
 /* SCF residuals (reference_scf.py:291-294): comm = F D - D F (stored if comm),
  * part[0][k] = sum |comm|^2, part[1][k] = sum |Pn - D|^2 (Pn may be NULL). */
 int hfx_scf_resid(int dtype, int nb, size_t nk, const void* F, const void* D, const void* Pn, void* comm,
                   double* part, void* stream);
+/* hfx_scf_resid plus, in the same pass, the new row of the DIIS B matrix
+ * (reference_scf.py:110-111): part[2 + v][k] = Re sum conj(comm) R_v for the
+ * nr <= 8 stored residuals Rs (a HOST array of device pointers; Rs[self] is
+ * the buffer comm is written to, its value is part[0][k]; self = -1: none).
+ * comm is required. */
+int hfx_scf_resid_dots(int dtype, int nb, size_t nk, const void* F, const void* D, const void* Pn, void* comm,
+                       const void* const* Rs, int nr, int self, double* part, void* stream);
 /* part[v][k] = Re sum conj(X_v) Y_v (mode 0, the inner products of solver.py:205-208 and the
  * DIIS B matrix reference_scf.py:110-111) or Re tr(X_v Y_v) (mode 1, reference_scf.py:319);
  * Xs, Ys are HOST arrays of nv <= 8 device pointers. */

@@ -109,6 +109,22 @@ def scf_resid(F, D, Pn, comm, part):
            s(F.device))
 
 
+def orthonormalize(U, steps=2):
+    """In place: U <- U (3 I - U^H U) / 2, `steps` times, per k."""
+    nb = U.shape[-1]
+    L.call("hfx_orthonormalize", dt_code(U), nb, U.numel() // (nb * nb), L.ptr(U), int(steps), s(U.device))
+
+
+def scf_resid_dots(F, D, Pn, comm, Rs, self_idx, part):
+    """scf_resid + the DIIS B row: part[2 + v][k] = <comm, Rs[v]> per k (Rs[self_idx] is comm)."""
+    import ctypes as C
+    nb = F.shape[-1]
+    nk = F.numel() // (nb * nb)
+    ra = (C.c_void_p * len(Rs))(*[r.data_ptr() for r in Rs])
+    L.call("hfx_scf_resid_dots", dt_code(F), nb, nk, L.ptr(F), L.ptr(D), L.ptr(Pn), L.ptr(comm),
+           C.cast(ra, C.c_void_p), len(Rs), int(self_idx), L.ptr(part), s(F.device))
+
+
 def dots(X, Ys, part, mode=0):
     """part[v][k] = per-k <X, Y_v> (X may be one matrix or a list paired with Ys)."""
     import ctypes as C

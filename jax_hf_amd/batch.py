@@ -291,7 +291,7 @@ class _BatchCG:
 
 # state-vector slots the batched polish reduces into (unused by the CG layout)
 _SL_RES = 20    # [20] sum w |FP - PF|^2, [21] sum w |P - P_prev|^2
-_SL_B = 22      # [22 ...] the new row of the DIIS B matrix
+_SL_B = 22      # [22 ...] the new row of the DIIS B matrix (= _SL_RES + 2: one reduction fills both)
 
 
 class _BatchPolish:
@@ -358,10 +358,9 @@ class _BatchPolish:
             idx = k % nh
             Fk, Rk = self.F_hist[idx], self.R_hist[idx]
             self.fk(self.P_cur, Fk, self.st)
-            K.scf_resid(Fk, self.P_cur, self.P_prev if k > 0 else None, Rk, self.part)
-            self._reduce(2, _SL_RES)
-            K.dots(Rk, self.R_hist, self.part, mode=0)                   # B row (reference_scf.py:110-111)
-            self._reduce(nh, _SL_B)
+            # residuals and the B row (reference_scf.py:110-111) in one pass: slots _SL_RES, _SL_B
+            K.scf_resid_dots(Fk, self.P_cur, self.P_prev if k > 0 else None, Rk, self.R_hist, idx, self.part)
+            self._reduce(2 + nh, _SL_RES)
             sv = self.st.cpu().numpy()                                    # one readback per iteration
             coefs = np.zeros((S, 8))
             for s in range(S):

@@ -22,7 +22,8 @@
 // strategy: a pair rotates only while |A_pq| > eps ||A||_F (below that the
 // element is a perturbation under the rounding of the eigenvalues); the sweeps
 // end after the first sweep without a rotation in either of the wave's
-// matrices.  Eigenvalues ascending, eigenvectors in the columns of V (the
+// matrices; a step in which no pair of the wave rotates is skipped.
+// Eigenvalues ascending, eigenvectors in the columns of V (the
 // reference's jnp.linalg.eigh convention, linalg.py:96-160, reference_scf.py:89).
 #include "hfx_common.h"
 #include "hfx_internal.h"
@@ -74,11 +75,11 @@ __device__ __forceinline__ void j16_step(cx<T> (&b)[8], cx<T> (&w)[8], int m, in
   cx<T> cij;                                                   // A[i][j]
   if constexpr (MH) cij = swzc<16>(b[ML]);
   else cij = b[ML];
+  int r = 0;
   if (h == 0) {
     const T app = b[0].re;
     T jd = (T)1, dn = app;
     cx<T> jo{(T)0, (T)0};
-    int r = 0;
     if (act && j < nb) {
       const T ac2 = cij.re * cij.re + cij.im * cij.im;
       if (ac2 > thr2) {
@@ -104,6 +105,9 @@ __device__ __forceinline__ void j16_step(cx<T> (&b)[8], cx<T> (&w)[8], int m, in
     sh.rot[m][i] = r;
   }
   __syncthreads();
+  // no pair of either matrix rotates: the step is the identity (the closing
+  // sweep, and most late steps of a warm start, end here)
+  if (!__any(r)) return;
   const int ih = i ^ (h << 3);          // the column of local slot s is ih ^ s
   // A <- A J, V <- V J:  new X[t] = J_xx X[t] + J_{x^M, x} X[t ^ M],  x = i ^ t
   cx<T> pb[8], pw[8];

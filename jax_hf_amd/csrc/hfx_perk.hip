@@ -179,13 +179,16 @@ __device__ void m_eigh(cx<T>* A, cx<T>* V, cx<T>* Vt, double* lam, int nb, doubl
   __syncthreads();
   const int m = nb + (nb & 1);    // even round-robin size (dummy index = nb when odd)
   cx<T> newA[PK_EPL], newV[PK_EPL];
-  __shared__ int nrot;
+  __shared__ int rf[PK_NBMAX];    // 1: the index is in a pair rotated this step
   // Threshold Jacobi: a pair (p, q) is rotated only while
   //   |a_pq| > eps * max(sqrt(|a_pp a_qq|), 0.01 * ||A||_F / sqrt(nb)),
-  // otherwise a_pq is set to zero (a perturbation below the rounding of the
-  // diagonal); the sweeps stop after the first sweep with no rotation.  The
-  // relative part keeps small eigenvalues accurate, the absolute floor stops
-  // rotations that would only stir rounding noise.
+  // otherwise it is left as is (below the rounding of the diagonal); the
+  // sweeps stop after the first sweep with no rotation.  The relative part
+  // keeps small eigenvalues accurate, the absolute floor stops rotations that
+  // would only stir rounding noise.  A step touches only the rows / columns
+  // of its rotated pairs (A) and their columns (V), and a step with no
+  // rotation is skipped: a warm start (nearly diagonal A) and the closing
+  // sweep cost little more than the pair tests.
   T floor_abs;
   {
     T tot = 0;
@@ -197,8 +200,7 @@ __device__ void m_eigh(cx<T>* A, cx<T>* V, cx<T>* Vt, double* lam, int nb, doubl
     floor_abs = (T)(0.01 * sqrt(tot_s / (double)nb));
   }
   for (int sweep = 0; sweep < 30 && nb > 1; ++sweep) {
-    if (threadIdx.x == 0) nrot = 0;
-    __syncthreads();
+    bool any = false;
     for (int step = 0; step < m - 1; ++step) {
       // pair table for this step: index m-1 meets `step`; others (step+k, step-k) mod (m-1)
       if ((int)threadIdx.x < m / 2) {
@@ -214,7 +216,7 @@ __device__ void m_eigh(cx<T>* A, cx<T>* V, cx<T>* Vt, double* lam, int nb, doubl
           const T ac = sqrt(c.re * c.re + c.im * c.im);
           const T app = A[p * PK_PITCH + p].re, aqq = A[q * PK_PITCH + q].re;
           if (ac > eps * fmax(sqrt(fabs(app * aqq)), floor_abs)) {
-            nrot = 1;
+            rf[p] = rf[q] = 1;
             const T zeta = (aqq - app) / ((T)2 * ac);
             const T t = (zeta >= (T)0 ? (T)1 : (T)-1) / (fabs(zeta) + sqrt((T)1 + zeta * zeta));
             const T cs = (T)1 / sqrt((T)1 + t * t), sn = t * cs;
@@ -226,24 +228,24 @@ __device__ void m_eigh(cx<T>* A, cx<T>* V, cx<T>* Vt, double* lam, int nb, doubl
             dnew[p] = app - t * ac;
             dnew[q] = aqq + t * ac;
           } else {
-            jd[p] = jd[q] = (T)1;
-            jo[p] = jo[q] = cx<T>{0, 0};
-            dnew[p] = app;
-            dnew[q] = aqq;
+            rf[p] = rf[q] = 0;
           }
         } else if (p < nb) {   // paired with the dummy: untouched this step
-          partner[p] = p;
-          jd[p] = (T)1;
-          jo[p] = cx<T>{0, 0};
-          dnew[p] = A[p * PK_PITCH + p].re;
+          rf[p] = 0;
         }
       }
       __syncthreads();
+      const unsigned long long rot = __ballot((int)threadIdx.x < nb && rf[threadIdx.x] != 0);
+      if (rot == 0ull) continue;   // uniform: nothing rotates in this step
+      any = true;
       // A'_ij = sum_{a in {i,i'}, b in {j,j'}} conj(J_ai) A_ab J_bj ;  V' = V J
+      // (only where i or j belongs to a rotated pair; V' only in rotated columns)
       PK_FOR_ELEM(nb, {
-        const int ip = partner[i], jp = partner[j];
-        const T di = jd[i], dj = jd[j];
-        const cx<T> oi = jo[i], oj = jo[j];
+        const bool ri = (rot >> i) & 1ull, rj = (rot >> j) & 1ull;
+        if (!(ri || rj)) continue;
+        const int ip = ri ? partner[i] : i, jp = rj ? partner[j] : j;
+        const T di = ri ? jd[i] : (T)1, dj = rj ? jd[j] : (T)1;
+        const cx<T> oi = ri ? jo[i] : cx<T>{0, 0}, oj = rj ? jo[j] : cx<T>{0, 0};
         cx<T> r;
         if (i == j) {
           r = cx<T>{dnew[i], (T)0};
@@ -252,27 +254,26 @@ __device__ void m_eigh(cx<T>* A, cx<T>* V, cx<T>* Vt, double* lam, int nb, doubl
         } else {
           const cx<T> aij = A[i * PK_PITCH + j];
           cx<T> x = dj * aij;
-          if (jp != j) x = x + A[i * PK_PITCH + jp] * oj;
+          if (rj) x = x + A[i * PK_PITCH + jp] * oj;
           cx<T> y{0, 0};
-          if (ip != i) {
+          if (ri) {
             y = dj * A[ip * PK_PITCH + j];
-            if (jp != j) y = y + A[ip * PK_PITCH + jp] * oj;
+            if (rj) y = y + A[ip * PK_PITCH + jp] * oj;
           }
           r = di * x + cmul(oi, y);
         }
         newA[m_] = r;
-        cx<T> v = dj * V[i * PK_PITCH + j];
-        if (jp != j) v = v + V[i * PK_PITCH + jp] * oj;
-        newV[m_] = v;
+        if (rj) newV[m_] = dj * V[i * PK_PITCH + j] + V[i * PK_PITCH + jp] * oj;
       })
       __syncthreads();
       PK_FOR_ELEM(nb, {
-        A[i * PK_PITCH + j] = newA[m_];
-        V[i * PK_PITCH + j] = newV[m_];
+        const bool ri = (rot >> i) & 1ull, rj = (rot >> j) & 1ull;
+        if (ri || rj) A[i * PK_PITCH + j] = newA[m_];
+        if (rj) V[i * PK_PITCH + j] = newV[m_];
       })
       __syncthreads();
     }
-    if (nrot == 0) break;
+    if (!any) break;
   }
   // ascending order: rank_i = #{j : l_j < l_i or (l_j == l_i and j < i)}
   __shared__ int rank[PK_NBMAX];
@@ -886,12 +887,44 @@ __global__ __launch_bounds__(64) void pk_rotate_density_kernel(int nb, cx<T>* __
   PK_FOR_ELEM(nb, { P[k * nb2 + e] = (T)0.5 * (sU[i * PK_PITCH + j] + conj(sU[j * PK_PITCH + i])); })
 }
 
+// Newton-Schulz re-orthonormalisation in place, `steps` times:
+// U <- U (3 I - U^dagger U) / 2.  Each step squares the deviation of U^dagger U
+// from I: an orbital basis carried in FP32 (~1e-7) reaches FP64 rounding in two.
+template <typename T>
+__global__ __launch_bounds__(64) void pk_orthonormalize_kernel(int nb, cx<T>* __restrict__ U, int steps) {
+  __shared__ cx<T> sU[PK_MAT], sS[PK_MAT], sX[PK_MAT];
+  const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  m_load(sU, U + k * nb2, nb);
+  __syncthreads();
+  for (int it = 0; it < steps; ++it) {
+    m_gemm<T, true, false>(sS, sU, sU, nb);     // S = U^H U
+    PK_FOR_ELEM(nb, {
+      const cx<T> v = sS[i * PK_PITCH + j];
+      sS[i * PK_PITCH + j] = cx<T>{(i == j ? (T)1.5 : (T)0) - (T)0.5 * v.re, (T)-0.5 * v.im};
+    })
+    __syncthreads();
+    m_gemm<T, false, false>(sX, sU, sS, nb);    // U (3 I - S) / 2
+    PK_FOR_ELEM(nb, { sU[i * PK_PITCH + j] = sX[i * PK_PITCH + j]; })
+    __syncthreads();
+  }
+  m_store(U + k * nb2, sU, nb);
+}
+
 // SCF residuals (reference_scf.py:291-294): comm = F D - D F (stored when
-// comm != NULL), part[0][k] = sum |comm|^2, part[1][k] = sum |Pn - D|^2.
+// comm != NULL), part[0][k] = sum |comm|^2, part[1][k] = sum |Pn - D|^2, and
+// with nr > 0 the new row of the DIIS B matrix in the same pass:
+// part[2 + v][k] = Re sum conj(comm) R_v over the stored residuals R_v
+// (reference_scf.py:110-111); for v == self (the slot comm is written to) the
+// value is part[0][k] itself.
+struct PkRes {
+  const void* r[PK_XMAX];
+  int nr, self;
+};
 template <typename T>
 __global__ __launch_bounds__(64) void pk_scf_resid_kernel(int nb, size_t Nk, const cx<T>* __restrict__ F,
                                                           const cx<T>* __restrict__ D, const cx<T>* __restrict__ Pn,
-                                                          cx<T>* __restrict__ comm, double* __restrict__ part) {
+                                                          cx<T>* __restrict__ comm, double* __restrict__ part,
+                                                          PkRes rs) {
   __shared__ cx<T> sF[PK_MAT], sD[PK_MAT], sX[PK_MAT], sY[PK_MAT];
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
   m_load(sF, F + k * nb2, nb);
@@ -899,7 +932,9 @@ __global__ __launch_bounds__(64) void pk_scf_resid_kernel(int nb, size_t Nk, con
   __syncthreads();
   m_gemm<T, false, false>(sX, sF, sD, nb);
   m_gemm<T, false, false>(sY, sD, sF, nb);
-  double c2 = 0.0, d2 = 0.0;
+  double c2 = 0.0, d2 = 0.0, bd[PK_XMAX];
+#pragma unroll
+  for (int v = 0; v < PK_XMAX; ++v) bd[v] = 0.0;
   PK_FOR_ELEM(nb, {
     const cx<T> c = sX[i * PK_PITCH + j] - sY[i * PK_PITCH + j];
     if (comm) comm[k * nb2 + e] = c;
@@ -908,6 +943,11 @@ __global__ __launch_bounds__(64) void pk_scf_resid_kernel(int nb, size_t Nk, con
       const cx<T> d = Pn[k * nb2 + e] - sD[i * PK_PITCH + j];
       d2 += (double)d.re * d.re + (double)d.im * d.im;
     }
+    _Pragma("unroll") for (int v = 0; v < PK_XMAX; ++v)
+      if (v < rs.nr && v != rs.self) {
+        const cx<T> r = reinterpret_cast<const cx<T>*>(rs.r[v])[k * nb2 + e];
+        bd[v] += (double)c.re * r.re + (double)c.im * r.im;
+      }
   })
   c2 = wave_sum(c2);
   d2 = wave_sum(d2);
@@ -915,6 +955,12 @@ __global__ __launch_bounds__(64) void pk_scf_resid_kernel(int nb, size_t Nk, con
     part[k] = c2;
     part[Nk + k] = d2;
   }
+#pragma unroll
+  for (int v = 0; v < PK_XMAX; ++v)
+    if (v < rs.nr) {
+      const double b = v == rs.self ? c2 : wave_sum(bd[v]);
+      if (threadIdx.x == 0) part[(size_t)(2 + v) * Nk + k] = b;
+    }
 }
 
 // Per-k inner products of X with up to 8 matrices Y_v:
@@ -1295,18 +1341,48 @@ int hfx_rotate_density(int dtype, int nb, size_t nk, void* U, const void* W, con
   return check_launch("pk_rotate_density_kernel");
 }
 
+static int scf_resid_launch(int dtype, int nb, size_t nk, const void* F, const void* D, const void* Pn, void* comm,
+                            PkRes rs, double* part, hipStream_t s) {
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(pk_scf_resid_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(double, F), PKC(double, D),
+                       PKC(double, Pn), PKM(double, comm), part, rs);
+  else
+    hipLaunchKernelGGL(pk_scf_resid_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(float, F), PKC(float, D),
+                       PKC(float, Pn), PKM(float, comm), part, rs);
+  return check_launch("pk_scf_resid_kernel");
+}
+
 int hfx_scf_resid(int dtype, int nb, size_t nk, const void* F, const void* D, const void* Pn, void* comm,
                   double* part, void* stream) {
   if (int rc = pk_check_nb(nb)) return rc;
   if (!F || !D || !part) return set_error(HFX_EINVAL, "hfx_scf_resid: NULL argument");
+  return scf_resid_launch(dtype, nb, nk, F, D, Pn, comm, PkRes{}, part, (hipStream_t)stream);
+}
+
+int hfx_orthonormalize(int dtype, int nb, size_t nk, void* U, int steps, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  if (!U || steps < 1 || steps > 4) return set_error(HFX_EINVAL, "hfx_orthonormalize: bad argument (steps=%d)", steps);
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_scf_resid_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(double, F), PKC(double, D),
-                       PKC(double, Pn), PKM(double, comm), part);
+    hipLaunchKernelGGL(pk_orthonormalize_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKM(double, U), steps);
   else
-    hipLaunchKernelGGL(pk_scf_resid_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(float, F), PKC(float, D),
-                       PKC(float, Pn), PKM(float, comm), part);
-  return check_launch("pk_scf_resid_kernel");
+    hipLaunchKernelGGL(pk_orthonormalize_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKM(float, U), steps);
+  return check_launch("pk_orthonormalize_kernel");
+}
+
+int hfx_scf_resid_dots(int dtype, int nb, size_t nk, const void* F, const void* D, const void* Pn, void* comm,
+                       const void* const* Rs, int nr, int self, double* part, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  if (!F || !D || !comm || !part || !Rs || nr < 1 || nr > PK_XMAX || self < -1 || self >= nr)
+    return set_error(HFX_EINVAL, "hfx_scf_resid_dots: bad argument (nr=%d, self=%d)", nr, self);
+  PkRes rs{};
+  rs.nr = nr;
+  rs.self = self;
+  for (int v = 0; v < nr; ++v) {
+    if (!Rs[v] && v != self) return set_error(HFX_EINVAL, "hfx_scf_resid_dots: NULL residual %d", v);
+    rs.r[v] = Rs[v];
+  }
+  return scf_resid_launch(dtype, nb, nk, F, D, Pn, comm, rs, part, (hipStream_t)stream);
 }
 
 int hfx_batched_dots(int dtype, int nb, size_t nk, const void* const* Xs, const void* const* Ys, int nv, int mode,

@@ -30,7 +30,7 @@ import torch
 
 from . import _lib as L
 from . import _ops as K
-from ._util import capture_graphs, convert, host_array, to_tensor
+from ._util import KIND, capture_graphs, convert, host_array, to_tensor
 from .reference_scf import diis_solve
 
 __all__ = ["SolverConfig", "SolveResult", "solve", "solve_direct_minimization"]
@@ -393,11 +393,12 @@ class _DIISPolish:
     at the previous mu, occupations, density) is one replayed HIP graph, and
     the object (buffers + graph) is kept on the kernel between solves.
 
-    After the first update of a run the eigh starts from the previous basis U:
-    U^H F U is nearly diagonal once the iteration settles, so the Jacobi
-    sweeps stop after one or two, and V = U W (the CG finish does the same,
-    solver.py:622-625).  Each run begins with a cold eigh, so rounding in the
-    accumulated basis stays bounded by one run's updates.
+    The eigh starts from the previous basis U: U^H F U is nearly diagonal once
+    the iteration settles, so the Jacobi sweeps stop after one or two, and
+    V = U W (the CG finish does the same, solver.py:622-625).  A run starts
+    from the CG orbitals (re-orthonormalised in FP64) when they are given,
+    else with a cold eigh; either way the rounding in the accumulated basis
+    stays bounded by one run's updates.
     """
 
     def __init__(self, kernel, n_electrons, nh=6):
@@ -421,7 +422,7 @@ class _DIISPolish:
         self.R_hist = [z() for _ in range(nh)]
         self.eps = torch.zeros((nk1, nk2, nb), dtype=torch.float64, device=dev)
         self.occ = torch.zeros_like(self.eps)
-        # one readback buffer: outs [0:2] residuals, [2] E, [8:] B row; then the state vector
+        # one readback buffer: outs [0:2] residuals, [2:2+nh] B row, [16] E; then the state vector
         self.io = torch.zeros(64 + K.ST_LEN, dtype=torch.float64, device=dev)
         self.outs, self.st = self.io[:64], self.io[64:]
         self.graph = None
@@ -457,13 +458,22 @@ class _DIISPolish:
     def _capture(self):
         return capture_graphs(self.dev, self._update)[0]
 
-    def run(self, P_lo, mu0, tol, max_iter):
+    def run(self, P_lo, mu0, tol, max_iter, Q_lo=None):
+        """Q_lo (optional): the orbitals P_lo was built from.  Re-orthonormalised in
+        FP64 (two Newton-Schulz steps), they are the first basis, so that the
+        first update is a warm eigh as well."""
         nh, nk = self.nh, self.nk
         K.hermitize(convert(P_lo, torch.complex128) if P_lo.dtype != torch.complex128 else P_lo, self.P_cur)
         self.st[K.ST_MU] = mu0
+        warm0 = Q_lo is not None
+        if warm0:
+            Qc = Q_lo.contiguous()
+            L.call("hfx_convert", L.ptr(Qc), KIND[Qc.dtype], L.ptr(self.U), KIND[torch.complex128], Qc.numel(),
+                   L.stream(self.dev))
+            K.orthonormalize(self.U, 2)
         part, outs = self.wsp.part, self.outs
-        o_res, o_B = outs[0:2], outs[8:8 + nh]
-        E_ptr = outs.data_ptr() + 8 * 2
+        o_rb = outs[0:2 + nh]                  # residuals [0:2], then the B row [2:2 + nh]
+        E_ptr = outs.data_ptr() + 8 * 16
         B = np.zeros((nh, nh))
         hE, hC = [], []
         k, converged, E, idx, redo = 0, False, float("nan"), 0, False
@@ -471,13 +481,9 @@ class _DIISPolish:
             idx = k % nh
             Fk, Rk = self.F_hist[idx], self.R_hist[idx]
             self.fock.build(self.P_cur, Fk, E_ptr=E_ptr)
-            K.scf_resid(Fk, self.P_cur, self.P_prev if k > 0 else None, Rk, part)
-            K.reduce(self.wsp, part, 2, nk, w=self.w, wmask=0b11, out=o_res)
-            for j0 in range(0, nh, 8):                                    # B row (reference_scf.py:110-111)
-                Ys = self.R_hist[j0:j0 + 8]
-                K.dots(Rk, Ys, part, mode=0)
-                K.reduce(self.wsp, part, len(Ys), nk, w=self.w, wmask=(1 << len(Ys)) - 1,
-                         out=o_B[j0:j0 + len(Ys)])
+            # residuals and the B row (reference_scf.py:110-111) in one pass, one reduction
+            K.scf_resid_dots(Fk, self.P_cur, self.P_prev if k > 0 else None, Rk, self.R_hist, idx, part)
+            K.reduce(self.wsp, part, 2 + nh, nk, w=self.w, wmask=(1 << (2 + nh)) - 1, out=o_rb)
             sv_all = self.io.cpu().numpy()                                # one readback per iteration
             if k > 0 and sv_all[64 + K.ST_MUDONE] == 0.0 and not redo:
                 # the last update's mu needed more than the graph's steps: finish
@@ -491,13 +497,13 @@ class _DIISPolish:
             sv = sv_all[:64]
             c_res = float(np.sqrt(sv[0] / self.ws_sum))
             d_res = float(np.sqrt(sv[1] / self.ws_sum)) if k > 0 else float("inf")
-            E = float(sv[2])
+            E = float(sv[16])
             hE.append(E)
             hC.append(c_res)
             if c_res <= tol and d_res <= tol:
                 converged = True
                 break
-            row = sv[8:8 + nh]
+            row = sv[2:2 + nh]
             B[idx, :] = row
             B[:, idx] = row
             count = min(k + 1, nh)
@@ -505,10 +511,10 @@ class _DIISPolish:
             coeffs, ok = diis_solve(B * np.outer(active, active), active)
             if count >= 2 and ok:
                 sel = [i for i in range(nh) if active[i] and coeffs[i] != 0.0]
-                self._extrapolate([self.F_hist[i] for i in sel], [float(coeffs[i]) for i in sel], k > 0)
+                self._extrapolate([self.F_hist[i] for i in sel], [float(coeffs[i]) for i in sel], k > 0 or warm0)
             else:
-                self._extrapolate([Fk], [1.0], k > 0)
-            if k == 0:
+                self._extrapolate([Fk], [1.0], k > 0 or warm0)
+            if k == 0 and not warm0:
                 self._update(warm=False)
             elif self.graph is None:
                 self._update()
@@ -523,9 +529,10 @@ class _DIISPolish:
         if not converged and k > 0:     # budget spent after an update: evaluate the returned density
             Ffin = self.Fx
             self.fock.build(self.P_cur, Ffin, E_ptr=E_ptr)
-            E = float(outs[2].item())
+            E = float(outs[16].item())
         # orbitals / occupations / mu of the returned state (solver.py:606-638 form)
-        if k > 0:
+        warm = k > 0 or warm0
+        if warm:
             K.extrap_rotate(self.Ft, self.U, [Ffin], [1.0])
             K.eigh(self.Ft, self.eps, self.W, herm_in=True)
         else:
@@ -533,7 +540,7 @@ class _DIISPolish:
         self._mu(self.MU_GRAPH_STEPS)
         if float(self.st[K.ST_MUDONE].item()) == 0.0:
             self._mu()
-        if k > 0:
+        if warm:
             K.rotate_density(self.U, self.W, self.occ, self.tmp)
         Q = self.U.clone()
         return (self.P_cur.clone(), Ffin.clone(), Q, self.occ.clone(), self.st[K.ST_MU].clone(), E, n_it,
@@ -563,12 +570,15 @@ def _polish_diis(kernel, runner, n_electrons, cfg, fock_build_fn, k_cg, hist_E, 
     """
     from .reference_scf import SCFConfig, _SCF
     dev = kernel.device
+    P_lo_given = P_lo is not None
     if P_lo is None:
-        P_lo = runner.density()
+        P_lo = runner.density()       # from the orbitals runner.Q, which then start the polish basis
     if fock_build_fn is None and cfg.project_fn is None and cfg.use_graph:
         pol = _polish_state(kernel, n_electrons)
         mu_start = runner.st[K.ST_MU] if mu0 is None else mu0
-        density, F, Q, p, mu, E, n2, conv, hE2, hC2 = pol.run(P_lo, mu_start, cfg.polish_tol, cfg.polish_max_iter)
+        Q_lo = runner.Q if (runner is not None and P_lo_given is False) else None
+        density, F, Q, p, mu, E, n2, conv, hE2, hC2 = pol.run(P_lo, mu_start, cfg.polish_tol, cfg.polish_max_iter,
+                                                              Q_lo=Q_lo)
         k = k_cg + n2
         hE = list(hist_E) + list(hE2)
         hG = list(hist_G) + list(hC2)

@@ -101,6 +101,59 @@ def test_matmul_operand_forms(dev, nb, dt):
             assert np.abs(_np(out) - ref).max() <= tol * scale, (ops, use_c)
 
 
+@pytest.mark.parametrize("nb", [5, 16])
+def test_orthonormalize_fp32_basis(dev, nb):
+    """Two Newton-Schulz steps take an FP32-rounded unitary to FP64 unitarity,
+    moving it by no more than its rounding (the polar factor is the nearest unitary)."""
+    import torch
+    from jax_hf_amd import _ops as K
+    rng = np.random.default_rng(nb)
+    Q = np.linalg.qr(rng.standard_normal((40, nb, nb)) + 1j * rng.standard_normal((40, nb, nb)))[0]
+    Q32 = Q.astype(np.complex64).astype(np.complex128)
+    U = torch.from_numpy(Q32).to(dev)
+    K.orthonormalize(U, 2)
+    Uh = _np(U)
+    eye = np.conj(np.swapaxes(Uh, -1, -2)) @ Uh
+    assert np.abs(eye - np.eye(nb)).max() < 1e-14
+    assert np.abs(Uh - Q32).max() < 1e-6
+    # polar factor of Q32 (SVD) -- the fixed point Newton-Schulz converges to
+    W, _, Vh = np.linalg.svd(Q32)
+    assert np.abs(Uh - W @ Vh).max() < 1e-13
+
+
+@pytest.mark.parametrize("dt", ["c128", "c64"])
+@pytest.mark.parametrize("nb", [5, 16])
+def test_scf_resid_dots_matches_numpy(dev, nb, dt):
+    """Residuals and the DIIS B row in one pass (reference_scf.py:110-111, 291-294)."""
+    import torch
+    from jax_hf_amd import _ops as K
+    rng = np.random.default_rng(3 + nb)
+    cdt = np.complex128 if dt == "c128" else np.complex64
+    tol = 1e-12 if dt == "c128" else 1e-4
+    nk = 37
+    shp = (nk, nb, nb)
+    cplx = lambda: (rng.standard_normal(shp) + 1j * rng.standard_normal(shp)).astype(cdt)  # noqa: E731
+    F, D, Pn = _herm(cplx()), _herm(cplx()), _herm(cplx())
+    Rs = [cplx() for _ in range(5)]
+    self_idx = 2
+    t = lambda X: torch.from_numpy(np.ascontiguousarray(X)).to(dev)  # noqa: E731
+    Rt = [t(R) for R in Rs]
+    part = torch.zeros((8, nk), dtype=torch.float64, device=dev)
+    K.scf_resid_dots(t(F), t(D), t(Pn), Rt[self_idx], Rt, self_idx, part)
+    p = _np(part)
+    F64, D64 = F.astype(np.complex128), D.astype(np.complex128)
+    comm = F64 @ D64 - D64 @ F64
+    assert np.abs(_np(Rt[self_idx]) - comm).max() <= tol * np.abs(comm).max()
+    c2 = np.sum(np.abs(comm) ** 2, axis=(1, 2))
+    d2 = np.sum(np.abs(Pn.astype(np.complex128) - D64) ** 2, axis=(1, 2))
+    assert np.allclose(p[0], c2, rtol=tol, atol=0) and np.allclose(p[1], d2, rtol=tol, atol=0)
+    for v, R in enumerate(Rs):
+        ref = c2 if v == self_idx else np.real(np.sum(np.conj(comm) * R, axis=(1, 2)))
+        assert np.abs(p[2 + v] - ref).max() <= tol * np.abs(c2).max(), v
+        if v == self_idx:
+            assert np.array_equal(p[2 + v], p[0])
+
+
 @pytest.mark.parametrize("T", [0.5, 0.05, 0.005])
 def test_mu_solvers_match_oracle(dev, T):
     import torch
