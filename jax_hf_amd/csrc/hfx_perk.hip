@@ -963,6 +963,60 @@ __global__ __launch_bounds__(64) void pk_scf_resid_kernel(int nb, size_t Nk, con
     }
 }
 
+// nb = 16: the same outputs with F D - D F accumulated on the matrix cores and
+// kept in registers (lane l holds the elements (row(l, r), l % 16)); two staged
+// matrices instead of four, so twice the waves per CU for this streaming pass.
+template <typename T>
+__global__ __launch_bounds__(64) void pk_scf_resid16_kernel(size_t Nk, const cx<T>* __restrict__ F,
+                                                            const cx<T>* __restrict__ D,
+                                                            const cx<T>* __restrict__ Pn, cx<T>* __restrict__ comm,
+                                                            double* __restrict__ part, PkRes rs) {
+  constexpr int nb = PK_NBMAX, nb2 = PK_NBMAX * PK_NBMAX;
+  __shared__ cx<T> sF[PK_MAT], sD[PK_MAT];
+  const size_t k = blockIdx.x;
+  m_load(sF, F + k * nb2, nb);
+  m_load(sD, D + k * nb2, nb);
+  __syncthreads();
+  MAcc<T> acc;
+  acc.zero();
+  mf_gemm16<T, false, false>(acc, sF, sD);
+  mf_gemm16<T, false, false>(acc, sD, sF, (T)-1);
+  const int l = threadIdx.x & 63, col = l & 15;
+  double c2 = 0.0, d2 = 0.0, bd[PK_XMAX];
+#pragma unroll
+  for (int v = 0; v < PK_XMAX; ++v) bd[v] = 0.0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int row = Mfma<T>::row(l, r);
+    const size_t e = k * nb2 + (size_t)row * nb + col;
+    const cx<T> c{acc.re[r], acc.im[r]};
+    if (comm) comm[e] = c;
+    c2 += (double)c.re * c.re + (double)c.im * c.im;
+    if (Pn) {
+      const cx<T> d = Pn[e] - sD[row * PK_PITCH + col];
+      d2 += (double)d.re * d.re + (double)d.im * d.im;
+    }
+#pragma unroll
+    for (int v = 0; v < PK_XMAX; ++v)
+      if (v < rs.nr && v != rs.self) {
+        const cx<T> q = reinterpret_cast<const cx<T>*>(rs.r[v])[e];
+        bd[v] += (double)c.re * q.re + (double)c.im * q.im;
+      }
+  }
+  c2 = wave_sum(c2);
+  d2 = wave_sum(d2);
+  if (threadIdx.x == 0) {
+    part[k] = c2;
+    part[Nk + k] = d2;
+  }
+#pragma unroll
+  for (int v = 0; v < PK_XMAX; ++v)
+    if (v < rs.nr) {
+      const double b = v == rs.self ? c2 : wave_sum(bd[v]);
+      if (threadIdx.x == 0) part[(size_t)(2 + v) * Nk + k] = b;
+    }
+}
+
 // Per-k inner products of X with up to 8 matrices Y_v:
 //   mode 0: part[v][k] = Re sum_ij conj(X_ij) Y_v,ij   (Frobenius, solver.py:205-208, reference_scf.py:110-111)
 //   mode 1: part[v][k] = Re sum_ij X_ij Y_v,ji          (trace of the product, reference_scf.py:319)
@@ -1343,6 +1397,15 @@ int hfx_rotate_density(int dtype, int nb, size_t nk, void* U, const void* W, con
 
 static int scf_resid_launch(int dtype, int nb, size_t nk, const void* F, const void* D, const void* Pn, void* comm,
                             PkRes rs, double* part, hipStream_t s) {
+  if (nb == PK_NBMAX) {
+    if (dtype == HFX_DTYPE_C128)
+      hipLaunchKernelGGL(pk_scf_resid16_kernel<double>, dim3(nk), dim3(64), 0, s, nk, PKC(double, F),
+                         PKC(double, D), PKC(double, Pn), PKM(double, comm), part, rs);
+    else
+      hipLaunchKernelGGL(pk_scf_resid16_kernel<float>, dim3(nk), dim3(64), 0, s, nk, PKC(float, F), PKC(float, D),
+                         PKC(float, Pn), PKM(float, comm), part, rs);
+    return check_launch("pk_scf_resid16_kernel");
+  }
   if (dtype == HFX_DTYPE_C128)
     hipLaunchKernelGGL(pk_scf_resid_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(double, F), PKC(double, D),
                        PKC(double, Pn), PKM(double, comm), part, rs);
