@@ -390,6 +390,14 @@ int hfx_fock_epilogue_batched(const hfx_plan* plan, int nsolve, const void* h, c
 int hfx_fock_epilogue_planar_batched(const hfx_plan* plan, int nsolve, const void* h, const void* Sigt, int herm,
                                      const void* P, const double* w2d, void* F, double* st, double* scratch,
                                      void* stream);
+/* Pulay C-DIIS coefficients on the device (_diis_coefficients, reference_scf.py:99-121):
+ * writes the new row brow (nh doubles, device) into row and column idx of the
+ * device B matrix (nh x nh), solves the masked, max-normalised augmented
+ * system over the first `count` slots (LU, partial pivoting) and writes
+ * coef[0..7]: the solution when count >= 2 and the solve is regular and
+ * finite, else the plain step e_idx.  nh <= 8. */
+int hfx_diis_coefficients(const double* brow, double* B, int nh, int idx, int count, double* coef, void* stream);
+
 /* Scratch (float64) for hfx_reduce_batched / hfx_solve_mu_batched. */
 size_t hfx_batch_scratch_doubles(int nseg);
 /* hfx_reduce per solve (segments of seg_len k-points); with op == HFX_OP_SUM

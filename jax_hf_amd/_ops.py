@@ -109,6 +109,12 @@ def scf_resid(F, D, Pn, comm, part):
            s(F.device))
 
 
+def diis_coefficients(brow, B, idx, count, coef):
+    """Device C-DIIS: B[idx] row/column <- brow, coef <- masked augmented solve (or e_idx)."""
+    L.call("hfx_diis_coefficients", L.ptr(brow), L.ptr(B), int(B.shape[0]), int(idx), int(count), L.ptr(coef),
+           s(B.device))
+
+
 def orthonormalize(U, steps=2):
     """In place: U <- U (3 I - U^H U) / 2, `steps` times, per k."""
     nb = U.shape[-1]

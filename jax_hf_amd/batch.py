@@ -340,12 +340,20 @@ class _BatchPolish:
                self.Nk, self.nb, self.T, int(maxiter), L.ptr(self.st), L.ptr(self.occ), L.ptr(self.scratch),
                L.stream(self.dev))
 
-    def run(self, P_lo, mu0, tol, max_iter):
+    def run(self, P_lo, mu0, tol, max_iter, Q_lo=None):
+        """Q_lo (optional): the stacked orbitals P_lo was built from; re-orthonormalised
+        in FP64 they are the first basis (the first update is a warm eigh)."""
         import ctypes as C
         S, nh = self.S, self.nh
         K.hermitize(convert(P_lo, torch.complex128) if P_lo.dtype != torch.complex128 else P_lo, self.P_cur)
         self.st.zero_()
         self.st[:, K.ST_MU] = mu0
+        warm0 = Q_lo is not None
+        if warm0:
+            Qc = Q_lo.contiguous()
+            L.call("hfx_convert", L.ptr(Qc), KIND[Qc.dtype], L.ptr(self.U), KIND[torch.complex128], Qc.numel(),
+                   L.stream(self.dev))
+            K.orthonormalize(self.U, 2)
         B = np.zeros((S, nh, nh))
         done = [False] * S
         hE = [[] for _ in range(S)]
@@ -388,7 +396,7 @@ class _BatchPolish:
                     coefs[s, nh:] = 0.0
             if all(done):
                 break
-            if k == 0:                    # cold first update: eigh of herm(F_0)
+            if k == 0 and not warm0:      # cold first update: eigh of herm(F_0)
                 K.lincomb_multi(self.Ft, [Fk], [1.0], herm=True)
                 K.eigh(self.Ft, self.eps, self.U, herm_in=True)
                 self._mu()
@@ -413,10 +421,15 @@ class _BatchPolish:
                     self.Ffin[s].copy_(self.tmp[s])
                     E_out[s] = float(sv[s, K.ST_E])
         # orbitals / occupations / mu of the returned states (solver.py:606-638 form)
-        K.extrap_rotate(self.Ft, self.U, [self.Ffin], [1.0])
-        K.eigh(self.Ft, self.eps, self.W, herm_in=True)
-        self._mu()
-        K.rotate_density(self.U, self.W, self.occ, self.tmp)
+        if warm0 or k > 0:                # U holds a basis: warm eigh in it
+            K.extrap_rotate(self.Ft, self.U, [self.Ffin], [1.0])
+            K.eigh(self.Ft, self.eps, self.W, herm_in=True)
+            self._mu()
+            K.rotate_density(self.U, self.W, self.occ, self.tmp)
+        else:                             # every solve stopped before an update
+            K.eigh(self.Ffin, self.eps, self.U, herm_in=True)
+            self._mu()
+            K.density(self.U, self.tmp, p=self.occ)
         return E_out, done, hE, hC
 
 
@@ -467,10 +480,12 @@ def solve_batch(kernels, P0s, n_electrons, *, config: SolverConfig | None = None
         if prec == "mixed":
             k_of, grad, hE, hG = b.run(cfg.mixed_tol_E, 0.0, max(cfg.max_iter // 2, 1), rel_tol=True)
             P_lo = b.density()
+            Q_lo = b.Q
             mu0 = b.st[:, K.ST_MU].clone()
             del b                                            # free the FP32 stack before the FP64 one
             pol = _BatchPolish(ks, [n_electrons[i] for i in members])
-            E2, conv2, hE2, hC2 = pol.run(P_lo, mu0, cfg.polish_tol, cfg.polish_max_iter)
+            E2, conv2, hE2, hC2 = pol.run(P_lo, mu0, cfg.polish_tol, cfg.polish_max_iter, Q_lo=Q_lo)
+            del Q_lo
             sv = pol.st.cpu()
             for j, i in enumerate(members):
                 n2 = len(hE2[j])

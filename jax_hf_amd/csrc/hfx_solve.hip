@@ -729,6 +729,97 @@ __global__ __launch_bounds__(MUS_BLOCK) void mu_occ_seg_kernel(MuSegArgs a) {
     a.p_out[base + e] = sigm((mu - a.eps[base + e]) / Tj);
 }
 
+// ---- Pulay C-DIIS coefficients on the device (reference_scf.py:99-121) ----------
+// One thread: B[idx][:] = B[:][idx] = brow (the new row, already reduced), then
+// the masked, max-normalised augmented system
+//   [ Bn + diag(1 - a)  -a ] [c]   [ 0]
+//   [ -a^T               0 ] [l] = [-1],   a_i = (i < count),
+// solved by LU with partial pivoting.  coef = c when count >= 2 and the
+// solve is regular and finite, else the plain step e_idx; coef[nh..7] = 0.
+constexpr int DIIS_MAX = 8;
+constexpr int DIIS_M = DIIS_MAX + 1;   // augmented size; rows past nh + 1 are identity padding
+__global__ __launch_bounds__(64) void diis_coef_kernel(const double* __restrict__ brow, double* __restrict__ B, int nh, int idx,
+                                 int count, double* __restrict__ coef) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  // every loop has a compile-time trip count so that A and x stay in registers
+  double row[DIIS_MAX];
+#pragma unroll
+  for (int j = 0; j < DIIS_MAX; ++j) row[j] = j < nh ? brow[j] : 0.0;
+#pragma unroll
+  for (int j = 0; j < DIIS_MAX; ++j)
+    if (j < nh) {
+      B[idx * nh + j] = row[j];
+      B[j * nh + idx] = row[j];
+    }
+  double A[DIIS_M][DIIS_M], x[DIIS_M];
+  double mx = 0.0;
+#pragma unroll
+  for (int i = 0; i < DIIS_MAX; ++i)
+#pragma unroll
+    for (int j = 0; j < DIIS_MAX; ++j) {
+      const double b = (i < count && j < count) ? (i == idx ? row[j] : (j == idx ? row[i] : B[i * nh + j])) : 0.0;
+      A[i][j] = b;
+      mx = fmax(mx, fabs(b));
+    }
+  const int N = nh;
+#pragma unroll
+  for (int i = 0; i < DIIS_M; ++i) {
+    const bool ai = i < count;
+#pragma unroll
+    for (int j = 0; j < DIIS_M; ++j) {
+      double a;
+      if (i < N && j < N) a = A[i][j] / fmax(mx, 1e-30) + ((i == j && !ai) ? 1.0 : 0.0);
+      else if (i < N && j == N) a = ai ? -1.0 : 0.0;
+      else if (i == N && j < N) a = (j < count) ? -1.0 : 0.0;
+      else if (i == N && j == N) a = 0.0;
+      else a = (i == j) ? 1.0 : 0.0;          // padding
+      A[i][j] = a;
+    }
+    x[i] = i == N ? -1.0 : 0.0;
+  }
+  bool ok = true;
+#pragma unroll
+  for (int c = 0; c < DIIS_M; ++c) {
+    int piv = c;
+    double best = fabs(A[c][c]);
+#pragma unroll
+    for (int r = c + 1; r < DIIS_M; ++r) {
+      const double v = fabs(A[r][c]);
+      if (v > best) { best = v; piv = r; }
+    }
+    if (best == 0.0) ok = false;
+#pragma unroll
+    for (int r = c + 1; r < DIIS_M; ++r)
+      if (r == piv) {
+#pragma unroll
+        for (int j = c; j < DIIS_M; ++j) { const double t = A[c][j]; A[c][j] = A[r][j]; A[r][j] = t; }
+        const double t = x[c]; x[c] = x[r]; x[r] = t;
+      }
+    const double pv = best == 0.0 ? 1.0 : A[c][c];
+#pragma unroll
+    for (int r = c + 1; r < DIIS_M; ++r) {
+      const double f = A[r][c] / pv;
+#pragma unroll
+      for (int j = c; j < DIIS_M; ++j) A[r][j] -= f * A[c][j];
+      x[r] -= f * x[c];
+    }
+  }
+#pragma unroll
+  for (int c = DIIS_M - 1; c >= 0; --c) {
+    double t = x[c];
+#pragma unroll
+    for (int j = c + 1; j < DIIS_M; ++j) t -= A[c][j] * x[j];
+    x[c] = t / (A[c][c] == 0.0 ? 1.0 : A[c][c]);
+  }
+#pragma unroll
+  for (int i = 0; i < DIIS_MAX; ++i)
+    if (i < N) ok = ok && isfinite(x[i]);
+  const bool use = ok && count >= 2;
+#pragma unroll
+  for (int i = 0; i < DIIS_MAX; ++i)
+    coef[i] = i < N ? (use ? (i < count ? x[i] : 0.0) : (i == idx ? 1.0 : 0.0)) : 0.0;
+}
+
 static int g_mu_blocks = 0;
 
 // Co-resident grid for the cooperative launch, sized to ~8 eigenvalues per
@@ -815,6 +906,13 @@ int hfx_solve_mu_steps(const double* eps, const double* w, size_t nk, int nb, do
     return check_launch("mu_occ_kernel");
   }
   return HFX_OK;
+}
+
+int hfx_diis_coefficients(const double* brow, double* B, int nh, int idx, int count, double* coef, void* stream) {
+  if (!brow || !B || !coef || nh < 1 || nh > DIIS_MAX || idx < 0 || idx >= nh || count < 1 || count > nh)
+    return set_error(HFX_EINVAL, "hfx_diis_coefficients: bad argument (nh=%d idx=%d count=%d)", nh, idx, count);
+  hipLaunchKernelGGL(diis_coef_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, brow, B, nh, idx, count, coef);
+  return check_launch("diis_coef_kernel");
 }
 
 size_t hfx_batch_scratch_doubles(int nseg) {

@@ -425,6 +425,12 @@ class _DIISPolish:
         # one readback buffer: outs [0:2] residuals, [2:2+nh] B row, [16] E; then the state vector
         self.io = torch.zeros(64 + K.ST_LEN, dtype=torch.float64, device=dev)
         self.outs, self.st = self.io[:64], self.io[64:]
+        self.io_host = torch.empty(self.io.shape, dtype=torch.float64, pin_memory=True)
+        self.ev = torch.cuda.Event()
+        self.Bd = torch.zeros((nh, nh), dtype=torch.float64, device=dev)   # DIIS B matrix
+        self.coef = torch.zeros(8, dtype=torch.float64, device=dev)        # DIIS coefficients
+        import ctypes as C
+        self._xa = (C.c_void_p * nh)(*[f.data_ptr() for f in self.F_hist])
         self.graph = None
 
     # Newton steps per mu solve inside the update graph: from the previous mu a
@@ -458,11 +464,57 @@ class _DIISPolish:
     def _capture(self):
         return capture_graphs(self.dev, self._update)[0]
 
+    def _eval(self, k):
+        """F_k = F[P_k] with E_k, the residuals and the new B row of iterate k
+        (P_cur), reduced into outs; then the readback, asynchronous."""
+        nh, part = self.nh, self.wsp.part
+        idx = k % nh
+        Fk, Rk = self.F_hist[idx], self.R_hist[idx]
+        self.fock.build(self.P_cur, Fk, E_ptr=self.outs.data_ptr() + 8 * 16)
+        # residuals and the B row (reference_scf.py:110-111) in one pass, one reduction
+        K.scf_resid_dots(Fk, self.P_cur, self.P_prev if k > 0 else None, Rk, self.R_hist, idx, part)
+        K.reduce(self.wsp, part, 2 + nh, self.nk, w=self.w, wmask=(1 << (2 + nh)) - 1, out=self.outs[0:2 + nh])
+        self.io_host.copy_(self.io, non_blocking=True)
+        self.ev.record()
+
+    def _advance(self, k, warm):
+        """Update from iterate k, launches only: DIIS coefficients on the device
+        (reference_scf.py:298-314), extrapolation in the basis U, eigh, mu,
+        density; then P_prev <- P_k, P_cur <- P_{k+1}."""
+        nh = self.nh
+        idx = k % nh
+        if warm:
+            count = min(k + 1, nh)           # active history slots 0 .. count-1
+            K.diis_coefficients(self.outs[2:2 + nh], self.Bd, idx, count, self.coef)
+            import ctypes as C
+            L.call("hfx_extrap_rotate_batched", K.dt_code(self.U), self.nb, self.nk, self.nk, count,
+                   C.cast(self._xa, C.c_void_p), L.ptr(self.coef), L.ptr(self.U), L.ptr(self.Ft),
+                   L.stream(self.dev))
+            if self.graph is None:
+                self._update()
+                self.graph = self._capture()
+            else:
+                self.graph.replay()
+        else:                             # first update of a cold run: the plain step, eigh of herm(F_0)
+            K.diis_coefficients(self.outs[2:2 + nh], self.Bd, idx, 1, self.coef)
+            self._extrapolate([self.F_hist[idx]], [1.0], False)
+            self._update(warm=False)
+        self.P_prev, self.P_cur = self.P_cur, self.P_prev
+        K.lincomb(self.P_cur, self.P_new, 1.0)
+
     def run(self, P_lo, mu0, tol, max_iter, Q_lo=None):
         """Q_lo (optional): the orbitals P_lo was built from.  Re-orthonormalised in
         FP64 (two Newton-Schulz steps), they are the first basis, so that the
-        first update is a warm eigh as well."""
-        nh, nk = self.nh, self.nk
+        first update is a warm eigh as well.
+
+        Pipelined: the update from iterate k is launched before iterate k's
+        residuals are back on the host (the DIIS coefficients are computed on
+        the device), so the host's stopping test overlaps the update instead of
+        stalling the GPU.  When the test stops at k, that update is discarded
+        (iterate k is kept in P_prev); when the residual trend predicts the
+        stop, the update waits for the test instead."""
+        nh = self.nh
+        max_iter = max(int(max_iter), 1)
         K.hermitize(convert(P_lo, torch.complex128) if P_lo.dtype != torch.complex128 else P_lo, self.P_cur)
         self.st[K.ST_MU] = mu0
         warm0 = Q_lo is not None
@@ -471,67 +523,64 @@ class _DIISPolish:
             L.call("hfx_convert", L.ptr(Qc), KIND[Qc.dtype], L.ptr(self.U), KIND[torch.complex128], Qc.numel(),
                    L.stream(self.dev))
             K.orthonormalize(self.U, 2)
-        part, outs = self.wsp.part, self.outs
-        o_rb = outs[0:2 + nh]                  # residuals [0:2], then the B row [2:2 + nh]
-        E_ptr = outs.data_ptr() + 8 * 16
-        B = np.zeros((nh, nh))
-        hE, hC = [], []
-        k, converged, E, idx, redo = 0, False, float("nan"), 0, False
-        while k < max_iter:
+        hE, hC, hD = [], [], []
+        k, converged, E, idx, redo = 0, False, float("nan"), 0, -1
+        P_res = self.P_cur
+        self._eval(0)
+        while True:
             idx = k % nh
-            Fk, Rk = self.F_hist[idx], self.R_hist[idx]
-            self.fock.build(self.P_cur, Fk, E_ptr=E_ptr)
-            # residuals and the B row (reference_scf.py:110-111) in one pass, one reduction
-            K.scf_resid_dots(Fk, self.P_cur, self.P_prev if k > 0 else None, Rk, self.R_hist, idx, part)
-            K.reduce(self.wsp, part, 2 + nh, nk, w=self.w, wmask=(1 << (2 + nh)) - 1, out=o_rb)
-            sv_all = self.io.cpu().numpy()                                # one readback per iteration
-            if k > 0 and sv_all[64 + K.ST_MUDONE] == 0.0 and not redo:
-                # the last update's mu needed more than the graph's steps: finish
-                # it, rebuild that density and evaluate this iteration again
-                self._mu()
-                K.density(self.U, self.tmp, p=self.occ)
-                K.hermitize(self.tmp, self.P_cur)
-                redo = True
-                continue
-            redo = False
+            Pk = self.P_cur
+            # speculate unless the residual trend says iterate k may be the last
+            spec = k + 1 < max_iter
+            if spec and len(hC) >= 2 and hC[-2] > 0 and 0 < hD[-2] < float("inf"):
+                pc = hC[-1] * hC[-1] / hC[-2]
+                pd = hD[-1] * hD[-1] / hD[-2]
+                spec = not (pc <= 2.0 * tol and pd <= 2.0 * tol)
+            if spec:
+                self._advance(k, warm=k > 0 or warm0)
+            self.ev.synchronize()
+            sv_all = self.io_host.numpy()
             sv = sv_all[:64]
             c_res = float(np.sqrt(sv[0] / self.ws_sum))
             d_res = float(np.sqrt(sv[1] / self.ws_sum)) if k > 0 else float("inf")
             E = float(sv[16])
+            if k > 0 and sv_all[64 + K.ST_MUDONE] == 0.0 and redo != k:
+                # the update that made P_k needed more mu steps than its graph has:
+                # P_k's electron count is off.  Finish mu on the newest eigenvalues
+                # and rebuild that density (after the speculative update: P_{k+1};
+                # else P_k, which is then evaluated again).
+                self._mu()
+                K.density(self.U, self.tmp, p=self.occ)
+                K.hermitize(self.tmp, self.P_cur)
+                if not spec:
+                    redo = k                  # once per iterate
+                    self._eval(k)
+                    continue
+                hE.append(E)
+                hC.append(c_res)
+                hD.append(d_res)
+                k += 1
+                self._eval(k)
+                continue
             hE.append(E)
             hC.append(c_res)
+            hD.append(d_res)
             if c_res <= tol and d_res <= tol:
                 converged = True
+                P_res = Pk
                 break
-            row = sv[2:2 + nh]
-            B[idx, :] = row
-            B[:, idx] = row
-            count = min(k + 1, nh)
-            active = np.arange(nh) < count
-            coeffs, ok = diis_solve(B * np.outer(active, active), active)
-            if count >= 2 and ok:
-                sel = [i for i in range(nh) if active[i] and coeffs[i] != 0.0]
-                self._extrapolate([self.F_hist[i] for i in sel], [float(coeffs[i]) for i in sel], k > 0 or warm0)
-            else:
-                self._extrapolate([Fk], [1.0], k > 0 or warm0)
-            if k == 0 and not warm0:
-                self._update(warm=False)
-            elif self.graph is None:
-                self._update()
-                self.graph = self._capture()
-            else:
-                self.graph.replay()
-            self.P_prev, self.P_cur = self.P_cur, self.P_prev
-            K.lincomb(self.P_cur, self.P_new, 1.0)
+            if k + 1 >= max_iter:
+                break
+            if not spec:
+                self._advance(k, warm=k > 0 or warm0)
             k += 1
+            self._eval(k)
         n_it = len(hE)
-        Ffin = self.F_hist[idx]
-        if not converged and k > 0:     # budget spent after an update: evaluate the returned density
-            Ffin = self.Fx
-            self.fock.build(self.P_cur, Ffin, E_ptr=E_ptr)
-            E = float(outs[16].item())
+        Ffin = self.F_hist[idx]           # F of the returned iterate (converged, or the last one)
+        if not converged:
+            P_res = self.P_cur
         # orbitals / occupations / mu of the returned state (solver.py:606-638 form)
-        warm = k > 0 or warm0
+        warm = k > 0 or warm0 or spec
         if warm:
             K.extrap_rotate(self.Ft, self.U, [Ffin], [1.0])
             K.eigh(self.Ft, self.eps, self.W, herm_in=True)
@@ -543,7 +592,7 @@ class _DIISPolish:
         if warm:
             K.rotate_density(self.U, self.W, self.occ, self.tmp)
         Q = self.U.clone()
-        return (self.P_cur.clone(), Ffin.clone(), Q, self.occ.clone(), self.st[K.ST_MU].clone(), E, n_it,
+        return (P_res.clone(), Ffin.clone(), Q, self.occ.clone(), self.st[K.ST_MU].clone(), E, n_it,
                 converged, hE, hC)
 
 

@@ -101,6 +101,42 @@ def test_matmul_operand_forms(dev, nb, dt):
             assert np.abs(_np(out) - ref).max() <= tol * scale, (ops, use_c)
 
 
+def test_diis_coefficients_match_host_solve(dev):
+    """Device C-DIIS (B update + masked augmented solve) against the host
+    diis_solve, which is pinned to the closed form B^-1 1 / (1^T B^-1 1)."""
+    import torch
+    from jax_hf_amd import _ops as K
+    from jax_hf_amd.reference_scf import diis_solve
+    rng = np.random.default_rng(11)
+    nh = 6
+    Bd = torch.zeros((nh, nh), dtype=torch.float64, device=dev)
+    Bh = np.zeros((nh, nh))
+    coef = torch.zeros(8, dtype=torch.float64, device=dev)
+    R = rng.standard_normal((12, 40)) * np.logspace(0, -3, 12)[:, None]   # shrinking residuals
+    slots = [None] * nh
+    for k in range(12):
+        idx, count = k % nh, min(k + 1, nh)
+        slots[idx] = R[k]
+        row = np.array([R[k] @ slots[j] if slots[j] is not None else 0.0 for j in range(nh)])
+        Bh[idx, :] = row
+        Bh[:, idx] = row
+        K.diis_coefficients(torch.from_numpy(row).to(dev), Bd, idx, count, coef)
+        active = np.arange(nh) < count
+        cs, ok = diis_solve(Bh * np.outer(active, active), active)
+        ref = np.zeros(8)
+        if count >= 2 and ok:
+            ref[:nh] = np.where(active, cs, 0.0)
+        else:
+            ref[idx] = 1.0
+        got = _np(coef)
+        assert np.abs(got - ref).max() <= 1e-7 * max(1.0, np.abs(ref).max()), k
+        assert np.allclose(_np(Bd), Bh, rtol=0, atol=0)
+    # exactly singular system (all-zero B): plain step
+    Bd.zero_()
+    K.diis_coefficients(torch.zeros(nh, dtype=torch.float64, device=dev), Bd, 2, 4, coef)
+    assert np.array_equal(_np(coef), np.eye(8)[2])
+
+
 @pytest.mark.parametrize("nb", [5, 16])
 def test_orthonormalize_fp32_basis(dev, nb):
     """Two Newton-Schulz steps take an FP32-rounded unitary to FP64 unitarity,
