@@ -178,6 +178,13 @@ int hfx_cmul(int dtype, size_t nk, int nch, const void* X, const void* Y, double
 /* out[k, c] = alpha w[k] X[k, c] (w may be NULL): the k-weighting of adjoint seeds. */
 int hfx_scale_k(size_t nk, int nch, int kind, const void* X, const double* w, double alpha, void* out,
                 void* stream);
+/* out[k] = alpha (*alpha_dev) w[k] op(X[k]) for nk complex nb x nb matrices
+ * (kind 2 = complex128, 3 = complex64): op = conjugate transpose when dagger
+ * (out must not alias X then); alpha_dev (a device scalar, e.g. the incoming
+ * gradient of the energy) and w optional.  The w P^dagger terms of the
+ * build_fock / hf_energy adjoint (jax autodiff of fock.py:100-111). */
+int hfx_scale_k_ex(size_t nk, int nb, int kind, const void* X, const double* w, double alpha, const double* alpha_dev,
+                   int dagger, void* out, void* stream);
 
 /* Y = 0.5 (X + X^dagger) over nmat nb x nb matrices (fock.py:13-15, 95-96). */
 int hfx_hermitize(const void* X, void* Y, size_t nmat, int nb, int dtype, void* stream);

@@ -44,6 +44,7 @@ _SIGS = {
     "hfx_fft2_weighted": [vp, vp, vp, vp, vp],
     "hfx_cmul": [i32, sz, i32, vp, vp, f64, f64, i32, vp, vp],
     "hfx_scale_k": [sz, i32, i32, vp, vp, f64, vp, vp],
+    "hfx_scale_k_ex": [sz, i32, i32, vp, vp, f64, vp, i32, vp, vp],
     "hfx_density_moments": [vp, vp, vp, vp, vp, vp],
     "hfx_kterms": [vp, vp, vp, i32, vp, vp, vp, vp, vp, vp],
     "hfx_fock_epilogue": [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, f64, vp],

@@ -34,15 +34,6 @@ from ._util import KIND, convert, to_tensor
 __all__ = ["HFEnergy", "hf_energy_autograd", "free_energy_gradients", "vr_from_vq"]
 
 
-def _dag(X):
-    """X^dagger = 2 herm(X) - X (device kernels only)."""
-    H = torch.empty_like(X)
-    K.hermitize(X, H)
-    out = torch.empty_like(X)
-    K.lincomb(out, H, 2.0, X, -1.0)
-    return out
-
-
 def _scale_k(X, w, alpha):
     nb2 = X.shape[-1] * X.shape[-2]
     out = torch.empty_like(X)
@@ -60,13 +51,11 @@ class HFEnergy(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, P, h, VR, spec):
-        from .fock import FockEngine
         P = P.detach().contiguous()
         h = h.detach().contiguous()
         VR = VR.detach().contiguous()
         refP, HH, w = spec.get("refP"), spec.get("HH"), spec["w2d"]
-        eng = FockEngine(h=h, VR=VR, refP=refP, HH=HH, w2d=w, include_exchange=True,
-                         include_hartree=HH is not None, herm=False)
+        eng = _forward_engine(h, VR, refP, HH, w)
         F = torch.empty_like(P)
         Sig = torch.empty_like(P)
         eng.build(P, F, Sigma_out=Sig, energy=True)
@@ -81,31 +70,40 @@ class HFEnergy(torch.autograd.Function):
         spec = ctx.spec
         w = spec["w2d"]
         refP, HH = spec.get("refP"), spec.get("HH")
-        g = float(gE.detach().cpu())
+        # the incoming gradient stays on the device (no host round trip): the
+        # scaling kernels read it as a device scalar
+        g = gE.detach().to(device=P.device, dtype=torch.float64).reshape(1).contiguous()
         nk1, nk2, nb = P.shape[0], P.shape[1], P.shape[-1]
+        nkt, kind = nk1 * nk2, KIND[P.dtype]
         plan = L.plan_for(nk1, nk2, nb, P.dtype, P.device)
         st = L.stream(P.device)
-        Pd = _dag(P)
-        grad_h = _scale_k(Pd, w, g) if ctx.needs_input_grad[1] else None
+        wp = L.ptr(w)
+
+        def scaled(X, alpha, gdev, dagger, out=None):
+            out = torch.empty_like(X) if out is None else out
+            L.call("hfx_scale_k_ex", nkt, X.shape[-1], kind, L.ptr(X), wp if dagger else None, float(alpha),
+                   L.ptr(gdev), int(dagger), L.ptr(out), st)
+            return out
+
+        grad_h = scaled(P, 1.0, g, True) if ctx.needs_input_grad[1] else None      # g w P^dagger
         grad_P = grad_VR = None
-        G = _scale_k(Pd, w, 0.5)                          # seed of the exchange term
+        G = scaled(P, 0.5, None, True)                     # seed of the exchange term: w P^dagger / 2
         if ctx.needs_input_grad[0]:
             X = torch.empty_like(P)
             K.lincomb(X, h, 1.0, Sig, 0.5)               # h + Sigma/2
             if ctx.Hdiag is not None:
                 X = _add_diag(X, ctx.Hdiag, 0.5)
-            gP = _scale_k(_dag(X), w, 1.0)
+            gP = scaled(X, 1.0, None, True)
             AH = torch.empty_like(P)
             vfull = 1 if VR.shape[-1] != 1 else 0
             L.call("hfx_selfenergy_adjoint", plan.handle, L.ptr(G), L.ptr(VR), vfull, L.ptr(AH), st)
             K.lincomb(gP, gP, 1.0, AH, 1.0)
             if HH is not None:
                 gP = _add_hartree_grad(gP, P, refP, HH, w)
-            K.lincomb(gP, gP, g)
-            grad_P = gP
+            grad_P = scaled(gP, 1.0, g, False, out=gP)
         if ctx.needs_input_grad[2]:
             dP = P if refP is None else _sub(P, refP)
-            w1 = torch.ones_like(w)
+            w1 = _ones_like_cached(w)
             Pf = torch.empty_like(P)
             Gf = torch.empty_like(P)
             L.call("hfx_fft2_weighted", plan.handle, L.ptr(dP), L.ptr(w1), L.ptr(Pf), st)
@@ -114,8 +112,41 @@ class HFEnergy(torch.autograd.Function):
             scalar = VR.shape[-1] == 1
             grad_VR = torch.empty(VR.shape, dtype=VR.dtype, device=VR.device)
             L.call("hfx_cmul", 0 if P.dtype == torch.complex128 else 1, nk1 * nk2, nb * nb, L.ptr(Pf), L.ptr(Gf),
-                   -g / N, 0.0, int(scalar), L.ptr(grad_VR), st)
+                   -1.0 / N, 0.0, int(scalar), L.ptr(grad_VR), st)
+            grad_VR = scaled(grad_VR, 1.0, g, False, out=grad_VR)
         return grad_P, grad_h, grad_VR, None
+
+
+# One forward engine, reused while the inputs are the same storage at the same
+# version (the packed channel-major VR and the build buffers are made once).
+_FWD = {"key": None, "eng": None}
+_ONES: dict = {}
+
+
+def _tkey(t):
+    if t is None:
+        return None
+    if isinstance(t, torch.Tensor):
+        return (t.data_ptr(), t._version, tuple(t.shape), t.dtype, str(t.device))
+    return ("host", id(t))
+
+
+def _forward_engine(h, VR, refP, HH, w):
+    from .fock import FockEngine
+    key = (_tkey(h), _tkey(VR), _tkey(refP), _tkey(HH), _tkey(w))
+    if _FWD["key"] != key:
+        _FWD["eng"] = FockEngine(h=h, VR=VR, refP=refP, HH=HH, w2d=w, include_exchange=True,
+                                 include_hartree=HH is not None, herm=False)
+        _FWD["key"] = key
+    return _FWD["eng"]
+
+
+def _ones_like_cached(w):
+    k = (tuple(w.shape), str(w.device))
+    o = _ONES.get(k)
+    if o is None:
+        o = _ONES[k] = torch.ones_like(w)
+    return o
 
 
 def _sub(A, B):

@@ -202,13 +202,24 @@ __global__ void cmul_kernel(size_t nk, int nch, const cx<T>* X, const cx<T>* Y, 
   }
 }
 
-// out[k, c] = alpha w[k] X[k, c]  (per-k weight scaling, complex storage kinds)
-__global__ void scale_k_kernel(size_t nk, int nch, int kind, const void* X, const double* w, double alpha, void* out) {
+// out[k, c] = alpha (*alpha_dev) w[k] X[k, c]  (per-k weight scaling, complex
+// storage kinds; alpha_dev and w optional).  dag_nb > 0: X[k] is dag_nb x dag_nb
+// and out[k] = alpha ... X[k]^dagger (out must not alias X then).
+__global__ void scale_k_kernel(size_t nk, int nch, int kind, const void* X, const double* w, double alpha,
+                               const double* alpha_dev, int dag_nb, void* out) {
   const size_t total = nk * nch;
+  const double a = alpha * (alpha_dev ? *alpha_dev : 1.0);
   for (size_t i = (size_t)blockIdx.x * UTIL_BLOCK + threadIdx.x; i < total; i += (size_t)gridDim.x * UTIL_BLOCK) {
     double re, im;
-    load_any(X, kind, i, re, im);
-    const double s = alpha * (w ? w[i / nch] : 1.0);
+    size_t src = i;
+    if (dag_nb > 0) {
+      const size_t k = i / nch;
+      const int c = (int)(i - k * nch), r = c / dag_nb, q = c - r * dag_nb;
+      src = k * nch + (size_t)q * dag_nb + r;
+    }
+    load_any(X, kind, src, re, im);
+    if (dag_nb > 0) im = -im;
+    const double s = a * (w ? w[i / nch] : 1.0);
     store_any(out, kind, i, s * re, s * im);
   }
 }
@@ -304,7 +315,16 @@ int hfx_scale_k(size_t nk, int nch, int kind, const void* X, const double* w, do
                 void* stream) {
   if (!X || !out || nch < 1 || kind < 0 || kind > 3) return set_error(HFX_EINVAL, "hfx_scale_k: bad argument");
   hipLaunchKernelGGL(scale_k_kernel, dim3(grid_for(nk * nch)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream, nk, nch, kind,
-                     X, w, alpha, out);
+                     X, w, alpha, nullptr, 0, out);
+  return check_launch("scale_k_kernel");
+}
+
+int hfx_scale_k_ex(size_t nk, int nb, int kind, const void* X, const double* w, double alpha, const double* alpha_dev,
+                   int dagger, void* out, void* stream) {
+  if (!X || !out || nb < 1 || kind < 2 || kind > 3 || (dagger && X == out))
+    return set_error(HFX_EINVAL, "hfx_scale_k_ex: bad argument");
+  hipLaunchKernelGGL(scale_k_kernel, dim3(grid_for(nk * (size_t)nb * nb)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream,
+                     nk, nb * nb, kind, X, w, alpha, alpha_dev, dagger ? nb : 0, out);
   return check_launch("scale_k_kernel");
 }
 

@@ -67,6 +67,10 @@ def test_adjoint_matches_finite_differences_and_torch_autograd(dev, layer):
         fd = (Ev(*args_p) - Ev(*args_m)) / (2 * t)
         an = float(torch.sum(torch.real(torch.conj(g) * D)))
         assert abs(fd - an) <= 1e-9 * max(1.0, abs(an)), (name, fd, an)
+    # a non-unit upstream gradient (read on the device by the backward) scales every gradient
+    E2 = hf_energy_autograd(Pt, ht, VRv, w) * 2.5
+    for a, b in zip(torch.autograd.grad(E2, (Pt, ht, VRv)), (gP, gh, gV)):
+        assert torch.allclose(a, 2.5 * b, rtol=1e-14, atol=0)
     # torch autograd of the CPU restatement (jax.grad stand-in)
     Pc = torch.from_numpy(P).requires_grad_(True)
     hc = torch.from_numpy(h).requires_grad_(True)
