@@ -404,6 +404,10 @@ int hfx_fock_epilogue_planar_batched(const hfx_plan* plan, int nsolve, const voi
  * coef[0..7]: the solution when count >= 2 and the solve is regular and
  * finite, else the plain step e_idx.  nh <= 8. */
 int hfx_diis_coefficients(const double* brow, double* B, int nh, int idx, int count, double* coef, void* stream);
+/* The same for nsolve stacked solves: solve s reads brow + s * brow_stride and
+ * owns B + s * nh * nh and coef + s * 8 (the batched polish of a scan). */
+int hfx_diis_coefficients_batched(const double* brow, size_t brow_stride, double* B, int nsolve, int nh, int idx,
+                                  int count, double* coef, void* stream);
 
 /* Scratch (float64) for hfx_reduce_batched / hfx_solve_mu_batched. */
 size_t hfx_batch_scratch_doubles(int nseg);

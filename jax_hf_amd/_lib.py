@@ -84,6 +84,7 @@ _SIGS = {
     "hfx_scf_resid_dots": [i32, i32, sz, vp, vp, vp, vp, vp, i32, i32, vp, vp],
     "hfx_orthonormalize": [i32, i32, sz, vp, i32, vp],
     "hfx_diis_coefficients": [vp, vp, i32, i32, i32, vp, vp],
+    "hfx_diis_coefficients_batched": [vp, sz, vp, i32, i32, i32, i32, vp, vp],
     "hfx_batched_dots": [i32, i32, sz, vp, vp, i32, i32, vp, vp],
     "hfx_vec_dots": [vp, vp, i32, sz, i32, vp, vp],
     "hfx_vec_shift": [vp, sz, vp, f64, vp],

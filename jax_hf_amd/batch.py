@@ -36,7 +36,6 @@ from . import _lib as L
 from . import _ops as K
 from . import fock as fock_mod
 from ._util import KIND, capture_graphs, convert, to_tensor
-from .reference_scf import diis_solve
 from .solver import SolveResult, SolverConfig, _precision
 
 __all__ = ["solve_batch"]
@@ -124,13 +123,16 @@ class _BatchCG:
         self.T = max(float(k0.T), 1e-12)
         c = dict(dtype=cdt, device=dev)
         r = dict(dtype=torch.float64, device=dev)
-        z = lambda: torch.zeros(self.shape4, **c)  # noqa: E731
-        self.Q, self.P, self.F, self.Ft, self.G, self.Gp = z(), z(), z(), z(), z(), z()
-        self.H, self.D, self.V = z(), z(), z()
-        v = lambda: torch.zeros((S, nk1, nk2, nb), **r)  # noqa: E731
+        # every buffer is written before it is read (run() clears the CG history
+        # terms), so none is filled at allocation: at 13 solves of 90^2 x 16 one
+        # stacked matrix is 0.4 GB
+        e = lambda: torch.empty(self.shape4, **c)  # noqa: E731
+        self.Q, self.P, self.F, self.Ft, self.G, self.Gp = e(), e(), e(), e(), e(), e()
+        self.H, self.D, self.V = e(), e(), e()
+        v = lambda: torch.empty((S, nk1, nk2, nb), **r)  # noqa: E731
         self.p, self.eps, self.gp, self.gpp, self.hp, self.dp, self.lam = v(), v(), v(), v(), v(), v(), v()
         self.st = torch.zeros((S, K.ST_LEN), **r)
-        self.part = torch.zeros((8, self.nk_tot), **r)
+        self.part = torch.empty((8, self.nk_tot), **r)
         self.scratch = torch.zeros(int(L.lib().hfx_batch_scratch_doubles(S)), **r)
         self.targets = torch.zeros(S, **r)
 
@@ -146,9 +148,14 @@ class _BatchCG:
     def fock(self, P, F):
         self.fk(P, F, self.st)
 
-    def mu(self):
+    # Newton steps per mu solve inside the captured iteration; a solve that needs
+    # more is finished after the iteration's readback (its done flag is in st)
+    MU_LOOP_STEPS = 8
+
+    def mu(self, steps=None):
+        n = self.cfg.mu_maxiter if steps is None else min(steps, self.cfg.mu_maxiter)
         L.call("hfx_solve_mu_batched", L.ptr(self.eps), L.ptr(self.w_norm), L.ptr(self.targets), self.nk_tot,
-               self.Nk, self.nb, self.T, int(self.cfg.mu_maxiter), L.ptr(self.st), L.ptr(self.p),
+               self.Nk, self.nb, self.T, int(n), L.ptr(self.st), L.ptr(self.p),
                L.ptr(self.scratch), L.stream(self.dev))
 
     def reduce(self, nval, wmask, op, params, out_slot=-1):
@@ -198,7 +205,7 @@ class _BatchCG:
                         [self.T, 0, 0, 0, cfg.bt_shrink, shrink_pow, 0, int(i == cfg.bt_max - 1), int(i == 0)])
         L.call("hfx_cg_retract_gj", dt, nb, nk, L.ptr(self.Q), L.ptr(self.D), L.ptr(self.Ft), L.ptr(st),
                L.ptr(self.eps), L.ptr(self.V), L.ptr(self.lam), seg, sm)      # :586-590
-        self.mu()                                                               # :591-592
+        self.mu(self.MU_LOOP_STEPS)                                             # :591-592
         self._swap()
 
     def _capture(self, shrink_pow):
@@ -258,6 +265,8 @@ class _BatchCG:
             else:
                 self._body(shrink_pow)
             sv = self.st.cpu().numpy()
+            if any(active[s] and sv[s, K.ST_MUDONE] == 0.0 for s in range(S)):
+                self.mu()                 # finish the mu solves the loop's steps left open
             for s in range(S):
                 if not active[s]:
                     continue
@@ -296,11 +305,15 @@ _SL_B = 22      # [22 ...] the new row of the DIIS B matrix (= _SL_RES + 2: one 
 
 class _BatchPolish:
     """FP64 C-DIIS polish of S stacked solves: the batched form of the mixed
-    solve's polish (one eigh per iteration, extrapolation in the previous
-    basis, reference_scf.py:292-314 per solve).  Each solve keeps its own B
-    matrix, coefficients (passed to the extrapolation kernel per solve) and
-    stopping test; a converged solve's state is copied out and it continues
-    with plain steps at its fixed point while the others finish."""
+    solve's polish (one eigh per iteration in a running basis,
+    reference_scf.py:292-314 per solve).  Each solve keeps its own B matrix and
+    coefficients (solved on the device, one thread per solve, and read per
+    solve by the extrapolation kernel) and its own stopping test; a converged
+    solve's iterate is copied out and it keeps iterating with the others.  As
+    in the single polish, the update from iterate k is launched before the
+    host has read iterate k's residuals."""
+
+    MU_STEPS = 10    # Newton steps per mu solve in an update; more are run on the readback
 
     def __init__(self, kernels, n_electrons, nh=6):
         k0 = kernels[0]
@@ -315,21 +328,28 @@ class _BatchPolish:
         self.nh = nh
         c = dict(dtype=cdt, device=dev)
         r = dict(dtype=torch.float64, device=dev)
-        z = lambda: torch.zeros(self.shape4, **c)  # noqa: E731
-        self.P_prev, self.P_cur, self.P_new, self.Ft, self.W, self.U, self.tmp = z(), z(), z(), z(), z(), z(), z()
-        self.Pfin, self.Ffin = z(), z()
-        self.F_hist = [z() for _ in range(nh)]
-        self.R_hist = [z() for _ in range(nh)]
-        self.eps = torch.zeros((S, nk1, nk2, nb), **r)
-        self.occ = torch.zeros_like(self.eps)
+        # written before read (history slots are used only once filled), so not
+        # filled at allocation: one stacked matrix is up to 0.4 GB here
+        e = lambda: torch.empty(self.shape4, **c)  # noqa: E731
+        self.P_prev, self.P_cur, self.P_new, self.Ft, self.W, self.U, self.tmp = e(), e(), e(), e(), e(), e(), e()
+        self.Pfin, self.Ffin = e(), e()
+        self.F_hist = [e() for _ in range(nh)]
+        self.R_hist = [e() for _ in range(nh)]
+        self.eps = torch.empty((S, nk1, nk2, nb), **r)
+        self.occ = torch.empty_like(self.eps)
         self.st = torch.zeros((S, K.ST_LEN), **r)
-        self.part = torch.zeros((8, self.nk_tot), **r)
+        self.part = torch.empty((8, self.nk_tot), **r)
         self.scratch = torch.zeros(int(L.lib().hfx_batch_scratch_doubles(S)), **r)
         self.w_norm = torch.stack([k._w64 / max(k.weight_sum, 1e-30) for k in kernels]).contiguous()
         self.targets = torch.tensor([float(n) / max(k.weight_sum, 1e-30) for n, k in zip(n_electrons, kernels)],
                                     dtype=torch.float64, device=dev)
         self.T = max(float(k0.T), 1e-12)
         self.coef = torch.zeros((S, 8), **r)
+        self.Bd = torch.zeros((S, nh, nh), **r)
+        self.st_host = torch.empty((S, K.ST_LEN), dtype=torch.float64, pin_memory=True)
+        self.ev = torch.cuda.Event()
+        import ctypes as C
+        self._xa = (C.c_void_p * nh)(*[f.data_ptr() for f in self.F_hist])
 
     def _reduce(self, nval, slot):
         L.call("hfx_reduce_batched", L.ptr(self.part), int(nval), self.nk_tot, self.Nk, L.ptr(self.w_norm),
@@ -340,11 +360,47 @@ class _BatchPolish:
                self.Nk, self.nb, self.T, int(maxiter), L.ptr(self.st), L.ptr(self.occ), L.ptr(self.scratch),
                L.stream(self.dev))
 
+    def _eval(self, k):
+        """F, E, residuals and B row of every solve's iterate k; asynchronous readback."""
+        nh = self.nh
+        idx, count = k % nh, min(k + 1, nh)
+        Fk, Rk = self.F_hist[idx], self.R_hist[idx]
+        self.fk(self.P_cur, Fk, self.st)
+        # residuals and the B row (reference_scf.py:110-111) in one pass: slots _SL_RES, _SL_B
+        K.scf_resid_dots(Fk, self.P_cur, self.P_prev if k > 0 else None, Rk, self.R_hist[:count], idx, self.part)
+        self._reduce(2 + count, _SL_RES)
+        self.st_host.copy_(self.st, non_blocking=True)
+        self.ev.record()
+
+    def _advance(self, k, warm):
+        """Update from iterate k for every solve (device DIIS coefficients), then
+        P_prev <- P_k, P_cur <- P_{k+1}."""
+        import ctypes as C
+        nh = self.nh
+        idx, count = k % nh, min(k + 1, nh)
+        L.call("hfx_diis_coefficients_batched", K.st_ptr(self.st, _SL_B), K.ST_LEN, L.ptr(self.Bd), self.S, nh,
+               idx, count, L.ptr(self.coef), L.stream(self.dev))
+        if warm:
+            L.call("hfx_extrap_rotate_batched", K.dt_code(self.U), self.nb, self.nk_tot, self.Nk, count,
+                   C.cast(self._xa, C.c_void_p), L.ptr(self.coef), L.ptr(self.U), L.ptr(self.Ft),
+                   L.stream(self.dev))
+            K.eigh(self.Ft, self.eps, self.W, herm_in=True)
+            self._mu(self.MU_STEPS)
+            K.rotate_density(self.U, self.W, self.occ, self.P_new)
+        else:                             # cold first update: eigh of herm(F_0) (count = 1: plain step)
+            K.lincomb_multi(self.Ft, [self.F_hist[idx]], [1.0], herm=True)
+            K.eigh(self.Ft, self.eps, self.U, herm_in=True)
+            self._mu(self.MU_STEPS)
+            K.density(self.U, self.tmp, p=self.occ)
+            K.hermitize(self.tmp, self.P_new)
+        self.P_prev, self.P_cur = self.P_cur, self.P_prev
+        K.lincomb(self.P_cur, self.P_new, 1.0)
+
     def run(self, P_lo, mu0, tol, max_iter, Q_lo=None):
         """Q_lo (optional): the stacked orbitals P_lo was built from; re-orthonormalised
         in FP64 they are the first basis (the first update is a warm eigh)."""
-        import ctypes as C
         S, nh = self.S, self.nh
+        max_iter = max(int(max_iter), 1)
         K.hermitize(convert(P_lo, torch.complex128) if P_lo.dtype != torch.complex128 else P_lo, self.P_cur)
         self.st.zero_()
         self.st[:, K.ST_MU] = mu0
@@ -354,25 +410,45 @@ class _BatchPolish:
             L.call("hfx_convert", L.ptr(Qc), KIND[Qc.dtype], L.ptr(self.U), KIND[torch.complex128], Qc.numel(),
                    L.stream(self.dev))
             K.orthonormalize(self.U, 2)
-        B = np.zeros((S, nh, nh))
         done = [False] * S
         hE = [[] for _ in range(S)]
         hC = [[] for _ in range(S)]
+        hD = [[] for _ in range(S)]
         E_out = [float("nan")] * S
-        xa = (C.c_void_p * nh)(*[f.data_ptr() for f in self.F_hist])
-        dt = K.dt_code(self.U)
-        k = 0
-        while k < max_iter:
+        k, redo, spec = 0, -1, False
+        self._eval(0)
+        while True:
             idx = k % nh
-            Fk, Rk = self.F_hist[idx], self.R_hist[idx]
-            self.fk(self.P_cur, Fk, self.st)
-            # residuals and the B row (reference_scf.py:110-111) in one pass: slots _SL_RES, _SL_B
-            K.scf_resid_dots(Fk, self.P_cur, self.P_prev if k > 0 else None, Rk, self.R_hist, idx, self.part)
-            self._reduce(2 + nh, _SL_RES)
-            sv = self.st.cpu().numpy()                                    # one readback per iteration
-            coefs = np.zeros((S, 8))
+            Pk = self.P_cur
+            spec = k + 1 < max_iter
+            if spec:                      # wait instead when every open solve is predicted to stop at k
+                preds = []
+                for s in range(S):
+                    if done[s]:
+                        continue
+                    c, d = hC[s], hD[s]
+                    if len(c) >= 2 and c[-2] > 0 and 0 < d[-2] < float("inf"):
+                        preds.append(c[-1] * c[-1] / c[-2] <= 2.0 * tol and d[-1] * d[-1] / d[-2] <= 2.0 * tol)
+                    else:
+                        preds.append(False)
+                spec = not (preds and all(preds))
+            if spec:
+                self._advance(k, warm=k > 0 or warm0)
+            self.ev.synchronize()
+            sv = self.st_host.numpy()
+            bad_mu = k > 0 and redo != k and any(not done[s] and sv[s, K.ST_MUDONE] == 0.0 for s in range(S))
+            if bad_mu:
+                # an update's mu needed more steps than it had: finish mu on the newest
+                # eigenvalues and rebuild those densities (P_{k+1} after the
+                # speculative update, else P_k, evaluated again)
+                self._mu()
+                K.density(self.U, self.tmp, p=self.occ)
+                K.hermitize(self.tmp, self.P_cur)
+                if not spec:
+                    redo = k
+                    self._eval(k)
+                    continue
             for s in range(S):
-                coefs[s, idx] = 1.0                                       # plain step unless DIIS engages
                 if done[s]:
                     continue
                 c_res = float(np.sqrt(max(sv[s, _SL_RES], 0.0)))
@@ -380,48 +456,23 @@ class _BatchPolish:
                 E_out[s] = float(sv[s, K.ST_E])
                 hE[s].append(E_out[s])
                 hC[s].append(c_res)
-                if c_res <= tol and d_res <= tol:                        # this iterate is the result
+                hD[s].append(d_res)
+                if not bad_mu and c_res <= tol and d_res <= tol:     # this iterate is the result
                     done[s] = True
-                    self.Pfin[s].copy_(self.P_cur[s])
-                    self.Ffin[s].copy_(Fk[s])
-                    continue
-                row = sv[s, _SL_B:_SL_B + nh]
-                B[s, idx, :] = row
-                B[s, :, idx] = row
-                count = min(k + 1, nh)
-                active = np.arange(nh) < count
-                cs, ok = diis_solve(B[s] * np.outer(active, active), active)
-                if count >= 2 and ok:
-                    coefs[s, :nh] = np.where(active, cs, 0.0)
-                    coefs[s, nh:] = 0.0
-            if all(done):
+                    self.Pfin[s].copy_(Pk[s])
+                    self.Ffin[s].copy_(self.F_hist[idx][s])
+            if all(done) or k + 1 >= max_iter:
                 break
-            if k == 0 and not warm0:      # cold first update: eigh of herm(F_0)
-                K.lincomb_multi(self.Ft, [Fk], [1.0], herm=True)
-                K.eigh(self.Ft, self.eps, self.U, herm_in=True)
-                self._mu()
-                K.density(self.U, self.tmp, p=self.occ)
-                K.hermitize(self.tmp, self.P_new)
-            else:                         # extrapolation in the previous basis, per-solve coefficients
-                self.coef.copy_(torch.from_numpy(coefs))
-                L.call("hfx_extrap_rotate_batched", dt, self.nb, self.nk_tot, self.Nk, nh, C.cast(xa, C.c_void_p),
-                       L.ptr(self.coef), L.ptr(self.U), L.ptr(self.Ft), L.stream(self.dev))
-                K.eigh(self.Ft, self.eps, self.W, herm_in=True)
-                self._mu()
-                K.rotate_density(self.U, self.W, self.occ, self.P_new)
-            self.P_prev, self.P_cur = self.P_cur, self.P_prev
-            K.lincomb(self.P_cur, self.P_new, 1.0)
+            if not spec:
+                self._advance(k, warm=k > 0 or warm0)
             k += 1
-        if not all(done):                 # budget spent: evaluate the returned densities
-            self.fk(self.P_cur, self.tmp, self.st)
-            sv = self.st.cpu().numpy()
-            for s in range(S):
-                if not done[s]:
-                    self.Pfin[s].copy_(self.P_cur[s])
-                    self.Ffin[s].copy_(self.tmp[s])
-                    E_out[s] = float(sv[s, K.ST_E])
+            self._eval(k)
+        for s in range(S):
+            if not done[s]:               # budget spent: the last evaluated iterate
+                self.Pfin[s].copy_(Pk[s])
+                self.Ffin[s].copy_(self.F_hist[idx][s])
         # orbitals / occupations / mu of the returned states (solver.py:606-638 form)
-        if warm0 or k > 0:                # U holds a basis: warm eigh in it
+        if warm0 or k > 0 or spec:        # U holds a basis: warm eigh in it
             K.extrap_rotate(self.Ft, self.U, [self.Ffin], [1.0])
             K.eigh(self.Ft, self.eps, self.W, herm_in=True)
             self._mu()

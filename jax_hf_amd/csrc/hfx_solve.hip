@@ -738,9 +738,16 @@ __global__ __launch_bounds__(MUS_BLOCK) void mu_occ_seg_kernel(MuSegArgs a) {
 // solve is regular and finite, else the plain step e_idx; coef[nh..7] = 0.
 constexpr int DIIS_MAX = 8;
 constexpr int DIIS_M = DIIS_MAX + 1;   // augmented size; rows past nh + 1 are identity padding
-__global__ __launch_bounds__(64) void diis_coef_kernel(const double* __restrict__ brow, double* __restrict__ B, int nh, int idx,
-                                 int count, double* __restrict__ coef) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Thread t handles solve t of a stack: brow + t * brow_stride, B + t * nh * nh,
+// coef + t * DIIS_MAX.
+__global__ __launch_bounds__(64) void diis_coef_kernel(const double* __restrict__ brow_all, size_t brow_stride,
+                                                       double* __restrict__ B_all, int nsolve, int nh, int idx,
+                                                       int count, double* __restrict__ coef_all) {
+  const int sidx = blockIdx.x * 64 + threadIdx.x;
+  if (sidx >= nsolve) return;
+  const double* brow = brow_all + (size_t)sidx * brow_stride;
+  double* B = B_all + (size_t)sidx * nh * nh;
+  double* coef = coef_all + (size_t)sidx * DIIS_MAX;
   // every loop has a compile-time trip count so that A and x stay in registers
   double row[DIIS_MAX];
 #pragma unroll
@@ -909,9 +916,16 @@ int hfx_solve_mu_steps(const double* eps, const double* w, size_t nk, int nb, do
 }
 
 int hfx_diis_coefficients(const double* brow, double* B, int nh, int idx, int count, double* coef, void* stream) {
-  if (!brow || !B || !coef || nh < 1 || nh > DIIS_MAX || idx < 0 || idx >= nh || count < 1 || count > nh)
+  return hfx_diis_coefficients_batched(brow, 0, B, 1, nh, idx, count, coef, stream);
+}
+
+int hfx_diis_coefficients_batched(const double* brow, size_t brow_stride, double* B, int nsolve, int nh, int idx,
+                                  int count, double* coef, void* stream) {
+  if (!brow || !B || !coef || nsolve < 1 || nh < 1 || nh > DIIS_MAX || idx < 0 || idx >= nh || count < 1 ||
+      count > nh)
     return set_error(HFX_EINVAL, "hfx_diis_coefficients: bad argument (nh=%d idx=%d count=%d)", nh, idx, count);
-  hipLaunchKernelGGL(diis_coef_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, brow, B, nh, idx, count, coef);
+  hipLaunchKernelGGL(diis_coef_kernel, dim3((unsigned)((nsolve + 63) / 64)), dim3(64), 0, (hipStream_t)stream, brow,
+                     brow_stride, B, nsolve, nh, idx, count, coef);
   return check_launch("diis_coef_kernel");
 }
 
