@@ -730,101 +730,105 @@ __global__ __launch_bounds__(MUS_BLOCK) void mu_occ_seg_kernel(MuSegArgs a) {
 }
 
 // ---- Pulay C-DIIS coefficients on the device (reference_scf.py:99-121) ----------
-// One thread: B[idx][:] = B[:][idx] = brow (the new row, already reduced), then
-// the masked, max-normalised augmented system
+// One wavefront per solve: B[idx][:] = B[:][idx] = brow (the new row, already
+// reduced), then the masked, max-normalised augmented system
 //   [ Bn + diag(1 - a)  -a ] [c]   [ 0]
 //   [ -a^T               0 ] [l] = [-1],   a_i = (i < count),
-// solved by LU with partial pivoting.  coef = c when count >= 2 and the
-// solve is regular and finite, else the plain step e_idx; coef[nh..7] = 0.
+// solved by LU with partial pivoting (first maximal pivot, as LAPACK's getrf).
+// Lane r holds row r of the (padded) system in registers; the pivot search,
+// row swap and pivot-row broadcast are wave shuffles, so the dependent chain
+// per column is a few shuffles and one FMA per element.  coef = c when
+// count >= 2 and the solve is regular and finite, else the plain step e_idx;
+// coef[nh..7] = 0.
 constexpr int DIIS_MAX = 8;
 constexpr int DIIS_M = DIIS_MAX + 1;   // augmented size; rows past nh + 1 are identity padding
-// Thread t handles solve t of a stack: brow + t * brow_stride, B + t * nh * nh,
-// coef + t * DIIS_MAX.
 __global__ __launch_bounds__(64) void diis_coef_kernel(const double* __restrict__ brow_all, size_t brow_stride,
                                                        double* __restrict__ B_all, int nsolve, int nh, int idx,
                                                        int count, double* __restrict__ coef_all) {
-  const int sidx = blockIdx.x * 64 + threadIdx.x;
+  const int sidx = blockIdx.x;
   if (sidx >= nsolve) return;
   const double* brow = brow_all + (size_t)sidx * brow_stride;
   double* B = B_all + (size_t)sidx * nh * nh;
   double* coef = coef_all + (size_t)sidx * DIIS_MAX;
-  // every loop has a compile-time trip count so that A and x stay in registers
-  double row[DIIS_MAX];
+  const int r = threadIdx.x;
+  const int N = nh;
+  const double br = r < nh ? brow[r] : 0.0;
+  double a[DIIS_M];
 #pragma unroll
-  for (int j = 0; j < DIIS_MAX; ++j) row[j] = j < nh ? brow[j] : 0.0;
-#pragma unroll
-  for (int j = 0; j < DIIS_MAX; ++j)
-    if (j < nh) {
-      B[idx * nh + j] = row[j];
-      B[j * nh + idx] = row[j];
-    }
-  double A[DIIS_M][DIIS_M], x[DIIS_M];
+  for (int j = 0; j < DIIS_M; ++j) {
+    const double bj = __shfl(br, j);
+    double v = 0.0;
+    if (r < count && j < count) v = r == idx ? bj : (j == idx ? br : B[r * nh + j]);
+    a[j] = v;
+  }
+  if (r < nh) {
+    B[idx * nh + r] = br;
+    B[r * nh + idx] = br;
+  }
   double mx = 0.0;
 #pragma unroll
-  for (int i = 0; i < DIIS_MAX; ++i)
+  for (int j = 0; j < DIIS_M; ++j) mx = fmax(mx, fabs(a[j]));
 #pragma unroll
-    for (int j = 0; j < DIIS_MAX; ++j) {
-      const double b = (i < count && j < count) ? (i == idx ? row[j] : (j == idx ? row[i] : B[i * nh + j])) : 0.0;
-      A[i][j] = b;
-      mx = fmax(mx, fabs(b));
-    }
-  const int N = nh;
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+  mx = fmax(mx, 1e-30);
+  const bool ar = r < count;
 #pragma unroll
-  for (int i = 0; i < DIIS_M; ++i) {
-    const bool ai = i < count;
-#pragma unroll
-    for (int j = 0; j < DIIS_M; ++j) {
-      double a;
-      if (i < N && j < N) a = A[i][j] / fmax(mx, 1e-30) + ((i == j && !ai) ? 1.0 : 0.0);
-      else if (i < N && j == N) a = ai ? -1.0 : 0.0;
-      else if (i == N && j < N) a = (j < count) ? -1.0 : 0.0;
-      else if (i == N && j == N) a = 0.0;
-      else a = (i == j) ? 1.0 : 0.0;          // padding
-      A[i][j] = a;
-    }
-    x[i] = i == N ? -1.0 : 0.0;
+  for (int j = 0; j < DIIS_M; ++j) {
+    double v;
+    if (r < N) v = j < N ? a[j] / mx + ((j == r && !ar) ? 1.0 : 0.0) : (j == N ? (ar ? -1.0 : 0.0) : 0.0);
+    else if (r == N) v = j < N ? (j < count ? -1.0 : 0.0) : 0.0;
+    else v = (j == r) ? 1.0 : 0.0;   // padding rows (and lanes past the system)
+    a[j] = v;
   }
+  double x = r == N ? -1.0 : 0.0;
   bool ok = true;
 #pragma unroll
   for (int c = 0; c < DIIS_M; ++c) {
-    int piv = c;
-    double best = fabs(A[c][c]);
+    // pivot: the first row r >= c with the largest |a[r][c]|
+    double bv = (r >= c && r < DIIS_M) ? fabs(a[c]) : -1.0;
+    int bi = r;
 #pragma unroll
-    for (int r = c + 1; r < DIIS_M; ++r) {
-      const double v = fabs(A[r][c]);
-      if (v > best) { best = v; piv = r; }
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(bv, o);
+      const int oi = __shfl_xor(bi, o);
+      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
     }
-    if (best == 0.0) ok = false;
+    if (bv == 0.0) ok = false;
+    const int src = r == c ? bi : (r == bi ? c : r);
 #pragma unroll
-    for (int r = c + 1; r < DIIS_M; ++r)
-      if (r == piv) {
+    for (int j = c; j < DIIS_M; ++j) a[j] = __shfl(a[j], src);
+    x = __shfl(x, src);
+    double pj[DIIS_M];
 #pragma unroll
-        for (int j = c; j < DIIS_M; ++j) { const double t = A[c][j]; A[c][j] = A[r][j]; A[r][j] = t; }
-        const double t = x[c]; x[c] = x[r]; x[r] = t;
-      }
-    const double pv = best == 0.0 ? 1.0 : A[c][c];
+    for (int j = c; j < DIIS_M; ++j) pj[j] = __shfl(a[j], c);
+    const double px = __shfl(x, c);
+    const double inv = bv == 0.0 ? 0.0 : 1.0 / pj[c];
+    if (r > c && r < DIIS_M) {
+      const double f = a[c] * inv;
 #pragma unroll
-    for (int r = c + 1; r < DIIS_M; ++r) {
-      const double f = A[r][c] / pv;
-#pragma unroll
-      for (int j = c; j < DIIS_M; ++j) A[r][j] -= f * A[c][j];
-      x[r] -= f * x[c];
+      for (int j = c; j < DIIS_M; ++j) a[j] -= f * pj[j];
+      x -= f * px;
     }
   }
+  double xs[DIIS_M];
 #pragma unroll
   for (int c = DIIS_M - 1; c >= 0; --c) {
-    double t = x[c];
+    double t = x;
 #pragma unroll
-    for (int j = c + 1; j < DIIS_M; ++j) t -= A[c][j] * x[j];
-    x[c] = t / (A[c][c] == 0.0 ? 1.0 : A[c][c]);
+    for (int j = c + 1; j < DIIS_M; ++j) t -= a[j] * xs[j];
+    xs[c] = __shfl(t / (a[c] == 0.0 ? 1.0 : a[c]), c);
   }
 #pragma unroll
   for (int i = 0; i < DIIS_MAX; ++i)
-    if (i < N) ok = ok && isfinite(x[i]);
+    if (i < N) ok = ok && isfinite(xs[i]);
   const bool use = ok && count >= 2;
+  if (r < DIIS_MAX) {
+    double mine = 0.0;
 #pragma unroll
-  for (int i = 0; i < DIIS_MAX; ++i)
-    coef[i] = i < N ? (use ? (i < count ? x[i] : 0.0) : (i == idx ? 1.0 : 0.0)) : 0.0;
+    for (int j = 0; j < DIIS_MAX; ++j)
+      if (j == r) mine = xs[j];
+    coef[r] = r < N ? (use ? (r < count ? mine : 0.0) : (r == idx ? 1.0 : 0.0)) : 0.0;
+  }
 }
 
 static int g_mu_blocks = 0;
@@ -924,8 +928,8 @@ int hfx_diis_coefficients_batched(const double* brow, size_t brow_stride, double
   if (!brow || !B || !coef || nsolve < 1 || nh < 1 || nh > DIIS_MAX || idx < 0 || idx >= nh || count < 1 ||
       count > nh)
     return set_error(HFX_EINVAL, "hfx_diis_coefficients: bad argument (nh=%d idx=%d count=%d)", nh, idx, count);
-  hipLaunchKernelGGL(diis_coef_kernel, dim3((unsigned)((nsolve + 63) / 64)), dim3(64), 0, (hipStream_t)stream, brow,
-                     brow_stride, B, nsolve, nh, idx, count, coef);
+  hipLaunchKernelGGL(diis_coef_kernel, dim3((unsigned)nsolve), dim3(64), 0, (hipStream_t)stream, brow, brow_stride, B,
+                     nsolve, nh, idx, count, coef);
   return check_launch("diis_coef_kernel");
 }
 
