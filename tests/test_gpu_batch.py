@@ -64,6 +64,23 @@ def test_solve_batch_mixed_reaches_fixed_point(dev):
         assert abs(float(rb.energy) - float(rs.energy)) < 1e-9
 
 
+def test_solve_batch_mixed_mu_completion_paths(dev, monkeypatch):
+    """One Newton mu step per captured batched mu solve forces the completion
+    on the readback (CG) and the density repair (pipelined polish); the batch
+    still lands on the serial solves' fixed points."""
+    import jax_hf_amd as J
+    from jax_hf_amd import batch as B
+    kernels, P0s, ne = _scan([12, 12, 16], nb=4, seed=7)
+    cfg = J.SolverConfig(max_iter=300, precision="mixed")
+    serial = [J.solve(k, P0, n, config=cfg) for k, P0, n in zip(kernels, P0s, ne)]
+    monkeypatch.setattr(B._BatchCG, "MU_LOOP_STEPS", 1)
+    monkeypatch.setattr(B._BatchPolish, "MU_STEPS", 1)
+    batch = J.solve_batch(kernels, P0s, ne, config=cfg)
+    for rs, rb in zip(serial, batch):
+        assert rb.converged and rs.converged
+        assert abs(float(rb.energy) - float(rs.energy)) < 1e-9
+
+
 def test_solve_batch_falls_back_for_unbatchable(dev):
     """A Hartree kernel runs through solve() and gives solve()'s result."""
     import jax_hf_amd as J

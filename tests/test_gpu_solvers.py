@@ -571,3 +571,46 @@ def test_newton_mixed_precision_matches_fp64(dev, nk, nb, T, ex, seed):
     assert abs(float(rmx.energy) - float(r64.energy)) < 1e-9 * nk * nk
     assert abs(float(rmx.energy) - r0.energy) < 1e-9 * nk * nk
     assert np.abs(_np(rmx.density) - _np(r64.density)).max() < 1e-5
+
+
+def test_mixed_solve_mu_completion_paths(dev, monkeypatch):
+    """One Newton mu step per captured solve (instead of 6-8) forces the
+    completion paths: the CG loop finishes mu on its readback, the pipelined
+    polish repairs the density after a speculative update.  The result is the
+    same fixed point as with the default step counts."""
+    import jax_hf_amd as J
+    from jax_hf_amd import solver as SV
+    c = syn.config0(nk=24, nb=4, seed=0)
+    cfg = J.SolverConfig(max_iter=400, precision="mixed", polish_tol=1e-9)
+    G = J.HartreeFockKernel(c["weights"], c["hamiltonian"], c["coulomb_q"], c["T"])
+    ref = J.solve(G, c["P0"], c["n_electrons"], config=cfg)
+    monkeypatch.setattr(SV._CG, "MU_LOOP_STEPS", 1)
+    monkeypatch.setattr(SV._DIISPolish, "MU_GRAPH_STEPS", 1)
+    G2 = J.HartreeFockKernel(c["weights"], c["hamiltonian"], c["coulomb_q"], c["T"])
+    r = J.solve(G2, c["P0"], c["n_electrons"], config=cfg)
+    assert ref.converged and r.converged
+    assert abs(float(r.energy) - float(ref.energy)) < 1e-10
+    assert np.abs(_np(r.density) - _np(ref.density)).max() < 1e-7
+    # the returned occupations hold the electron count
+    w = c["weights"] / c["weights"].sum()
+    assert abs(float(np.sum(w[..., None] * _np(r.p))) - c["n_electrons"] / c["weights"].sum()) < 1e-10
+
+
+def test_polish_budget_spent_returns_last_iterate(dev):
+    """With a polish budget too small to converge, the mixed solve returns its
+    last evaluated iterate: not converged, and energy / Fock matrix those of
+    the returned density."""
+    import jax_hf_amd as J
+    c = syn.config0(nk=24, nb=4, seed=0)
+    G = J.HartreeFockKernel(c["weights"], c["hamiltonian"], c["coulomb_q"], c["T"])
+    r = J.solve(G, c["P0"], c["n_electrons"], config=J.SolverConfig(max_iter=400, precision="mixed",
+                                                                       polish_tol=1e-13, polish_max_iter=2))
+    assert not r.converged
+    a = G.as_args()
+    S, H, F = J.build_fock(r.density, h=a["h"], VR=a["VR"], refP=a["refP"], HH=a["HH"], w2d=G.w2d,
+                           include_exchange=a["include_exchange"], include_hartree=a["include_hartree"],
+                           exchange_hermitian_channel_packing=a["exchange_hermitian_channel_packing"],
+                           contact_g=a["contact_g"], contact_Oi=a["contact_Oi"], contact_Oj=a["contact_Oj"])
+    assert np.abs(_np(F) - _np(r.fock)).max() < 1e-12
+    E = float(J.hf_energy(r.density, h=G.h, Sigma=S, H=H, weights_b=G.weights_b))
+    assert abs(E - float(r.energy)) < 1e-12
