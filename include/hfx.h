@@ -228,6 +228,13 @@ int hfx_entropy(const void* p, int real_kind, const double* w, size_t nk, int nb
  * hermitises A first.  dtype: HFX_C128 / HFX_C64. */
 int hfx_eigh_batched(int dtype, int nb, size_t nk, const void* A, double* lam, void* V, int herm_in,
                      void* stream);
+/* hfx_eigh_batched with the Jacobi rotations stopped once every off-diagonal
+ * |A_pq| <= rel_tol ||A||_F (rel_tol below machine epsilon: the full
+ * precision of hfx_eigh_batched).  For eigendecompositions inside an
+ * iteration whose own residual test decides convergence (the DIIS polish
+ * updates); rel_tol <= 1e-3. */
+int hfx_eigh_batched_tol(int dtype, int nb, size_t nk, const void* A, double* lam, void* V, int herm_in,
+                         double rel_tol, void* stream);
 
 /* P = herm(Q diag(p) Q^dagger) (solver.py:42-44 + 462, utils.py:473-474);
  * when p is NULL, p = sigmoid((*mu - eps)/T) is formed (and stored to pout). */

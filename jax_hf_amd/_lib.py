@@ -57,6 +57,7 @@ _SIGS = {
     "hfx_absmax_masked": [vp, sz, i32, i32, vp, vp, vp, vp],
     "hfx_entropy": [vp, i32, vp, sz, i32, vp, f64, vp, vp, vp],
     "hfx_eigh_batched": [i32, i32, sz, vp, vp, vp, i32, vp],
+    "hfx_eigh_batched_tol": [i32, i32, sz, vp, vp, vp, i32, f64, vp],
     "hfx_density_from_Qp": [i32, i32, sz, vp, vp, vp, vp, f64, vp, vp, vp],
     "hfx_cg_grad": [i32, i32, sz, vp, vp, vp, vp, vp, vp, vp, vp],
     "hfx_cg_precond": [i32, i32, sz, vp, vp, vp, vp, vp, vp, f64, vp, vp, vp, vp, i32, vp],

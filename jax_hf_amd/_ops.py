@@ -44,10 +44,11 @@ def s(dev):
     return L.stream(dev)
 
 
-def eigh(A, lam, V, herm_in=False):
+def eigh(A, lam, V, herm_in=False, rel_tol=0.0):
+    """Batched Hermitian eigh; rel_tol > 0 stops the Jacobi rotations at |A_pq| <= rel_tol ||A||_F."""
     nk = A.numel() // (A.shape[-1] * A.shape[-1])
-    L.call("hfx_eigh_batched", dt_code(A), A.shape[-1], nk, L.ptr(A), L.ptr(lam), L.ptr(V), int(herm_in),
-           s(A.device))
+    L.call("hfx_eigh_batched_tol", dt_code(A), A.shape[-1], nk, L.ptr(A), L.ptr(lam), L.ptr(V), int(herm_in),
+           float(rel_tol), s(A.device))
 
 
 def density(Q, P, p=None, eps=None, mu_ptr=None, T=1.0, pout=None):

@@ -314,6 +314,7 @@ class _BatchPolish:
     host has read iterate k's residuals."""
 
     MU_STEPS = 10    # Newton steps per mu solve in an update; more are run on the readback
+    EIGH_TOL_PER_TOL = 1e-5   # update eigh tolerance per unit of the polish tolerance (as in solve)
 
     def __init__(self, kernels, n_electrons, nh=6):
         k0 = kernels[0]
@@ -384,7 +385,7 @@ class _BatchPolish:
             L.call("hfx_extrap_rotate_batched", K.dt_code(self.U), self.nb, self.nk_tot, self.Nk, count,
                    C.cast(self._xa, C.c_void_p), L.ptr(self.coef), L.ptr(self.U), L.ptr(self.Ft),
                    L.stream(self.dev))
-            K.eigh(self.Ft, self.eps, self.W, herm_in=True)
+            K.eigh(self.Ft, self.eps, self.W, herm_in=True, rel_tol=self._eigh_tol)
             self._mu(self.MU_STEPS)
             K.rotate_density(self.U, self.W, self.occ, self.P_new)
         else:                             # cold first update: eigh of herm(F_0) (count = 1: plain step)
@@ -401,6 +402,7 @@ class _BatchPolish:
         in FP64 they are the first basis (the first update is a warm eigh)."""
         S, nh = self.S, self.nh
         max_iter = max(int(max_iter), 1)
+        self._eigh_tol = self.EIGH_TOL_PER_TOL * float(tol)
         K.hermitize(convert(P_lo, torch.complex128) if P_lo.dtype != torch.complex128 else P_lo, self.P_cur)
         self.st.zero_()
         self.st[:, K.ST_MU] = mu0

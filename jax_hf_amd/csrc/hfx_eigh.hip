@@ -19,10 +19,11 @@
 // t = sign(zeta)/(|zeta| + sqrt(1 + zeta^2)), cs = 1/sqrt(1 + t^2), sn = t cs,
 // J_pp = J_qq = cs, J_pq = sn e^{i phi}, J_qp = -sn e^{-i phi}; afterwards
 // A_pp = A_pp - t|c|, A_qq = A_qq + t|c|, A_pq = 0 exactly.  Threshold
-// strategy: a pair rotates only while |A_pq| > eps ||A||_F (below that the
-// element is a perturbation under the rounding of the eigenvalues); the sweeps
-// end after the first sweep without a rotation in either of the wave's
-// matrices; a step in which no pair of the wave rotates is skipped.
+// strategy: a pair rotates only while |A_pq| > tol ||A||_F, tol = eps (below
+// eps ||A||_F the element is a perturbation under the rounding of the
+// eigenvalues) or a caller's larger relative tolerance; the sweeps end after
+// the first sweep without a rotation in either of the wave's matrices; a step
+// in which no pair of the wave rotates is skipped.
 // Eigenvalues ascending, eigenvectors in the columns of V (the
 // reference's jnp.linalg.eigh convention, linalg.py:96-160, reference_scf.py:89).
 #include "hfx_common.h"
@@ -157,7 +158,8 @@ __device__ __forceinline__ void j16_sweep(cx<T> (&b)[8], cx<T> (&w)[8], int m, i
 
 template <typename T, int P>
 __global__ __launch_bounds__(64) void eigh16_kernel(int nb, size_t nk, const cx<T>* __restrict__ A,
-                                                    double* __restrict__ lam, cx<T>* __restrict__ V, int herm_in) {
+                                                    double* __restrict__ lam, cx<T>* __restrict__ V, int herm_in,
+                                                    double rel_tol) {
   __shared__ J16Lds sh;
   __shared__ double sd[2][16];
   __shared__ int srank[2][16];
@@ -187,7 +189,8 @@ __global__ __launch_bounds__(64) void eigh16_kernel(int nb, size_t nk, const cx<
   rs += swz<8>(rs);
   rs += swz<16>(rs);                    // ||A||_F^2 of this lane's matrix
   const T eps = sizeof(T) == 8 ? (T)2.220446049250313e-16 : (T)1.1920929e-07;
-  const T thr2 = eps * eps * rs;
+  const T tol = rel_tol > (double)eps ? (T)rel_tol : eps;   // rotate while |A_pq| > tol ||A||_F
+  const T thr2 = tol * tol * rs;
   for (int sweep = 0; sweep < 30; ++sweep) {
     int rot = 0;
     j16_sweep<T, P>(b, w, m, h, i, nb, act, thr2, rot, sh);
@@ -215,23 +218,25 @@ __global__ __launch_bounds__(64) void eigh16_kernel(int nb, size_t nk, const cx<
 }
 
 template <typename T>
-static void launch_eigh16(int nb, size_t nk, const cx<T>* A, double* lam, cx<T>* V, int herm_in, hipStream_t s) {
+static void launch_eigh16(int nb, size_t nk, const cx<T>* A, double* lam, cx<T>* V, int herm_in, double rel_tol,
+                          hipStream_t s) {
   const dim3 grid((unsigned)((nk + 1) / 2));
   if (nb <= 2)
-    hipLaunchKernelGGL((eigh16_kernel<T, 2>), grid, dim3(64), 0, s, nb, nk, A, lam, V, herm_in);
+    hipLaunchKernelGGL((eigh16_kernel<T, 2>), grid, dim3(64), 0, s, nb, nk, A, lam, V, herm_in, rel_tol);
   else if (nb <= 4)
-    hipLaunchKernelGGL((eigh16_kernel<T, 4>), grid, dim3(64), 0, s, nb, nk, A, lam, V, herm_in);
+    hipLaunchKernelGGL((eigh16_kernel<T, 4>), grid, dim3(64), 0, s, nb, nk, A, lam, V, herm_in, rel_tol);
   else if (nb <= 8)
-    hipLaunchKernelGGL((eigh16_kernel<T, 8>), grid, dim3(64), 0, s, nb, nk, A, lam, V, herm_in);
+    hipLaunchKernelGGL((eigh16_kernel<T, 8>), grid, dim3(64), 0, s, nb, nk, A, lam, V, herm_in, rel_tol);
   else
-    hipLaunchKernelGGL((eigh16_kernel<T, 16>), grid, dim3(64), 0, s, nb, nk, A, lam, V, herm_in);
+    hipLaunchKernelGGL((eigh16_kernel<T, 16>), grid, dim3(64), 0, s, nb, nk, A, lam, V, herm_in, rel_tol);
 }
 
-int eigh16_launch(int dtype, int nb, size_t nk, const void* A, double* lam, void* V, int herm_in, hipStream_t s) {
+int eigh16_launch(int dtype, int nb, size_t nk, const void* A, double* lam, void* V, int herm_in, double rel_tol,
+                  hipStream_t s) {
   if (dtype == HFX_DTYPE_C128)
-    launch_eigh16<double>(nb, nk, (const cx<double>*)A, lam, (cx<double>*)V, herm_in, s);
+    launch_eigh16<double>(nb, nk, (const cx<double>*)A, lam, (cx<double>*)V, herm_in, rel_tol, s);
   else
-    launch_eigh16<float>(nb, nk, (const cx<float>*)A, lam, (cx<float>*)V, herm_in, s);
+    launch_eigh16<float>(nb, nk, (const cx<float>*)A, lam, (cx<float>*)V, herm_in, rel_tol, s);
   return check_launch("eigh16_kernel");
 }
 

@@ -1177,10 +1177,16 @@ using namespace hfx;
 extern "C" {
 
 int hfx_eigh_batched(int dtype, int nb, size_t nk, const void* A, double* lam, void* V, int herm_in, void* stream) {
+  return hfx_eigh_batched_tol(dtype, nb, nk, A, lam, V, herm_in, 0.0, stream);
+}
+
+int hfx_eigh_batched_tol(int dtype, int nb, size_t nk, const void* A, double* lam, void* V, int herm_in,
+                         double rel_tol, void* stream) {
   if (int rc = pk_check_nb(nb)) return rc;
-  if (!A || !lam || !V) return set_error(HFX_EINVAL, "hfx_eigh_batched: NULL argument");
+  if (!A || !lam || !V || !(rel_tol >= 0.0) || rel_tol > 1e-3)
+    return set_error(HFX_EINVAL, "hfx_eigh_batched: bad argument (rel_tol=%g)", rel_tol);
   if (nk == 0) return HFX_OK;
-  return eigh16_launch(dtype, nb, nk, A, lam, V, herm_in, (hipStream_t)stream);
+  return eigh16_launch(dtype, nb, nk, A, lam, V, herm_in, rel_tol, (hipStream_t)stream);
 }
 
 int hfx_density_from_Qp(int dtype, int nb, size_t nk, const void* Q, const double* p, const double* eps,
