@@ -52,6 +52,32 @@ def test_eigh_batched(dev, nb, dt):
     assert np.abs(eye - np.eye(nb)).max() <= 50 * tol
 
 
+@pytest.mark.parametrize("rel_tol", [1e-10, 1e-6])
+def test_eigh_relative_tolerance(dev, rel_tol):
+    """The tolerance form stops at off-diagonals <= rel_tol ||A||_F: the basis
+    stays unitary to rounding, A is reproduced to ~rel_tol ||A||_F, and from a
+    nearly diagonal start (the polish's warm case) it is as exact as the
+    full-precision form up to that size."""
+    import torch
+    from jax_hf_amd import _ops as K
+    rng = np.random.default_rng(5)
+    nb = 16
+    D = np.diag(np.linspace(-3, 3, nb))
+    E = _herm(rng.standard_normal((40, nb, nb)) + 1j * rng.standard_normal((40, nb, nb)))
+    A = D + 1e-3 * E
+    At = torch.from_numpy(A).to(dev)
+    lam = torch.empty((40, nb), dtype=torch.float64, device=dev)
+    V = torch.empty_like(At)
+    K.eigh(At, lam, V, rel_tol=rel_tol)
+    Vh, lh = _np(V), _np(lam)
+    eye = np.conj(np.swapaxes(Vh, -1, -2)) @ Vh
+    assert np.abs(eye - np.eye(nb)).max() < 1e-13
+    nrm = np.sqrt(np.sum(np.abs(A) ** 2, axis=(1, 2))).max()
+    rec = Vh @ (lh[..., None] * np.conj(np.swapaxes(Vh, -1, -2)))
+    assert np.abs(rec - A).max() <= 4 * nb * rel_tol * nrm
+    assert np.abs(lh - np.linalg.eigvalsh(A)).max() <= 4 * nb * rel_tol * nrm
+
+
 def test_density_and_matmul(dev):
     import torch
     from jax_hf_amd import _ops as K
