@@ -157,7 +157,7 @@ __device__ __forceinline__ void j16_sweep(cx<T> (&b)[8], cx<T> (&w)[8], int m, i
 }
 
 template <typename T, int P>
-__global__ __launch_bounds__(64) void eigh16_kernel(int nb, size_t nk, const cx<T>* __restrict__ A,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void eigh16_kernel(int nb, size_t nk, const cx<T>* __restrict__ A,
                                                     double* __restrict__ lam, cx<T>* __restrict__ V, int herm_in,
                                                     double rel_tol) {
   __shared__ J16Lds sh;
