@@ -642,6 +642,65 @@ __device__ void m_cayley_gj(cx<T>* sM, cx<T>* sU, const cx<T>* sX, double tau, i
   }
 }
 
+// nb = 16: the same elimination with the matrices in registers.  Lane l holds
+// column j = l % 16 of rows i = l / 16 + 4 r (r = 0..3); the pivot row comes
+// from the lanes of row p by a wave shuffle, the pivot column entry of each
+// row from the lane of column p in the same 16-lane group; no LDS round trip
+// or barrier inside the elimination.  Reads D from sX, leaves U in sU (sU may
+// be sX: every lane has read D before the first store).  Same pivots, same
+// arithmetic per element as m_cayley_gj.
+template <typename T>
+__device__ __forceinline__ cx<T> shfl_cx(cx<T> v, int src) {
+  return cx<T>{__shfl(v.re, src), __shfl(v.im, src)};
+}
+
+template <typename T>
+__device__ void m_cayley_gj16(cx<T>* sU, const cx<T>* sX, double tau) {
+  const T h = (T)(0.5 * tau);
+  const int l = threadIdx.x & 63, j = l & 15, g = l >> 4;
+  cx<T> M[4], U[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = g + 4 * r;
+    const cx<T> a = sX[i * PK_PITCH + j], b = sX[j * PK_PITCH + i];
+    const cx<T> d{(T)0.5 * (a.re - b.re), (T)0.5 * (a.im + b.im)};   // (D - D^dagger)/2
+    const T one = i == j ? (T)1 : (T)0;
+    M[r] = cx<T>{one + h * d.re, h * d.im};
+    U[r] = cx<T>{one - h * d.re, -h * d.im};
+  }
+  __syncthreads();   // sU may alias sX
+  // pivot p = 4 rp + q: rp selects the register (compile time), q stays a loop
+  // variable so that the 16 steps are not unrolled into one long schedule
+#pragma unroll
+  for (int rp = 0; rp < 4; ++rp) {
+#pragma unroll 1
+    for (int q = 0; q < 4; ++q) {
+      const int p = 4 * rp + q, src_row = (q << 4) | j;
+      const cx<T> mpj0 = shfl_cx(M[rp], src_row), upj0 = shfl_cx(U[rp], src_row);
+      const cx<T> mp = shfl_cx(M[rp], (q << 4) | p);              // M_pp
+      const T n2 = mp.re * mp.re + mp.im * mp.im;
+      const cx<T> ip{mp.re / n2, -mp.im / n2};                     // 1 / M_pp
+      const cx<T> mpj = mpj0 * ip, upj = upj0 * ip;
+      cx<T> f[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) f[r] = shfl_cx(M[r], (g << 4) | p);   // M[i][p]
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (g + 4 * r == p) {
+          M[r] = mpj;
+          U[r] = upj;
+        } else {
+          M[r] = M[r] - f[r] * mpj;
+          U[r] = U[r] - f[r] * upj;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) sU[(g + 4 * r) * PK_PITCH + j] = U[r];
+  __syncthreads();
+}
+
 // eps_b = Re sum_i conj(U_ib) (Ft U)_ib into et[b] (Y: LDS scratch)
 template <typename T>
 __device__ __forceinline__ void m_diag_UFU(const cx<T>* sF, const cx<T>* sU, cx<T>* sY, double* et, int nb) {
@@ -702,7 +761,10 @@ __global__ __launch_bounds__(64) void pk_cg_trial_gj_kernel(int nb, size_t Nk, c
   m_load(sX, D + k * nb2, nb);
   m_load(sF, Ft + k * nb2, nb);
   __syncthreads();
-  m_cayley_gj(sM, sU, sX, tau, nb);
+  if (nb == PK_NBMAX)
+    m_cayley_gj16(sU, sX, tau);
+  else
+    m_cayley_gj(sM, sU, sX, tau, nb);
   m_diag_UFU(sF, sU, sX, et, nb);
   if (U_out) {   // kept for the retraction when this trial is the accepted one
     m_store(U_out + k * nb2, sU, nb);
@@ -761,7 +823,10 @@ __global__ __launch_bounds__(64) void pk_cg_retract_gj_kernel(int nb, cx<T>* __r
     m_load(sX, D + k * nb2, nb);
     m_load(sF, Ft + k * nb2, nb);
     __syncthreads();
-    m_cayley_gj(sM, sU, sX, tau, nb);
+    if (nb == PK_NBMAX)
+      m_cayley_gj16(sU, sX, tau);
+    else
+      m_cayley_gj(sM, sU, sX, tau, nb);
     m_diag_UFU(sF, sU, sX, et, nb);
     if ((int)threadIdx.x < nb) eps_new[k * nb + threadIdx.x] = et[threadIdx.x];
   }
