@@ -435,7 +435,7 @@ class _DIISPolish:
         self.coef = torch.zeros(8, dtype=torch.float64, device=dev)        # DIIS coefficients
         import ctypes as C
         self._xa = (C.c_void_p * nh)(*[f.data_ptr() for f in self.F_hist])
-        self.graph = None
+        self.graphs = {}
 
     # Relative off-diagonal tolerance of the update's eigh, per unit of the
     # polish tolerance.  The update's density is exact for its (unitary) basis
@@ -460,21 +460,22 @@ class _DIISPolish:
         else:
             K.lincomb_multi(self.Fx, Xs, cs, herm=True)
 
-    def _update(self, warm=True):
-        """-> P_new: launches only (the warm form is captured once).  Warm: eigh of
-        Ft = U^H F U (eps, W), mu, then U <- U W and P_new from (U, occ) in one pass."""
+    def _update(self, out, warm=True):
+        """-> out (the next iterate): launches only (the warm form is captured,
+        once per output buffer).  Warm: eigh of Ft = U^H F U (eps, W), mu, then
+        U <- U W and the density from (U, occ) in one pass."""
         if warm:
             K.eigh(self.Ft, self.eps, self.W, herm_in=True, rel_tol=self._eigh_tol)
             self._mu(self.MU_GRAPH_STEPS)
-            K.rotate_density(self.U, self.W, self.occ, self.P_new)
+            K.rotate_density(self.U, self.W, self.occ, out)
         else:
             K.eigh(self.Fx, self.eps, self.U, herm_in=True)
             self._mu(self.MU_GRAPH_STEPS)
             K.density(self.U, self.tmp, p=self.occ)
-            K.hermitize(self.tmp, self.P_new)
+            K.hermitize(self.tmp, out)
 
-    def _capture(self):
-        return capture_graphs(self.dev, self._update)[0]
+    def _capture(self, out):
+        return capture_graphs(self.dev, lambda: self._update(out))[0]
 
     def _eval(self, k):
         """F_k = F[P_k] with E_k, the residuals and the new B row of iterate k
@@ -502,17 +503,18 @@ class _DIISPolish:
             L.call("hfx_extrap_rotate_batched", K.dt_code(self.U), self.nb, self.nk, self.nk, count,
                    C.cast(self._xa, C.c_void_p), L.ptr(self.coef), L.ptr(self.U), L.ptr(self.Ft),
                    L.stream(self.dev))
-            if self.graph is None:
-                self._update()
-                self.graph = self._capture()
+            key = self.P_new.data_ptr()   # the three iterate buffers rotate: one graph per output
+            g = self.graphs.get(key)
+            if g is None:
+                self._update(self.P_new)
+                self.graphs[key] = self._capture(self.P_new)
             else:
-                self.graph.replay()
+                g.replay()
         else:                             # first update of a cold run: the plain step, eigh of herm(F_0)
             K.diis_coefficients(self.outs[2:2 + nh], self.Bd, idx, 1, self.coef)
             self._extrapolate([self.F_hist[idx]], [1.0], False)
-            self._update(warm=False)
-        self.P_prev, self.P_cur = self.P_cur, self.P_prev
-        K.lincomb(self.P_cur, self.P_new, 1.0)
+            self._update(self.P_new, warm=False)
+        self.P_prev, self.P_cur, self.P_new = self.P_cur, self.P_new, self.P_prev
 
     def run(self, P_lo, mu0, tol, max_iter, Q_lo=None):
         """Q_lo (optional): the orbitals P_lo was built from.  Re-orthonormalised in
@@ -528,8 +530,8 @@ class _DIISPolish:
         nh = self.nh
         max_iter = max(int(max_iter), 1)
         et = self.EIGH_TOL_PER_TOL * float(tol)
-        if et != self._eigh_tol:          # the captured update holds the eigh tolerance
-            self._eigh_tol, self.graph = et, None
+        if et != self._eigh_tol:          # the captured updates hold the eigh tolerance
+            self._eigh_tol, self.graphs = et, {}
         K.hermitize(convert(P_lo, torch.complex128) if P_lo.dtype != torch.complex128 else P_lo, self.P_cur)
         self.st[K.ST_MU] = mu0
         warm0 = Q_lo is not None
