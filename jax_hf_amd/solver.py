@@ -176,7 +176,7 @@ class _CG:
         if form not in ("gauss_jordan", "spectral"):
             raise ValueError(f"cayley must be 'gauss_jordan' or 'spectral'; got {cfg.cayley!r}")
         self.gauss_jordan = form == "gauss_jordan"
-        self.graph = None        # captured iteration body, kept while the runner is cached
+        self.graph = None        # captured iteration bodies {G buffer: graph}, kept with the runner
 
     # -- pieces ---------------------------------------------------------------
     def density(self):
@@ -247,7 +247,8 @@ class _CG:
             if self.graph is None and use_graph and k >= 1:   # k = 0 ran eagerly: buffers exist
                 self.graph = self._capture(shrink_pow)
             if use_graph and self.graph is not None:
-                self.graph.replay()
+                self.graph[self.G.data_ptr()].replay()
+                self._swap()
             else:
                 self._body(shrink_pow)
             sv = st.cpu().numpy()                                               # one readback
@@ -309,14 +310,19 @@ class _CG:
             L.call("hfx_cg_retract", dt, nb, nk, L.ptr(self.Q), L.ptr(self.V), L.ptr(self.lam), L.ptr(self.Fe),
                    L.ptr(st), 0.0, 0, L.ptr(self.eps), None, sm)                # :586-590
         self.mu(self.eps, True, steps=self.MU_LOOP_STEPS)                      # :591-592
-        # this iteration's gradient / preconditioned norms become the previous ones
-        # (copies, not reference swaps, so one captured graph serves every iteration)
-        K.lincomb(self.Gp, self.G, 1.0)
-        K.lincomb(self.gpp, self.gp, 1.0)
+        self._swap()     # this iteration's gradient / preconditioned norms become the previous ones
+
+    def _swap(self):
+        self.G, self.Gp = self.Gp, self.G
+        self.gp, self.gpp = self.gpp, self.gp
 
     def _capture(self, shrink_pow):
-        """HIP graph of the iteration body (identical launches to the eager body)."""
-        return capture_graphs(self.dev, lambda: self._body(shrink_pow))[0]
+        """HIP graphs of the iteration body (identical launches to the eager body),
+        one per parity of the buffer swap it makes, keyed by the G buffer each was
+        captured with (the two captures swap twice: the references end as they began)."""
+        g0 = self.G.data_ptr()
+        gs = capture_graphs(self.dev, lambda: self._body(shrink_pow), n=2)
+        return {g0: gs[0], self.Gp.data_ptr(): gs[1]}
 
     def finalize(self):
         """solver.py:606-638: Fock, eigh canonicalisation, mu, p, Fock + E."""
