@@ -313,9 +313,11 @@ int hfx_scf_resid(int dtype, int nb, size_t nk, const void* F, const void* D, co
  * (reference_scf.py:110-111): part[2 + v][k] = Re sum conj(comm) R_v for the
  * nr <= 8 stored residuals Rs (a HOST array of device pointers; Rs[self] is
  * the buffer comm is written to, its value is part[0][k]; self = -1: none).
- * comm is required. */
+ * comm is required.  hist_dtype: storage of comm and Rs -- dtype, or
+ * HFX_C64 for a complex128 build at nb = 16 (the residual norms are formed in
+ * FP64 before the store; the B row reads the FP32 history). */
 int hfx_scf_resid_dots(int dtype, int nb, size_t nk, const void* F, const void* D, const void* Pn, void* comm,
-                       const void* const* Rs, int nr, int self, double* part, void* stream);
+                       const void* const* Rs, int nr, int self, int hist_dtype, double* part, void* stream);
 /* part[v][k] = Re sum conj(X_v) Y_v (mode 0, the inner products of solver.py:205-208 and the
  * DIIS B matrix reference_scf.py:110-111) or Re tr(X_v Y_v) (mode 1, reference_scf.py:319);
  * Xs, Ys are HOST arrays of nv <= 8 device pointers. */

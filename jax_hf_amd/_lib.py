@@ -18,7 +18,7 @@ LIB_PATH = _HERE / "libhfx.so"
 
 HFX_C128 = 0
 HFX_C64 = 1
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 _lib = None
 _lock = threading.Lock()
@@ -82,7 +82,7 @@ _SIGS = {
     "hfx_extrap_rotate_batched": [i32, i32, sz, i32, i32, vp, vp, vp, vp, vp],
     "hfx_rotate_density": [i32, i32, sz, vp, vp, vp, vp, vp],
     "hfx_scf_resid": [i32, i32, sz, vp, vp, vp, vp, vp, vp],
-    "hfx_scf_resid_dots": [i32, i32, sz, vp, vp, vp, vp, vp, i32, i32, vp, vp],
+    "hfx_scf_resid_dots": [i32, i32, sz, vp, vp, vp, vp, vp, i32, i32, i32, vp, vp],
     "hfx_orthonormalize": [i32, i32, sz, vp, i32, vp],
     "hfx_diis_coefficients": [vp, vp, i32, i32, i32, vp, vp],
     "hfx_diis_coefficients_batched": [vp, sz, vp, i32, i32, i32, i32, vp, vp],

@@ -129,7 +129,7 @@ def scf_resid_dots(F, D, Pn, comm, Rs, self_idx, part):
     nk = F.numel() // (nb * nb)
     ra = (C.c_void_p * len(Rs))(*[r.data_ptr() for r in Rs])
     L.call("hfx_scf_resid_dots", dt_code(F), nb, nk, L.ptr(F), L.ptr(D), L.ptr(Pn), L.ptr(comm),
-           C.cast(ra, C.c_void_p), len(Rs), int(self_idx), L.ptr(part), s(F.device))
+           C.cast(ra, C.c_void_p), len(Rs), int(self_idx), dt_code(comm), L.ptr(part), s(F.device))
 
 
 def dots(X, Ys, part, mode=0):

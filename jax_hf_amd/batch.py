@@ -335,7 +335,9 @@ class _BatchPolish:
         self.P_prev, self.P_cur, self.P_new, self.Ft, self.W, self.U, self.tmp = e(), e(), e(), e(), e(), e(), e()
         self.Pfin, self.Ffin = e(), e()
         self.F_hist = [e() for _ in range(nh)]
-        self.R_hist = [e() for _ in range(nh)]
+        # residual history in complex64 at nb = 16, as in the single polish
+        rdt = torch.complex64 if nb == 16 else cdt
+        self.R_hist = [torch.empty(self.shape4, dtype=rdt, device=dev) for _ in range(nh)]
         self.eps = torch.empty((S, nk1, nk2, nb), **r)
         self.occ = torch.empty_like(self.eps)
         self.st = torch.zeros((S, K.ST_LEN), **r)

@@ -984,6 +984,7 @@ __global__ __launch_bounds__(64) void pk_orthonormalize_kernel(int nb, cx<T>* __
 struct PkRes {
   const void* r[PK_XMAX];
   int nr, self;
+  int r32;   // FP64 kernels: the residual history (comm included) stored as complex64
 };
 template <typename T>
 __global__ __launch_bounds__(64) void pk_scf_resid_kernel(int nb, size_t Nk, const cx<T>* __restrict__ F,
@@ -1055,7 +1056,10 @@ __global__ __launch_bounds__(64) void pk_scf_resid16_kernel(size_t Nk, const cx<
     const int row = Mfma<T>::row(l, r);
     const size_t e = k * nb2 + (size_t)row * nb + col;
     const cx<T> c{acc.re[r], acc.im[r]};
-    if (comm) comm[e] = c;
+    if (comm) {
+      if (rs.r32) reinterpret_cast<cx<float>*>(comm)[e] = cx<float>{(float)c.re, (float)c.im};
+      else comm[e] = c;
+    }
     c2 += (double)c.re * c.re + (double)c.im * c.im;
     if (Pn) {
       const cx<T> d = Pn[e] - sD[row * PK_PITCH + col];
@@ -1064,8 +1068,17 @@ __global__ __launch_bounds__(64) void pk_scf_resid16_kernel(size_t Nk, const cx<
 #pragma unroll
     for (int v = 0; v < PK_XMAX; ++v)
       if (v < rs.nr && v != rs.self) {
-        const cx<T> q = reinterpret_cast<const cx<T>*>(rs.r[v])[e];
-        bd[v] += (double)c.re * q.re + (double)c.im * q.im;
+        double qre, qim;
+        if (rs.r32) {
+          const cx<float> q = reinterpret_cast<const cx<float>*>(rs.r[v])[e];
+          qre = q.re;
+          qim = q.im;
+        } else {
+          const cx<T> q = reinterpret_cast<const cx<T>*>(rs.r[v])[e];
+          qre = (double)q.re;
+          qim = (double)q.im;
+        }
+        bd[v] += (double)c.re * qre + (double)c.im * qim;
       }
   }
   c2 = wave_sum(c2);
@@ -1505,13 +1518,16 @@ int hfx_orthonormalize(int dtype, int nb, size_t nk, void* U, int steps, void* s
 }
 
 int hfx_scf_resid_dots(int dtype, int nb, size_t nk, const void* F, const void* D, const void* Pn, void* comm,
-                       const void* const* Rs, int nr, int self, double* part, void* stream) {
+                       const void* const* Rs, int nr, int self, int hist_dtype, double* part, void* stream) {
   if (int rc = pk_check_nb(nb)) return rc;
-  if (!F || !D || !comm || !part || !Rs || nr < 1 || nr > PK_XMAX || self < -1 || self >= nr)
-    return set_error(HFX_EINVAL, "hfx_scf_resid_dots: bad argument (nr=%d, self=%d)", nr, self);
+  if (!F || !D || !comm || !part || !Rs || nr < 1 || nr > PK_XMAX || self < -1 || self >= nr ||
+      (hist_dtype != dtype && !(dtype == HFX_DTYPE_C128 && hist_dtype == HFX_DTYPE_C64 && nb == PK_NBMAX)))
+    return set_error(HFX_EINVAL, "hfx_scf_resid_dots: bad argument (nr=%d, self=%d, hist_dtype=%d)", nr, self,
+                     hist_dtype);
   PkRes rs{};
   rs.nr = nr;
   rs.self = self;
+  rs.r32 = hist_dtype != dtype;
   for (int v = 0; v < nr; ++v) {
     if (!Rs[v] && v != self) return set_error(HFX_EINVAL, "hfx_scf_resid_dots: NULL residual %d", v);
     rs.r[v] = Rs[v];
