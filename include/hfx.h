@@ -300,6 +300,19 @@ int hfx_extrap_rotate_batched(int dtype, int nb, size_t nk, int seg_len, int nin
 int hfx_rotate_density(int dtype, int nb, size_t nk, void* U, const void* W, const double* occ, void* P,
                        void* stream);
 
+/* Symmetry averaging of per-k matrices, one group average of make_project_fn
+ * (symmetry.py:46-150: unitary conjugation sums, the same-k / flipped-k
+ * group, the spatial average over k-index maps, time reversal):
+ *   out[k] = mask[k] ? weight sum_g U_g op_g(A[src_g(k)]) U_g^dagger : A[k]
+ * for ng <= 64 elements: Ug (ng, nb, nb) in the storage precision, src
+ * (ng, nk) int32 linear source k-points (the mod / flip negations and index
+ * maps as tables; entries outside [0, nk) read A[k]), op_g = complex
+ * conjugation when bit g of conj_bits is set, mask (nk) bytes or NULL.  out
+ * must not alias A. */
+int hfx_symmetry_average(int dtype, int nb, size_t nk, int ng, const void* A, const void* Ug, const int* src,
+                         unsigned long long conj_bits, const unsigned char* mask, double weight, void* out,
+                         void* stream);
+
 /* Newton-Schulz re-orthonormalisation of nk nb x nb bases in place, steps in
  * [1, 4] times U <- U (3 I - U^dagger U) / 2 (each step squares the deviation
  * from unitarity; the CG orbitals handed to the FP64 polish, solver.py:42-44). */

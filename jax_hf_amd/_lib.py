@@ -84,6 +84,7 @@ _SIGS = {
     "hfx_scf_resid": [i32, i32, sz, vp, vp, vp, vp, vp, vp],
     "hfx_scf_resid_dots": [i32, i32, sz, vp, vp, vp, vp, vp, i32, i32, i32, vp, vp],
     "hfx_orthonormalize": [i32, i32, sz, vp, i32, vp],
+    "hfx_symmetry_average": [i32, i32, sz, i32, vp, vp, vp, C.c_ulonglong, vp, f64, vp, vp],
     "hfx_diis_coefficients": [vp, vp, i32, i32, i32, vp, vp],
     "hfx_diis_coefficients_batched": [vp, sz, vp, i32, i32, i32, i32, vp, vp],
     "hfx_batched_dots": [i32, i32, sz, vp, vp, i32, i32, vp, vp],

@@ -1095,6 +1095,48 @@ __global__ __launch_bounds__(64) void pk_scf_resid16_kernel(size_t Nk, const cx<
     }
 }
 
+// ---- symmetry averaging (src/jax_hf/symmetry.py:46-150) ----------------------
+//   out[k] = mask[k] ? weight sum_g U_g op_g(A[src_g(k)]) U_g^dagger : A[k]
+// op_g = complex conjugation for the elements flagged in conj_bits (time
+// reversal).  Every k-map of the reference (identity, the mod / flip
+// negations, per-element index maps) is a gather table src (ng, Nk) made on
+// the host.  One wave per k; the two products per element run on the matrix
+// cores at nb = 16.  Out-of-range sources (rejected by the host) read A[k].
+constexpr int SYM_MAX = 64;
+template <typename T>
+__global__ __launch_bounds__(64) void pk_sym_avg_kernel(int nb, size_t Nk, int ng, const cx<T>* __restrict__ A,
+                                                        const cx<T>* __restrict__ Ug, const int* __restrict__ src,
+                                                        unsigned long long conj_bits,
+                                                        const unsigned char* __restrict__ mask, double weight,
+                                                        cx<T>* __restrict__ out) {
+  __shared__ cx<T> sA[PK_MAT], sU[PK_MAT], sX[PK_MAT];
+  const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
+  if (mask && !mask[k]) {
+    PK_FOR_ELEM(nb, { out[k * nb2 + e] = A[k * nb2 + e]; })
+    return;
+  }
+  cx<T> acc[PK_EPL];
+#pragma unroll
+  for (int m = 0; m < PK_EPL; ++m) acc[m] = cx<T>{0, 0};
+  for (int g = 0; g < ng; ++g) {
+    long long ks = src[(size_t)g * Nk + k];
+    if (ks < 0 || ks >= (long long)Nk) ks = (long long)k;
+    const bool cj = (conj_bits >> g) & 1ull;
+    PK_FOR_ELEM(nb, {
+      cx<T> a = A[(size_t)ks * nb2 + e];
+      if (cj) a.im = -a.im;
+      sA[i * PK_PITCH + j] = a;
+      sU[i * PK_PITCH + j] = Ug[(size_t)g * nb2 + e];
+    })
+    __syncthreads();
+    m_gemm<T, false, false>(sX, sU, sA, nb);   // U op(A)
+    m_gemm<T, false, true>(sA, sX, sU, nb);    // (U op(A)) U^dagger
+    PK_FOR_ELEM(nb, { acc[m_] = acc[m_] + sA[i * PK_PITCH + j]; })
+    __syncthreads();
+  }
+  PK_FOR_ELEM(nb, { out[k * nb2 + e] = (T)weight * acc[m_]; })
+}
+
 // Per-k inner products of X with up to 8 matrices Y_v:
 //   mode 0: part[v][k] = Re sum_ij conj(X_ij) Y_v,ij   (Frobenius, solver.py:205-208, reference_scf.py:110-111)
 //   mode 1: part[v][k] = Re sum_ij X_ij Y_v,ji          (trace of the product, reference_scf.py:319)
@@ -1504,6 +1546,23 @@ int hfx_scf_resid(int dtype, int nb, size_t nk, const void* F, const void* D, co
   if (int rc = pk_check_nb(nb)) return rc;
   if (!F || !D || !part) return set_error(HFX_EINVAL, "hfx_scf_resid: NULL argument");
   return scf_resid_launch(dtype, nb, nk, F, D, Pn, comm, PkRes{}, part, (hipStream_t)stream);
+}
+
+int hfx_symmetry_average(int dtype, int nb, size_t nk, int ng, const void* A, const void* Ug, const int* src,
+                         unsigned long long conj_bits, const unsigned char* mask, double weight, void* out,
+                         void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  if (!A || !Ug || !src || !out || A == out || ng < 1 || ng > SYM_MAX)
+    return set_error(HFX_EINVAL, "hfx_symmetry_average: bad argument (ng=%d, out must not alias A)", ng);
+  if (nk == 0) return HFX_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(pk_sym_avg_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nk, ng, PKC(double, A),
+                       PKC(double, Ug), src, conj_bits, mask, weight, PKM(double, out));
+  else
+    hipLaunchKernelGGL(pk_sym_avg_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nk, ng, PKC(float, A),
+                       PKC(float, Ug), src, conj_bits, mask, weight, PKM(float, out));
+  return check_launch("pk_sym_avg_kernel");
 }
 
 int hfx_orthonormalize(int dtype, int nb, size_t nk, void* U, int steps, void* stream) {

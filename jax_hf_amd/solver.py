@@ -99,19 +99,27 @@ class _Fock:
         self.block_specs = None if block_sizes is None else (("sizes", tuple(int(s) for s in block_sizes)),)
         self.eng = kernel.fock_engine(dtype=cdt)
         self.eng_nohs = None
-        if self.block_specs is not None or fock_build_fn is not None:
-            self._args = kernel.as_args()
+        self._args = kernel.as_args() if (self.block_specs is not None or fock_build_fn is not None) else None
 
     def graph_safe(self, project_fn=None):
-        """True when a build is device launches only (no user callable, no host check)."""
-        return self.fn is None and self.block_specs is None and project_fn is None
+        """True when an iteration is device launches only (no user callable, no host
+        check); a projector of ``jax_hf_amd.symmetry.make_project_fn`` qualifies."""
+        pf_ok = project_fn is None or bool(getattr(project_fn, "hfx_graph_safe", False))
+        return self.fn is None and self.block_specs is None and pf_ok
 
     def build(self, P, F, E_ptr=None, project_fn=None, refP="kernel", sub_h=False):
         eng = self.eng
-        if self.fn is None and self.block_specs is None and project_fn is None:
+        if self.fn is None and self.block_specs is None and (project_fn is None or not sub_h):
             ref = "default" if isinstance(refP, str) else refP
             eng.build(P, F, energy=E_ptr is not None, E_ptr=E_ptr, refP=ref, sub_h=sub_h)
+            if project_fn is not None:      # fock.py:94-95: F = herm(project_fn(F)); E is unchanged
+                X = to_tensor(project_fn(F), P.device, self.cdt)
+                if X.data_ptr() == F.data_ptr():
+                    X = X.clone()
+                K.hermitize(X, F)
             return
+        if self._args is None:
+            self._args = self.kernel.as_args()
         a = self._args
         fn = self.fn
         if fn is None:
