@@ -268,6 +268,39 @@ def adjoint_bench(J, syn, dev, args):
             "grad_h_norm": float(torch.linalg.vector_norm(grads["h"]).item())}
 
 
+def symmetry_bench(J, syn, dev, args):
+    """Symmetry projection at nk = 96x96, nb = 16 (the projection a symmetric CG
+    applies to P every iteration, symmetry.py:153-281): a 3-element spatial
+    group on C3 k-index maps, then a 2-element orbital group, then time
+    reversal -- three group averages, one launch each.  Algorithmic bytes per average:
+    (elements + 1) x c Nk nb^2 (every element gathers one matrix per k, plus
+    the output)."""
+    import torch
+    from jax_hf_amd.symmetry import make_project_fn
+    nk, nb = 96, 16
+    rng = np.random.default_rng(5)
+    half = nk // 2
+    ii, jj = np.meshgrid(np.arange(nk), np.arange(nk), indexing="ij")
+    idx_i = np.stack([ii, (-jj) % nk, (jj - ii + half) % nk])
+    idx_j = np.stack([jj, (ii - jj + half) % nk, (-ii) % nk])
+    w = np.exp(2j * np.pi / 3)
+    C = np.diag(w ** (np.arange(nb) % 3))
+    z = np.diag(np.where(np.arange(nb) % 2 == 0, 1.0, -1.0)).astype(complex)
+    Uq = np.linalg.qr(rng.standard_normal((nb, nb)) + 1j * rng.standard_normal((nb, nb)))[0]
+    proj = make_project_fn(spatial_group=np.stack([np.eye(nb), C, C @ C]), spatial_k_index_maps=(idx_i, idx_j),
+                           unitary_group=np.stack([np.eye(nb), z]), time_reversal_U=Uq)
+    A = torch.from_numpy(rng.standard_normal((nk, nk, nb, nb)) + 1j * rng.standard_normal((nk, nk, nb, nb))).to(dev)
+    proj(A)
+    n = 50
+    _, t = _median_time(lambda: [proj(A) for _ in range(n)], 3)
+    c, Nk = 16, nk * nk
+    elems = [3, 2, 2]                                         # spatial, orbital, time reversal
+    algo = sum((g + 1) * c * Nk * nb * nb for g in elems)
+    return {"workload": "C3 (k-index maps) x 2 orbital x time reversal, 3 group averages, complex128",
+            "us_per_projection": t / n * 1e6, "algorithmic_bytes": algo,
+            "GBps": algo / (t / n) / 1e9}
+
+
 SOLVE_T = 0.1
 POLISH_TOL = 1e-6
 
@@ -481,6 +514,7 @@ def main():
         if not args.no_forms:
             out["newton_64x64_nb8"] = newton_bench(J, syn, dev, args)
             out["adjoint_48x48_nb16"] = adjoint_bench(J, syn, dev, args)
+            out["symmetry_96x96_nb16"] = symmetry_bench(J, syn, dev, args)
     out["fock"] = fock
     out["fock_build_GBps"] = fock["GBps"]
     out["roofline"] = roof
