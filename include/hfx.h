@@ -313,6 +313,16 @@ int hfx_symmetry_average(int dtype, int nb, size_t nk, int ng, const void* A, co
                          unsigned long long conj_bits, const unsigned char* mask, double weight, void* out,
                          void* stream);
 
+/* Periodic linear resampling of a centred (n0, n1, ...) k-grid array onto an
+ * (nk, nk, ...) grid (utils.py:477-535, resample_kgrid with method "linear"):
+ * axis 0 then axis 1, source coordinate (j - nk/2) * (n_old/nk) + n_old/2
+ * rounded in the array's real precision, neighbours taken modulo n_old; an
+ * axis already of length nk is copied.  m = elements per k-point (nb * nb for
+ * matrices), kind 0 = float64, 1 = float32, 2 = complex128, 3 = complex64.
+ * out must not alias X.  The coarse-to-fine seed of solve_continuation
+ * (continuation.py:48-51). */
+int hfx_resample_kgrid(int kind, int n0, int n1, size_t m, int nk, const void* X, void* out, void* stream);
+
 /* Newton-Schulz re-orthonormalisation of nk nb x nb bases in place, steps in
  * [1, 4] times U <- U (3 I - U^dagger U) / 2 (each step squares the deviation
  * from unitarity; the CG orbitals handed to the FP64 polish, solver.py:42-44). */

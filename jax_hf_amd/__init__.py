@@ -3,7 +3,8 @@
 Drop-in for the ``jax_hf`` hot path (``src/jax_hf/__init__.py:32-95``):
 ``HartreeFockKernel``, ``build_fock``, ``hf_energy``, ``free_energy``,
 ``occupation_entropy``, ``solve`` / ``solve_direct_minimization``,
-``solve_scf`` and their config / result types.  Arrays are torch tensors on a
+``solve_scf``, the drivers ``solve_continuation`` / ``resample_kgrid`` and
+``solve_deflated``, and their config / result types.  Arrays are torch tensors on a
 ROCm device; all arithmetic runs in the hand-written HIP kernels of
 ``libhfx.so`` (``jax_hf_amd/csrc``), called through ctypes.
 
@@ -43,4 +44,7 @@ _extend([
     ("solver", ["SolverConfig", "SolveResult", "solve", "solve_direct_minimization"]),
     ("reference_scf", ["SCFConfig", "SCFResult", "solve_scf"]),
     ("batch", ["solve_batch"]),
+    ("utils", ["resample_kgrid"]),
+    ("continuation", ["ContinuationResult", "solve_continuation"]),
+    ("deflation", ["DeflatedResult", "solve_deflated"]),
 ])

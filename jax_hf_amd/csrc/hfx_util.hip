@@ -230,6 +230,73 @@ static int grid_for(size_t n) {
   return (int)(g < 1 ? 1 : g);
 }
 
+// Periodic linear resampling of a centred k-grid (utils.py:477-535): axis 0 is
+// interpolated first, then axis 1, each from the source coordinate
+// u = (j - n_new/2) * (n_old/n_new) + n_old/2 in the array's real precision.
+// Every product and sum is rounded on its own (no fused multiply-add), so the
+// floor index and the weights are those of the same expression evaluated in
+// IEEE arithmetic on the host.  One thread per output real scalar; a complex
+// array interpolates its real and imaginary parts with the same weights.
+// (__dmul_rn and friends are plain operators in clang's HIP headers, which
+// HIP's default -ffp-contract=fast-honor-pragmas would fuse into an FMA with
+// the following add; the pragma keeps each operation rounded on its own.)
+__device__ __forceinline__ double rn_mul(double a, double b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+__device__ __forceinline__ double rn_add(double a, double b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+__device__ __forceinline__ float rn_mul(float a, float b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+__device__ __forceinline__ float rn_add(float a, float b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+
+template <typename R>
+__device__ __forceinline__ void resample_src(int j, int n_old, int n_new, R ratio, int& i0, int& i1, R& f) {
+  const R u = rn_add(rn_mul((R)j - (R)(n_new / 2), ratio), (R)(n_old / 2));
+  const R fl = floor(u);
+  f = rn_add(u, -fl);
+  int i = (int)fl % n_old;
+  if (i < 0) i += n_old;
+  i0 = i;
+  i1 = i + 1 == n_old ? 0 : i + 1;
+}
+
+template <typename R>
+__device__ __forceinline__ R resample_lerp(R x0, R x1, R f) {
+  return rn_add(rn_mul(rn_add((R)1, -f), x0), rn_mul(f, x1));
+}
+
+template <typename R>
+__global__ void resample_kernel(int n0, int n1, int nk, size_t m, R r0, R r1, const R* __restrict__ X,
+                                R* __restrict__ out) {
+  const size_t total = (size_t)nk * nk * m;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+    const size_t c = t % m, ij = t / m;
+    const int i = (int)(ij / nk), j = (int)(ij % nk);
+    int a0 = i, a1 = i, b0 = j, b1 = j;
+    R fa = 0, fb = 0;
+    if (n0 != nk) resample_src(i, n0, nk, r0, a0, a1, fa);
+    if (n1 != nk) resample_src(j, n1, nk, r1, b0, b1, fb);
+    const R* row0 = X + (size_t)a0 * n1 * m + c;
+    const R* row1 = X + (size_t)a1 * n1 * m + c;
+    R y0 = row0[(size_t)b0 * m], y1 = 0;
+    if (n0 != nk) y0 = resample_lerp(y0, row1[(size_t)b0 * m], fa);
+    if (n1 != nk) {
+      y1 = row0[(size_t)b1 * m];
+      if (n0 != nk) y1 = resample_lerp(y1, row1[(size_t)b1 * m], fa);
+      y0 = resample_lerp(y0, y1, fb);
+    }
+    out[t] = y0;
+  }
+}
+
 }  // namespace hfx
 
 using namespace hfx;
@@ -374,6 +441,23 @@ int hfx_entropy(const void* p, int real_kind, const double* w, size_t nk, int nb
   if (rc) return rc;
   hipLaunchKernelGGL(entropy_final_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, scratch, g, E, T, out);
   return check_launch("entropy_final_kernel");
+}
+
+int hfx_resample_kgrid(int kind, int n0, int n1, size_t m, int nk, const void* X, void* out, void* stream) {
+  if (!X || !out || kind < 0 || kind > 3 || n0 < 1 || n1 < 1 || nk < 1)
+    return set_error(HFX_EINVAL, "hfx_resample_kgrid: bad argument (kind=%d, n0=%d, n1=%d, nk=%d)", kind, n0, n1, nk);
+  if (X == out) return set_error(HFX_EINVAL, "hfx_resample_kgrid: out aliases the input");
+  const size_t reals = m * (kind >= K_C128 ? 2 : 1);
+  const size_t n = (size_t)nk * nk * reals;
+  if (n == 0) return HFX_OK;
+  if (kind == K_F64 || kind == K_C128) {
+    hipLaunchKernelGGL(resample_kernel<double>, dim3(grid_for(n)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream, n0, n1,
+                       nk, reals, (double)n0 / (double)nk, (double)n1 / (double)nk, (const double*)X, (double*)out);
+  } else {
+    hipLaunchKernelGGL(resample_kernel<float>, dim3(grid_for(n)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream, n0, n1,
+                       nk, reals, (float)n0 / (float)nk, (float)n1 / (float)nk, (const float*)X, (float*)out);
+  }
+  return check_launch("resample_kernel");
 }
 
 }  // extern "C"

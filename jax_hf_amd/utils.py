@@ -3,7 +3,8 @@
 Mirrors ``src/jax_hf/utils.py``: ``validate_electron_count`` (:21-43, host
 check), ``hermitize`` (:46-48), ``fermidirac`` (:51-53), ``selfenergy_fft``
 (:66-137), ``find_chemical_potential`` (:314-442, bisection or safeguarded
-Newton, one cooperative launch) and ``density_matrix_from_fock`` (:445-474).
+Newton, one cooperative launch), ``density_matrix_from_fock`` (:445-474) and
+``resample_kgrid`` (:477-535, one launch of ``hfx_resample_kgrid``).
 """
 
 from __future__ import annotations
@@ -18,7 +19,7 @@ from . import _ops as K
 from ._util import convert, host_array, to_tensor
 
 __all__ = ["validate_electron_count", "hermitize", "fermidirac", "selfenergy_fft",
-           "find_chemical_potential", "density_matrix_from_fock"]
+           "find_chemical_potential", "density_matrix_from_fock", "resample_kgrid"]
 
 
 def validate_electron_count(weights: Any, nbands: int, n_electrons: Any, *, context: str = "n_electrons") -> None:
@@ -142,3 +143,35 @@ def density_matrix_from_fock(F, weights, n_electrons: float, T: float, *, eigh_b
     P = torch.empty_like(F)
     K.density(U, P, p=occ)
     return P, mu
+
+
+_RESAMPLE_KIND = {torch.float64: 0, torch.float32: 1, torch.complex128: 2, torch.complex64: 3}
+
+
+def resample_kgrid(values, nk: int, *, method: str = "linear"):
+    """Resample a centred, periodic (n0, n1, ...) k-grid array to (nk, nk, ...)
+    (utils.py:512-535): periodic linear interpolation along axis 0, then axis 1,
+    with k = 0 at index n // 2.  Returns ``values`` itself when both axes
+    already have length nk.  Device tensors of float64 / float32 / complex128 /
+    complex64; other array-likes are uploaded first."""
+    nk = int(nk)
+    x = to_tensor(values)
+    if x.dim() < 2:
+        raise ValueError("resample_kgrid expects an array with at least 2 dimensions (nk,nk,...).")
+    if x.shape[0] == nk and x.shape[1] == nk:
+        return x
+    method = str(method).lower()
+    if method != "linear":
+        raise ValueError(f"Unsupported resample method {method!r}. Only 'linear' is supported.")
+    if nk <= 0:
+        raise ValueError("n_new must be a positive integer.")
+    kind = _RESAMPLE_KIND.get(x.dtype)
+    if kind is None:
+        raise TypeError(f"resample_kgrid: unsupported dtype {x.dtype}")
+    m = 1
+    for d in x.shape[2:]:
+        m *= int(d)
+    out = torch.empty((nk, nk) + tuple(x.shape[2:]), dtype=x.dtype, device=x.device)
+    L.call("hfx_resample_kgrid", kind, int(x.shape[0]), int(x.shape[1]), m, nk, L.ptr(x), L.ptr(out),
+           L.stream(x.device))
+    return out

@@ -25,6 +25,13 @@ Pinning (what makes each function trustworthy, see ``tests/test_oracle.py``):
   state (E, P, mu), commutator invariant, cross-solver agreement, Newton HVP vs
   central finite differences, DIIS coefficients vs a dense augmented solve.
 
+* ``oracle/drivers.py`` (``resample_kgrid``, ``solve_continuation``,
+  ``solve_deflated``) and ``deflation_bias_terms``: ``numpy.interp`` with a
+  period, round trips and coincident samples; the reference tests' manual
+  continuation driver and cold fixed point; the bistable model's analytic
+  solutions; a central-difference gradient identity and closed forms of the
+  bias (``tests/test_oracle_drivers.py``).
+
 Parity unpinned (documented): iteration counts / history arrays of the
 solvers vs JAX (depend on XLA round-off at the line-search accept boundary)
 and the eigenvector gauge returned by ``jnp.linalg.eigh``.

@@ -26,7 +26,7 @@ __all__ = [
     "cayley_unitary_from_spectrum", "diag_UFU_from_spectrum", "qr_retract",
     "SolverConfig", "SolveResult", "solve_cg", "solve_newton", "solve",
     "SCFConfig", "SCFResult", "solve_scf", "diis_coefficients",
-    "direct_exchange_sum", "FFT_WORKERS",
+    "direct_exchange_sum", "deflation_bias_terms", "FFT_WORKERS",
 ]
 
 # scipy.fft worker count (the CPU baseline sets this to the host core count).
@@ -738,6 +738,20 @@ def solve_cg(kernel, P0, n_electrons, config=None, fock_build_fn=None, trace=Non
                        converged=bool(converged), history=dict(E=hist_E, grad_norm=hist_G))
 
 
+def deflation_bias_terms(P, targets, w2d, sigma, inv2L2):
+    """Gaussian deflation bias around targets (solver.py:799-812):
+    (Phi = sum_i sigma exp(-d_i^2 inv2L2), S_pen = sum_i 2 phi_i' (P - P*_i),
+    diag = 2 sum_i phi_i') with d_i^2 = sum_k w_k ||P_k - P*_{i,k}||_F^2 and
+    phi_i' = -phi_i inv2L2."""
+    diff = P[None] - targets
+    per_k = np.sum(np.abs(diff) ** 2, axis=(-2, -1))
+    d2 = np.sum(w2d[None] * per_k, axis=(1, 2))
+    phi = sigma * np.exp(-d2 * inv2L2)
+    phip = -phi * inv2L2
+    S = np.sum((2.0 * phip)[:, None, None, None, None] * diff, axis=0)
+    return float(np.sum(phi)), S, float(2.0 * np.sum(phip))
+
+
 def solve_newton(kernel, P0, n_electrons, config=None, fock_build_fn=None, trace=None):
     """solver.py:686-1078 (_solve_newton_impl): trust-region Newton / Steihaug CG."""
     cfg = config or SolverConfig(optimizer="newton")
@@ -778,14 +792,8 @@ def solve_newton(kernel, P0, n_electrons, config=None, fock_build_fn=None, trace
     def jnorm(aX, ap):
         return float(np.sqrt(max(0.0, jip(aX, ap, aX, ap))))
 
-    def deflation_bias(P):                                             # :804-812
-        diff = P[None] - targets
-        per_k = np.sum(np.abs(diff) ** 2, axis=(-2, -1))
-        d2 = np.sum(w2d[None] * per_k, axis=(1, 2))
-        phi = dsig * np.exp(-d2 * inv2L2)
-        phip = -phi * inv2L2
-        S = np.sum((2.0 * phip)[:, None, None, None, None] * diff, axis=0)
-        return float(np.sum(phi)), S, float(2.0 * np.sum(phip))
+    def deflation_bias(P):
+        return deflation_bias_terms(P, targets, w2d, dsig, inv2L2)
 
     def eval_state(Q, p):                                              # :814-837
         P = _herm(project(_density_from_Qp(Q, p)))
