@@ -301,6 +301,56 @@ def symmetry_bench(J, syn, dev, args):
             "GBps": algo / (t / n) / 1e9}
 
 
+def superlattice_bench(J, dev, args):
+    """Superlattice Fock build (superlattice.py:167-377) on a hexagonal basis of
+    Gamma + two shells (13 G vectors), nk = 6, 2 orbitals per G (D = 26): extended
+    grid 60 x 60, every dG channel at once.  Algorithmic bytes per build: the
+    planes written by the scatter, read and written by the exchange, and the
+    (k, p, q) arrays read (P, h, two plane reads per output) and written
+    (Sigma, F).  The oracle's NumPy streaming build of the same layout is timed
+    beside it on one host core."""
+    import torch
+    from jax_hf_amd.superlattice import build_extended_layout, make_superlattice_build_fock_fn
+    b0 = 4 * np.pi / np.sqrt(3)
+    B = np.stack([np.array([np.sqrt(3) / 2, -0.5]) * b0, np.array([0.0, 1.0]) * b0], axis=1)
+    fr = np.array([(a, b) for a in range(-3, 4) for b in range(-3, 4)], np.int64)
+    norms = np.linalg.norm(fr @ B.T, axis=1)
+    g = fr[norms <= np.sort(np.unique(np.round(norms, 6)))[2] + 1e-3]
+    nk, dorb = 6, 2
+    n_G = g.shape[0]
+    D = n_G * dorb
+    vf = lambda q: 1.0 / np.sqrt(q ** 2 + 0.5 ** 2)  # noqa: E731
+    lay = build_extended_layout(g, B, nk, nk, vf, 1.0 / nk ** 2, device=dev)
+    rng = np.random.default_rng(3)
+    HH = rng.standard_normal((n_G, n_G))
+    HH = HH + HH.T
+    np.fill_diagonal(HH, 0.0)
+    fb = make_superlattice_build_fock_fn(lay, n_G, dorb, nk, nk, HH, 2.0)
+    P = rng.standard_normal((nk, nk, D, D)) + 1j * rng.standard_normal((nk, nk, D, D))
+    h = rng.standard_normal((nk, nk, D, D)) + 0j
+    w = np.full((nk, nk), 1.0 / nk ** 2)
+    Pt, ht, wt = (torch.from_numpy(x).to(dev) for x in (P, h, w))
+    kw = dict(h=ht, refP=None, w2d=wt, include_exchange=True, include_hartree=True)
+    fb(Pt, **kw)
+    n = 200
+    _, t = _median_time(lambda: [fb(Pt, **kw) for _ in range(n)], 3)
+    nplanes = lay.n_delta * dorb * dorb * lay.N_ext_x * lay.N_ext_y
+    algo = 16 * (3 * nplanes + 6 * nk * nk * D * D)
+    out = {"workload": (f"hexagonal Gamma + 2 shells (n_G={n_G}), nk={nk}x{nk}, dim_orb={dorb}, extended grid "
+                        f"{lay.N_ext_x}x{lay.N_ext_y}, {lay.n_delta} dG channels, exchange + Hartree, complex128"),
+           "us_per_build": t / n * 1e6, "algorithmic_bytes": algo, "GBps": algo / (t / n) / 1e9}
+    if not args.no_cpu_baseline:
+        from oracle import superlattice as OS
+        ref = OS.build_extended_layout(g, B, nk, nk, vf, 1.0 / nk ** 2)
+        f0 = OS.make_superlattice_build_fock_fn(ref, n_G, dorb, nk, nk, HH, 2.0)
+        t0 = time.perf_counter()
+        S0, _, F0 = f0(P, h=h, w2d=w, include_exchange=True, include_hartree=True)
+        out["cpu_oracle_ms"] = (time.perf_counter() - t0) * 1e3
+        F1 = fb(Pt, **kw)[2].cpu().numpy()
+        out["F_max_rel_vs_oracle"] = float(np.abs(F1 - F0).max() / np.abs(F0).max())
+    return out
+
+
 SOLVE_T = 0.1
 POLISH_TOL = 1e-6
 
@@ -515,6 +565,7 @@ def main():
             out["newton_64x64_nb8"] = newton_bench(J, syn, dev, args)
             out["adjoint_48x48_nb16"] = adjoint_bench(J, syn, dev, args)
             out["symmetry_96x96_nb16"] = symmetry_bench(J, syn, dev, args)
+            out["superlattice_nk6_nG13"] = superlattice_bench(J, dev, args)
     out["fock"] = fock
     out["fock_build_GBps"] = fock["GBps"]
     out["roofline"] = roof
