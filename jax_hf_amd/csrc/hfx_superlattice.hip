@@ -7,16 +7,18 @@
 // once instead of streaming them:
 //   1. sl_scatter_kernel writes the slot planes X[d][a,b](x, y) of the extended
 //      grid: the (nkx, nky) block of (P - refP)[(i,a), (j,b)] sits at tile
-//      g_off[i] of plane d = dG(i, j); everything else is zero.
+//      g_off[i] of plane d = dG(i, j) (a (dG, tile) -> pair table); everything
+//      else is zero.
 //   2. the planar exchange of the plan (hfx_selfenergy_planar, one shared
 //      kernel plane V_lag_fft) turns each plane into -ifft2(fft2(X) V), written
 //      to a second set of planes.
 //   3. sl_gather_kernel reads every (k, p, q) of Sigma back from its plane and
 //      tile, adds the k-independent Hartree shift and forms F = herm(h + Sigma)
 //      in the same pass.
-// The Hartree shift (superlattice.py:322-357) is two small launches: weighted
-// k sums of the orbital-diagonal blocks, then the per-dG sums contracted with
-// the orbital-resolved Coulomb table.
+// The Hartree shift (superlattice.py:322-357) needs one small launch before
+// the gather: the weighted k sums of the orbital-diagonal blocks summed over
+// the pairs of each dG, R(d, c); the gather contracts them with the
+// orbital-resolved Coulomb table per entry.
 #include "hfx_common.h"
 #include "hfx_internal.h"
 
@@ -35,46 +37,53 @@ struct SlGeom {
   int nkx, nky, nG, dorb, ndelta, Nx, Ny;
 };
 
+// One workgroup row per dG (blockIdx.y = d); each thread owns one point (x, y)
+// of the extended grid and writes it in all dim_orb^2 planes of d.
+// tile_pair[d][tile] = i * n_G + j for the pair (G_i, G_j) of d whose block sits
+// at that tile (the tile of -G_i), or -1.
 template <typename T>
 __global__ void sl_scatter_kernel(SlGeom g, const cx<T>* __restrict__ P, const cx<T>* __restrict__ refP,
-                                  const int* __restrict__ tile_G, const int* __restrict__ partner,
-                                  cx<T>* __restrict__ planes) {
-  const int D = g.nG * g.dorb, nch = g.dorb * g.dorb, tiles_y = g.Ny / g.nky;
-  const size_t plane = (size_t)g.Nx * g.Ny, total = (size_t)g.ndelta * nch * plane;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-    const int xy = (int)(e % plane);
-    const int t = (int)((e / plane) % nch), d = (int)(e / (plane * nch));
-    const int x = xy / g.Ny, y = xy % g.Ny, tx = x / g.nkx, ty = y / g.nky;
-    cx<T> v{0, 0};
-    const int i = tile_G[tx * tiles_y + ty];
-    if (i >= 0) {
-      const int j = partner[d * g.nG + i];
-      if (j >= 0) {
-        const int a = t / g.dorb, b = t % g.dorb;
-        const size_t k = (size_t)(x - tx * g.nkx) * g.nky + (y - ty * g.nky);
-        const size_t src = (k * D + i * g.dorb + a) * D + j * g.dorb + b;
-        v = P[src];
-        if (refP) v = v - refP[src];
-      }
+                                  const int* __restrict__ tile_pair, cx<T>* __restrict__ planes) {
+  const int D = g.nG * g.dorb, nch = g.dorb * g.dorb, tiles_x = g.Nx / g.nkx, tiles_y = g.Ny / g.nky;
+  const int NN = g.Nx * g.Ny;
+  const int xy = blockIdx.x * blockDim.x + threadIdx.x;
+  if (xy >= NN) return;
+  const int d = blockIdx.y;
+  const int x = xy / g.Ny, y = xy - x * g.Ny, tx = x / g.nkx, ty = y / g.nky;
+  const int pr = tile_pair[(d * tiles_x + tx) * tiles_y + ty];
+  cx<T>* out = planes + (size_t)d * nch * NN + xy;
+  if (pr < 0) {
+    for (int t = 0; t < nch; ++t) out[(size_t)t * NN] = cx<T>{0, 0};
+    return;
+  }
+  const int i = pr / g.nG, j = pr - i * g.nG;
+  const int k = (x - tx * g.nkx) * g.nky + (y - ty * g.nky);
+  const size_t row0 = ((size_t)k * D + i * g.dorb) * D + j * g.dorb;
+  for (int a = 0; a < g.dorb; ++a) {
+    for (int b = 0; b < g.dorb; ++b) {
+      const size_t src = row0 + (size_t)a * D + b;
+      cx<T> v = P[src];
+      if (refP) v = v - refP[src];
+      out[(size_t)(a * g.dorb + b) * NN] = v;
     }
-    planes[e] = v;
   }
 }
 
-// rho_po[(A nG + B) dorb + c] = sum_k w[k] (P - refP)[k, (A, c), (B, c)]
+// R[d dorb + c] = sum over the (A, B) pairs of dG_d and over k of
+// w[k] (P - refP)[k, (A, c), (B, c)]: one workgroup per (d, c).
 template <typename T>
 __global__ void sl_hartree_moments_kernel(int nk, int nG, int dorb, const cx<T>* __restrict__ P,
                                           const cx<T>* __restrict__ refP, const double* __restrict__ w,
-                                          cx<double>* __restrict__ rho_po) {
-  const int D = nG * dorb, n = nG * nG * dorb;
-  const int o = blockIdx.x;                      // one workgroup per output
-  if (o >= n) return;
-  const int c = o % dorb, AB = o / dorb, A = AB / nG, B = AB % nG;
-  const size_t off = (size_t)(A * dorb + c) * D + B * dorb + c;
+                                          const int* __restrict__ dstart, const int* __restrict__ di,
+                                          const int* __restrict__ dj, cx<double>* __restrict__ R) {
+  const int D = nG * dorb, d = blockIdx.x / dorb, c = blockIdx.x - d * dorb;
+  const int s0 = dstart[d], npair = dstart[d + 1] - s0;
   double re = 0.0, im = 0.0;
-  for (int k = threadIdx.x; k < nk; k += blockDim.x) {
-    cx<T> v = P[(size_t)k * D * D + off];
-    if (refP) v = v - refP[(size_t)k * D * D + off];
+  for (int u = threadIdx.x; u < npair * nk; u += blockDim.x) {
+    const int s = s0 + u / nk, k = u % nk;
+    const size_t off = (size_t)k * D * D + (size_t)(di[s] * dorb + c) * D + dj[s] * dorb + c;
+    cx<T> v = P[off];
+    if (refP) v = v - refP[off];
     re += w[k] * (double)v.re;
     im += w[k] * (double)v.im;
   }
@@ -93,39 +102,29 @@ __global__ void sl_hartree_moments_kernel(int nk, int nG, int dorb, const cx<T>*
       sr += sh[q];
       si += sh[nw + q];
     }
-    rho_po[o] = cx<double>{sr, si};
+    R[blockIdx.x] = cx<double>{sr, si};
   }
 }
 
-// sigma_H[(A, a), (B, b)] = delta_ab deg sum_c HH[A, B, a, c] R(dG(A, B), c),
-// R(d, c) = sum over the (A', B') pairs of d of rho_po[A', B', c]
-template <typename T>
-__global__ void sl_hartree_shift_kernel(int nG, int dorb, const int* __restrict__ pair_to_delta,
-                                        const int* __restrict__ dstart, const int* __restrict__ di,
-                                        const int* __restrict__ dj, const double* __restrict__ HH, double deg,
-                                        const cx<double>* __restrict__ rho_po, cx<T>* __restrict__ sigH) {
-  const int D = nG * dorb;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= D * D) return;
-  const int p = e / D, q = e % D, A = p / dorb, a = p % dorb, B = q / dorb, b = q % dorb;
+struct SlHartree {
+  const cx<double>* R;   // (n_delta, dorb) or NULL
+  const double* HH;      // (nG, nG, dorb, dorb)
+  double deg;
+};
+
+// sigma_H[(A, a), (B, b)] = delta_ab deg sum_c HH[A, B, a, c] R[dG(A, B), c]
+__device__ __forceinline__ cx<double> sl_hartree_at(const SlGeom& g, const SlHartree& hs, const int* p2d, int A,
+                                                    int a, int B, int b) {
   double re = 0.0, im = 0.0;
   if (a == b) {
-    const int d = pair_to_delta[A * nG + B];
-    for (int c = 0; c < dorb; ++c) {
-      double rr = 0.0, ri = 0.0;
-      for (int s = dstart[d]; s < dstart[d + 1]; ++s) {
-        const cx<double> v = rho_po[(di[s] * nG + dj[s]) * dorb + c];
-        rr += v.re;
-        ri += v.im;
-      }
-      const double hv = HH[((size_t)(A * nG + B) * dorb + a) * dorb + c];
-      re += hv * rr;
-      im += hv * ri;
+    const cx<double>* r = hs.R + p2d[A * g.nG + B] * g.dorb;
+    const double* hh = hs.HH + ((size_t)(A * g.nG + B) * g.dorb + a) * g.dorb;
+    for (int c = 0; c < g.dorb; ++c) {
+      re += hh[c] * r[c].re;
+      im += hh[c] * r[c].im;
     }
-    re *= deg;
-    im *= deg;
   }
-  sigH[e] = cx<T>{(T)re, (T)im};
+  return cx<double>{hs.deg * re, hs.deg * im};
 }
 
 template <typename T>
@@ -136,42 +135,41 @@ __device__ __forceinline__ cx<T> sl_plane_at(const SlGeom& g, const cx<T>* plane
   return planes[(((size_t)d * g.dorb * g.dorb + a * g.dorb + b) * g.Nx + x) * g.Ny + y];
 }
 
+// blockIdx.y = k, threads over the D x D entries of that k-point.
 template <typename T>
 __global__ void sl_gather_kernel(SlGeom g, const cx<T>* __restrict__ planes, const int* __restrict__ p2d,
-                                 const int* __restrict__ goff, const cx<T>* __restrict__ sigH,
-                                 const cx<T>* __restrict__ h, cx<T>* __restrict__ Sigma, cx<T>* __restrict__ F) {
-  const int D = g.nG * g.dorb;
-  const size_t DD = (size_t)D * D, total = (size_t)g.nkx * g.nky * DD;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-    const size_t k = e / DD;
-    const int pq = (int)(e % DD), p = pq / D, q = pq % D;
-    const int kx = (int)(k / g.nky), ky = (int)(k % g.nky);
-    const int i = p / g.dorb, a = p % g.dorb, j = q / g.dorb, b = q % g.dorb;
-    cx<T> s{0, 0}, st{0, 0};
-    if (planes) {
-      s = sl_plane_at(g, planes, p2d, goff, i, a, j, b, kx, ky);
-      st = sl_plane_at(g, planes, p2d, goff, j, b, i, a, kx, ky);
-    }
-    if (sigH) {
-      s = s + sigH[pq];
-      st = st + sigH[(size_t)q * D + p];
-    }
-    Sigma[e] = s;
-    const cx<T> f = h[e] + s, ft = h[k * DD + (size_t)q * D + p] + st;
-    F[e] = cx<T>{(T)0.5 * (f.re + ft.re), (T)0.5 * (f.im - ft.im)};
+                                 const int* __restrict__ goff, SlHartree hs, const cx<T>* __restrict__ h,
+                                 cx<T>* __restrict__ Sigma, cx<T>* __restrict__ F) {
+  const int D = g.nG * g.dorb, DD = D * D;
+  const int pq = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pq >= DD) return;
+  const int k = blockIdx.y, kx = k / g.nky, ky = k - kx * g.nky;
+  const int p = pq / D, q = pq - p * D;
+  const int i = p / g.dorb, a = p - i * g.dorb, j = q / g.dorb, b = q - j * g.dorb;
+  cx<T> s{0, 0}, st{0, 0};
+  if (planes) {
+    s = sl_plane_at(g, planes, p2d, goff, i, a, j, b, kx, ky);
+    st = sl_plane_at(g, planes, p2d, goff, j, b, i, a, kx, ky);
   }
+  if (hs.R) {
+    s = s + cvt<T>(sl_hartree_at(g, hs, p2d, i, a, j, b));
+    st = st + cvt<T>(sl_hartree_at(g, hs, p2d, j, b, i, a));
+  }
+  const size_t base = (size_t)k * DD;
+  Sigma[base + pq] = s;
+  const cx<T> f = h[base + pq] + s, ft = h[base + (size_t)q * D + p] + st;
+  F[base + pq] = cx<T>{(T)0.5 * (f.re + ft.re), (T)0.5 * (f.im - ft.im)};
 }
 
 template <typename T>
-static int sl_build(const SlGeom& g, const void* P, const void* refP, const int* tile_G, const int* partner,
-                    const int* p2d, const int* goff, const hfx_plan* plan, const void* Vlag, void* planes,
-                    void* splanes, int hartree, const double* w, const int* dstart, const int* di, const int* dj, const double* HH,
-                    double deg, void* rho_po, void* sigH, const void* h, void* Sigma, void* F, hipStream_t s) {
-  const int D = g.nG * g.dorb;
-  const size_t nplanes = (size_t)g.ndelta * g.dorb * g.dorb * g.Nx * g.Ny;
+static int sl_build(const SlGeom& g, const void* P, const void* refP, const int* tile_pair, const int* p2d, const int* goff, const hfx_plan* plan, const void* Vlag, void* planes,
+                    void* splanes, int hartree, const double* w, const int* dstart, const int* di, const int* dj,
+                    const double* HH, double deg, void* R, const void* h, void* Sigma, void* F, hipStream_t s) {
+  const int D = g.nG * g.dorb, nk = g.nkx * g.nky;
   if (planes) {
-    hipLaunchKernelGGL(sl_scatter_kernel<T>, dim3(sl_grid(nplanes)), dim3(SL_BLOCK), 0, s, g, (const cx<T>*)P,
-                       (const cx<T>*)refP, tile_G, partner, (cx<T>*)planes);
+    const dim3 grid((g.Nx * g.Ny + SL_BLOCK - 1) / SL_BLOCK, g.ndelta);
+    hipLaunchKernelGGL(sl_scatter_kernel<T>, grid, dim3(SL_BLOCK), 0, s, g, (const cx<T>*)P, (const cx<T>*)refP,
+                       tile_pair, (cx<T>*)planes);
     int rc = check_launch("sl_scatter_kernel");
     if (rc) return rc;
     rc = hfx_selfenergy_planar(plan, g.ndelta, planes, Vlag, 0, nullptr, g.dorb * g.dorb, 0, 0, nullptr, splanes,
@@ -179,20 +177,15 @@ static int sl_build(const SlGeom& g, const void* P, const void* refP, const int*
     if (rc) return rc;
   }
   if (hartree) {
-    const int nmom = g.nG * g.nG * g.dorb;
-    hipLaunchKernelGGL(sl_hartree_moments_kernel<T>, dim3(nmom), dim3(SL_BLOCK), 0, s, g.nkx * g.nky, g.nG, g.dorb,
-                       (const cx<T>*)P, (const cx<T>*)refP, w, (cx<double>*)rho_po);
+    hipLaunchKernelGGL(sl_hartree_moments_kernel<T>, dim3(g.ndelta * g.dorb), dim3(SL_BLOCK), 0, s, nk, g.nG,
+                       g.dorb, (const cx<T>*)P, (const cx<T>*)refP, w, dstart, di, dj, (cx<double>*)R);
     int rc = check_launch("sl_hartree_moments_kernel");
     if (rc) return rc;
-    hipLaunchKernelGGL(sl_hartree_shift_kernel<T>, dim3((D * D + SL_BLOCK - 1) / SL_BLOCK), dim3(SL_BLOCK), 0, s,
-                       g.nG, g.dorb, p2d, dstart, di, dj, HH, deg, (const cx<double>*)rho_po, (cx<T>*)sigH);
-    rc = check_launch("sl_hartree_shift_kernel");
-    if (rc) return rc;
   }
-  const size_t nout = (size_t)g.nkx * g.nky * D * D;
-  hipLaunchKernelGGL(sl_gather_kernel<T>, dim3(sl_grid(nout)), dim3(SL_BLOCK), 0, s, g,
-                     planes ? (const cx<T>*)splanes : nullptr, p2d,
-                     goff, hartree ? (const cx<T>*)sigH : nullptr, (const cx<T>*)h, (cx<T>*)Sigma, (cx<T>*)F);
+  const SlHartree hs{hartree ? (const cx<double>*)R : nullptr, HH, deg};
+  const dim3 grid((D * D + SL_BLOCK - 1) / SL_BLOCK, nk);
+  hipLaunchKernelGGL(sl_gather_kernel<T>, grid, dim3(SL_BLOCK), 0, s, g, planes ? (const cx<T>*)splanes : nullptr,
+                     p2d, goff, hs, (const cx<T>*)h, (cx<T>*)Sigma, (cx<T>*)F);
   return check_launch("sl_gather_kernel");
 }
 
@@ -203,35 +196,38 @@ using namespace hfx;
 extern "C" {
 
 int hfx_superlattice_fock(int dtype, int nkx, int nky, int n_G, int dim_orb, int n_delta, int Nx, int Ny,
-                          const void* P, const void* refP, const int* tile_G, const int* partner,
-                          const int* pair_to_delta, const int* g_off, const hfx_plan* plan, const void* V_lag_fft,
-                          void* planes, void* sigma_planes, int hartree, const double* w2d, const int* delta_start,
-                          const int* delta_i, const int* delta_j, const double* HH_orb, double degeneracy,
-                          void* rho_scratch, void* sigma_H, const void* h, void* Sigma, void* F, void* stream) {
+                          const void* P, const void* refP, const int* tile_pair, const int* pair_to_delta, const int* g_off, const hfx_plan* plan, const void* V_lag_fft,
+                          void* planes, void* sigma_planes, int hartree, const double* w2d,
+                          const int* delta_start, const int* delta_i, const int* delta_j, const double* HH_orb,
+                          double degeneracy, void* delta_scratch, const void* h, void* Sigma, void* F,
+                          void* stream) {
   if (dtype != HFX_C128 && dtype != HFX_C64)
     return set_error(HFX_EINVAL, "hfx_superlattice_fock: dtype must be HFX_C128 or HFX_C64");
   if (nkx < 1 || nky < 1 || n_G < 1 || dim_orb < 1 || n_delta < 1 || Nx < nkx || Ny < nky || Nx % nkx ||
       Ny % nky)
     return set_error(HFX_EINVAL, "hfx_superlattice_fock: bad geometry (nk %dx%d, n_G %d, dim_orb %d, grid %dx%d)",
                      nkx, nky, n_G, dim_orb, Nx, Ny);
+  if ((long long)n_delta * dim_orb * dim_orb > 65535 || (long long)nkx * nky > 65535 ||
+      (long long)n_G * dim_orb * n_G * dim_orb > (1LL << 30))
+    return set_error(HFX_EUNSUPPORTED, "hfx_superlattice_fock: %d planes / %d k-points / D=%d out of range",
+                     n_delta * dim_orb * dim_orb, nkx * nky, n_G * dim_orb);
   if (!P || !pair_to_delta || !g_off || !h || !Sigma || !F)
     return set_error(HFX_EINVAL, "hfx_superlattice_fock: NULL argument");
-  if (planes && (!tile_G || !partner || !plan || !V_lag_fft || !sigma_planes || sigma_planes == planes))
+  if (planes && (!tile_pair || !plan || !V_lag_fft || !sigma_planes || sigma_planes == planes))
     return set_error(HFX_EINVAL,
-                     "hfx_superlattice_fock: the exchange needs tile_G, partner, plan, V_lag_fft and a separate "
+                     "hfx_superlattice_fock: the exchange needs tile_pair, plan, V_lag_fft and a separate "
                      "sigma_planes buffer");
-  if (hartree && (!w2d || !delta_start || !delta_i || !delta_j || !HH_orb || !rho_scratch || !sigma_H))
+  if (hartree && (!w2d || !delta_start || !delta_i || !delta_j || !HH_orb || !delta_scratch))
     return set_error(HFX_EINVAL, "hfx_superlattice_fock: the Hartree shift needs its tables and scratch");
   const SlGeom g{nkx, nky, n_G, dim_orb, n_delta, Nx, Ny};
   const hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_C128)
-    return sl_build<double>(g, P, refP, tile_G, partner, pair_to_delta, g_off, plan, V_lag_fft, planes, sigma_planes,
-                            hartree,
-                            w2d, delta_start, delta_i, delta_j, HH_orb, degeneracy, rho_scratch, sigma_H, h, Sigma,
+    return sl_build<double>(g, P, refP, tile_pair, pair_to_delta, g_off, plan, V_lag_fft, planes, sigma_planes,
+                            hartree, w2d, delta_start, delta_i, delta_j, HH_orb, degeneracy, delta_scratch, h, Sigma,
                             F, s);
-  return sl_build<float>(g, P, refP, tile_G, partner, pair_to_delta, g_off, plan, V_lag_fft, planes, sigma_planes,
-                         hartree, w2d,
-                         delta_start, delta_i, delta_j, HH_orb, degeneracy, rho_scratch, sigma_H, h, Sigma, F, s);
+  return sl_build<float>(g, P, refP, tile_pair, pair_to_delta, g_off, plan, V_lag_fft, planes, sigma_planes,
+                         hartree, w2d, delta_start, delta_i, delta_j, HH_orb, degeneracy, delta_scratch, h, Sigma, F,
+                         s);
 }
 
 }  // extern "C"
