@@ -305,9 +305,9 @@ def superlattice_bench(J, dev, args):
     """Superlattice Fock build (superlattice.py:167-377) on a hexagonal basis of
     Gamma + two shells (13 G vectors), nk = 6, 2 orbitals per G (D = 26): extended
     grid 60 x 60, every dG channel at once.  Algorithmic bytes per build: the
-    planes written by the scatter, read and written by the exchange, and the
-    (k, p, q) arrays read (P, h, two plane reads per output) and written
-    (Sigma, F).  The oracle's NumPy streaming build of the same layout is timed
+    pair blocks written by the scatter, the planes read and written by the
+    exchange, and the (k, p, q) arrays read (P, h, two plane reads per output)
+    and written (Sigma, F).  The oracle's NumPy streaming build of the same layout is timed
     beside it on one host core."""
     import torch
     from jax_hf_amd.superlattice import build_extended_layout, make_superlattice_build_fock_fn
@@ -335,7 +335,7 @@ def superlattice_bench(J, dev, args):
     n = 200
     _, t = _median_time(lambda: [fb(Pt, **kw) for _ in range(n)], 3)
     nplanes = lay.n_delta * dorb * dorb * lay.N_ext_x * lay.N_ext_y
-    algo = 16 * (3 * nplanes + 6 * nk * nk * D * D)
+    algo = 16 * (2 * nplanes + n_G * n_G * dorb * dorb * nk * nk + 6 * nk * nk * D * D)
     out = {"workload": (f"hexagonal Gamma + 2 shells (n_G={n_G}), nk={nk}x{nk}, dim_orb={dorb}, extended grid "
                         f"{lay.N_ext_x}x{lay.N_ext_y}, {lay.n_delta} dG channels, exchange + Hartree, complex128"),
            "us_per_build": t / n * 1e6, "algorithmic_bytes": algo, "GBps": algo / (t / n) / 1e9}

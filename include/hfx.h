@@ -328,8 +328,9 @@ int hfx_resample_kgrid(int kind, int n0, int n1, size_t m, int nk, const void* X
  * P, refP (NULL = 0), h, Sigma, F: (nkx, nky, D, D), D = n_G * dim_orb, in the
  * plan's precision (dtype HFX_C128 / HFX_C64).  Exchange (planes != NULL): the
  * dG channel blocks of P - refP are scattered into planes (n_delta, dim_orb^2,
- * Nx, Ny) -- tile_pair (n_delta, Nx/nkx, Ny/nky) holds i * n_G + j for the
- * pair (G_i, G_j) of dG_d whose block sits at that tile, or -1 -- convolved by
+ * Nx, Ny) at tile g_off[i] of plane pair_to_delta[i][j] for every (i, j) in
+ * the CSR lists delta_i / delta_j (planes must be zero-filled by the caller
+ * before the first call: a build writes only the pair blocks) -- convolved by
  * the (Nx, Ny, dim_orb) plan with the single plane V_lag_fft into sigma_planes
  * (-ifft2(fft2(X) V_lag_fft)), and gathered back through pair_to_delta (n_G,
  * n_G) and g_off (n_G, 2) tile offsets.  Hartree (hartree != 0): the
@@ -337,10 +338,9 @@ int hfx_resample_kgrid(int kind, int n0, int n1, size_t m, int nk, const void* X
  * on the orbital diagonal, with rho_c(d) the w2d-weighted k sum of P - refP
  * over the pairs of d (CSR delta_start / delta_i / delta_j); delta_scratch
  * n_delta * dim_orb complex128.  Sigma = exchange + Hartree, F = herm(h + Sigma).
- * Limits: n_delta * dim_orb^2 and nkx * nky <= 65535. */
+ * Limits: n_G^2, n_delta * dim_orb^2 and nkx * nky <= 65535. */
 int hfx_superlattice_fock(int dtype, int nkx, int nky, int n_G, int dim_orb, int n_delta, int Nx, int Ny,
-                          const void* P, const void* refP, const int* tile_pair, const int* pair_to_delta,
-                          const int* g_off, const hfx_plan* plan, const void* V_lag_fft,
+                          const void* P, const void* refP, const int* pair_to_delta, const int* g_off, const hfx_plan* plan, const void* V_lag_fft,
                           void* planes, void* sigma_planes, int hartree, const double* w2d,
                           const int* delta_start, const int* delta_i, const int* delta_j, const double* HH_orb,
                           double degeneracy, void* delta_scratch, const void* h, void* Sigma, void* F,

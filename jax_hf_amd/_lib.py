@@ -18,7 +18,7 @@ LIB_PATH = _HERE / "libhfx.so"
 
 HFX_C128 = 0
 HFX_C64 = 1
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 _lib = None
 _lock = threading.Lock()
@@ -86,7 +86,7 @@ _SIGS = {
     "hfx_orthonormalize": [i32, i32, sz, vp, i32, vp],
     "hfx_symmetry_average": [i32, i32, sz, i32, vp, vp, vp, C.c_ulonglong, vp, f64, vp, vp],
     "hfx_resample_kgrid": [i32, i32, i32, sz, i32, vp, vp, vp],
-    "hfx_superlattice_fock": [i32, i32, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp,
+    "hfx_superlattice_fock": [i32, i32, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp,
                               vp, vp, i32, vp, vp, vp, vp, vp, f64, vp, vp, vp, vp, vp],
     "hfx_diis_coefficients": [vp, vp, i32, i32, i32, vp, vp],
     "hfx_diis_coefficients_batched": [vp, sz, vp, i32, i32, i32, i32, vp, vp],

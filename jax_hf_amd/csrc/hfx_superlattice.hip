@@ -7,8 +7,8 @@
 // once instead of streaming them:
 //   1. sl_scatter_kernel writes the slot planes X[d][a,b](x, y) of the extended
 //      grid: the (nkx, nky) block of (P - refP)[(i,a), (j,b)] sits at tile
-//      g_off[i] of plane d = dG(i, j) (a (dG, tile) -> pair table); everything
-//      else is zero.
+//      g_off[i] of plane d = dG(i, j); everything else is zero (filled once by
+//      the caller, never written by a build).
 //   2. the planar exchange of the plan (hfx_selfenergy_planar, one shared
 //      kernel plane V_lag_fft) turns each plane into -ifft2(fft2(X) V), written
 //      to a second set of planes.
@@ -37,36 +37,27 @@ struct SlGeom {
   int nkx, nky, nG, dorb, ndelta, Nx, Ny;
 };
 
-// One workgroup row per dG (blockIdx.y = d); each thread owns one point (x, y)
-// of the extended grid and writes it in all dim_orb^2 planes of d.
-// tile_pair[d][tile] = i * n_G + j for the pair (G_i, G_j) of d whose block sits
-// at that tile (the tile of -G_i), or -1.
+// Writes only the pair blocks: one workgroup row per (G_i, G_j) pair
+// (blockIdx.y = CSR position s), threads over (channel, k) with k fastest so
+// the stores run along the extended-grid rows.  The rest of every plane is
+// zero and is never written (the exchange reads planes and writes a second
+// buffer), so the caller zero-fills planes once.
 template <typename T>
 __global__ void sl_scatter_kernel(SlGeom g, const cx<T>* __restrict__ P, const cx<T>* __restrict__ refP,
-                                  const int* __restrict__ tile_pair, cx<T>* __restrict__ planes) {
-  const int D = g.nG * g.dorb, nch = g.dorb * g.dorb, tiles_x = g.Nx / g.nkx, tiles_y = g.Ny / g.nky;
-  const int NN = g.Nx * g.Ny;
-  const int xy = blockIdx.x * blockDim.x + threadIdx.x;
-  if (xy >= NN) return;
-  const int d = blockIdx.y;
-  const int x = xy / g.Ny, y = xy - x * g.Ny, tx = x / g.nkx, ty = y / g.nky;
-  const int pr = tile_pair[(d * tiles_x + tx) * tiles_y + ty];
-  cx<T>* out = planes + (size_t)d * nch * NN + xy;
-  if (pr < 0) {
-    for (int t = 0; t < nch; ++t) out[(size_t)t * NN] = cx<T>{0, 0};
-    return;
-  }
-  const int i = pr / g.nG, j = pr - i * g.nG;
-  const int k = (x - tx * g.nkx) * g.nky + (y - ty * g.nky);
-  const size_t row0 = ((size_t)k * D + i * g.dorb) * D + j * g.dorb;
-  for (int a = 0; a < g.dorb; ++a) {
-    for (int b = 0; b < g.dorb; ++b) {
-      const size_t src = row0 + (size_t)a * D + b;
-      cx<T> v = P[src];
-      if (refP) v = v - refP[src];
-      out[(size_t)(a * g.dorb + b) * NN] = v;
-    }
-  }
+                                  const int* __restrict__ di, const int* __restrict__ dj,
+                                  const int* __restrict__ p2d, const int* __restrict__ goff,
+                                  cx<T>* __restrict__ planes) {
+  const int D = g.nG * g.dorb, nch = g.dorb * g.dorb, nk = g.nkx * g.nky;
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= nch * nk) return;
+  const int s = blockIdx.y, i = di[s], j = dj[s], d = p2d[i * g.nG + j];
+  const int t = u / nk, k = u - t * nk, a = t / g.dorb, b = t - a * g.dorb;
+  const int kx = k / g.nky, ky = k - kx * g.nky;
+  const size_t src = ((size_t)k * D + i * g.dorb + a) * D + j * g.dorb + b;
+  cx<T> v = P[src];
+  if (refP) v = v - refP[src];
+  const int x = g.nkx * goff[2 * i] + kx, y = g.nky * goff[2 * i + 1] + ky;
+  planes[(((size_t)d * nch + t) * g.Nx + x) * g.Ny + y] = v;
 }
 
 // R[d dorb + c] = sum over the (A, B) pairs of dG_d and over k of
@@ -162,14 +153,14 @@ __global__ void sl_gather_kernel(SlGeom g, const cx<T>* __restrict__ planes, con
 }
 
 template <typename T>
-static int sl_build(const SlGeom& g, const void* P, const void* refP, const int* tile_pair, const int* p2d, const int* goff, const hfx_plan* plan, const void* Vlag, void* planes,
+static int sl_build(const SlGeom& g, const void* P, const void* refP, const int* p2d, const int* goff, const hfx_plan* plan, const void* Vlag, void* planes,
                     void* splanes, int hartree, const double* w, const int* dstart, const int* di, const int* dj,
                     const double* HH, double deg, void* R, const void* h, void* Sigma, void* F, hipStream_t s) {
   const int D = g.nG * g.dorb, nk = g.nkx * g.nky;
   if (planes) {
-    const dim3 grid((g.Nx * g.Ny + SL_BLOCK - 1) / SL_BLOCK, g.ndelta);
-    hipLaunchKernelGGL(sl_scatter_kernel<T>, grid, dim3(SL_BLOCK), 0, s, g, (const cx<T>*)P, (const cx<T>*)refP,
-                       tile_pair, (cx<T>*)planes);
+    const dim3 grid((g.dorb * g.dorb * nk + SL_BLOCK - 1) / SL_BLOCK, g.nG * g.nG);
+    hipLaunchKernelGGL(sl_scatter_kernel<T>, grid, dim3(SL_BLOCK), 0, s, g, (const cx<T>*)P, (const cx<T>*)refP, di,
+                       dj, p2d, goff, (cx<T>*)planes);
     int rc = check_launch("sl_scatter_kernel");
     if (rc) return rc;
     rc = hfx_selfenergy_planar(plan, g.ndelta, planes, Vlag, 0, nullptr, g.dorb * g.dorb, 0, 0, nullptr, splanes,
@@ -196,7 +187,7 @@ using namespace hfx;
 extern "C" {
 
 int hfx_superlattice_fock(int dtype, int nkx, int nky, int n_G, int dim_orb, int n_delta, int Nx, int Ny,
-                          const void* P, const void* refP, const int* tile_pair, const int* pair_to_delta, const int* g_off, const hfx_plan* plan, const void* V_lag_fft,
+                          const void* P, const void* refP, const int* pair_to_delta, const int* g_off, const hfx_plan* plan, const void* V_lag_fft,
                           void* planes, void* sigma_planes, int hartree, const double* w2d,
                           const int* delta_start, const int* delta_i, const int* delta_j, const double* HH_orb,
                           double degeneracy, void* delta_scratch, const void* h, void* Sigma, void* F,
@@ -207,25 +198,25 @@ int hfx_superlattice_fock(int dtype, int nkx, int nky, int n_G, int dim_orb, int
       Ny % nky)
     return set_error(HFX_EINVAL, "hfx_superlattice_fock: bad geometry (nk %dx%d, n_G %d, dim_orb %d, grid %dx%d)",
                      nkx, nky, n_G, dim_orb, Nx, Ny);
-  if ((long long)n_delta * dim_orb * dim_orb > 65535 || (long long)nkx * nky > 65535 ||
+  if ((long long)n_delta * dim_orb * dim_orb > 65535 || (long long)nkx * nky > 65535 || n_G * n_G > 65535 ||
       (long long)n_G * dim_orb * n_G * dim_orb > (1LL << 30))
     return set_error(HFX_EUNSUPPORTED, "hfx_superlattice_fock: %d planes / %d k-points / D=%d out of range",
                      n_delta * dim_orb * dim_orb, nkx * nky, n_G * dim_orb);
   if (!P || !pair_to_delta || !g_off || !h || !Sigma || !F)
     return set_error(HFX_EINVAL, "hfx_superlattice_fock: NULL argument");
-  if (planes && (!tile_pair || !plan || !V_lag_fft || !sigma_planes || sigma_planes == planes))
+  if (planes && (!delta_i || !delta_j || !plan || !V_lag_fft || !sigma_planes || sigma_planes == planes))
     return set_error(HFX_EINVAL,
-                     "hfx_superlattice_fock: the exchange needs tile_pair, plan, V_lag_fft and a separate "
+                     "hfx_superlattice_fock: the exchange needs delta_i, delta_j, plan, V_lag_fft and a separate "
                      "sigma_planes buffer");
   if (hartree && (!w2d || !delta_start || !delta_i || !delta_j || !HH_orb || !delta_scratch))
     return set_error(HFX_EINVAL, "hfx_superlattice_fock: the Hartree shift needs its tables and scratch");
   const SlGeom g{nkx, nky, n_G, dim_orb, n_delta, Nx, Ny};
   const hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_C128)
-    return sl_build<double>(g, P, refP, tile_pair, pair_to_delta, g_off, plan, V_lag_fft, planes, sigma_planes,
+    return sl_build<double>(g, P, refP, pair_to_delta, g_off, plan, V_lag_fft, planes, sigma_planes,
                             hartree, w2d, delta_start, delta_i, delta_j, HH_orb, degeneracy, delta_scratch, h, Sigma,
                             F, s);
-  return sl_build<float>(g, P, refP, tile_pair, pair_to_delta, g_off, plan, V_lag_fft, planes, sigma_planes,
+  return sl_build<float>(g, P, refP, pair_to_delta, g_off, plan, V_lag_fft, planes, sigma_planes,
                          hartree, w2d, delta_start, delta_i, delta_j, HH_orb, degeneracy, delta_scratch, h, Sigma, F,
                          s);
 }

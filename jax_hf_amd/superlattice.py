@@ -119,12 +119,11 @@ class _SLFock:
         tx, ty = Nx // self.nkx, Ny // self.nky
         if goff.min() < 0 or goff[:, 0].max() >= tx or goff[:, 1].max() >= ty:
             raise ValueError("layout tile offsets fall outside the extended grid")
-        # (dG, tile) -> i * n_G + j for the pair of that dG whose block sits at the tile of -G_i
-        tile_pair = np.full((int(layout.n_delta), tx, ty), -1, dtype=np.int32)
-        ii, jj = np.meshgrid(np.arange(self.n_G), np.arange(self.n_G), indexing="ij")
-        ii, jj = ii.reshape(-1), jj.reshape(-1)
-        tile_pair[p2d.reshape(-1), goff[ii, 0], goff[ii, 1]] = ii * self.n_G + jj
-        self.host = dict(tile_pair=tile_pair, p2d=p2d.astype(np.int32), goff=goff.astype(np.int32),
+        di, dj = np.asarray(layout.delta_pair_i), np.asarray(layout.delta_pair_j)
+        if di.shape != (self.n_G * self.n_G,) or dj.shape != di.shape or p2d.min() < 0 or \
+                p2d.max() >= int(layout.n_delta):
+            raise ValueError("layout pair tables do not cover the n_G x n_G pairs")
+        self.host = dict(p2d=p2d.astype(np.int32), goff=goff.astype(np.int32),
                          start=np.asarray(layout.delta_pair_start, np.int32),
                          di=np.asarray(layout.delta_pair_i, np.int32), dj=np.asarray(layout.delta_pair_j, np.int32))
         self.HH_orb = None if HH_orb is None else np.ascontiguousarray(HH_orb, dtype=np.float64)
@@ -141,7 +140,7 @@ class _SLFock:
             st["V"] = to_tensor(lay.V_lag_fft, device, dtype).reshape(Nx, Ny).contiguous()
             st["plan"] = L.plan_for(Nx, Ny, self.dorb, dtype, device)
             npl = (int(lay.n_delta), self.dorb * self.dorb, Nx, Ny)
-            st["planes"] = torch.empty(npl, dtype=dtype, device=device)
+            st["planes"] = torch.zeros(npl, dtype=dtype, device=device)     # zero outside the pair blocks
             st["splanes"] = torch.empty(npl, dtype=dtype, device=device)
             st["R"] = torch.empty(int(lay.n_delta) * self.dorb, dtype=torch.complex128, device=device)
             if self.HH_orb is not None:
@@ -161,7 +160,7 @@ class _SLFock:
         w = None if not hartree else to_tensor(w2d, P.device, torch.float64).reshape(-1).contiguous()
         lay = self.layout
         L.call("hfx_superlattice_fock", dt, self.nkx, self.nky, self.n_G, self.dorb, int(lay.n_delta),
-               int(lay.N_ext_x), int(lay.N_ext_y), L.ptr(P), L.ptr(refP), L.ptr(st["tile_pair"]), L.ptr(st["p2d"]), L.ptr(st["goff"]), st["plan"].handle, L.ptr(st["V"]),
+               int(lay.N_ext_x), int(lay.N_ext_y), L.ptr(P), L.ptr(refP), L.ptr(st["p2d"]), L.ptr(st["goff"]), st["plan"].handle, L.ptr(st["V"]),
                L.ptr(st["planes"]) if exchange else None, L.ptr(st["splanes"]), int(bool(hartree)), L.ptr(w),
                L.ptr(st["start"]), L.ptr(st["di"]), L.ptr(st["dj"]), L.ptr(st.get("HH")), self.deg,
                L.ptr(st["R"]), L.ptr(h), L.ptr(Sigma), L.ptr(F), L.stream(P.device))
