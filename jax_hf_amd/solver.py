@@ -102,10 +102,13 @@ class _Fock:
         self._args = kernel.as_args() if (self.block_specs is not None or fock_build_fn is not None) else None
 
     def graph_safe(self, project_fn=None):
-        """True when an iteration is device launches only (no user callable, no host
-        check); a projector of ``jax_hf_amd.symmetry.make_project_fn`` qualifies."""
+        """True when an iteration is device launches only (no host check, no
+        callable that synchronises); callables marked ``hfx_graph_safe`` -- the
+        projectors of ``jax_hf_amd.symmetry.make_project_fn`` and the build of
+        ``jax_hf_amd.superlattice.make_superlattice_build_fock_fn`` -- qualify."""
         pf_ok = project_fn is None or bool(getattr(project_fn, "hfx_graph_safe", False))
-        return self.fn is None and self.block_specs is None and pf_ok
+        fn_ok = self.fn is None or bool(getattr(self.fn, "hfx_graph_safe", False))
+        return fn_ok and self.block_specs is None and pf_ok
 
     def build(self, P, F, E_ptr=None, project_fn=None, refP="kernel", sub_h=False):
         eng = self.eng

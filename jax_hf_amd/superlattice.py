@@ -189,7 +189,9 @@ def make_superlattice_build_fock_fn(layout: ExtendedGridLayout, n_G: int, dim_or
     """superlattice.py:233-377: a callable with ``build_fock``'s keyword
     signature for the ``fock_build_fn`` seam of ``solve`` / ``solve_scf``.
     Sigma = streaming exchange of P - refP + the k-independent superlattice
-    Hartree shift; H is zero; F = herm(h + Sigma), then herm(project_fn(F))."""
+    Hartree shift; H is zero; F = herm(h + Sigma), then herm(project_fn(F)).
+    The CG loop of ``solve`` captures it in its iteration graph
+    (``hfx_graph_safe``; the first iteration runs eagerly)."""
     n_G, dim_orb = int(n_G), int(dim_orb)
     if HH_GG_orbital is not None:
         HH_orb = np.asarray(host_array(HH_GG_orbital), dtype=np.float64)
@@ -222,4 +224,7 @@ def make_superlattice_build_fock_fn(layout: ExtendedGridLayout, n_G: int, dim_or
             K.hermitize(X, F)
         return Sigma, torch.zeros_like(P), F
 
+    # after its first call (device tables, plan, planes) a build only allocates
+    # its outputs and enqueues launches, so a solver may capture it in a graph
+    build_fock_sl.hfx_graph_safe = True
     return build_fock_sl

@@ -5,8 +5,9 @@ channels): layout tables identical, V_lag_fft within 1e-12 of numpy.fft,
 the exchange within 1e-12 (complex128) / 2e-6 (complex64) relative, the
 build_fock adapter (exchange + scalar / orbital Hartree, refP, h) within
 1e-12, zero density -> zero, the reference's orbital == scalar Hartree
-equivalence and q = 0 warning, and SCF / CG solves through the
-fock_build_fn seam matching the oracle solvers with the oracle adapter."""
+equivalence and q = 0 warning, SCF / CG solves through the fock_build_fn
+seam matching the oracle solvers with the oracle adapter, and the CG loop's
+captured graph with the build replaying the eager loop bit for bit."""
 
 import numpy as np
 import pytest
@@ -168,3 +169,24 @@ def test_solve_through_fock_build_fn_matches_oracle(dev, solver):
         r0 = O.solve(K0, P0, ne, config=O.SolverConfig(**cfg), fock_build_fn=fb0)
         assert r.converged and r0.converged
         assert abs(float(r.energy) - float(r0.energy)) < 1e-9
+
+
+def test_cg_graph_with_superlattice_build_replays_eager_loop(dev):
+    """The CG loop captures the superlattice build (hfx_graph_safe): every
+    iteration's energy and the final density equal the eager loop's bit for bit."""
+    import jax_hf_amd as J
+    from jax_hf_amd.superlattice import build_extended_layout, make_superlattice_build_fock_fn
+    nk, dorb = 6, 1
+    args, g, B, n_G, D = _sl_problem(nk, dorb)
+    lay = build_extended_layout(g, B, nk, nk, vfunc, 1.0 / nk ** 2)
+    fb = make_superlattice_build_fock_fn(lay, n_G, dorb, nk, nk, np.zeros((n_G, n_G)), 1.0)
+    G = J.HartreeFockKernel(*args)
+    P0 = np.zeros((nk, nk, D, D), complex)
+    res = {}
+    for use_graph in (False, True):
+        cfg = J.SolverConfig(precision="fp64", max_iter=40, tol_E=1e-13, use_graph=use_graph)
+        res[use_graph] = J.solve(G, P0, 2.0, config=cfg, fock_build_fn=fb)
+    a, b = res[False], res[True]
+    assert a.n_iter == b.n_iter and a.n_iter > 3
+    np.testing.assert_array_equal(_np(a.history["E"]), _np(b.history["E"]))
+    np.testing.assert_array_equal(_np(a.density), _np(b.density))
