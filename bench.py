@@ -339,6 +339,29 @@ def superlattice_bench(J, dev, args):
     out = {"workload": (f"hexagonal Gamma + 2 shells (n_G={n_G}), nk={nk}x{nk}, dim_orb={dorb}, extended grid "
                         f"{lay.N_ext_x}x{lay.N_ext_y}, {lay.n_delta} dG channels, exchange + Hartree, complex128"),
            "us_per_build": t / n * 1e6, "algorithmic_bytes": algo, "GBps": algo / (t / n) / 1e9}
+    # a CG solve through the plug-in with one orbital per G (D = 13: the
+    # solvers' per-k kernels take nb <= 16; kinetic |k - G|^2 bands plus a
+    # random Hermitian coupling, T = 0.05, 4 electrons per k): the iteration
+    # graph captures the build; the eager loop is timed beside it
+    k1 = (np.arange(nk) - nk // 2) / nk
+    KX, KY = np.meshgrid(k1, k1, indexing="ij")
+    h0 = np.zeros((nk, nk, n_G, n_G), complex)
+    for A in range(n_G):
+        h0[:, :, A, A] = 0.05 * np.sum(((np.stack([KX, KY], -1) - g[A]) @ B.T) ** 2, axis=-1)
+    U = (rng.standard_normal((n_G, n_G)) + 1j * rng.standard_normal((n_G, n_G))) * 0.2
+    h0 = h0 + 0.5 * (U + U.conj().T)[None, None]
+    G = J.HartreeFockKernel(w, h0, np.ones((nk, nk, 1, 1)), 0.05, device=dev)
+    fb0 = make_superlattice_build_fock_fn(lay, n_G, 1, nk, nk, HH, 2.0)
+    P0 = torch.zeros((nk, nk, n_G, n_G), dtype=torch.complex128, device=dev)
+    sol = {}
+    for use_graph in (True, False):
+        cfg = J.SolverConfig(precision="fp64", max_iter=200, tol_E=1e-10, use_graph=use_graph)
+        r, ts = _timed_solves_fb(J, G, P0, 4.0, cfg, fb0, 3)
+        sol["graph" if use_graph else "eager"] = {"ms_median": float(np.median(ts)) * 1e3, "n_iter": r.n_iter,
+                                                  "converged": bool(r.converged), "energy": float(r.energy)}
+    sol["speedup_graph_vs_eager"] = sol["eager"]["ms_median"] / sol["graph"]["ms_median"]
+    sol["abs_dE_graph_vs_eager"] = abs(sol["graph"]["energy"] - sol["eager"]["energy"])
+    out["cg_solve_fp64"] = sol
     if not args.no_cpu_baseline:
         from oracle import superlattice as OS
         ref = OS.build_extended_layout(g, B, nk, nk, vf, 1.0 / nk ** 2)
@@ -362,6 +385,18 @@ def _timed_solves(J, G, P0, ne, cfg, n):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         r = J.solve(G, P0, ne, config=cfg)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    return r, times
+
+
+def _timed_solves_fb(J, G, P0, ne, cfg, fb, n):
+    import torch
+    times, r = [], None
+    for _ in range(n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = J.solve(G, P0, ne, config=cfg, fock_build_fn=fb)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
     return r, times

@@ -358,13 +358,18 @@ def _cg_runner(kernel, cdt, cfg, fock_build_fn, n_electrons):
     Kept on the kernel between solves when the iteration is graph-captured:
     the buffers and the captured graph (which bakes in these controls and the
     electron count) are reused, so a repeated solve allocates and captures
-    nothing.  At most four runners are kept per kernel.
+    nothing.  At most four runners are kept per kernel.  A user Fock build or
+    projector marked ``hfx_graph_safe`` is part of the key (by identity; the
+    runner keeps a reference, so the identity cannot be reused while cached).
     """
-    if fock_build_fn is not None or cfg.project_fn is not None or cfg.block_sizes is not None \
-            or not cfg.use_graph:
+    def safe(f):
+        return f is None or bool(getattr(f, "hfx_graph_safe", False))
+
+    if not (safe(fock_build_fn) and safe(cfg.project_fn)) or cfg.block_sizes is not None or not cfg.use_graph:
         return _CG(kernel, cdt, cfg, fock_build_fn)
     key = (cdt, float(cfg.denom_scale), float(cfg.max_step), int(cfg.cg_restart), float(cfg.bt_shrink),
-           int(cfg.bt_max), int(cfg.mu_maxiter), str(cfg.cayley).lower(), float(n_electrons))
+           int(cfg.bt_max), int(cfg.mu_maxiter), str(cfg.cayley).lower(), float(n_electrons),
+           id(fock_build_fn), id(cfg.project_fn))
     cache = kernel.__dict__.setdefault("_hfx_cg", {})
     r = cache.get(key)
     if r is None:
@@ -372,6 +377,7 @@ def _cg_runner(kernel, cdt, cfg, fock_build_fn, n_electrons):
             cache.pop(next(iter(cache)))
         r = _CG(kernel, cdt, cfg, fock_build_fn)
         r.kernel = r.fock.kernel = weakref.proxy(kernel)    # the kernel owns the runner, not both ways
+        r.keep = (fock_build_fn, cfg.project_fn)
         cache[key] = r
     r.cfg = cfg
     return r

@@ -126,7 +126,7 @@ class _SLFock:
         self.host = dict(p2d=p2d.astype(np.int32), goff=goff.astype(np.int32),
                          start=np.asarray(layout.delta_pair_start, np.int32),
                          di=np.asarray(layout.delta_pair_i, np.int32), dj=np.asarray(layout.delta_pair_j, np.int32))
-        self.HH_orb = None if HH_orb is None else np.ascontiguousarray(HH_orb, dtype=np.float64)
+        self.HH_orb = None if HH_orb is None else np.array(HH_orb, dtype=np.float64, order="C", copy=True)
         self.deg = float(degeneracy)
         self._dev = {}
 
