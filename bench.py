@@ -613,7 +613,9 @@ def main():
         Fg = F_gpu.cpu().numpy()
         out["parity"] = {"fock_E_abs_diff": abs(fock["energy"] - E_cpu),
                          "fock_F_max_rel": float(np.abs(Fg - F_cpu).max() / np.abs(F_cpu).max())}
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
+    # free cached graphs, solver states and plans while the runtime is whole
+    L.release_all()
 
 
 if __name__ == "__main__":

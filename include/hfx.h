@@ -408,12 +408,14 @@ int hfx_reduce(const double* part, int nval, size_t nk, const double* w, unsigne
 #define HFX_MU_NEWTON_BRACKET 0
 #define HFX_MU_BISECTION 1
 #define HFX_MU_NEWTON 2
-/* Chemical potential in one cooperative launch: Newton with bracket
- * (solver.py:62-101), safeguarded Newton (utils.py:385-442) or bisection
- * (utils.py:353-382), the latter twice with
- * the level shift of reference_scf.py:87-96 when level_shift != 0.  Result to
- * st[HFX_ST_MU] and/or mu_out; occupations to p_out (sigmoid((mu-eps)/T), or
- * fermidirac of the shifted levels). */
+/* Chemical potential: Newton with bracket (solver.py:62-101) or safeguarded
+ * Newton (utils.py:385-442) in one cooperative launch; bisection
+ * (utils.py:353-382; maxiter levels, 54 in FP64 / 30 in FP32), twice with the
+ * level shift of reference_scf.py:87-96 when level_shift != 0, as a
+ * multisection of ordinary launches (six levels of the bisection tree per
+ * pass, the same midpoints and tests).  Result to st[HFX_ST_MU] and/or
+ * mu_out; occupations to p_out (sigmoid((mu-eps)/T), or fermidirac of the
+ * shifted levels).  scratch: hfx_solver_scratch_doubles() doubles. */
 int hfx_solve_mu(const double* eps, const double* w, size_t nk, int nb, double target, double T, double mu0,
                  int mu0_from_state, int maxiter, int mode, double level_shift, double* st, double* mu_out,
                  double* p_out, double* scratch, void* stream);
@@ -443,15 +445,17 @@ int hfx_selfenergy_batched(const hfx_plan* plan, int nsolve, const void* P, cons
                            void* Sigma, void* stream);
 /* F = herm(h + Sigma) and E_s = sum_k w_k Re Tr[(h + Sigma/2) P] into
  * st[s][HFX_ST_E] (exchange-only Fock build of fock.py:76-97, 109-111);
- * w2d (nsolve, nk1, nk2); scratch >= 64 * nsolve doubles. */
+ * w2d (nsolve, nk1, nk2); scratch of scratch_doubles >= 64 * nsolve doubles
+ * (HFX_EINVAL when smaller). */
 int hfx_fock_epilogue_batched(const hfx_plan* plan, int nsolve, const void* h, const void* Sigma, const void* P,
-                              const double* w2d, void* F, double* st, double* scratch, void* stream);
+                              const double* w2d, void* F, double* st, double* scratch, size_t scratch_doubles,
+                              void* stream);
 /* hfx_fock_epilogue_batched reading Sigma from the slot planes of
- * hfx_selfenergy_planar (default channel set for herm); scratch >= 256 * nsolve
- * doubles. */
+ * hfx_selfenergy_planar (default channel set for herm); scratch of
+ * scratch_doubles >= 256 * nsolve doubles (HFX_EINVAL when smaller). */
 int hfx_fock_epilogue_planar_batched(const hfx_plan* plan, int nsolve, const void* h, const void* Sigt, int herm,
                                      const void* P, const double* w2d, void* F, double* st, double* scratch,
-                                     void* stream);
+                                     size_t scratch_doubles, void* stream);
 /* Pulay C-DIIS coefficients on the device (_diis_coefficients, reference_scf.py:99-121):
  * writes the new row brow (nh doubles, device) into row and column idx of the
  * device B matrix (nh x nh), solves the masked, max-normalised augmented
@@ -464,16 +468,22 @@ int hfx_diis_coefficients(const double* brow, double* B, int nh, int idx, int co
 int hfx_diis_coefficients_batched(const double* brow, size_t brow_stride, double* B, int nsolve, int nh, int idx,
                                   int count, double* coef, void* stream);
 
-/* Scratch (float64) for hfx_reduce_batched / hfx_solve_mu_batched. */
+/* Scratch (float64) for hfx_reduce_batched / hfx_solve_mu_batched over nseg
+ * solves: block partials, then one completion-ticket word per solve at the
+ * end.  The two entry points return HFX_EINVAL when scratch_doubles is
+ * below this size; the scratch must not be shared with a buffer that other
+ * launches write in the meantime (the tickets must stay zero between calls). */
 size_t hfx_batch_scratch_doubles(int nseg);
 /* hfx_reduce per solve (segments of seg_len k-points); with op == HFX_OP_SUM
  * and out_slot >= 0 the sums go to st[s][out_slot ...]. */
 int hfx_reduce_batched(const double* part, int nval, size_t nk_total, int seg_len, const double* w, unsigned wmask,
-                       double* st, int op, const double* params, int out_slot, double* scratch, void* stream);
+                       double* st, int op, const double* params, int out_slot, double* scratch,
+                       size_t scratch_doubles, void* stream);
 /* hfx_solve_mu_steps per solve (targets[s] electrons, mu warm-started from
  * st[s][HFX_ST_MU]). */
 int hfx_solve_mu_batched(const double* eps, const double* w, const double* targets, size_t nk_total, int seg_len,
-                         int nb, double T, int maxiter, double* st, double* p_out, double* scratch, void* stream);
+                         int nb, double T, int maxiter, double* st, double* p_out, double* scratch,
+                         size_t scratch_doubles, void* stream);
 
 #ifdef __cplusplus
 }

@@ -6,9 +6,12 @@ library raises immediately (there is no CPU fallback in the product path).
 
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import gc
 import os
 import threading
+import weakref
 from pathlib import Path
 
 import torch
@@ -18,7 +21,7 @@ LIB_PATH = _HERE / "libhfx.so"
 
 HFX_C128 = 0
 HFX_C64 = 1
-ABI_VERSION = 11
+ABI_VERSION = 12
 
 _lib = None
 _lock = threading.Lock()
@@ -67,14 +70,14 @@ _SIGS = {
     "hfx_cg_trial_gj": [i32, i32, sz, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, i32, vp],
     "hfx_cg_retract_gj": [i32, i32, sz, vp, vp, vp, vp, vp, vp, vp, i32, vp],
     "hfx_selfenergy_batched": [vp, i32, vp, vp, i32, i32, vp, vp],
-    "hfx_fock_epilogue_batched": [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp],
+    "hfx_fock_epilogue_batched": [vp, i32, vp, vp, vp, vp, vp, vp, vp, sz, vp],
     "hfx_batch_scratch_doubles": [i32],
     "hfx_planar_pack": [vp, i32, vp, vp, vp, i32, i32, vp, vp],
     "hfx_selfenergy_planar": [vp, i32, vp, vp, i32, vp, i32, i32, i32, vp, vp, vp, vp],
     "hfx_fock_epilogue_planar": [vp, vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, f64, vp],
-    "hfx_fock_epilogue_planar_batched": [vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp],
-    "hfx_reduce_batched": [vp, i32, sz, i32, vp, C.c_uint, vp, i32, vp, i32, vp, vp],
-    "hfx_solve_mu_batched": [vp, vp, vp, sz, i32, i32, f64, i32, vp, vp, vp, vp],
+    "hfx_fock_epilogue_planar_batched": [vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, sz, vp],
+    "hfx_reduce_batched": [vp, i32, sz, i32, vp, C.c_uint, vp, i32, vp, i32, vp, sz, vp],
+    "hfx_solve_mu_batched": [vp, vp, vp, sz, i32, i32, f64, i32, vp, vp, vp, sz, vp],
     "hfx_cg_retract": [i32, i32, sz, vp, vp, vp, vp, vp, f64, i32, vp, vp, vp],
     "hfx_rotate_eigh": [i32, i32, sz, vp, vp, vp, vp],
     "hfx_matmul_batched": [i32, i32, sz, vp, vp, vp, i32, vp, i32, vp, vp],
@@ -129,6 +132,8 @@ def lib():
         if v != ABI_VERSION:
             raise ImportError(f"libhfx ABI version {v} != expected {ABI_VERSION}; rebuild")
         _lib = L
+        # registered after torch's own exit handlers, so it runs before them
+        atexit.register(release_all)
         return L
 
 
@@ -209,10 +214,16 @@ class Plan:
                  C.byref(best), stream(self.device))
         return best.value, {self.VARIANTS[v]: t[v] for v in range(1, 5)}
 
+    def destroy(self):
+        """Free the plan's device tables now (idempotent)."""
+        h, self.handle = self.handle, None
+        self.scratch = None
+        if h and _lib is not None:
+            _lib.hfx_plan_destroy(h)
+
     def __del__(self):
         try:
-            if self.handle and _lib is not None:
-                _lib.hfx_plan_destroy(self.handle)
+            self.destroy()
         except Exception:  # pragma: no cover
             pass
 
@@ -229,6 +240,56 @@ def plan_for(nk1: int, nk2: int, nb: int, cdtype: torch.dtype, device) -> Plan:
         p = Plan(nk1, nk2, nb, dt, dev)
         _plans[key] = p
     return p
+
+
+# Objects that keep device state between calls (kernels with cached solver
+# states and captured graphs, Fock-build plug-ins) and module caches of device
+# buffers.  release_all() drops all of it while the HIP runtime is still whole.
+_owners: "weakref.WeakSet" = weakref.WeakSet()
+_cache_clearers: list = []
+
+
+def register_owner(obj):
+    """obj._hfx_release() is called by release_all()."""
+    _owners.add(obj)
+
+
+def register_cache(clear_fn):
+    """clear_fn() empties a module-level cache of device buffers."""
+    _cache_clearers.append(clear_fn)
+
+
+def release_all():
+    """Release every cached graph, solver state, workspace, pinned buffer and
+    plan, after the device has finished its work.
+
+    Runs at interpreter exit (registered when the library is first loaded, so
+    before torch's and the HIP runtime's own teardown) and may be called by a
+    program at any point; later calls rebuild what they need.  Without it the
+    plans and graphs are freed by destructors during interpreter teardown, in
+    no fixed order relative to the runtime's and any loaded tool library's
+    finalisation.
+    """
+    if _lib is None or not torch.cuda.is_available() or not torch.cuda.is_initialized():
+        return
+    try:
+        torch.cuda.synchronize()
+    except Exception:  # pragma: no cover - a faulted device: nothing to release safely
+        return
+    for o in list(_owners):
+        try:
+            o._hfx_release()
+        except ReferenceError:  # pragma: no cover
+            pass
+    for clear in _cache_clearers:
+        clear()
+    gc.collect()
+    torch.cuda.synchronize()
+    for p in list(_plans.values()):
+        p.destroy()
+    _plans.clear()
+    gc.collect()
+    torch.cuda.synchronize()
 
 
 def require_gpu(device=None):

@@ -40,6 +40,9 @@ class Workspace:
         self.out = torch.zeros(16, dtype=torch.float64, device=device)
 
 
+L.register_cache(Workspace._cache.clear)
+
+
 def s(dev):
     return L.stream(dev)
 

@@ -121,6 +121,7 @@ class HFEnergy(torch.autograd.Function):
 # version (the packed channel-major VR and the build buffers are made once).
 _FWD = {"key": None, "eng": None}
 _ONES: dict = {}
+L.register_cache(lambda: (_FWD.update(key=None, eng=None), _ONES.clear()))
 
 
 def _tkey(t):

@@ -96,12 +96,12 @@ class _BatchFock:
             L.call("hfx_selfenergy_planar", h, self.S, L.ptr(self.Pt), L.ptr(self.VRc), self.vfull, None, 0, herm, 0,
                    None, L.ptr(self.St), None, sm)
             L.call("hfx_fock_epilogue_planar_batched", h, self.S, L.ptr(self.h), L.ptr(self.St), herm, L.ptr(P),
-                   L.ptr(self.w_raw), L.ptr(F), L.ptr(st), L.ptr(self.epi), sm)
+                   L.ptr(self.w_raw), L.ptr(F), L.ptr(st), L.ptr(self.epi), self.epi.numel(), sm)
             return
         L.call("hfx_selfenergy_batched", h, self.S, L.ptr(P), L.ptr(self.VR), self.vfull,
                int(self.herm), L.ptr(self.Sig), sm)
         L.call("hfx_fock_epilogue_batched", h, self.S, L.ptr(self.h), L.ptr(self.Sig), L.ptr(P),
-               L.ptr(self.w_raw), L.ptr(F), L.ptr(st), L.ptr(self.epi), sm)
+               L.ptr(self.w_raw), L.ptr(F), L.ptr(st), L.ptr(self.epi), self.epi.numel(), sm)
 
 
 class _BatchCG:
@@ -156,7 +156,7 @@ class _BatchCG:
         n = self.cfg.mu_maxiter if steps is None else min(steps, self.cfg.mu_maxiter)
         L.call("hfx_solve_mu_batched", L.ptr(self.eps), L.ptr(self.w_norm), L.ptr(self.targets), self.nk_tot,
                self.Nk, self.nb, self.T, int(n), L.ptr(self.st), L.ptr(self.p),
-               L.ptr(self.scratch), L.stream(self.dev))
+               L.ptr(self.scratch), self.scratch.numel(), L.stream(self.dev))
 
     def reduce(self, nval, wmask, op, params, out_slot=-1):
         vals = list(params) + [0.0] * (10 - len(params))
@@ -164,7 +164,7 @@ class _BatchCG:
         arr = (C.c_double * 10)(*[float(x) for x in vals])
         L.call("hfx_reduce_batched", L.ptr(self.part), int(nval), self.nk_tot, self.Nk, L.ptr(self.w_norm),
                int(wmask), L.ptr(self.st), int(op), C.cast(arr, C.c_void_p), int(out_slot), L.ptr(self.scratch),
-               L.stream(self.dev))
+               self.scratch.numel(), L.stream(self.dev))
 
     def init_from(self, P0s):
         """solver.py:380-394 per solve: eigh(F[herm(P0)]) -> (Q, eps), mu, p."""
@@ -356,12 +356,13 @@ class _BatchPolish:
 
     def _reduce(self, nval, slot):
         L.call("hfx_reduce_batched", L.ptr(self.part), int(nval), self.nk_tot, self.Nk, L.ptr(self.w_norm),
-               (1 << nval) - 1, L.ptr(self.st), K.OP_SUM, None, int(slot), L.ptr(self.scratch), L.stream(self.dev))
+               (1 << nval) - 1, L.ptr(self.st), K.OP_SUM, None, int(slot), L.ptr(self.scratch),
+               self.scratch.numel(), L.stream(self.dev))
 
     def _mu(self, maxiter=54):
         L.call("hfx_solve_mu_batched", L.ptr(self.eps), L.ptr(self.w_norm), L.ptr(self.targets), self.nk_tot,
                self.Nk, self.nb, self.T, int(maxiter), L.ptr(self.st), L.ptr(self.occ), L.ptr(self.scratch),
-               L.stream(self.dev))
+               self.scratch.numel(), L.stream(self.dev))
 
     def _eval(self, k):
         """F, E, residuals and B row of every solve's iterate k; asynchronous readback."""

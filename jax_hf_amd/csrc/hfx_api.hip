@@ -70,7 +70,7 @@ using namespace hfx;
 
 extern "C" {
 
-int hfx_abi_version(void) { return 11; }
+int hfx_abi_version(void) { return 12; }
 
 int hfx_last_error(char* buf, size_t n) {
   if (!buf || n == 0) return HFX_EINVAL;
@@ -326,9 +326,12 @@ int hfx_kterms(const hfx_plan* plan, const void* rho, const double* HH, int ntc,
 }
 
 int hfx_fock_epilogue_batched(const hfx_plan* plan, int nsolve, const void* h, const void* Sigma, const void* P,
-                              const double* w2d, void* F, double* st, double* scratch, void* stream) {
+                              const double* w2d, void* F, double* st, double* scratch, size_t scratch_doubles,
+                              void* stream) {
   HFX_REQUIRE(plan && h && Sigma && P && w2d && F && st && scratch, "hfx_fock_epilogue_batched: NULL argument");
   HFX_REQUIRE(nsolve >= 1, "hfx_fock_epilogue_batched: nsolve=%d", nsolve);
+  HFX_REQUIRE(scratch_doubles >= (size_t)64 * nsolve,
+              "hfx_fock_epilogue_batched: scratch of %zu doubles, need %zu", scratch_doubles, (size_t)64 * nsolve);
   return epilogue_seg_launch(plan, nsolve, h, Sigma, P, w2d, F, st, scratch, (hipStream_t)stream);
 }
 
@@ -358,10 +361,13 @@ int hfx_fock_epilogue_planar(const hfx_plan* plan, const void* h, const void* Si
 
 int hfx_fock_epilogue_planar_batched(const hfx_plan* plan, int nsolve, const void* h, const void* Sigt, int herm,
                                      const void* P, const double* w2d, void* F, double* st, double* scratch,
-                                     void* stream) {
+                                     size_t scratch_doubles, void* stream) {
   HFX_REQUIRE(plan && h && Sigt && P && w2d && F && st && scratch,
               "hfx_fock_epilogue_planar_batched: NULL argument");
   HFX_REQUIRE(nsolve >= 1, "hfx_fock_epilogue_planar_batched: nsolve=%d", nsolve);
+  HFX_REQUIRE(scratch_doubles >= (size_t)256 * nsolve,
+              "hfx_fock_epilogue_planar_batched: scratch of %zu doubles, need %zu", scratch_doubles,
+              (size_t)256 * nsolve);
   const int* chans = nullptr;
   int nchan = 0, cover = 0;
   resolve_chans(plan, chans, nchan, herm, &cover);

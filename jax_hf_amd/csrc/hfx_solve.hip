@@ -8,13 +8,15 @@
 // acceptance) directly into the device scalar state, so a CG iteration needs
 // no host round trip before its convergence test.
 //
-// Chemical potential: one cooperative launch runs all iterations of
-//   * the Newton-with-bracket solver of solver.py:62-101, or
-//   * the pure bisection of utils.py:353-382 (optionally twice, with the
-//     level shift of reference_scf.py:87-96),
-// each iteration being a grid-wide (count, slope) reduction followed by a
-// grid barrier; every block then evaluates the same fixed-order sum, so all
-// blocks step mu identically.
+// Chemical potential:
+//   * one cooperative launch runs all iterations of the Newton-with-bracket
+//     solver of solver.py:62-101 or the Newton of utils.py:385-442, each
+//     iteration a grid-wide (count, slope) reduction followed by a grid
+//     barrier; every block then evaluates the same fixed-order sum, so all
+//     blocks step mu identically;
+//   * the pure bisection of utils.py:353-382 (optionally twice, with the level
+//     shift of reference_scf.py:87-96) runs as a multisection: ordinary
+//     launches, six levels of the bisection tree per pass.
 #include <hip/hip_cooperative_groups.h>
 
 #include "hfx_common.h"
@@ -275,75 +277,11 @@ __device__ void grid_minmax(cg::grid_group& grid, const double* v, size_t n, dou
   __syncthreads();
 }
 
-// value of element e after the optional level shift (reference_scf.py:91-92)
-__device__ __forceinline__ double shifted(const MuArgs& a, size_t e, double mu_raw, bool use_shift) {
-  const double x = a.eps[e];
-  if (!use_shift) return x;
-  const double f = sigm(-(x - mu_raw) / (a.T + 1e-12));
-  return x + a.level_shift * (1.0 - f);
-}
-
-__device__ double bisect(cg::grid_group& grid, const MuArgs& a, double mu_raw, bool use_shift, double* sh) {
-  const size_t n = a.nk * a.nb, stride = (size_t)gridDim.x * blockDim.x;
-  double mn = INFINITY, mx = -INFINITY;
-  {
-    double lo = INFINITY, hi = -INFINITY;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
-      const double v = shifted(a, e, mu_raw, use_shift);
-      lo = fmin(lo, v);
-      hi = fmax(hi, v);
-    }
-    // min / max via the (min, -max) pair of a grid "sum" with min/max combine
-    lo = wave_min(lo);
-    hi = wave_max(hi);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    if (lane == 0) { sh[wid] = lo; sh[nw + wid] = hi; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int w = 0; w < nw; ++w) { lo = fmin(lo, sh[w]); hi = fmax(hi, sh[nw + w]); }
-      a.scratch[2 * blockIdx.x] = lo;
-      a.scratch[2 * blockIdx.x + 1] = hi;
-    }
-    grid.sync();
-    double p = INFINITY, q = -INFINITY;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
-      p = fmin(p, a.scratch[2 * b]);
-      q = fmax(q, a.scratch[2 * b + 1]);
-    }
-    p = wave_min(p);
-    q = wave_max(q);
-    __syncthreads();
-    if (lane == 0) { sh[wid] = p; sh[nw + wid] = q; }
-    __syncthreads();
-    for (int w = 0; w < nw; ++w) { mn = fmin(mn, sh[w]); mx = fmax(mx, sh[nw + w]); }
-    __syncthreads();
-    grid.sync();   // slot 0 of the scratch is reused right after
-  }
-  const double span = mx - mn + 10.0 * fmax(a.T, 1e-6);
-  double lo = mn - span, hi = mx + span;
-  for (int it = 0; it < a.maxiter; ++it) {
-    const double mid = 0.5 * (lo + hi);
-    if (mid == lo || mid == hi) break;   // bracket collapsed: later iterations change nothing
-    double c = 0.0;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
-      const double v = shifted(a, e, mu_raw, use_shift);
-      c += a.w[e / a.nb] * sigm(-(v - mid) / (a.T + 1e-12));
-    }
-    double cnt, dummy;
-    grid_sum2(grid, c, 0.0, a.scratch, 1 + (it & 1), sh, cnt, dummy);
-    if (cnt > a.target) hi = mid;
-    else lo = mid;
-  }
-  return 0.5 * (lo + hi);
-}
-
 __global__ __launch_bounds__(MU_BLOCK) void mu_kernel(MuArgs a) {
   cg::grid_group grid = cg::this_grid();
   __shared__ double sh[2 * (MU_BLOCK / 64)];
   const size_t n = a.nk * a.nb, stride = (size_t)gridDim.x * blockDim.x;
   double mu;
-  bool use_shift = false;
-  double mu_raw = 0.0;
   if (a.mode == HFX_MU_NEWTON_BRACKET) {   // solver.py:62-101
     const double Tj = fmax(a.T, 1e-12);
     double mn, mx;
@@ -373,7 +311,7 @@ __global__ __launch_bounds__(MU_BLOCK) void mu_kernel(MuArgs a) {
       if (!isfinite(mu_new)) mu_new = mu_bis;
       mu = mu_new;
     }
-  } else if (a.mode == HFX_MU_NEWTON) {   // utils.py:385-442
+  } else {   // HFX_MU_NEWTON, utils.py:385-442 (bisection: ms_bisection_launch)
     const double Tj = fmax(a.T, 1e-12);
     double mn, mx;
     grid_minmax(grid, a.eps, n, a.scratch, sh, mn, mx);
@@ -406,13 +344,6 @@ __global__ __launch_bounds__(MU_BLOCK) void mu_kernel(MuArgs a) {
       mu = mu_new;
     }
     if (fabs(N - a.target) > 1e-12) mu = 0.5 * (lo + hi);
-  } else {   // HFX_MU_BISECTION (utils.py:353-382) with optional level shift
-    mu = bisect(grid, a, 0.0, false, sh);
-    if (a.level_shift != 0.0) {
-      mu_raw = mu;
-      use_shift = true;
-      mu = bisect(grid, a, mu_raw, true, sh);
-    }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     if (a.st) a.st[HFX_ST_MU] = mu;
@@ -421,8 +352,7 @@ __global__ __launch_bounds__(MU_BLOCK) void mu_kernel(MuArgs a) {
   if (a.p_out) {
     const double Tj = fmax(a.T, 1e-12);
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
-      if (a.mode != HFX_MU_BISECTION) a.p_out[e] = sigm((mu - a.eps[e]) / Tj);   // solver.py:394, 592
-      else a.p_out[e] = sigm(-(shifted(a, e, mu_raw, use_shift) - mu) / (a.T + 1e-12));  // fermidirac
+      a.p_out[e] = sigm((mu - a.eps[e]) / Tj);   // solver.py:394, 592
     }
   }
 }
@@ -831,6 +761,246 @@ __global__ __launch_bounds__(64) void diis_coef_kernel(const double* __restrict_
   }
 }
 
+// ---- bisection by multisection (utils.py:353-382, reference_scf.py:87-96) ---
+// The bisection walks a binary tree: each level takes mid = 0.5 * (lo + hi)
+// and keeps [lo, mid] when the count at mid exceeds the target, else
+// [mid, hi].  One pass evaluates the counts at all 2^L - 1 midpoints of the
+// next L <= 6 levels of that tree in one launch (every block builds the same
+// tree from the same bracket with the same expression), and one workgroup then
+// walks the L levels with those counts: the same midpoints and the same tests
+// as the level-by-level loop, so the result is the bisection's mu, reached in
+// ceil(maxiter / 6) passes of two ordinary launches each (no grid barrier, so
+// a graph may capture it).  The bracket, the collapse flag and mu live in the
+// scratch; the per-block node counts follow them.
+constexpr int MS_BLOCK = 512;
+constexpr int MS_MAX_BLOCKS = 256;
+constexpr int MS_LEVELS = 6;
+constexpr int MS_NODES = 64;   // 2^6 - 1 tree nodes, padded to a wavefront
+constexpr size_t MS_OFF = MU_TICKET_OFF + 8;
+constexpr size_t MS_PART_OFF = MS_OFF + 16;
+constexpr size_t MS_END = MS_PART_OFF + (size_t)MS_MAX_BLOCKS * MS_NODES;
+enum { MS_LO = 0, MS_HI = 1, MS_DONE = 2, MS_MURAW = 3, MS_MU = 4 };
+
+struct MsArgs {
+  const double* eps;   // (n) eigenvalues, nb per k-point
+  const double* w;     // (n / nb) weights
+  size_t n;
+  int nb;
+  double T, target, level_shift;
+  int shift;           // count the level-shifted values (reads state[MS_MURAW])
+  double* state;       // scratch + MS_OFF
+  double* part;        // scratch + MS_PART_OFF
+};
+
+// eigenvalue e, level-shifted when a.shift (reference_scf.py:91-92)
+__device__ __forceinline__ double ms_value(const MsArgs& a, size_t e, double mu_raw) {
+  const double x = a.eps[e];
+  if (!a.shift) return x;
+  const double f = sigm(-(x - mu_raw) / (a.T + 1e-12));
+  return x + a.level_shift * (1.0 - f);
+}
+
+__global__ __launch_bounds__(MS_BLOCK) void ms_minmax_part(MsArgs a) {
+  __shared__ double sh[2 * (MS_BLOCK / 64)];
+  const double mu_raw = a.shift ? a.state[MS_MURAW] : 0.0;
+  double lo = INFINITY, hi = -INFINITY;
+  for (size_t e = (size_t)blockIdx.x * MS_BLOCK + threadIdx.x; e < a.n; e += (size_t)gridDim.x * MS_BLOCK) {
+    const double v = ms_value(a, e, mu_raw);
+    lo = fmin(lo, v);
+    hi = fmax(hi, v);
+  }
+  lo = wave_min(lo);
+  hi = wave_max(hi);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { sh[wid] = lo; sh[MS_BLOCK / 64 + wid] = hi; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < MS_BLOCK / 64; ++q) { lo = fmin(lo, sh[q]); hi = fmax(hi, sh[MS_BLOCK / 64 + q]); }
+    a.part[2 * blockIdx.x] = lo;
+    a.part[2 * blockIdx.x + 1] = hi;
+  }
+}
+
+// initial bracket: span = max - min + 10 max(T, 1e-6) on both sides (utils.py:366-370)
+__global__ __launch_bounds__(256) void ms_minmax_fin(MsArgs a, int nblk) {
+  __shared__ double sh[2 * 4];
+  double lo = INFINITY, hi = -INFINITY;
+  for (int b = threadIdx.x; b < nblk; b += 256) {
+    lo = fmin(lo, a.part[2 * b]);
+    hi = fmax(hi, a.part[2 * b + 1]);
+  }
+  lo = wave_min(lo);
+  hi = wave_max(hi);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { sh[wid] = lo; sh[4 + wid] = hi; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < 4; ++q) { lo = fmin(lo, sh[q]); hi = fmax(hi, sh[4 + q]); }
+    const double span = hi - lo + 10.0 * fmax(a.T, 1e-6);
+    a.state[MS_LO] = lo - span;
+    a.state[MS_HI] = hi + span;
+    a.state[MS_DONE] = 0.0;
+  }
+}
+
+// Wavefront sum of 32 values per lane, scattered: lane l returns the sum over
+// the 64 lanes of v[l & 31] (a full butterfly step across the two half-waves,
+// then a reduce-scatter: each step keeps the half of the remaining values the
+// lane's bit selects and adds the partner's copy of it; 63 shuffles).
+constexpr int MS_CHUNK = 32;
+__device__ __forceinline__ double wave_scatter_sum32(double (&v)[MS_CHUNK]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int j = 0; j < MS_CHUNK; ++j) v[j] += __shfl_xor(v[j], 32, 64);
+#pragma unroll
+  for (int h = MS_CHUNK / 2; h >= 1; h >>= 1) {
+    const bool up = (lane & h) != 0;
+#pragma unroll
+    for (int j = 0; j < h; ++j) {
+      const double send = up ? v[j] : v[j + h];
+      const double keep = up ? v[j + h] : v[j];
+      v[j] = keep + __shfl_xor(send, h, 64);
+    }
+  }
+  return v[0];
+}
+
+// counts at the nodes of the next levels of the bisection tree (breadth-first:
+// node i has children 2i+1 = [lo_i, mid_i] and 2i+2 = [mid_i, hi_i]), in
+// chunks of 32 nodes (32 accumulators per thread)
+__global__ __launch_bounds__(MS_BLOCK) void ms_count_part(MsArgs a, int nodes) {
+  __shared__ double plo[MS_NODES], phi[MS_NODES], pmid[MS_NODES];
+  __shared__ double wsum[MS_BLOCK / 64][MS_NODES];
+  if (a.state[MS_DONE] != 0.0) return;   // the bracket collapsed in an earlier pass
+  if (threadIdx.x == 0) {
+    plo[0] = a.state[MS_LO];
+    phi[0] = a.state[MS_HI];
+    for (int i = 0; i < nodes; ++i) {
+      const double m = 0.5 * (plo[i] + phi[i]);
+      pmid[i] = m;
+      const int c = 2 * i + 1;
+      if (c + 1 < nodes) {
+        plo[c] = plo[i];
+        phi[c] = m;
+        plo[c + 1] = m;
+        phi[c + 1] = phi[i];
+      }
+    }
+  }
+  __syncthreads();
+  const double mu_raw = a.shift ? a.state[MS_MURAW] : 0.0;
+  const double Td = a.T + 1e-12;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int c0 = 0; c0 < nodes; c0 += MS_CHUNK) {
+    const int nc = nodes - c0 < MS_CHUNK ? nodes - c0 : MS_CHUNK;
+    double acc[MS_CHUNK];
+#pragma unroll
+    for (int j = 0; j < MS_CHUNK; ++j) acc[j] = 0.0;
+    for (size_t e = (size_t)blockIdx.x * MS_BLOCK + threadIdx.x; e < a.n; e += (size_t)gridDim.x * MS_BLOCK) {
+      const double v = ms_value(a, e, mu_raw);
+      const double wk = a.w[e / a.nb];
+#pragma unroll
+      for (int j = 0; j < MS_CHUNK; ++j)
+        if (j < nc) acc[j] += wk * sigm(-(v - pmid[c0 + j]) / Td);   // fermidirac(eps - mid, T)
+    }
+    const double t = wave_scatter_sum32(acc);
+    if (lane < MS_CHUNK) wsum[wid][c0 + lane] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < MS_NODES) {
+    double s = 0.0;
+    for (int q = 0; q < MS_BLOCK / 64; ++q) s += wsum[q][threadIdx.x];
+    a.part[(size_t)blockIdx.x * MS_NODES + threadIdx.x] = s;
+  }
+}
+
+// walk `levels` levels with the pass's counts; store: 1 -> mu, 2 -> mu and mu_raw
+__global__ __launch_bounds__(256) void ms_count_fin(MsArgs a, int nblk, int levels, int store) {
+  __shared__ double tot[4][MS_NODES];
+  const bool done = a.state[MS_DONE] != 0.0;
+  if (!done && levels > 0) {
+    const int t = threadIdx.x & 63, q = threadIdx.x >> 6;
+    double s = 0.0;
+    for (int b = q; b < nblk; b += 4) s += a.part[(size_t)b * MS_NODES + t];
+    tot[q][t] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double lo = a.state[MS_LO], hi = a.state[MS_HI];
+  if (!done) {
+    int node = 0;
+    for (int lev = 0; lev < levels; ++lev) {
+      const double mid = 0.5 * (lo + hi);
+      if (mid == lo || mid == hi) {   // collapsed: later levels change nothing
+        a.state[MS_DONE] = 1.0;
+        break;
+      }
+      const double cnt = ((tot[0][node] + tot[1][node]) + tot[2][node]) + tot[3][node];
+      if (cnt > a.target) {
+        hi = mid;
+        node = 2 * node + 1;
+      } else {
+        lo = mid;
+        node = 2 * node + 2;
+      }
+    }
+    a.state[MS_LO] = lo;
+    a.state[MS_HI] = hi;
+  }
+  if (store) {
+    const double mu = 0.5 * (lo + hi);
+    a.state[MS_MU] = mu;
+    if (store == 2) a.state[MS_MURAW] = mu;
+  }
+}
+
+__global__ __launch_bounds__(MS_BLOCK) void ms_out(MsArgs a, double* st, double* mu_out, double* p_out) {
+  const double mu = a.state[MS_MU];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (st) st[HFX_ST_MU] = mu;
+    if (mu_out) *mu_out = mu;
+  }
+  if (!p_out) return;
+  const double mu_raw = a.shift ? a.state[MS_MURAW] : 0.0;
+  for (size_t e = (size_t)blockIdx.x * MS_BLOCK + threadIdx.x; e < a.n; e += (size_t)gridDim.x * MS_BLOCK)
+    p_out[e] = sigm(-(ms_value(a, e, mu_raw) - mu) / (a.T + 1e-12));   // fermidirac
+}
+
+// Bisection mu (and occupations) for nk x nb eigenvalues; scratch >= MS_END doubles.
+int ms_bisection_launch(const double* eps, const double* w, size_t nk, int nb, double target, double T,
+                        int maxiter, double level_shift, double* st, double* mu_out, double* p_out, double* scratch,
+                        hipStream_t s) {
+  const size_t n = nk * (size_t)nb;
+  size_t g = (n + MS_BLOCK - 1) / MS_BLOCK;
+  if (g > (size_t)MS_MAX_BLOCKS) g = MS_MAX_BLOCKS;
+  if (g < 1) g = 1;
+  const int nblk = (int)g;
+  MsArgs a{eps, w, n, nb, T, target, level_shift, 0, scratch + MS_OFF, scratch + MS_PART_OFF};
+  const int phases = level_shift != 0.0 ? 2 : 1;
+  for (int ph = 0; ph < phases; ++ph) {
+    a.shift = ph;
+    hipLaunchKernelGGL(ms_minmax_part, dim3(g), dim3(MS_BLOCK), 0, s, a);
+    if (int rc = check_launch("ms_minmax_part")) return rc;
+    hipLaunchKernelGGL(ms_minmax_fin, dim3(1), dim3(256), 0, s, a, nblk);
+    if (int rc = check_launch("ms_minmax_fin")) return rc;
+    const int store = ph == 0 && phases == 2 ? 2 : 1;
+    int left = maxiter > 0 ? maxiter : 0;
+    do {
+      const int L = left < MS_LEVELS ? left : MS_LEVELS;
+      left -= L;
+      if (L > 0) {
+        hipLaunchKernelGGL(ms_count_part, dim3(g), dim3(MS_BLOCK), 0, s, a, (1 << L) - 1);
+        if (int rc = check_launch("ms_count_part")) return rc;
+      }
+      hipLaunchKernelGGL(ms_count_fin, dim3(1), dim3(256), 0, s, a, nblk, L, left == 0 ? store : 0);
+      if (int rc = check_launch("ms_count_fin")) return rc;
+    } while (left > 0);
+  }
+  if (!st && !mu_out && !p_out) return HFX_OK;
+  hipLaunchKernelGGL(ms_out, dim3(g), dim3(MS_BLOCK), 0, s, a, st, mu_out, p_out);
+  return check_launch("ms_out");
+}
+
 static int g_mu_blocks = 0;
 
 // Co-resident grid for the cooperative launch, sized to ~8 eigenvalues per
@@ -858,7 +1028,7 @@ using namespace hfx;
 extern "C" {
 
 size_t hfx_solver_scratch_doubles(void) {
-  return (size_t)MU_TICKET_OFF + 8;
+  return MS_END;
 }
 
 int hfx_reduce(const double* part, int nval, size_t nk, const double* w, unsigned wmask, double* out, double* st,
@@ -939,9 +1109,15 @@ size_t hfx_batch_scratch_doubles(int nseg) {
 }
 
 int hfx_reduce_batched(const double* part, int nval, size_t nk_total, int seg_len, const double* w, unsigned wmask,
-                       double* st, int op, const double* params, int out_slot, double* scratch, void* stream) {
+                       double* st, int op, const double* params, int out_slot, double* scratch,
+                       size_t scratch_doubles, void* stream) {
   if (!part || !st || !scratch || nval < 1 || nval > RED_MAX_VALS || seg_len < 1 || nk_total % (size_t)seg_len)
     return set_error(HFX_EINVAL, "hfx_reduce_batched: bad argument");
+  if (nk_total / seg_len > 0x7FFFFFFFull)
+    return set_error(HFX_EINVAL, "hfx_reduce_batched: too many solves");
+  if (scratch_doubles < hfx_batch_scratch_doubles((int)(nk_total / seg_len)))
+    return set_error(HFX_EINVAL, "hfx_reduce_batched: scratch of %zu doubles, need %zu", scratch_doubles,
+                     hfx_batch_scratch_doubles((int)(nk_total / seg_len)));
   if (out_slot >= 0 && (op != HFX_OP_SUM || out_slot + nval > HFX_ST_LEN))
     return set_error(HFX_EINVAL, "hfx_reduce_batched: out_slot only with HFX_OP_SUM, inside the state vector");
   const int nseg = (int)(nk_total / seg_len);
@@ -967,10 +1143,14 @@ int hfx_reduce_batched(const double* part, int nval, size_t nk_total, int seg_le
 }
 
 int hfx_solve_mu_batched(const double* eps, const double* w, const double* targets, size_t nk_total, int seg_len,
-                         int nb, double T, int maxiter, double* st, double* p_out, double* scratch, void* stream) {
+                         int nb, double T, int maxiter, double* st, double* p_out, double* scratch,
+                         size_t scratch_doubles, void* stream) {
   if (!eps || !w || !targets || !st || !scratch || nb < 1 || seg_len < 1 || nk_total % (size_t)seg_len ||
-      maxiter < 0)
+      maxiter < 0 || nk_total / seg_len > 0x7FFFFFFFull)
     return set_error(HFX_EINVAL, "hfx_solve_mu_batched: bad argument");
+  if (scratch_doubles < hfx_batch_scratch_doubles((int)(nk_total / seg_len)))
+    return set_error(HFX_EINVAL, "hfx_solve_mu_batched: scratch of %zu doubles, need %zu", scratch_doubles,
+                     hfx_batch_scratch_doubles((int)(nk_total / seg_len)));
   const int nseg = (int)(nk_total / seg_len);
   const size_t n = (size_t)seg_len * nb;
   size_t g = (n + (size_t)MUS_BLOCK * 4 - 1) / ((size_t)MUS_BLOCK * 4);
@@ -998,6 +1178,10 @@ int hfx_solve_mu(const double* eps, const double* w, size_t nk, int nb, double t
   if (mode != HFX_MU_NEWTON_BRACKET && mode != HFX_MU_BISECTION && mode != HFX_MU_NEWTON)
     return set_error(HFX_EINVAL, "hfx_solve_mu: unknown mode %d", mode);
   if (mu0_from_state && !st) return set_error(HFX_EINVAL, "hfx_solve_mu: mu0_from_state needs st");
+  if (maxiter < 0) return set_error(HFX_EINVAL, "hfx_solve_mu: maxiter=%d", maxiter);
+  if (mode == HFX_MU_BISECTION)
+    return ms_bisection_launch(eps, w, nk, nb, target, T, maxiter, level_shift, st, mu_out, p_out, scratch,
+                               (hipStream_t)stream);
   MuArgs a{eps, w, nk, nb, target, T, mu0, level_shift, maxiter, mode, mu0_from_state, st, mu_out, p_out, scratch};
   const int blocks = mu_blocks(nk * (size_t)nb);
   void* args[] = {&a};

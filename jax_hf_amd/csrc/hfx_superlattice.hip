@@ -208,6 +208,10 @@ int hfx_superlattice_fock(int dtype, int nkx, int nky, int n_G, int dim_orb, int
     return set_error(HFX_EINVAL,
                      "hfx_superlattice_fock: the exchange needs delta_i, delta_j, plan, V_lag_fft and a separate "
                      "sigma_planes buffer");
+  if (planes && (plan->n1 != Nx || plan->n2 != Ny || plan->nb != dim_orb || plan->dtype != dtype))
+    return set_error(HFX_EINVAL,
+                     "hfx_superlattice_fock: plan (%dx%d, nb %d, dtype %d) does not match the grid %dx%d, dim_orb %d, "
+                     "dtype %d", plan->n1, plan->n2, plan->nb, plan->dtype, Nx, Ny, dim_orb, dtype);
   if (hartree && (!w2d || !delta_start || !delta_i || !delta_j || !HH_orb || !delta_scratch))
     return set_error(HFX_EINVAL, "hfx_superlattice_fock: the Hartree shift needs its tables and scratch");
   const SlGeom g{nkx, nky, n_G, dim_orb, n_delta, Nx, Ny};

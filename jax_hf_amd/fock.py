@@ -262,6 +262,7 @@ def _sub_inplace(X, Y):
 
 
 _zero_cache: dict = {}
+L.register_cache(_zero_cache.clear)
 
 
 def _zeros_like_cached(P):

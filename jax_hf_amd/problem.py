@@ -36,6 +36,7 @@ class HartreeFockKernel:
                  contact_terms=None, embed_reference_potential: bool = False,
                  center_embedded_hartree: bool = True, *, device=None):
         L.require_gpu()
+        L.register_owner(self)
         device = torch.device(device) if device is not None else default_device()
         h = to_tensor(hamiltonian, device)
         cdt = complex_dtype_of(h)
@@ -187,6 +188,14 @@ class HartreeFockKernel:
                     include_hartree=self.include_hartree, include_exchange=self.include_exchange,
                     exchange_hermitian_channel_packing=self.exchange_hermitian_channel_packing,
                     contact_g=self.contact_g, contact_Oi=self.contact_Oi, contact_Oj=self.contact_Oj)
+
+    def _hfx_release(self):
+        """Drop the cached solver states and their captured graphs (rebuilt on
+        the next solve)."""
+        for key in ("_hfx_cg", "_hfx_polish"):
+            cache = self.__dict__.pop(key, None)
+            if cache:
+                cache.clear()
 
     def fock_engine(self, *, herm=None, refP="kernel", dtype=None):
         """FockEngine bound to this kernel (internal, used by the solvers)."""

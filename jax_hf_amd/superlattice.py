@@ -129,6 +129,10 @@ class _SLFock:
         self.HH_orb = None if HH_orb is None else np.array(HH_orb, dtype=np.float64, order="C", copy=True)
         self.deg = float(degeneracy)
         self._dev = {}
+        L.register_owner(self)
+
+    def _hfx_release(self):
+        self._dev.clear()
 
     def _state(self, dtype, device):
         key = (dtype, str(device))
@@ -148,16 +152,40 @@ class _SLFock:
             self._dev[key] = st
         return st
 
+    def _per_k(self, X, name, P):
+        """X as a contiguous (nkx, nky, D, D) tensor of P's dtype on P's device;
+        a k-independent (D, D) matrix is broadcast over k (as ``h + Sigma``
+        broadcasts in superlattice.py:370)."""
+        X = to_tensor(X, P.device, P.dtype)
+        D = self.D
+        if tuple(X.shape) == (D, D):
+            X = X.expand(self.nkx, self.nky, D, D)
+        if tuple(X.shape) != tuple(P.shape):
+            raise ValueError(f"superlattice Fock: {name} of shape {tuple(X.shape)}, expected {tuple(P.shape)} "
+                             f"or ({D}, {D})")
+        return X.contiguous()
+
     def __call__(self, P, h, refP=None, w2d=None, exchange=True, hartree=False):
         if P.dtype not in (torch.complex128, torch.complex64):
             raise TypeError(f"superlattice Fock: P must be complex128 or complex64, got {P.dtype}")
         shape = (self.nkx, self.nky, self.D, self.D)
         if tuple(P.shape) != shape:
             raise ValueError(f"superlattice Fock: density of shape {tuple(P.shape)}, expected {shape}")
+        if not P.is_contiguous():
+            P = P.contiguous()
+        h = self._per_k(h, "h", P)
+        refP = None if refP is None else self._per_k(refP, "refP", P)
+        w = None
+        if hartree:
+            if w2d is None:
+                raise ValueError("superlattice Fock: the Hartree shift needs w2d")
+            w = to_tensor(w2d, P.device, torch.float64).reshape(-1).contiguous()
+            if w.numel() != self.nkx * self.nky:
+                raise ValueError(f"superlattice Fock: w2d has {w.numel()} entries, expected "
+                                 f"{self.nkx * self.nky} ({self.nkx} x {self.nky})")
         st = self._state(P.dtype, P.device)
         dt = L.HFX_C128 if P.dtype == torch.complex128 else L.HFX_C64
         Sigma, F = torch.empty_like(P), torch.empty_like(P)
-        w = None if not hartree else to_tensor(w2d, P.device, torch.float64).reshape(-1).contiguous()
         lay = self.layout
         L.call("hfx_superlattice_fock", dt, self.nkx, self.nky, self.n_G, self.dorb, int(lay.n_delta),
                int(lay.N_ext_x), int(lay.N_ext_y), L.ptr(P), L.ptr(refP), L.ptr(st["p2d"]), L.ptr(st["goff"]), st["plan"].handle, L.ptr(st["V"]),

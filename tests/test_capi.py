@@ -60,3 +60,48 @@ def test_package_import_is_lazy():
             "assert L._lib is None; print('ok')" % str(ROOT))
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+
+
+def test_batched_entry_points_reject_small_scratch_without_gpu():
+    """The segmented reductions keep one completion ticket per solve at the end
+    of their scratch; an undersized buffer would put those tickets in memory
+    another launch writes.  The sizes are checked before any device call (the
+    dummy addresses below are never dereferenced)."""
+    lib_path = ROOT / "jax_hf_amd" / "libhfx.so"
+    if not lib_path.exists():
+        pytest.skip("libhfx.so not built")
+    from jax_hf_amd import _lib
+    L = _lib.lib()
+    nseg, seg = 3, 64
+    need = int(L.hfx_batch_scratch_doubles(nseg))
+    assert need > int(L.hfx_batch_scratch_doubles(1)) * 2
+    d = ctypes.c_void_p(4096)
+    assert L.hfx_reduce_batched(d, 2, nseg * seg, seg, d, 3, d, 0, None, -1, d, need - 1, None) == -1
+    assert "scratch" in _lib.last_error() and str(need) in _lib.last_error()
+    assert L.hfx_solve_mu_batched(d, d, d, nseg * seg, seg, 4, 0.1, 8, d, d, d, need - 1, None) == -1
+    assert "scratch" in _lib.last_error()
+    plan = ctypes.c_void_p(4096)
+    assert L.hfx_fock_epilogue_batched(plan, nseg, d, d, d, d, d, d, d, 64 * nseg - 1, None) == -1
+    assert "scratch" in _lib.last_error()
+    assert L.hfx_fock_epilogue_planar_batched(plan, nseg, d, d, 1, d, d, d, d, d, 256 * nseg - 1, None) == -1
+    assert "scratch" in _lib.last_error()
+
+
+def test_superlattice_entry_rejects_mismatched_plan_without_gpu():
+    """hfx_superlattice_fock refuses a plan whose grid, orbital count or dtype
+    differs from the extended grid it is given, before any launch.  The plan
+    here is a host struct holding only the leading (n1, n2, nb, dtype) fields
+    the check reads; the other addresses are never dereferenced."""
+    lib_path = ROOT / "jax_hf_amd" / "libhfx.so"
+    if not lib_path.exists():
+        pytest.skip("libhfx.so not built")
+    from jax_hf_amd import _lib
+    L = _lib.lib()
+    Nx = Ny = 36
+    d = ctypes.c_void_p(4096)
+    for fields in ((Nx, Ny, 3, 0), (Nx, 30, 2, 0), (Nx, Ny, 2, 1)):
+        plan = (ctypes.c_int * 64)(*fields)
+        rc = L.hfx_superlattice_fock(_lib.HFX_C128, 6, 6, 7, 2, 19, Nx, Ny, d, None, d, d,
+                                     ctypes.cast(plan, ctypes.c_void_p), d, d, ctypes.c_void_p(8192), 0, None, None,
+                                     d, d, None, 1.0, None, d, d, d, None)
+        assert rc == -1 and "does not match" in _lib.last_error(), fields

@@ -195,3 +195,110 @@ def test_cg_graph_with_superlattice_build_replays_eager_loop(dev):
         assert a.n_iter == b.n_iter
         np.testing.assert_array_equal(_np(a.history["E"]), _np(b.history["E"]))
         np.testing.assert_array_equal(_np(a.density), _np(b.density))
+
+
+def _sl_hartree_setup(nk, dorb, orbital, seed=3):
+    """A superlattice problem with the Hartree shift on (refP = 0, the kernel's
+    own Hartree matrix unused by the plug-in) and the GPU / oracle builders."""
+    from jax_hf_amd.superlattice import build_extended_layout, make_superlattice_build_fock_fn
+    args, g, B, n_G, D = _sl_problem(nk, dorb)
+    rng = np.random.default_rng(seed)
+    HH = np.abs(rng.standard_normal((n_G, n_G))) * 0.05
+    HH = HH + HH.T
+    np.fill_diagonal(HH, 0.0)
+    HHo = None
+    if orbital:
+        HHo = np.abs(rng.standard_normal((n_G, n_G, dorb, dorb))) * 0.05
+        HHo = HHo + HHo.transpose(1, 0, 3, 2)
+        HHo[np.arange(n_G), np.arange(n_G)] = 0.0
+    lay = build_extended_layout(g, B, nk, nk, vfunc, 1.0 / nk ** 2)
+    ref = OS.build_extended_layout(g, B, nk, nk, vfunc, 1.0 / nk ** 2)
+    fb = make_superlattice_build_fock_fn(lay, n_G, dorb, nk, nk, HH, 1.3, HH_GG_orbital=HHo)
+    fb0 = OS.make_superlattice_build_fock_fn(ref, n_G, dorb, nk, nk, HH, 1.3, HH_GG_orbital=HHo)
+    kw = dict(include_hartree=True, hartree_matrix=np.zeros((D, D)),
+              reference_density=np.zeros((nk, nk, D, D), complex))
+    return args, kw, fb, fb0, D
+
+
+@pytest.mark.parametrize("orbital", [False, True])
+@pytest.mark.parametrize("form", ["scf", "cg_fp64", "cg_mixed"])
+def test_solve_with_superlattice_hartree_matches_oracle(dev, orbital, form):
+    """Solves through fock_build_fn with the superlattice Hartree shift (scalar
+    and orbital tables, dim_orb 2) against the oracle solvers with the oracle
+    adapter; the CG forms run the build inside the captured iteration graph,
+    the mixed one in complex64 with the FP64 polish through the same plug-in."""
+    import jax_hf_amd as J
+    nk, dorb = 6, 2
+    args, kw, fb, fb0, D = _sl_hartree_setup(nk, dorb, orbital)
+    G = J.HartreeFockKernel(*args, **kw)
+    K0 = O.OracleKernel(*args, **kw)
+    P0 = np.zeros((nk, nk, D, D), complex)
+    ne = 3.0
+    scf_cfg = dict(max_iter=300, density_tol=1e-11, comm_tol=1e-10, acceleration="diis")
+    ref = O.solve_scf(K0, P0, ne, config=O.SCFConfig(**scf_cfg), fock_build_fn=fb0)
+    assert ref.converged
+    if form == "scf":
+        r = J.solve_scf(G, P0, ne, config=J.SCFConfig(**scf_cfg), fock_build_fn=fb)
+        assert r.converged
+        assert abs(float(r.energy) - ref.energy) < 1e-10
+        assert np.abs(_np(r.density_matrix) - ref.density_matrix).max() < 1e-8
+        return
+    prec = "fp64" if form == "cg_fp64" else "mixed"
+    cfg = dict(max_iter=400, tol_E=1e-12, precision=prec)
+    if prec == "fp64":
+        cfg.update(polish="diis")
+    r = J.solve(G, P0, ne, config=J.SolverConfig(**cfg), fock_build_fn=fb)
+    assert r.converged
+    # both forms end on the FP64 SCF fixed point through the polish
+    assert abs(float(r.energy) - ref.energy) < 1e-9
+    assert np.abs(_np(r.density) - ref.density_matrix).max() < 1e-6
+    # and the FP64 CG trajectory follows the oracle CG's first iterations
+    if prec == "fp64":
+        r0 = O.solve(K0, P0, ne, config=O.SolverConfig(max_iter=400, tol_E=1e-12), fock_build_fn=fb0)
+        np.testing.assert_allclose(_np(r.history["E"])[:5], r0.history["E"][:5], atol=1e-10)
+
+
+def test_cg_graph_with_superlattice_hartree_replays_eager_loop(dev):
+    """The captured CG iteration with the Hartree-moment launches of the
+    superlattice build replays the eager loop bit for bit."""
+    import jax_hf_amd as J
+    nk, dorb = 6, 2
+    args, kw, fb, _, D = _sl_hartree_setup(nk, dorb, True)
+    G = J.HartreeFockKernel(*args, **kw)
+    P0 = np.zeros((nk, nk, D, D), complex)
+    res = {}
+    for use_graph in (False, True):
+        cfg = J.SolverConfig(precision="fp64", max_iter=30, tol_E=1e-13, use_graph=use_graph)
+        res[use_graph] = J.solve(G, P0, 3.0, config=cfg, fock_build_fn=fb)
+    a, b = res[False], res[True]
+    assert a.n_iter > 3 and a.n_iter == b.n_iter
+    np.testing.assert_array_equal(_np(a.history["E"]), _np(b.history["E"]))
+    np.testing.assert_array_equal(_np(a.density), _np(b.density))
+
+
+def test_adapter_checks_shapes_and_broadcasts_h(dev):
+    """h / refP must be per-k (nkx, nky, D, D) or one (D, D) matrix (broadcast
+    over k as h + Sigma broadcasts); w2d must hold nkx * nky weights."""
+    import torch
+    from jax_hf_amd.superlattice import make_superlattice_build_fock_fn
+    nk, dorb = 6, 2
+    lay, ref, n_G = _layouts(nk)
+    D = n_G * dorb
+    rng = np.random.default_rng(8)
+    HH = np.zeros((n_G, n_G))
+    P = _rand(rng, (nk, nk, D, D)) * 1e-2
+    hk = _rand(rng, (D, D))
+    w = np.full((nk, nk), 1.0 / nk ** 2)
+    t = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)  # noqa: E731
+    fb = make_superlattice_build_fock_fn(lay, n_G, dorb, nk, nk, HH, 1.0)
+    fb0 = OS.make_superlattice_build_fock_fn(ref, n_G, dorb, nk, nk, HH, 1.0)
+    kw = dict(include_exchange=True, include_hartree=True)
+    S1, _, F1 = fb(t(P), h=t(hk), refP=None, w2d=t(w), **kw)
+    S0, _, F0 = fb0(P, h=np.broadcast_to(hk, P.shape), refP=None, w2d=w, **kw)
+    assert np.abs(_np(F1) - F0).max() <= 1e-12 * np.abs(F0).max()
+    with pytest.raises(ValueError, match="h of shape"):
+        fb(t(P), h=t(hk[:-1]), refP=None, w2d=t(w), **kw)
+    with pytest.raises(ValueError, match="refP of shape"):
+        fb(t(P), h=t(hk), refP=t(P[:, :-1]), w2d=t(w), **kw)
+    with pytest.raises(ValueError, match="w2d has"):
+        fb(t(P), h=t(hk), refP=None, w2d=t(w[:-1]), **kw)

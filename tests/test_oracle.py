@@ -378,3 +378,213 @@ def test_synthetic_generators_shapes_and_hermiticity():
     # minimal image is symmetric under q -> -q on the centred mesh
     q = syn.qnorm_min_image(6, 6)
     np.testing.assert_allclose(q[1:, 1:], q[1:, 1:][::-1, ::-1], atol=1e-12)
+
+
+# ------------------------------------------- the reference's solver property tests
+# tests/test_scf_acceleration.py:43-101 and tests/test_solver_newton.py:204-241,
+# run on the oracle in FP64 (the reference runs the SCF ones in float32).  They pin
+# the branches of solve_scf (DIIS, ODA step, trust-radius clip, level shift) and
+# of Newton's trust region by properties the mathematics fixes, not by the
+# oracle's own formulas.
+
+def _ref_kernel(case):
+    import ref_cases as RC
+    args, kw = case
+    return O.OracleKernel(*args, **kw)
+
+
+def _dirac_scf(acc, **extra):
+    import ref_cases as RC
+    K = _ref_kernel(RC.dirac())
+    cfg = dict(max_iter=800, density_tol=1e-6, comm_tol=1e-5, mixing=0.5)
+    cfg.update(extra)
+    return K, O.solve_scf(K, np.zeros_like(K.h), 16.0, config=O.SCFConfig(acceleration=acc, **cfg))
+
+
+def test_scf_schemes_agree_on_energy():
+    """test_scf_acceleration.py:43-54 (spread < 1e-3); FP64 agrees far tighter."""
+    E = {acc: _dirac_scf(acc)[1].energy for acc in ("linear", "diis", "oda")}
+    assert max(E.values()) - min(E.values()) < 1e-8, E
+
+
+def test_scf_diis_far_faster_than_linear():
+    """test_scf_acceleration.py:57-68: DIIS needs < 1/3 of linear mixing's steps."""
+    import ref_cases as RC
+    K, lin = _dirac_scf("linear")
+    _, diis = _dirac_scf("diis")
+    assert lin.converged and diis.converged
+    assert diis.iterations < lin.iterations // 3
+    np.testing.assert_allclose(diis.energy, lin.energy, atol=1e-4, rtol=1e-5)
+    assert RC.comm_rms(diis.fock_matrix, diis.density_matrix, K.w2d) < 1e-4
+
+
+def test_scf_trust_radius_and_damping_converge():
+    """test_scf_acceleration.py:70-88: the clipped and the damped DIIS reach the
+    free DIIS energy; every clipped step is within the trust radius."""
+    K, free = _dirac_scf("diis")
+    _, clipped = _dirac_scf("diis", trust_radius=0.5)
+    _, damped = _dirac_scf("diis", diis_damping=0.7)
+    assert clipped.converged and damped.converged
+    np.testing.assert_allclose(clipped.energy, free.energy, atol=1e-4, rtol=1e-5)
+    np.testing.assert_allclose(damped.energy, free.energy, atol=1e-4, rtol=1e-5)
+    # a radius below the first step must bind: the clipped run then differs from
+    # the free one while reaching the same fixed point
+    _, tight = _dirac_scf("diis", trust_radius=0.05)
+    assert tight.converged and tight.iterations > free.iterations
+    np.testing.assert_allclose(tight.energy, free.energy, atol=1e-8)
+
+
+def test_scf_oda_energy_matches_and_is_monotone():
+    """test_scf_acceleration.py:91-101: ODA reaches linear mixing's energy and its
+    energy history never rises (the step is the exact line minimum of the
+    quadratic model, so a wrong sign of lambda would climb)."""
+    _, lin = _dirac_scf("linear")
+    _, oda = _dirac_scf("oda")
+    np.testing.assert_allclose(oda.energy, lin.energy, atol=1e-3, rtol=1e-5)
+    hE = np.asarray(oda.history["E"])
+    assert hE.size > 1 and np.all(np.diff(hE) <= 1e-4)
+    assert oda.iterations < lin.iterations
+
+
+def _two_level_case(nk=2, e1=-0.2, e2=0.3, T=0.1, seed=5):
+    """Every k holds the same 2x2 matrix U diag(e1, e2) U^H (U random unitary),
+    no interaction: F = h at every iteration."""
+    rng = np.random.default_rng(seed)
+    Z = rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))
+    U, _ = np.linalg.qr(Z)
+    hk = U @ np.diag([e1, e2]) @ U.conj().T
+    h = np.broadcast_to(hk, (nk, nk, 2, 2)).copy()
+    return (np.ones((nk, nk)), h, np.zeros((nk, nk, 1, 1), np.complex128), T), U
+
+
+@pytest.mark.parametrize("shift", [0.0, 0.4, 2.5])
+def test_level_shift_two_level_closed_form(shift):
+    """reference_scf.py:87-96 on a two-level system.  Half filling of two levels
+    fixes mu = (a + b)/2 for any pair (f(x) + f(-x) = 1), so mu_raw sits midway,
+    the shift moves the levels apart to Delta' = Delta + s tanh(Delta / 4T), and
+    the lower level's occupation is f(-Delta'/2).  A wrong sign, the occupied
+    instead of the empty levels shifted, or an ignored shift all give another
+    occupation."""
+    e1, e2, T = -0.2, 0.3, 0.1
+    (w, h, V, _), U = _two_level_case(e1=e1, e2=e2, T=T)
+    K = O.OracleKernel(w, h, V, T)          # V = 0: F = h
+    r = O.solve_scf(K, np.zeros_like(h), 4.0, config=O.SCFConfig(
+        max_iter=1, mixing=1.0, level_shift=shift))
+    D = e2 - e1
+    Dp = D + shift * np.tanh(D / (4 * T))
+    p1 = 1.0 / (1.0 + np.exp(-Dp / (2 * T)))
+    P = U @ np.diag([p1, 1.0 - p1]) @ U.conj().T
+    np.testing.assert_allclose(r.density_matrix, np.broadcast_to(P, h.shape), atol=1e-13)
+
+
+def test_level_shift_keeps_gapped_fixed_point():
+    """A shift larger than the gap leaves a cold insulator's fixed point alone
+    (only empty levels move up, the occupations stay 0/1), while the converged
+    state stays stationary ([F, P] = 0)."""
+    import ref_cases as RC
+    K = _ref_kernel(RC.gapped())
+    P0 = np.zeros_like(K.h)
+    kw = dict(max_iter=300, density_tol=1e-11, comm_tol=1e-10, acceleration="diis")
+    r0 = O.solve_scf(K, P0, 16.0, config=O.SCFConfig(**kw))
+    r1 = O.solve_scf(K, P0, 16.0, config=O.SCFConfig(level_shift=10.0, **kw))
+    ev = np.linalg.eigvalsh(r0.fock_matrix)
+    assert ev[..., 1].min() - ev[..., 0].max() < 10.0        # the shift exceeds the gap
+    assert r0.converged and r1.converged
+    assert abs(r1.energy - r0.energy) < 1e-10
+    np.testing.assert_allclose(r1.density_matrix, r0.density_matrix, atol=1e-9)
+    assert RC.comm_rms(r1.fock_matrix, r1.density_matrix, K.w2d) < 1e-9
+
+
+@pytest.mark.parametrize("nk,nb,T,ex,seed", [(4, 4, 0.05, 0.3, 7), (5, 5, 0.05, 0.4, 4),
+                                             (4, 5, 0.08, 0.5, 11)])
+def test_newton_multiband_stationary_and_conserves_particles(nk, nb, T, ex, seed):
+    """test_solver_newton.py:216-230."""
+    import ref_cases as RC
+    K = _ref_kernel(RC.multiband(nk, nb, T, ex, seed))
+    ne = float(nk * nk * nb * 0.5)
+    r = O.solve(K, np.zeros_like(K.h), ne, config=O.SolverConfig(
+        max_iter=200, tol_grad=1e-6, optimizer="newton"))
+    assert r.converged
+    assert RC.comm_rms(r.fock, r.density, K.w2d) < 1e-4
+    np.testing.assert_allclose(float(np.sum(K.w2d[..., None] * r.p)), ne, atol=1e-4)
+
+
+def test_newton_superlinear_on_dirac():
+    """test_solver_newton.py:232-241: Newton reaches CG's energy in fewer than a
+    fifth of its outer iterations."""
+    import ref_cases as RC
+    K = _ref_kernel(RC.dirac(ex=0.5))
+    P0 = np.zeros_like(K.h)
+    cfg = dict(max_iter=2000, tol_grad=1e-7)
+    cg = O.solve(K, P0, 16.0, config=O.SolverConfig(optimizer="cg", **cfg))
+    nw = O.solve(K, P0, 16.0, config=O.SolverConfig(optimizer="newton", **cfg))
+    assert nw.converged
+    np.testing.assert_allclose(nw.energy, cg.energy, atol=1e-6, rtol=1e-7)
+    assert nw.n_iter < cg.n_iter // 5
+
+
+def _hartree_multiband(hh=2.0):
+    """multiband(4, 4) with a strong on-band Hartree repulsion: a full SCF step
+    overshoots, so the ODA line minimum lies inside (0, 1)."""
+    import ref_cases as RC
+    (w, h, V, T), _ = RC.multiband(4, 4, 0.05, 0.3, 7)
+    return (w, h, V, T), dict(include_hartree=True, hartree_matrix=hh * np.eye(4),
+                              reference_density=np.zeros_like(h))
+
+
+def _thermal_density(K, h, n, T, scale, seed):
+    """A particle-conserving thermal density of a randomly perturbed h."""
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal(h.shape) + 1j * rng.standard_normal(h.shape)
+    ev, U = np.linalg.eigh(h + scale * (X + np.conj(np.swapaxes(X, -1, -2))))
+    mu = O.find_chemical_potential(ev, K.w2d, n, T)
+    return np.einsum("...in,...n,...jn->...ij", U, O.fermidirac(ev - mu, T), np.conj(U))
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_oda_step_is_the_line_minimum(seed):
+    """reference_scf.py:315-328.  One ODA step from P moves along the Roothaan
+    step d = P_new - P to the minimum of the energy on the segment [P, P + d]:
+    the energy there is below every point of a 201-point grid on the segment
+    and its central difference along d vanishes (the energy is quadratic in P,
+    so the minimum is where dE/dlambda = 0 unless it is clipped to an end)."""
+    (w, h, V, T), kw = _hartree_multiband()
+    K = O.OracleKernel(w, h, V, T, **kw)
+    a = K.as_args()
+
+    def energy(P):
+        S, H, _ = O.build_fock(P, h=a["h"], VR=a["VR"], refP=a["refP"], HH=a["HH"], w2d=K.w2d,
+                               include_exchange=True, include_hartree=True,
+                               exchange_hermitian_channel_packing=a["exchange_hermitian_channel_packing"])
+        return O.hf_energy(P, h=a["h"], Sigma=S, H=H, weights_b=K.weights_b)
+
+    P0 = _thermal_density(K, h, 32.0, T, 0.5 * seed, seed)
+    Pn = O.solve_scf(K, P0, 32.0, config=O.SCFConfig(max_iter=1, mixing=1.0)).density_matrix
+    Po = O.solve_scf(K, P0, 32.0, config=O.SCFConfig(max_iter=1, acceleration="oda")).density_matrix
+    d = Pn - P0
+    lam = float(np.real(np.vdot(d, Po - P0)) / np.real(np.vdot(d, d)))
+    np.testing.assert_allclose(Po, P0 + lam * d, atol=1e-13)
+    assert 0.02 < lam < 0.98
+    grid = np.linspace(0.0, 1.0, 201)
+    assert energy(Po) <= min(energy(P0 + g * d) for g in grid) + 1e-12
+    eps = 1e-4
+    slope = (energy(P0 + (lam + eps) * d) - energy(P0 + (lam - eps) * d)) / (2 * eps)
+    assert abs(slope) < 1e-7 * max(1.0, abs(energy(Po)))
+
+
+def test_trust_radius_clips_the_step_to_the_radius():
+    """reference_scf.py:332-339: a step longer than trust_radius (weighted norm
+    sqrt(sum_k w_k |S_k|^2 / sum w)) is scaled onto the radius, direction kept;
+    a shorter one is untouched."""
+    import ref_cases as RC
+    args, kw = RC.dirac()
+    K = O.OracleKernel(*args, **kw)
+    P0 = _thermal_density(K, K.h, 16.0, K.T, 0.8, 3)
+    free = O.solve_scf(K, P0, 16.0, config=O.SCFConfig(max_iter=1, mixing=0.5)).density_matrix - P0
+    n_free = np.sqrt(np.sum(K.w2d * np.sum(np.abs(free) ** 2, axis=(-2, -1))) / np.sum(K.w2d))
+    for r in (0.25 * n_free, 0.5 * n_free, 2.0 * n_free):
+        s = O.solve_scf(K, P0, 16.0, config=O.SCFConfig(max_iter=1, mixing=0.5, trust_radius=r)
+                        ).density_matrix - P0
+        n = np.sqrt(np.sum(K.w2d * np.sum(np.abs(s) ** 2, axis=(-2, -1))) / np.sum(K.w2d))
+        np.testing.assert_allclose(n, min(r, n_free), rtol=1e-12)
+        np.testing.assert_allclose(s, free * (n / n_free), atol=1e-13)
