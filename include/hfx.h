@@ -420,6 +420,19 @@ int hfx_solve_mu(const double* eps, const double* w, size_t nk, int nb, double t
                  int mu0_from_state, int maxiter, int mode, double level_shift, double* st, double* mu_out,
                  double* p_out, double* scratch, void* stream);
 
+/* No counterpart in the reference.  hfx_solve_mu without the scratch argument,
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the scratch
+ * is a library-owned static device array of hfx_noalloc_scratch_doubles()
+ * doubles (256 KiB) per GPU, held until the process ends; the call allocates
+ * nothing.  nk * nb must be below 2^32 (HFX_EINVAL otherwise).  All calls on
+ * one GPU share that array: calls that overlap on one GPU (several streams or
+ * unordered host threads) race. */
+size_t hfx_noalloc_scratch_doubles(void);
+int hfx_solve_mu_noalloc(const double* eps, const double* w, size_t nk, int nb, double target, double T,
+                         double mu0, int mu0_from_state, int maxiter, int mode, double level_shift, double* st,
+                         double* mu_out, double* p_out, void* stream);
+
 /* The Newton-with-bracket mu solver of solver.py:62-101 as a fixed sequence
  * of ordinary launches (bracket, maxiter steps, occupations) that can be
  * captured in a HIP graph: each step is a grid reduction whose last block

@@ -103,8 +103,11 @@ _SIGS = {
     "hfx_reduce": [vp, i32, sz, vp, C.c_uint, vp, vp, i32, vp, vp, vp],
     "hfx_solve_mu_steps": [vp, vp, sz, i32, f64, f64, f64, i32, i32, vp, vp, vp, vp],
     "hfx_solve_mu": [vp, vp, sz, i32, f64, f64, f64, i32, i32, i32, f64, vp, vp, vp, vp, vp],
+    "hfx_noalloc_scratch_doubles": [],
+    "hfx_solve_mu_noalloc": [vp, vp, sz, i32, f64, f64, f64, i32, i32, i32, f64, vp, vp, vp, vp],
 }
-_RESTYPES = {"hfx_energy_scratch_bytes": sz, "hfx_solver_scratch_doubles": sz, "hfx_batch_scratch_doubles": sz}
+_RESTYPES = {"hfx_energy_scratch_bytes": sz, "hfx_solver_scratch_doubles": sz, "hfx_batch_scratch_doubles": sz,
+             "hfx_noalloc_scratch_doubles": sz}
 
 
 class HfxError(RuntimeError):
