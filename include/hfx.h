@@ -412,7 +412,7 @@ int hfx_reduce(const double* part, int nval, size_t nk, const double* w, unsigne
  * Newton (utils.py:385-442) in one cooperative launch; bisection
  * (utils.py:353-382; maxiter levels, 54 in FP64 / 30 in FP32), twice with the
  * level shift of reference_scf.py:87-96 when level_shift != 0, as a
- * multisection of ordinary launches (six levels of the bisection tree per
+ * multisection of ordinary launches (three levels of the bisection tree per
  * pass, the same midpoints and tests).  Result to st[HFX_ST_MU] and/or
  * mu_out; occupations to p_out (sigmoid((mu-eps)/T), or fermidirac of the
  * shifted levels).  scratch: hfx_solver_scratch_doubles() doubles. */

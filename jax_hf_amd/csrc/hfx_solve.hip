@@ -16,7 +16,7 @@
 //     blocks step mu identically;
 //   * the pure bisection of utils.py:353-382 (optionally twice, with the level
 //     shift of reference_scf.py:87-96) runs as a multisection: ordinary
-//     launches, six levels of the bisection tree per pass.
+//     launches, three levels of the bisection tree per pass.
 #include <hip/hip_cooperative_groups.h>
 
 #include "hfx_common.h"
@@ -765,30 +765,39 @@ __global__ __launch_bounds__(64) void diis_coef_kernel(const double* __restrict_
 // The bisection walks a binary tree: each level takes mid = 0.5 * (lo + hi)
 // and keeps [lo, mid] when the count at mid exceeds the target, else
 // [mid, hi].  One pass evaluates the counts at all 2^L - 1 midpoints of the
-// next L <= 6 levels of that tree in one launch (every block builds the same
-// tree from the same bracket with the same expression), and one workgroup then
-// walks the L levels with those counts: the same midpoints and the same tests
-// as the level-by-level loop, so the result is the bisection's mu, reached in
-// ceil(maxiter / 6) passes of two ordinary launches each (no grid barrier, so
-// a graph may capture it).  The bracket, the collapse flag and mu live in the
-// scratch; the per-block node counts follow them.
-constexpr int MS_BLOCK = 512;
-constexpr int MS_MAX_BLOCKS = 256;
-constexpr int MS_LEVELS = 6;
-constexpr int MS_NODES = 64;   // 2^6 - 1 tree nodes, padded to a wavefront
-constexpr size_t MS_OFF = MU_TICKET_OFF + 8;
-constexpr size_t MS_PART_OFF = MS_OFF + 16;
-constexpr size_t MS_END = MS_PART_OFF + (size_t)MS_MAX_BLOCKS * MS_NODES;
-enum { MS_LO = 0, MS_HI = 1, MS_DONE = 2, MS_MURAW = 3, MS_MU = 4 };
+// next L <= 3 levels of that tree (every block builds the same tree from the
+// same bracket with the same expression) and writes per-block node counts;
+// the next pass starts with every block summing those counts in one fixed
+// order (so all blocks get bitwise-identical totals) and walking the previous
+// pass's L levels: the same midpoints and the same tests as the level-by-level
+// loop, in ceil(maxiter / 3) ordinary launches (no grid barrier, so a graph may
+// capture them), and a one-workgroup launch walks the last pass.  Three
+// levels per pass balance the launch count against the 2^L - 1 Fermi
+// functions per eigenvalue (six levels spent 64 us per pass in them at
+// 96 x 96 x 16).  An eigenvalue 37.5 T or more below the bracket contributes
+// exactly its weight at every node (1 + e^x rounds to 1); one 37.5 T above it
+// contributes less than 6e-17 of its weight and is skipped.
+constexpr int MS_BLOCK = 256;
+constexpr int MS_MAX_BLOCKS = 512;
+constexpr int MS_LEVELS = 3;
+constexpr int MS_NODES = 8;    // 2^3 - 1 tree nodes, padded
+constexpr size_t MS_OFF = MU_TICKET_OFF + 8;                         // state
+constexpr size_t MS_MM_OFF = MS_OFF + 16;                            // [block] min, max
+constexpr size_t MS_PART_OFF = MS_MM_OFF + 2 * (size_t)MS_MAX_BLOCKS;  // [2][block][MS_NODES]
+constexpr size_t MS_END = MS_PART_OFF + 2 * (size_t)MS_MAX_BLOCKS * MS_NODES;
+// state: bracket slot s (pass parity) at 4 s: lo, hi, collapsed; then mu, mu_raw
+enum { MS_LO = 0, MS_HI = 1, MS_DONE = 2, MS_MU = 8, MS_MURAW = 9 };
+constexpr double MS_SAT = 37.5;   // e^-37.5 < 2^-54
 
 struct MsArgs {
   const double* eps;   // (n) eigenvalues, nb per k-point
   const double* w;     // (n / nb) weights
   size_t n;
-  int nb;
+  int nb, nblk;
   double T, target, level_shift;
   int shift;           // count the level-shifted values (reads state[MS_MURAW])
   double* state;       // scratch + MS_OFF
+  double* mm;          // scratch + MS_MM_OFF
   double* part;        // scratch + MS_PART_OFF
 };
 
@@ -816,127 +825,60 @@ __global__ __launch_bounds__(MS_BLOCK) void ms_minmax_part(MsArgs a) {
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int q = 0; q < MS_BLOCK / 64; ++q) { lo = fmin(lo, sh[q]); hi = fmax(hi, sh[MS_BLOCK / 64 + q]); }
-    a.part[2 * blockIdx.x] = lo;
-    a.part[2 * blockIdx.x + 1] = hi;
+    a.mm[2 * blockIdx.x] = lo;
+    a.mm[2 * blockIdx.x + 1] = hi;
   }
 }
 
-// initial bracket: span = max - min + 10 max(T, 1e-6) on both sides (utils.py:366-370)
-__global__ __launch_bounds__(256) void ms_minmax_fin(MsArgs a, int nblk) {
-  __shared__ double sh[2 * 4];
-  double lo = INFINITY, hi = -INFINITY;
-  for (int b = threadIdx.x; b < nblk; b += 256) {
-    lo = fmin(lo, a.part[2 * b]);
-    hi = fmax(hi, a.part[2 * b + 1]);
-  }
-  lo = wave_min(lo);
-  hi = wave_max(hi);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) { sh[wid] = lo; sh[4 + wid] = hi; }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int q = 0; q < 4; ++q) { lo = fmin(lo, sh[q]); hi = fmax(hi, sh[4 + q]); }
-    const double span = hi - lo + 10.0 * fmax(a.T, 1e-6);
-    a.state[MS_LO] = lo - span;
-    a.state[MS_HI] = hi + span;
-    a.state[MS_DONE] = 0.0;
-  }
-}
-
-// Wavefront sum of 32 values per lane, scattered: lane l returns the sum over
-// the 64 lanes of v[l & 31] (a full butterfly step across the two half-waves,
-// then a reduce-scatter: each step keeps the half of the remaining values the
-// lane's bit selects and adds the partner's copy of it; 63 shuffles).
-constexpr int MS_CHUNK = 32;
-__device__ __forceinline__ double wave_scatter_sum32(double (&v)[MS_CHUNK]) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int j = 0; j < MS_CHUNK; ++j) v[j] += __shfl_xor(v[j], 32, 64);
-#pragma unroll
-  for (int h = MS_CHUNK / 2; h >= 1; h >>= 1) {
-    const bool up = (lane & h) != 0;
-#pragma unroll
-    for (int j = 0; j < h; ++j) {
-      const double send = up ? v[j] : v[j + h];
-      const double keep = up ? v[j + h] : v[j];
-      v[j] = keep + __shfl_xor(send, h, 64);
+// The bracket at the start of pass `pass` into br[0..2] (lo, hi, collapsed),
+// identical in every block.  Pass 0: the initial bracket from the min / max
+// partials, span = max - min + 10 max(T, 1e-6) on both sides (utils.py:366-370).
+// Later passes: the previous pass's bracket walked down its `prev_levels`
+// levels with the summed node counts.
+__device__ void ms_bracket(const MsArgs& a, int pass, int prev_levels, double* sh, double* br) {
+  if (pass == 0) {
+    double lo = INFINITY, hi = -INFINITY;
+    for (int b = threadIdx.x; b < a.nblk; b += MS_BLOCK) {
+      lo = fmin(lo, a.mm[2 * b]);
+      hi = fmax(hi, a.mm[2 * b + 1]);
     }
-  }
-  return v[0];
-}
-
-// counts at the nodes of the next levels of the bisection tree (breadth-first:
-// node i has children 2i+1 = [lo_i, mid_i] and 2i+2 = [mid_i, hi_i]), in
-// chunks of 32 nodes (32 accumulators per thread)
-__global__ __launch_bounds__(MS_BLOCK) void ms_count_part(MsArgs a, int nodes) {
-  __shared__ double plo[MS_NODES], phi[MS_NODES], pmid[MS_NODES];
-  __shared__ double wsum[MS_BLOCK / 64][MS_NODES];
-  if (a.state[MS_DONE] != 0.0) return;   // the bracket collapsed in an earlier pass
-  if (threadIdx.x == 0) {
-    plo[0] = a.state[MS_LO];
-    phi[0] = a.state[MS_HI];
-    for (int i = 0; i < nodes; ++i) {
-      const double m = 0.5 * (plo[i] + phi[i]);
-      pmid[i] = m;
-      const int c = 2 * i + 1;
-      if (c + 1 < nodes) {
-        plo[c] = plo[i];
-        phi[c] = m;
-        plo[c + 1] = m;
-        phi[c + 1] = phi[i];
-      }
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) { sh[wid] = lo; sh[MS_BLOCK / 64 + wid] = hi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int q = 0; q < MS_BLOCK / 64; ++q) { lo = fmin(lo, sh[q]); hi = fmax(hi, sh[MS_BLOCK / 64 + q]); }
+      const double span = hi - lo + 10.0 * fmax(a.T, 1e-6);
+      br[0] = lo - span;
+      br[1] = hi + span;
+      br[2] = 0.0;
     }
+    __syncthreads();
+    return;
   }
-  __syncthreads();
-  const double mu_raw = a.shift ? a.state[MS_MURAW] : 0.0;
-  const double Td = a.T + 1e-12;
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int c0 = 0; c0 < nodes; c0 += MS_CHUNK) {
-    const int nc = nodes - c0 < MS_CHUNK ? nodes - c0 : MS_CHUNK;
-    double acc[MS_CHUNK];
+  const double* st = a.state + 4 * ((pass - 1) & 1);
+  const bool done = st[MS_DONE] != 0.0;   // uniform
+  double tot[MS_NODES];
 #pragma unroll
-    for (int j = 0; j < MS_CHUNK; ++j) acc[j] = 0.0;
-    for (size_t e = (size_t)blockIdx.x * MS_BLOCK + threadIdx.x; e < a.n; e += (size_t)gridDim.x * MS_BLOCK) {
-      const double v = ms_value(a, e, mu_raw);
-      const double wk = a.w[e / a.nb];
-#pragma unroll
-      for (int j = 0; j < MS_CHUNK; ++j)
-        if (j < nc) acc[j] += wk * sigm(-(v - pmid[c0 + j]) / Td);   // fermidirac(eps - mid, T)
-    }
-    const double t = wave_scatter_sum32(acc);
-    if (lane < MS_CHUNK) wsum[wid][c0 + lane] = t;
-  }
-  __syncthreads();
-  if (threadIdx.x < MS_NODES) {
-    double s = 0.0;
-    for (int q = 0; q < MS_BLOCK / 64; ++q) s += wsum[q][threadIdx.x];
-    a.part[(size_t)blockIdx.x * MS_NODES + threadIdx.x] = s;
-  }
-}
-
-// walk `levels` levels with the pass's counts; store: 1 -> mu, 2 -> mu and mu_raw
-__global__ __launch_bounds__(256) void ms_count_fin(MsArgs a, int nblk, int levels, int store) {
-  __shared__ double tot[4][MS_NODES];
-  const bool done = a.state[MS_DONE] != 0.0;
-  if (!done && levels > 0) {
-    const int t = threadIdx.x & 63, q = threadIdx.x >> 6;
-    double s = 0.0;
-    for (int b = q; b < nblk; b += 4) s += a.part[(size_t)b * MS_NODES + t];
-    tot[q][t] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  double lo = a.state[MS_LO], hi = a.state[MS_HI];
+  for (int j = 0; j < MS_NODES; ++j) tot[j] = 0.0;
   if (!done) {
+    const double* part = a.part + (size_t)((pass - 1) & 1) * MS_MAX_BLOCKS * MS_NODES;
+    for (int b = threadIdx.x; b < a.nblk; b += MS_BLOCK)
+#pragma unroll
+      for (int j = 0; j < MS_NODES; ++j) tot[j] += part[(size_t)b * MS_NODES + j];
+    block_sum<MS_NODES>(tot, sh);
+  }
+  if (threadIdx.x == 0) {
+    double lo = st[MS_LO], hi = st[MS_HI], col = done ? 1.0 : 0.0;
     int node = 0;
-    for (int lev = 0; lev < levels; ++lev) {
+    for (int lev = 0; lev < prev_levels && col == 0.0; ++lev) {
       const double mid = 0.5 * (lo + hi);
       if (mid == lo || mid == hi) {   // collapsed: later levels change nothing
-        a.state[MS_DONE] = 1.0;
+        col = 1.0;
         break;
       }
-      const double cnt = ((tot[0][node] + tot[1][node]) + tot[2][node]) + tot[3][node];
-      if (cnt > a.target) {
+      if (tot[node] > a.target) {
         hi = mid;
         node = 2 * node + 1;
       } else {
@@ -944,11 +886,77 @@ __global__ __launch_bounds__(256) void ms_count_fin(MsArgs a, int nblk, int leve
         node = 2 * node + 2;
       }
     }
-    a.state[MS_LO] = lo;
-    a.state[MS_HI] = hi;
+    br[0] = lo;
+    br[1] = hi;
+    br[2] = col;
   }
-  if (store) {
-    const double mu = 0.5 * (lo + hi);
+  __syncthreads();
+}
+
+// One pass: the bracket, then the counts at the NODES = 2^L - 1 nodes of the
+// next L levels (breadth-first: node i has children 2i+1 = [lo_i, mid_i] and
+// 2i+2 = [mid_i, hi_i]) into part[pass & 1][block][MS_NODES].
+template <int NODES>
+__global__ __launch_bounds__(MS_BLOCK) void ms_count_pass(MsArgs a, int pass, int prev_levels) {
+  __shared__ double sh[MS_NODES * (MS_BLOCK / 64)];
+  __shared__ double br[3];
+  ms_bracket(a, pass, prev_levels, sh, br);
+  const double lo = br[0], hi = br[1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double* st = a.state + 4 * (pass & 1);
+    st[MS_LO] = lo;
+    st[MS_HI] = hi;
+    st[MS_DONE] = br[2];
+  }
+  if (br[2] != 0.0) return;   // collapsed in an earlier pass (uniform)
+  double plo[NODES], phi[NODES], mid[NODES];
+  plo[0] = lo;
+  phi[0] = hi;
+#pragma unroll
+  for (int i = 0; i < NODES; ++i) {
+    mid[i] = 0.5 * (plo[i] + phi[i]);
+    if (2 * i + 2 < NODES) {
+      plo[2 * i + 1] = plo[i];
+      phi[2 * i + 1] = mid[i];
+      plo[2 * i + 2] = mid[i];
+      phi[2 * i + 2] = phi[i];
+    }
+  }
+  const double mu_raw = a.shift ? a.state[MS_MURAW] : 0.0;
+  const double Td = a.T + 1e-12;
+  const double win = MS_SAT * Td;
+  double acc[MS_NODES], below = 0.0;
+#pragma unroll
+  for (int j = 0; j < MS_NODES; ++j) acc[j] = 0.0;
+  for (size_t e = (size_t)blockIdx.x * MS_BLOCK + threadIdx.x; e < a.n; e += (size_t)gridDim.x * MS_BLOCK) {
+    const double v = ms_value(a, e, mu_raw);
+    const double wk = a.w[e / a.nb];
+    if (v - lo <= -win) {
+      below += wk;                // occupied at every node
+    } else if (v - hi < win) {
+#pragma unroll
+      for (int j = 0; j < NODES; ++j) acc[j] += wk * sigm(-(v - mid[j]) / Td);   // fermidirac(eps - mid, T)
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < MS_NODES; ++j) acc[j] += below;
+  __syncthreads();   // sh is reused
+  block_sum<MS_NODES>(acc, sh);
+  if (threadIdx.x == 0) {
+    double* part = a.part + (size_t)(pass & 1) * MS_MAX_BLOCKS * MS_NODES + (size_t)blockIdx.x * MS_NODES;
+#pragma unroll
+    for (int j = 0; j < MS_NODES; ++j) part[j] = acc[j];
+  }
+}
+
+// walk the last pass (`passes` passes done, the last of `levels` levels):
+// mu = the midpoint of the final bracket; store 2 also keeps it as mu_raw
+__global__ __launch_bounds__(MS_BLOCK) void ms_finish(MsArgs a, int passes, int levels, int store) {
+  __shared__ double sh[MS_NODES * (MS_BLOCK / 64)];
+  __shared__ double br[3];
+  ms_bracket(a, passes, levels, sh, br);
+  if (threadIdx.x == 0) {
+    const double mu = 0.5 * (br[0] + br[1]);
     a.state[MS_MU] = mu;
     if (store == 2) a.state[MS_MURAW] = mu;
   }
@@ -974,27 +982,26 @@ int ms_bisection_launch(const double* eps, const double* w, size_t nk, int nb, d
   size_t g = (n + MS_BLOCK - 1) / MS_BLOCK;
   if (g > (size_t)MS_MAX_BLOCKS) g = MS_MAX_BLOCKS;
   if (g < 1) g = 1;
-  const int nblk = (int)g;
-  MsArgs a{eps, w, n, nb, T, target, level_shift, 0, scratch + MS_OFF, scratch + MS_PART_OFF};
+  MsArgs a{eps, w, n, nb, (int)g, T, target, level_shift, 0,
+           scratch + MS_OFF, scratch + MS_MM_OFF, scratch + MS_PART_OFF};
   const int phases = level_shift != 0.0 ? 2 : 1;
   for (int ph = 0; ph < phases; ++ph) {
     a.shift = ph;
     hipLaunchKernelGGL(ms_minmax_part, dim3(g), dim3(MS_BLOCK), 0, s, a);
     if (int rc = check_launch("ms_minmax_part")) return rc;
-    hipLaunchKernelGGL(ms_minmax_fin, dim3(1), dim3(256), 0, s, a, nblk);
-    if (int rc = check_launch("ms_minmax_fin")) return rc;
-    const int store = ph == 0 && phases == 2 ? 2 : 1;
-    int left = maxiter > 0 ? maxiter : 0;
-    do {
+    int left = maxiter > 0 ? maxiter : 0, pass = 0, prev = 0;
+    while (left > 0) {
       const int L = left < MS_LEVELS ? left : MS_LEVELS;
+      if (L == 3) hipLaunchKernelGGL(ms_count_pass<7>, dim3(g), dim3(MS_BLOCK), 0, s, a, pass, prev);
+      if (L == 2) hipLaunchKernelGGL(ms_count_pass<3>, dim3(g), dim3(MS_BLOCK), 0, s, a, pass, prev);
+      if (L == 1) hipLaunchKernelGGL(ms_count_pass<1>, dim3(g), dim3(MS_BLOCK), 0, s, a, pass, prev);
+      if (int rc = check_launch("ms_count_pass")) return rc;
       left -= L;
-      if (L > 0) {
-        hipLaunchKernelGGL(ms_count_part, dim3(g), dim3(MS_BLOCK), 0, s, a, (1 << L) - 1);
-        if (int rc = check_launch("ms_count_part")) return rc;
-      }
-      hipLaunchKernelGGL(ms_count_fin, dim3(1), dim3(256), 0, s, a, nblk, L, left == 0 ? store : 0);
-      if (int rc = check_launch("ms_count_fin")) return rc;
-    } while (left > 0);
+      prev = L;
+      ++pass;
+    }
+    hipLaunchKernelGGL(ms_finish, dim3(1), dim3(MS_BLOCK), 0, s, a, pass, prev, ph == 0 && phases == 2 ? 2 : 1);
+    if (int rc = check_launch("ms_finish")) return rc;
   }
   if (!st && !mu_out && !p_out) return HFX_OK;
   hipLaunchKernelGGL(ms_out, dim3(g), dim3(MS_BLOCK), 0, s, a, st, mu_out, p_out);
