@@ -420,6 +420,12 @@ int hfx_solve_mu(const double* eps, const double* w, size_t nk, int nb, double t
                  int mu0_from_state, int maxiter, int mode, double level_shift, double* st, double* mu_out,
                  double* p_out, double* scratch, void* stream);
 
+/* QR retraction (_qr_retract, solver.py:108-119): U(k) = Q of
+ * I - tau[k] G(k) with the phases normalised so that R has a positive real
+ * diagonal (the unique such Q); G and U (nk, nb, nb), U may alias G, tau (nk)
+ * float64.  nb <= 16. */
+int hfx_qr_retract(int dtype, int nb, size_t nk, const void* G, const double* tau, void* U, void* stream);
+
 /* No counterpart in the reference.  hfx_solve_mu without the scratch argument,
  * for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc): the scratch

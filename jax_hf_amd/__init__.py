@@ -4,8 +4,8 @@ Drop-in for the ``jax_hf`` hot path (``src/jax_hf/__init__.py:32-95``):
 ``HartreeFockKernel``, ``build_fock``, ``hf_energy``, ``free_energy``,
 ``occupation_entropy``, ``solve`` / ``solve_direct_minimization``,
 ``solve_scf``, the drivers ``solve_continuation`` / ``resample_kgrid`` and
-``solve_deflated``, the superlattice Fock plug-in factories, and their config /
-result types (the export list of ``src/jax_hf/__init__.py:32-95``).  Arrays are torch tensors on a
+``solve_deflated``, the superlattice Fock plug-in factories, the npz case
+readers / writers (``cases``), and their config / result types (the export list of ``src/jax_hf/__init__.py:32-95``).  Arrays are torch tensors on a
 ROCm device; all arithmetic runs in the hand-written HIP kernels of
 ``libhfx.so`` (``jax_hf_amd/csrc``), called through ctypes.
 
@@ -50,4 +50,5 @@ _extend([
     ("deflation", ["DeflatedResult", "solve_deflated"]),
     ("superlattice", ["ExtendedGridLayout", "build_extended_layout", "make_superlattice_fock_fn",
                       "make_superlattice_build_fock_fn"]),
+    ("cases", ["RegressionCase", "load_case", "save_case", "case_from_scf", "save_scan", "load_scan"]),
 ])

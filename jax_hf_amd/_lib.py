@@ -21,7 +21,7 @@ LIB_PATH = _HERE / "libhfx.so"
 
 HFX_C128 = 0
 HFX_C64 = 1
-ABI_VERSION = 12
+ABI_VERSION = 13
 
 _lib = None
 _lock = threading.Lock()
@@ -103,6 +103,7 @@ _SIGS = {
     "hfx_reduce": [vp, i32, sz, vp, C.c_uint, vp, vp, i32, vp, vp, vp],
     "hfx_solve_mu_steps": [vp, vp, sz, i32, f64, f64, f64, i32, i32, vp, vp, vp, vp],
     "hfx_solve_mu": [vp, vp, sz, i32, f64, f64, f64, i32, i32, i32, f64, vp, vp, vp, vp, vp],
+    "hfx_qr_retract": [i32, i32, sz, vp, vp, vp, vp],
     "hfx_noalloc_scratch_doubles": [],
     "hfx_solve_mu_noalloc": [vp, vp, sz, i32, f64, f64, f64, i32, i32, i32, f64, vp, vp, vp, vp],
 }

@@ -147,6 +147,33 @@ class _Fock:
             L.call("hfx_convert", L.ptr(E), 0, E_ptr, 0, 1, L.stream(P.device))
 
 
+def _qr_retract(G, tau):
+    """U = QR(I - tau G) with the R-diagonal phases normalised (solver.py:108-119).
+
+    ``G`` (..., nb, nb) complex128 / complex64 device tensor (other array-likes
+    are uploaded), ``tau`` a scalar or an array broadcastable to ``G.shape[:-2]``.
+    Returns the unique Q whose R has a positive real diagonal, computed by one
+    launch of ``hfx_qr_retract`` (Gram-Schmidt with re-orthogonalisation, one
+    group of lanes per matrix).
+    """
+    Gt = to_tensor(G)
+    if not Gt.is_complex():
+        Gt = convert(Gt, torch.complex128 if Gt.dtype == torch.float64 else torch.complex64)
+    if Gt.dim() < 2 or Gt.shape[-1] != Gt.shape[-2]:
+        raise ValueError(f"G must have shape (..., nb, nb); got {tuple(Gt.shape)}")
+    batch = tuple(Gt.shape[:-2])
+    nb = int(Gt.shape[-1])
+    tau_np = np.broadcast_to(np.asarray(host_array(tau), dtype=np.float64), batch)
+    tt = torch.from_numpy(np.array(tau_np, dtype=np.float64).reshape(-1)).to(Gt.device)
+    U = torch.empty_like(Gt)
+    nk = int(np.prod(batch)) if batch else 1
+    L.call("hfx_qr_retract", K.dt_code(Gt), nb, nk, L.ptr(Gt), L.ptr(tt), L.ptr(U), L.stream(Gt.device))
+    return U
+
+
+qr_retract = _qr_retract
+
+
 def _project_herm(P, project_fn, tmp):
     """P <- herm(project_fn(P))  (solver.py:462)."""
     if project_fn is None:

@@ -644,3 +644,30 @@ def test_polish_budget_spent_returns_last_iterate(dev):
     assert np.abs(_np(F) - _np(r.fock)).max() < 1e-12
     E = float(J.hf_energy(r.density, h=G.h, Sigma=S, H=H, weights_b=G.weights_b))
     assert abs(E - float(r.energy)) < 1e-12
+
+
+@pytest.mark.parametrize("nb", [1, 2, 3, 4, 5, 8, 13, 16])
+@pytest.mark.parametrize("dt", ["c128", "c64"])
+def test_qr_retract_matches_oracle(dev, nb, dt):
+    """_qr_retract (solver.py:108-119) on the GPU against the oracle's
+    numpy.linalg.qr with the R-diagonal phase fix: general and skew-Hermitian
+    G, per-k tau including 0; unitary to rounding."""
+    import torch
+    from jax_hf_amd.solver import _qr_retract
+    rng = np.random.default_rng(nb)
+    G = rng.standard_normal((5, 7, nb, nb)) + 1j * rng.standard_normal((5, 7, nb, nb))
+    tdt, tol = (torch.complex128, 1e-12) if dt == "c128" else (torch.complex64, 3e-5)
+    for Gx in (0.3 * G, 0.5 * (G - np.conj(np.swapaxes(G, -1, -2)))):
+        if dt == "c64":
+            Gx = Gx.astype(np.complex64).astype(np.complex128)
+        tau = rng.uniform(0.0, 1.2, (5, 7))
+        tau[0, 0] = 0.0
+        U = _np(_qr_retract(torch.from_numpy(Gx).to(dev).to(tdt), tau)).astype(np.complex128)
+        U0 = O.qr_retract(Gx, tau)
+        assert np.abs(U - U0).max() < tol * (1 + np.abs(tau).max() * np.abs(Gx).max())
+        np.testing.assert_allclose(np.conj(np.swapaxes(U, -1, -2)) @ U,
+                                   np.broadcast_to(np.eye(nb), U.shape), atol=tol)
+        np.testing.assert_allclose(U[0, 0], np.eye(nb), atol=tol)
+    # a scalar tau broadcasts over k
+    Us = _np(_qr_retract(torch.from_numpy(0.2 * G).to(dev).to(tdt), 0.7)).astype(np.complex128)
+    assert np.abs(Us - O.qr_retract(0.2 * G, np.full((5, 7), 0.7))).max() < 10 * tol

@@ -588,3 +588,21 @@ def test_trust_radius_clips_the_step_to_the_radius():
         n = np.sqrt(np.sum(K.w2d * np.sum(np.abs(s) ** 2, axis=(-2, -1))) / np.sum(K.w2d))
         np.testing.assert_allclose(n, min(r, n_free), rtol=1e-12)
         np.testing.assert_allclose(s, free * (n / n_free), atol=1e-13)
+
+
+@pytest.mark.parametrize("nb", [1, 3, 8])
+def test_qr_retract_has_positive_real_r_diagonal(nb):
+    """solver.py:108-119: after the phase normalisation, U^H (I - tau G) is
+    upper triangular with a positive real diagonal (the unique QR factor), for
+    general and skew-Hermitian G and per-k tau."""
+    rng = np.random.default_rng(nb)
+    G = rng.standard_normal((2, 3, nb, nb)) + 1j * rng.standard_normal((2, 3, nb, nb))
+    for Gx in (G, OH._skew(G)):
+        tau = rng.uniform(0.0, 1.5, (2, 3))
+        U = O.qr_retract(Gx, tau)
+        A = np.eye(nb) - tau[..., None, None] * Gx
+        R = np.conj(np.swapaxes(U, -1, -2)) @ A
+        np.testing.assert_allclose(np.tril(R, -1), 0, atol=1e-12)
+        d = np.diagonal(R, axis1=-2, axis2=-1)
+        assert np.all(np.real(d) > 0) and np.abs(np.imag(d)).max() < 1e-12
+        np.testing.assert_allclose(U @ R, A, atol=1e-12)
