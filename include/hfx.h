@@ -40,13 +40,18 @@ int hfx_plan_destroy(hfx_plan* plan);
 /* Exchange-kernel variants for the compile-time square meshes (12 .. 96):
  * HFX_EXCH_V1 runs one Stockham stage per prime-power radix; HFX_EXCH_V2_A/B/C
  * run two composite-radix stages per axis with workgroups of 1024/768/512
- * (large meshes) threads.  HFX_EXCH_AUTO is the built-in per-mesh default.
+ * (large meshes) threads; HFX_EXCH_V3_A/B/C (slot planes only, else the V2 of
+ * the same size) fuse the VR multiply with the last forward and the first
+ * inverse column stage.  HFX_EXCH_AUTO is the built-in per-mesh default.
  * Other meshes always use the runtime-radix kernel. */
 #define HFX_EXCH_AUTO 0
 #define HFX_EXCH_V1 1
 #define HFX_EXCH_V2_A 2
 #define HFX_EXCH_V2_B 3
 #define HFX_EXCH_V2_C 4
+#define HFX_EXCH_V3_A 5
+#define HFX_EXCH_V3_B 6
+#define HFX_EXCH_V3_C 7
 
 /* Selects the exchange-kernel variant used by every later call on this plan.
  * No counterpart in the reference (XLA chooses its own FFT plan). */
@@ -56,8 +61,8 @@ int hfx_plan_get_exchange_variant(const hfx_plan* plan);
 /* Times every exchange variant (reps launches each, all nb*nb channels, on the
  * caller's buffers; Sigma is overwritten) and keeps the fastest on the plan.
  * planar: P / Sigma are slot planes and VR channel-major (hfx_selfenergy_planar).
- * times_ms (optional, 5 doubles) receives the mean time per launch of variants
- * 0 .. 4 (0 = AUTO's choice is not timed and is reported as 0).  Synchronises
+ * times_ms (optional, 8 doubles) receives the mean time per launch of variants
+ * 0 .. 7 (0 = AUTO's choice is not timed and is reported as 0).  Synchronises
  * the stream.  No counterpart in the reference (XLA autotunes its own FFTs). */
 int hfx_exchange_autotune(hfx_plan* plan, const void* P, const void* VR, int vr_full, int planar, void* Sigma,
                           int reps, double* times_ms, int* best, void* stream);

@@ -191,7 +191,7 @@ class Plan:
         nbytes = lib().hfx_energy_scratch_bytes(h)
         self.scratch = torch.zeros(nbytes // 8, dtype=torch.float64, device=device)
 
-    VARIANTS = {0: "auto", 1: "v1", 2: "v2_a", 3: "v2_b", 4: "v2_c"}
+    VARIANTS = {0: "auto", 1: "v1", 2: "v2_a", 3: "v2_b", 4: "v2_c", 5: "v3_a", 6: "v3_b", 7: "v3_c"}
 
     @property
     def exchange_variant(self) -> int:
@@ -207,7 +207,7 @@ class Plan:
         ``planar``: P / Sigma are nb^2 slot planes and VR the channel-major VR
         (or the scalar VR).  Returns (best variant, {name: ms per launch}).
         """
-        t = (C.c_double * 5)()
+        t = (C.c_double * 8)()
         best = C.c_int()
         if planar:
             vfull = 1 if VR.numel() > P.numel() // (self.nb * self.nb) else 0
@@ -216,7 +216,7 @@ class Plan:
         with torch.cuda.device(self.device):
             call("hfx_exchange_autotune", self.handle, ptr(P), ptr(VR), vfull, int(planar), ptr(Sigma), int(reps), t,
                  C.byref(best), stream(self.device))
-        return best.value, {self.VARIANTS[v]: t[v] for v in range(1, 5)}
+        return best.value, {self.VARIANTS[v]: t[v] for v in range(1, 8)}
 
     def destroy(self):
         """Free the plan's device tables now (idempotent)."""

@@ -251,7 +251,7 @@ int hfx_fft2_weighted(const hfx_plan* plan, const void* X, const double* w, void
 
 int hfx_plan_set_exchange_variant(hfx_plan* plan, int variant) {
   HFX_REQUIRE(plan, "hfx_plan_set_exchange_variant: NULL plan");
-  HFX_REQUIRE(variant >= HFX_EXCH_AUTO && variant <= HFX_EXCH_V2_C,
+  HFX_REQUIRE(variant >= HFX_EXCH_AUTO && variant <= HFX_EXCH_V3_C,
               "hfx_plan_set_exchange_variant: variant %d out of range", variant);
   plan->exch_variant = variant;
   return HFX_OK;
@@ -274,7 +274,7 @@ int hfx_exchange_autotune(hfx_plan* plan, const void* P, const void* VR, int vr_
   int rc = HFX_OK, arg = HFX_EXCH_V1;
   double tbest = 1e300;
   if (times_ms) times_ms[0] = 0.0;
-  for (int v = HFX_EXCH_V1; v <= HFX_EXCH_V2_C && rc == HFX_OK; ++v) {
+  for (int v = HFX_EXCH_V1; v <= HFX_EXCH_V3_C && rc == HFX_OK; ++v) {
     plan->exch_variant = v;
     const int nch = plan->nb * plan->nb;
     for (int r = -1; r < reps && rc == HFX_OK; ++r) {  // r = -1: warm-up launch

@@ -432,6 +432,44 @@ constexpr int v2_split(int R) {
   return R == 6 ? 2 : R == 9 ? 3 : R == 10 ? 2 : R == 12 ? 3 : R == 15 ? 3 : R == 16 ? 4 : 0;
 }
 
+// W_R^m = exp(-+ 2 pi i m / R) for the radices that dft_any splits, as
+// compile-time constants (their indices are constants once the split loops
+// unroll): no table reads for the butterflies' internal twiddles.
+template <typename T, int R>
+__device__ __forceinline__ cx<T> croot(int m, bool inv) {
+  if constexpr (R == 6) {
+    constexpr double c[6] = {1.0, 0.5, -0.5, -1.0, -0.5, 0.5};
+    constexpr double s[6] = {0.0, 0.8660254037844386467637, 0.8660254037844386467637, 0.0, -0.8660254037844386467637, -0.8660254037844386467637};
+    return cx<T>{(T)c[m], (T)(inv ? s[m] : -s[m])};
+  }
+  if constexpr (R == 9) {
+    constexpr double c[9] = {1.0, 0.7660444431189780352024, 0.1736481776669303488517, -0.5, -0.9396926207859083840541, -0.9396926207859083840541, -0.5, 0.1736481776669303488517, 0.7660444431189780352024};
+    constexpr double s[9] = {0.0, 0.6427876096865393263226, 0.9848077530122080593667, 0.8660254037844386467637, 0.3420201433256687330441, -0.3420201433256687330441, -0.8660254037844386467637, -0.9848077530122080593667, -0.6427876096865393263226};
+    return cx<T>{(T)c[m], (T)(inv ? s[m] : -s[m])};
+  }
+  if constexpr (R == 10) {
+    constexpr double c[10] = {1.0, 0.8090169943749474241023, 0.3090169943749474241023, -0.3090169943749474241023, -0.8090169943749474241023, -1.0, -0.8090169943749474241023, -0.3090169943749474241023, 0.3090169943749474241023, 0.8090169943749474241023};
+    constexpr double s[10] = {0.0, 0.5877852522924731291687, 0.9510565162951535721164, 0.9510565162951535721164, 0.5877852522924731291687, 0.0, -0.5877852522924731291687, -0.9510565162951535721164, -0.9510565162951535721164, -0.5877852522924731291687};
+    return cx<T>{(T)c[m], (T)(inv ? s[m] : -s[m])};
+  }
+  if constexpr (R == 12) {
+    constexpr double c[12] = {1.0, 0.8660254037844386467637, 0.5, 0.0, -0.5, -0.8660254037844386467637, -1.0, -0.8660254037844386467637, -0.5, 0.0, 0.5, 0.8660254037844386467637};
+    constexpr double s[12] = {0.0, 0.5, 0.8660254037844386467637, 1.0, 0.8660254037844386467637, 0.5, 0.0, -0.5, -0.8660254037844386467637, -1.0, -0.8660254037844386467637, -0.5};
+    return cx<T>{(T)c[m], (T)(inv ? s[m] : -s[m])};
+  }
+  if constexpr (R == 15) {
+    constexpr double c[15] = {1.0, 0.9135454576426008955021, 0.6691306063588582138263, 0.3090169943749474241023, -0.1045284632676534713998, -0.5, -0.8090169943749474241023, -0.9781476007338056379286, -0.9781476007338056379286, -0.8090169943749474241023, -0.5, -0.1045284632676534713998, 0.3090169943749474241023, 0.6691306063588582138263, 0.9135454576426008955021};
+    constexpr double s[15] = {0.0, 0.406736643075800207754, 0.7431448254773942350147, 0.9510565162951535721164, 0.9945218953682733369227, 0.8660254037844386467637, 0.5877852522924731291687, 0.2079116908177593371017, -0.2079116908177593371017, -0.5877852522924731291687, -0.8660254037844386467637, -0.9945218953682733369227, -0.9510565162951535721164, -0.7431448254773942350147, -0.406736643075800207754};
+    return cx<T>{(T)c[m], (T)(inv ? s[m] : -s[m])};
+  }
+  if constexpr (R == 16) {
+    constexpr double c[16] = {1.0, 0.9238795325112867561282, 0.7071067811865475244008, 0.3826834323650897717285, 0.0, -0.3826834323650897717285, -0.7071067811865475244008, -0.9238795325112867561282, -1.0, -0.9238795325112867561282, -0.7071067811865475244008, -0.3826834323650897717285, 0.0, 0.3826834323650897717285, 0.7071067811865475244008, 0.9238795325112867561282};
+    constexpr double s[16] = {0.0, 0.3826834323650897717285, 0.7071067811865475244008, 0.9238795325112867561282, 1.0, 0.9238795325112867561282, 0.7071067811865475244008, 0.3826834323650897717285, 0.0, -0.3826834323650897717285, -0.7071067811865475244008, -0.9238795325112867561282, -1.0, -0.9238795325112867561282, -0.7071067811865475244008, -0.3826834323650897717285};
+    return cx<T>{(T)c[m], (T)(inv ? s[m] : -s[m])};
+  }
+  return cx<T>{(T)1, (T)0};
+}
+
 template <typename T, int R, bool INV>
 __device__ __forceinline__ void dft_any(cx<T> (&v)[R], const cx<T>* tw, int NT) {
   constexpr int A = v2_split(R);
@@ -449,10 +487,7 @@ __device__ __forceinline__ void dft_any(cx<T> (&v)[R], const cx<T>* tw, int NT) 
 #pragma unroll
       for (int q = 0; q < A; ++q) {
         cx<T> x = t[q];
-        if (b > 0 && q > 0) {
-          const cx<T> w = tw[((b * q) % R) * (NT / R)];
-          x = INV ? mulc(x, w) : x * w;
-        }
+        if (b > 0 && q > 0) x = x * croot<T, R>((b * q) % R, INV);
         y[b][q] = x;
       }
     }
@@ -600,25 +635,136 @@ __global__ __launch_bounds__(BLOCK) void exch_v2_kernel(ExchArgs<T> a) {
   }
 }
 
+// ---- v3 (slot planes): the VR multiply between a forward and an inverse column
+// stage of the same radix ---------------------------------------------------
+// The forward column pass runs its radices in the order (A, B) and the inverse
+// pass in the order (B, A).  The last forward stage (radix B, stride A) leaves
+// each thread holding the outputs j + A r (r < B) of its column, in natural
+// order; the first inverse stage (radix B, stride 1) reads exactly the inputs
+// j + A r.  So one stage does the forward radix-B butterflies, the VR multiply
+// and the inverse radix-B butterflies in registers: one LDS round trip and one
+// barrier fewer than v2 (six in all).  The fused stage's VR values are loaded
+// right after the first row stage, as global loads: the barriers wait only for
+// LDS (lgkmcnt), so that HBM read overlaps the second row and the first column
+// stages.
+template <typename T>
+__device__ __forceinline__ cx<T> ld_global(const cx<T>* p, size_t i) {
+  typedef T v2t __attribute__((ext_vector_type(2)));
+  const v2t x = ((const __attribute__((address_space(1))) v2t*)(p + i))[0];
+  return cx<T>{x.x, x.y};
+}
+
+template <typename T, int R, int N, int BLOCK, int A, int MAXIT>
+__device__ __forceinline__ void v3_fused_cols(cx<T>* plane, const cx<T>* tw, const cx<T> (&vr)[MAXIT][R],
+                                              T scale, bool conjv) {
+  constexpr int PITCH = lds_pitch(N), ITEMS = N * A;
+  static_assert(A * R == N, "column radices");
+  cx<T> v[MAXIT][R];
+#pragma unroll
+  for (int m = 0; m < MAXIT; ++m) {
+    const int it = threadIdx.x + m * BLOCK;
+    if (ITEMS % BLOCK == 0 || it < ITEMS) {
+      const int line = it % N, j = it / N;   // column k2 = line, butterfly j < A
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        cx<T> x = plane[line + (j + r * A) * PITCH];
+        if (r > 0) x = x * tw[r * j];          // forward stride-A stage: W_N^{r j}
+        v[m][r] = x;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < MAXIT; ++m) {
+    const int it = threadIdx.x + m * BLOCK;
+    if (ITEMS % BLOCK == 0 || it < ITEMS) {
+      const int line = it % N, j = it / N;
+      dft_any<T, R, false>(v[m], tw, N);        // outputs at k1 = j + A r
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[m][r] = scale * (conjv ? mulc(v[m][r], vr[m][r]) : v[m][r] * vr[m][r]);
+      dft_any<T, R, true>(v[m], tw, N);         // inverse stride-1 stage: outputs at j R + r
+#pragma unroll
+      for (int r = 0; r < R; ++r) plane[line + (j * R + r) * PITCH] = v[m][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int N, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void exch_v3_kernel(ExchArgs<T> a) {
+  constexpr int PITCH = lds_pitch(N);
+  constexpr int RA2 = v2_row_a(N), RB2 = N / RA2;   // row pass
+  constexpr int RA1 = v2_col_a(N), RB1 = N / RA1;   // column pass: forward (RA1, RB1), inverse (RB1, RA1)
+  constexpr int FITEMS = N * RA1, FMAX = (FITEMS + BLOCK - 1) / BLOCK;
+  __shared__ cx<T> plane[N * PITCH];
+  __shared__ cx<T> tw[N];
+  __shared__ double red[2 * (BLOCK / 64)];
+  const int c = exch_channel<T, BLOCK>(a);
+  for (int m = threadIdx.x; m < N; m += BLOCK) tw[m] = a.g.tw1[m];
+  double mre = 0.0, mim = 0.0;
+  __syncthreads();
+  // forward rows: lines = k1 rows, positions along k2; the first stage loads from HBM
+  v2_stage<T, RA2, N, N, PITCH, 1, BLOCK, 1, false, 1, 0, false, N, true>(plane, tw, a, c, mre, mim);
+  // VR of the fused column stage, loaded after P (loads retire in order, so a
+  // wait for P would also wait for anything issued before it)
+  cx<T> vr[FMAX][RB1];
+#pragma unroll
+  for (int m = 0; m < FMAX; ++m) {
+    const int it = threadIdx.x + m * BLOCK;
+    if (FITEMS % BLOCK == 0 || it < FITEMS) {
+      const int line = it % N, j = it / N;
+#pragma unroll
+      for (int r = 0; r < RB1; ++r) vr[m][r] = ld_global(a.VR, (size_t)(j + RA1 * r) * N + line);
+    }
+  }
+  v2_stage<T, RB2, N, N, PITCH, 1, BLOCK, RA2, false, 0, 0, false, N, true>(plane, tw, a, c, mre, mim);
+  // columns: forward radix RA1, then [forward RB1, x VR, inverse RB1], then inverse RA1
+  v2_stage<T, RA1, N, N, 1, PITCH, BLOCK, 1, false, 0, 0, false, N, true>(plane, tw, a, c, mre, mim);
+  v3_fused_cols<T, RB1, N, BLOCK, RA1, FMAX>(plane, tw, vr, (T)(-1.0 / ((double)N * N)), a.conjv != 0);
+  v2_stage<T, RA1, N, N, 1, PITCH, BLOCK, RB1, true, 0, 0, false, N, true>(plane, tw, a, c, mre, mim);
+  // inverse rows, last stage straight to HBM
+  v2_stage<T, RA2, N, N, PITCH, 1, BLOCK, 1, true, 0, 0, false, N, true>(plane, tw, a, c, mre, mim);
+  v2_stage<T, RB2, N, N, PITCH, 1, BLOCK, RA2, true, 0, 1, false, N, true>(plane, tw, a, c, mre, mim);
+  if (a.rho) {
+    double acc[2] = {mre, mim};
+    block_sum<2>(acc, red);
+    if (threadIdx.x == 0) {
+      const int nb = a.g.nb, ci = c / nb, cj = c - ci * nb;
+      a.rho[c] = cx<double>{acc[0], acc[1]};
+      if (a.herm && ci != cj) a.rho[cj * nb + ci] = cx<double>{acc[0], -acc[1]};
+    }
+  }
+}
+
 template <int N>
 constexpr int ct_block() { return N * N >= 4096 ? 1024 : (N * N >= 1024 ? 512 : 256); }
 
 #define HFX_CT_SIZES(X) X(12) X(16) X(24) X(32) X(36) X(48) X(60) X(64) X(72) X(90) X(96)
 
-// Workgroup sizes of the two-stage kernel, per variant (HFX_EXCH_V2_*).
+// Workgroup sizes of the two-stage kernels, per variant (HFX_EXCH_V2_* / V3_*).
 template <int N>
 constexpr int v2_block(int variant) {
   constexpr int big[3] = {1024, 768, 512}, mid[3] = {512, 384, 256}, small[3] = {256, 192, 128};
-  const int i = variant == HFX_EXCH_V2_B ? 1 : variant == HFX_EXCH_V2_C ? 2 : 0;
+  const int i = (variant == HFX_EXCH_V2_B || variant == HFX_EXCH_V3_B) ? 1
+              : (variant == HFX_EXCH_V2_C || variant == HFX_EXCH_V3_C) ? 2 : 0;
   return N * N >= 4096 ? big[i] : (N * N >= 1024 ? mid[i] : small[i]);
 }
 
-// Default variant per mesh (used when the plan's variant is HFX_EXCH_AUTO).
-constexpr int v2_default(int N) { return N == 96 ? HFX_EXCH_V2_B : HFX_EXCH_V2_A; }
+// Default variant per mesh (used when the plan's variant is HFX_EXCH_AUTO): at
+// 96 x 96 the fused-column form on slot planes (29.7 vs 31.1 us for 256 FP64
+// channels); other meshes keep the two-stage kernel until timed
+// (hfx_exchange_autotune picks per plan).
+constexpr int v2_default(int N, bool planar) {
+  return N == 96 ? (planar ? HFX_EXCH_V3_B : HFX_EXCH_V2_B) : HFX_EXCH_V2_A;
+}
 
 template <typename T, int N, int V>
 static int launch_v2(const ExchArgs<T>& a, hipStream_t s) {
   constexpr int B = v2_block<N>(V);
+  if (a.planar && V >= HFX_EXCH_V3_A) {
+    hipLaunchKernelGGL((exch_v3_kernel<T, N, B>), dim3(a.nchan * a.nsolve), dim3(B), 0, s, a);
+    return check_launch("exch_v3_kernel");
+  }
   if (a.planar)
     hipLaunchKernelGGL((exch_v2_kernel<T, N, N, B, true>), dim3(a.nchan * a.nsolve), dim3(B), 0, s, a);
   else
@@ -635,10 +781,13 @@ static int launch_exch(const ExchArgs<T>& a, int variant, hipStream_t s) {
 #define HFX_CASE(N)                                                                        \
   case N:                                                                                  \
     if ((size_t)N * lds_pitch(N) * sizeof(cx<T>) <= (size_t)MAXE * sizeof(cx<T>)) {        \
-      const int v = variant == HFX_EXCH_AUTO ? v2_default(N) : variant;                    \
+      const int v = variant == HFX_EXCH_AUTO ? v2_default(N, a.planar != 0) : variant;     \
       if (a.mode == 0 && a.shift == 0 && v != HFX_EXCH_V1) {                               \
         if (v == HFX_EXCH_V2_B) return launch_v2<T, N, HFX_EXCH_V2_B>(a, s);                \
         if (v == HFX_EXCH_V2_C) return launch_v2<T, N, HFX_EXCH_V2_C>(a, s);                \
+        if (v == HFX_EXCH_V3_A) return launch_v2<T, N, HFX_EXCH_V3_A>(a, s);                \
+        if (v == HFX_EXCH_V3_B) return launch_v2<T, N, HFX_EXCH_V3_B>(a, s);                \
+        if (v == HFX_EXCH_V3_C) return launch_v2<T, N, HFX_EXCH_V3_C>(a, s);                \
         return launch_v2<T, N, HFX_EXCH_V2_A>(a, s);                                       \
       }                                                                                    \
       hipLaunchKernelGGL((exch_ct_kernel<T, N, N, ct_block<N>()>), dim3(a.nchan * a.nsolve),           \

@@ -199,7 +199,8 @@ def test_entropy_and_free_energy(dev):
 @pytest.mark.parametrize("nk", [12, 16, 24, 32, 36, 48, 60, 64, 72, 90, 96])
 def test_exchange_variants_match_oracle(dev, nk, cdt):
     """Every exchange-kernel variant (one-stage-per-radix, two-stage at three
-    workgroup sizes) gives the oracle's Sigma and channel moments, with refP,
+    workgroup sizes, the fused-column two-stage form, which falls back to the
+    two-stage kernel in this layout) gives the oracle's Sigma and channel moments, with refP,
     full and Hermitian-packed channel lists, and the conjugated-VR adjoint."""
     import torch
     import jax_hf_amd as J
@@ -223,7 +224,7 @@ def test_exchange_variants_match_oracle(dev, nk, cdt):
     Rt = torch.from_numpy(R.astype(npd)).to(G.device)
     plan = eng.plan
     try:
-        for v in (1, 2, 3, 4):
+        for v in range(1, 8):
             plan.exchange_variant = v
             for herm in (False, True):
                 S = torch.full_like(Pt, float("nan"))
@@ -238,7 +239,7 @@ def test_exchange_variants_match_oracle(dev, nk, cdt):
             assert _rel(_np(A), A_ref) < tol, v
         S = torch.empty_like(Pt)
         best, times = plan.autotune(Pt, eng.VR, S, reps=2)
-        assert best in (1, 2, 3, 4) and plan.exchange_variant == best
+        assert 1 <= best <= 7 and plan.exchange_variant == best
         assert all(t > 0 for t in times.values()), times
     finally:
         plan.exchange_variant = 0
@@ -290,7 +291,7 @@ def test_planar_exchange_matches_oracle(dev, nk, cdt):
                 L.call("hfx_planar_pack", plan.handle, 1, L.ptr(Pt), L.ptr(Rt), L.ptr(ct), nch, herm, L.ptr(X), st)
                 D = (Pt - Rt).reshape(nk, nk, nb * nb)
                 assert torch.equal(X, D[..., cl].permute(2, 0, 1)), (herm, chans)
-                for v in (1, 2, 3, 4):
+                for v in range(1, 8):
                     plan.exchange_variant = v
                     S = torch.full_like(X, float("nan"))
                     eng.rho.zero_()
