@@ -84,6 +84,15 @@ def _pmc_traffic(kname, nk, nb):
     return None
 
 
+def _exchange_kernel_name(v, planar, nk):
+    """The kernel the plan's exchange variant launches on a square nk mesh."""
+    if v == 0:      # the plan's built-in default (v2_default in hfx_fft.hip)
+        v = (6 if planar else 3) if nk == 96 else 2
+    if v == 1:
+        return "exch_ct_kernel"
+    return "exch_v3_kernel" if (planar and v >= 5) else "exch_v2_kernel"
+
+
 def fock_bench(J, L, syn, inp, dev, nk, nb, args):
     import torch
     G = J.HartreeFockKernel(inp["weights"], inp["hamiltonian"], inp["coulomb_q"], inp["T"], device=dev)
@@ -106,7 +115,7 @@ def fock_bench(J, L, syn, inp, dev, nk, nb, args):
             _, variant_ms = eng.plan.autotune(Pt, VRc, St, reps=10, planar=True)
         else:
             _, variant_ms = eng.plan.autotune(P, eng.VR, Sig, reps=10)
-    kname = {1: "exch_ct_kernel"}.get(eng.plan.exchange_variant, "exch_v2_kernel")
+    kname = _exchange_kernel_name(eng.plan.exchange_variant, planar, nk)
     Nk = nk * nk
     bytes_exch = 16 * Nk * (3 * nb * nb)               # P + VR in, Sigma out
     bytes_build = 16 * Nk * (4 * nb * nb + nb * nb)    # + h in, F out

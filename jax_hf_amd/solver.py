@@ -70,6 +70,10 @@ class SolverConfig:
     polish: str | None = None      # "diis": FP64 DIIS SCF to the fixed point after CG (default for mixed)
     polish_tol: float = 1e-8       # density and commutator residual of the polish
     polish_max_iter: int = 60
+    # mixed form: a complex64 DIIS stage between the FP32 CG and the FP64 polish,
+    # to residuals <= this (or until its rounding floor stops it), so that the
+    # FP64 part is a few refinement builds; 0 skips the stage
+    polish32_tol: float = 0.0
 
 
 class SolveResult(NamedTuple):
@@ -454,8 +458,8 @@ class _DIISPolish:
     stays bounded by one run's updates.
     """
 
-    def __init__(self, kernel, n_electrons, nh=6):
-        cdt = torch.complex128
+    def __init__(self, kernel, n_electrons, nh=6, cdt=torch.complex128):
+        self.cdt = cdt
         self.kernel = kernel
         self.dev = dev = kernel.device
         nk1, nk2, nb = kernel.h.shape[0], kernel.h.shape[1], kernel.h.shape[-1]
@@ -473,8 +477,9 @@ class _DIISPolish:
         self.Ft, self.W = z(), z()
         self.F_hist = [z() for _ in range(nh)]
         # DIIS residual history: complex64 at nb = 16 (R and its norms are formed
-        # in FP64; only the stored copies that feed the B matrix are rounded)
-        rdt = torch.complex64 if (self.HIST32 and nb == 16) else cdt
+        # in FP64; only the stored copies that feed the B matrix are rounded);
+        # the complex64 stage keeps it in its own precision
+        rdt = torch.complex64 if (self.HIST32 and nb == 16) or cdt == torch.complex64 else cdt
         self.R_hist = [torch.zeros(self.shape4, dtype=rdt, device=dev) for _ in range(nh)]
         self.eps = torch.zeros((nk1, nk2, nb), dtype=torch.float64, device=dev)
         self.occ = torch.zeros_like(self.eps)
@@ -574,7 +579,7 @@ class _DIISPolish:
             self._update(self.P_new, warm=False)
         self.P_prev, self.P_cur, self.P_new = self.P_cur, self.P_new, self.P_prev
 
-    def run(self, P_lo, mu0, tol, max_iter, Q_lo=None):
+    def run(self, P_lo, mu0, tol, max_iter, Q_lo=None, stall=0.0):
         """Q_lo (optional): the orbitals P_lo was built from.  Re-orthonormalised in
         FP64 (two Newton-Schulz steps), they are the first basis, so that the
         first update is a warm eigh as well.
@@ -584,20 +589,27 @@ class _DIISPolish:
         the device), so the host's stopping test overlaps the update instead of
         stalling the GPU.  When the test stops at k, that update is discarded
         (iterate k is kept in P_prev); when the residual trend predicts the
-        stop, the update waits for the test instead."""
+        stop, the update waits for the test instead.
+
+        stall > 0: also stop (not converged) once the commutator residual has
+        not fallen below stall x its smallest value for two iterations (the
+        rounding floor of a complex64 stage)."""
         nh = self.nh
         max_iter = max(int(max_iter), 1)
         et = self.EIGH_TOL_PER_TOL * float(tol)
         if et != self._eigh_tol:          # the captured updates hold the eigh tolerance
             self._eigh_tol, self.graphs = et, {}
-        K.hermitize(convert(P_lo, torch.complex128) if P_lo.dtype != torch.complex128 else P_lo, self.P_cur)
+        K.hermitize(convert(P_lo, self.cdt) if P_lo.dtype != self.cdt else P_lo, self.P_cur)
         self.st[K.ST_MU] = mu0
         warm0 = Q_lo is not None
         if warm0:
             Qc = Q_lo.contiguous()
-            L.call("hfx_convert", L.ptr(Qc), KIND[Qc.dtype], L.ptr(self.U), KIND[torch.complex128], Qc.numel(),
+            L.call("hfx_convert", L.ptr(Qc), KIND[Qc.dtype], L.ptr(self.U), KIND[self.cdt], Qc.numel(),
                    L.stream(self.dev))
-            K.orthonormalize(self.U, 2)
+            # Newton-Schulz to this precision's unitarity: two steps from complex64
+            # orbitals in FP64, one in complex64
+            steps = 2 if self.cdt == torch.complex128 and Qc.dtype != torch.complex128 else 1
+            K.orthonormalize(self.U, steps)
         hE, hC, hD = [], [], []
         k, converged, E, idx, redo = 0, False, float("nan"), 0, -1
         P_res = self.P_cur
@@ -646,6 +658,8 @@ class _DIISPolish:
                 break
             if k + 1 >= max_iter:
                 break
+            if stall > 0.0 and len(hC) >= 4 and min(hC[-2:]) > stall * min(hC[:-2]):
+                break
             if not spec:
                 self._advance(k, warm=k > 0 or warm0)
             k += 1
@@ -671,13 +685,14 @@ class _DIISPolish:
                 converged, hE, hC)
 
 
-def _polish_state(kernel, n_electrons):
+def _polish_state(kernel, n_electrons, cdt=torch.complex128):
     cache = kernel.__dict__.setdefault("_hfx_polish", {})
-    key = float(n_electrons)
+    key = (float(n_electrons), cdt)
     pol = cache.get(key)
     if pol is None:
-        cache.clear()                 # one polish state per kernel (the last electron count)
-        pol = _DIISPolish(kernel, n_electrons)
+        for k in [k for k in cache if k[1] == cdt]:
+            del cache[k]              # one polish state per kernel and precision (the last electron count)
+        pol = _DIISPolish(kernel, n_electrons, cdt=cdt)
         pol.kernel = pol.fock.kernel = weakref.proxy(kernel)
         cache[key] = pol
     return pol
@@ -698,19 +713,33 @@ def _polish_diis(kernel, runner, n_electrons, cfg, fock_build_fn, k_cg, hist_E, 
     if P_lo is None:
         P_lo = runner.density()       # from the orbitals runner.Q, which then start the polish basis
     if fock_build_fn is None and cfg.project_fn is None and cfg.use_graph:
-        pol = _polish_state(kernel, n_electrons)
         mu_start = runner.st[K.ST_MU] if mu0 is None else mu0
         Q_lo = runner.Q if (runner is not None and P_lo_given is False) else None
+        phases = [k_cg]
+        hE, hG = list(hist_E), list(hist_G)
+        stage32 = (_precision(kernel, cfg) == "mixed" and float(cfg.polish32_tol) > float(cfg.polish_tol)
+                   and Q_lo is not None and Q_lo.dtype == torch.complex64)
+        if stage32:
+            # complex64 DIIS from the CG orbitals to its own floor: the FP64 polish
+            # then starts a few refinement builds from the fixed point
+            p32 = _polish_state(kernel, n_electrons, torch.complex64)
+            P_lo, _, Q_lo, _, mu_start, _, n32, _, hE32, hC32 = p32.run(
+                P_lo, mu_start, float(cfg.polish32_tol), cfg.polish_max_iter, Q_lo=Q_lo, stall=0.7)
+            phases.append(n32)
+            hE += list(hE32)
+            hG += list(hC32)
+        pol = _polish_state(kernel, n_electrons)
         density, F, Q, p, mu, E, n2, conv, hE2, hC2 = pol.run(P_lo, mu_start, cfg.polish_tol, cfg.polish_max_iter,
                                                               Q_lo=Q_lo)
-        k = k_cg + n2
-        hE = list(hist_E) + list(hE2)
-        hG = list(hist_G) + list(hC2)
+        phases.append(n2)
+        k = sum(phases)
+        hE += list(hE2)
+        hG += list(hC2)
         hEt = torch.zeros(max(cfg.max_iter, k), dtype=torch.float64)
         hGt = torch.zeros_like(hEt)
         hEt[:k] = torch.tensor(hE[:k], dtype=torch.float64)
         hGt[:k] = torch.tensor(hG[:k], dtype=torch.float64)
-        hist = dict(E=hEt.to(dev), grad_norm=hGt.to(dev), phase_iters=torch.tensor([k_cg, n2], dtype=torch.int64))
+        hist = dict(E=hEt.to(dev), grad_norm=hGt.to(dev), phase_iters=torch.tensor(phases, dtype=torch.int64))
         return SolveResult(Q=Q, p=p, mu=mu, density=density, fock=F,
                            energy=torch.tensor(E, dtype=torch.float64, device=dev), n_iter=int(k),
                            converged=bool(conv), history=hist)

@@ -326,6 +326,27 @@ def test_cg_mixed_precision_energy_error(dev):
     assert r64.converged and abs(float(r64.energy) - r0.energy) < 1e-8
 
 
+@pytest.mark.parametrize("nk,nb", [(24, 4), (24, 16)])
+def test_mixed_complex64_diis_stage(dev, nk, nb):
+    """The optional complex64 DIIS stage between the FP32 CG and the FP64
+    polish (polish32_tol > 0): three phases in the history, the FP64 part a
+    few refinement builds, and the same FP64 fixed point (|dE| < 1e-8, density
+    1e-6) as the oracle's DIIS SCF."""
+    import jax_hf_amd as J
+    c = syn.config0(nk=nk, nb=nb, seed=0)
+    G = J.HartreeFockKernel(c["weights"], c["hamiltonian"], c["coulomb_q"], c["T"])
+    K0 = O.OracleKernel(c["weights"], c["hamiltonian"], c["coulomb_q"], c["T"])
+    r0 = O.solve_scf(K0, c["P0"], c["n_electrons"],
+                     config=O.SCFConfig(max_iter=300, acceleration="diis", density_tol=1e-11, comm_tol=1e-11))
+    r = J.solve(G, c["P0"], c["n_electrons"], config=J.SolverConfig(max_iter=400, precision="mixed",
+                                                                       polish32_tol=1e-4))
+    assert r.converged
+    ph = [int(x) for x in _np(r.history["phase_iters"])]
+    assert len(ph) == 3 and sum(ph) == int(r.n_iter) and ph[1] >= 1
+    assert abs(float(r.energy) - r0.energy) < 1e-8
+    assert np.abs(_np(r.density) - np.asarray(r0.density_matrix)).max() < 1e-6
+
+
 def _contact_kernels(nk=2, g=0.3, T=0.05):
     import jax_hf_amd as J
     h = np.zeros((nk, nk, 2, 2), np.complex128)
