@@ -51,7 +51,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=6.0, help="budget of the CPU Fock-build sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the built-in exchange-kernel variant instead of timing all of them first")
+                    help="keep the built-in exchange-kernel variants (no per-plan timing of the variants)")
     ap.add_argument("--fock-only", action="store_true", help="only the Fock-build measurement")
     ap.add_argument("--no-scan", action="store_true", help="skip the 64-solve density scan")
     ap.add_argument("--scan-solves", type=int, default=64)
@@ -582,6 +582,8 @@ def main():
     import jax_hf_amd as J
     from jax_hf_amd import _lib as L
     from jax_hf_amd import synthetic as syn
+    if args.no_autotune:
+        J.fock.AUTOTUNE = False           # every plan keeps the built-in exchange variant
 
     dev = torch.device("cuda:0")
     nk, nb = args.nk, args.nb

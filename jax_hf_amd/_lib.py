@@ -218,6 +218,25 @@ class Plan:
                  C.byref(best), stream(self.device))
         return best.value, {self.VARIANTS[v]: t[v] for v in range(1, 8)}
 
+    def tune_once(self, VR, vfull: bool, reps: int = 5):
+        """Autotune the exchange variant once per plan, on zero slot planes of
+        all nb^2 channels with this VR (channel-major when vfull).  Skipped
+        while the current stream is capturing; a mesh the slot-plane exchange
+        does not support keeps the built-in default."""
+        if getattr(self, "_tuned", False) or torch.cuda.is_current_stream_capturing():
+            return
+        self._tuned = True
+        cdt = torch.complex128 if self.dtype == HFX_C128 else torch.complex64
+        shape = (self.nb * self.nb if vfull else 1, self.nk1, self.nk2)
+        if tuple(VR.shape) != shape and VR.numel() != shape[0] * shape[1] * shape[2]:
+            raise ValueError(f"tune_once: VR of {tuple(VR.shape)} elements for a plan of {shape}")
+        P = torch.zeros((self.nb * self.nb, self.nk1, self.nk2), dtype=cdt, device=self.device)
+        S = torch.empty_like(P)
+        try:
+            self.autotune(P, VR, S, reps=reps, planar=True)
+        except HfxError:
+            self.exchange_variant = 0
+
     def destroy(self):
         """Free the plan's device tables now (idempotent)."""
         h, self.handle = self.handle, None

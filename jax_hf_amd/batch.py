@@ -83,6 +83,8 @@ class _BatchFock:
                        L.ptr(self.VRc), L.stream(dev))
             else:
                 self.VRc = self.VR.contiguous()
+            if fock_mod.AUTOTUNE:
+                self.plan.tune_once(self.VRc, bool(self.vfull))
         else:
             self.Sig = torch.zeros((S, nk1, nk2, nb, nb), **c)
         self.epi = torch.zeros(256 * S, dtype=torch.float64, device=dev)   # energy partials, 256 per solve
@@ -135,6 +137,8 @@ class _BatchCG:
         self.part = torch.empty((8, self.nk_tot), **r)
         self.scratch = torch.zeros(int(L.lib().hfx_batch_scratch_doubles(S)), **r)
         self.targets = torch.zeros(S, **r)
+        self.st_host = torch.empty((S, K.ST_LEN), dtype=torch.float64, pin_memory=True)
+        self.ev = torch.cuda.Event()
 
     # -- pieces ---------------------------------------------------------------
     def set_targets(self, n_electrons):
@@ -214,6 +218,12 @@ class _BatchCG:
 
     def run(self, tol_E, tol_grad, max_iter, rel_tol=False):
         """solver.py:422-449 per solve; a solve meeting its rule is frozen."""
+        return _drive(self.run_steps(tol_E, tol_grad, max_iter, rel_tol))
+
+    def run_steps(self, tol_E, tol_grad, max_iter, rel_tol=False):
+        """run() as a generator that yields once per iteration, after queueing
+        the iteration and the asynchronous readback of the state vectors and
+        before waiting for them (the caller may queue other work meanwhile)."""
         cfg = self.cfg
         S = self.S
         self.G.zero_()
@@ -264,7 +274,11 @@ class _BatchCG:
                 self._swap()
             else:
                 self._body(shrink_pow)
-            sv = self.st.cpu().numpy()
+            self.st_host.copy_(self.st, non_blocking=True)
+            self.ev.record()
+            yield
+            self.ev.synchronize()
+            sv = self.st_host.numpy()
             if any(active[s] and sv[s, K.ST_MUDONE] == 0.0 for s in range(S)):
                 self.mu()                 # finish the mu solves the loop's steps left open
             for s in range(S):
@@ -403,6 +417,10 @@ class _BatchPolish:
     def run(self, P_lo, mu0, tol, max_iter, Q_lo=None):
         """Q_lo (optional): the stacked orbitals P_lo was built from; re-orthonormalised
         in FP64 they are the first basis (the first update is a warm eigh)."""
+        return _drive(self.run_steps(P_lo, mu0, tol, max_iter, Q_lo))
+
+    def run_steps(self, P_lo, mu0, tol, max_iter, Q_lo=None):
+        """run() as a generator: yields before each wait for a readback."""
         S, nh = self.S, self.nh
         max_iter = max(int(max_iter), 1)
         self._eigh_tol = self.EIGH_TOL_PER_TOL * float(tol)
@@ -439,6 +457,7 @@ class _BatchPolish:
                 spec = not (preds and all(preds))
             if spec:
                 self._advance(k, warm=k > 0 or warm0)
+            yield
             self.ev.synchronize()
             sv = self.st_host.numpy()
             bad_mu = k > 0 and redo != k and any(not done[s] and sv[s, K.ST_MUDONE] == 0.0 for s in range(S))
@@ -489,14 +508,134 @@ class _BatchPolish:
         return E_out, done, hE, hC
 
 
-def solve_batch(kernels, P0s, n_electrons, *, config: SolverConfig | None = None):
+def _drive(gen):
+    """Run a step generator to its end; its return value."""
+    try:
+        while True:
+            next(gen)
+    except StopIteration as e:
+        return e.value
+
+
+def _interleave(gens, dev, nstreams):
+    """Run step generators side by side, each on its own stream (round robin
+    over ``nstreams`` streams of default priority): each resume queues one
+    iteration of that generator and its readback, so while the host waits for
+    one mesh group the others' iterations are on the GPU."""
+    cur = torch.cuda.current_stream(dev)
+    if len(gens) == 1 or nstreams <= 1:
+        for g in gens:
+            _drive(g)
+        return
+    streams = [torch.cuda.Stream(dev) for _ in range(min(nstreams, len(gens)))]
+    for sm in streams:
+        sm.wait_stream(cur)             # the inputs were written on the caller's stream
+    live = [(g, streams[i % len(streams)]) for i, g in enumerate(gens)]
+    try:
+        while live:
+            nxt = []
+            for g, sm in live:
+                with torch.cuda.stream(sm):
+                    try:
+                        next(g)
+                        nxt.append((g, sm))
+                    except StopIteration:
+                        pass
+            live = nxt
+    finally:
+        for sm in streams:
+            cur.wait_stream(sm)
+
+
+def _same_vr(a, b):
+    """Exact VR equality of two kernels, remembered on the kernels (a kernel's
+    VR is fixed at construction) so that repeated scans compare on the host."""
+    ta, tb = a.__dict__.get("_hfx_vr_token"), b.__dict__.get("_hfx_vr_token")
+    if ta is not None and ta is tb:
+        return True
+    if a._VR_shifted.shape != b._VR_shifted.shape or a._VR_shifted.dtype != b._VR_shifted.dtype:
+        return False
+    if not torch.equal(a._VR_shifted, b._VR_shifted):
+        return False
+    tok = ta if ta is not None else (tb if tb is not None else object())
+    a.__dict__["_hfx_vr_token"] = b.__dict__["_hfx_vr_token"] = tok
+    return True
+
+
+def _histories(rows, m, dev):
+    """(S, m) device tensor of per-solve histories (zero-padded), one copy."""
+    H = np.zeros((len(rows), m), np.float64)
+    for j, r in enumerate(rows):
+        H[j, :len(r)] = r[:m]
+    return torch.from_numpy(H).to(dev)
+
+
+def _keep_for(stream, *tensors):
+    """The caller's stream uses these after the call: tell the allocator."""
+    for t in tensors:
+        t.record_stream(stream)
+
+
+def _group_mixed(kernels, members, P0s, n_electrons, cfg, results, caller):
+    """FP32 stacked CG then the FP64 stacked polish of one mesh group (steps)."""
+    ks = [kernels[i] for i in members]
+    ne = [n_electrons[i] for i in members]
+    b = _BatchCG(ks, torch.complex64, cfg)
+    b.set_targets(ne)
+    b.init_from([P0s[i] for i in members])
+    k_of, grad, hE, hG = yield from b.run_steps(cfg.mixed_tol_E, 0.0, max(cfg.max_iter // 2, 1), rel_tol=True)
+    P_lo = b.density()
+    Q_lo = b.Q
+    mu0 = b.st[:, K.ST_MU].clone()
+    del b                                            # free the FP32 stack before the FP64 one
+    pol = _BatchPolish(ks, ne)
+    E2, conv2, hE2, hC2 = yield from pol.run_steps(P_lo, mu0, cfg.polish_tol, cfg.polish_max_iter, Q_lo=Q_lo)
+    del Q_lo, P_lo
+    dev = pol.dev
+    n_it = [k_of[j] + len(hE2[j]) for j in range(len(members))]
+    m = max([cfg.max_iter] + n_it)
+    hEt = _histories([list(hE[j]) + list(hE2[j]) for j in range(len(members))], m, dev)
+    hGt = _histories([list(hG[j]) + list(hC2[j]) for j in range(len(members))], m, dev)
+    Et = torch.tensor(E2, dtype=torch.float64, device=dev)
+    _keep_for(caller, pol.U, pol.occ, pol.st, pol.Pfin, pol.Ffin, Et, hEt, hGt)
+    for j, i in enumerate(members):
+        results[i] = SolveResult(Q=pol.U[j], p=pol.occ[j], mu=pol.st[j, K.ST_MU], density=pol.Pfin[j],
+                                 fock=pol.Ffin[j], energy=Et[j], n_iter=int(n_it[j]), converged=bool(conv2[j]),
+                                 history=dict(E=hEt[j], grad_norm=hGt[j],
+                                              phase_iters=torch.tensor([k_of[j], len(hE2[j])], dtype=torch.int64)))
+
+
+def _group_cg(kernels, members, P0s, n_electrons, cfg, cdt, results, caller):
+    """Stacked CG of one mesh group to the reference stopping rule (steps)."""
+    ks = [kernels[i] for i in members]
+    b = _BatchCG(ks, cdt, cfg)
+    b.set_targets([n_electrons[i] for i in members])
+    b.init_from([P0s[i] for i in members])
+    k_of, grad, hE, hG = yield from b.run_steps(cfg.tol_E, cfg.tol_grad, cfg.max_iter)
+    Pfin, Ffin = b.finalize()
+    dev = b.dev
+    m = max([cfg.max_iter] + list(k_of))
+    hEt = _histories(hE, m, dev)
+    hGt = _histories(hG, m, dev)
+    _keep_for(caller, b.Q, b.p, b.st, Pfin, Ffin, hEt, hGt)
+    for j, i in enumerate(members):
+        kk = k_of[j]
+        conv = (grad[j] <= cfg.tol_grad) if cfg.tol_grad > 0.0 else (kk < cfg.max_iter)
+        results[i] = SolveResult(Q=b.Q[j], p=b.p[j], mu=b.st[j, K.ST_MU], density=Pfin[j], fock=Ffin[j],
+                                 energy=b.st[j, K.ST_E], n_iter=int(kk), converged=bool(conv),
+                                 history=dict(E=hEt[j], grad_norm=hGt[j]))
+
+
+def solve_batch(kernels, P0s, n_electrons, *, config: SolverConfig | None = None, streams: int = 4):
     """Independent CG solves of a scan, stacked into shared launches.
 
     ``kernels``, ``P0s`` and ``n_electrons`` are sequences of equal length
     (``P0s`` / ``n_electrons`` may also be a single density / count used for
-    all).  Returns one ``SolveResult`` per solve, in order.  Solves are
-    stacked per (mesh, bands, dtype, T, interaction); the rest go through
-    ``solve``.
+    all).  Returns one ``SolveResult`` per solve, in order; the result
+    tensors of one mesh group are views of that group's stacked buffers.
+    Solves are stacked per (mesh, bands, dtype, T, interaction); the mesh
+    groups run side by side on up to ``streams`` streams, their host loops
+    interleaved; the rest go through ``solve``.
     """
     from .solver import solve
     from .utils import validate_electron_count
@@ -520,59 +659,23 @@ def solve_batch(kernels, P0s, n_electrons, *, config: SolverConfig | None = None
         key = _group_key(kern)
         placed = False
         for gkey, members in groups.items():
-            if gkey[0] == key and torch.equal(kernels[members[0]]._VR_shifted, kern._VR_shifted):
+            if gkey[0] == key and _same_vr(kernels[members[0]], kern):
                 members.append(i)
                 placed = True
                 break
         if not placed:
             groups[(key, i)] = [i]
+    if not groups:
+        return results
+    dev = kernels[next(iter(groups.values()))[0]].device
+    caller = torch.cuda.current_stream(dev)
+    gens = []
     for members in groups.values():
-        ks = [kernels[i] for i in members]
-        prec = _precision(ks[0], cfg)
-        cdt = torch.complex128 if prec == "fp64" else torch.complex64
-        b = _BatchCG(ks, cdt, cfg)
-        b.set_targets([n_electrons[i] for i in members])
-        b.init_from([P0s[i] for i in members])
+        prec = _precision(kernels[members[0]], cfg)
         if prec == "mixed":
-            k_of, grad, hE, hG = b.run(cfg.mixed_tol_E, 0.0, max(cfg.max_iter // 2, 1), rel_tol=True)
-            P_lo = b.density()
-            Q_lo = b.Q
-            mu0 = b.st[:, K.ST_MU].clone()
-            del b                                            # free the FP32 stack before the FP64 one
-            pol = _BatchPolish(ks, [n_electrons[i] for i in members])
-            E2, conv2, hE2, hC2 = pol.run(P_lo, mu0, cfg.polish_tol, cfg.polish_max_iter, Q_lo=Q_lo)
-            del Q_lo
-            sv = pol.st.cpu()
-            for j, i in enumerate(members):
-                n2 = len(hE2[j])
-                kk = k_of[j] + n2
-                hEj = list(hE[j]) + list(hE2[j])
-                hGj = list(hG[j]) + list(hC2[j])
-                m = max(cfg.max_iter, kk)
-                hEt = torch.zeros(m, dtype=torch.float64)
-                hGt = torch.zeros(m, dtype=torch.float64)
-                hEt[:kk] = torch.tensor(hEj[:kk], dtype=torch.float64)
-                hGt[:kk] = torch.tensor(hGj[:kk], dtype=torch.float64)
-                results[i] = SolveResult(Q=pol.U[j].clone(), p=pol.occ[j].clone(), mu=sv[j, K.ST_MU].to(pol.dev),
-                                         density=pol.Pfin[j].clone(), fock=pol.Ffin[j].clone(),
-                                         energy=torch.tensor(E2[j], dtype=torch.float64, device=pol.dev),
-                                         n_iter=int(kk), converged=bool(conv2[j]),
-                                         history=dict(E=hEt.to(pol.dev), grad_norm=hGt.to(pol.dev),
-                                                      phase_iters=torch.tensor([k_of[j], n2], dtype=torch.int64)))
-            continue
-        k_of, grad, hE, hG = b.run(cfg.tol_E, cfg.tol_grad, cfg.max_iter)
-        Pfin, Ffin = b.finalize()
-        sv = b.st.cpu()
-        for j, i in enumerate(members):
-            kk = k_of[j]
-            conv = (grad[j] <= cfg.tol_grad) if cfg.tol_grad > 0.0 else (kk < cfg.max_iter)
-            m = max(cfg.max_iter, kk)
-            hEt = torch.zeros(m, dtype=torch.float64)
-            hGt = torch.zeros(m, dtype=torch.float64)
-            hEt[:kk] = torch.tensor(hE[j][:kk], dtype=torch.float64)
-            hGt[:kk] = torch.tensor(hG[j][:kk], dtype=torch.float64)
-            results[i] = SolveResult(Q=b.Q[j].clone(), p=b.p[j].clone(), mu=sv[j, K.ST_MU].to(b.dev),
-                                     density=Pfin[j].clone(), fock=Ffin[j].clone(),
-                                     energy=sv[j, K.ST_E].to(b.dev), n_iter=int(kk), converged=bool(conv),
-                                     history=dict(E=hEt.to(b.dev), grad_norm=hGt.to(b.dev)))
+            gens.append(_group_mixed(kernels, members, P0s, n_electrons, cfg, results, caller))
+        else:
+            cdt = torch.complex128 if prec == "fp64" else torch.complex64
+            gens.append(_group_cg(kernels, members, P0s, n_electrons, cfg, cdt, results, caller))
+    _interleave(gens, dev, int(streams))
     return results

@@ -51,19 +51,20 @@ __device__ __forceinline__ cx<T> swzc(cx<T> v) {
   return cx<T>{swz<X>(v.re), swz<X>(v.im)};
 }
 
+template <typename T>
 struct J16Lds {
   // per (matrix in the wave, row/column x): J_xx, J_{x^M, x}, the new
   // diagonal entry, and whether pair (x, x^M) rotated
-  double jd[2][16];
-  double jre[2][16];
-  double jim[2][16];
-  double dn[2][16];
+  T jd[2][16];
+  T jre[2][16];
+  T jim[2][16];
+  T dn[2][16];
   int rot[2][16];
 };
 
 template <typename T, int M>
 __device__ __forceinline__ void j16_step(cx<T> (&b)[8], cx<T> (&w)[8], int m, int h, int i_in, int nb, bool act,
-                                         T thr2, int& rot, J16Lds& sh) {
+                                         T thr2, int& rot, J16Lds<T>& sh) {
   __builtin_amdgcn_sched_barrier(0);
   // an opaque copy of the row index, so that the compiler does not hoist every
   // step's index arithmetic out of the sweep loop
@@ -86,23 +87,26 @@ __device__ __forceinline__ void j16_step(cx<T> (&b)[8], cx<T> (&w)[8], int m, in
       if (ac2 > thr2) {
         r = 1;
         const bool lo = i < j;          // this row is p, the smaller index of the pair
-        const T ac = sqrt(ac2);
+        // 1/|c| once (zeta and the phase are products with it), cs = rsqrt(1 + t^2):
+        // one division and one square root per rotation
+        const T rac = rsqrt(ac2);
+        const T ac = ac2 * rac;
         const cx<T> c = lo ? cij : conj(cij);
         const T App = lo ? app : aqq, Aqq = lo ? aqq : app;
-        const T zeta = (Aqq - App) / ((T)2 * ac);
+        const T zeta = (Aqq - App) * ((T)0.5 * rac);
         const T t = (zeta >= (T)0 ? (T)1 : (T)-1) / (fabs(zeta) + sqrt((T)1 + zeta * zeta));
-        const T cs = (T)1 / sqrt((T)1 + t * t), sn = t * cs;
-        const T pre = c.re / ac, pim = c.im / ac;
+        const T cs = rsqrt((T)1 + t * t), sn = t * cs;
+        const T pre = c.re * rac, pim = c.im * rac;
         jd = cs;
         jo = lo ? cx<T>{-sn * pre, sn * pim} : cx<T>{sn * pre, sn * pim};   // J_qp : J_pq
         dn = lo ? App - t * ac : Aqq + t * ac;
       }
     }
     rot |= r;
-    sh.jd[m][i] = (double)jd;
-    sh.jre[m][i] = (double)jo.re;
-    sh.jim[m][i] = (double)jo.im;
-    sh.dn[m][i] = (double)dn;
+    sh.jd[m][i] = jd;
+    sh.jre[m][i] = jo.re;
+    sh.jim[m][i] = jo.im;
+    sh.dn[m][i] = dn;
     sh.rot[m][i] = r;
   }
   __syncthreads();
@@ -125,8 +129,8 @@ __device__ __forceinline__ void j16_step(cx<T> (&b)[8], cx<T> (&w)[8], int m, in
 #pragma unroll
   for (int s = 0; s < 8; ++s) {
     const int x = ih ^ s;
-    const T dx = (T)sh.jd[m][x];
-    const cx<T> ox{(T)sh.jre[m][x], (T)sh.jim[m][x]};
+    const T dx = sh.jd[m][x];
+    const cx<T> ox{sh.jre[m][x], sh.jim[m][x]};
     b[s] = dx * b[s] + ox * pb[s];
     w[s] = dx * w[s] + ox * pw[s];
     // materialise V here: otherwise the compiler sinks every step's V update
@@ -134,8 +138,8 @@ __device__ __forceinline__ void j16_step(cx<T> (&b)[8], cx<T> (&w)[8], int m, in
     asm volatile("" : "+v"(w[s].re), "+v"(w[s].im));
   }
   // A <- J^H A: row i with the partner row i ^ M
-  const T jdi = (T)sh.jd[m][i];
-  const cx<T> cjo{(T)sh.jre[m][i], -(T)sh.jim[m][i]};
+  const T jdi = sh.jd[m][i];
+  const cx<T> cjo{sh.jre[m][i], -sh.jim[m][i]};
   constexpr int XR = MH ? (16 | M) : M;
 #pragma unroll
   for (int s = 0; s < 8; ++s) pb[s] = swzc<XR>(b[s ^ ML]);
@@ -143,7 +147,7 @@ __device__ __forceinline__ void j16_step(cx<T> (&b)[8], cx<T> (&w)[8], int m, in
   for (int s = 0; s < 8; ++s) b[s] = jdi * b[s] + cjo * pb[s];
   // exact values on a rotated pair: A_ii = the new diagonal, A_{i, i^M} = 0
   if (sh.rot[m][i]) {
-    if (h == 0) b[0] = cx<T>{(T)sh.dn[m][i], (T)0};
+    if (h == 0) b[0] = cx<T>{sh.dn[m][i], (T)0};
     if (h == MH) b[ML] = cx<T>{(T)0, (T)0};
   }
   __syncthreads();
@@ -151,7 +155,7 @@ __device__ __forceinline__ void j16_step(cx<T> (&b)[8], cx<T> (&w)[8], int m, in
 
 template <typename T, int P, int M = 1>
 __device__ __forceinline__ void j16_sweep(cx<T> (&b)[8], cx<T> (&w)[8], int m, int h, int i, int nb, bool act,
-                                          T thr2, int& rot, J16Lds& sh) {
+                                          T thr2, int& rot, J16Lds<T>& sh) {
   j16_step<T, M>(b, w, m, h, i, nb, act, thr2, rot, sh);
   if constexpr (M + 1 < P) j16_sweep<T, P, M + 1>(b, w, m, h, i, nb, act, thr2, rot, sh);
 }
@@ -160,7 +164,7 @@ template <typename T, int P>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void eigh16_kernel(int nb, size_t nk, const cx<T>* __restrict__ A,
                                                     double* __restrict__ lam, cx<T>* __restrict__ V, int herm_in,
                                                     double rel_tol) {
-  __shared__ J16Lds sh;
+  __shared__ J16Lds<T> sh;
   __shared__ double sd[2][16];
   __shared__ int srank[2][16];
   const int lane = threadIdx.x, m = lane >> 5, h = (lane >> 4) & 1, i = lane & 15;

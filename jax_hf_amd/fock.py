@@ -37,6 +37,11 @@ __all__ = ["build_fock", "hf_energy", "occupation_entropy", "free_energy", "Fock
 # assign False to use the (k, i, j)-layout exchange kernels).
 PLANAR = os.environ.get("HFX_PLANAR", "1") != "0"
 
+# The first slot-plane Fock engine of each plan times the exchange variants and
+# keeps the fastest (set HFX_AUTOTUNE=0 or assign False to keep the built-in
+# per-mesh default).
+AUTOTUNE = os.environ.get("HFX_AUTOTUNE", "1") != "0"
+
 # ---------------------------------------------------------------------------
 # block specs (linalg.py:19-90)
 # ---------------------------------------------------------------------------
@@ -141,6 +146,8 @@ class FockEngine:
         self._sig_ws = None
         self._vrc = None
         self._pl_ws = None
+        if AUTOTUNE and PLANAR and self.include_exchange and nb <= 64 and VR is not None:
+            self.plan.tune_once(self.vr_planar(), bool(self.vfull))
 
     # -- pieces -------------------------------------------------------------
     def selfenergy(self, P, out, refP=None, chans=None, nchan=0, herm=None, rho=True, ifftshift=False):

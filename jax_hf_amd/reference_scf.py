@@ -104,6 +104,10 @@ class _SCF:
         self.Pn = torch.zeros(self.shape4, **c)
         self.tmp = torch.zeros(self.shape4, **c)
         self.U = torch.zeros(self.shape4, **c)
+        self.Ft = torch.empty(self.shape4, **c)
+        self.W = torch.empty(self.shape4, **c)
+        self.n_warm = -1                  # eigh calls in the running basis U (-1: no basis yet)
+        self.allow_warm = False
         self.eps = torch.zeros((nk1, nk2, nb), dtype=torch.float64, device=self.dev)
         self.occ = torch.zeros_like(self.eps)
         self.st = torch.zeros(K.ST_LEN, dtype=torch.float64, device=self.dev)
@@ -115,14 +119,41 @@ class _SCF:
         K.hermitize(src, out)
         return out
 
+    # the running basis is re-orthonormalised (one Newton-Schulz step) every
+    # this many warm eigh calls, so that its drift from unitarity stays at
+    # rounding level over long runs
+    REORTH_EVERY = 16
+    # False: every eigh from scratch
+    WARM_EIGH = True
+    # The warm start engages once the commutator residual is below this.  Far
+    # from the fixed point the DIIS history can be exactly degenerate (the
+    # density repeats, so rows of B coincide and the augmented solve reports a
+    # singular system); a cold eigh reproduces a repeated density bit for bit
+    # and keeps that degeneracy exact, as the reference's does, while a basis
+    # that moved between the calls perturbs it at rounding level into a nearly
+    # singular solve with huge coefficients.
+    WARM_BELOW = 1e-3
+
     def density_from_fock(self, F, P_out, mu_slot):
         """reference_scf.py:87-96: eigh(herm F), two bisections with level shift, P.
+
+        Once the commutator residual is below WARM_BELOW the calls work in the
+        previous call's eigenvectors U (warm start: Ft = U^H herm(F) U is
+        nearly diagonal, so the Jacobi eigh needs a few sweeps instead of a
+        cold run's ~9; then U <- U W); before that every call diagonalises
+        herm(F) from scratch.  The eigenvalues, the occupations and
+        P = U diag(occ) U^H are those of eigh(herm F) to rounding.
 
         With ``mu_newton`` (the FP64 polish of the mixed CG form, level shift 0)
         mu comes from the safeguarded Newton solver warm-started at the previous
         mu instead: the same root to rounding, in a few steps instead of 54.
         """
-        K.eigh(F, self.eps, self.U, herm_in=True)
+        warm = self.n_warm >= 0 and self.WARM_EIGH and self.allow_warm
+        if warm:
+            K.extrap_rotate(self.Ft, self.U, [F], [1.0])
+            K.eigh(self.Ft, self.eps, self.W, herm_in=True)
+        else:
+            K.eigh(F, self.eps, self.U, herm_in=True)
         if self.mu_newton and self.cfg.level_shift == 0.0:
             K.solve_mu_steps(self.wsp, self.eps, self.w, self.nk, self.nb, self.n_e, self.kernel.T,
                              mu0_from_state=self.mu_warm, maxiter=54, st=self.st, p_out=self.occ)
@@ -133,7 +164,14 @@ class _SCF:
             K.solve_mu(self.wsp, self.eps, self.w, self.nk, self.nb, self.n_e, self.kernel.T,
                        maxiter=30 if self.single else 54, mode=K.MU_BISECTION,
                        level_shift=self.cfg.level_shift, mu_out_ptr=K.st_ptr(self.st, mu_slot), p_out=self.occ)
-        K.density(self.U, self.tmp, p=self.occ)
+        if warm:
+            K.rotate_density(self.U, self.W, self.occ, self.tmp)
+            self.n_warm += 1
+            if self.n_warm % self.REORTH_EVERY == 0:
+                K.orthonormalize(self.U, 1)
+        else:
+            K.density(self.U, self.tmp, p=self.occ)
+            self.n_warm = 0
         return self.herm_project(self.tmp, P_out)
 
     def run(self, P0, n_electrons):
@@ -180,6 +218,7 @@ class _SCF:
             d_res = float(np.sqrt(sv[1] / self.ws_sum))
             E = float(sv[2])
             converged = d_res <= cfg.density_tol and c_res <= cfg.comm_tol  # :295
+            self.allow_warm = c_res < self.WARM_BELOW
             engaged = False
             if use_diis:                                                  # :298-314
                 row = sv[8:8 + nh]

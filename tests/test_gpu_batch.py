@@ -95,3 +95,22 @@ def test_solve_batch_falls_back_for_unbatchable(dev):
                         [ne[0], 2.0, ne[1]], config=cfg)
     ref = J.solve(kh, np.zeros_like(h), 2.0, config=cfg)
     assert float(out[1].energy) == float(ref.energy)
+
+
+def test_solve_batch_streams_match_one_stream(dev):
+    """Mesh groups interleaved on four streams give bit-identical results to
+    the groups run one after another on the caller's stream, and the results
+    are ready for the caller's stream on return."""
+    import torch
+    import jax_hf_amd as J
+    kernels, P0s, ne = _scan([12, 16, 12, 20, 16], nb=4, seed=11)
+    for prec in ("fp64", "mixed"):
+        cfg = J.SolverConfig(max_iter=300, precision=prec, tol_E=1e-10)
+        one = J.solve_batch(kernels, P0s, ne, config=cfg, streams=1)
+        four = J.solve_batch(kernels, P0s, ne, config=cfg, streams=4)
+        for a, b in zip(one, four):
+            assert float(a.energy) == float(b.energy)
+            assert int(a.n_iter) == int(b.n_iter) and bool(a.converged) == bool(b.converged)
+            assert torch.equal(a.density, b.density) and torch.equal(a.fock, b.fock)
+            assert torch.equal(a.Q, b.Q) and torch.equal(a.p, b.p)
+            assert float(a.mu) == float(b.mu)
