@@ -229,6 +229,30 @@ __device__ __forceinline__ int exch_channel(ExchArgs<T>& a) {
   return c;
 }
 
+// Slot planes: the plane's P / refP / Sigma offsets need only the plane index.
+// Returns the channel (VR plane, moment slot) without using it, so that a
+// kernel can issue its first P loads before the channel lookup returns; the
+// caller applies the VR offset (exch_plane_vr) once it needs the channel.
+template <typename T>
+__device__ __forceinline__ int exch_plane_begin(ExchArgs<T>& a) {
+  int b = blockIdx.x;
+  if (a.xcd_remap) {  // consecutive planes on one XCD (blocks round-robin over 8 XCDs)
+    const int per = (a.nchan * a.nsolve) >> 3;
+    b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  }
+  const int s = b / a.nchan;
+  const int c = a.chans[b - s * a.nchan];
+  const size_t off = (size_t)b * a.g.n1 * a.g.n2;
+  a.P += off;
+  if (a.refP) a.refP += off;
+  a.Sigma += off;
+  return c;
+}
+template <typename T>
+__device__ __forceinline__ void exch_plane_vr(ExchArgs<T>& a, int c) {
+  if (a.vfull) a.VR += (size_t)c * a.g.n1 * a.g.n2;
+}
+
 // Index of channel c at k-point e in P / refP / Sigma, and in VR.
 template <typename T>
 __device__ __forceinline__ size_t exch_ix(const ExchArgs<T>& a, size_t e, int c, int nb2) {
@@ -503,6 +527,11 @@ __device__ __forceinline__ void dft_any(cx<T> (&v)[R], const cx<T>* tw, int NT) 
   }
 }
 
+// radices whose butterflies use compile-time twiddles only (no table reads)
+constexpr bool tw_free_radix(int R) {
+  return R == 2 || R == 3 || R == 4 || R == 5 || R == 8 || v2_split(R) != 0;
+}
+
 // first (row-pass) and column-pass radix pairs per mesh size
 constexpr int v2_row_a(int N) {
   return N == 96 ? 12 : N == 90 ? 10 : N == 72 ? 8 : N == 64 ? 8 : N == 60 ? 6 : N == 48 ? 6 : N == 36 ? 6
@@ -531,28 +560,41 @@ __device__ __forceinline__ void v2_stage(cx<T>* plane, const cx<T>* tw, const Ex
       // planar HBM stages: lanes walk along the row (coalesced); otherwise across lines
       constexpr bool JF = PL && (SRC == 1 || DST == 1);
       const int line = JF ? it / NR : it % NL, j = JF ? it % NR : it / NL, k = j % NS;
+      if constexpr (SRC == 1) {
+        // all R loads of the item first, then the optional terms: the optional
+        // branches would otherwise split the loop and serialise the loads
+        // (one HBM round trip per element)
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int pos = j + r * NR;
-        cx<T> x;
-        if constexpr (SRC == 1) {
-          const size_t e = (size_t)line * N2 + pos;
-          const size_t ix = PL ? e : e * nb2 + c;
-          x = a.P[ix];
-          if (a.refP) x = x - a.refP[ix];
-          if (a.w2d) {
-            const double w = a.w2d[e];
-            mre += w * (double)x.re;
-            mim += w * (double)x.im;
+        for (int r = 0; r < R; ++r) {
+          const size_t e = (size_t)line * N2 + j + r * NR;
+          v[m][r] = a.P[PL ? e : e * nb2 + c];
+        }
+        if (a.refP) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const size_t e = (size_t)line * N2 + j + r * NR;
+            v[m][r] = v[m][r] - a.refP[PL ? e : e * nb2 + c];
           }
-        } else {
-          x = plane[line * LS + pos * ES];
         }
-        if (NS > 1 && r > 0) {
-          const cx<T> w = tw[r * k * TSTEP];
-          x = INV ? mulc(x, w) : x * w;
+        if (a.w2d) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const double w = a.w2d[(size_t)line * N2 + j + r * NR];
+            mre += w * (double)v[m][r].re;
+            mim += w * (double)v[m][r].im;
+          }
         }
-        v[m][r] = x;
+        static_assert(SRC != 1 || NS == 1, "the HBM stage is the first stage of its pass");
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          cx<T> x = plane[line * LS + (j + r * NR) * ES];
+          if (NS > 1 && r > 0) {
+            const cx<T> w = tw[r * k * TSTEP];
+            x = INV ? mulc(x, w) : x * w;
+          }
+          v[m][r] = x;
+        }
       }
       if constexpr (VRM) {   // prefetch VR at the output positions (k1 = base + r NS, k2 = line)
         const int base = (j / NS) * NS * R + k;
@@ -608,13 +650,43 @@ __global__ __launch_bounds__(BLOCK) void exch_v2_kernel(ExchArgs<T> a) {
   __shared__ cx<T> plane[N1 * PITCH];
   __shared__ cx<T> tw1[N1], tw2[N2];
   __shared__ double red[2 * (BLOCK / 64)];
-  const int c = exch_channel<T, BLOCK>(a);
-  for (int m = threadIdx.x; m < N1; m += BLOCK) tw1[m] = a.g.tw1[m];
-  for (int m = threadIdx.x; m < N2; m += BLOCK) tw2[m] = a.g.tw2[m];
   double mre = 0.0, mim = 0.0;
-  __syncthreads();
-  // forward rows: lines = k1 rows (N1 of them), stride PITCH; positions along k2
-  v2_stage<T, RA2, N2, N1, PITCH, 1, BLOCK, 1, false, 1, 0, false, N2, PL>(plane, tw2, a, c, mre, mim);
+  int c;
+  if constexpr (PL) {
+    // slot planes: the first stage's P loads go out before the channel and the
+    // twiddle tables arrive (as in exch_v3_kernel)
+    static_assert(tw_free_radix(RA2), "the first row stage must not read the twiddle table");
+    c = exch_plane_begin<T>(a);
+    constexpr int T1 = (N1 + BLOCK - 1) / BLOCK, T2 = (N2 + BLOCK - 1) / BLOCK;
+    cx<T> t1r[T1], t2r[T2];
+#pragma unroll
+    for (int m = 0; m < T1; ++m)
+      if (threadIdx.x + m * BLOCK < N1) t1r[m] = a.g.tw1[threadIdx.x + m * BLOCK];
+#pragma unroll
+    for (int m = 0; m < T2; ++m)
+      if (threadIdx.x + m * BLOCK < N2) t2r[m] = a.g.tw2[threadIdx.x + m * BLOCK];
+    v2_stage<T, RA2, N2, N1, PITCH, 1, BLOCK, 1, false, 1, 0, false, N2, PL>(plane, tw2, a, c, mre, mim);
+    asm volatile("" : "+v"(c));
+#pragma unroll
+    for (int m = 0; m < T1; ++m) {
+      asm volatile("" : "+v"(t1r[m].re), "+v"(t1r[m].im));
+      if (threadIdx.x + m * BLOCK < N1) tw1[threadIdx.x + m * BLOCK] = t1r[m];
+    }
+#pragma unroll
+    for (int m = 0; m < T2; ++m) {
+      asm volatile("" : "+v"(t2r[m].re), "+v"(t2r[m].im));
+      if (threadIdx.x + m * BLOCK < N2) tw2[threadIdx.x + m * BLOCK] = t2r[m];
+    }
+    exch_plane_vr<T>(a, c);
+    __syncthreads();
+  } else {
+    c = exch_channel<T, BLOCK>(a);
+    for (int m = threadIdx.x; m < N1; m += BLOCK) tw1[m] = a.g.tw1[m];
+    for (int m = threadIdx.x; m < N2; m += BLOCK) tw2[m] = a.g.tw2[m];
+    __syncthreads();
+    // forward rows: lines = k1 rows (N1 of them), stride PITCH; positions along k2
+    v2_stage<T, RA2, N2, N1, PITCH, 1, BLOCK, 1, false, 1, 0, false, N2, PL>(plane, tw2, a, c, mre, mim);
+  }
   v2_stage<T, RB2, N2, N1, PITCH, 1, BLOCK, RA2, false, 0, 0, false, N2, PL>(plane, tw2, a, c, mre, mim);
   // forward columns (+ VR multiply), inverse columns
   v2_stage<T, RA1, N1, N2, 1, PITCH, BLOCK, 1, false, 0, 0, false, N2, PL>(plane, tw1, a, c, mre, mim);
@@ -699,12 +771,29 @@ __global__ __launch_bounds__(BLOCK) void exch_v3_kernel(ExchArgs<T> a) {
   __shared__ cx<T> plane[N * PITCH];
   __shared__ cx<T> tw[N];
   __shared__ double red[2 * (BLOCK / 64)];
-  const int c = exch_channel<T, BLOCK>(a);
-  for (int m = threadIdx.x; m < N; m += BLOCK) tw[m] = a.g.tw1[m];
+  // The slot-plane offsets need only the plane index, so the first stage's P
+  // loads go out at once; the channel (for VR and the moment) and the twiddle
+  // table arrive meanwhile (the first stage's radix reads no table twiddles)
+  static_assert(tw_free_radix(RA2), "the first row stage must not read the twiddle table");
+  int c = exch_plane_begin<T>(a);
+  constexpr int TWM = (N + BLOCK - 1) / BLOCK;
+  cx<T> twr[TWM];
+#pragma unroll
+  for (int m = 0; m < TWM; ++m)
+    if (threadIdx.x + m * BLOCK < N) twr[m] = a.g.tw1[threadIdx.x + m * BLOCK];
   double mre = 0.0, mim = 0.0;
-  __syncthreads();
   // forward rows: lines = k1 rows, positions along k2; the first stage loads from HBM
   v2_stage<T, RA2, N, N, PITCH, 1, BLOCK, 1, false, 1, 0, false, N, true>(plane, tw, a, c, mre, mim);
+  // opaque after the stage: the compiler would otherwise consume the channel
+  // and the table (and so wait for them) before the stage's loads
+  asm volatile("" : "+v"(c));
+#pragma unroll
+  for (int m = 0; m < TWM; ++m) {
+    asm volatile("" : "+v"(twr[m].re), "+v"(twr[m].im));
+    if (threadIdx.x + m * BLOCK < N) tw[threadIdx.x + m * BLOCK] = twr[m];
+  }
+  exch_plane_vr<T>(a, c);
+  __syncthreads();
   // VR of the fused column stage, loaded after P (loads retire in order, so a
   // wait for P would also wait for anything issued before it)
   cx<T> vr[FMAX][RB1];
