@@ -120,21 +120,22 @@ def fock_bench(J, L, syn, inp, dev, nk, nb, args):
     bytes_exch = 16 * Nk * (3 * nb * nb)               # P + VR in, Sigma out
     bytes_build = 16 * Nk * (4 * nb * nb + nb * nb)    # + h in, F out
 
-    def step(evs=None):
-        # the steps of FockEngine.build, spelled out so the exchange launch and
-        # the whole build can be timed on the launch stream
-        if evs is not None:
+    def step(evs=None, whole=False):
+        # the steps of FockEngine.build, spelled out so the exchange launch
+        # (evs[0], evs[1]) or the whole build (whole: evs[2], evs[3]) can be
+        # timed on the launch stream
+        if evs is not None and whole:
             evs[2].record()
         if planar:
             L.call("hfx_planar_pack", h, 1, L.ptr(P), None, None, 0, 0, L.ptr(Pt), st)
-        if evs is not None:
+        if evs is not None and not whole:
             evs[0].record()
         if planar:
             L.call("hfx_selfenergy_planar", h, 1, L.ptr(Pt), L.ptr(VRc), eng.vfull, None, 0, 0, 0, None,
                    L.ptr(St), None, st)
         else:
             eng.selfenergy(P, Sig, refP=None, rho=False)
-        if evs is not None:
+        if evs is not None and not whole:
             evs[1].record()
         if planar:
             L.call("hfx_fock_epilogue_planar", h, L.ptr(eng.h), L.ptr(St), None, 0, 0, None, None, L.ptr(P),
@@ -142,20 +143,28 @@ def fock_bench(J, L, syn, inp, dev, nk, nb, args):
         else:
             L.call("hfx_fock_epilogue", h, L.ptr(eng.h), L.ptr(Sig), None, None, L.ptr(P),
                    L.ptr(eng.w2d), L.ptr(F), None, L.ptr(eng.plan.scratch), L.ptr(E), 0.0, st)
-        if evs is not None:
+        if evs is not None and whole:
             evs[3].record()
 
     for _ in range(20):
         step()
     torch.cuda.synchronize()
     n = args.fock_steps
-    evs = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(4)) for _ in range(n)]
-    torch.cuda.synchronize()
+    # builds/s without events in the loop; then the exchange launch and the
+    # whole build each timed with a pair of events per build (an event record
+    # between two launches adds its own gap, so each timing loop records only
+    # the pair it reads)
     t0 = time.perf_counter()
     for i in range(n):
-        step(evs[i])
+        step()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    evs = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(4)) for _ in range(n)]
+    for i in range(n):
+        step(evs[i])
+    for i in range(n):
+        step(evs[i], whole=True)
+    torch.cuda.synchronize()
     exch_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
     build_ms = float(np.mean([e[2].elapsed_time(e[3]) for e in evs]))
     achieved = bytes_exch / (exch_ms * 1e-3) / 1e9

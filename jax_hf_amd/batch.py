@@ -638,8 +638,7 @@ def solve_batch(kernels, P0s, n_electrons, *, config: SolverConfig | None = None
     interleaved; the rest go through ``solve``.
     """
     from .solver import solve
-    from .utils import validate_electron_count
-    from ._util import host_array
+    from .utils import validate_kernel_electron_count
     cfg = config or SolverConfig()
     n = len(kernels)
     if not isinstance(P0s, (list, tuple)):
@@ -649,7 +648,7 @@ def solve_batch(kernels, P0s, n_electrons, *, config: SolverConfig | None = None
     if len(P0s) != n or len(n_electrons) != n:
         raise ValueError("kernels, P0s and n_electrons must have the same length")
     for kern, ne in zip(kernels, n_electrons):
-        validate_electron_count(host_array(kern._w64), kern.h.shape[-1], ne, context="n_electrons")
+        validate_kernel_electron_count(kern, ne, context="n_electrons")
     results = [None] * n
     groups: dict = {}
     for i, kern in enumerate(kernels):

@@ -735,13 +735,10 @@ def _polish_diis(kernel, runner, n_electrons, cfg, fock_build_fn, k_cg, hist_E, 
         k = sum(phases)
         hE += list(hE2)
         hG += list(hC2)
-        hEt = torch.zeros(max(cfg.max_iter, k), dtype=torch.float64)
-        hGt = torch.zeros_like(hEt)
-        hEt[:k] = torch.tensor(hE[:k], dtype=torch.float64)
-        hGt[:k] = torch.tensor(hG[:k], dtype=torch.float64)
-        hist = dict(E=hEt.to(dev), grad_norm=hGt.to(dev), phase_iters=torch.tensor(phases, dtype=torch.int64))
+        hEt, hGt = _histories_to_device([hE[:k], hG[:k]], max(cfg.max_iter, k), dev)
+        hist = dict(E=hEt, grad_norm=hGt, phase_iters=torch.tensor(phases, dtype=torch.int64))
         return SolveResult(Q=Q, p=p, mu=mu, density=density, fock=F,
-                           energy=torch.tensor(E, dtype=torch.float64, device=dev), n_iter=int(k),
+                           energy=torch.full((), E, dtype=torch.float64, device=dev), n_iter=int(k),
                            converged=bool(conv), history=hist)
     P64 = convert(P_lo, torch.complex128) if P_lo.dtype != torch.complex128 else P_lo.clone()
     scfg = SCFConfig(max_iter=cfg.polish_max_iter, acceleration="diis", density_tol=cfg.polish_tol,
@@ -766,6 +763,18 @@ def _polish_diis(kernel, runner, n_electrons, cfg, fock_build_fn, k_cg, hist_E, 
                 phase_iters=torch.tensor([k_cg, n2], dtype=torch.int64))
     return SolveResult(Q=Q, p=p, mu=mu.to(dev), density=density, fock=F, energy=E, n_iter=int(k),
                        converged=bool(conv), history=hist)
+
+
+def _histories_to_device(rows, m, dev):
+    """Per-iteration histories as zero-padded length-m device vectors: one
+    pinned host buffer and one asynchronous copy (no wait on the GPU)."""
+    H = torch.zeros((len(rows), m), dtype=torch.float64, pin_memory=True)
+    for j, r in enumerate(rows):
+        n = min(len(r), m)
+        if n:
+            H[j, :n] = torch.tensor(r[:n], dtype=torch.float64)
+    D = H.to(dev, non_blocking=True)
+    return tuple(D[j] for j in range(len(rows)))
 
 
 def _result(runner, Pfin, Ffin, sv, k, converged, hist_E, hist_G, max_iter, dev):
@@ -796,10 +805,10 @@ def _precision(kernel, cfg):
 def solve_direct_minimization(kernel, P0, n_electrons: float, *, config: SolverConfig | None = None,
                               fock_build_fn=None) -> SolveResult:
     """Direct-minimisation HF solver (solver.py:1098-1247): CG or trust-region Newton."""
-    from .utils import validate_electron_count
+    from .utils import validate_kernel_electron_count
     if config is None:
         config = SolverConfig()
-    validate_electron_count(host_array(kernel._w64), kernel.h.shape[-1], n_electrons, context="n_electrons")
+    validate_kernel_electron_count(kernel, n_electrons, context="n_electrons")
     optimizer = str(config.optimizer).lower()
     if optimizer not in ("cg", "newton"):
         raise ValueError(f"Unknown optimizer {config.optimizer!r}; expected 'cg' or 'newton'.")

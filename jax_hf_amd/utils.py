@@ -24,7 +24,16 @@ __all__ = ["validate_electron_count", "hermitize", "fermidirac", "selfenergy_fft
 
 def validate_electron_count(weights: Any, nbands: int, n_electrons: Any, *, context: str = "n_electrons") -> None:
     """utils.py:21-43."""
-    total_weight = float(np.real(host_array(weights)).sum())
+    _check_count(float(np.real(host_array(weights)).sum()), nbands, n_electrons, context)
+
+
+def validate_kernel_electron_count(kernel, n_electrons: Any, *, context: str = "n_electrons") -> None:
+    """validate_electron_count for a HartreeFockKernel, from the weight sum it
+    keeps on the host (no device read)."""
+    _check_count(float(kernel.weight_sum), kernel.h.shape[-1], n_electrons, context)
+
+
+def _check_count(total_weight: float, nbands: int, n_electrons: Any, context: str) -> None:
     target = float(np.asarray(host_array(n_electrons)))
     max_electrons = total_weight * int(nbands)
     tol = max(1e-8, 1e-6 * max(1.0, abs(max_electrons)))
