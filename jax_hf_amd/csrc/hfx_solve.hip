@@ -203,6 +203,33 @@ __device__ __forceinline__ bool mu_converged(double g, double Z, double N, doubl
          hi - lo <= tol_mu;
 }
 
+// One safeguarded Newton update of mu (solver.py:80-99) from the residual g
+// and slope Z at mu: bracket update, Newton step, bisection when the step
+// leaves the bracket.  Returns true when it was a Newton step d small enough
+// that the error it leaves is below the convergence tolerance -- with
+// |N''| <= N'/T the quadratic term is at most d^2 / (2T) -- so the solve can
+// end after this update instead of spending one more step to confirm it.
+__device__ __forceinline__ bool mu_newton_update(double g, double Z, double& mu, double Tj, double& lo,
+                                                 double& hi) {
+  if (g < 0) lo = mu;
+  if (g > 0) hi = mu;
+  const double d = g / fmax(Z, 1e-18);
+  double mu_new = mu - d;
+  const double mu_bis = 0.5 * (lo + hi);
+  bool newton = true;
+  if (mu_new <= lo || mu_new >= hi) {
+    mu_new = mu_bis;
+    newton = false;
+  }
+  mu_new = fmin(fmax(mu_new, lo), hi);
+  if (!isfinite(mu_new)) {
+    mu_new = mu_bis;
+    newton = false;
+  }
+  mu = mu_new;
+  return newton && d * d <= 0.5 * Tj * 8.9e-16 * fmax(fabs(mu_new), Tj);
+}
+
 // Block-wide sum of the first n entries of a global array (strided per
 // thread, then a block tree): every block of the grid evaluates the same
 // expression tree over the same data, so all blocks get bitwise-identical totals.
@@ -456,16 +483,12 @@ __global__ __launch_bounds__(MUS_BLOCK) void mu_step_fin(MuStepArgs a, int nblk)
     if (mu_converged(g, Z, N, mu, Tj, lo, hi)) {
       a.st[HFX_ST_MUDONE] = 1.0;
     } else {
-      if (g < 0) lo = mu;
-      if (g > 0) hi = mu;
-      double mu_new = mu - g / fmax(Z, 1e-18);
-      const double mu_bis = 0.5 * (lo + hi);
-      if (mu_new <= lo || mu_new >= hi) mu_new = mu_bis;
-      mu_new = fmin(fmax(mu_new, lo), hi);
-      if (!isfinite(mu_new)) mu_new = mu_bis;
+      double mu_new = mu;
+      const bool fin = mu_newton_update(g, Z, mu_new, Tj, lo, hi);
       a.st[HFX_ST_MU] = mu_new;
       a.st[HFX_ST_MULO] = lo;
       a.st[HFX_ST_MUHI] = hi;
+      if (fin) a.st[HFX_ST_MUDONE] = 1.0;
     }
   }
 }
@@ -634,16 +657,12 @@ __global__ __launch_bounds__(MUS_BLOCK) void mu_step_seg_kernel(MuSegArgs a) {
     if (mu_converged(g, Z, N, mu, Tj, lo, hi)) {
       st[HFX_ST_MUDONE] = 1.0;
     } else {
-      if (g < 0) lo = mu;
-      if (g > 0) hi = mu;
-      double mu_new = mu - g / fmax(Z, 1e-18);
-      const double mu_bis = 0.5 * (lo + hi);
-      if (mu_new <= lo || mu_new >= hi) mu_new = mu_bis;
-      mu_new = fmin(fmax(mu_new, lo), hi);
-      if (!isfinite(mu_new)) mu_new = mu_bis;
+      double mu_new = mu;
+      const bool fin = mu_newton_update(g, Z, mu_new, Tj, lo, hi);
       st[HFX_ST_MU] = mu_new;
       st[HFX_ST_MULO] = lo;
       st[HFX_ST_MUHI] = hi;
+      if (fin) st[HFX_ST_MUDONE] = 1.0;
     }
     *ticket = 0u;
   }
