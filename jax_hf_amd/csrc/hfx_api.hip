@@ -39,14 +39,19 @@ int hf_energy_launch(const hfx_plan* p, const void* P, const void* h, const void
                      int H_bcast, const double* w2d, double* partials, double* E_out, hipStream_t s);
 int energy_partials_max();
 
-// Radix plan for a length-N axis: 8s first, then 4, 2, 3, 5, 7 (Stockham
-// stages may come in any order).
+// Radix plan for a length-N axis: 8s first, then 4, 2, 3, 5, 7, then any
+// larger prime factor as a stage of its own (Stockham stages may come in any
+// order; the exchange kernels run a prime above 7 as an O(N p) generic stage).
 static int factorize(int N, int* rad, int* nst) {
   int n = N, k = 0;
   while (n % 8 == 0 && k < HFX_MAX_STAGES) { rad[k++] = 8; n /= 8; }
   const int rs[5] = {4, 2, 3, 5, 7};
   for (int q = 0; q < 5; ++q)
     while (n % rs[q] == 0 && k < HFX_MAX_STAGES) { rad[k++] = rs[q]; n /= rs[q]; }
+  for (int p = 11; n > 1 && k < HFX_MAX_STAGES; p += 2) {
+    if ((long long)p * p > n) p = n;   // what is left is prime
+    while (n % p == 0 && k < HFX_MAX_STAGES) { rad[k++] = p; n /= p; }
+  }
   *nst = k;
   return n == 1 ? 0 : -1;
 }
@@ -96,12 +101,15 @@ int hfx_plan_create(int nk1, int nk2, int nb, int dtype, hfx_plan** out) {
   int rad1[HFX_MAX_STAGES], rad2[HFX_MAX_STAGES], ns1 = 0, ns2 = 0;
   if (factorize(nk1, rad1, &ns1) || factorize(nk2, rad2, &ns2)) {
     delete p;
-    return set_error(HFX_EUNSUPPORTED, "hfx_plan_create: mesh (%d, %d) has a prime factor > 7", nk1, nk2);
+    return set_error(HFX_EUNSUPPORTED, "hfx_plan_create: mesh (%d, %d) needs more than %d FFT stages", nk1, nk2,
+                     HFX_MAX_STAGES);
   }
-  if (exchange_lds_elems(nk1, nk2) > exchange_max_plane(dtype)) {
+  // planes beyond LDS run the HBM-staged exchange, whose tiles hold one whole
+  // row (odd pitch) and one whole column of the plane
+  if (nk1 > exchange_max_plane(dtype) - 1 || nk2 > exchange_max_plane(dtype) - 1) {
     delete p;
-    return set_error(HFX_EUNSUPPORTED, "hfx_plan_create: mesh %dx%d exceeds the LDS-resident plane limit %d",
-                     nk1, nk2, exchange_max_plane(dtype));
+    return set_error(HFX_EUNSUPPORTED, "hfx_plan_create: mesh axis %d exceeds %d", nk1 > nk2 ? nk1 : nk2,
+                     exchange_max_plane(dtype) - 1);
   }
   std::vector<long double> r1, i1, r2, i2;
   twiddles(nk1, r1, i1);

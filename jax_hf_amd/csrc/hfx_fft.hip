@@ -154,6 +154,54 @@ __device__ __forceinline__ void fft_stage(cx<T>* buf, int nlines, int N, int lst
   __syncthreads();
 }
 
+// One Stockham stage of a runtime radix R (a prime factor above 7), computed
+// output by output.  Output m of butterfly j (k = j mod Ns) is
+//   y = sum_r x[j + r N/R] W_N^{r (k tstep + m N/R)},
+// the stage twiddle and the R-point DFT folded into one table entry per term
+// (W_R^{r m} = W_N^{r m N/R}); conjugated for the inverse.  O(N R) per line
+// instead of O(N log N): meant for the odd prime factor of an unusual mesh.
+template <typename T, int BLOCK, int MAXE>
+__device__ void fft_stage_any(cx<T>* buf, int nlines, int N, int lstride, int estride, bool lines_fast, int Ns,
+                              int R, const cx<T>* __restrict__ tw, bool inv) {
+  constexpr int MAXO = (MAXE + BLOCK - 1) / BLOCK;
+  const int NR = N / R, total = nlines * N, tstep = N / (Ns * R);
+  cx<T> y[MAXO];
+#pragma unroll
+  for (int m = 0; m < MAXO; ++m) {
+    const int it = threadIdx.x + m * BLOCK;
+    if (it < total) {
+      int line, q;
+      if (lines_fast) { line = it % nlines; q = it / nlines; }
+      else { line = it / N; q = it - line * N; }
+      // q = (j / Ns) Ns R + mm Ns + k with k < Ns, mm < R
+      const int k = q % Ns, t = q / Ns, mm = t % R, j = (t / R) * Ns + k;
+      const int step = (k * tstep + mm * NR) % N;
+      const cx<T>* src = buf + line * lstride;
+      cx<T> acc = src[j * estride];
+      int ex = 0;
+      for (int r = 1; r < R; ++r) {
+        ex += step;
+        if (ex >= N) ex -= N;
+        const cx<T> w = tw[ex], x = src[(j + r * NR) * estride];
+        acc = acc + (inv ? mulc(x, w) : x * w);
+      }
+      y[m] = acc;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < MAXO; ++m) {
+    const int it = threadIdx.x + m * BLOCK;
+    if (it < total) {
+      int line, q;
+      if (lines_fast) { line = it % nlines; q = it / nlines; }
+      else { line = it / N; q = it - line * N; }
+      buf[line * lstride + q * estride] = y[m];
+    }
+  }
+  __syncthreads();
+}
+
 template <typename T, int BLOCK, int MAXE>
 __device__ void fft_lines(cx<T>* buf, int nlines, int N, int lstride, int estride, bool lines_fast,
                           int nst, const int* rad, const cx<T>* __restrict__ tw, bool inv) {
@@ -167,7 +215,9 @@ __device__ void fft_lines(cx<T>* buf, int nlines, int N, int lstride, int estrid
       case 5: fft_stage<T, 5, BLOCK, MAXE>(buf, nlines, N, lstride, estride, lines_fast, Ns, tw, inv); break;
       case 7: fft_stage<T, 7, BLOCK, MAXE>(buf, nlines, N, lstride, estride, lines_fast, Ns, tw, inv); break;
       case 8: fft_stage<T, 8, BLOCK, MAXE>(buf, nlines, N, lstride, estride, lines_fast, Ns, tw, inv); break;
-      default: break;  // plan creation rejects other radices
+      default:
+        fft_stage_any<T, BLOCK, MAXE>(buf, nlines, N, lstride, estride, lines_fast, Ns, R, tw, inv);
+        break;
     }
     Ns *= R;
   }
@@ -204,14 +254,8 @@ __host__ __device__ constexpr int lds_pitch(int n2) { return (n2 % 2 == 0) ? n2 
 // Sigma pointers to this workgroup's solve (VR is shared by the batch).  In
 // the planar layout the pointers move to this workgroup's planes, after which
 // k-point e of the channel is element e (exch_ix / exch_vix).
-template <typename T, int BLOCK>
-__device__ __forceinline__ int exch_channel(ExchArgs<T>& a) {
-  int b = blockIdx.x;
-  const int planes = a.nchan * a.nsolve;
-  if (a.xcd_remap) {  // consecutive planes on one XCD (blocks round-robin over 8 XCDs)
-    const int per = planes >> 3;
-    b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  }
+template <typename T>
+__device__ __forceinline__ int exch_channel_at(ExchArgs<T>& a, int b) {
   const int s = b / a.nchan;
   const int c = a.chans[b - s * a.nchan];
   if (a.planar) {
@@ -227,6 +271,16 @@ __device__ __forceinline__ int exch_channel(ExchArgs<T>& a) {
     a.Sigma += off;
   }
   return c;
+}
+
+template <typename T, int BLOCK>
+__device__ __forceinline__ int exch_channel(ExchArgs<T>& a) {
+  int b = blockIdx.x;
+  if (a.xcd_remap) {  // consecutive planes on one XCD (blocks round-robin over 8 XCDs)
+    const int per = (a.nchan * a.nsolve) >> 3;
+    b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  }
+  return exch_channel_at(a, b);
 }
 
 // Slot planes: the plane's P / refP / Sigma offsets need only the plane index.
@@ -861,6 +915,179 @@ static int launch_v2(const ExchArgs<T>& a, hipStream_t s) {
   return check_launch("exch_v2_kernel");
 }
 
+// ---- HBM-staged exchange (planes larger than LDS) ---------------------------
+// Three passes over tiles that fit in LDS; the channel's own positions of the
+// output array hold the intermediate, so no workspace is needed:
+//   rows          a tile of k1-rows, forward along k2 (loads P - refP, or w V
+//                 in mode 1) -> X;
+//   columns       a tile of k2-columns, forward along k1, then (mode 0) the VR
+//                 multiply with -1/(n1 n2) and the inverse along k1, or (mode 1)
+//                 the ifftshift phase -> X (the result in mode 1);
+//   inverse rows  (mode 0) inverse along k2 -> Sigma, with the Hermitian mirror.
+// An ifftshift of the result permutes k1 in the column pass and k2 in the
+// inverse row pass, so each pass writes only its own tile's lines.  The
+// weighted channel moment is a separate per-plane reduction run first.
+// Traffic per channel: P in, VR in, 5 plane-sized transfers of the
+// intermediate, Sigma out (vs P + VR + Sigma for the LDS-resident kernels).
+struct XsTiles {
+  int tr, ntr;   // rows per row tile, row tiles per plane
+  int tc, ntc;   // columns per column tile, column tiles per plane
+};
+
+template <typename T, int BLOCK, int MAXE>
+__global__ __launch_bounds__(BLOCK) void exst_rows_kernel(ExchArgs<T> a, XsTiles t, int inv) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* tile = reinterpret_cast<cx<T>*>(smem_raw);
+  const int n1 = a.g.n1, n2 = a.g.n2, pitch = lds_pitch(n2), nb = a.g.nb, nb2 = nb * nb;
+  const int b = blockIdx.x / t.ntr, r0 = (blockIdx.x - b * t.ntr) * t.tr;
+  const int nr = min(t.tr, n1 - r0), cnt = nr * n2;
+  const int c = exch_channel_at(a, b);
+  for (int it = threadIdx.x; it < cnt; it += BLOCK) {
+    const int r = it / n2, k2 = it - r * n2;
+    const size_t e = (size_t)(r0 + r) * n2 + k2, ix = exch_ix(a, e, c, nb2);
+    cx<T> v;
+    if (inv) {
+      v = a.Sigma[ix];
+    } else {
+      v = a.P[ix];
+      if (a.refP) v = v - a.refP[ix];
+      if (a.mode == 1) v = (T)a.w2d[e] * v;   // w[q] V[q] (problem.py:88)
+    }
+    tile[r * pitch + k2] = v;
+  }
+  __syncthreads();
+  fft_lines<T, BLOCK, MAXE>(tile, nr, n2, pitch, 1, true, a.g.nst2, a.g.rad2, a.g.tw2, inv != 0);
+  const int ci = c / nb, cj = c - ci * nb, cm = cj * nb + ci, s2 = n2 / 2;
+  const bool mirror = inv && a.herm && ci != cj && !a.planar;
+  for (int it = threadIdx.x; it < cnt; it += BLOCK) {
+    const int r = it / n2, k2 = it - r * n2;
+    const int o2 = (inv && a.shift) ? (k2 - s2 + n2) % n2 : k2;   // ifftshift along k2
+    const size_t e = (size_t)(r0 + r) * n2 + o2;
+    const cx<T> v = tile[r * pitch + k2];
+    a.Sigma[exch_ix(a, e, c, nb2)] = v;
+    if (mirror) a.Sigma[e * nb2 + cm] = conj(v);
+  }
+}
+
+template <typename T, int BLOCK, int MAXE>
+__global__ __launch_bounds__(BLOCK) void exst_cols_kernel(ExchArgs<T> a, XsTiles t) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* tile = reinterpret_cast<cx<T>*>(smem_raw);
+  const int n1 = a.g.n1, n2 = a.g.n2, nb2 = a.g.nb * a.g.nb;
+  const int b = blockIdx.x / t.ntc, c0 = (blockIdx.x - b * t.ntc) * t.tc;
+  const int nc = min(t.tc, n2 - c0), cp = lds_pitch(nc), cnt = n1 * nc;
+  const int c = exch_channel_at(a, b);
+  for (int it = threadIdx.x; it < cnt; it += BLOCK) {
+    const int k1 = it / nc, q = it - k1 * nc;
+    tile[k1 * cp + q] = a.Sigma[exch_ix(a, (size_t)k1 * n2 + c0 + q, c, nb2)];
+  }
+  __syncthreads();
+  fft_lines<T, BLOCK, MAXE>(tile, nc, n1, 1, cp, true, a.g.nst1, a.g.rad1, a.g.tw1, false);
+  if (a.mode == 0) {
+    const T scale = (T)(-1.0 / ((double)n1 * n2));
+    for (int it = threadIdx.x; it < cnt; it += BLOCK) {
+      const int k1 = it / nc, q = it - k1 * nc;
+      const cx<T> vr = a.VR[exch_vix(a, (size_t)k1 * n2 + c0 + q, c, nb2)];
+      cx<T>& x = tile[k1 * cp + q];
+      x = scale * (a.conjv ? mulc(x, vr) : x * vr);
+    }
+    __syncthreads();
+    fft_lines<T, BLOCK, MAXE>(tile, nc, n1, 1, cp, true, a.g.nst1, a.g.rad1, a.g.tw1, true);
+  }
+  const int s1 = n1 / 2, s2 = n2 / 2;
+  for (int it = threadIdx.x; it < cnt; it += BLOCK) {
+    const int k1 = it / nc, q = it - k1 * nc, k2 = c0 + q;
+    cx<T> v = tile[k1 * cp + q];
+    int o1 = k1;
+    if (a.mode == 1) {
+      if (a.shift) {  // VR * exp(2 pi i (k1 s1 / n1 + k2 s2 / n2))  (problem.py:90-98)
+        const cx<T> ph = a.g.tw1[(int)(((long long)k1 * s1) % n1)] * a.g.tw2[(int)(((long long)k2 * s2) % n2)];
+        v = mulc(v, ph);
+      }
+    } else if (a.shift) {
+      o1 = (k1 - s1 + n1) % n1;   // ifftshift along k1
+    }
+    a.Sigma[exch_ix(a, (size_t)o1 * n2 + k2, c, nb2)] = v;
+  }
+}
+
+// Weighted channel moment rho_c = sum_k w_k (P - refP)_c(k) of the staged path
+// (the LDS kernels fuse it into their load), one workgroup per plane.
+template <typename T, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void exst_moment_kernel(ExchArgs<T> a) {
+  __shared__ double red[2 * (BLOCK / 64)];
+  const int c = exch_channel_at(a, blockIdx.x);
+  const int nb = a.g.nb, nb2 = nb * nb, N = a.g.n1 * a.g.n2;
+  double acc[2] = {0.0, 0.0};
+  for (int e = threadIdx.x; e < N; e += BLOCK) {
+    const size_t ix = exch_ix(a, (size_t)e, c, nb2);
+    cx<T> v = a.P[ix];
+    if (a.refP) v = v - a.refP[ix];
+    const double w = a.w2d[e];
+    acc[0] += w * (double)v.re;
+    acc[1] += w * (double)v.im;
+  }
+  block_sum<2>(acc, red);
+  if (threadIdx.x == 0) {
+    const int ci = c / nb, cj = c - ci * nb;
+    a.rho[c] = cx<double>{acc[0], acc[1]};
+    if (a.herm && ci != cj) a.rho[cj * nb + ci] = cx<double>{acc[0], -acc[1]};
+  }
+}
+
+// Largest mesh axis the staged path takes: one row (odd pitch) and one column
+// of the plane must fit in a tile.
+template <typename T>
+constexpr int staged_max_axis() { return (sizeof(T) == 8 ? MAXE_F64 : MAXE_F32) - 1; }
+
+template <typename T, int BLOCK, int MAXE>
+static int launch_staged_form(const ExchArgs<T>& a, hipStream_t s) {
+  const int n1 = a.g.n1, n2 = a.g.n2, planes = a.nchan * a.nsolve, p2 = lds_pitch(n2);
+  XsTiles t;
+  t.tr = max(1, min(n1, MAXE / p2));
+  t.ntr = (n1 + t.tr - 1) / t.tr;
+  int tc = min(16, n2);
+  while (tc > 1 && n1 * lds_pitch(tc) > MAXE) --tc;
+  t.tc = tc;
+  t.ntc = (n2 + tc - 1) / tc;
+  const size_t rows_lds = (size_t)t.tr * p2 * sizeof(cx<T>);
+  const size_t cols_lds = (size_t)n1 * lds_pitch(tc) * sizeof(cx<T>);
+  auto krows = exst_rows_kernel<T, BLOCK, MAXE>;
+  auto kcols = exst_cols_kernel<T, BLOCK, MAXE>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    HFX_HIP_TRY(hipFuncSetAttribute((const void*)krows, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(MAXE * sizeof(cx<T>))));
+    HFX_HIP_TRY(hipFuncSetAttribute((const void*)kcols, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(MAXE * sizeof(cx<T>))));
+    attr_set = true;
+  }
+  if (a.mode == 0 && a.rho) {
+    hipLaunchKernelGGL((exst_moment_kernel<T, 256>), dim3(planes), dim3(256), 0, s, a);
+    if (int rc = check_launch("exst_moment_kernel")) return rc;
+  }
+  hipLaunchKernelGGL(krows, dim3((unsigned)planes * t.ntr), dim3(BLOCK), rows_lds, s, a, t, 0);
+  if (int rc = check_launch("exst_rows_kernel")) return rc;
+  hipLaunchKernelGGL(kcols, dim3((unsigned)planes * t.ntc), dim3(BLOCK), cols_lds, s, a, t);
+  if (int rc = check_launch("exst_cols_kernel")) return rc;
+  if (a.mode == 0) {
+    hipLaunchKernelGGL(krows, dim3((unsigned)planes * t.ntr), dim3(BLOCK), rows_lds, s, a, t, 1);
+    return check_launch("exst_rows_kernel");
+  }
+  return HFX_OK;
+}
+
+template <typename T>
+static int launch_staged(const ExchArgs<T>& a, hipStream_t s) {
+  constexpr int MAXE = sizeof(T) == 8 ? MAXE_F64 : MAXE_F32;
+  constexpr int SMALL = 2048;   // tile elements of the 256-thread form (32 KiB in FP64)
+  const int n1 = a.g.n1, n2 = a.g.n2;
+  if (n1 > staged_max_axis<T>() || n2 > staged_max_axis<T>())
+    return set_error(HFX_EUNSUPPORTED, "exchange: mesh axis beyond %d", staged_max_axis<T>());
+  if (lds_pitch(n2) <= SMALL && n1 <= SMALL) return launch_staged_form<T, 256, SMALL>(a, s);
+  return launch_staged_form<T, EXCH_BLOCK, MAXE>(a, s);
+}
+
 template <typename T>
 static int launch_exch(const ExchArgs<T>& a, int variant, hipStream_t s) {
   constexpr int MAXE = sizeof(T) == 8 ? MAXE_F64 : MAXE_F32;
@@ -889,6 +1116,7 @@ static int launch_exch(const ExchArgs<T>& a, int variant, hipStream_t s) {
       default: break;
     }
   }
+  if (exchange_lds_elems(n1, n2) > MAXE) return launch_staged<T>(a, s);
   const size_t lds = (size_t)n1 * lds_pitch(n2) * sizeof(cx<T>);
   auto kern = exch_conv_kernel<T, EXCH_BLOCK, MAXE>;
   static bool attr_set = false;

@@ -5,9 +5,12 @@ Mirrors ``src/jax_hf/fock.py:18-129`` (``build_fock``, ``hf_energy``,
 Arrays are torch device tensors (numpy inputs are uploaded); every arithmetic
 step runs in libhfx HIP kernels:
 
-* exchange: fused LDS-resident 2D FFT convolution per (i, j) channel,
-  optionally on the Hermitian upper-triangle channel set or on block-diagonal
-  channel subsets.  The Fock build stages the channels as slot planes
+* exchange: fused 2D FFT convolution per (i, j) channel, optionally on the
+  Hermitian upper-triangle channel set or on block-diagonal channel subsets;
+  any mesh: a channel plane that fits in LDS is transformed there in one pass
+  (compile-time kernels for the common square meshes, a runtime-radix kernel
+  otherwise, prime factors above 7 as a generic radix stage), a larger plane
+  by three HBM-staged passes over row / column tiles.  The Fock build stages the channels as slot planes
   (``hfx_planar_pack``), runs the exchange on them (``hfx_selfenergy_planar``)
   and reads Sigma back in the epilogue (``hfx_fock_epilogue_planar``), so
   every exchange access is coalesced; ``PLANAR = False`` selects the

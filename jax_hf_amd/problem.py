@@ -3,8 +3,8 @@
 Mirrors ``HartreeFockKernel`` of ``src/jax_hf/problem.py:14-217``: same
 constructor arguments, attributes and ``as_args()``.  The interaction
 transform ``VR = fftn(w * Vq)`` and its ifftshift phase (problem.py:88-98) are
-computed once on the GPU by the same LDS-resident FFT kernel that runs the
-exchange convolution (``hfx_prepare_vr``), always in FP64 (a complex64 kernel
+computed once on the GPU by the same FFT kernels that run the exchange
+convolution (``hfx_prepare_vr``), always in FP64 (a complex64 kernel
 receives the FP64 result rounded once).
 """
 

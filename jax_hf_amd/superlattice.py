@@ -7,12 +7,11 @@ independent convolution per (dG = G_a - G_b, alpha, beta) channel
 tables on the host and the FFT of the Cartesian Coulomb kernel on the lag grid
 on the GPU; the returned functions run ``hfx_superlattice_fock``: scatter of
 every channel block onto the extended grid, one planar exchange launch over
-all n_delta * dim_orb^2 planes (the LDS-resident FFT convolution of the Fock
-build), and a gather that adds the k-independent Hartree shift and forms
+all n_delta * dim_orb^2 planes (the FFT convolution of the Fock build), and a gather that adds the k-independent Hartree shift and forms
 F = herm(h + Sigma).  The reference streams one dG channel at a time to bound
 device memory at O(N_ext^2 dim_orb^2); with 288 GB of HBM all channels fit at
-once for any layout the LDS-resident FFT accepts, and one launch over all
-planes fills the GPU.
+once, and one launch over all planes fills the GPU (extended grids beyond the
+LDS plane run the HBM-staged exchange).
 """
 
 from __future__ import annotations
