@@ -206,7 +206,8 @@ int hfx_absmax_masked(const void* X, size_t nmat, int nb, int dtype, const unsig
 int hfx_entropy(const void* p, int real_kind, const double* w, size_t nk, int nb, const double* E, double T,
                 double* scratch, double* out, void* stream);
 
-/* ---- per-k batched kernels (nb <= 16; one wavefront per k-point) --------- */
+/* ---- per-k batched kernels (nb <= 32: one wavefront per k-point up to nb = 16,
+ * a 256-thread workgroup per k-point above) ----------------------------- */
 /* Device scalar state of the solver loops: float64[HFX_ST_LEN]. */
 #define HFX_ST_MU 0
 #define HFX_ST_BETA 1
@@ -428,7 +429,7 @@ int hfx_solve_mu(const double* eps, const double* w, size_t nk, int nb, double t
 /* QR retraction (_qr_retract, solver.py:108-119): U(k) = Q of
  * I - tau[k] G(k) with the phases normalised so that R has a positive real
  * diagonal (the unique such Q); G and U (nk, nb, nb), U may alias G, tau (nk)
- * float64.  nb <= 16. */
+ * float64.  nb <= 32. */
 int hfx_qr_retract(int dtype, int nb, size_t nk, const void* G, const double* tau, void* U, void* stream);
 
 /* No counterpart in the reference.  hfx_solve_mu without the scratch argument,

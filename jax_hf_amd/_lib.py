@@ -21,7 +21,10 @@ LIB_PATH = _HERE / "libhfx.so"
 
 HFX_C128 = 0
 HFX_C64 = 1
-ABI_VERSION = 13
+ABI_VERSION = 14
+# Largest band count of the per-k solver kernels (nb <= 16: one wavefront per
+# k-point, nb <= 32: a 256-thread workgroup per k-point).
+PERK_NB_MAX = 32
 
 _lib = None
 _lock = threading.Lock()

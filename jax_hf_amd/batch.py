@@ -47,7 +47,7 @@ def _batchable(kernel, cfg):
     return (not kernel.include_hartree and kernel.include_exchange and kernel._refP is None and not contact
             and cfg.block_sizes is None and cfg.project_fn is None
             and str(cfg.optimizer).lower() == "cg" and str(cfg.cayley).lower() == "gauss_jordan"
-            and kernel.h.shape[-1] <= 16)
+            and kernel.h.shape[-1] <= L.PERK_NB_MAX)
 
 
 def _group_key(kernel):

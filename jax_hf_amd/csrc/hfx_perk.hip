@@ -14,30 +14,57 @@
 
 namespace hfx {
 
-constexpr int PK_NBMAX = 16;
-constexpr int PK_PITCH = PK_NBMAX + 1;
-constexpr int PK_MAT = PK_NBMAX * PK_PITCH;
+// Size class NB (16 or 32): the largest nb one kernel instance takes.  NB = 16
+// runs one 64-lane wavefront per k-point (with the matrix-core products at
+// nb = 16); NB = 32 a 256-thread workgroup per k-point.  Either way every
+// thread owns PK_EPL <= 4 matrix elements and a staged matrix has pitch NB + 1.
+// The macros below are written in terms of the template parameter NB of the
+// function they appear in.
+#define PK_NBMAX (NB)
+#define PK_PITCH ((NB) + 1)
+#define PK_MAT ((NB) * ((NB) + 1))
+#define PKB ((NB) <= 16 ? 64 : 256)
+#define PK_EPL (((NB) * (NB) + PKB - 1) / PKB)
 
 // state vector of the solve that k-point k belongs to (seg_len = 0: one solve)
 __device__ __forceinline__ const double* seg_state(const double* st, size_t k, int seg_len) {
   return seg_len > 0 ? st + (k / (size_t)seg_len) * HFX_ST_LEN : st;
-}   // LDS elements per staged matrix
-constexpr int PK_EPL = (PK_NBMAX * PK_NBMAX + 63) / 64;
+}
+
+// Sum over the workgroup of one value per thread (deterministic order; the
+// result is valid in every thread).  Every thread of the workgroup must call it.
+template <int BLOCK>
+__device__ __forceinline__ double pk_sum(double v) {
+  v = wave_sum(v);
+  if constexpr (BLOCK == 64) {
+    return v;
+  } else {
+    __shared__ double sh[BLOCK / 64];
+    __syncthreads();   // the previous call's readers are done with sh
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) t += sh[w];
+    return t;
+  }
+}
+#define PK_SUM(x) pk_sum<PKB>(x)
 
 #define PK_FOR_ELEM(nb, ...)                                    \
   _Pragma("unroll") for (int m_ = 0; m_ < PK_EPL; ++m_) {       \
-    const int e = threadIdx.x + 64 * m_;                        \
+    const int e = threadIdx.x + PKB * m_;                        \
     if (e < (nb) * (nb)) {                                      \
       const int i = e / (nb), j = e - i * (nb);                 \
       __VA_ARGS__                                               \
     }                                                           \
   }
 
-template <typename T>
+template <int NB, typename T>
 __device__ __forceinline__ void m_load(cx<T>* s, const cx<T>* __restrict__ g, int nb) {
   PK_FOR_ELEM(nb, { s[i * PK_PITCH + j] = g[e]; })
 }
-template <typename T>
+template <int NB, typename T>
 __device__ __forceinline__ void m_store(cx<T>* __restrict__ g, const cx<T>* s, int nb) {
   PK_FOR_ELEM(nb, { g[e] = s[i * PK_PITCH + j]; })
 }
@@ -80,7 +107,7 @@ struct MAcc {
 };
 
 // acc += sgn * op(A) op(B) for 16 x 16 LDS matrices (pitch PK_PITCH); all 64 lanes.
-template <typename T, bool AH, bool BH>
+template <int NB, typename T, bool AH, bool BH>
 __device__ __forceinline__ void mf_gemm16(MAcc<T>& acc, const cx<T>* A, const cx<T>* B, T sgn = (T)1) {
   const int l = threadIdx.x & 63, x = l & 15, kq = l >> 4;
 #pragma unroll
@@ -97,7 +124,7 @@ __device__ __forceinline__ void mf_gemm16(MAcc<T>& acc, const cx<T>* A, const cx
   }
 }
 
-template <typename T>
+template <int NB, typename T>
 __device__ __forceinline__ void mf_store16(cx<T>* C, const MAcc<T>& acc) {
   const int l = threadIdx.x & 63;
 #pragma unroll
@@ -106,15 +133,17 @@ __device__ __forceinline__ void mf_store16(cx<T>* C, const MAcc<T>& acc) {
 
 // C = op(A) op(B); op = conjugate transpose when the flag is set.  C must not
 // alias A or B.  Ends with a barrier.
-template <typename T, bool AH, bool BH>
+template <int NB, typename T, bool AH, bool BH>
 __device__ __forceinline__ void m_gemm(cx<T>* C, const cx<T>* A, const cx<T>* B, int nb) {
-  if (nb == PK_NBMAX) {
-    MAcc<T> acc;
-    acc.zero();
-    mf_gemm16<T, AH, BH>(acc, A, B);
-    mf_store16(C, acc);
-    __syncthreads();
-    return;
+  if constexpr (NB == 16) {
+    if (nb == 16) {
+      MAcc<T> acc;
+      acc.zero();
+      mf_gemm16<NB, T, AH, BH>(acc, A, B);
+      mf_store16<NB>(C, acc);
+      __syncthreads();
+      return;
+    }
   }
   PK_FOR_ELEM(nb, {
     cx<T> acc{0, 0};
@@ -140,7 +169,7 @@ __device__ __forceinline__ cx<T> cayley(double lam, double tau) {
 // diag(U^dagger Ft U) from the spectral form (solver.py:185-197):
 // eps_b = Re sum_j (V M)_bj conj(V_bj),  M = conj(c) (x) Ft_eig (x) c.
 // V and Fe are LDS matrices, c per column in cs[], scratch W (LDS).
-template <typename T>
+template <int NB, typename T>
 __device__ __forceinline__ void spectral_diag(const cx<T>* V, const cx<T>* Fe, const cx<T>* cs, cx<T>* W,
                                               double* outd, int nb) {
   PK_FOR_ELEM(nb, { W[i * PK_PITCH + j] = conj(cs[i]) * Fe[i * PK_PITCH + j] * cs[j]; })
@@ -166,7 +195,7 @@ __device__ __forceinline__ void spectral_diag(const cx<T>* V, const cx<T>* Fe, c
 // For pair (p, q) with c = A_pq = |c| e^{i phi}: zeta = (A_qq - A_pp)/(2|c|),
 // t = sign(zeta)/(|zeta| + sqrt(1 + zeta^2)), cs = 1/sqrt(1+t^2), sn = t cs,
 // J_pp = J_qq = cs, J_pq = sn e^{i phi}, J_qp = -sn e^{-i phi}; A <- J^H A J.
-template <typename T>
+template <int NB, typename T>
 __device__ void m_eigh(cx<T>* A, cx<T>* V, cx<T>* Vt, double* lam, int nb, double* red) {
   __shared__ int partner[PK_NBMAX];
   __shared__ T jd[PK_NBMAX];
@@ -196,7 +225,7 @@ __device__ void m_eigh(cx<T>* A, cx<T>* V, cx<T>* Vt, double* lam, int nb, doubl
       const cx<T> a = A[i * PK_PITCH + j];
       tot += a.re * a.re + a.im * a.im;
     })
-    const double tot_s = wave_sum((double)tot);
+    const double tot_s = PK_SUM((double)tot);
     floor_abs = (T)(0.01 * sqrt(tot_s / (double)nb));
   }
   for (int sweep = 0; sweep < 30 && nb > 1; ++sweep) {
@@ -235,7 +264,9 @@ __device__ void m_eigh(cx<T>* A, cx<T>* V, cx<T>* Vt, double* lam, int nb, doubl
         }
       }
       __syncthreads();
-      const unsigned long long rot = __ballot((int)threadIdx.x < nb && rf[threadIdx.x] != 0);
+      // every wave forms the same mask from the table (lane l reads rf[l])
+      const int ln = threadIdx.x & 63;
+      const unsigned long long rot = __ballot(ln < nb && rf[ln] != 0);
       if (rot == 0ull) continue;   // uniform: nothing rotates in this step
       any = true;
       // A'_ij = sum_{a in {i,i'}, b in {j,j'}} conj(J_ai) A_ab J_bj ;  V' = V J
@@ -300,13 +331,13 @@ __device__ void m_eigh(cx<T>* A, cx<T>* V, cx<T>* Vt, double* lam, int nb, doubl
 
 // Batched eigh: A (nk, nb, nb) Hermitian -> lam (nk, nb) ascending, V (nk, nb, nb).
 // herm_in: hermitise the input first (reference_scf.py:89 eigh(_herm(F))).
-template <typename T>
-__global__ __launch_bounds__(64) void pk_eigh_kernel(int nb, const cx<T>* __restrict__ A, double* __restrict__ lam,
+template <int NB, typename T>
+__global__ __launch_bounds__(PKB) void pk_eigh_kernel(int nb, const cx<T>* __restrict__ A, double* __restrict__ lam,
                                                      cx<T>* __restrict__ V, int herm_in) {
   __shared__ cx<T> sA[PK_MAT], sV[PK_MAT], sT[PK_MAT];
   __shared__ double sl[PK_NBMAX];
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
-  m_load(sA, A + k * nb2, nb);
+  m_load<NB>(sA, A + k * nb2, nb);
   __syncthreads();
   if (herm_in) {
     PK_FOR_ELEM(nb, { sT[i * PK_PITCH + j] = (T)0.5 * (sA[i * PK_PITCH + j] + conj(sA[j * PK_PITCH + i])); })
@@ -314,15 +345,15 @@ __global__ __launch_bounds__(64) void pk_eigh_kernel(int nb, const cx<T>* __rest
     PK_FOR_ELEM(nb, { sA[i * PK_PITCH + j] = sT[i * PK_PITCH + j]; })
     __syncthreads();
   }
-  m_eigh(sA, sV, sT, sl, nb, nullptr);
+  m_eigh<NB>(sA, sV, sT, sl, nb, nullptr);
   if ((int)threadIdx.x < nb) lam[k * nb + threadIdx.x] = sl[threadIdx.x];
-  m_store(V + k * nb2, sV, nb);
+  m_store<NB>(V + k * nb2, sV, nb);
 }
 
 // P = herm(Q diag(p) Q^dagger)  (solver.py:42-44, 462; utils.py:473-474)
 // p from pvec, or (when pvec is NULL) p = sigmoid((mu - eps)/T) and written to pout.
-template <typename T>
-__global__ __launch_bounds__(64) void pk_density_kernel(int nb, const cx<T>* __restrict__ Q,
+template <int NB, typename T>
+__global__ __launch_bounds__(PKB) void pk_density_kernel(int nb, const cx<T>* __restrict__ Q,
                                                         const double* __restrict__ pvec,
                                                         const double* __restrict__ eps, const double* __restrict__ mu,
                                                         double T_, double* __restrict__ pout,
@@ -339,30 +370,30 @@ __global__ __launch_bounds__(64) void pk_density_kernel(int nb, const cx<T>* __r
     }
     sp[threadIdx.x] = pv;
   }
-  m_load(sQ, Q + k * nb2, nb);
+  m_load<NB>(sQ, Q + k * nb2, nb);
   __syncthreads();
   PK_FOR_ELEM(nb, { sX[i * PK_PITCH + j] = (T)sp[j] * sQ[i * PK_PITCH + j]; })
   __syncthreads();
-  m_gemm<T, false, true>(sP, sX, sQ, nb);
+  m_gemm<NB, T, false, true>(sP, sX, sQ, nb);
   PK_FOR_ELEM(nb, { P[k * nb2 + e] = (T)0.5 * (sP[i * PK_PITCH + j] + conj(sP[j * PK_PITCH + i])); })
 }
 
 // CG step 1 (solver.py:472-491): Ft = Q^dagger F Q, eps = Re diag Ft,
 // G = skew((p_j - p_i) Ft) offdiag.  part[0][k] = sum |G|^2, part[1][k] = sum eps^2.
-template <typename T>
-__global__ __launch_bounds__(64) void pk_cg_grad_kernel(int nb, size_t Nk, const cx<T>* __restrict__ Q,
+template <int NB, typename T>
+__global__ __launch_bounds__(PKB) void pk_cg_grad_kernel(int nb, size_t Nk, const cx<T>* __restrict__ Q,
                                                         const cx<T>* __restrict__ F, const double* __restrict__ p,
                                                         cx<T>* __restrict__ Ft, double* __restrict__ eps,
                                                         cx<T>* __restrict__ G, double* __restrict__ part) {
   __shared__ cx<T> sQ[PK_MAT], sF[PK_MAT], sX[PK_MAT], sT[PK_MAT];
   __shared__ double sp[PK_NBMAX];
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
-  m_load(sQ, Q + k * nb2, nb);
-  m_load(sF, F + k * nb2, nb);
+  m_load<NB>(sQ, Q + k * nb2, nb);
+  m_load<NB>(sF, F + k * nb2, nb);
   if ((int)threadIdx.x < nb) sp[threadIdx.x] = p[k * nb + threadIdx.x];
   __syncthreads();
-  m_gemm<T, false, false>(sX, sF, sQ, nb);
-  m_gemm<T, true, false>(sT, sQ, sX, nb);
+  m_gemm<NB, T, false, false>(sX, sF, sQ, nb);
+  m_gemm<NB, T, true, false>(sT, sQ, sX, nb);
   double g2 = 0.0, e2 = 0.0;
   PK_FOR_ELEM(nb, {
     const cx<T> fij = sT[i * PK_PITCH + j];
@@ -380,8 +411,8 @@ __global__ __launch_bounds__(64) void pk_cg_grad_kernel(int nb, size_t Nk, const
       e2 += (double)fij.re * (double)fij.re;
     }
   })
-  g2 = wave_sum(g2);
-  e2 = wave_sum(e2);
+  g2 = PK_SUM(g2);
+  e2 = PK_SUM(e2);
   if (threadIdx.x == 0) {
     part[k] = g2;
     part[Nk + k] = e2;
@@ -391,8 +422,8 @@ __global__ __launch_bounds__(64) void pk_cg_grad_kernel(int nb, size_t Nk, const
 // CG step 2 (solver.py:486, 493-519): preconditioned gradients and the six
 // Polak-Ribiere inner products per k (weights applied in the reduction).
 // st: device scalar state (mu, lam).  Outputs H, gp, hp; part[0..5][k].
-template <typename T>
-__global__ __launch_bounds__(64) void pk_cg_precond_kernel(int nb, size_t Nk, const cx<T>* __restrict__ G,
+template <int NB, typename T>
+__global__ __launch_bounds__(PKB) void pk_cg_precond_kernel(int nb, size_t Nk, const cx<T>* __restrict__ G,
                                                            const cx<T>* __restrict__ Gprev,
                                                            const double* __restrict__ eps,
                                                            const double* __restrict__ p,
@@ -430,12 +461,12 @@ __global__ __launch_bounds__(64) void pk_cg_precond_kernel(int nb, size_t Nk, co
     v_ph = gq * h;
     v_pp = gq * gq;
   }
-  s_gh = wave_sum(s_gh);
-  s_ph = wave_sum(s_ph);
-  s_pp = wave_sum(s_pp);
-  v_gh = wave_sum(v_gh);
-  v_ph = wave_sum(v_ph);
-  v_pp = wave_sum(v_pp);
+  s_gh = PK_SUM(s_gh);
+  s_ph = PK_SUM(s_ph);
+  s_pp = PK_SUM(s_pp);
+  v_gh = PK_SUM(v_gh);
+  v_ph = PK_SUM(v_ph);
+  v_pp = PK_SUM(v_pp);
   if (threadIdx.x == 0) {
     part[0 * Nk + k] = s_gh;
     part[1 * Nk + k] = s_ph;
@@ -449,8 +480,8 @@ __global__ __launch_bounds__(64) void pk_cg_precond_kernel(int nb, size_t Nk, co
 // CG step 3 (solver.py:531-546): d = H + beta d_prev (in place in D),
 // dp likewise, eigh(herm(i d)) -> V, lam_d, Ft_eig = V^dagger Ft V,
 // part[k] = sum |d|^2.
-template <typename T>
-__global__ __launch_bounds__(64) void pk_cg_direction_kernel(int nb, const cx<T>* __restrict__ H,
+template <int NB, typename T>
+__global__ __launch_bounds__(PKB) void pk_cg_direction_kernel(int nb, const cx<T>* __restrict__ H,
                                                              cx<T>* __restrict__ D, const double* __restrict__ hp,
                                                              double* __restrict__ dp, const cx<T>* __restrict__ Ft,
                                                              const double* __restrict__ st, cx<T>* __restrict__ Vout,
@@ -473,7 +504,7 @@ __global__ __launch_bounds__(64) void pk_cg_direction_kernel(int nb, const cx<T>
     const size_t q = k * nb + threadIdx.x;
     dp[q] = hp[q] + (double)beta * dp[q];
   }
-  m_load(sF, Ft + k * nb2, nb);
+  m_load<NB>(sF, Ft + k * nb2, nb);
   __syncthreads();
   // herm(i d): x = i d_ij, y = i d_ji, 0.5 (x + conj(y))  (solver.py:163-164)
   PK_FOR_ELEM(nb, {
@@ -482,13 +513,13 @@ __global__ __launch_bounds__(64) void pk_cg_direction_kernel(int nb, const cx<T>
     sA[i * PK_PITCH + j] = (T)0.5 * (x + conj(y));
   })
   __syncthreads();
-  m_eigh(sA, sV, sT, sl, nb, nullptr);
+  m_eigh<NB>(sA, sV, sT, sl, nb, nullptr);
   if ((int)threadIdx.x < nb) lam_d[k * nb + threadIdx.x] = sl[threadIdx.x];
-  m_store(Vout + k * nb2, sV, nb);
-  m_gemm<T, false, false>(sT, sF, sV, nb);
-  m_gemm<T, true, false>(sA, sV, sT, nb);
-  m_store(Fe + k * nb2, sA, nb);
-  dn = wave_sum(dn);
+  m_store<NB>(Vout + k * nb2, sV, nb);
+  m_gemm<NB, T, false, false>(sT, sF, sV, nb);
+  m_gemm<NB, T, true, false>(sA, sV, sT, nb);
+  m_store<NB>(Fe + k * nb2, sA, nb);
+  dn = PK_SUM(dn);
   if (threadIdx.x == 0) part[k] = dn;
 }
 
@@ -496,8 +527,8 @@ __global__ __launch_bounds__(64) void pk_cg_direction_kernel(int nb, const cx<T>
 // part[0][k] = sum_b p_t eps_t, part[1][k] = sum_b [p_t ln p_t + (1-p_t) ln(1-p_t)].
 // With omega0 != 0 also the tau = 0 point (solver.py:559-561) into part[2], part[3].
 // Skipped (no writes) once st[HFX_ST_ACC] is set.
-template <typename T>
-__global__ __launch_bounds__(64) void pk_cg_trial_kernel(int nb, size_t Nk, const cx<T>* __restrict__ V,
+template <int NB, typename T>
+__global__ __launch_bounds__(PKB) void pk_cg_trial_kernel(int nb, size_t Nk, const cx<T>* __restrict__ V,
                                                          const double* __restrict__ lam_d,
                                                          const cx<T>* __restrict__ Fe, const double* __restrict__ p,
                                                          const double* __restrict__ dp,
@@ -509,11 +540,11 @@ __global__ __launch_bounds__(64) void pk_cg_trial_kernel(int nb, size_t Nk, cons
   __shared__ double et[PK_NBMAX];
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
   const double tau = st[HFX_ST_TAU];
-  m_load(sV, V + k * nb2, nb);
-  m_load(sF, Fe + k * nb2, nb);
+  m_load<NB>(sV, V + k * nb2, nb);
+  m_load<NB>(sF, Fe + k * nb2, nb);
   if ((int)threadIdx.x < nb) cs[threadIdx.x] = cayley<T>(lam_d[k * nb + threadIdx.x], tau);
   __syncthreads();
-  spectral_diag(sV, sF, cs, sW, et, nb);
+  spectral_diag<NB>(sV, sF, cs, sW, et, nb);
   double ef = 0.0, sf = 0.0, e0 = 0.0, s0 = 0.0;
   if ((int)threadIdx.x < nb) {
     const size_t q = k * nb + threadIdx.x;
@@ -528,10 +559,10 @@ __global__ __launch_bounds__(64) void pk_cg_trial_kernel(int nb, size_t Nk, cons
       s0 = x * log(x) + (1.0 - x) * log1p(-x);
     }
   }
-  ef = wave_sum(ef);
-  sf = wave_sum(sf);
-  e0 = wave_sum(e0);
-  s0 = wave_sum(s0);
+  ef = PK_SUM(ef);
+  sf = PK_SUM(sf);
+  e0 = PK_SUM(e0);
+  s0 = PK_SUM(s0);
   if (threadIdx.x == 0) {
     part[k] = ef;
     part[Nk + k] = sf;
@@ -544,8 +575,8 @@ __global__ __launch_bounds__(64) void pk_cg_trial_kernel(int nb, size_t Nk, cons
 
 // CG retraction (solver.py:586-590): U = V diag(c) V^dagger, Q <- Q U,
 // eps_new = diag(U^dagger Ft U) (spectral), tau from st[HFX_ST_TAUF].
-template <typename T>
-__global__ __launch_bounds__(64) void pk_cg_retract_kernel(int nb, cx<T>* __restrict__ Q, const cx<T>* __restrict__ V,
+template <int NB, typename T>
+__global__ __launch_bounds__(PKB) void pk_cg_retract_kernel(int nb, cx<T>* __restrict__ Q, const cx<T>* __restrict__ V,
                                                            const double* __restrict__ lam_d,
                                                            const cx<T>* __restrict__ Fe, const double* __restrict__ st,
                                                            double tau_override, int use_override,
@@ -555,20 +586,20 @@ __global__ __launch_bounds__(64) void pk_cg_retract_kernel(int nb, cx<T>* __rest
   __shared__ double et[PK_NBMAX];
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
   const double tau = use_override ? tau_override : st[HFX_ST_TAUF];
-  m_load(sQ, Q + k * nb2, nb);
-  m_load(sV, V + k * nb2, nb);
+  m_load<NB>(sQ, Q + k * nb2, nb);
+  m_load<NB>(sV, V + k * nb2, nb);
   if ((int)threadIdx.x < nb) cs[threadIdx.x] = cayley<T>(lam_d[k * nb + threadIdx.x], tau);
   __syncthreads();
   PK_FOR_ELEM(nb, { sX[i * PK_PITCH + j] = sV[i * PK_PITCH + j] * cs[j]; })
   __syncthreads();
-  m_gemm<T, false, true>(sU, sX, sV, nb);     // U
-  m_gemm<T, false, false>(sX, sQ, sU, nb);    // Q U
+  m_gemm<NB, T, false, true>(sU, sX, sV, nb);     // U
+  m_gemm<NB, T, false, false>(sX, sQ, sU, nb);    // Q U
   if constexpr (sizeof(T) == 4) {
     // FP32: V (and so U) is unitary only to ~1e-7, and the error would build up
     // over the iterations (the density then holds more or fewer than n electrons
     // and the energy drifts).  One Newton-Schulz step Q <- Q (3 I - Q^dagger Q) / 2
     // squares the deviation from unitarity, holding Q at FP32 rounding.
-    m_gemm<T, true, false>(sQ, sX, sX, nb);   // S = X^dagger X  (sQ is free here)
+    m_gemm<NB, T, true, false>(sQ, sX, sX, nb);   // S = X^dagger X  (sQ is free here)
     PK_FOR_ELEM(nb, {
       const cx<T> v = sQ[i * PK_PITCH + j];
       sQ[i * PK_PITCH + j] = cx<T>{(i == j ? (T)1.5 : (T)0) - (T)0.5 * v.re, (T)-0.5 * v.im};
@@ -581,20 +612,20 @@ __global__ __launch_bounds__(64) void pk_cg_retract_kernel(int nb, cx<T>* __rest
     })
     __syncthreads();
   } else {
-    m_store(Q + k * nb2, sX, nb);
+    m_store<NB>(Q + k * nb2, sX, nb);
   }
   if (eps_new) {
-    m_load(sQ, Fe + k * nb2, nb);             // reuse sQ for Ft_eig
+    m_load<NB>(sQ, Fe + k * nb2, nb);             // reuse sQ for Ft_eig
     __syncthreads();
-    spectral_diag(sV, sQ, cs, sX, et, nb);
+    spectral_diag<NB>(sV, sQ, cs, sX, et, nb);
     if ((int)threadIdx.x < nb) eps_new[k * nb + threadIdx.x] = et[threadIdx.x];
   }
   if (Ft_out) {   // Ft_out = U^dagger Ft U with Ft given in Fe (Newton trial, solver.py:1008)
-    m_load(sQ, Fe + k * nb2, nb);
+    m_load<NB>(sQ, Fe + k * nb2, nb);
     __syncthreads();
-    m_gemm<T, false, false>(sX, sQ, sU, nb);
-    m_gemm<T, true, false>(sV, sU, sX, nb);
-    m_store(Ft_out + k * nb2, sV, nb);
+    m_gemm<NB, T, false, false>(sX, sQ, sU, nb);
+    m_gemm<NB, T, true, false>(sV, sU, sX, nb);
+    m_store<NB>(Ft_out + k * nb2, sV, nb);
   }
 }
 
@@ -606,7 +637,7 @@ __global__ __launch_bounds__(64) void pk_cg_retract_kernel(int nb, cx<T>* __rest
 // elimination needs no pivoting: I + (tau/2) d has Hermitian part I, so every
 // Schur complement keeps a positive definite Hermitian part and every pivot a
 // real part >= 1.  On entry sX holds D; on exit sU holds U (sM, sX clobbered).
-template <typename T>
+template <int NB, typename T>
 __device__ void m_cayley_gj(cx<T>* sM, cx<T>* sU, const cx<T>* sX, double tau, int nb) {
   const T h = (T)(0.5 * tau);
   PK_FOR_ELEM(nb, {
@@ -654,7 +685,7 @@ __device__ __forceinline__ cx<T> shfl_cx(cx<T> v, int src) {
   return cx<T>{__shfl(v.re, src), __shfl(v.im, src)};
 }
 
-template <typename T>
+template <int NB, typename T>
 __device__ void m_cayley_gj16(cx<T>* sU, const cx<T>* sX, double tau) {
   const T h = (T)(0.5 * tau);
   const int l = threadIdx.x & 63, j = l & 15, g = l >> 4;
@@ -702,9 +733,9 @@ __device__ void m_cayley_gj16(cx<T>* sU, const cx<T>* sX, double tau) {
 }
 
 // eps_b = Re sum_i conj(U_ib) (Ft U)_ib into et[b] (Y: LDS scratch)
-template <typename T>
+template <int NB, typename T>
 __device__ __forceinline__ void m_diag_UFU(const cx<T>* sF, const cx<T>* sU, cx<T>* sY, double* et, int nb) {
-  m_gemm<T, false, false>(sY, sF, sU, nb);
+  m_gemm<NB, T, false, false>(sY, sF, sU, nb);
   if ((int)threadIdx.x < nb) {
     const int b = threadIdx.x;
     double acc = 0.0;
@@ -719,8 +750,8 @@ __device__ __forceinline__ void m_diag_UFU(const cx<T>* sF, const cx<T>* sU, cx<
 
 // Direction without the spectral setup: D <- H + beta D, dp <- hp + beta dp,
 // part[k] = |D_k|^2 (solver.py:530-531, 545).
-template <typename T>
-__global__ __launch_bounds__(64) void pk_cg_dir_kernel(int nb, const cx<T>* __restrict__ H, cx<T>* __restrict__ D,
+template <int NB, typename T>
+__global__ __launch_bounds__(PKB) void pk_cg_dir_kernel(int nb, const cx<T>* __restrict__ H, cx<T>* __restrict__ D,
                                                        const double* __restrict__ hp, double* __restrict__ dp,
                                                        const double* __restrict__ st_all, double* __restrict__ part,
                                                        int seg_len) {
@@ -737,14 +768,14 @@ __global__ __launch_bounds__(64) void pk_cg_dir_kernel(int nb, const cx<T>* __re
     const size_t q = k * nb + threadIdx.x;
     dp[q] = hp[q] + (double)beta * dp[q];
   }
-  dn = wave_sum(dn);
+  dn = PK_SUM(dn);
   if (threadIdx.x == 0) part[k] = dn;
 }
 
 // Line-search trial with the Gauss-Jordan Cayley factor; same outputs as
 // pk_cg_trial_kernel.  Skipped once st[HFX_ST_ACC] is set.
-template <typename T>
-__global__ __launch_bounds__(64) void pk_cg_trial_gj_kernel(int nb, size_t Nk, const cx<T>* __restrict__ D,
+template <int NB, typename T>
+__global__ __launch_bounds__(PKB) void pk_cg_trial_gj_kernel(int nb, size_t Nk, const cx<T>* __restrict__ D,
                                                             const cx<T>* __restrict__ Ft,
                                                             const double* __restrict__ p,
                                                             const double* __restrict__ dp,
@@ -758,16 +789,18 @@ __global__ __launch_bounds__(64) void pk_cg_trial_gj_kernel(int nb, size_t Nk, c
   __shared__ double et[PK_NBMAX];
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
   const double tau = st[HFX_ST_TAU];
-  m_load(sX, D + k * nb2, nb);
-  m_load(sF, Ft + k * nb2, nb);
+  m_load<NB>(sX, D + k * nb2, nb);
+  m_load<NB>(sF, Ft + k * nb2, nb);
   __syncthreads();
-  if (nb == PK_NBMAX)
-    m_cayley_gj16(sU, sX, tau);
-  else
-    m_cayley_gj(sM, sU, sX, tau, nb);
-  m_diag_UFU(sF, sU, sX, et, nb);
+  if constexpr (NB == 16) {
+    if (nb == 16) m_cayley_gj16<NB>(sU, sX, tau);
+    else m_cayley_gj<NB>(sM, sU, sX, tau, nb);
+  } else {
+    m_cayley_gj<NB>(sM, sU, sX, tau, nb);
+  }
+  m_diag_UFU<NB>(sF, sU, sX, et, nb);
   if (U_out) {   // kept for the retraction when this trial is the accepted one
-    m_store(U_out + k * nb2, sU, nb);
+    m_store<NB>(U_out + k * nb2, sU, nb);
     if ((int)threadIdx.x < nb) eps_out[k * nb + threadIdx.x] = et[threadIdx.x];
   }
   double ef = 0.0, sf = 0.0, e0 = 0.0, s0 = 0.0;
@@ -784,10 +817,10 @@ __global__ __launch_bounds__(64) void pk_cg_trial_gj_kernel(int nb, size_t Nk, c
       s0 = x * log(x) + (1.0 - x) * log1p(-x);
     }
   }
-  ef = wave_sum(ef);
-  sf = wave_sum(sf);
-  e0 = wave_sum(e0);
-  s0 = wave_sum(s0);
+  ef = PK_SUM(ef);
+  sf = PK_SUM(sf);
+  e0 = PK_SUM(e0);
+  s0 = PK_SUM(s0);
   if (threadIdx.x == 0) {
     part[k] = ef;
     part[Nk + k] = sf;
@@ -801,8 +834,8 @@ __global__ __launch_bounds__(64) void pk_cg_trial_gj_kernel(int nb, size_t Nk, c
 // Retraction with the Gauss-Jordan Cayley factor (tau = st[HFX_ST_TAUF]):
 // Q <- Q U (FP32: plus one Newton-Schulz re-orthonormalisation step),
 // eps_new = diag(U^dagger Ft U)  (solver.py:586-592).
-template <typename T>
-__global__ __launch_bounds__(64) void pk_cg_retract_gj_kernel(int nb, cx<T>* __restrict__ Q,
+template <int NB, typename T>
+__global__ __launch_bounds__(PKB) void pk_cg_retract_gj_kernel(int nb, cx<T>* __restrict__ Q,
                                                               const cx<T>* __restrict__ D,
                                                               const cx<T>* __restrict__ Ft,
                                                               const double* __restrict__ st_all,
@@ -816,80 +849,82 @@ __global__ __launch_bounds__(64) void pk_cg_retract_gj_kernel(int nb, cx<T>* __r
   if (st[HFX_ST_FROZEN] != 0.0) return;
   if (U_acc && st[HFX_ST_ACC] != 0.0) {
     // a trial was accepted: tau_final is its tau, and its U / eps were kept
-    m_load(sU, U_acc + k * nb2, nb);
+    m_load<NB>(sU, U_acc + k * nb2, nb);
     if ((int)threadIdx.x < nb) eps_new[k * nb + threadIdx.x] = eps_acc[k * nb + threadIdx.x];
   } else {
     const double tau = st[HFX_ST_TAUF];
-    m_load(sX, D + k * nb2, nb);
-    m_load(sF, Ft + k * nb2, nb);
+    m_load<NB>(sX, D + k * nb2, nb);
+    m_load<NB>(sF, Ft + k * nb2, nb);
     __syncthreads();
-    if (nb == PK_NBMAX)
-      m_cayley_gj16(sU, sX, tau);
-    else
-      m_cayley_gj(sM, sU, sX, tau, nb);
-    m_diag_UFU(sF, sU, sX, et, nb);
+    if constexpr (NB == 16) {
+      if (nb == 16) m_cayley_gj16<NB>(sU, sX, tau);
+      else m_cayley_gj<NB>(sM, sU, sX, tau, nb);
+    } else {
+      m_cayley_gj<NB>(sM, sU, sX, tau, nb);
+    }
+    m_diag_UFU<NB>(sF, sU, sX, et, nb);
     if ((int)threadIdx.x < nb) eps_new[k * nb + threadIdx.x] = et[threadIdx.x];
   }
-  m_load(sF, Q + k * nb2, nb);
+  m_load<NB>(sF, Q + k * nb2, nb);
   __syncthreads();
-  m_gemm<T, false, false>(sX, sF, sU, nb);    // X = Q U
+  m_gemm<NB, T, false, false>(sX, sF, sU, nb);    // X = Q U
   if constexpr (sizeof(T) == 4) {
-    m_gemm<T, true, false>(sM, sX, sX, nb);   // S = X^dagger X
+    m_gemm<NB, T, true, false>(sM, sX, sX, nb);   // S = X^dagger X
     PK_FOR_ELEM(nb, {
       const cx<T> v = sM[i * PK_PITCH + j];
       sM[i * PK_PITCH + j] = cx<T>{(i == j ? (T)1.5 : (T)0) - (T)0.5 * v.re, (T)-0.5 * v.im};
     })
     __syncthreads();
-    m_gemm<T, false, false>(sU, sX, sM, nb);  // Q <- X (3 I - S) / 2
-    m_store(Q + k * nb2, sU, nb);
+    m_gemm<NB, T, false, false>(sU, sX, sM, nb);  // Q <- X (3 I - S) / 2
+    m_store<NB>(Q + k * nb2, sU, nb);
   } else {
-    m_store(Q + k * nb2, sX, nb);
+    m_store<NB>(Q + k * nb2, sX, nb);
   }
 }
 
 // Ft = Q^dagger F Q, eigh(Ft) -> (eps, W), Q <- Q W  (solver.py:622-625)
-template <typename T>
-__global__ __launch_bounds__(64) void pk_rotate_eigh_kernel(int nb, cx<T>* __restrict__ Q, const cx<T>* __restrict__ F,
+template <int NB, typename T>
+__global__ __launch_bounds__(PKB) void pk_rotate_eigh_kernel(int nb, cx<T>* __restrict__ Q, const cx<T>* __restrict__ F,
                                                             double* __restrict__ eps) {
   __shared__ cx<T> sQ[PK_MAT], sF[PK_MAT], sX[PK_MAT], sT[PK_MAT];
   __shared__ double sl[PK_NBMAX];
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
-  m_load(sQ, Q + k * nb2, nb);
-  m_load(sF, F + k * nb2, nb);
+  m_load<NB>(sQ, Q + k * nb2, nb);
+  m_load<NB>(sF, F + k * nb2, nb);
   __syncthreads();
-  m_gemm<T, false, false>(sX, sF, sQ, nb);
-  m_gemm<T, true, false>(sT, sQ, sX, nb);     // Ft
-  m_eigh(sT, sX, sF, sl, nb, nullptr);        // W in sX
-  m_gemm<T, false, false>(sF, sQ, sX, nb);    // Q W
-  m_store(Q + k * nb2, sF, nb);
+  m_gemm<NB, T, false, false>(sX, sF, sQ, nb);
+  m_gemm<NB, T, true, false>(sT, sQ, sX, nb);     // Ft
+  m_eigh<NB>(sT, sX, sF, sl, nb, nullptr);        // W in sX
+  m_gemm<NB, T, false, false>(sF, sQ, sX, nb);    // Q W
+  m_store<NB>(Q + k * nb2, sF, nb);
   if ((int)threadIdx.x < nb) eps[k * nb + threadIdx.x] = sl[threadIdx.x];
 }
 
 // C = op(A) op(B) op(C0)?  Generic batched products: out = opA(A) opB(B) [opC(Cm)]
-template <typename T, bool AH, bool BH, bool CH>
-__global__ __launch_bounds__(64) void pk_mm_kernel(int nb, const cx<T>* __restrict__ A, const cx<T>* __restrict__ B,
+template <int NB, typename T, bool AH, bool BH, bool CH>
+__global__ __launch_bounds__(PKB) void pk_mm_kernel(int nb, const cx<T>* __restrict__ A, const cx<T>* __restrict__ B,
                                                    const cx<T>* __restrict__ Cm, cx<T>* __restrict__ out,
                                                    const cx<T>* __restrict__ Bsub, int herm_out) {
   __shared__ cx<T> sA[PK_MAT], sB[PK_MAT], sC[PK_MAT], sX[PK_MAT];
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
-  m_load(sA, A + k * nb2, nb);
-  m_load(sB, B + k * nb2, nb);
-  if (Cm) m_load(sC, Cm + k * nb2, nb);
+  m_load<NB>(sA, A + k * nb2, nb);
+  m_load<NB>(sB, B + k * nb2, nb);
+  if (Cm) m_load<NB>(sC, Cm + k * nb2, nb);
   __syncthreads();
   if (Bsub) {   // B <- B - Bsub   (e.g. F[dP] - h)
     PK_FOR_ELEM(nb, { sB[i * PK_PITCH + j] = sB[i * PK_PITCH + j] - Bsub[k * nb2 + e]; })
     __syncthreads();
   }
-  m_gemm<T, AH, BH>(sX, sA, sB, nb);
+  m_gemm<NB, T, AH, BH>(sX, sA, sB, nb);
   const cx<T>* R = sX;
   if (Cm) {
-    m_gemm<T, false, CH>(sA, sX, sC, nb);
+    m_gemm<NB, T, false, CH>(sA, sX, sC, nb);
     R = sA;
   }
   if (herm_out) {
     PK_FOR_ELEM(nb, { out[k * nb2 + e] = (T)0.5 * (R[i * PK_PITCH + j] + conj(R[j * PK_PITCH + i])); })
   } else {
-    m_store(out + k * nb2, R, nb);
+    m_store<NB>(out + k * nb2, R, nb);
   }
 }
 
@@ -904,8 +939,8 @@ struct PkInputs {
 
 // coef_dev (optional): per-solve coefficients, coef_dev[s * PK_XMAX + v] for the
 // solve s = k / seg_len of a stack of solves (the batched polish).
-template <typename T>
-__global__ __launch_bounds__(64) void pk_extrap_rotate_kernel(int nb, int nin, PkInputs in,
+template <int NB, typename T>
+__global__ __launch_bounds__(PKB) void pk_extrap_rotate_kernel(int nb, int nin, PkInputs in,
                                                               const double* __restrict__ coef_dev, int seg_len,
                                                               const cx<T>* __restrict__ U, cx<T>* __restrict__ Ft) {
   __shared__ cx<T> sF[PK_MAT], sU[PK_MAT], sX[PK_MAT];
@@ -919,60 +954,60 @@ __global__ __launch_bounds__(64) void pk_extrap_rotate_kernel(int nb, int nin, P
     for (int v = 0; v < nin; ++v) a = a + (T)sc[v] * reinterpret_cast<const cx<T>*>(in.x[v])[k * nb2 + e];
     sX[i * PK_PITCH + j] = a;
   })
-  m_load(sU, U + k * nb2, nb);
+  m_load<NB>(sU, U + k * nb2, nb);
   __syncthreads();
   PK_FOR_ELEM(nb, { sF[i * PK_PITCH + j] = (T)0.5 * (sX[i * PK_PITCH + j] + conj(sX[j * PK_PITCH + i])); })
   __syncthreads();
-  m_gemm<T, true, false>(sX, sU, sF, nb);     // U^H F
-  m_gemm<T, false, false>(sF, sX, sU, nb);    // (U^H F) U
+  m_gemm<NB, T, true, false>(sX, sU, sF, nb);     // U^H F
+  m_gemm<NB, T, false, false>(sF, sX, sU, nb);    // (U^H F) U
   PK_FOR_ELEM(nb, { Ft[k * nb2 + e] = (T)0.5 * (sF[i * PK_PITCH + j] + conj(sF[j * PK_PITCH + i])); })
 }
 
 // Back from the rotated basis: U <- U W (in place, per k), and the density
 // P = herm(U diag(occ) U^dagger) of the new orbitals (solver.py:42-44).
-template <typename T>
-__global__ __launch_bounds__(64) void pk_rotate_density_kernel(int nb, cx<T>* __restrict__ U,
+template <int NB, typename T>
+__global__ __launch_bounds__(PKB) void pk_rotate_density_kernel(int nb, cx<T>* __restrict__ U,
                                                                const cx<T>* __restrict__ W,
                                                                const double* __restrict__ occ,
                                                                cx<T>* __restrict__ P) {
   __shared__ cx<T> sU[PK_MAT], sW[PK_MAT], sX[PK_MAT];
   __shared__ double sp[PK_NBMAX];
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
-  m_load(sU, U + k * nb2, nb);
-  m_load(sW, W + k * nb2, nb);
+  m_load<NB>(sU, U + k * nb2, nb);
+  m_load<NB>(sW, W + k * nb2, nb);
   if ((int)threadIdx.x < nb) sp[threadIdx.x] = occ[k * nb + threadIdx.x];
   __syncthreads();
-  m_gemm<T, false, false>(sX, sU, sW, nb);    // new orbitals U W
+  m_gemm<NB, T, false, false>(sX, sU, sW, nb);    // new orbitals U W
   PK_FOR_ELEM(nb, {
     U[k * nb2 + e] = sX[i * PK_PITCH + j];
     sW[i * PK_PITCH + j] = (T)sp[j] * sX[i * PK_PITCH + j];
   })
   __syncthreads();
-  m_gemm<T, false, true>(sU, sW, sX, nb);     // (U W) diag(occ) (U W)^H
+  m_gemm<NB, T, false, true>(sU, sW, sX, nb);     // (U W) diag(occ) (U W)^H
   PK_FOR_ELEM(nb, { P[k * nb2 + e] = (T)0.5 * (sU[i * PK_PITCH + j] + conj(sU[j * PK_PITCH + i])); })
 }
 
 // Newton-Schulz re-orthonormalisation in place, `steps` times:
 // U <- U (3 I - U^dagger U) / 2.  Each step squares the deviation of U^dagger U
 // from I: an orbital basis carried in FP32 (~1e-7) reaches FP64 rounding in two.
-template <typename T>
-__global__ __launch_bounds__(64) void pk_orthonormalize_kernel(int nb, cx<T>* __restrict__ U, int steps) {
+template <int NB, typename T>
+__global__ __launch_bounds__(PKB) void pk_orthonormalize_kernel(int nb, cx<T>* __restrict__ U, int steps) {
   __shared__ cx<T> sU[PK_MAT], sS[PK_MAT], sX[PK_MAT];
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
-  m_load(sU, U + k * nb2, nb);
+  m_load<NB>(sU, U + k * nb2, nb);
   __syncthreads();
   for (int it = 0; it < steps; ++it) {
-    m_gemm<T, true, false>(sS, sU, sU, nb);     // S = U^H U
+    m_gemm<NB, T, true, false>(sS, sU, sU, nb);     // S = U^H U
     PK_FOR_ELEM(nb, {
       const cx<T> v = sS[i * PK_PITCH + j];
       sS[i * PK_PITCH + j] = cx<T>{(i == j ? (T)1.5 : (T)0) - (T)0.5 * v.re, (T)-0.5 * v.im};
     })
     __syncthreads();
-    m_gemm<T, false, false>(sX, sU, sS, nb);    // U (3 I - S) / 2
+    m_gemm<NB, T, false, false>(sX, sU, sS, nb);    // U (3 I - S) / 2
     PK_FOR_ELEM(nb, { sU[i * PK_PITCH + j] = sX[i * PK_PITCH + j]; })
     __syncthreads();
   }
-  m_store(U + k * nb2, sU, nb);
+  m_store<NB>(U + k * nb2, sU, nb);
 }
 
 // SCF residuals (reference_scf.py:291-294): comm = F D - D F (stored when
@@ -986,18 +1021,18 @@ struct PkRes {
   int nr, self;
   int r32;   // FP64 kernels: the residual history (comm included) stored as complex64
 };
-template <typename T>
-__global__ __launch_bounds__(64) void pk_scf_resid_kernel(int nb, size_t Nk, const cx<T>* __restrict__ F,
+template <int NB, typename T>
+__global__ __launch_bounds__(PKB) void pk_scf_resid_kernel(int nb, size_t Nk, const cx<T>* __restrict__ F,
                                                           const cx<T>* __restrict__ D, const cx<T>* __restrict__ Pn,
                                                           cx<T>* __restrict__ comm, double* __restrict__ part,
                                                           PkRes rs) {
   __shared__ cx<T> sF[PK_MAT], sD[PK_MAT], sX[PK_MAT], sY[PK_MAT];
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
-  m_load(sF, F + k * nb2, nb);
-  m_load(sD, D + k * nb2, nb);
+  m_load<NB>(sF, F + k * nb2, nb);
+  m_load<NB>(sD, D + k * nb2, nb);
   __syncthreads();
-  m_gemm<T, false, false>(sX, sF, sD, nb);
-  m_gemm<T, false, false>(sY, sD, sF, nb);
+  m_gemm<NB, T, false, false>(sX, sF, sD, nb);
+  m_gemm<NB, T, false, false>(sY, sD, sF, nb);
   double c2 = 0.0, d2 = 0.0, bd[PK_XMAX];
 #pragma unroll
   for (int v = 0; v < PK_XMAX; ++v) bd[v] = 0.0;
@@ -1015,8 +1050,8 @@ __global__ __launch_bounds__(64) void pk_scf_resid_kernel(int nb, size_t Nk, con
         bd[v] += (double)c.re * r.re + (double)c.im * r.im;
       }
   })
-  c2 = wave_sum(c2);
-  d2 = wave_sum(d2);
+  c2 = PK_SUM(c2);
+  d2 = PK_SUM(d2);
   if (threadIdx.x == 0) {
     part[k] = c2;
     part[Nk + k] = d2;
@@ -1024,7 +1059,7 @@ __global__ __launch_bounds__(64) void pk_scf_resid_kernel(int nb, size_t Nk, con
 #pragma unroll
   for (int v = 0; v < PK_XMAX; ++v)
     if (v < rs.nr) {
-      const double b = v == rs.self ? c2 : wave_sum(bd[v]);
+      const double b = v == rs.self ? c2 : PK_SUM(bd[v]);
       if (threadIdx.x == 0) part[(size_t)(2 + v) * Nk + k] = b;
     }
 }
@@ -1032,21 +1067,22 @@ __global__ __launch_bounds__(64) void pk_scf_resid_kernel(int nb, size_t Nk, con
 // nb = 16: the same outputs with F D - D F accumulated on the matrix cores and
 // kept in registers (lane l holds the elements (row(l, r), l % 16)); two staged
 // matrices instead of four, so twice the waves per CU for this streaming pass.
-template <typename T>
-__global__ __launch_bounds__(64) void pk_scf_resid16_kernel(size_t Nk, const cx<T>* __restrict__ F,
+template <int NB, typename T>
+__global__ __launch_bounds__(PKB) void pk_scf_resid16_kernel(size_t Nk, const cx<T>* __restrict__ F,
                                                             const cx<T>* __restrict__ D,
                                                             const cx<T>* __restrict__ Pn, cx<T>* __restrict__ comm,
                                                             double* __restrict__ part, PkRes rs) {
-  constexpr int nb = PK_NBMAX, nb2 = PK_NBMAX * PK_NBMAX;
+  static_assert(NB == 16, "the matrix-core residual pass is the nb = 16 form");
+  constexpr int nb = 16, nb2 = 256;
   __shared__ cx<T> sF[PK_MAT], sD[PK_MAT];
   const size_t k = blockIdx.x;
-  m_load(sF, F + k * nb2, nb);
-  m_load(sD, D + k * nb2, nb);
+  m_load<NB>(sF, F + k * nb2, nb);
+  m_load<NB>(sD, D + k * nb2, nb);
   __syncthreads();
   MAcc<T> acc;
   acc.zero();
-  mf_gemm16<T, false, false>(acc, sF, sD);
-  mf_gemm16<T, false, false>(acc, sD, sF, (T)-1);
+  mf_gemm16<NB, T, false, false>(acc, sF, sD);
+  mf_gemm16<NB, T, false, false>(acc, sD, sF, (T)-1);
   const int l = threadIdx.x & 63, col = l & 15;
   double c2 = 0.0, d2 = 0.0, bd[PK_XMAX];
 #pragma unroll
@@ -1081,8 +1117,8 @@ __global__ __launch_bounds__(64) void pk_scf_resid16_kernel(size_t Nk, const cx<
         bd[v] += (double)c.re * qre + (double)c.im * qim;
       }
   }
-  c2 = wave_sum(c2);
-  d2 = wave_sum(d2);
+  c2 = PK_SUM(c2);
+  d2 = PK_SUM(d2);
   if (threadIdx.x == 0) {
     part[k] = c2;
     part[Nk + k] = d2;
@@ -1090,7 +1126,7 @@ __global__ __launch_bounds__(64) void pk_scf_resid16_kernel(size_t Nk, const cx<
 #pragma unroll
   for (int v = 0; v < PK_XMAX; ++v)
     if (v < rs.nr) {
-      const double b = v == rs.self ? c2 : wave_sum(bd[v]);
+      const double b = v == rs.self ? c2 : PK_SUM(bd[v]);
       if (threadIdx.x == 0) part[(size_t)(2 + v) * Nk + k] = b;
     }
 }
@@ -1103,8 +1139,8 @@ __global__ __launch_bounds__(64) void pk_scf_resid16_kernel(size_t Nk, const cx<
 // the host.  One wave per k; the two products per element run on the matrix
 // cores at nb = 16.  Out-of-range sources (rejected by the host) read A[k].
 constexpr int SYM_MAX = 64;
-template <typename T>
-__global__ __launch_bounds__(64) void pk_sym_avg_kernel(int nb, size_t Nk, int ng, const cx<T>* __restrict__ A,
+template <int NB, typename T>
+__global__ __launch_bounds__(PKB) void pk_sym_avg_kernel(int nb, size_t Nk, int ng, const cx<T>* __restrict__ A,
                                                         const cx<T>* __restrict__ Ug, const int* __restrict__ src,
                                                         unsigned long long conj_bits,
                                                         const unsigned char* __restrict__ mask, double weight,
@@ -1129,8 +1165,8 @@ __global__ __launch_bounds__(64) void pk_sym_avg_kernel(int nb, size_t Nk, int n
       sU[i * PK_PITCH + j] = Ug[(size_t)g * nb2 + e];
     })
     __syncthreads();
-    m_gemm<T, false, false>(sX, sU, sA, nb);   // U op(A)
-    m_gemm<T, false, true>(sA, sX, sU, nb);    // (U op(A)) U^dagger
+    m_gemm<NB, T, false, false>(sX, sU, sA, nb);   // U op(A)
+    m_gemm<NB, T, false, true>(sA, sX, sU, nb);    // (U op(A)) U^dagger
     PK_FOR_ELEM(nb, { acc[m_] = acc[m_] + sA[i * PK_PITCH + j]; })
     __syncthreads();
   }
@@ -1144,8 +1180,8 @@ struct DotList {
   const void* x[8];
   const void* y[8];
 };
-template <typename T>
-__global__ __launch_bounds__(64) void pk_dots_kernel(int nb, size_t Nk, DotList L, int nv, int mode,
+template <int NB, typename T>
+__global__ __launch_bounds__(PKB) void pk_dots_kernel(int nb, size_t Nk, DotList L, int nv, int mode,
                                                      double* __restrict__ part) {
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
   for (int v = 0; v < nv; ++v) {
@@ -1157,21 +1193,21 @@ __global__ __launch_bounds__(64) void pk_dots_kernel(int nb, size_t Nk, DotList 
       const cx<T> y = mode == 0 ? Y[k * nb2 + e] : Y[k * nb2 + (size_t)j * nb + i];
       acc += mode == 0 ? ((double)x.re * y.re + (double)x.im * y.im) : ((double)x.re * y.re - (double)x.im * y.im);
     })
-    acc = wave_sum(acc);
+    acc = PK_SUM(acc);
     if (threadIdx.x == 0) part[(size_t)v * Nk + k] = acc;
   }
 }
 
 // Newton HVP, density part (solver.py:917-919):
 //   M = X (x) diff_p + diag(dp), diff_p[i][j] = p_j - p_i;  dP = herm(Q M Q^dagger).
-template <typename T>
-__global__ __launch_bounds__(64) void pk_newton_dP_kernel(int nb, const cx<T>* __restrict__ Q, const cx<T>* __restrict__ X,
+template <int NB, typename T>
+__global__ __launch_bounds__(PKB) void pk_newton_dP_kernel(int nb, const cx<T>* __restrict__ Q, const cx<T>* __restrict__ X,
                                                           const double* __restrict__ p, const double* __restrict__ dp,
                                                           cx<T>* __restrict__ dP) {
   __shared__ cx<T> sQ[PK_MAT], sM[PK_MAT], sX[PK_MAT], sP[PK_MAT];
   __shared__ double sp[PK_NBMAX], sd[PK_NBMAX];
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
-  m_load(sQ, Q + k * nb2, nb);
+  m_load<NB>(sQ, Q + k * nb2, nb);
   if ((int)threadIdx.x < nb) {
     sp[threadIdx.x] = p[k * nb + threadIdx.x];
     sd[threadIdx.x] = dp[k * nb + threadIdx.x];
@@ -1183,8 +1219,8 @@ __global__ __launch_bounds__(64) void pk_newton_dP_kernel(int nb, const cx<T>* _
     sM[i * PK_PITCH + j] = m;
   })
   __syncthreads();
-  m_gemm<T, false, false>(sX, sQ, sM, nb);
-  m_gemm<T, false, true>(sP, sX, sQ, nb);
+  m_gemm<NB, T, false, false>(sX, sQ, sM, nb);
+  m_gemm<NB, T, false, true>(sP, sX, sQ, nb);
   PK_FOR_ELEM(nb, { dP[k * nb2 + e] = (T)0.5 * (sP[i * PK_PITCH + j] + conj(sP[j * PK_PITCH + i])); })
 }
 
@@ -1192,8 +1228,8 @@ __global__ __launch_bounds__(64) void pk_newton_dP_kernel(int nb, const cx<T>* _
 //   St = Q^dagger R Q, A = Ft X - X Ft + St,
 //   HX = skew(diff_dp (x) Ft + diff_p (x) A) offdiag,  Hp = Re diag(A) + T/(pc(1-pc)) dp
 // (Hp before the particle-conserving projection); part[k] = sum_b Hp.
-template <typename T>
-__global__ __launch_bounds__(64) void pk_newton_hvp_kernel(int nb, const cx<T>* __restrict__ Q,
+template <int NB, typename T>
+__global__ __launch_bounds__(PKB) void pk_newton_hvp_kernel(int nb, const cx<T>* __restrict__ Q,
                                                            const cx<T>* __restrict__ R, const cx<T>* __restrict__ Ft,
                                                            const cx<T>* __restrict__ X, const double* __restrict__ p,
                                                            const double* __restrict__ dp, double T_,
@@ -1202,21 +1238,21 @@ __global__ __launch_bounds__(64) void pk_newton_hvp_kernel(int nb, const cx<T>* 
   __shared__ cx<T> sQ[PK_MAT], sR[PK_MAT], sF[PK_MAT], sX[PK_MAT], sY[PK_MAT], sA[PK_MAT];
   __shared__ double sp[PK_NBMAX], sd[PK_NBMAX];
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
-  m_load(sQ, Q + k * nb2, nb);
-  m_load(sR, R + k * nb2, nb);
-  m_load(sF, Ft + k * nb2, nb);
-  m_load(sX, X + k * nb2, nb);
+  m_load<NB>(sQ, Q + k * nb2, nb);
+  m_load<NB>(sR, R + k * nb2, nb);
+  m_load<NB>(sF, Ft + k * nb2, nb);
+  m_load<NB>(sX, X + k * nb2, nb);
   if ((int)threadIdx.x < nb) {
     sp[threadIdx.x] = p[k * nb + threadIdx.x];
     sd[threadIdx.x] = dp[k * nb + threadIdx.x];
   }
   __syncthreads();
-  m_gemm<T, false, false>(sY, sR, sQ, nb);    // R Q
-  m_gemm<T, true, false>(sA, sQ, sY, nb);     // St
-  m_gemm<T, false, false>(sY, sF, sX, nb);    // Ft X
+  m_gemm<NB, T, false, false>(sY, sR, sQ, nb);    // R Q
+  m_gemm<NB, T, true, false>(sA, sQ, sY, nb);     // St
+  m_gemm<NB, T, false, false>(sY, sF, sX, nb);    // Ft X
   PK_FOR_ELEM(nb, { sA[i * PK_PITCH + j] = sA[i * PK_PITCH + j] + sY[i * PK_PITCH + j]; })
   __syncthreads();
-  m_gemm<T, false, false>(sY, sX, sF, nb);    // X Ft
+  m_gemm<NB, T, false, false>(sY, sX, sF, nb);    // X Ft
   PK_FOR_ELEM(nb, { sR[i * PK_PITCH + j] = sA[i * PK_PITCH + j] - sY[i * PK_PITCH + j]; })   // A
   __syncthreads();
   PK_FOR_ELEM(nb, {
@@ -1237,14 +1273,14 @@ __global__ __launch_bounds__(64) void pk_newton_hvp_kernel(int nb, const cx<T>* 
     Hp[k * nb + b] = v;
     acc = v;
   }
-  acc = wave_sum(acc);
+  acc = PK_SUM(acc);
   if (threadIdx.x == 0) part[k] = acc;
 }
 
 // Newton preconditioner (solver.py:905-913): ZX = RX / sqrt(gap^2 + lam^2)
 // (lam = st[HFX_ST_LAM]), zp = rp pc(1-pc)/T before projection; part[k] = sum_b zp.
-template <typename T>
-__global__ __launch_bounds__(64) void pk_newton_prec_kernel(int nb, const cx<T>* __restrict__ RX,
+template <int NB, typename T>
+__global__ __launch_bounds__(PKB) void pk_newton_prec_kernel(int nb, const cx<T>* __restrict__ RX,
                                                             const double* __restrict__ rp, const double* __restrict__ eps,
                                                             const double* __restrict__ p, const double* __restrict__ st,
                                                             double T_, cx<T>* __restrict__ ZX, double* __restrict__ zp,
@@ -1265,25 +1301,35 @@ __global__ __launch_bounds__(64) void pk_newton_prec_kernel(int nb, const cx<T>*
     zp[q] = v;
     acc = v;
   }
-  acc = wave_sum(acc);
+  acc = PK_SUM(acc);
   if (threadIdx.x == 0) part[k] = acc;
 }
 
 // ============================ launchers ======================================
 
-#define PK_DISPATCH(dtype, K, grid, ...)                                                        \
-  do {                                                                                         \
-    if ((dtype) == HFX_DTYPE_C128)                                                             \
-      hipLaunchKernelGGL(K<double>, dim3(grid), dim3(64), 0, s, __VA_ARGS__);                   \
-    else                                                                                       \
-      hipLaunchKernelGGL(K<float>, dim3(grid), dim3(64), 0, s, __VA_ARGS__);                    \
+// Size-class dispatch: nb <= 16 -> one wave per k (NB = 16), nb <= 32 -> a
+// 256-thread workgroup per k (NB = 32).  `nb`, `nk` and `s` in scope.
+constexpr int PK_NB_LIMIT = 32;
+#define PK_LAUNCH(K, T, ...)                                                                 \
+  do {                                                                                       \
+    if (nb <= 16)                                                                            \
+      hipLaunchKernelGGL((K<16, T>), dim3(nk), dim3(64), 0, s, __VA_ARGS__);                  \
+    else                                                                                     \
+      hipLaunchKernelGGL((K<32, T>), dim3(nk), dim3(256), 0, s, __VA_ARGS__);                 \
+  } while (0)
+#define PK_LAUNCH_MM(T, a, b, c, ...)                                                        \
+  do {                                                                                       \
+    if (nb <= 16)                                                                            \
+      hipLaunchKernelGGL((pk_mm_kernel<16, T, a, b, c>), dim3(nk), dim3(64), 0, s, __VA_ARGS__); \
+    else                                                                                     \
+      hipLaunchKernelGGL((pk_mm_kernel<32, T, a, b, c>), dim3(nk), dim3(256), 0, s, __VA_ARGS__); \
   } while (0)
 
 template <typename T> using C_ = cx<T>;
 
 int pk_check_nb(int nb) {
-  if (nb < 1 || nb > PK_NBMAX) return set_error(HFX_EUNSUPPORTED, "per-k kernels support 1 <= nb <= %d (got %d)",
-                                                PK_NBMAX, nb);
+  if (nb < 1 || nb > PK_NB_LIMIT)
+    return set_error(HFX_EUNSUPPORTED, "per-k kernels support 1 <= nb <= %d (got %d)", PK_NB_LIMIT, nb);
   return HFX_OK;
 }
 
@@ -1306,7 +1352,17 @@ int hfx_eigh_batched_tol(int dtype, int nb, size_t nk, const void* A, double* la
   if (!A || !lam || !V || !(rel_tol >= 0.0) || rel_tol > 1e-3)
     return set_error(HFX_EINVAL, "hfx_eigh_batched: bad argument (rel_tol=%g)", rel_tol);
   if (nk == 0) return HFX_OK;
-  return eigh16_launch(dtype, nb, nk, A, lam, V, herm_in, rel_tol, (hipStream_t)stream);
+  if (nb <= 16) return eigh16_launch(dtype, nb, nk, A, lam, V, herm_in, rel_tol, (hipStream_t)stream);
+  // 16 < nb <= 32: LDS-staged cyclic Jacobi, one 256-thread workgroup per
+  // matrix, rotating to its own threshold (rel_tol only ever stops earlier)
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL((pk_eigh_kernel<32, double>), dim3(nk), dim3(256), 0, s, nb, PKC(double, A), lam,
+                       PKM(double, V), herm_in);
+  else
+    hipLaunchKernelGGL((pk_eigh_kernel<32, float>), dim3(nk), dim3(256), 0, s, nb, PKC(float, A), lam,
+                       PKM(float, V), herm_in);
+  return check_launch("pk_eigh_kernel");
 }
 
 int hfx_density_from_Qp(int dtype, int nb, size_t nk, const void* Q, const double* p, const double* eps,
@@ -1316,10 +1372,10 @@ int hfx_density_from_Qp(int dtype, int nb, size_t nk, const void* Q, const doubl
   if (nk == 0) return HFX_OK;
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_density_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKC(double, Q), p, eps, mu, T, pout,
+    PK_LAUNCH(pk_density_kernel, double, nb, PKC(double, Q), p, eps, mu, T, pout,
                        PKM(double, P));
   else
-    hipLaunchKernelGGL(pk_density_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKC(float, Q), p, eps, mu, T, pout,
+    PK_LAUNCH(pk_density_kernel, float, nb, PKC(float, Q), p, eps, mu, T, pout,
                        PKM(float, P));
   return check_launch("pk_density_kernel");
 }
@@ -1329,10 +1385,10 @@ int hfx_cg_grad(int dtype, int nb, size_t nk, const void* Q, const void* F, cons
   if (int rc = pk_check_nb(nb)) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_cg_grad_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(double, Q), PKC(double, F), p,
+    PK_LAUNCH(pk_cg_grad_kernel, double, nb, nk, PKC(double, Q), PKC(double, F), p,
                        PKM(double, Ft), eps, PKM(double, G), part);
   else
-    hipLaunchKernelGGL(pk_cg_grad_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(float, Q), PKC(float, F), p,
+    PK_LAUNCH(pk_cg_grad_kernel, float, nb, nk, PKC(float, Q), PKC(float, F), p,
                        PKM(float, Ft), eps, PKM(float, G), part);
   return check_launch("pk_cg_grad_kernel");
 }
@@ -1343,10 +1399,10 @@ int hfx_cg_precond(int dtype, int nb, size_t nk, const void* G, const void* Gpre
   if (int rc = pk_check_nb(nb)) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_cg_precond_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(double, G),
+    PK_LAUNCH(pk_cg_precond_kernel, double, nb, nk, PKC(double, G),
                        PKC(double, Gprev), eps, p, gp_prev, st, T, PKM(double, H), gp, hp, part, seg_len);
   else
-    hipLaunchKernelGGL(pk_cg_precond_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(float, G), PKC(float, Gprev),
+    PK_LAUNCH(pk_cg_precond_kernel, float, nb, nk, PKC(float, G), PKC(float, Gprev),
                        eps, p, gp_prev, st, T, PKM(float, H), gp, hp, part, seg_len);
   return check_launch("pk_cg_precond_kernel");
 }
@@ -1356,10 +1412,10 @@ int hfx_cg_direction(int dtype, int nb, size_t nk, const void* H, void* D, const
   if (int rc = pk_check_nb(nb)) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_cg_direction_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKC(double, H), PKM(double, D),
+    PK_LAUNCH(pk_cg_direction_kernel, double, nb, PKC(double, H), PKM(double, D),
                        hp, dp, PKC(double, Ft), st, PKM(double, V), lam_d, PKM(double, Fe), part);
   else
-    hipLaunchKernelGGL(pk_cg_direction_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKC(float, H), PKM(float, D), hp,
+    PK_LAUNCH(pk_cg_direction_kernel, float, nb, PKC(float, H), PKM(float, D), hp,
                        dp, PKC(float, Ft), st, PKM(float, V), lam_d, PKM(float, Fe), part);
   return check_launch("pk_cg_direction_kernel");
 }
@@ -1369,10 +1425,10 @@ int hfx_cg_trial(int dtype, int nb, size_t nk, const void* V, const double* lam_
   if (int rc = pk_check_nb(nb)) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_cg_trial_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(double, V), lam_d,
+    PK_LAUNCH(pk_cg_trial_kernel, double, nb, nk, PKC(double, V), lam_d,
                        PKC(double, Fe), p, dp, eps, st, omega0, part);
   else
-    hipLaunchKernelGGL(pk_cg_trial_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(float, V), lam_d,
+    PK_LAUNCH(pk_cg_trial_kernel, float, nb, nk, PKC(float, V), lam_d,
                        PKC(float, Fe), p, dp, eps, st, omega0, part);
   return check_launch("pk_cg_trial_kernel");
 }
@@ -1382,10 +1438,10 @@ int hfx_cg_retract(int dtype, int nb, size_t nk, void* Q, const void* V, const d
   if (int rc = pk_check_nb(nb)) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_cg_retract_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKM(double, Q), PKC(double, V),
+    PK_LAUNCH(pk_cg_retract_kernel, double, nb, PKM(double, Q), PKC(double, V),
                        lam_d, PKC(double, Fe), st, tau, use_tau, eps_new, PKM(double, Ft_out));
   else
-    hipLaunchKernelGGL(pk_cg_retract_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKM(float, Q), PKC(float, V), lam_d,
+    PK_LAUNCH(pk_cg_retract_kernel, float, nb, PKM(float, Q), PKC(float, V), lam_d,
                        PKC(float, Fe), st, tau, use_tau, eps_new, PKM(float, Ft_out));
   return check_launch("pk_cg_retract_kernel");
 }
@@ -1395,10 +1451,10 @@ int hfx_cg_direction_gj(int dtype, int nb, size_t nk, const void* H, void* D, co
   if (int rc = pk_check_nb(nb)) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_cg_dir_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKC(double, H), PKM(double, D), hp,
+    PK_LAUNCH(pk_cg_dir_kernel, double, nb, PKC(double, H), PKM(double, D), hp,
                        dp, st, part, seg_len);
   else
-    hipLaunchKernelGGL(pk_cg_dir_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKC(float, H), PKM(float, D), hp, dp,
+    PK_LAUNCH(pk_cg_dir_kernel, float, nb, PKC(float, H), PKM(float, D), hp, dp,
                        st, part, seg_len);
   return check_launch("pk_cg_dir_kernel");
 }
@@ -1410,10 +1466,10 @@ int hfx_cg_trial_gj(int dtype, int nb, size_t nk, const void* D, const void* Ft,
   if (!U_keep != !eps_keep) return set_error(HFX_EINVAL, "hfx_cg_trial_gj: U_keep and eps_keep go together");
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_cg_trial_gj_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(double, D),
+    PK_LAUNCH(pk_cg_trial_gj_kernel, double, nb, nk, PKC(double, D),
                        PKC(double, Ft), p, dp, eps, st, omega0, part, PKM(double, U_keep), eps_keep, seg_len);
   else
-    hipLaunchKernelGGL(pk_cg_trial_gj_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(float, D),
+    PK_LAUNCH(pk_cg_trial_gj_kernel, float, nb, nk, PKC(float, D),
                        PKC(float, Ft), p, dp, eps, st, omega0, part, PKM(float, U_keep), eps_keep, seg_len);
   return check_launch("pk_cg_trial_gj_kernel");
 }
@@ -1425,10 +1481,10 @@ int hfx_cg_retract_gj(int dtype, int nb, size_t nk, void* Q, const void* D, cons
   if (!U_keep != !eps_keep) return set_error(HFX_EINVAL, "hfx_cg_retract_gj: U_keep and eps_keep go together");
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_cg_retract_gj_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKM(double, Q),
+    PK_LAUNCH(pk_cg_retract_gj_kernel, double, nb, PKM(double, Q),
                        PKC(double, D), PKC(double, Ft), st, eps_new, PKC(double, U_keep), eps_keep, seg_len);
   else
-    hipLaunchKernelGGL(pk_cg_retract_gj_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKM(float, Q),
+    PK_LAUNCH(pk_cg_retract_gj_kernel, float, nb, PKM(float, Q),
                        PKC(float, D), PKC(float, Ft), st, eps_new, PKC(float, U_keep), eps_keep, seg_len);
   return check_launch("pk_cg_retract_gj_kernel");
 }
@@ -1437,9 +1493,9 @@ int hfx_rotate_eigh(int dtype, int nb, size_t nk, void* Q, const void* F, double
   if (int rc = pk_check_nb(nb)) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_rotate_eigh_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKM(double, Q), PKC(double, F), eps);
+    PK_LAUNCH(pk_rotate_eigh_kernel, double, nb, PKM(double, Q), PKC(double, F), eps);
   else
-    hipLaunchKernelGGL(pk_rotate_eigh_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKM(float, Q), PKC(float, F), eps);
+    PK_LAUNCH(pk_rotate_eigh_kernel, float, nb, PKM(float, Q), PKC(float, F), eps);
   return check_launch("pk_rotate_eigh_kernel");
 }
 
@@ -1452,10 +1508,10 @@ int hfx_matmul_batched(int dtype, int nb, size_t nk, const void* A, const void* 
 #define MM_CASE(o, a, b, c)                                                                                  \
   case o:                                                                                                   \
     if (dtype == HFX_DTYPE_C128)                                                                            \
-      hipLaunchKernelGGL((pk_mm_kernel<double, a, b, c>), dim3(nk), dim3(64), 0, s, nb, PKC(double, A),        \
+      PK_LAUNCH_MM(double, a, b, c, nb, PKC(double, A),        \
                          PKC(double, B), PKC(double, Cm), PKM(double, out), PKC(double, Bsub), herm_out);        \
     else                                                                                                    \
-      hipLaunchKernelGGL((pk_mm_kernel<float, a, b, c>), dim3(nk), dim3(64), 0, s, nb, PKC(float, A),          \
+      PK_LAUNCH_MM(float, a, b, c, nb, PKC(float, A),          \
                          PKC(float, B), PKC(float, Cm), PKM(float, out), PKC(float, Bsub), herm_out);            \
     break;
   switch (ops & 7) {
@@ -1486,10 +1542,10 @@ static int extrap_rotate_launch(int dtype, int nb, size_t nk, int nin, const voi
   if (nk == 0) return HFX_OK;
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_extrap_rotate_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nin, in, coef_dev, seg_len,
+    PK_LAUNCH(pk_extrap_rotate_kernel, double, nb, nin, in, coef_dev, seg_len,
                        PKC(double, U), PKM(double, Ft));
   else
-    hipLaunchKernelGGL(pk_extrap_rotate_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nin, in, coef_dev, seg_len,
+    PK_LAUNCH(pk_extrap_rotate_kernel, float, nb, nin, in, coef_dev, seg_len,
                        PKC(float, U), PKM(float, Ft));
   return check_launch("pk_extrap_rotate_kernel");
 }
@@ -1513,30 +1569,30 @@ int hfx_rotate_density(int dtype, int nb, size_t nk, void* U, const void* W, con
   if (nk == 0) return HFX_OK;
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_rotate_density_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKM(double, U),
+    PK_LAUNCH(pk_rotate_density_kernel, double, nb, PKM(double, U),
                        PKC(double, W), occ, PKM(double, P));
   else
-    hipLaunchKernelGGL(pk_rotate_density_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKM(float, U),
+    PK_LAUNCH(pk_rotate_density_kernel, float, nb, PKM(float, U),
                        PKC(float, W), occ, PKM(float, P));
   return check_launch("pk_rotate_density_kernel");
 }
 
 static int scf_resid_launch(int dtype, int nb, size_t nk, const void* F, const void* D, const void* Pn, void* comm,
                             PkRes rs, double* part, hipStream_t s) {
-  if (nb == PK_NBMAX) {
+  if (nb == 16) {
     if (dtype == HFX_DTYPE_C128)
-      hipLaunchKernelGGL(pk_scf_resid16_kernel<double>, dim3(nk), dim3(64), 0, s, nk, PKC(double, F),
+      hipLaunchKernelGGL((pk_scf_resid16_kernel<16, double>), dim3(nk), dim3(64), 0, s, nk, PKC(double, F),
                          PKC(double, D), PKC(double, Pn), PKM(double, comm), part, rs);
     else
-      hipLaunchKernelGGL(pk_scf_resid16_kernel<float>, dim3(nk), dim3(64), 0, s, nk, PKC(float, F), PKC(float, D),
+      hipLaunchKernelGGL((pk_scf_resid16_kernel<16, float>), dim3(nk), dim3(64), 0, s, nk, PKC(float, F), PKC(float, D),
                          PKC(float, Pn), PKM(float, comm), part, rs);
     return check_launch("pk_scf_resid16_kernel");
   }
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_scf_resid_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(double, F), PKC(double, D),
+    PK_LAUNCH(pk_scf_resid_kernel, double, nb, nk, PKC(double, F), PKC(double, D),
                        PKC(double, Pn), PKM(double, comm), part, rs);
   else
-    hipLaunchKernelGGL(pk_scf_resid_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nk, PKC(float, F), PKC(float, D),
+    PK_LAUNCH(pk_scf_resid_kernel, float, nb, nk, PKC(float, F), PKC(float, D),
                        PKC(float, Pn), PKM(float, comm), part, rs);
   return check_launch("pk_scf_resid_kernel");
 }
@@ -1557,10 +1613,10 @@ int hfx_symmetry_average(int dtype, int nb, size_t nk, int ng, const void* A, co
   if (nk == 0) return HFX_OK;
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_sym_avg_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nk, ng, PKC(double, A),
+    PK_LAUNCH(pk_sym_avg_kernel, double, nb, nk, ng, PKC(double, A),
                        PKC(double, Ug), src, conj_bits, mask, weight, PKM(double, out));
   else
-    hipLaunchKernelGGL(pk_sym_avg_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nk, ng, PKC(float, A),
+    PK_LAUNCH(pk_sym_avg_kernel, float, nb, nk, ng, PKC(float, A),
                        PKC(float, Ug), src, conj_bits, mask, weight, PKM(float, out));
   return check_launch("pk_sym_avg_kernel");
 }
@@ -1570,9 +1626,9 @@ int hfx_orthonormalize(int dtype, int nb, size_t nk, void* U, int steps, void* s
   if (!U || steps < 1 || steps > 4) return set_error(HFX_EINVAL, "hfx_orthonormalize: bad argument (steps=%d)", steps);
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_orthonormalize_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKM(double, U), steps);
+    PK_LAUNCH(pk_orthonormalize_kernel, double, nb, PKM(double, U), steps);
   else
-    hipLaunchKernelGGL(pk_orthonormalize_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKM(float, U), steps);
+    PK_LAUNCH(pk_orthonormalize_kernel, float, nb, PKM(float, U), steps);
   return check_launch("pk_orthonormalize_kernel");
 }
 
@@ -1580,7 +1636,7 @@ int hfx_scf_resid_dots(int dtype, int nb, size_t nk, const void* F, const void* 
                        const void* const* Rs, int nr, int self, int hist_dtype, double* part, void* stream) {
   if (int rc = pk_check_nb(nb)) return rc;
   if (!F || !D || !comm || !part || !Rs || nr < 1 || nr > PK_XMAX || self < -1 || self >= nr ||
-      (hist_dtype != dtype && !(dtype == HFX_DTYPE_C128 && hist_dtype == HFX_DTYPE_C64 && nb == PK_NBMAX)))
+      (hist_dtype != dtype && !(dtype == HFX_DTYPE_C128 && hist_dtype == HFX_DTYPE_C64 && nb == 16)))
     return set_error(HFX_EINVAL, "hfx_scf_resid_dots: bad argument (nr=%d, self=%d, hist_dtype=%d)", nr, self,
                      hist_dtype);
   PkRes rs{};
@@ -1605,9 +1661,9 @@ int hfx_batched_dots(int dtype, int nb, size_t nk, const void* const* Xs, const 
   }
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_dots_kernel<double>, dim3(nk), dim3(64), 0, s, nb, nk, L, nv, mode, part);
+    PK_LAUNCH(pk_dots_kernel, double, nb, nk, L, nv, mode, part);
   else
-    hipLaunchKernelGGL(pk_dots_kernel<float>, dim3(nk), dim3(64), 0, s, nb, nk, L, nv, mode, part);
+    PK_LAUNCH(pk_dots_kernel, float, nb, nk, L, nv, mode, part);
   return check_launch("pk_dots_kernel");
 }
 
@@ -1616,10 +1672,10 @@ int hfx_newton_dP(int dtype, int nb, size_t nk, const void* Q, const void* X, co
   if (int rc = pk_check_nb(nb)) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_newton_dP_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKC(double, Q), PKC(double, X), p,
+    PK_LAUNCH(pk_newton_dP_kernel, double, nb, PKC(double, Q), PKC(double, X), p,
                        dp, PKM(double, dP));
   else
-    hipLaunchKernelGGL(pk_newton_dP_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKC(float, Q), PKC(float, X), p, dp,
+    PK_LAUNCH(pk_newton_dP_kernel, float, nb, PKC(float, Q), PKC(float, X), p, dp,
                        PKM(float, dP));
   return check_launch("pk_newton_dP_kernel");
 }
@@ -1629,10 +1685,10 @@ int hfx_newton_hvp(int dtype, int nb, size_t nk, const void* Q, const void* R, c
   if (int rc = pk_check_nb(nb)) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_newton_hvp_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKC(double, Q), PKC(double, R),
+    PK_LAUNCH(pk_newton_hvp_kernel, double, nb, PKC(double, Q), PKC(double, R),
                        PKC(double, Ft), PKC(double, X), p, dp, T, PKM(double, HX), Hp, part);
   else
-    hipLaunchKernelGGL(pk_newton_hvp_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKC(float, Q), PKC(float, R),
+    PK_LAUNCH(pk_newton_hvp_kernel, float, nb, PKC(float, Q), PKC(float, R),
                        PKC(float, Ft), PKC(float, X), p, dp, T, PKM(float, HX), Hp, part);
   return check_launch("pk_newton_hvp_kernel");
 }
@@ -1642,10 +1698,10 @@ int hfx_newton_prec(int dtype, int nb, size_t nk, const void* RX, const double* 
   if (int rc = pk_check_nb(nb)) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    hipLaunchKernelGGL(pk_newton_prec_kernel<double>, dim3(nk), dim3(64), 0, s, nb, PKC(double, RX), rp, eps, p, st, T,
+    PK_LAUNCH(pk_newton_prec_kernel, double, nb, PKC(double, RX), rp, eps, p, st, T,
                        PKM(double, ZX), zp, part);
   else
-    hipLaunchKernelGGL(pk_newton_prec_kernel<float>, dim3(nk), dim3(64), 0, s, nb, PKC(float, RX), rp, eps, p, st, T,
+    PK_LAUNCH(pk_newton_prec_kernel, float, nb, PKC(float, RX), rp, eps, p, st, T,
                        PKM(float, ZX), zp, part);
   return check_launch("pk_newton_prec_kernel");
 }

@@ -1,13 +1,13 @@
 // QR retraction (solver.py:108-119): U = Q of I - tau G, phases normalised
 // so that R has a positive real diagonal.  That Q is unique for a nonsingular
 // I - tau G (A = QR = (Q D)(D^* R) for any diagonal unitary D), so it does not
-// depend on how the factorisation is computed: here classical Gram-Schmidt
-// with one full re-orthogonalisation per column ("twice is enough"), whose R
-// diagonal is the column norm and so already positive.  I - tau G has singular
+// depend on how the factorisation is computed: here modified Gram-Schmidt
+// with one full re-orthogonalisation pass per column ("twice is enough"),
+// whose R diagonal is the column norm and so already positive.  I - tau G has singular
 // values in [1, sqrt(1 + tau^2 |G|^2)] for skew-Hermitian G, the solvers'
 // case, so the orthogonality of Q is at rounding level.
 //
-// Layout: NB lanes per matrix (NB = 4, 8 or 16; a smaller nb is embedded in
+// Layout: NB lanes per matrix (NB = 4, 8, 16 or 32; a smaller nb is embedded in
 // the top-left block of an NB x NB identity, whose QR is block-diagonal with
 // the identity, exactly), lane r holding row r of every column in registers;
 // 64 / NB matrices per wavefront, four wavefronts per block.  The column dot
@@ -53,17 +53,14 @@ __global__ __launch_bounds__(QR_BLOCK) void qr_retract_kernel(int nb, size_t nk,
     cx<T> v = q[j];
 #pragma unroll
     for (int pass = 0; pass < 2; ++pass) {
-      cx<T> rr[NB];
 #pragma unroll
-      for (int i = 0; i < NB; ++i) {   // r_i = q_i^H v over the column
+      for (int i = 0; i < NB; ++i) {   // v -= q_i (q_i^H v), one column at a time
         if (i < j) {
           const cx<T> p = cmul(q[i], v);
-          rr[i] = cx<T>{group_sum<NB>(p.re), group_sum<NB>(p.im)};
+          const cx<T> ri{group_sum<NB>(p.re), group_sum<NB>(p.im)};
+          v = v - q[i] * ri;
         }
       }
-#pragma unroll
-      for (int i = 0; i < NB; ++i)
-        if (i < j) v = v - q[i] * rr[i];
     }
     const T nrm = sqrt(group_sum<NB>(v.re * v.re + v.im * v.im));
     const T inv = nrm > (T)0 ? (T)1 / nrm : (T)1;
@@ -93,7 +90,7 @@ extern "C" {
 int hfx_qr_retract(int dtype, int nb, size_t nk, const void* G, const double* tau, void* U, void* stream) {
   if (dtype != HFX_DTYPE_C128 && dtype != HFX_DTYPE_C64)
     return set_error(HFX_EINVAL, "hfx_qr_retract: dtype must be HFX_C128 or HFX_C64");
-  if (nb < 1 || nb > 16) return set_error(HFX_EUNSUPPORTED, "hfx_qr_retract: 1 <= nb <= 16 (got %d)", nb);
+  if (nb < 1 || nb > 32) return set_error(HFX_EUNSUPPORTED, "hfx_qr_retract: 1 <= nb <= 32 (got %d)", nb);
   if (!G || !tau || !U) return set_error(HFX_EINVAL, "hfx_qr_retract: NULL argument");
   if (nk == 0) return HFX_OK;
   if (nk > ((size_t)1 << 34)) return set_error(HFX_EINVAL, "hfx_qr_retract: nk=%zu too large", nk);
@@ -101,7 +98,8 @@ int hfx_qr_retract(int dtype, int nb, size_t nk, const void* G, const double* ta
   const bool d = dtype == HFX_DTYPE_C128;
   if (nb <= 4) d ? qr_launch<double, 4>(nb, nk, G, tau, U, s) : qr_launch<float, 4>(nb, nk, G, tau, U, s);
   else if (nb <= 8) d ? qr_launch<double, 8>(nb, nk, G, tau, U, s) : qr_launch<float, 8>(nb, nk, G, tau, U, s);
-  else d ? qr_launch<double, 16>(nb, nk, G, tau, U, s) : qr_launch<float, 16>(nb, nk, G, tau, U, s);
+  else if (nb <= 16) d ? qr_launch<double, 16>(nb, nk, G, tau, U, s) : qr_launch<float, 16>(nb, nk, G, tau, U, s);
+  else d ? qr_launch<double, 32>(nb, nk, G, tau, U, s) : qr_launch<float, 32>(nb, nk, G, tau, U, s);
   return check_launch("qr_retract_kernel");
 }
 

@@ -19,6 +19,7 @@ from typing import Any, Callable
 import numpy as np
 import torch
 
+from . import _lib as L
 from . import _ops as K
 from ._util import KIND, to_tensor
 
@@ -286,8 +287,8 @@ def solve_scf(kernel, P0, n_electrons: float, *, config: SCFConfig | None = None
     """Reference SCF solver (reference_scf.py:124-219)."""
     if config is None:
         config = SCFConfig()
-    if kernel.h.shape[-1] > 16:
-        raise ValueError("the GPU solvers support nb <= 16 bands")
+    if kernel.h.shape[-1] > L.PERK_NB_MAX:
+        raise ValueError(f"the GPU solvers support nb <= {L.PERK_NB_MAX} bands")
     runner = _SCF(kernel, config, fock_build_fn)
     density, F, E, mu, n_iter, converged, hE, hD, hC = runner.run(P0, n_electrons)
     if converged:

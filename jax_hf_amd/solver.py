@@ -816,8 +816,8 @@ def solve_direct_minimization(kernel, P0, n_electrons: float, *, config: SolverC
     if deflation_on and optimizer != "newton":
         raise ValueError("deflation is only supported with optimizer='newton' "
                          f"(got optimizer={config.optimizer!r}).")
-    if kernel.h.shape[-1] > 16:
-        raise ValueError("the GPU solvers support nb <= 16 bands")
+    if kernel.h.shape[-1] > L.PERK_NB_MAX:
+        raise ValueError(f"the GPU solvers support nb <= {L.PERK_NB_MAX} bands")
     if optimizer == "newton":
         from .newton import solve_newton
         if kernel.dtype != torch.complex128 and _precision(kernel, config) != "fp64":

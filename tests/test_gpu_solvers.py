@@ -24,7 +24,7 @@ def _herm(X):
     return 0.5 * (X + np.conj(np.swapaxes(X, -1, -2)))
 
 
-@pytest.mark.parametrize("nb", [1, 2, 3, 4, 5, 8, 13, 16])
+@pytest.mark.parametrize("nb", [1, 2, 3, 4, 5, 8, 13, 16, 17, 24, 32])
 @pytest.mark.parametrize("dt", ["c128", "c64"])
 def test_eigh_batched(dev, nb, dt):
     import torch
@@ -36,7 +36,7 @@ def test_eigh_batched(dev, nb, dt):
     if nb >= 2:
         A[0, 2] = np.diag(np.r_[np.ones(nb - 1), 2.0])       # degenerate block
     cdt = torch.complex128 if dt == "c128" else torch.complex64
-    At = torch.from_numpy(A).to(dev).to(cdt)
+    At = torch.from_numpy(np.ascontiguousarray(A)).to(dev).to(cdt)
     lam = torch.empty((5, 7, nb), dtype=torch.float64, device=dev)
     V = torch.empty_like(At)
     K.eigh(At, lam, V)
@@ -99,10 +99,11 @@ def test_density_and_matmul(dev):
 
 
 @pytest.mark.parametrize("dt", ["c128", "c64"])
-@pytest.mark.parametrize("nb", [6, 16])
+@pytest.mark.parametrize("nb", [6, 16, 24, 32])
 def test_matmul_operand_forms(dev, nb, dt):
     """Every op(A) op(B) [op(C)] form of the batched product against NumPy.
-    nb = 16 runs on the matrix cores (16x16x4 MFMA), nb = 6 on the LDS loop."""
+    nb = 16 runs on the matrix cores (16x16x4 MFMA), nb = 6 on the LDS loop,
+    nb = 24 / 32 on the 256-thread workgroup form."""
     import torch
     from jax_hf_amd import _ops as K
     rng = np.random.default_rng(nb)
@@ -163,7 +164,7 @@ def test_diis_coefficients_match_host_solve(dev):
     assert np.array_equal(_np(coef), np.eye(8)[2])
 
 
-@pytest.mark.parametrize("nb", [5, 16])
+@pytest.mark.parametrize("nb", [5, 16, 32])
 def test_orthonormalize_fp32_basis(dev, nb):
     """Two Newton-Schulz steps take an FP32-rounded unitary to FP64 unitarity,
     moving it by no more than its rounding (the polar factor is the nearest unitary)."""
@@ -184,7 +185,7 @@ def test_orthonormalize_fp32_basis(dev, nb):
 
 
 @pytest.mark.parametrize("dt", ["c128", "c64", "c128h32"])
-@pytest.mark.parametrize("nb", [5, 16])
+@pytest.mark.parametrize("nb", [5, 16, 24])
 def test_scf_resid_dots_matches_numpy(dev, nb, dt):
     """Residuals and the DIIS B row in one pass (reference_scf.py:110-111, 291-294);
     c128h32: FP64 build with the residual history stored in complex64 (nb = 16)."""
@@ -575,7 +576,7 @@ def test_repeated_solves_reuse_cached_state(dev):
 
 
 @pytest.mark.parametrize("dt", ["c128", "c64"])
-@pytest.mark.parametrize("nb", [3, 16])
+@pytest.mark.parametrize("nb", [3, 16, 32])
 def test_extrap_rotate_and_rotate_density(dev, dt, nb):
     """The fused per-k kernels of the DIIS polish against NumPy: Ft =
     U^H herm(sum_v c_v X_v) U, and U <- U W with P = herm(U diag(occ) U^H)."""
@@ -667,7 +668,7 @@ def test_polish_budget_spent_returns_last_iterate(dev):
     assert abs(E - float(r.energy)) < 1e-12
 
 
-@pytest.mark.parametrize("nb", [1, 2, 3, 4, 5, 8, 13, 16])
+@pytest.mark.parametrize("nb", [1, 2, 3, 4, 5, 8, 13, 16, 24, 32])
 @pytest.mark.parametrize("dt", ["c128", "c64"])
 def test_qr_retract_matches_oracle(dev, nb, dt):
     """_qr_retract (solver.py:108-119) on the GPU against the oracle's
