@@ -110,15 +110,24 @@ def fock_bench(J, L, syn, inp, dev, nk, nb, args):
     h = eng.plan.handle
     if planar:
         L.call("hfx_planar_pack", h, 1, L.ptr(P), None, None, 0, 0, L.ptr(Pt), st)
+    vmap = eng._vmap if planar else None
     if not args.no_autotune:
         if planar:
-            _, variant_ms = eng.plan.autotune(Pt, VRc, St, reps=10, planar=True)
+            _, variant_ms = eng.plan.autotune(Pt, VRc, St, reps=10, planar=True, vr_map=vmap)
         else:
             _, variant_ms = eng.plan.autotune(P, eng.VR, Sig, reps=10)
     kname = _exchange_kernel_name(eng.plan.exchange_variant, planar, nk)
     Nk = nk * nk
-    bytes_exch = 16 * Nk * (3 * nb * nb)               # P + VR in, Sigma out
-    bytes_build = 16 * Nk * (4 * nb * nb + nb * nb)    # + h in, F out
+    # exchange launch, counted like the build's algorithmic bytes (VR read once
+    # in the API layout, nb^2 planes): P + VR in, Sigma out.  The kernel reads
+    # each distinct VR plane once (the channels of a layer-resolved V share the
+    # planes of their layer pair), so its traffic is below this figure; the
+    # distinct-plane count is reported beside it.
+    n_vr = nb * nb if vmap is None else int(torch.unique(vmap).numel())
+    bytes_exch = 16 * Nk * (3 * nb * nb)
+    bytes_exch_distinct = 16 * Nk * (2 * nb * nb + n_vr)
+    # build_fock + hf_energy in the API layout (P, h, VR in, Sigma and F out)
+    bytes_build = 16 * Nk * (4 * nb * nb + nb * nb)
 
     def step(evs=None, whole=False):
         # the steps of FockEngine.build, spelled out so the exchange launch
@@ -131,8 +140,8 @@ def fock_bench(J, L, syn, inp, dev, nk, nb, args):
         if evs is not None and not whole:
             evs[0].record()
         if planar:
-            L.call("hfx_selfenergy_planar", h, 1, L.ptr(Pt), L.ptr(VRc), eng.vfull, None, 0, 0, 0, None,
-                   L.ptr(St), None, st)
+            L.call("hfx_selfenergy_planar", h, 1, L.ptr(Pt), L.ptr(VRc), eng.vfull, L.ptr(vmap), None, 0, 0, 0,
+                   None, L.ptr(St), None, st)
         else:
             eng.selfenergy(P, Sig, refP=None, rho=False)
         if evs is not None and not whole:
@@ -173,10 +182,12 @@ def fock_bench(J, L, syn, inp, dev, nk, nb, args):
            "GBps": bytes_build / (dt / n) / 1e9, "algorithmic_bytes_build": bytes_build,
            "energy": float(E), "exchange_variant": L.Plan.VARIANTS[eng.plan.exchange_variant],
            "layout": "slot planes" if planar else "(k, i, j)",
-           "exchange_variants_ms": variant_ms}
+           "distinct_vr_planes": n_vr, "exchange_variants_ms": variant_ms}
     roof = {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": _pmc_traffic(kname, nk, nb),
             "algorithmic_bytes": bytes_exch, "avg_ms": exch_ms,
+            "distinct_vr_bytes": bytes_exch_distinct,
+            "distinct_vr_frac": bytes_exch_distinct / (exch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             # the whole build (pack, exchange, epilogue with the energy partials,
             # energy sum): algorithmic bytes of build_fock + hf_energy over the
             # event time of its launches on the same stream

@@ -64,8 +64,8 @@ int hfx_plan_get_exchange_variant(const hfx_plan* plan);
  * times_ms (optional, 8 doubles) receives the mean time per launch of variants
  * 0 .. 7 (0 = AUTO's choice is not timed and is reported as 0).  Synchronises
  * the stream.  No counterpart in the reference (XLA autotunes its own FFTs). */
-int hfx_exchange_autotune(hfx_plan* plan, const void* P, const void* VR, int vr_full, int planar, void* Sigma,
-                          int reps, double* times_ms, int* best, void* stream);
+int hfx_exchange_autotune(hfx_plan* plan, const void* P, const void* VR, int vr_full, const int* vr_map, int planar,
+                          void* Sigma, int reps, double* times_ms, int* best, void* stream);
 
 /* Bytes of caller-provided float64 scratch needed by the energy reductions
  * (calls sharing one scratch must not overlap). */
@@ -132,10 +132,14 @@ int hfx_planar_pack(const hfx_plan* plan, int nsolve, const void* X, const void*
                     int herm, void* Xt, void* stream);
 /* Sigma_t = exchange of the slot planes Pt (nsolve, nchan, nk1, nk2); VRc the
  * channel-major VR (nb^2, nk1, nk2) when vr_full, else (nk1, nk2); rho (nb, nb)
- * weighted moments of Pt (nsolve == 1 only, mirrored with herm). */
+ * weighted moments of Pt (nsolve == 1 only, mirrored with herm).  vr_map
+ * (optional, device int32 nb^2, vr_full only): channel c reads VR plane
+ * vr_map[c] -- a layer-resolved V that depends only on the layer pair has a
+ * few distinct planes, which are then read from HBM once per build instead of
+ * once per channel (no counterpart in the reference, whose VR is dense). */
 int hfx_selfenergy_planar(const hfx_plan* plan, int nsolve, const void* Pt, const void* VRc, int vr_full,
-                          const int* chans, int nchan, int herm, int ifftshift, const double* w2d, void* Sigt,
-                          void* rho, void* stream);
+                          const int* vr_map, const int* chans, int nchan, int herm, int ifftshift,
+                          const double* w2d, void* Sigt, void* rho, void* stream);
 /* hfx_fock_epilogue reading Sigma from slot planes (mirrored with herm;
  * channels outside a given chans list are zero). */
 int hfx_fock_epilogue_planar(const hfx_plan* plan, const void* h, const void* Sigt, const int* chans, int nchan,

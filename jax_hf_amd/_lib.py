@@ -21,7 +21,7 @@ LIB_PATH = _HERE / "libhfx.so"
 
 HFX_C128 = 0
 HFX_C64 = 1
-ABI_VERSION = 14
+ABI_VERSION = 15
 # Largest band count of the per-k solver kernels (nb <= 16: one wavefront per
 # k-point, nb <= 32: a 256-thread workgroup per k-point).
 PERK_NB_MAX = 32
@@ -43,7 +43,7 @@ _SIGS = {
     "hfx_energy_scratch_bytes": [vp],
     "hfx_plan_set_exchange_variant": [vp, i32],
     "hfx_plan_get_exchange_variant": [vp],
-    "hfx_exchange_autotune": [vp, vp, vp, i32, i32, vp, i32, vp, vp, vp],
+    "hfx_exchange_autotune": [vp, vp, vp, i32, vp, i32, vp, i32, vp, vp, vp],
     "hfx_selfenergy": [vp, vp, vp, vp, i32, vp, i32, i32, i32, vp, vp, vp, vp],
     "hfx_prepare_vr": [vp, vp, i32, vp, i32, vp, vp],
     "hfx_selfenergy_adjoint": [vp, vp, vp, i32, vp, vp],
@@ -76,7 +76,7 @@ _SIGS = {
     "hfx_fock_epilogue_batched": [vp, i32, vp, vp, vp, vp, vp, vp, vp, sz, vp],
     "hfx_batch_scratch_doubles": [i32],
     "hfx_planar_pack": [vp, i32, vp, vp, vp, i32, i32, vp, vp],
-    "hfx_selfenergy_planar": [vp, i32, vp, vp, i32, vp, i32, i32, i32, vp, vp, vp, vp],
+    "hfx_selfenergy_planar": [vp, i32, vp, vp, i32, vp, vp, i32, i32, i32, vp, vp, vp, vp],
     "hfx_fock_epilogue_planar": [vp, vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, f64, vp],
     "hfx_fock_epilogue_planar_batched": [vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, sz, vp],
     "hfx_reduce_batched": [vp, i32, sz, i32, vp, C.c_uint, vp, i32, vp, i32, vp, sz, vp],
@@ -204,11 +204,12 @@ class Plan:
     def exchange_variant(self, v: int):
         call("hfx_plan_set_exchange_variant", self.handle, int(v))
 
-    def autotune(self, P, VR, Sigma, reps: int = 5, planar: bool = False):
+    def autotune(self, P, VR, Sigma, reps: int = 5, planar: bool = False, vr_map=None):
         """Time every exchange-kernel variant on (P, VR) -> Sigma; keep the fastest.
 
         ``planar``: P / Sigma are nb^2 slot planes and VR the channel-major VR
-        (or the scalar VR).  Returns (best variant, {name: ms per launch}).
+        (or the scalar VR), read through ``vr_map`` when given.  Returns
+        (best variant, {name: ms per launch}).
         """
         t = (C.c_double * 8)()
         best = C.c_int()
@@ -217,11 +218,11 @@ class Plan:
         else:
             vfull = 0 if VR.shape[-1] == 1 else 1
         with torch.cuda.device(self.device):
-            call("hfx_exchange_autotune", self.handle, ptr(P), ptr(VR), vfull, int(planar), ptr(Sigma), int(reps), t,
-                 C.byref(best), stream(self.device))
+            call("hfx_exchange_autotune", self.handle, ptr(P), ptr(VR), vfull, ptr(vr_map), int(planar), ptr(Sigma),
+                 int(reps), t, C.byref(best), stream(self.device))
         return best.value, {self.VARIANTS[v]: t[v] for v in range(1, 8)}
 
-    def tune_once(self, VR, vfull: bool, reps: int = 5):
+    def tune_once(self, VR, vfull: bool, reps: int = 5, vr_map=None):
         """Autotune the exchange variant once per plan, on zero slot planes of
         all nb^2 channels with this VR (channel-major when vfull).  Skipped
         while the current stream is capturing; a mesh the slot-plane exchange
@@ -236,7 +237,7 @@ class Plan:
         P = torch.zeros((self.nb * self.nb, self.nk1, self.nk2), dtype=cdt, device=self.device)
         S = torch.empty_like(P)
         try:
-            self.autotune(P, VR, S, reps=reps, planar=True)
+            self.autotune(P, VR, S, reps=reps, planar=True, vr_map=vr_map)
         except HfxError:
             self.exchange_variant = 0
 

@@ -77,14 +77,17 @@ class _BatchFock:
             nch = nb * (nb + 1) // 2 if self.herm else nb * nb
             self.Pt = torch.empty((S, nch, nk1, nk2), **c)
             self.St = torch.empty((S, nch, nk1, nk2), **c)
+            self.vmap = None
             if self.vfull:
                 self.VRc = torch.empty((nb * nb, nk1, nk2), **c)
                 L.call("hfx_planar_pack", self.plan.handle, 1, L.ptr(self.VR.contiguous()), None, None, 0, 0,
                        L.ptr(self.VRc), L.stream(dev))
+                if fock_mod.DEDUP_VR:
+                    self.vmap = fock_mod.vr_plane_map(self.VRc)
             else:
                 self.VRc = self.VR.contiguous()
             if fock_mod.AUTOTUNE:
-                self.plan.tune_once(self.VRc, bool(self.vfull))
+                self.plan.tune_once(self.VRc, bool(self.vfull), vr_map=self.vmap)
         else:
             self.Sig = torch.zeros((S, nk1, nk2, nb, nb), **c)
         self.epi = torch.zeros(256 * S, dtype=torch.float64, device=dev)   # energy partials, 256 per solve
@@ -95,8 +98,8 @@ class _BatchFock:
         if self.planar:
             herm = int(self.herm)
             L.call("hfx_planar_pack", h, self.S, L.ptr(P), None, None, 0, herm, L.ptr(self.Pt), sm)
-            L.call("hfx_selfenergy_planar", h, self.S, L.ptr(self.Pt), L.ptr(self.VRc), self.vfull, None, 0, herm, 0,
-                   None, L.ptr(self.St), None, sm)
+            L.call("hfx_selfenergy_planar", h, self.S, L.ptr(self.Pt), L.ptr(self.VRc), self.vfull,
+                   L.ptr(self.vmap), None, 0, herm, 0, None, L.ptr(self.St), None, sm)
             L.call("hfx_fock_epilogue_planar_batched", h, self.S, L.ptr(self.h), L.ptr(self.St), herm, L.ptr(P),
                    L.ptr(self.w_raw), L.ptr(F), L.ptr(st), L.ptr(self.epi), self.epi.numel(), sm)
             return

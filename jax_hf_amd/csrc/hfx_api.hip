@@ -75,7 +75,7 @@ using namespace hfx;
 
 extern "C" {
 
-int hfx_abi_version(void) { return 14; }
+int hfx_abi_version(void) { return 15; }
 
 int hfx_last_error(char* buf, size_t n) {
   if (!buf || n == 0) return HFX_EINVAL;
@@ -232,8 +232,8 @@ int hfx_planar_pack(const hfx_plan* plan, int nsolve, const void* X, const void*
 }
 
 int hfx_selfenergy_planar(const hfx_plan* plan, int nsolve, const void* Pt, const void* VRc, int vr_full,
-                          const int* chans, int nchan, int herm, int ifftshift, const double* w2d, void* Sigt,
-                          void* rho, void* stream) {
+                          const int* vr_map, const int* chans, int nchan, int herm, int ifftshift,
+                          const double* w2d, void* Sigt, void* rho, void* stream) {
   HFX_REQUIRE(plan && Pt && VRc && Sigt, "hfx_selfenergy_planar: NULL argument");
   HFX_REQUIRE(!rho || (w2d && nsolve == 1), "hfx_selfenergy_planar: rho needs w2d and one solve");
   HFX_REQUIRE(nsolve >= 1 && (long long)nsolve * plan->nb * plan->nb <= (1LL << 30),
@@ -241,7 +241,7 @@ int hfx_selfenergy_planar(const hfx_plan* plan, int nsolve, const void* Pt, cons
   resolve_chans(plan, chans, nchan, herm, nullptr);
   HFX_REQUIRE(nchan >= 0 && nchan <= plan->nb * plan->nb, "hfx_selfenergy_planar: nchan=%d out of range", nchan);
   return exchange_launch(plan, Pt, nullptr, VRc, vr_full ? 1 : 0, chans, nchan, herm ? 1 : 0, w2d, Sigt, rho, 0,
-                         ifftshift ? 1 : 0, 0, (hipStream_t)stream, nsolve, /*planar=*/1);
+                         ifftshift ? 1 : 0, 0, (hipStream_t)stream, nsolve, /*planar=*/1, vr_map);
 }
 
 int hfx_selfenergy_adjoint(const hfx_plan* plan, const void* G, const void* VR, int vr_full, void* out,
@@ -267,8 +267,8 @@ int hfx_plan_set_exchange_variant(hfx_plan* plan, int variant) {
 
 int hfx_plan_get_exchange_variant(const hfx_plan* plan) { return plan ? plan->exch_variant : HFX_EINVAL; }
 
-int hfx_exchange_autotune(hfx_plan* plan, const void* P, const void* VR, int vr_full, int planar, void* Sigma,
-                          int reps, double* times_ms, int* best, void* stream) {
+int hfx_exchange_autotune(hfx_plan* plan, const void* P, const void* VR, int vr_full, const int* vr_map, int planar,
+                          void* Sigma, int reps, double* times_ms, int* best, void* stream) {
   HFX_REQUIRE(plan && P && VR && Sigma, "hfx_exchange_autotune: NULL argument");
   HFX_REQUIRE(reps > 0 && reps <= 1000, "hfx_exchange_autotune: reps must be in 1..1000");
   const hipStream_t s = (hipStream_t)stream;
@@ -289,7 +289,7 @@ int hfx_exchange_autotune(hfx_plan* plan, const void* P, const void* VR, int vr_
       if (r == 0 && hipEventRecord(e0, s) != hipSuccess) rc = set_error(HFX_EHIP, "hipEventRecord failed");
       if (rc == HFX_OK)
         rc = exchange_launch(plan, P, nullptr, VR, vr_full, plan->chans_full, nch, 0, nullptr, Sigma, nullptr,
-                             0, 0, 0, s, 1, planar ? 1 : 0);
+                             0, 0, 0, s, 1, planar ? 1 : 0, planar ? vr_map : nullptr);
     }
     float ms = 0.f;
     if (rc == HFX_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||

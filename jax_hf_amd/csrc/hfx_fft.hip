@@ -229,6 +229,8 @@ struct ExchArgs {
   const cx<T>* refP;   // may be null
   const cx<T>* VR;     // (n1, n2, vb, vb)
   const int* chans;    // channel list (i * nb + j)
+  const int* vmap;     // full VR: VR plane of channel c is vmap[c] (null: plane c); planes that
+                       // repeat (a V depending only on the layer pair) are then read once from HBM
   const double* w2d;   // (n1, n2) weights for the channel moment, may be null
   cx<T>* Sigma;        // (n1, n2, nb, nb)
   cx<double>* rho;     // (nb, nb) weighted channel moments, may be null
@@ -255,6 +257,11 @@ __host__ __device__ constexpr int lds_pitch(int n2) { return (n2 % 2 == 0) ? n2 
 // the planar layout the pointers move to this workgroup's planes, after which
 // k-point e of the channel is element e (exch_ix / exch_vix).
 template <typename T>
+__device__ __forceinline__ int vr_chan(const ExchArgs<T>& a, int c) {
+  return a.vmap ? a.vmap[c] : c;
+}
+
+template <typename T>
 __device__ __forceinline__ int exch_channel_at(ExchArgs<T>& a, int b) {
   const int s = b / a.nchan;
   const int c = a.chans[b - s * a.nchan];
@@ -263,7 +270,7 @@ __device__ __forceinline__ int exch_channel_at(ExchArgs<T>& a, int b) {
     a.P += off;
     if (a.refP) a.refP += off;
     a.Sigma += off;
-    if (a.vfull) a.VR += (size_t)c * nk;
+    if (a.vfull) a.VR += (size_t)vr_chan(a, c) * nk;
   } else if (s > 0) {
     const size_t off = (size_t)s * a.plane_stride;
     a.P += off;
@@ -304,7 +311,7 @@ __device__ __forceinline__ int exch_plane_begin(ExchArgs<T>& a) {
 }
 template <typename T>
 __device__ __forceinline__ void exch_plane_vr(ExchArgs<T>& a, int c) {
-  if (a.vfull) a.VR += (size_t)c * a.g.n1 * a.g.n2;
+  if (a.vfull) a.VR += (size_t)vr_chan(a, c) * a.g.n1 * a.g.n2;
 }
 
 // Index of channel c at k-point e in P / refP / Sigma, and in VR.
@@ -314,7 +321,7 @@ __device__ __forceinline__ size_t exch_ix(const ExchArgs<T>& a, size_t e, int c,
 }
 template <typename T>
 __device__ __forceinline__ size_t exch_vix(const ExchArgs<T>& a, size_t e, int c, int nb2) {
-  return (a.planar || !a.vfull) ? e : e * nb2 + c;
+  return (a.planar || !a.vfull) ? e : e * nb2 + vr_chan(a, c);
 }
 
 template <typename T, int BLOCK>
@@ -652,7 +659,7 @@ __device__ __forceinline__ void v2_stage(cx<T>* plane, const cx<T>* tw, const Ex
       }
       if constexpr (VRM) {   // prefetch VR at the output positions (k1 = base + r NS, k2 = line)
         const int base = (j / NS) * NS * R + k;
-        const int vc = a.vfull ? c : 0, vstride = a.vfull ? nb2 : 1;
+        const int vc = a.vfull ? vr_chan(a, c) : 0, vstride = a.vfull ? nb2 : 1;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const size_t e = (size_t)(base + r * NS) * N2 + line;
@@ -1131,17 +1138,19 @@ static int launch_exch(const ExchArgs<T>& a, int variant, hipStream_t s) {
 
 int exchange_launch(const hfx_plan* plan, const void* P, const void* refP, const void* VR, int vfull,
                     const int* chans, int nchan, int herm, const double* w2d, void* Sigma, void* rho,
-                    int mode, int shift, int conjv, hipStream_t s, int nsolve, int planar) {
+                    int mode, int shift, int conjv, hipStream_t s, int nsolve, int planar, const int* vmap) {
   if (nchan <= 0 || nsolve <= 0) return HFX_OK;
   const int remap = ((nchan * nsolve) % 8 == 0) ? 1 : 0;
   const size_t stride = (size_t)plan->n1 * plan->n2 * plan->nb * plan->nb;
   if (plan->dtype == HFX_DTYPE_C128) {
-    ExchArgs<double> a{(const cx<double>*)P, (const cx<double>*)refP, (const cx<double>*)VR, chans, w2d,
+    ExchArgs<double> a{(const cx<double>*)P, (const cx<double>*)refP, (const cx<double>*)VR, chans,
+                       vfull ? vmap : nullptr, w2d,
                        (cx<double>*)Sigma, (cx<double>*)rho, nchan, herm, vfull, remap, nsolve, stride, planar,
                        /*mode=*/mode, /*shift=*/shift, /*conjv=*/conjv, plan->g64};
     return launch_exch<double>(a, plan->exch_variant, s);
   }
-  ExchArgs<float> a{(const cx<float>*)P, (const cx<float>*)refP, (const cx<float>*)VR, chans, w2d,
+  ExchArgs<float> a{(const cx<float>*)P, (const cx<float>*)refP, (const cx<float>*)VR, chans,
+                    vfull ? vmap : nullptr, w2d,
                     (cx<float>*)Sigma, (cx<double>*)rho, nchan, herm, vfull, remap, nsolve, stride, planar,
                     /*mode=*/mode, /*shift=*/shift, /*conjv=*/conjv, plan->g32};
   return launch_exch<float>(a, plan->exch_variant, s);

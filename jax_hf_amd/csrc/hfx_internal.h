@@ -38,7 +38,8 @@ struct hfx_plan {
 namespace hfx {
 int exchange_launch(const hfx_plan* plan, const void* P, const void* refP, const void* VR, int vfull,
                     const int* chans, int nchan, int herm, const double* w2d, void* Sigma, void* rho,
-                    int mode, int shift, int conjv, hipStream_t s, int nsolve = 1, int planar = 0);
+                    int mode, int shift, int conjv, hipStream_t s, int nsolve = 1, int planar = 0,
+                    const int* vmap = nullptr);
 int exchange_max_plane(int dtype);
 int epilogue_seg_launch(const hfx_plan* p, int nsolve, const void* h, const void* Sig, const void* P, const double* w2d,
                         void* F, double* st, double* part, hipStream_t s);

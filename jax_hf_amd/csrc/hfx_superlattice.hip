@@ -163,8 +163,8 @@ static int sl_build(const SlGeom& g, const void* P, const void* refP, const int*
                        dj, p2d, goff, (cx<T>*)planes);
     int rc = check_launch("sl_scatter_kernel");
     if (rc) return rc;
-    rc = hfx_selfenergy_planar(plan, g.ndelta, planes, Vlag, 0, nullptr, g.dorb * g.dorb, 0, 0, nullptr, splanes,
-                               nullptr, (void*)s);
+    rc = hfx_selfenergy_planar(plan, g.ndelta, planes, Vlag, 0, nullptr, nullptr, g.dorb * g.dorb, 0, 0, nullptr,
+                               splanes, nullptr, (void*)s);
     if (rc) return rc;
   }
   if (hartree) {

@@ -45,6 +45,10 @@ PLANAR = os.environ.get("HFX_PLANAR", "1") != "0"
 # per-mesh default).
 AUTOTUNE = os.environ.get("HFX_AUTOTUNE", "1") != "0"
 
+# Channels whose VR planes are identical read one copy (vr_plane_map; set
+# HFX_DEDUP_VR=0 or assign False to read every channel's own plane).
+DEDUP_VR = os.environ.get("HFX_DEDUP_VR", "1") != "0"
+
 # ---------------------------------------------------------------------------
 # block specs (linalg.py:19-90)
 # ---------------------------------------------------------------------------
@@ -104,6 +108,48 @@ def _block_sets(kind, data, nb):
 # Engine: one Fock build path with preallocated device buffers
 # ---------------------------------------------------------------------------
 
+def vr_plane_map(VRc):
+    """Channel -> VR plane map of a channel-major VR (nb^2, nk1, nk2).
+
+    Channel c is mapped to the first plane identical to its own, as a device
+    int32 tensor (None when all planes differ): a layer-resolved V that depends
+    only on the layer pair has a handful of distinct planes, and the exchange
+    then reads each from HBM once per build instead of once per channel.
+    Planes are grouped by a random projection and confirmed equal element for
+    element, so the map never joins planes that differ in any bit.
+    """
+    n = int(VRc.shape[0])
+    if n < 2:
+        return None
+    flat = torch.view_as_real(VRc.reshape(n, -1)).reshape(n, -1).to(torch.float64)
+    gen = torch.Generator(device="cpu").manual_seed(0x5EED)
+    r = torch.randn(flat.shape[1], dtype=torch.float64, generator=gen).to(flat.device)
+    key = (flat @ r).cpu().numpy()
+    scale = float(np.abs(key).max()) or 1.0
+    order = np.argsort(key, kind="stable")
+    rep = np.arange(n)
+    i = 0
+    while i < n:                       # clusters of (nearly) equal projections
+        j = i + 1
+        while j < n and key[order[j]] - key[order[j - 1]] <= 1e-12 * scale:
+            j += 1
+        if j - i > 1:
+            members = sorted(int(m) for m in order[i:j])
+            left = members
+            while len(left) > 1:
+                head = left[0]
+                idx = torch.tensor(left[1:], device=flat.device)
+                same = (flat[idx] == flat[head]).all(dim=1).cpu().numpy()
+                for m, eq in zip(left[1:], same):
+                    if eq:
+                        rep[m] = head
+                left = [m for m, eq in zip(left[1:], same) if not eq]
+        i = j
+    if np.all(rep == np.arange(n)):
+        return None
+    return torch.tensor(rep, dtype=torch.int32, device=VRc.device)
+
+
 class FockEngine:
     """Fock build + energy for fixed (h, VR, refP, HH, contact) on one device.
 
@@ -148,9 +194,10 @@ class FockEngine:
         self.E = torch.zeros((), **f64)
         self._sig_ws = None
         self._vrc = None
+        self._vmap = None
         self._pl_ws = None
         if AUTOTUNE and PLANAR and self.include_exchange and nb <= 64 and VR is not None:
-            self.plan.tune_once(self.vr_planar(), bool(self.vfull))
+            self.plan.tune_once(self.vr_planar(), bool(self.vfull), vr_map=self._vmap)
 
     # -- pieces -------------------------------------------------------------
     def selfenergy(self, P, out, refP=None, chans=None, nchan=0, herm=None, rho=True, ifftshift=False):
@@ -178,6 +225,8 @@ class FockEngine:
                 self._vrc = torch.empty((nb * nb, nk1, nk2), dtype=self.VR.dtype, device=self.dev)
                 L.call("hfx_planar_pack", self.plan.handle, 1, L.ptr(self.VR.contiguous()), None, None, 0, 0,
                        L.ptr(self._vrc), L.stream(self.dev))
+                if DEDUP_VR:
+                    self._vmap = vr_plane_map(self._vrc)
         return self._vrc
 
     def planar_workspace(self, nchan):
@@ -239,7 +288,8 @@ class FockEngine:
         herm = int(self.herm)
         L.call("hfx_planar_pack", h, 1, L.ptr(P), L.ptr(refP), L.ptr(chans), int(nchan), herm, L.ptr(Pt), st)
         rho_ex = self.need_rho and not moments_separately
-        L.call("hfx_selfenergy_planar", h, 1, L.ptr(Pt), L.ptr(self.vr_planar()), self.vfull, L.ptr(chans),
+        VRc = self.vr_planar()
+        L.call("hfx_selfenergy_planar", h, 1, L.ptr(Pt), L.ptr(VRc), self.vfull, L.ptr(self._vmap), L.ptr(chans),
                int(nchan), herm, 0, L.ptr(self.w2d) if rho_ex else None, L.ptr(St),
                L.ptr(self.rho) if rho_ex else None, st)
         if self.need_rho and moments_separately:

@@ -295,7 +295,7 @@ def test_planar_exchange_matches_oracle(dev, nk, cdt):
                     plan.exchange_variant = v
                     S = torch.full_like(X, float("nan"))
                     eng.rho.zero_()
-                    L.call("hfx_selfenergy_planar", plan.handle, 1, L.ptr(X), L.ptr(VRc), eng.vfull, L.ptr(ct), nch,
+                    L.call("hfx_selfenergy_planar", plan.handle, 1, L.ptr(X), L.ptr(VRc), eng.vfull, None, L.ptr(ct), nch,
                            herm, shift, L.ptr(eng.w2d), L.ptr(S), L.ptr(eng.rho), st)
                     Sn = _np(S)
                     for t, c in enumerate(cl):
@@ -329,3 +329,44 @@ def test_planar_exchange_matches_oracle(dev, nk, cdt):
                 e = _np(E)
                 etol = 1e-13 if cdt == "c128" else 1e-5
                 assert abs(e[0] - e[1]) <= etol * max(1.0, abs(e[0])), (e, herm, chans)
+
+
+@pytest.mark.parametrize("cdt", ["c128", "c64"])
+@pytest.mark.parametrize("herm", [False, True])
+def test_vr_plane_map_build_matches_dense(dev, cdt, herm, monkeypatch):
+    """A layer-resolved V with repeated planes: the Fock build reading one copy
+    per distinct plane equals the build reading every channel's own plane bit
+    for bit, and the oracle to the FP64 / complex64 bar."""
+    import torch
+    import jax_hf_amd as J
+    from jax_hf_amd import fock as fock_mod
+    nk, nb = 24, 8
+    rng = np.random.default_rng(77)
+    npd = np.complex128 if cdt == "c128" else np.complex64
+    h = syn.tight_binding(nk, nk, nb, seed=3)
+    V = syn.coulomb_layer(nk, nk, nb, 4)
+    w = np.full((nk, nk), 1.0 / nk ** 2)
+    P = syn.haar_density(nk, nk, nb, rng)
+    out = {}
+    for dedup in (True, False):
+        monkeypatch.setattr(fock_mod, "DEDUP_VR", dedup)
+        G = J.HartreeFockKernel(w, h.astype(npd), V, 0.1)
+        eng = G.fock_engine(herm=herm)
+        eng.vr_planar()
+        assert (eng._vmap is not None) == dedup
+        if dedup:
+            assert int(torch.unique(eng._vmap).numel()) == 4
+        Pt = torch.from_numpy(P.astype(npd)).to(dev)
+        F = torch.empty_like(Pt)
+        S = torch.empty_like(Pt)
+        eng.build(Pt, F, Sigma_out=S, energy=True, refP=None)
+        out[dedup] = (F.clone(), S.clone(), float(eng.E))
+    assert torch.equal(out[True][0], out[False][0]) and torch.equal(out[True][1], out[False][1])
+    assert out[True][2] == out[False][2]
+    K = O.OracleKernel(w, h, V, 0.1)
+    b = K.as_args()
+    S0, H0, F0 = O.build_fock(P, h=b["h"], VR=b["VR"], refP=b["refP"], HH=b["HH"], w2d=K.w2d,
+                              include_exchange=True, include_hartree=False,
+                              exchange_hermitian_channel_packing=False)   # P Hermitian: the same Sigma
+    tol = 1e-11 if cdt == "c128" else 2e-6
+    assert _rel(_np(out[True][0]).astype(np.complex128), F0) < tol
