@@ -155,9 +155,10 @@ class _BatchCG:
     def fock(self, P, F):
         self.fk(P, F, self.st)
 
-    # Newton steps per mu solve inside the captured iteration; a solve that needs
-    # more is finished after the iteration's readback (its done flag is in st)
-    MU_LOOP_STEPS = 8
+    # mu steps per mu solve inside the captured iteration after the bracket pass
+    # (which takes the first step); a solve that needs more is finished after
+    # the iteration's readback (its done flag is in st)
+    MU_LOOP_STEPS = 4
 
     def mu(self, steps=None):
         n = self.cfg.mu_maxiter if steps is None else min(steps, self.cfg.mu_maxiter)
@@ -330,7 +331,7 @@ class _BatchPolish:
     in the single polish, the update from iterate k is launched before the
     host has read iterate k's residuals."""
 
-    MU_STEPS = 10    # Newton steps per mu solve in an update; more are run on the readback
+    MU_STEPS = 4     # mu steps per mu solve in an update after the bracket pass; more are run on the readback
     EIGH_TOL_PER_TOL = 1e-5   # update eigh tolerance per unit of the polish tolerance (as in solve)
 
     def __init__(self, kernels, n_electrons, nh=6):

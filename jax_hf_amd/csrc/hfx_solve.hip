@@ -203,31 +203,99 @@ __device__ __forceinline__ bool mu_converged(double g, double Z, double N, doubl
          hi - lo <= tol_mu;
 }
 
-// One safeguarded Newton update of mu (solver.py:80-99) from the residual g
-// and slope Z at mu: bracket update, Newton step, bisection when the step
-// leaves the bracket.  Returns true when it was a Newton step d small enough
-// that the error it leaves is below the convergence tolerance -- with
-// |N''| <= N'/T the quadratic term is at most d^2 / (2T) -- so the solve can
-// end after this update instead of spending one more step to confirm it.
-__device__ __forceinline__ bool mu_newton_update(double g, double Z, double& mu, double Tj, double& lo,
-                                                 double& hi) {
+// The four sums of one mu evaluation, accumulated per eigenvalue: N and its
+// first three derivatives in mu, N^(j) = sum_e w_e f^(j)(x_e) / T^j with
+// x_e = (mu - eps_e) / T, f' = f (1 - f), f'' = f' (1 - 2 f) and
+// f^(3) = f' (1 - 6 f + 6 f^2): one exponential for all four.
+constexpr int MU_NS = 4;
+__device__ __forceinline__ void mu_accum(double (&acc)[MU_NS], double mu, double ev, double wk, double Tj) {
+  const double f = sigm((mu - ev) / Tj);
+  const double q = wk * f * (1.0 - f) / Tj;
+  acc[0] += wk * f;
+  acc[1] += q;
+  acc[2] += q * (1.0 - 2.0 * f) / Tj;
+  acc[3] += q * ((1.0 - 6.0 * f) + 6.0 * f * f) / (Tj * Tj);
+}
+
+// One safeguarded update of mu from g = N - target and N^(1..3) at mu: the
+// step is the root of the cubic Taylor model
+//   g + N^(1) d + N^(2) d^2 / 2 + N^(3) d^3 / 6 = 0
+// reached by Newton iterations on the model from the Newton step -g / N^(1)
+// (solver.py:80-99 takes that Newton step itself; both iterations converge to
+// the same root, this one with a fourth-order instead of a second-order
+// error, so a warm-started solve needs one or two evaluations instead of
+// three or four).  A model step that fails (no positive model slope, a
+// non-finite or a much longer step than Newton's) falls back to the Newton
+// step; a step leaving the bracket (updated from the sign of g) becomes the
+// bisection midpoint, as in the reference loop.  Returns true
+// when the step leaves an error below the convergence tolerance, so the solve
+// ends without a confirming evaluation: |f^(4)| <= f' bounds the model's
+// remainder by d^4 / (24 T^3) (|d| <= T / 10 keeps N^(1) at the intermediate
+// point within 11 % of its value at mu); a Newton step's by d^2 / (2 T).
+__device__ __forceinline__ bool mu_cubic_update(double g, double N1, double N2, double N3, double& mu, double Tj,
+                                                double& lo, double& hi) {
   if (g < 0) lo = mu;
   if (g > 0) hi = mu;
-  const double d = g / fmax(Z, 1e-18);
-  double mu_new = mu - d;
+  const double Z = fmax(N1, 1e-18);
+  const double dn = -g / Z;   // Newton step
+  double d = dn;
+  bool model = true;
+#pragma unroll 1
+  for (int i = 0; i < 6; ++i) {
+    const double pm = g + d * (Z + d * (0.5 * N2 + d * (N3 / 6.0)));
+    const double dp = Z + d * (N2 + 0.5 * d * N3);
+    if (!(dp > 0.0)) { model = false; break; }
+    d -= pm / dp;
+  }
+  if (!model || !isfinite(d) || fabs(d) > 2.0 * fabs(dn)) {
+    d = dn;
+    model = false;
+  }
+  double mu_new = mu + d;
   const double mu_bis = 0.5 * (lo + hi);
-  bool newton = true;
+  bool step = true;
   if (mu_new <= lo || mu_new >= hi) {
     mu_new = mu_bis;
-    newton = false;
+    step = false;
   }
   mu_new = fmin(fmax(mu_new, lo), hi);
   if (!isfinite(mu_new)) {
     mu_new = mu_bis;
-    newton = false;
+    step = false;
   }
   mu = mu_new;
-  return newton && d * d <= 0.5 * Tj * 8.9e-16 * fmax(fabs(mu_new), Tj);
+  if (!step) return false;
+  const double tol = 8.9e-16 * fmax(fabs(mu_new), Tj);
+  if (model) {
+    const double d2 = d * d;
+    return fabs(d) <= 0.1 * Tj && d2 * d2 <= 3.0 * Tj * Tj * Tj * tol;   // d^4 / (24 T^3) <= tol / 8
+  }
+  return d * d <= 0.5 * Tj * tol;
+}
+
+// Block-wide sums of NV interleaved values per entry, src[stride q + v] for
+// q < n (strided per thread, then a block tree; every block evaluates the
+// same expression tree, so all blocks get bitwise-identical totals).  All
+// threads receive the totals.
+template <int NV>
+__device__ __forceinline__ void block_gather(const double* src, int n, int stride, double* sh, double (&tot)[NV]) {
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] += src[(size_t)stride * q + v];
+  }
+  block_sum<NV>(acc, sh);
+  __shared__ double t[NV];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) t[v] = acc[v];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int v = 0; v < NV; ++v) tot[v] = t[v];
+  __syncthreads();
 }
 
 // Block-wide sum of the first n entries of a global array (strided per
@@ -397,100 +465,130 @@ struct MuStepArgs {
   double target, T, mu0;
   int mu0_from_state;
   double* st;
-  double* part;        // 2 * gridDim.x doubles
+  double* part;        // 6 * gridDim.x doubles
   double* p_out;
 };
 
-// Each of the bracket and Newton steps is two launches: block partials, then
-// one workgroup that combines them and updates the state vector (cheaper in a
-// graph than a completion ticket, whose device-scope fence writes back the
-// XCD's L2).  A step after convergence returns at once in both launches.
-__global__ __launch_bounds__(MUS_BLOCK) void mu_bracket_part(MuStepArgs a) {
-  __shared__ double sh[2 * (MUS_BLOCK / 64)];
-  const size_t n = a.nk * a.nb, stride = (size_t)gridDim.x * MUS_BLOCK;
-  double lo = INFINITY, hi = -INFINITY;
+// Block min / max of eps and the four sums at mu into out[0..5]: lo, hi, N,
+// N^(1), N^(2), N^(3) (min / max only with `minmax`).
+__device__ __forceinline__ void mu_eval_block(const double* eps, const double* w, size_t n, int nb, size_t stride,
+                                              double mu, double Tj, bool minmax, double* sh, double* out) {
+  double lo = INFINITY, hi = -INFINITY, acc[MU_NS] = {0.0, 0.0, 0.0, 0.0};
   for (size_t e = (size_t)blockIdx.x * MUS_BLOCK + threadIdx.x; e < n; e += stride) {
-    lo = fmin(lo, a.eps[e]);
-    hi = fmax(hi, a.eps[e]);
+    const double ev = eps[e];
+    if (minmax) {
+      lo = fmin(lo, ev);
+      hi = fmax(hi, ev);
+    }
+    mu_accum(acc, mu, ev, w[(unsigned)e / (unsigned)nb], Tj);   // n < 2^32
   }
-  lo = wave_min(lo);
-  hi = wave_max(hi);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = MUS_BLOCK / 64;
-  if (lane == 0) { sh[wid] = lo; sh[nw + wid] = hi; }
-  __syncthreads();
+  block_sum<MU_NS>(acc, sh);
+  if (minmax) {
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = MUS_BLOCK / 64;
+    if (lane == 0) { sh[wid] = lo; sh[nw + wid] = hi; }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int q = 0; q < nw; ++q) { lo = fmin(lo, sh[q]); hi = fmax(hi, sh[nw + q]); }
+  }
   if (threadIdx.x == 0) {
-    for (int q = 0; q < nw; ++q) { lo = fmin(lo, sh[q]); hi = fmax(hi, sh[nw + q]); }
-    a.part[2 * blockIdx.x] = lo;
-    a.part[2 * blockIdx.x + 1] = hi;
+    out[0] = lo;
+    out[1] = hi;
+#pragma unroll
+    for (int v = 0; v < MU_NS; ++v) out[2 + v] = acc[v];
   }
 }
 
-__global__ __launch_bounds__(MUS_BLOCK) void mu_bracket_fin(MuStepArgs a, int nblk) {
-  __shared__ double sh[2 * (MUS_BLOCK / 64)];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = MUS_BLOCK / 64;
-  double mn = INFINITY, mx = -INFINITY;   // min / max are order-free
-  for (int b = threadIdx.x; b < nblk; b += MUS_BLOCK) {
-    mn = fmin(mn, a.part[2 * b]);
-    mx = fmax(mx, a.part[2 * b + 1]);
+// Convergence test or one update from the sums at st[MU]; bracket in st.
+__device__ __forceinline__ void mu_finish_step(double* st, const double (&t)[MU_NS], double target, double Tj) {
+  const double mu = st[HFX_ST_MU];
+  const double g = t[0] - target;   // solver.py:80-99
+  double lo = st[HFX_ST_MULO], hi = st[HFX_ST_MUHI];
+  if (mu_converged(g, t[1], t[0], mu, Tj, lo, hi)) {
+    st[HFX_ST_MUDONE] = 1.0;
+    return;
+  }
+  double mu_new = mu;
+  const bool fin = mu_cubic_update(g, t[1], t[2], t[3], mu_new, Tj, lo, hi);
+  st[HFX_ST_MU] = mu_new;
+  st[HFX_ST_MULO] = lo;
+  st[HFX_ST_MUHI] = hi;
+  if (fin) st[HFX_ST_MUDONE] = 1.0;
+}
+
+// Bracket lo / hi = min / max(eps) -/+ 50 T (solver.py:70-72), mu0 clamped into
+// it; when mu0 was inside (so the sums t were taken at the clamped mu) the
+// first step as well.
+__device__ __forceinline__ void mu_finish_bracket(double* st, double mu0, double mn, double mx,
+                                                  const double (&t)[MU_NS], double target, double Tj) {
+  const double blo = mn - 50.0 * Tj, bhi = mx + 50.0 * Tj;
+  const double mu = fmin(fmax(mu0, blo), bhi);
+  st[HFX_ST_MU] = mu;
+  st[HFX_ST_MULO] = blo;
+  st[HFX_ST_MUHI] = bhi;
+  st[HFX_ST_MUDONE] = 0.0;
+  if (mu == mu0) mu_finish_step(st, t, target, Tj);
+}
+
+// Block min / max over the first n (lo, hi) pairs at stride 6, order-free; the
+// result in thread 0.
+__device__ __forceinline__ void block_minmax6(const double* src, int n, double* sh, double& mn, double& mx) {
+  mn = INFINITY;
+  mx = -INFINITY;
+  for (int b = threadIdx.x; b < n; b += blockDim.x) {
+    mn = fmin(mn, src[6 * b]);
+    mx = fmax(mx, src[6 * b + 1]);
   }
   mn = wave_min(mn);
   mx = wave_max(mx);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
   if (lane == 0) { sh[wid] = mn; sh[nw + wid] = mx; }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
     for (int q = 0; q < nw; ++q) { mn = fmin(mn, sh[q]); mx = fmax(mx, sh[nw + q]); }
-    const double Tj = fmax(a.T, 1e-12);
-    const double blo = mn - 50.0 * Tj, bhi = mx + 50.0 * Tj;   // solver.py:70-72
-    double mu = a.mu0_from_state ? a.st[HFX_ST_MU] : a.mu0;
-    mu = fmin(fmax(mu, blo), bhi);
-    a.st[HFX_ST_MU] = mu;
-    a.st[HFX_ST_MULO] = blo;
-    a.st[HFX_ST_MUHI] = bhi;
-    a.st[HFX_ST_MUDONE] = 0.0;
+  __syncthreads();
+}
+
+// The bracket and the first evaluation, at the warm-start mu0, share one pass
+// over eps: block partials (6 doubles per block), then one workgroup that
+// sets the bracket and takes the first step.  Each later step is two launches
+// as well (cheaper in a graph than a completion ticket, whose device-scope
+// fence writes back the XCD's L2); a step after convergence returns at once
+// in both launches.
+__global__ __launch_bounds__(MUS_BLOCK) void mu_bracket_part(MuStepArgs a) {
+  __shared__ double sh[MU_NS * (MUS_BLOCK / 64)];
+  const size_t n = a.nk * a.nb, stride = (size_t)gridDim.x * MUS_BLOCK;
+  const double mu0 = a.mu0_from_state ? a.st[HFX_ST_MU] : a.mu0;
+  mu_eval_block(a.eps, a.w, n, a.nb, stride, mu0, fmax(a.T, 1e-12), true, sh, a.part + 6 * (size_t)blockIdx.x);
+}
+
+__global__ __launch_bounds__(MUS_BLOCK) void mu_bracket_fin(MuStepArgs a, int nblk) {
+  __shared__ double sh[MU_NS * (MUS_BLOCK / 64)];
+  double mn, mx, t[MU_NS];
+  block_minmax6(a.part, nblk, sh, mn, mx);
+  block_gather<MU_NS>(a.part + 2, nblk, 6, sh, t);
+  if (threadIdx.x == 0) {
+    const double mu0 = a.mu0_from_state ? a.st[HFX_ST_MU] : a.mu0;
+    mu_finish_bracket(a.st, mu0, mn, mx, t, a.target, fmax(a.T, 1e-12));
   }
 }
 
 __global__ __launch_bounds__(MUS_BLOCK) void mu_step_part(MuStepArgs a) {
   if (a.st[HFX_ST_MUDONE] != 0.0) return;
-  __shared__ double sh[2 * (MUS_BLOCK / 64)];
+  __shared__ double sh[MU_NS * (MUS_BLOCK / 64)];
   const size_t n = a.nk * a.nb, stride = (size_t)gridDim.x * MUS_BLOCK;
-  const double Tj = fmax(a.T, 1e-12);
-  const double mu = a.st[HFX_ST_MU];
-  double acc[2] = {0.0, 0.0};
-  for (size_t e = (size_t)blockIdx.x * MUS_BLOCK + threadIdx.x; e < n; e += stride) {
-    const double pv = sigm((mu - a.eps[e]) / Tj);
-    const double wk = a.w[(unsigned)e / (unsigned)a.nb];   // n < 2^32
-    acc[0] += wk * pv;
-    acc[1] += wk * pv * (1.0 - pv) / Tj;
-  }
-  block_sum<2>(acc, sh);
-  if (threadIdx.x == 0) {
-    a.part[2 * blockIdx.x] = acc[0];
-    a.part[2 * blockIdx.x + 1] = acc[1];
-  }
+  mu_eval_block(a.eps, a.w, n, a.nb, stride, a.st[HFX_ST_MU], fmax(a.T, 1e-12), false, sh,
+                a.part + 6 * (size_t)blockIdx.x);
 }
 
 __global__ __launch_bounds__(MUS_BLOCK) void mu_step_fin(MuStepArgs a, int nblk) {
   if (a.st[HFX_ST_MUDONE] != 0.0) return;
-  __shared__ double sh[2 * (MUS_BLOCK / 64)];
-  const double Tj = fmax(a.T, 1e-12);
-  const double mu = a.st[HFX_ST_MU];
-  double N, Z;
-  block_gather2(a.part, nblk, sh, N, Z);
-  if (threadIdx.x == 0) {
-    const double g = N - a.target;   // solver.py:80-99
-    double lo = a.st[HFX_ST_MULO], hi = a.st[HFX_ST_MUHI];
-    if (mu_converged(g, Z, N, mu, Tj, lo, hi)) {
-      a.st[HFX_ST_MUDONE] = 1.0;
-    } else {
-      double mu_new = mu;
-      const bool fin = mu_newton_update(g, Z, mu_new, Tj, lo, hi);
-      a.st[HFX_ST_MU] = mu_new;
-      a.st[HFX_ST_MULO] = lo;
-      a.st[HFX_ST_MUHI] = hi;
-      if (fin) a.st[HFX_ST_MUDONE] = 1.0;
-    }
-  }
+  __shared__ double sh[MU_NS * (MUS_BLOCK / 64)];
+  double t[MU_NS];
+  block_gather<MU_NS>(a.part + 2, nblk, 6, sh, t);
+  if (threadIdx.x == 0) mu_finish_step(a.st, t, a.target, fmax(a.T, 1e-12));
 }
 
 __global__ __launch_bounds__(MUS_BLOCK) void mu_occ_kernel(MuStepArgs a) {
@@ -505,10 +603,11 @@ __global__ __launch_bounds__(MUS_BLOCK) void mu_occ_kernel(MuStepArgs a) {
 // solve has its own 32-double state vector (st + HFX_ST_LEN * s), block
 // partials and completion ticket; frozen solves skip their launches.
 constexpr int SEG_BLOCKS = 32;   // blocks per solve (k-points per solve <= 8192 at 256 threads each)
+constexpr int MU_SEG_VALS = 6;   // mu partials per block: lo, hi, N, N^(1), N^(2), N^(3)
 
 struct SegScratch {
   double* s1;          // [nseg][RED_MAX_VALS][SEG_BLOCKS]
-  double* mpart;       // [nseg][2][SEG_BLOCKS]
+  double* mpart;       // [nseg][MU_SEG_VALS * SEG_BLOCKS]: per block lo, hi and the four mu sums
   unsigned* tickets;   // [nseg][2]
 };
 
@@ -516,7 +615,7 @@ __host__ __device__ inline SegScratch seg_scratch(double* base, int nseg) {
   SegScratch t;
   t.s1 = base;
   t.mpart = base + (size_t)nseg * RED_MAX_VALS * SEG_BLOCKS;
-  t.tickets = reinterpret_cast<unsigned*>(t.mpart + (size_t)nseg * 2 * SEG_BLOCKS);
+  t.tickets = reinterpret_cast<unsigned*>(t.mpart + (size_t)nseg * MU_SEG_VALS * SEG_BLOCKS);
   return t;
 }
 
@@ -588,39 +687,26 @@ struct MuSegArgs {
   SegScratch scr;
 };
 
+// Batched form of the bracket pass with the first evaluation (the last block
+// of each solve finishes it through the solve's completion ticket).
 __global__ __launch_bounds__(MUS_BLOCK) void mu_bracket_seg_kernel(MuSegArgs a) {
-  __shared__ double sh[2 * (MUS_BLOCK / 64)];
+  __shared__ double sh[MU_NS * (MUS_BLOCK / 64)];
   const int sidx = blockIdx.y;
   double* st = a.st_all + (size_t)sidx * HFX_ST_LEN;
   if (st[HFX_ST_FROZEN] != 0.0) return;
   const size_t n = (size_t)a.seg_len * a.nb, base = (size_t)sidx * n, stride = (size_t)gridDim.x * MUS_BLOCK;
-  double lo = INFINITY, hi = -INFINITY;
-  for (size_t e = (size_t)blockIdx.x * MUS_BLOCK + threadIdx.x; e < n; e += stride) {
-    lo = fmin(lo, a.eps[base + e]);
-    hi = fmax(hi, a.eps[base + e]);
-  }
-  lo = wave_min(lo);
-  hi = wave_max(hi);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = MUS_BLOCK / 64;
-  if (lane == 0) { sh[wid] = lo; sh[nw + wid] = hi; }
-  __syncthreads();
-  double* mp = a.scr.mpart + (size_t)sidx * 2 * SEG_BLOCKS;
-  if (threadIdx.x == 0) {
-    for (int q = 0; q < nw; ++q) { lo = fmin(lo, sh[q]); hi = fmax(hi, sh[nw + q]); }
-    mp[2 * blockIdx.x] = lo;
-    mp[2 * blockIdx.x + 1] = hi;
-  }
+  const double Tj = fmax(a.T, 1e-12);
+  const double mu0 = st[HFX_ST_MU];   // warm start from the state
+  double* mp = a.scr.mpart + (size_t)sidx * MU_SEG_VALS * SEG_BLOCKS;
+  mu_eval_block(a.eps + base, a.w + (size_t)sidx * a.seg_len, n, a.nb, stride, mu0, Tj, true, sh,
+                mp + 6 * (size_t)blockIdx.x);
   unsigned* ticket = a.scr.tickets + 2 * sidx + 1;
   if (!seg_last_block(ticket, gridDim.x)) return;
+  double mn, mx, t[MU_NS];
+  block_minmax6(mp, (int)gridDim.x, sh, mn, mx);
+  block_gather<MU_NS>(mp + 2, (int)gridDim.x, 6, sh, t);
   if (threadIdx.x == 0) {
-    double mn = INFINITY, mx = -INFINITY;
-    for (unsigned b = 0; b < gridDim.x; ++b) { mn = fmin(mn, mp[2 * b]); mx = fmax(mx, mp[2 * b + 1]); }
-    const double Tj = fmax(a.T, 1e-12);
-    const double blo = mn - 50.0 * Tj, bhi = mx + 50.0 * Tj;   // solver.py:70-72
-    st[HFX_ST_MU] = fmin(fmax(st[HFX_ST_MU], blo), bhi);       // warm start from the state
-    st[HFX_ST_MULO] = blo;
-    st[HFX_ST_MUHI] = bhi;
-    st[HFX_ST_MUDONE] = 0.0;
+    mu_finish_bracket(st, mu0, mn, mx, t, a.targets[sidx], Tj);
     *ticket = 0u;
   }
 }
@@ -629,41 +715,18 @@ __global__ __launch_bounds__(MUS_BLOCK) void mu_step_seg_kernel(MuSegArgs a) {
   const int sidx = blockIdx.y;
   double* st = a.st_all + (size_t)sidx * HFX_ST_LEN;
   if (st[HFX_ST_FROZEN] != 0.0 || st[HFX_ST_MUDONE] != 0.0) return;
-  __shared__ double sh[2 * (MUS_BLOCK / 64)];
+  __shared__ double sh[MU_NS * (MUS_BLOCK / 64)];
   const size_t n = (size_t)a.seg_len * a.nb, base = (size_t)sidx * n, stride = (size_t)gridDim.x * MUS_BLOCK;
   const double Tj = fmax(a.T, 1e-12);
-  const double mu = st[HFX_ST_MU];
-  const size_t kbase = (size_t)sidx * a.seg_len;
-  double acc[2] = {0.0, 0.0};
-  for (size_t e = (size_t)blockIdx.x * MUS_BLOCK + threadIdx.x; e < n; e += stride) {
-    const double pv = sigm((mu - a.eps[base + e]) / Tj);
-    const double wk = a.w[kbase + e / a.nb];
-    acc[0] += wk * pv;
-    acc[1] += wk * pv * (1.0 - pv) / Tj;
-  }
-  block_sum<2>(acc, sh);
-  double* mp = a.scr.mpart + (size_t)sidx * 2 * SEG_BLOCKS;
-  if (threadIdx.x == 0) {
-    mp[2 * blockIdx.x] = acc[0];
-    mp[2 * blockIdx.x + 1] = acc[1];
-  }
+  double* mp = a.scr.mpart + (size_t)sidx * MU_SEG_VALS * SEG_BLOCKS;
+  mu_eval_block(a.eps + base, a.w + (size_t)sidx * a.seg_len, n, a.nb, stride, st[HFX_ST_MU], Tj, false, sh,
+                mp + 6 * (size_t)blockIdx.x);
   unsigned* ticket = a.scr.tickets + 2 * sidx + 1;
   if (!seg_last_block(ticket, gridDim.x)) return;
-  double N, Z;
-  block_gather2(mp, (int)gridDim.x, sh, N, Z);
+  double t[MU_NS];
+  block_gather<MU_NS>(mp + 2, (int)gridDim.x, 6, sh, t);
   if (threadIdx.x == 0) {
-    const double g = N - a.targets[sidx];   // solver.py:80-99
-    double lo = st[HFX_ST_MULO], hi = st[HFX_ST_MUHI];
-    if (mu_converged(g, Z, N, mu, Tj, lo, hi)) {
-      st[HFX_ST_MUDONE] = 1.0;
-    } else {
-      double mu_new = mu;
-      const bool fin = mu_newton_update(g, Z, mu_new, Tj, lo, hi);
-      st[HFX_ST_MU] = mu_new;
-      st[HFX_ST_MULO] = lo;
-      st[HFX_ST_MUHI] = hi;
-      if (fin) st[HFX_ST_MUDONE] = 1.0;
-    }
+    mu_finish_step(st, t, a.targets[sidx], Tj);
     *ticket = 0u;
   }
 }
@@ -1131,7 +1194,7 @@ int hfx_diis_coefficients_batched(const double* brow, size_t brow_stride, double
 
 size_t hfx_batch_scratch_doubles(int nseg) {
   if (nseg < 1) nseg = 1;
-  return (size_t)nseg * (RED_MAX_VALS * SEG_BLOCKS + 2 * SEG_BLOCKS + 1);
+  return (size_t)nseg * (RED_MAX_VALS * SEG_BLOCKS + MU_SEG_VALS * SEG_BLOCKS + 1);
 }
 
 int hfx_reduce_batched(const double* part, int nval, size_t nk_total, int seg_len, const double* w, unsigned wmask,

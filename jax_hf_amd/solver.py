@@ -225,13 +225,15 @@ class _CG:
         K.density(self.Q, self.P, p=self.p)
         return _project_herm(self.P, self.cfg.project_fn, self.tmp)
 
-    # Newton steps per mu solve inside the iteration (a step after convergence
-    # returns at once but still costs a dispatch); from the previous mu a few
-    # converge, and a solve that needs more is finished on the readback
-    MU_LOOP_STEPS = 8
+    # mu steps per mu solve inside the iteration after the bracket pass (which
+    # takes the first step; a step after convergence returns at once but still
+    # costs a dispatch): the fourth-order steps converge from the previous mu in
+    # one or two, and a solve that needs more is finished on the readback
+    MU_LOOP_STEPS = 4
 
     def mu(self, eps, from_state, mu0=0.0, steps=None):
-        """_solve_mu (solver.py:62-101): Newton with bracket, graph-capturable launches."""
+        """_solve_mu (solver.py:62-101): the safeguarded root of N(mu) = n inside the
+        bracket, graph-capturable launches (fourth-order steps on the device)."""
         n = self.cfg.mu_maxiter if steps is None else min(steps, self.cfg.mu_maxiter)
         K.solve_mu_steps(self.ws, eps, self.w_norm, self.nk, self.nb, self.n_norm, self.T, mu0=mu0,
                          mu0_from_state=from_state, maxiter=n, st=self.st, p_out=self.p)
@@ -508,9 +510,10 @@ class _DIISPolish:
     # energies as FP64 storage, and the residual pass reads 1/3 fewer bytes
     HIST32 = True
 
-    # Newton steps per mu solve inside the update graph: from the previous mu a
-    # handful converge; a solve that needs more is completed on the next readback
-    MU_GRAPH_STEPS = 6
+    # mu steps per mu solve inside the update graph after the bracket pass: from
+    # the previous mu one more converges; a solve that needs more is completed
+    # on the next readback
+    MU_GRAPH_STEPS = 3
 
     def _mu(self, maxiter=54):
         K.solve_mu_steps(self.wsp, self.eps, self.w, self.nk, self.nb, self.n_e, self.kernel.T,
