@@ -160,11 +160,26 @@ class _BatchCG:
     # the iteration's readback (its done flag is in st)
     MU_LOOP_STEPS = 4
 
-    def mu(self, steps=None):
-        n = self.cfg.mu_maxiter if steps is None else min(steps, self.cfg.mu_maxiter)
+    # mu steps of an eager (not captured) solve before the done flags are read
+    MU_EAGER_STEPS = 6
+
+    def _mu_launch(self, n):
         L.call("hfx_solve_mu_batched", L.ptr(self.eps), L.ptr(self.w_norm), L.ptr(self.targets), self.nk_tot,
                self.Nk, self.nb, self.T, int(n), L.ptr(self.st), L.ptr(self.p),
                L.ptr(self.scratch), self.scratch.numel(), L.stream(self.dev))
+
+    def mu(self, steps=None):
+        """Per-solve mu: ``steps`` steps inside a captured iteration, else an eager
+        solve to convergence within mu_maxiter steps (the remaining steps are
+        launched only when a done flag is still clear after the first few)."""
+        n = self.cfg.mu_maxiter if steps is None else min(steps, self.cfg.mu_maxiter)
+        if steps is None and n > self.MU_EAGER_STEPS:
+            self._mu_launch(self.MU_EAGER_STEPS)
+            sv = self.st.view(-1, K.ST_LEN).cpu().numpy()
+            if np.all((sv[:, K.ST_MUDONE] != 0.0) | (sv[:, K.ST_FROZEN] != 0.0)):
+                return
+            n -= self.MU_EAGER_STEPS
+        self._mu_launch(n)
 
     def reduce(self, nval, wmask, op, params, out_slot=-1):
         vals = list(params) + [0.0] * (10 - len(params))

@@ -231,10 +231,22 @@ class _CG:
     # one or two, and a solve that needs more is finished on the readback
     MU_LOOP_STEPS = 4
 
+    # mu steps of an eager (not captured) solve before its done flag is read:
+    # the remaining steps of mu_maxiter are launched only when it is not set
+    MU_EAGER_STEPS = 6
+
     def mu(self, eps, from_state, mu0=0.0, steps=None):
         """_solve_mu (solver.py:62-101): the safeguarded root of N(mu) = n inside the
-        bracket, graph-capturable launches (fourth-order steps on the device)."""
+        bracket, graph-capturable launches (fourth-order steps on the device).
+        ``steps`` given: that many steps (inside a captured iteration); else an
+        eager solve to convergence within mu_maxiter steps."""
         n = self.cfg.mu_maxiter if steps is None else min(steps, self.cfg.mu_maxiter)
+        if steps is None and n > self.MU_EAGER_STEPS:
+            K.solve_mu_steps(self.ws, eps, self.w_norm, self.nk, self.nb, self.n_norm, self.T, mu0=mu0,
+                             mu0_from_state=from_state, maxiter=self.MU_EAGER_STEPS, st=self.st, p_out=self.p)
+            if float(self.st[K.ST_MUDONE].item()) != 0.0:
+                return
+            from_state, n = True, n - self.MU_EAGER_STEPS
         K.solve_mu_steps(self.ws, eps, self.w_norm, self.nk, self.nb, self.n_norm, self.T, mu0=mu0,
                          mu0_from_state=from_state, maxiter=n, st=self.st, p_out=self.p)
 
