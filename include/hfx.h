@@ -384,6 +384,37 @@ int hfx_vec_dots(const double* const* xs, const double* const* ys, int nv, size_
                  void* stream);
 /* x[i] -= scale * (*s), s a device scalar (the projection of solver.py:224-233). */
 int hfx_vec_shift(double* x, size_t n, const double* s, double scale, void* stream);
+/* out_i = a' X_i + b' Y_i for two arrays in one launch (n2 = 0: the first
+ * only; Y NULL: zero), a' = a * (*a_dev) when a_dev is given, else a (b' alike):
+ * the updates of the Newton subproblem's truncated CG with coefficients that
+ * stay on the device (solver.py:947-991 forms them as traced scalars).
+ * out may alias X or Y. */
+int hfx_lincomb_dev(size_t n1, int kind1, const void* X1, const void* Y1, void* out1, size_t n2, int kind2,
+                    const void* X2, const void* Y2, void* out2, double a, const double* a_dev, double b,
+                    const double* b_dev, void* stream);
+/* Device scalars of the Steihaug truncated CG (solver.py:947-991), float64[16]. */
+#define HFX_NT_RZ 0      /* r^T z of the current residual */
+#define HFX_NT_DELTA 1   /* trust radius (set by the caller) */
+#define HFX_NT_GRAD 2    /* gradient norm of the outer iterate (caller) */
+#define HFX_NT_ALPHA 3
+#define HFX_NT_TBD 4     /* step to the trust-region boundary along d */
+#define HFX_NT_NEG 5     /* d^T H d <= 1e-14 d^T d */
+#define HFX_NT_DONE 6    /* the loop has stopped: later steps change nothing */
+#define HFX_NT_CV 7      /* v += CV d,  H v += CV H d */
+#define HFX_NT_CR 8      /* r += CR H d */
+#define HFX_NT_CDA 9     /* d = CDA z + CDB d */
+#define HFX_NT_CDB 10
+#define HFX_NT_NALPHA 11 /* -alpha */
+#define HFX_NT_STEPS 12  /* inner steps taken */
+#define HFX_NT_LEN 16
+/* One scalar step of that loop from the reduced inner products `sums` of
+ * npairs vector pairs (matrix parts at [0, npairs), weighted occupation parts
+ * at [npairs, 2 npairs)):
+ * phase 2 starts it (rz = sums 0), phase 0 follows the Hessian-vector product
+ * (d^T H d, d^T d, v^T d, v^T v -> alpha, the boundary step), phase 1 the
+ * preconditioned residual (|v + alpha d|^2, |r - alpha H d|^2, r^T z -> the stop
+ * test and the update coefficients; identity updates once stopped). */
+int hfx_newton_tr_scalars(int phase, const double* sums, int npairs, double* sc, void* stream);
 
 /* Trust-region Newton HVP pieces (solver.py:916-930):
  *   hfx_newton_dP:  dP = herm(Q (X (x) diff_p + diag(dp)) Q^dagger)

@@ -122,6 +122,91 @@ __global__ void lincomb_kernel(size_t n, int kind, double ar, double ai, const v
   }
 }
 
+// out_i = a' X_i + b' Y_i over two arrays (the matrix part and the occupation
+// part of a Newton vector) in one launch; a' = a (* a_dev), b' = b (* b_dev)
+// with a_dev / b_dev device scalars read by every thread, so the coefficients
+// of the truncated-CG updates never leave the device.
+__global__ void lincomb_dev_kernel(size_t n1, int kind1, const void* X1, const void* Y1, void* out1, size_t n2,
+                                   int kind2, const void* X2, const void* Y2, void* out2, double a,
+                                   const double* a_dev, double b, const double* b_dev) {
+  const double ca = a_dev ? a * a_dev[0] : a, cb = b_dev ? b * b_dev[0] : b;
+  const size_t n = n1 + n2;
+  for (size_t i = (size_t)blockIdx.x * UTIL_BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * UTIL_BLOCK) {
+    const bool first = i < n1;
+    const size_t j = first ? i : i - n1;
+    const int kind = first ? kind1 : kind2;
+    const void* X = first ? X1 : X2;
+    const void* Y = first ? Y1 : Y2;
+    double xr, xi, yr = 0.0, yi = 0.0;
+    load_any(X, kind, j, xr, xi);
+    if (Y) load_any(Y, kind, j, yr, yi);
+    store_any(first ? out1 : out2, kind, j, ca * xr + cb * yr, ca * xi + cb * yi);
+  }
+}
+
+// Scalar steps of the Steihaug truncated CG of the trust-region Newton
+// subproblem (solver.py:947-991), one thread.  sums: the reduced inner products
+// of np pairs (matrix parts at [0, np), w-weighted occupation parts at
+// [np, 2 np)); sc: the
+// loop's device scalars (HFX_NT_*).  phase 0 (after d^T H d, d^T d, v^T d,
+// v^T v): alpha and the boundary step t of |v + t d| = Delta; phase 1 (after
+// |v + alpha d|^2, |r - alpha H d|^2, (r - alpha H d)^T z): the stop test and
+// the update coefficients, identity once the loop has stopped; phase 2: start
+// (rz = r^T z).
+__global__ void newton_tr_kernel(int phase, const double* __restrict__ sums, int np, double* __restrict__ sc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double tiny = 1e-30;
+  auto jip = [&](int i) { return sums[i] + sums[np + i]; };
+  if (phase == 2) {
+    sc[HFX_NT_RZ] = jip(0);
+    sc[HFX_NT_DONE] = 0.0;
+    sc[HFX_NT_STEPS] = 0.0;
+    return;
+  }
+  const bool done = sc[HFX_NT_DONE] != 0.0;
+  if (phase == 0) {
+    if (done) {
+      sc[HFX_NT_ALPHA] = 0.0;
+      sc[HFX_NT_NALPHA] = 0.0;
+      return;
+    }
+    const double dHd = jip(0), dd = jip(1), vd = jip(2), vv = jip(3), Delta = sc[HFX_NT_DELTA];
+    const bool neg = dHd <= 1e-14 * dd;
+    const double b = 2.0 * vd, cc = vv - Delta * Delta;
+    const double disc = fmax(b * b - 4.0 * dd * cc, 0.0);
+    const double alpha = sc[HFX_NT_RZ] / (neg ? 1.0 : dHd);
+    sc[HFX_NT_TBD] = (-b + sqrt(disc)) / (2.0 * dd + 1e-30);
+    sc[HFX_NT_NEG] = neg ? 1.0 : 0.0;
+    sc[HFX_NT_ALPHA] = alpha;
+    sc[HFX_NT_NALPHA] = -alpha;
+    return;
+  }
+  if (done) {
+    sc[HFX_NT_CV] = 0.0;
+    sc[HFX_NT_CR] = 0.0;
+    sc[HFX_NT_CDA] = 0.0;
+    sc[HFX_NT_CDB] = 1.0;
+    return;
+  }
+  const double na2 = jip(0), rn2 = jip(1), rz_new = jip(2), rz = sc[HFX_NT_RZ], alpha = sc[HFX_NT_ALPHA];
+  const bool take_bd = sc[HFX_NT_NEG] != 0.0 || sqrt(fmax(0.0, na2)) >= sc[HFX_NT_DELTA];
+  const bool inner = sqrt(fmax(0.0, rn2)) < 0.05 * sc[HFX_NT_GRAD];
+  const double beta = rz_new / (fabs(rz) > tiny ? rz : 1.0);
+  const bool stop = take_bd || inner;
+  sc[HFX_NT_CV] = take_bd ? sc[HFX_NT_TBD] : alpha;   // v += c d, H v += c H d
+  sc[HFX_NT_CR] = take_bd ? 0.0 : -alpha;             // r += c H d
+  if (stop) {
+    sc[HFX_NT_CDA] = 0.0;
+    sc[HFX_NT_CDB] = 1.0;
+    sc[HFX_NT_DONE] = 1.0;
+  } else {
+    sc[HFX_NT_CDA] = 1.0;                              // d = z + beta d
+    sc[HFX_NT_CDB] = beta;
+    sc[HFX_NT_RZ] = rz_new;
+  }
+  sc[HFX_NT_STEPS] += 1.0;
+}
+
 // out = sum_v c_v X_v over nmat nb x nb matrices, optionally Hermitised per
 // matrix: out_ij = (Y_ij + conj(Y_ji)) / 2 with Y = sum_v c_v X_v.
 constexpr int LCM_MAX = 8;
@@ -319,6 +404,26 @@ int hfx_lincomb(size_t n, int kind, double ar, double ai, const void* X, double 
   hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(n)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream, n, kind, ar, ai,
                      X, br, bi, Y, out);
   return check_launch("lincomb_kernel");
+}
+
+int hfx_lincomb_dev(size_t n1, int kind1, const void* X1, const void* Y1, void* out1, size_t n2, int kind2,
+                    const void* X2, const void* Y2, void* out2, double a, const double* a_dev, double b,
+                    const double* b_dev, void* stream) {
+  if (!X1 || !out1 || kind1 < 0 || kind1 > 3 || (n2 && (!X2 || !out2 || kind2 < 0 || kind2 > 3)))
+    return set_error(HFX_EINVAL, "hfx_lincomb_dev: bad argument");
+  const size_t n = n1 + n2;
+  if (n == 0) return HFX_OK;
+  hipLaunchKernelGGL(lincomb_dev_kernel, dim3(grid_for(n)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream, n1, kind1, X1,
+                     Y1, out1, n2, kind2, X2, Y2, out2, a, a_dev, b, b_dev);
+  return check_launch("lincomb_dev_kernel");
+}
+
+int hfx_newton_tr_scalars(int phase, const double* sums, int npairs, double* sc, void* stream) {
+  const int need = phase == 0 ? 4 : phase == 1 ? 3 : 1;
+  if (!sums || !sc || phase < 0 || phase > 2 || npairs < need || npairs > 8)
+    return set_error(HFX_EINVAL, "hfx_newton_tr_scalars: bad argument (phase %d, npairs %d)", phase, npairs);
+  hipLaunchKernelGGL(newton_tr_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, phase, sums, npairs, sc);
+  return check_launch("newton_tr_kernel");
 }
 
 int hfx_lincomb_multi(size_t nmat, int nb, int kind, int nin, const void* const* X, const double* coef, int herm,

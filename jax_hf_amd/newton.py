@@ -5,8 +5,13 @@ Same algorithm as ``_solve_newton_impl``: Steihaug truncated CG on the joint
 (``F[dP] - h`` with refP = 0), block preconditioner, Cayley retraction with
 tau = -1, trust-region ratio with the precision-scaled noise floor, the cold
 start occupation relaxation and the optional deflation bias.  Every array op
-is a libhfx kernel; the Steihaug control flow runs on the host from a few
-reduced scalars per inner iteration.
+is a libhfx kernel.  The Steihaug loop keeps its scalars on the device
+(``hfx_newton_tr_scalars`` forms alpha, the boundary step, the stop test and
+the update coefficients; ``hfx_lincomb_dev`` applies them), so an inner step
+is launches only: it is captured as a HIP graph and replayed, and the host
+reads one stop flag per step, one step behind (the step after the stop is a
+no-op on the loop's state).  The outer trust-region logic runs on the host
+from a few reduced scalars per outer iteration.
 """
 
 from __future__ import annotations
@@ -16,9 +21,13 @@ import torch
 
 from . import _lib as L
 from . import _ops as K
-from ._util import KIND, convert, to_tensor
+from ._util import KIND, capture_graphs, convert, to_tensor
 
 __all__ = ["solve_newton"]
+
+# device scalars of the truncated CG (HFX_NT_* of include/hfx.h)
+NT_RZ, NT_DELTA, NT_GRAD, NT_ALPHA, NT_TBD, NT_NEG, NT_DONE = 0, 1, 2, 3, 4, 5, 6
+NT_CV, NT_CR, NT_CDA, NT_CDB, NT_NALPHA, NT_STEPS = 7, 8, 9, 10, 11, 12
 
 
 class _Newton:
@@ -53,6 +62,11 @@ class _Newton:
         self.dt = K.dt_code(torch.empty(0, dtype=self.cdt))
         self.st = torch.zeros(K.ST_LEN, dtype=torch.float64, device=self.dev)
         self.out = torch.zeros(32, dtype=torch.float64, device=self.dev)
+        self.nt = torch.zeros(16, dtype=torch.float64, device=self.dev)        # HFX_NT_* scalars
+        self.nt_in = torch.zeros(2, dtype=torch.float64, pin_memory=True)      # Delta, grad
+        self.flags = torch.zeros(max(64, int(cfg.tr_cg_max)), dtype=torch.float64,
+                                 pin_memory=True)                               # stop flag per inner step
+        self._tr_graphs = {}
         self.zero_ref = torch.zeros(self.shape4, dtype=self.cdt, device=self.dev)
         self.has_defl = has_defl
         if has_defl:
@@ -76,15 +90,52 @@ class _Newton:
     def sums(self, nval, w, wmask, off=0):
         K.reduce(self.ws, self.ws.part, nval, self.nk, w=w, wmask=wmask, out=self.out[off:off + nval])
 
-    def jips(self, pairs):
-        """[jip(aX, ap, bX, bp)] for pairs of (X, p) vectors: X unweighted, p weighted by w2d."""
+    def jdots(self, pairs, wx=0):
+        """Reduce the inner products of n pairs of (X, p) vectors into out[0:2n],
+        one reduction: X parts at out[i] (weighted by w2d for the bits of wx),
+        p parts weighted by w2d at out[n + i]; launches only."""
         n = len(pairs)
-        K.dots([a[0] for a, _ in pairs], [b[0] for _, b in pairs], self.ws.part, mode=0)
-        self.sums(n, None, 0, 0)
-        K.vdots([a[1] for a, _ in pairs], [b[1] for _, b in pairs], self.ws.part)
-        self.sums(n, self.w2d, (1 << n) - 1, 8)
-        o = self.out.cpu().numpy()
-        return [float(o[i] + o[8 + i]) for i in range(n)]
+        part = self.ws.part.view(-1)
+        K.dots([a[0] for a, _ in pairs], [b[0] for _, b in pairs], part, mode=0)
+        K.vdots([a[1] for a, _ in pairs], [b[1] for _, b in pairs], part[n * self.nk:])
+        self.sums(2 * n, self.w2d, int(wx) | (((1 << n) - 1) << n), 0)
+
+    def nt_ptr(self, idx):
+        return None if idx is None else self.nt.data_ptr() + 8 * int(idx)
+
+    def axpy(self, out, a, b, ca=1.0, ca_dev=None, cb=1.0, cb_dev=None):
+        """(X, p) pair out = ca' a + cb' b, the coefficients scaled by the device
+        scalars nt[ca_dev] / nt[cb_dev] when given; one launch."""
+        (oX, op), (aX, ap), (bX, bp) = out, a, b
+        L.call("hfx_lincomb_dev", oX.numel(), KIND[oX.dtype], L.ptr(aX), L.ptr(bX), L.ptr(oX),
+               op.numel(), KIND[op.dtype], L.ptr(ap), L.ptr(bp), L.ptr(op),
+               float(ca), self.nt_ptr(ca_dev), float(cb), self.nt_ptr(cb_dev), L.stream(self.dev))
+
+    def tr_scalars(self, phase, npairs):
+        L.call("hfx_newton_tr_scalars", int(phase), L.ptr(self.out), int(npairs), L.ptr(self.nt),
+               L.stream(self.dev))
+
+    def tr_step(self, B, Q, p, Ft, eps, pen):
+        """One step of the Steihaug truncated CG (solver.py:947-991), launches only:
+        the Hessian-vector product, alpha and the boundary step, the trial point
+        and residual, the preconditioned residual, the stop test and the updates
+        of v, Hv, r and d with coefficients from the device."""
+        dX, dp, HdX, Hdp = B["dX"], B["dp"], B["HdX"], B["Hdp"]
+        v, Hv, r = (B["vX"], B["vp"]), (B["HvX"], B["Hvp"]), (B["rX"], B["rp"])
+        va, ra, za = (B["vXa"], B["vpa"]), (B["rXa"], B["rpa"]), (B["zXa"], B["zpa"])
+        d, Hd = (dX, dp), (HdX, Hdp)
+        self.hvp(dX, dp, HdX, Hdp, Q, p, Ft, pen, B["dP"], B["R"])
+        self.jdots([(d, Hd), (d, d), (v, d), (v, v)])
+        self.tr_scalars(0, 4)
+        self.axpy(va, v, d, cb_dev=NT_ALPHA)                     # v + alpha d
+        self.axpy(ra, r, Hd, cb_dev=NT_NALPHA)                   # r - alpha H d
+        self.prec(ra[0], ra[1], za[0], za[1], eps, p)
+        self.jdots([(va, va), (ra, ra), (ra, za)])
+        self.tr_scalars(1, 3)
+        self.axpy(v, v, d, cb_dev=NT_CV)                         # boundary step or v + alpha d
+        self.axpy(Hv, Hv, Hd, cb_dev=NT_CV)
+        self.axpy(r, r, Hd, cb_dev=NT_CR)
+        self.axpy(d, za, d, ca_dev=NT_CDA, cb_dev=NT_CDB)        # d = z + beta d (unchanged once stopped)
 
     def project_occ(self, vec, part_written=True):
         """vec -= sum_k w_k sum_b vec / (sum w nb) (solver.py:224-233); part holds per-k sums."""
@@ -108,6 +159,7 @@ class _Newton:
         K.reduce(self.ws, self.ws.part, 2, self.nk, w=self.w_norm, wmask=0b01, st=self.st, op=K.OP_GRAD,
                  params=[self.T, self.cfg.denom_scale, self.nk * self.nb])
         sv = self.st.cpu().numpy()
+        self.last_mu = float(sv[K.ST_MU])            # the mu this state's occupations were solved with
         return float(sv[K.ST_E]) + Phi, float(sv[K.ST_GRAD]), pen, float(sv[K.ST_LAM])
 
     def deflation(self, P, F):
@@ -170,8 +222,7 @@ class _Newton:
         p, eps = rz(), rz()
         Qt, Ftt, Gt = cz(), cz(), cz()
         pt, epst = rz(), rz()
-        dP, R, V, Fe, Dtmp = cz(), cz(), cz(), cz(), cz()
-        lam_v, vtmp = rz(), rz()
+        dP, R = cz(), cz()
 
         def mu_solve(e, pout, from_state, mu0=0.0):
             K.solve_mu(self.ws, e, self.w_norm, self.nk, self.nb, n_norm, self.T, mu0=mu0,
@@ -199,12 +250,17 @@ class _Newton:
         hE, hG = [], []
         k = 0
         Delta = float(cfg.tr_delta0)
-        rX, rp_, zX, zp = cz(), rz(), cz(), rz()
-        dX, dpv = cz(), rz()
-        vX, vp, HvX, Hvp = cz(), rz(), cz(), rz()
-        HdX, Hdp = cz(), rz()
-        vXa, vpa, rXa, rpa = cz(), rz(), cz(), rz()
-        zXa, zpa = cz(), rz()
+        B = {k_: cz() for k_ in ("rX", "zX", "dX", "vX", "HvX", "HdX", "vXa", "rXa", "zXa")}
+        B.update({k_: rz() for k_ in ("rp", "zp", "dp", "vp", "Hvp", "Hdp", "vpa", "rpa", "zpa")})
+        B["dP"], B["R"] = dP, R
+        rX, rp_, zX, zp = B["rX"], B["rp"], B["zX"], B["zp"]
+        dX, dpv, vX, vp, HvX, Hvp = B["dX"], B["dp"], B["vX"], B["vp"], B["HvX"], B["Hvp"]
+        use_graph = (bool(getattr(cfg, "use_graph", True)) and not self.has_defl
+                     and self.fock.graph_safe() and (self.hvp_fock is None or self.hvp_fock.graph_safe()))
+        evs = [torch.cuda.Event() for _ in range(cfg.tr_cg_max)]
+        if use_graph and self.hvp_fock is not None:
+            # first build of the complex64 engine outside the capture (its workspaces)
+            self.hvp_fock.build(self.dP32, self.R32, refP=None, sub_h=True)
         while k < cfg.max_iter and grad > tol_g and Delta > delta_min:      # :886-891
             hE.append(E)
             hG.append(grad)
@@ -214,71 +270,49 @@ class _Newton:
             self.prec(rX, rp_, zX, zp, eps, p)
             dX.copy_(zX)
             dpv.copy_(zp)
-            (rzv,) = self.jips([((rX, rp_), (zX, zp))])
+            self.jdots([((rX, rp_), (zX, zp))])
+            self.tr_scalars(2, 1)                                           # rz, not stopped
+            self.nt_in[0], self.nt_in[1] = Delta, grad
+            self.nt[NT_DELTA:NT_DELTA + 2].copy_(self.nt_in, non_blocking=True)
             vX.zero_(); vp.zero_(); HvX.zero_(); Hvp.zero_()                # noqa: E702
-            cg = 0
-            done = False
-            while cg < cfg.tr_cg_max and not done:                          # :947-991
-                self.hvp(dX, dpv, HdX, Hdp, Q, p, Ft, pen, dP, R)
-                dHd, dd, vd, vv = self.jips([((dX, dpv), (HdX, Hdp)), ((dX, dpv), (dX, dpv)),
-                                             ((vX, vp), (dX, dpv)), ((vX, vp), (vX, vp))])
-                neg = dHd <= 1e-14 * dd
-                b = 2.0 * vd
-                cc = vv - Delta * Delta
-                disc = max(b * b - 4.0 * dd * cc, 0.0)
-                t_bd = (-b + np.sqrt(disc)) / (2.0 * dd + 1e-30)
-                alpha = rzv / (1.0 if neg else dHd)
-                K.lincomb(vXa, vX, 1.0, dX, alpha)
-                K.lincomb(vpa, vp, 1.0, dpv, alpha)
-                K.lincomb(rXa, rX, 1.0, HdX, -alpha)
-                K.lincomb(rpa, rp_, 1.0, Hdp, -alpha)
-                self.prec(rXa, rpa, zXa, zpa, eps, p)
-                na2, rn2, rz_new = self.jips([((vXa, vpa), (vXa, vpa)), ((rXa, rpa), (rXa, rpa)),
-                                              ((rXa, rpa), (zXa, zpa))])
-                take_bd = neg or float(np.sqrt(max(0.0, na2))) >= Delta
-                inner = float(np.sqrt(max(0.0, rn2))) < 0.05 * grad
-                beta = rz_new / (rzv if abs(rzv) > tiny else 1.0)
-                stop = take_bd or inner
-                if take_bd:
-                    K.lincomb(vX, vX, 1.0, dX, t_bd)
-                    K.lincomb(vp, vp, 1.0, dpv, t_bd)
-                    K.lincomb(HvX, HvX, 1.0, HdX, t_bd)
-                    K.lincomb(Hvp, Hvp, 1.0, Hdp, t_bd)
+            graph = None
+            if use_graph:
+                key = Q.data_ptr()
+                graph = self._tr_graphs.get(key)
+                if graph is None:
+                    graph = capture_graphs(self.dev, lambda: self.tr_step(B, Q, p, Ft, eps, pen))[0]
+                    self._tr_graphs[key] = graph
+            for cg in range(cfg.tr_cg_max):                                 # :947-991
+                if graph is not None:
+                    graph.replay()
                 else:
-                    vX, vXa = vXa, vX
-                    vp, vpa = vpa, vp
-                    K.lincomb(HvX, HvX, 1.0, HdX, alpha)
-                    K.lincomb(Hvp, Hvp, 1.0, Hdp, alpha)
-                    rX, rXa = rXa, rX
-                    rp_, rpa = rpa, rp_
-                if not stop:
-                    K.lincomb(dX, zXa, 1.0, dX, beta)
-                    K.lincomb(dpv, zpa, 1.0, dpv, beta)
-                    rzv = rz_new
-                done = stop
-                cg += 1
+                    self.tr_step(B, Q, p, Ft, eps, pen)
+                self.flags[cg:cg + 1].copy_(self.nt[NT_DONE:NT_DONE + 1], non_blocking=True)
+                evs[cg].record()
+                # the stop flag of the previous step (this one is then a no-op)
+                if cg >= 1:
+                    evs[cg - 1].synchronize()
+                    if float(self.flags[cg - 1]) != 0.0:
+                        break
 
-            # predicted reduction, all w2d-weighted (:998-1000)
-            K.dots([G, vX], [vX, HvX], self.ws.part, mode=0)
-            self.sums(2, self.w2d, 0b11, 0)
-            K.vdots([vp], [Hvp], self.ws.part)
-            self.sums(1, self.w2d, 1, 8)
+            # predicted reduction, all w2d-weighted (:998-1000), and the step norm: one
+            # reduction, one readback (G.v needs no p part: the p part of the pair is v.v)
+            self.jdots([((G, vp), (vX, vp)), ((vX, vp), (HvX, Hvp)), ((vX, vp), (vX, vp))], wx=0b011)
             o = self.out.cpu().numpy()
-            gv, vHv = float(o[0]), float(o[1] + o[8])
+            gv, vHv = float(o[0]), float(o[1] + o[4])
             pred = -(gv + 0.5 * vHv)
-            (step_norm2,) = self.jips([((vX, vp), (vX, vp))])
-            # retraction Q <- Q Cayley(tau=-1, vX) and frozen-F trial occupations (:1005-1011)
-            self.st[K.ST_BETA] = 0.0
-            L.call("hfx_cg_direction", self.dt, self.nb, self.nk, L.ptr(vX), L.ptr(Dtmp), L.ptr(vtmp),
-                   L.ptr(vtmp), L.ptr(Ft), L.ptr(self.st), L.ptr(V), L.ptr(lam_v), L.ptr(Fe),
-                   L.ptr(self.ws.part), L.stream(self.dev))
+            step_norm2 = float(o[2] + o[5])
+            # retraction Q <- Q Cayley(tau=-1, vX) and frozen-F trial occupations (:1005-1011):
+            # the Cayley factor by Gauss-Jordan elimination (the same unitary as the
+            # spectral form of solver.py:140-197, without diagonalising i vX)
             Qt.copy_(Q)
-            L.call("hfx_cg_retract", self.dt, self.nb, self.nk, L.ptr(Qt), L.ptr(V), L.ptr(lam_v), L.ptr(Fe),
-                   L.ptr(self.st), -1.0, 1, L.ptr(epst), None, L.stream(self.dev))
+            self.st[K.ST_TAUF] = -1.0
+            L.call("hfx_cg_retract_gj", self.dt, self.nb, self.nk, L.ptr(Qt), L.ptr(vX), L.ptr(Ft), L.ptr(self.st),
+                   L.ptr(epst), None, None, 0, L.stream(self.dev))
             self.st[K.ST_MU] = mu
             mu_solve(epst, pt, True)
-            mu_t = float(self.st[K.ST_MU])
             E_t, g_t, pen_t, lam_t = self.eval_state(Qt, pt, Ftt, Gt, epst, P, F)   # :1014
+            mu_t = self.last_mu
             noise = noise_floor * max(abs(E), 1.0)                          # :1021-1046
             if pred <= noise:
                 accept = (g_t <= grad) and (E_t <= E + noise)
