@@ -384,6 +384,10 @@ int hfx_vec_dots(const double* const* xs, const double* const* ys, int nv, size_
                  void* stream);
 /* x[i] -= scale * (*s), s a device scalar (the projection of solver.py:224-233). */
 int hfx_vec_shift(double* x, size_t n, const double* s, double scale, void* stream);
+/* x[i] -= scale * sum_k w[k] part[k] (part: per-k sums, nk of them), the sum
+ * formed inside the launch: the projection of solver.py:224-233 in one launch. */
+int hfx_vec_shift_sum(double* x, size_t n, const double* part, const double* w, size_t nk, double scale,
+                      void* stream);
 /* out_i = a' X_i + b' Y_i for two arrays in one launch (n2 = 0: the first
  * only; Y NULL: zero), a' = a * (*a_dev) when a_dev is given, else a (b' alike):
  * the updates of the Newton subproblem's truncated CG with coefficients that
@@ -407,14 +411,6 @@ int hfx_lincomb_dev(size_t n1, int kind1, const void* X1, const void* Y1, void* 
 #define HFX_NT_NALPHA 11 /* -alpha */
 #define HFX_NT_STEPS 12  /* inner steps taken */
 #define HFX_NT_LEN 16
-/* One scalar step of that loop from the reduced inner products `sums` of
- * npairs vector pairs (matrix parts at [0, npairs), weighted occupation parts
- * at [npairs, 2 npairs)):
- * phase 2 starts it (rz = sums 0), phase 0 follows the Hessian-vector product
- * (d^T H d, d^T d, v^T d, v^T v -> alpha, the boundary step), phase 1 the
- * preconditioned residual (|v + alpha d|^2, |r - alpha H d|^2, r^T z -> the stop
- * test and the update coefficients; identity updates once stopped). */
-int hfx_newton_tr_scalars(int phase, const double* sums, int npairs, double* sc, void* stream);
 
 /* Trust-region Newton HVP pieces (solver.py:916-930):
  *   hfx_newton_dP:  dP = herm(Q (X (x) diff_p + diag(dp)) Q^dagger)
@@ -435,6 +431,16 @@ int hfx_newton_prec(int dtype, int nb, size_t nk, const void* RX, const double* 
 #define HFX_OP_BETA 2
 #define HFX_OP_TAU0 3
 #define HFX_OP_TRIAL 4
+/* Steihaug truncated-CG scalar steps (solver.py:947-991) on nval / 2 reduced
+ * vector pairs, matrix parts first, then the weighted occupation parts; st is
+ * the loop's HFX_NT_* block.  NT2 starts the loop (rz = pair 0); NT0 follows the
+ * Hessian-vector product (d^T H d, d^T d, v^T d, v^T v -> alpha, the boundary
+ * step); NT1 the preconditioned residual (|v + alpha d|^2, |r - alpha H d|^2,
+ * r^T z -> the stop test and the update coefficients, identity updates once
+ * stopped). */
+#define HFX_OP_NT0 5
+#define HFX_OP_NT1 6
+#define HFX_OP_NT2 7
 /* Scratch (float64) for hfx_reduce / hfx_solve_mu. */
 size_t hfx_solver_scratch_doubles(void);
 /* out[v] = sum_k (w[k] if bit v of wmask) part[v][k] in a fixed order; then the

@@ -116,6 +116,11 @@ __device__ __forceinline__ void post_op(const double (&acc)[RED_MAX_VALS], int n
         if (pa.last) st[HFX_ST_TAUF] = st[HFX_ST_TAU0] * pa.tau_fallback;
       }
     } break;
+    case HFX_OP_NT0:
+    case HFX_OP_NT1:
+    case HFX_OP_NT2:      // Steihaug scalars from the reduced pairs (st: the loop's HFX_NT_* block)
+      newton_tr_phase(pa.op == HFX_OP_NT0 ? 0 : pa.op == HFX_OP_NT1 ? 1 : 2, acc, nval / 2, st);
+      break;
     default:
       break;
   }

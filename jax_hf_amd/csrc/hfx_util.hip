@@ -144,69 +144,6 @@ __global__ void lincomb_dev_kernel(size_t n1, int kind1, const void* X1, const v
   }
 }
 
-// Scalar steps of the Steihaug truncated CG of the trust-region Newton
-// subproblem (solver.py:947-991), one thread.  sums: the reduced inner products
-// of np pairs (matrix parts at [0, np), w-weighted occupation parts at
-// [np, 2 np)); sc: the
-// loop's device scalars (HFX_NT_*).  phase 0 (after d^T H d, d^T d, v^T d,
-// v^T v): alpha and the boundary step t of |v + t d| = Delta; phase 1 (after
-// |v + alpha d|^2, |r - alpha H d|^2, (r - alpha H d)^T z): the stop test and
-// the update coefficients, identity once the loop has stopped; phase 2: start
-// (rz = r^T z).
-__global__ void newton_tr_kernel(int phase, const double* __restrict__ sums, int np, double* __restrict__ sc) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double tiny = 1e-30;
-  auto jip = [&](int i) { return sums[i] + sums[np + i]; };
-  if (phase == 2) {
-    sc[HFX_NT_RZ] = jip(0);
-    sc[HFX_NT_DONE] = 0.0;
-    sc[HFX_NT_STEPS] = 0.0;
-    return;
-  }
-  const bool done = sc[HFX_NT_DONE] != 0.0;
-  if (phase == 0) {
-    if (done) {
-      sc[HFX_NT_ALPHA] = 0.0;
-      sc[HFX_NT_NALPHA] = 0.0;
-      return;
-    }
-    const double dHd = jip(0), dd = jip(1), vd = jip(2), vv = jip(3), Delta = sc[HFX_NT_DELTA];
-    const bool neg = dHd <= 1e-14 * dd;
-    const double b = 2.0 * vd, cc = vv - Delta * Delta;
-    const double disc = fmax(b * b - 4.0 * dd * cc, 0.0);
-    const double alpha = sc[HFX_NT_RZ] / (neg ? 1.0 : dHd);
-    sc[HFX_NT_TBD] = (-b + sqrt(disc)) / (2.0 * dd + 1e-30);
-    sc[HFX_NT_NEG] = neg ? 1.0 : 0.0;
-    sc[HFX_NT_ALPHA] = alpha;
-    sc[HFX_NT_NALPHA] = -alpha;
-    return;
-  }
-  if (done) {
-    sc[HFX_NT_CV] = 0.0;
-    sc[HFX_NT_CR] = 0.0;
-    sc[HFX_NT_CDA] = 0.0;
-    sc[HFX_NT_CDB] = 1.0;
-    return;
-  }
-  const double na2 = jip(0), rn2 = jip(1), rz_new = jip(2), rz = sc[HFX_NT_RZ], alpha = sc[HFX_NT_ALPHA];
-  const bool take_bd = sc[HFX_NT_NEG] != 0.0 || sqrt(fmax(0.0, na2)) >= sc[HFX_NT_DELTA];
-  const bool inner = sqrt(fmax(0.0, rn2)) < 0.05 * sc[HFX_NT_GRAD];
-  const double beta = rz_new / (fabs(rz) > tiny ? rz : 1.0);
-  const bool stop = take_bd || inner;
-  sc[HFX_NT_CV] = take_bd ? sc[HFX_NT_TBD] : alpha;   // v += c d, H v += c H d
-  sc[HFX_NT_CR] = take_bd ? 0.0 : -alpha;             // r += c H d
-  if (stop) {
-    sc[HFX_NT_CDA] = 0.0;
-    sc[HFX_NT_CDB] = 1.0;
-    sc[HFX_NT_DONE] = 1.0;
-  } else {
-    sc[HFX_NT_CDA] = 1.0;                              // d = z + beta d
-    sc[HFX_NT_CDB] = beta;
-    sc[HFX_NT_RZ] = rz_new;
-  }
-  sc[HFX_NT_STEPS] += 1.0;
-}
-
 // out = sum_v c_v X_v over nmat nb x nb matrices, optionally Hermitised per
 // matrix: out_ij = (Y_ij + conj(Y_ji)) / 2 with Y = sum_v c_v X_v.
 constexpr int LCM_MAX = 8;
@@ -253,6 +190,24 @@ __global__ void fermi_kernel(const double* x, const double* mu, double T, size_t
 // x[i] -= scale * (*s) : the particle-conserving shift of solver.py:224-233
 __global__ void vec_shift_kernel(double* x, size_t n, const double* s, double scale) {
   const double d = (*s) * scale;
+  for (size_t i = (size_t)blockIdx.x * UTIL_BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * UTIL_BLOCK) x[i] -= d;
+}
+
+// x[i] -= scale * sum_k w[k] part[k] with the sum formed by every block (the
+// same strided partial sums and block tree in each, so the totals are
+// bitwise identical): the projection of a Newton vector in one launch instead
+// of a two-launch reduction and a shift.
+constexpr int VSS_MAX_BLOCKS = 64;
+__global__ void vec_shift_sum_kernel(double* x, size_t n, const double* __restrict__ part,
+                                     const double* __restrict__ w, size_t nk, double scale) {
+  __shared__ double sh[UTIL_BLOCK / 64];
+  __shared__ double tot;
+  double acc[1] = {0.0};
+  for (size_t k = threadIdx.x; k < nk; k += UTIL_BLOCK) acc[0] += w[k] * part[k];
+  block_sum<1>(acc, sh);
+  if (threadIdx.x == 0) tot = acc[0];
+  __syncthreads();
+  const double d = tot * scale;
   for (size_t i = (size_t)blockIdx.x * UTIL_BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * UTIL_BLOCK) x[i] -= d;
 }
 
@@ -418,14 +373,6 @@ int hfx_lincomb_dev(size_t n1, int kind1, const void* X1, const void* Y1, void* 
   return check_launch("lincomb_dev_kernel");
 }
 
-int hfx_newton_tr_scalars(int phase, const double* sums, int npairs, double* sc, void* stream) {
-  const int need = phase == 0 ? 4 : phase == 1 ? 3 : 1;
-  if (!sums || !sc || phase < 0 || phase > 2 || npairs < need || npairs > 8)
-    return set_error(HFX_EINVAL, "hfx_newton_tr_scalars: bad argument (phase %d, npairs %d)", phase, npairs);
-  hipLaunchKernelGGL(newton_tr_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, phase, sums, npairs, sc);
-  return check_launch("newton_tr_kernel");
-}
-
 int hfx_lincomb_multi(size_t nmat, int nb, int kind, int nin, const void* const* X, const double* coef, int herm,
                       void* out, void* stream) {
   if (!X || !coef || !out || nin < 1 || nin > LCM_MAX || nb < 1 || kind < 0 || kind > 3)
@@ -449,6 +396,17 @@ int hfx_fermi(const double* x, const double* mu, double T, size_t n, double* out
   if (n == 0) return HFX_OK;
   hipLaunchKernelGGL(fermi_kernel, dim3(grid_for(n)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream, x, mu, T, n, out);
   return check_launch("fermi_kernel");
+}
+
+int hfx_vec_shift_sum(double* x, size_t n, const double* part, const double* w, size_t nk, double scale,
+                      void* stream) {
+  if (!x || !part || !w || nk == 0) return set_error(HFX_EINVAL, "hfx_vec_shift_sum: bad argument");
+  if (n == 0) return HFX_OK;
+  size_t g = grid_for(n);
+  if (g > (size_t)VSS_MAX_BLOCKS) g = VSS_MAX_BLOCKS;
+  hipLaunchKernelGGL(vec_shift_sum_kernel, dim3((unsigned)g), dim3(UTIL_BLOCK), 0, (hipStream_t)stream, x, n, part, w,
+                     nk, scale);
+  return check_launch("vec_shift_sum_kernel");
 }
 
 int hfx_vec_shift(double* x, size_t n, const double* s, double scale, void* stream) {

@@ -5,9 +5,10 @@ Same algorithm as ``_solve_newton_impl``: Steihaug truncated CG on the joint
 (``F[dP] - h`` with refP = 0), block preconditioner, Cayley retraction with
 tau = -1, trust-region ratio with the precision-scaled noise floor, the cold
 start occupation relaxation and the optional deflation bias.  Every array op
-is a libhfx kernel.  The Steihaug loop keeps its scalars on the device
-(``hfx_newton_tr_scalars`` forms alpha, the boundary step, the stop test and
-the update coefficients; ``hfx_lincomb_dev`` applies them), so an inner step
+is a libhfx kernel.  The Steihaug loop keeps its scalars on the device (the
+last workgroup of each inner-product reduction forms alpha, the boundary
+step, the stop test and the update coefficients -- the HFX_OP_NT* post-ops --
+and ``hfx_lincomb_dev`` applies them), so an inner step
 is launches only: it is captured as a HIP graph and replayed, and the host
 reads one stop flag per step, one step behind (the step after the stop is a
 no-op on the loop's state).  The outer trust-region logic runs on the host
@@ -90,15 +91,21 @@ class _Newton:
     def sums(self, nval, w, wmask, off=0):
         K.reduce(self.ws, self.ws.part, nval, self.nk, w=w, wmask=wmask, out=self.out[off:off + nval])
 
-    def jdots(self, pairs, wx=0):
+    def jdots(self, pairs, wx=0, phase=None):
         """Reduce the inner products of n pairs of (X, p) vectors into out[0:2n],
         one reduction: X parts at out[i] (weighted by w2d for the bits of wx),
-        p parts weighted by w2d at out[n + i]; launches only."""
+        p parts weighted by w2d at out[n + i]; with ``phase`` the reduction's
+        last workgroup also takes that Steihaug scalar step; launches only."""
         n = len(pairs)
         part = self.ws.part.view(-1)
         K.dots([a[0] for a, _ in pairs], [b[0] for _, b in pairs], part, mode=0)
         K.vdots([a[1] for a, _ in pairs], [b[1] for _, b in pairs], part[n * self.nk:])
-        self.sums(2 * n, self.w2d, int(wx) | (((1 << n) - 1) << n), 0)
+        wmask = int(wx) | (((1 << n) - 1) << n)
+        if phase is None:
+            self.sums(2 * n, self.w2d, wmask, 0)
+        else:
+            K.reduce(self.ws, self.ws.part, 2 * n, self.nk, w=self.w2d, wmask=wmask, out=self.out[0:2 * n],
+                     st=self.nt, op=K.OP_NT0 + int(phase))
 
     def nt_ptr(self, idx):
         return None if idx is None else self.nt.data_ptr() + 8 * int(idx)
@@ -111,10 +118,6 @@ class _Newton:
                op.numel(), KIND[op.dtype], L.ptr(ap), L.ptr(bp), L.ptr(op),
                float(ca), self.nt_ptr(ca_dev), float(cb), self.nt_ptr(cb_dev), L.stream(self.dev))
 
-    def tr_scalars(self, phase, npairs):
-        L.call("hfx_newton_tr_scalars", int(phase), L.ptr(self.out), int(npairs), L.ptr(self.nt),
-               L.stream(self.dev))
-
     def tr_step(self, B, Q, p, Ft, eps, pen):
         """One step of the Steihaug truncated CG (solver.py:947-991), launches only:
         the Hessian-vector product, alpha and the boundary step, the trial point
@@ -125,22 +128,20 @@ class _Newton:
         va, ra, za = (B["vXa"], B["vpa"]), (B["rXa"], B["rpa"]), (B["zXa"], B["zpa"])
         d, Hd = (dX, dp), (HdX, Hdp)
         self.hvp(dX, dp, HdX, Hdp, Q, p, Ft, pen, B["dP"], B["R"])
-        self.jdots([(d, Hd), (d, d), (v, d), (v, v)])
-        self.tr_scalars(0, 4)
+        self.jdots([(d, Hd), (d, d), (v, d), (v, v)], phase=0)
         self.axpy(va, v, d, cb_dev=NT_ALPHA)                     # v + alpha d
         self.axpy(ra, r, Hd, cb_dev=NT_NALPHA)                   # r - alpha H d
         self.prec(ra[0], ra[1], za[0], za[1], eps, p)
-        self.jdots([(va, va), (ra, ra), (ra, za)])
-        self.tr_scalars(1, 3)
+        self.jdots([(va, va), (ra, ra), (ra, za)], phase=1)
         self.axpy(v, v, d, cb_dev=NT_CV)                         # boundary step or v + alpha d
         self.axpy(Hv, Hv, Hd, cb_dev=NT_CV)
         self.axpy(r, r, Hd, cb_dev=NT_CR)
         self.axpy(d, za, d, ca_dev=NT_CDA, cb_dev=NT_CDB)        # d = z + beta d (unchanged once stopped)
 
-    def project_occ(self, vec, part_written=True):
-        """vec -= sum_k w_k sum_b vec / (sum w nb) (solver.py:224-233); part holds per-k sums."""
-        K.reduce(self.ws, self.ws.part, 1, self.nk, w=self.w2d, wmask=1, out=self.out[16:17])
-        L.call("hfx_vec_shift", L.ptr(vec), vec.numel(), self.out.data_ptr() + 16 * 8,
+    def project_occ(self, vec):
+        """vec -= sum_k w_k sum_b vec / (sum w nb) (solver.py:224-233); part holds
+        the per-k sums; one launch."""
+        L.call("hfx_vec_shift_sum", L.ptr(vec), vec.numel(), L.ptr(self.ws.part), L.ptr(self.w2d), self.nk,
                1.0 / (self.wsum * self.nb), L.stream(self.dev))
 
     # ---- model pieces ------------------------------------------------------
@@ -270,8 +271,7 @@ class _Newton:
             self.prec(rX, rp_, zX, zp, eps, p)
             dX.copy_(zX)
             dpv.copy_(zp)
-            self.jdots([((rX, rp_), (zX, zp))])
-            self.tr_scalars(2, 1)                                           # rz, not stopped
+            self.jdots([((rX, rp_), (zX, zp))], phase=2)                   # rz, not stopped
             self.nt_in[0], self.nt_in[1] = Delta, grad
             self.nt[NT_DELTA:NT_DELTA + 2].copy_(self.nt_in, non_blocking=True)
             vX.zero_(); vp.zero_(); HvX.zero_(); Hvp.zero_()                # noqa: E702
