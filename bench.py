@@ -58,6 +58,11 @@ def parse():
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the Newton (64x64, nb=8) and adjoint (48x48, nb=16) sections")
     ap.add_argument("--extra", action="store_true", help="also time the packed and complex64 Fock builds")
+    ap.add_argument("--one-solve", choices=["mixed", "newton"], default=None,
+                    help="profiling aid: warm up, then one solve between two marker kernels (a 1024-float "
+                         "add and multiply), for a kernel trace of exactly one solve: the mixed solve of the "
+                         "headline workload, or the trust-region Newton of the 64x64 nb=8 precision form "
+                         "with max_iter 20")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of the exchange kernel's byte count "
                          "(hfx_lincomb out = X), to scale FETCH_SIZE / WRITE_SIZE counters")
@@ -593,6 +598,40 @@ def cpu_baseline(inp, budget_s):
             "fock_builds_per_s": fock_rate}, E, F
 
 
+def one_solve(J, syn, dev, nk, nb, form):
+    """One solve between two marker kernels (after warm-up solves), timed; the
+    kernels between the markers in a kernel trace are exactly that solve's."""
+    import torch
+    if form == "mixed":
+        inp = syn.config1(nk=nk, nb=nb, seed=1)
+        G = J.HartreeFockKernel(inp["weights"], inp["hamiltonian"], inp["coulomb_q"], SOLVE_T, device=dev)
+        ne = nb / 2.0
+        cfg = J.SolverConfig(max_iter=500, precision="mixed", polish_tol=POLISH_TOL)
+        shape = inp["hamiltonian"].shape
+    else:
+        c = syn.config2()
+        G = J.HartreeFockKernel(c["weights"], c["hamiltonian"], c["coulomb_q"], c["T"], device=dev)
+        ne = c["n_electrons"]
+        cfg = J.SolverConfig(optimizer="newton", tol_grad=1e-6, precision="fp64", max_iter=20)
+        shape = c["hamiltonian"].shape
+    P0 = torch.zeros(shape, dtype=torch.complex128, device=dev)
+    for _ in range(3):
+        J.solve(G, P0, ne, config=cfg)
+    torch.cuda.synchronize()
+    marker = torch.zeros(1024, device=dev)
+    marker.add_(1.0)
+    t0 = time.perf_counter()
+    r = J.solve(G, P0, ne, config=cfg)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    marker.mul_(3.0)
+    torch.cuda.synchronize()
+    print(json.dumps({"one_solve": form, "ms": dt * 1e3, "n_iter": int(r.n_iter), "energy": float(r.energy)}),
+          flush=True)
+    from jax_hf_amd import _lib as L
+    L.release_all()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -615,6 +654,9 @@ def main():
             L.call("hfx_lincomb", n, 2, 1.0, 0.0, L.ptr(X), 0.0, 0.0, None, L.ptr(Y), L.stream(dev))
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_read": n * 16, "copy_calibration_bytes_written": n * 16}))
+        return
+    if args.one_solve:
+        one_solve(J, syn, dev, nk, nb, args.one_solve)
         return
     inp = syn.config1(nk=nk, nb=nb, seed=1)
     fock, roof, F_gpu = fock_bench(J, L, syn, inp, dev, nk, nb, args)

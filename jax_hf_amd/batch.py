@@ -79,11 +79,7 @@ class _BatchFock:
             self.St = torch.empty((S, nch, nk1, nk2), **c)
             self.vmap = None
             if self.vfull:
-                self.VRc = torch.empty((nb * nb, nk1, nk2), **c)
-                L.call("hfx_planar_pack", self.plan.handle, 1, L.ptr(self.VR.contiguous()), None, None, 0, 0,
-                       L.ptr(self.VRc), L.stream(dev))
-                if fock_mod.DEDUP_VR:
-                    self.vmap = fock_mod.vr_plane_map(self.VRc)
+                self.VRc, self.vmap = fock_mod._channel_major_vr(self.plan, self.VR, (nk1, nk2, nb))
             else:
                 self.VRc = self.VR.contiguous()
             if fock_mod.AUTOTUNE:

@@ -24,6 +24,7 @@ from __future__ import annotations
 
 from typing import Any
 
+import collections
 import os
 
 import numpy as np
@@ -150,6 +151,32 @@ def vr_plane_map(VRc):
     return torch.tensor(rep, dtype=torch.int32, device=VRc.device)
 
 
+# Channel-major VR planes and their plane map, kept per VR tensor (by identity
+# and version) so that engines made for the same VR -- a kernel's solvers, or
+# build_fock called repeatedly with the kernel's VR -- pack and compare the
+# planes once.
+_vr_cache: "collections.OrderedDict" = collections.OrderedDict()
+_VR_CACHE_MAX = 8
+L.register_cache(_vr_cache.clear)
+
+
+def _channel_major_vr(plan, VR, shape):
+    key = (VR.data_ptr(), VR._version, tuple(VR.shape), VR.dtype, VR.device, bool(DEDUP_VR))
+    hit = _vr_cache.get(key)
+    if hit is not None and hit[0] is VR:
+        _vr_cache.move_to_end(key)
+        return hit[1], hit[2]
+    nk1, nk2, nb = shape
+    VRc = torch.empty((nb * nb, nk1, nk2), dtype=VR.dtype, device=VR.device)
+    L.call("hfx_planar_pack", plan.handle, 1, L.ptr(VR.contiguous()), None, None, 0, 0, L.ptr(VRc),
+           L.stream(VR.device))
+    vmap = vr_plane_map(VRc) if DEDUP_VR else None
+    _vr_cache[key] = (VR, VRc, vmap)
+    while len(_vr_cache) > _VR_CACHE_MAX:
+        _vr_cache.popitem(last=False)
+    return VRc, vmap
+
+
 class FockEngine:
     """Fock build + energy for fixed (h, VR, refP, HH, contact) on one device.
 
@@ -221,12 +248,7 @@ class FockEngine:
             if not self.vfull:
                 self._vrc = self.VR.contiguous()
             else:
-                nk1, nk2, nb = self.shape
-                self._vrc = torch.empty((nb * nb, nk1, nk2), dtype=self.VR.dtype, device=self.dev)
-                L.call("hfx_planar_pack", self.plan.handle, 1, L.ptr(self.VR.contiguous()), None, None, 0, 0,
-                       L.ptr(self._vrc), L.stream(self.dev))
-                if DEDUP_VR:
-                    self._vmap = vr_plane_map(self._vrc)
+                self._vrc, self._vmap = _channel_major_vr(self.plan, self.VR, self.shape)
         return self._vrc
 
     def planar_workspace(self, nchan):
