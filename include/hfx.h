@@ -486,13 +486,15 @@ int hfx_solve_mu_noalloc(const double* eps, const double* w, size_t nk, int nb, 
                          double mu0, int mu0_from_state, int maxiter, int mode, double level_shift, double* st,
                          double* mu_out, double* p_out, void* stream);
 
-/* The Newton-with-bracket mu solver of solver.py:62-101 as a fixed sequence
- * of ordinary launches (bracket, maxiter steps, occupations) that can be
- * captured in a HIP graph: each step is a grid reduction whose last block
- * updates st[HFX_ST_MU] / [MULO] / [MUHI]; once a Newton step no longer moves
- * mu, st[HFX_ST_MUDONE] is set and the remaining steps return at once.
- * Occupations sigmoid((mu - eps) / T) to p_out.  Same result as
- * hfx_solve_mu(HFX_MU_NEWTON_BRACKET) up to the order of the sums. */
+/* The safeguarded mu root of solver.py:62-101 as a fixed sequence of
+ * maxiter + 2 ordinary launches that can be captured in a HIP graph: a
+ * bracket pass, maxiter chained steps (each finishes the previous evaluation
+ * -- a fourth-order step from N and its first three derivatives, inside the
+ * bracket -- and evaluates at the new mu), and a finish that writes
+ * st[HFX_ST_MU] / [MULO] / [MUHI] / [MUDONE] and the occupations
+ * sigmoid((mu - eps) / T) to p_out.  Once converged, st[HFX_ST_MUDONE] is set
+ * and the remaining steps only carry the state.  Same root as
+ * hfx_solve_mu(HFX_MU_NEWTON_BRACKET) to its tolerance. */
 int hfx_solve_mu_steps(const double* eps, const double* w, size_t nk, int nb, double target, double T,
                        double mu0, int mu0_from_state, int maxiter, double* st, double* p_out, double* scratch,
                        void* stream);

@@ -557,48 +557,98 @@ __device__ __forceinline__ void block_minmax6(const double* src, int n, double* 
 }
 
 // The bracket and the first evaluation, at the warm-start mu0, share one pass
-// over eps: block partials (6 doubles per block), then one workgroup that
-// sets the bracket and takes the first step.  Each later step is two launches
-// as well (cheaper in a graph than a completion ticket, whose device-scope
-// fence writes back the XCD's L2); a step after convergence returns at once
-// in both launches.
+// over eps (block partials, 6 doubles per block).  Each later launch of the
+// chain finishes the previous evaluation itself -- every block sums the
+// previous launch's partials in the same order, so all blocks reach the same
+// state bit for bit -- and then evaluates at the new mu; block 0 records the
+// state in the slot the next launch reads.  Two partial buffers and two state
+// slots alternate, so a launch never reads what it writes.  One launch per
+// step: no one-block finish launch in between, and no completion ticket (whose
+// device-scope fence writes back the XCD's L2).  A launch after convergence
+// only carries the state forward.
+constexpr int MU_CHAIN_MAX_BLK = 320;
+constexpr size_t MU_CHAIN_BUF = 6 * (size_t)MU_CHAIN_MAX_BLK;   // doubles per partial buffer
+static_assert(2 * MU_CHAIN_BUF + 8 <= MU_TICKET_OFF - MU_PART_OFF, "mu chain scratch");
+
+__device__ __forceinline__ double* mu_chain_buf(const MuStepArgs& a, int j) {
+  return a.part + (size_t)(j & 1) * MU_CHAIN_BUF;
+}
+__device__ __forceinline__ double* mu_chain_slot(const MuStepArgs& a, int j) {   // mu, lo, hi, done
+  return a.part + 2 * MU_CHAIN_BUF + 4 * (j & 1);
+}
+
 __global__ __launch_bounds__(MUS_BLOCK) void mu_bracket_part(MuStepArgs a) {
   __shared__ double sh[MU_NS * (MUS_BLOCK / 64)];
   const size_t n = a.nk * a.nb, stride = (size_t)gridDim.x * MUS_BLOCK;
   const double mu0 = a.mu0_from_state ? a.st[HFX_ST_MU] : a.mu0;
-  mu_eval_block(a.eps, a.w, n, a.nb, stride, mu0, fmax(a.T, 1e-12), true, sh, a.part + 6 * (size_t)blockIdx.x);
+  mu_eval_block(a.eps, a.w, n, a.nb, stride, mu0, fmax(a.T, 1e-12), true, sh, mu_chain_buf(a, 0) + 6 * (size_t)blockIdx.x);
 }
 
-__global__ __launch_bounds__(MUS_BLOCK) void mu_bracket_fin(MuStepArgs a, int nblk) {
-  __shared__ double sh[MU_NS * (MUS_BLOCK / 64)];
-  double mn, mx, t[MU_NS];
-  block_minmax6(a.part, nblk, sh, mn, mx);
-  block_gather<MU_NS>(a.part + 2, nblk, 6, sh, t);
-  if (threadIdx.x == 0) {
-    const double mu0 = a.mu0_from_state ? a.st[HFX_ST_MU] : a.mu0;
-    mu_finish_bracket(a.st, mu0, mn, mx, t, a.target, fmax(a.T, 1e-12));
+// The state after evaluation j - 1 of the chain (evaluation 0 is the bracket
+// pass) into sst[HFX_ST_MU / MULO / MUHI / MUDONE], the same in every block.
+__device__ __forceinline__ void mu_chain_finish(const MuStepArgs& a, int nblk, int j, double* sst, double* sh) {
+  const double Tj = fmax(a.T, 1e-12);
+  const double* src = mu_chain_buf(a, j - 1);
+  if (j == 1) {
+    double mn, mx, t[MU_NS];
+    block_minmax6(src, nblk, sh, mn, mx);
+    block_gather<MU_NS>(src + 2, nblk, 6, sh, t);
+    if (threadIdx.x == 0) {
+      const double mu0 = a.mu0_from_state ? a.st[HFX_ST_MU] : a.mu0;
+      mu_finish_bracket(sst, mu0, mn, mx, t, a.target, Tj);
+    }
+  } else {
+    const double* slot = mu_chain_slot(a, j - 1);
+    const bool done = slot[3] != 0.0;
+    if (threadIdx.x == 0) {
+      sst[HFX_ST_MU] = slot[0];
+      sst[HFX_ST_MULO] = slot[1];
+      sst[HFX_ST_MUHI] = slot[2];
+      sst[HFX_ST_MUDONE] = slot[3];
+    }
+    if (!done) {
+      double t[MU_NS];
+      block_gather<MU_NS>(src + 2, nblk, 6, sh, t);
+      if (threadIdx.x == 0) mu_finish_step(sst, t, a.target, Tj);
+    }
   }
+  __syncthreads();
 }
 
-__global__ __launch_bounds__(MUS_BLOCK) void mu_step_part(MuStepArgs a) {
-  if (a.st[HFX_ST_MUDONE] != 0.0) return;
+__global__ __launch_bounds__(MUS_BLOCK) void mu_chain_step(MuStepArgs a, int nblk, int j) {
   __shared__ double sh[MU_NS * (MUS_BLOCK / 64)];
+  __shared__ double sst[HFX_ST_MUDONE + 1];
+  mu_chain_finish(a, nblk, j, sst, sh);
+  const double mu = sst[HFX_ST_MU];
+  const bool done = sst[HFX_ST_MUDONE] != 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double* o = mu_chain_slot(a, j);
+    o[0] = mu;
+    o[1] = sst[HFX_ST_MULO];
+    o[2] = sst[HFX_ST_MUHI];
+    o[3] = sst[HFX_ST_MUDONE];
+  }
+  if (done) return;
   const size_t n = a.nk * a.nb, stride = (size_t)gridDim.x * MUS_BLOCK;
-  mu_eval_block(a.eps, a.w, n, a.nb, stride, a.st[HFX_ST_MU], fmax(a.T, 1e-12), false, sh,
-                a.part + 6 * (size_t)blockIdx.x);
+  mu_eval_block(a.eps, a.w, n, a.nb, stride, mu, fmax(a.T, 1e-12), false, sh, mu_chain_buf(a, j) + 6 * (size_t)blockIdx.x);
 }
 
-__global__ __launch_bounds__(MUS_BLOCK) void mu_step_fin(MuStepArgs a, int nblk) {
-  if (a.st[HFX_ST_MUDONE] != 0.0) return;
+// Last launch: finish evaluation j - 1 into the solver state, and the
+// occupations at the final mu (all blocks) when p_out is given.
+__global__ __launch_bounds__(MUS_BLOCK) void mu_chain_fin(MuStepArgs a, int nblk, int j) {
   __shared__ double sh[MU_NS * (MUS_BLOCK / 64)];
-  double t[MU_NS];
-  block_gather<MU_NS>(a.part + 2, nblk, 6, sh, t);
-  if (threadIdx.x == 0) mu_finish_step(a.st, t, a.target, fmax(a.T, 1e-12));
-}
-
-__global__ __launch_bounds__(MUS_BLOCK) void mu_occ_kernel(MuStepArgs a) {
+  __shared__ double sst[HFX_ST_MUDONE + 1];
+  mu_chain_finish(a, nblk, j, sst, sh);
+  const double mu = sst[HFX_ST_MU];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.st[HFX_ST_MU] = mu;
+    a.st[HFX_ST_MULO] = sst[HFX_ST_MULO];
+    a.st[HFX_ST_MUHI] = sst[HFX_ST_MUHI];
+    a.st[HFX_ST_MUDONE] = sst[HFX_ST_MUDONE];
+  }
+  if (!a.p_out) return;
   const size_t n = a.nk * a.nb, stride = (size_t)gridDim.x * MUS_BLOCK;
-  const double Tj = fmax(a.T, 1e-12), mu = a.st[HFX_ST_MU];
+  const double Tj = fmax(a.T, 1e-12);
   for (size_t e = (size_t)blockIdx.x * MUS_BLOCK + threadIdx.x; e < n; e += stride)
     a.p_out[e] = sigm((mu - a.eps[e]) / Tj);   // solver.py:394, 592
 }
@@ -1162,25 +1212,20 @@ int hfx_solve_mu_steps(const double* eps, const double* w, size_t nk, int nb, do
     return set_error(HFX_EINVAL, "hfx_solve_mu_steps: bad argument");
   const size_t n = nk * (size_t)nb;
   size_t nbk = (n + (size_t)MUS_BLOCK * 4 - 1) / ((size_t)MUS_BLOCK * 4);
-  if (nbk > 512) nbk = 512;
+  if (nbk > (size_t)MU_CHAIN_MAX_BLK) nbk = MU_CHAIN_MAX_BLK;
   if (nbk < 1) nbk = 1;
   MuStepArgs a{eps, w, nk, nb, target, T, mu0, mu0_from_state, st, scratch + MU_PART_OFF, p_out};
   hipStream_t s = (hipStream_t)stream;
+  // bracket pass, maxiter chained steps, then the finish (with the occupations)
   hipLaunchKernelGGL(mu_bracket_part, dim3((unsigned)nbk), dim3(MUS_BLOCK), 0, s, a);
   if (int rc = check_launch("mu_bracket_part")) return rc;
-  hipLaunchKernelGGL(mu_bracket_fin, dim3(1), dim3(MUS_BLOCK), 0, s, a, (int)nbk);
-  if (int rc = check_launch("mu_bracket_fin")) return rc;
-  for (int it = 0; it < maxiter; ++it) {
-    hipLaunchKernelGGL(mu_step_part, dim3((unsigned)nbk), dim3(MUS_BLOCK), 0, s, a);
-    if (int rc = check_launch("mu_step_part")) return rc;
-    hipLaunchKernelGGL(mu_step_fin, dim3(1), dim3(MUS_BLOCK), 0, s, a, (int)nbk);
-    if (int rc = check_launch("mu_step_fin")) return rc;
+  for (int it = 1; it <= maxiter; ++it) {
+    hipLaunchKernelGGL(mu_chain_step, dim3((unsigned)nbk), dim3(MUS_BLOCK), 0, s, a, (int)nbk, it);
+    if (int rc = check_launch("mu_chain_step")) return rc;
   }
-  if (p_out) {
-    hipLaunchKernelGGL(mu_occ_kernel, dim3((unsigned)nbk), dim3(MUS_BLOCK), 0, s, a);
-    return check_launch("mu_occ_kernel");
-  }
-  return HFX_OK;
+  hipLaunchKernelGGL(mu_chain_fin, dim3(p_out ? (unsigned)nbk : 1u), dim3(MUS_BLOCK), 0, s, a, (int)nbk,
+                     maxiter + 1);
+  return check_launch("mu_chain_fin");
 }
 
 int hfx_diis_coefficients(const double* brow, double* B, int nh, int idx, int count, double* coef, void* stream) {
