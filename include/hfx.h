@@ -245,6 +245,13 @@ int hfx_eigh_batched(int dtype, int nb, size_t nk, const void* A, double* lam, v
  * updates); rel_tol <= 1e-3. */
 int hfx_eigh_batched_tol(int dtype, int nb, size_t nk, const void* A, double* lam, void* V, int herm_in,
                          double rel_tol, void* stream);
+/* The eigendecomposition of A = U^dagger F U started from the basis U, as
+ * the reference's warm eigh of the rotated Fock matrix followed by the change
+ * of basis (solver.py:622-625, reference_scf.py:87-96): (lam, W) = eigh(A)
+ * to rel_tol (as hfx_eigh_batched_tol) and U <- U W in place, the rotations
+ * accumulated onto U's rows (no W is formed).  A must not alias U. */
+int hfx_eigh_batched_basis(int dtype, int nb, size_t nk, const void* A, double* lam, void* U, int herm_in,
+                           double rel_tol, void* stream);
 
 /* P = herm(Q diag(p) Q^dagger) (solver.py:42-44 + 462, utils.py:473-474);
  * when p is NULL, p = sigmoid((*mu - eps)/T) is formed (and stored to pout). */

@@ -55,6 +55,13 @@ def eigh(A, lam, V, herm_in=False, rel_tol=0.0):
            float(rel_tol), s(A.device))
 
 
+def eigh_basis(A, lam, U, herm_in=False, rel_tol=0.0):
+    """Eigendecomposition of A = U^H F U from the basis U: U <- U W in place, lam ascending."""
+    nk = A.numel() // (A.shape[-1] * A.shape[-1])
+    L.call("hfx_eigh_batched_basis", dt_code(A), A.shape[-1], nk, L.ptr(A), L.ptr(lam), L.ptr(U), int(herm_in),
+           float(rel_tol), s(A.device))
+
+
 def density(Q, P, p=None, eps=None, mu_ptr=None, T=1.0, pout=None):
     nk = Q.numel() // (Q.shape[-1] ** 2)
     L.call("hfx_density_from_Qp", dt_code(Q), Q.shape[-1], nk, L.ptr(Q), L.ptr(p), L.ptr(eps), mu_ptr,

@@ -361,7 +361,7 @@ class _BatchPolish:
         # written before read (history slots are used only once filled), so not
         # filled at allocation: one stacked matrix is up to 0.4 GB here
         e = lambda: torch.empty(self.shape4, **c)  # noqa: E731
-        self.P_prev, self.P_cur, self.P_new, self.Ft, self.W, self.U, self.tmp = e(), e(), e(), e(), e(), e(), e()
+        self.P_prev, self.P_cur, self.P_new, self.Ft, self.U, self.tmp = e(), e(), e(), e(), e(), e()
         self.Pfin, self.Ffin = e(), e()
         self.F_hist = [e() for _ in range(nh)]
         # residual history in complex64 at nb = 16, as in the single polish
@@ -417,9 +417,9 @@ class _BatchPolish:
             L.call("hfx_extrap_rotate_batched", K.dt_code(self.U), self.nb, self.nk_tot, self.Nk, count,
                    C.cast(self._xa, C.c_void_p), L.ptr(self.coef), L.ptr(self.U), L.ptr(self.Ft),
                    L.stream(self.dev))
-            K.eigh(self.Ft, self.eps, self.W, herm_in=True, rel_tol=self._eigh_tol)
+            K.eigh_basis(self.Ft, self.eps, self.U, herm_in=True, rel_tol=self._eigh_tol)   # U <- U W
             self._mu(self.MU_STEPS)
-            K.rotate_density(self.U, self.W, self.occ, self.P_new)
+            K.density(self.U, self.P_new, p=self.occ)
         else:                             # cold first update: eigh of herm(F_0) (count = 1: plain step)
             K.lincomb_multi(self.Ft, [self.F_hist[idx]], [1.0], herm=True)
             K.eigh(self.Ft, self.eps, self.U, herm_in=True)
@@ -513,9 +513,8 @@ class _BatchPolish:
         # orbitals / occupations / mu of the returned states (solver.py:606-638 form)
         if warm0 or k > 0 or spec:        # U holds a basis: warm eigh in it
             K.extrap_rotate(self.Ft, self.U, [self.Ffin], [1.0])
-            K.eigh(self.Ft, self.eps, self.W, herm_in=True)
+            K.eigh_basis(self.Ft, self.eps, self.U, herm_in=True)      # U <- U W
             self._mu()
-            K.rotate_density(self.U, self.W, self.occ, self.tmp)
         else:                             # every solve stopped before an update
             K.eigh(self.Ffin, self.eps, self.U, herm_in=True)
             self._mu()

@@ -26,6 +26,9 @@
 // in which no pair of the wave rotates is skipped.
 // Eigenvalues ascending, eigenvectors in the columns of V (the
 // reference's jnp.linalg.eigh convention, linalg.py:96-160, reference_scf.py:89).
+// With a basis U0 (A = U0^H F U0 for some F) the rotations are accumulated
+// onto the rows of U0 instead of the identity: V = U0 W, the eigenvectors of F,
+// with no separate product (V may be U0: each wave reads its rows first).
 #include "hfx_common.h"
 #include "hfx_internal.h"
 
@@ -162,8 +165,8 @@ __device__ __forceinline__ void j16_sweep(cx<T> (&b)[8], cx<T> (&w)[8], int m, i
 
 template <typename T, int P>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void eigh16_kernel(int nb, size_t nk, const cx<T>* __restrict__ A,
-                                                    double* __restrict__ lam, cx<T>* __restrict__ V, int herm_in,
-                                                    double rel_tol) {
+                                                    double* __restrict__ lam, cx<T>* V, const cx<T>* U0,
+                                                    int herm_in, double rel_tol) {
   __shared__ J16Lds<T> sh;
   __shared__ double sd[2][16];
   __shared__ int srank[2][16];
@@ -174,17 +177,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void ei
   const int ih = i ^ (h << 3);
   cx<T> b[8], w[8];
   T rs = (T)0;
+  // every load issued unconditionally (clamped to an element of this matrix,
+  // or of matrix 0 on idle lanes) and the padding selected away afterwards:
+  // loads under per-lane branches are waited for one branch at a time
+  const int ic = i < nb ? i : 0;
+  const bool u0 = U0 != nullptr;
+  cx<T> la[8], lt[8], lu[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const int x = ih ^ s, xc = x < nb ? x : 0;
+    la[s] = A[base + (size_t)ic * nb + xc];
+    if (herm_in) lt[s] = A[base + (size_t)xc * nb + ic];
+    if (u0) lu[s] = U0[base + (size_t)ic * nb + xc];
+  }
 #pragma unroll
   for (int s = 0; s < 8; ++s) {
     const int x = ih ^ s;
-    cx<T> v{(T)0, (T)0};
-    if (act && x < nb) {
-      v = A[base + (size_t)i * nb + x];
-      if (herm_in) v = (T)0.5 * (v + conj(A[base + (size_t)x * nb + i]));
-      if (x == i) v.im = (T)0;          // exact Hermitian diagonal
-    }
+    const bool in = act && x < nb;
+    cx<T> v = la[s];
+    if (herm_in) v = (T)0.5 * (v + conj(lt[s]));
+    if (x == i) v.im = (T)0;            // exact Hermitian diagonal
+    if (!in) v = cx<T>{(T)0, (T)0};
     b[s] = v;
-    w[s] = cx<T>{(x == i && act) ? (T)1 : (T)0, (T)0};
+    if (u0) w[s] = in ? lu[s] : cx<T>{(T)0, (T)0};
+    else w[s] = cx<T>{(x == i && act) ? (T)1 : (T)0, (T)0};
     rs += v.re * v.re + v.im * v.im;
   }
   rs += swz<1>(rs);
@@ -222,25 +238,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void ei
 }
 
 template <typename T>
-static void launch_eigh16(int nb, size_t nk, const cx<T>* A, double* lam, cx<T>* V, int herm_in, double rel_tol,
-                          hipStream_t s) {
+static void launch_eigh16(int nb, size_t nk, const cx<T>* A, double* lam, cx<T>* V, const cx<T>* U0, int herm_in,
+                          double rel_tol, hipStream_t s) {
   const dim3 grid((unsigned)((nk + 1) / 2));
   if (nb <= 2)
-    hipLaunchKernelGGL((eigh16_kernel<T, 2>), grid, dim3(64), 0, s, nb, nk, A, lam, V, herm_in, rel_tol);
+    hipLaunchKernelGGL((eigh16_kernel<T, 2>), grid, dim3(64), 0, s, nb, nk, A, lam, V, U0, herm_in, rel_tol);
   else if (nb <= 4)
-    hipLaunchKernelGGL((eigh16_kernel<T, 4>), grid, dim3(64), 0, s, nb, nk, A, lam, V, herm_in, rel_tol);
+    hipLaunchKernelGGL((eigh16_kernel<T, 4>), grid, dim3(64), 0, s, nb, nk, A, lam, V, U0, herm_in, rel_tol);
   else if (nb <= 8)
-    hipLaunchKernelGGL((eigh16_kernel<T, 8>), grid, dim3(64), 0, s, nb, nk, A, lam, V, herm_in, rel_tol);
+    hipLaunchKernelGGL((eigh16_kernel<T, 8>), grid, dim3(64), 0, s, nb, nk, A, lam, V, U0, herm_in, rel_tol);
   else
-    hipLaunchKernelGGL((eigh16_kernel<T, 16>), grid, dim3(64), 0, s, nb, nk, A, lam, V, herm_in, rel_tol);
+    hipLaunchKernelGGL((eigh16_kernel<T, 16>), grid, dim3(64), 0, s, nb, nk, A, lam, V, U0, herm_in, rel_tol);
 }
 
 int eigh16_launch(int dtype, int nb, size_t nk, const void* A, double* lam, void* V, int herm_in, double rel_tol,
-                  hipStream_t s) {
+                  hipStream_t s, const void* U0) {
   if (dtype == HFX_DTYPE_C128)
-    launch_eigh16<double>(nb, nk, (const cx<double>*)A, lam, (cx<double>*)V, herm_in, rel_tol, s);
+    launch_eigh16<double>(nb, nk, (const cx<double>*)A, lam, (cx<double>*)V, (const cx<double>*)U0, herm_in,
+                          rel_tol, s);
   else
-    launch_eigh16<float>(nb, nk, (const cx<float>*)A, lam, (cx<float>*)V, herm_in, rel_tol, s);
+    launch_eigh16<float>(nb, nk, (const cx<float>*)A, lam, (cx<float>*)V, (const cx<float>*)U0, herm_in, rel_tol,
+                         s);
   return check_launch("eigh16_kernel");
 }
 

@@ -105,7 +105,7 @@ int exchange_max_plane(int dtype);
 int epilogue_seg_launch(const hfx_plan* p, int nsolve, const void* h, const void* Sig, const void* P, const double* w2d,
                         void* F, double* st, double* part, hipStream_t s);
 int eigh16_launch(int dtype, int nb, size_t nk, const void* A, double* lam, void* V, int herm_in, double rel_tol,
-                  hipStream_t s);
+                  hipStream_t s, const void* U0 = nullptr);
 int exchange_lds_elems(int n1, int n2);
 int pack_planar_launch(const hfx_plan* p, int nsolve, const void* X, const void* refX, const int* chans, int nchan,
                        void* Xt, hipStream_t s);

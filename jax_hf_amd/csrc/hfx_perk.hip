@@ -333,7 +333,7 @@ __device__ void m_eigh(cx<T>* A, cx<T>* V, cx<T>* Vt, double* lam, int nb, doubl
 // herm_in: hermitise the input first (reference_scf.py:89 eigh(_herm(F))).
 template <int NB, typename T>
 __global__ __launch_bounds__(PKB) void pk_eigh_kernel(int nb, const cx<T>* __restrict__ A, double* __restrict__ lam,
-                                                     cx<T>* __restrict__ V, int herm_in) {
+                                                     cx<T>* V, const cx<T>* U0, int herm_in) {
   __shared__ cx<T> sA[PK_MAT], sV[PK_MAT], sT[PK_MAT];
   __shared__ double sl[PK_NBMAX];
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
@@ -347,7 +347,14 @@ __global__ __launch_bounds__(PKB) void pk_eigh_kernel(int nb, const cx<T>* __res
   }
   m_eigh<NB>(sA, sV, sT, sl, nb, nullptr);
   if ((int)threadIdx.x < nb) lam[k * nb + threadIdx.x] = sl[threadIdx.x];
-  m_store<NB>(V + k * nb2, sV, nb);
+  if (U0) {   // eigenvectors in the basis U0: V = U0 W (V may be U0)
+    m_load<NB>(sT, U0 + k * nb2, nb);
+    __syncthreads();
+    m_gemm<NB, T, false, false>(sA, sT, sV, nb);
+    m_store<NB>(V + k * nb2, sA, nb);
+  } else {
+    m_store<NB>(V + k * nb2, sV, nb);
+  }
 }
 
 // P = herm(Q diag(p) Q^dagger)  (solver.py:42-44, 462; utils.py:473-474)
@@ -1346,23 +1353,36 @@ int hfx_eigh_batched(int dtype, int nb, size_t nk, const void* A, double* lam, v
   return hfx_eigh_batched_tol(dtype, nb, nk, A, lam, V, herm_in, 0.0, stream);
 }
 
-int hfx_eigh_batched_tol(int dtype, int nb, size_t nk, const void* A, double* lam, void* V, int herm_in,
-                         double rel_tol, void* stream) {
-  if (int rc = pk_check_nb(nb)) return rc;
-  if (!A || !lam || !V || !(rel_tol >= 0.0) || rel_tol > 1e-3)
-    return set_error(HFX_EINVAL, "hfx_eigh_batched: bad argument (rel_tol=%g)", rel_tol);
+static int eigh_any(int dtype, int nb, size_t nk, const void* A, double* lam, void* V, const void* U0, int herm_in,
+                    double rel_tol, void* stream) {
   if (nk == 0) return HFX_OK;
-  if (nb <= 16) return eigh16_launch(dtype, nb, nk, A, lam, V, herm_in, rel_tol, (hipStream_t)stream);
+  if (nb <= 16) return eigh16_launch(dtype, nb, nk, A, lam, V, herm_in, rel_tol, (hipStream_t)stream, U0);
   // 16 < nb <= 32: LDS-staged cyclic Jacobi, one 256-thread workgroup per
   // matrix, rotating to its own threshold (rel_tol only ever stops earlier)
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
     hipLaunchKernelGGL((pk_eigh_kernel<32, double>), dim3(nk), dim3(256), 0, s, nb, PKC(double, A), lam,
-                       PKM(double, V), herm_in);
+                       PKM(double, V), PKC(double, U0), herm_in);
   else
     hipLaunchKernelGGL((pk_eigh_kernel<32, float>), dim3(nk), dim3(256), 0, s, nb, PKC(float, A), lam,
-                       PKM(float, V), herm_in);
+                       PKM(float, V), PKC(float, U0), herm_in);
   return check_launch("pk_eigh_kernel");
+}
+
+int hfx_eigh_batched_tol(int dtype, int nb, size_t nk, const void* A, double* lam, void* V, int herm_in,
+                         double rel_tol, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  if (!A || !lam || !V || !(rel_tol >= 0.0) || rel_tol > 1e-3)
+    return set_error(HFX_EINVAL, "hfx_eigh_batched: bad argument (rel_tol=%g)", rel_tol);
+  return eigh_any(dtype, nb, nk, A, lam, V, nullptr, herm_in, rel_tol, stream);
+}
+
+int hfx_eigh_batched_basis(int dtype, int nb, size_t nk, const void* A, double* lam, void* U, int herm_in,
+                           double rel_tol, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  if (!A || !lam || !U || A == U || !(rel_tol >= 0.0) || rel_tol > 1e-3)
+    return set_error(HFX_EINVAL, "hfx_eigh_batched_basis: bad argument (rel_tol=%g)", rel_tol);
+  return eigh_any(dtype, nb, nk, A, lam, U, U, herm_in, rel_tol, stream);
 }
 
 int hfx_density_from_Qp(int dtype, int nb, size_t nk, const void* Q, const double* p, const double* eps,

@@ -106,7 +106,6 @@ class _SCF:
         self.tmp = torch.zeros(self.shape4, **c)
         self.U = torch.zeros(self.shape4, **c)
         self.Ft = torch.empty(self.shape4, **c)
-        self.W = torch.empty(self.shape4, **c)
         self.n_warm = -1                  # eigh calls in the running basis U (-1: no basis yet)
         self.allow_warm = False
         self.eps = torch.zeros((nk1, nk2, nb), dtype=torch.float64, device=self.dev)
@@ -152,7 +151,7 @@ class _SCF:
         warm = self.n_warm >= 0 and self.WARM_EIGH and self.allow_warm
         if warm:
             K.extrap_rotate(self.Ft, self.U, [F], [1.0])
-            K.eigh(self.Ft, self.eps, self.W, herm_in=True)
+            K.eigh_basis(self.Ft, self.eps, self.U, herm_in=True)      # U <- U W
         else:
             K.eigh(F, self.eps, self.U, herm_in=True)
         if self.mu_newton and self.cfg.level_shift == 0.0:
@@ -165,13 +164,12 @@ class _SCF:
             K.solve_mu(self.wsp, self.eps, self.w, self.nk, self.nb, self.n_e, self.kernel.T,
                        maxiter=30 if self.single else 54, mode=K.MU_BISECTION,
                        level_shift=self.cfg.level_shift, mu_out_ptr=K.st_ptr(self.st, mu_slot), p_out=self.occ)
+        K.density(self.U, self.tmp, p=self.occ)
         if warm:
-            K.rotate_density(self.U, self.W, self.occ, self.tmp)
             self.n_warm += 1
             if self.n_warm % self.REORTH_EVERY == 0:
                 K.orthonormalize(self.U, 1)
         else:
-            K.density(self.U, self.tmp, p=self.occ)
             self.n_warm = 0
         return self.herm_project(self.tmp, P_out)
 

@@ -384,11 +384,10 @@ class _CG:
         """solver.py:606-638: Fock, eigh canonicalisation, mu, p, Fock + E."""
         P = self.density()
         self.fock.build(P, self.F)
-        # Q <- Q W, (eps, W) = eigh(Q^dagger F Q)  (:622-625), with the register eigensolver
+        # Q <- Q W, (eps, W) = eigh(Q^dagger F Q)  (:622-625): the register eigensolver
+        # accumulates the rotations onto Q in place (the captured graph holds this buffer as Q)
         K.matmul(self.Q, self.F, self.Ft, C=self.Q, ops=0b001)              # Ft = Q^H F Q
-        K.eigh(self.Ft, self.eps, self.V, herm_in=True)
-        K.matmul(self.Q, self.V, self.H)                                      # Q W
-        K.lincomb(self.Q, self.H, 1.0)       # copy back: the captured graph holds this buffer as Q
+        K.eigh_basis(self.Ft, self.eps, self.Q, herm_in=True)
         self.mu(self.eps, True)
         P = self.density()
         Pout = P.clone()
@@ -488,7 +487,7 @@ class _DIISPolish:
         c = dict(dtype=cdt, device=dev)
         z = lambda: torch.zeros(self.shape4, **c)  # noqa: E731
         self.P_prev, self.P_cur, self.P_new, self.Fx, self.U, self.tmp = z(), z(), z(), z(), z(), z()
-        self.Ft, self.W = z(), z()
+        self.Ft = z()
         self.F_hist = [z() for _ in range(nh)]
         # DIIS residual history: complex64 at nb = 16 (R and its norms are formed
         # in FP64; only the stored copies that feed the B matrix are rounded);
@@ -540,12 +539,12 @@ class _DIISPolish:
 
     def _update(self, out, warm=True):
         """-> out (the next iterate): launches only (the warm form is captured,
-        once per output buffer).  Warm: eigh of Ft = U^H F U (eps, W), mu, then
-        U <- U W and the density from (U, occ) in one pass."""
+        once per output buffer).  Warm: eigh of Ft = U^H F U with the rotations
+        accumulated onto U (U <- U W, eps), mu, then the density from (U, occ)."""
         if warm:
-            K.eigh(self.Ft, self.eps, self.W, herm_in=True, rel_tol=self._eigh_tol)
+            K.eigh_basis(self.Ft, self.eps, self.U, herm_in=True, rel_tol=self._eigh_tol)
             self._mu(self.MU_GRAPH_STEPS)
-            K.rotate_density(self.U, self.W, self.occ, out)
+            K.density(self.U, out, p=self.occ)
         else:
             K.eigh(self.Fx, self.eps, self.U, herm_in=True)
             self._mu(self.MU_GRAPH_STEPS)
@@ -687,14 +686,12 @@ class _DIISPolish:
         warm = k > 0 or warm0 or spec
         if warm:
             K.extrap_rotate(self.Ft, self.U, [Ffin], [1.0])
-            K.eigh(self.Ft, self.eps, self.W, herm_in=True)
+            K.eigh_basis(self.Ft, self.eps, self.U, herm_in=True)       # U <- U W
         else:
             K.eigh(Ffin, self.eps, self.U, herm_in=True)
         self._mu(self.MU_GRAPH_STEPS)
         if float(self.st[K.ST_MUDONE].item()) == 0.0:
             self._mu()
-        if warm:
-            K.rotate_density(self.U, self.W, self.occ, self.tmp)
         Q = self.U.clone()
         return (P_res.clone(), Ffin.clone(), Q, self.occ.clone(), self.st[K.ST_MU].clone(), E, n_it,
                 converged, hE, hC)

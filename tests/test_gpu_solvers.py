@@ -605,6 +605,47 @@ def test_extrap_rotate_and_rotate_density(dev, dt, nb):
     assert np.abs(_np(P) - Pr).max() <= tol * 10
 
 
+@pytest.mark.parametrize("dt", ["c128", "c64"])
+@pytest.mark.parametrize("nb", [3, 8, 16, 24])
+@pytest.mark.parametrize("warm", [False, True])
+def test_eigh_basis(dev, dt, nb, warm):
+    """hfx_eigh_batched_basis: eigh of A = U^H F U with the rotations accumulated
+    onto U in place -- the new U diagonalises F (U^H F U = diag(lam) against
+    LAPACK's eigenvalues of F), stays unitary, and equals U times the plain
+    eigh's eigenvectors of A up to each column's phase (non-degenerate F).
+    warm: U already near F's eigenvectors (the polish's running basis)."""
+    import torch
+    from jax_hf_amd import _ops as K
+    rng = np.random.default_rng(100 + nb)
+    nk = 29
+    cdt = torch.complex128 if dt == "c128" else torch.complex64
+    tol = 1e-12 if dt == "c128" else 3e-5
+    F = _herm(rng.standard_normal((nk, nb, nb)) + 1j * rng.standard_normal((nk, nb, nb)))
+    if warm:
+        U = np.linalg.eigh(F + 1e-3 * _herm(rng.standard_normal((nk, nb, nb))))[1]
+    else:
+        U = np.linalg.qr(rng.standard_normal((nk, nb, nb)) + 1j * rng.standard_normal((nk, nb, nb)))[0]
+    A = np.conj(np.swapaxes(U, -1, -2)) @ F @ U
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev).to(cdt)   # noqa: E731
+    At, Ut = t(A), t(U)
+    lam = torch.empty((nk, nb), dtype=torch.float64, device=dev)
+    K.eigh_basis(At, lam, Ut, herm_in=True)
+    Un, ln = _np(Ut).astype(np.complex128), _np(lam)
+    lref = np.linalg.eigvalsh(F)
+    sc = np.abs(lref).max()
+    assert np.abs(ln - lref).max() <= tol * sc * 10
+    D = np.conj(np.swapaxes(Un, -1, -2)) @ F @ Un
+    assert np.abs(D - ln[:, None, :] * np.eye(nb)).max() <= tol * sc * 20
+    assert np.abs(np.conj(np.swapaxes(Un, -1, -2)) @ Un - np.eye(nb)).max() <= tol * 20
+    V = torch.empty_like(At)
+    lam2 = torch.empty_like(lam)
+    K.eigh(t(A), lam2, V, herm_in=True)
+    Uv = U @ _np(V).astype(np.complex128)
+    ph = np.einsum("kin,kin->kn", np.conj(Uv), Un)          # column overlaps, |.| = 1
+    assert np.abs(np.abs(ph) - 1).max() <= tol * 1e3
+    assert np.abs(_np(lam2) - ln).max() <= tol * sc * 10
+
+
 @pytest.mark.parametrize("nk,nb,T,ex,seed", [(4, 4, 0.05, 0.3, 7), (5, 5, 0.05, 0.4, 4)])
 def test_newton_mixed_precision_matches_fp64(dev, nk, nb, T, ex, seed):
     """precision="mixed" Newton (complex64 Fock-response builds inside the
