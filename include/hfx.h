@@ -395,6 +395,17 @@ int hfx_vec_shift(double* x, size_t n, const double* s, double scale, void* stre
  * formed inside the launch: the projection of solver.py:224-233 in one launch. */
 int hfx_vec_shift_sum(double* x, size_t n, const double* part, const double* w, size_t nk, double scale,
                       void* stream);
+/* Up to four hfx_lincomb_dev updates of arrays of one shape in one launch:
+ * op j: out1[j] = a_j' X1[j] + b_j' Y1[j] (n1 elements of kind1; Y1[j] NULL:
+ * zero) and the same on the second arrays (n2 of kind2), a_j' = a[j] *
+ * dev_scalars[a_idx[j]] when a_idx[j] >= 0, else a[j] (b alike).  The ops are
+ * applied in order at each element, so op j may read an array that a later
+ * op writes: the four updates of one truncated-CG inner step of the Newton
+ * subproblem (solver.py:975-989) as one pass. */
+int hfx_lincomb_dev_batch(int nops, size_t n1, int kind1, size_t n2, int kind2, const void* const* X1,
+                          const void* const* Y1, void* const* out1, const void* const* X2, const void* const* Y2,
+                          void* const* out2, const double* a, const int* a_idx, const double* b, const int* b_idx,
+                          const double* dev_scalars, void* stream);
 /* out_i = a' X_i + b' Y_i for two arrays in one launch (n2 = 0: the first
  * only; Y NULL: zero), a' = a * (*a_dev) when a_dev is given, else a (b' alike):
  * the updates of the Newton subproblem's truncated CG with coefficients that
@@ -458,6 +469,17 @@ size_t hfx_solver_scratch_doubles(void);
  * then advances st[HFX_ST_ITER]. */
 int hfx_reduce(const double* part, int nval, size_t nk, const double* w, unsigned wmask, double* out,
                double* st, int op, const double* params, double* scratch, void* stream);
+
+/* The weighted inner products of np <= 4 (matrix, vector) pairs of the
+ * Newton subproblem (solver.py:947-991, the _jip of the (X, p) tangent
+ * vectors): out[v] = sum_k (w_k if bit v of wmask) Re <X_v[k], Y_v[k]>_F and
+ * out[np + v] = sum_k (w_k if bit np + v) <vx_v[k], vy_v[k]> (vectors of nb
+ * reals per k), in a fixed order; then the scalar step op (HFX_OP_SUM or
+ * HFX_OP_NT0..2 on st) as hfx_reduce.  X, Y: (nk, nb, nb) of dtype; w: (nk).
+ * Two launches; scratch as hfx_reduce. */
+int hfx_pair_dots(int dtype, int nb, size_t nk, int np, const void* const* X, const void* const* Y,
+                  const double* const* vx, const double* const* vy, const double* w, unsigned wmask, double* out,
+                  double* st, int op, double* scratch, void* stream);
 
 #define HFX_MU_NEWTON_BRACKET 0
 #define HFX_MU_BISECTION 1

@@ -21,7 +21,7 @@ LIB_PATH = _HERE / "libhfx.so"
 
 HFX_C128 = 0
 HFX_C64 = 1
-ABI_VERSION = 18
+ABI_VERSION = 19
 # Largest band count of the per-k solver kernels (nb <= 16: one wavefront per
 # k-point, nb <= 32: a 256-thread workgroup per k-point).
 PERK_NB_MAX = 32
@@ -110,6 +110,8 @@ _SIGS = {
     "hfx_solve_mu": [vp, vp, sz, i32, f64, f64, f64, i32, i32, i32, f64, vp, vp, vp, vp, vp],
     "hfx_qr_retract": [i32, i32, sz, vp, vp, vp, vp],
     "hfx_lincomb_dev": [sz, i32, vp, vp, vp, sz, i32, vp, vp, vp, f64, vp, f64, vp, vp],
+    "hfx_lincomb_dev_batch": [i32, sz, i32, sz, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
+    "hfx_pair_dots": [i32, i32, sz, i32, vp, vp, vp, vp, vp, C.c_uint, vp, vp, i32, vp, vp],
     "hfx_noalloc_scratch_doubles": [],
     "hfx_solve_mu_noalloc": [vp, vp, sz, i32, f64, f64, f64, i32, i32, i32, f64, vp, vp, vp, vp],
 }

@@ -177,6 +177,59 @@ int reduce_blocks(size_t nk) {
   return (int)(g < 1 ? 1 : g);
 }
 
+// Inner products of (matrix, vector) pairs straight into block partials:
+//   value v      = sum_k (w_k if bit v of wmask) Re <X_v[k], Y_v[k]>_F
+//   value np + v = sum_k (w_k if bit np + v)     <vx_v[k], vy_v[k]>
+// (the weighted inner products of the Newton subproblem, solver.py:947-991);
+// a grid-stride pass over the elements, one block sum per value, then
+// reduce_stage2's fixed-order sum and scalar step.  Two launches in place of a
+// per-k dots pass, a vector dots pass and a two-launch reduction.
+constexpr int PD_MAX = 4;
+struct PairDots {
+  const void* x[PD_MAX];
+  const void* y[PD_MAX];
+  const double* vx[PD_MAX];
+  const double* vy[PD_MAX];
+};
+
+template <typename T>
+__global__ __launch_bounds__(RED_BLOCK) void pair_dots_kernel(PairDots d, int np, size_t nk, int nb,
+                                                              const double* __restrict__ w, unsigned wmask,
+                                                              double* __restrict__ s1) {
+  __shared__ double sh[RED_MAX_VALS * (RED_BLOCK / 64)];
+  double acc[2 * PD_MAX];
+#pragma unroll
+  for (int v = 0; v < 2 * PD_MAX; ++v) acc[v] = 0.0;
+  const unsigned nb2 = (unsigned)(nb * nb), unb = (unsigned)nb;
+  const size_t nm = nk * nb2, nvec = nk * unb, stride = (size_t)gridDim.x * RED_BLOCK;
+  for (size_t e = (size_t)blockIdx.x * RED_BLOCK + threadIdx.x; e < nm; e += stride) {
+    const double wk = w ? w[(unsigned)e / nb2] : 1.0;   // nk nb^2 < 2^32
+#pragma unroll
+    for (int v = 0; v < PD_MAX; ++v)
+      if (v < np) {
+        const cx<T> x = reinterpret_cast<const cx<T>*>(d.x[v])[e];
+        const cx<T> y = reinterpret_cast<const cx<T>*>(d.y[v])[e];
+        const double t = (double)x.re * y.re + (double)x.im * y.im;
+        acc[v] += ((wmask >> v) & 1u) ? wk * t : t;
+      }
+  }
+  for (size_t e = (size_t)blockIdx.x * RED_BLOCK + threadIdx.x; e < nvec; e += stride) {
+    const double wk = w ? w[(unsigned)e / unb] : 1.0;
+#pragma unroll
+    for (int v = 0; v < PD_MAX; ++v)
+      if (v < np) {
+        const double t = d.vx[v][e] * d.vy[v][e];
+        acc[PD_MAX + v] += ((wmask >> (np + v)) & 1u) ? wk * t : t;
+      }
+  }
+  block_sum<2 * PD_MAX>(acc, sh);
+  if (threadIdx.x == 0)
+    for (int v = 0; v < np; ++v) {
+      s1[(size_t)v * gridDim.x + blockIdx.x] = acc[v];
+      s1[(size_t)(np + v) * gridDim.x + blockIdx.x] = acc[PD_MAX + v];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // chemical potential
 // ---------------------------------------------------------------------------
@@ -1202,6 +1255,40 @@ int hfx_reduce(const double* part, int nval, size_t nk, const double* w, unsigne
   hipLaunchKernelGGL(reduce_stage1, dim3(g), dim3(RED_BLOCK), 0, s, part, nval, nk, w, wmask, scratch, st, op);
   if (int rc = check_launch("reduce_stage1")) return rc;
   hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(RED_BLOCK), 0, s, scratch, nval, g, out, st, pa);
+  return check_launch("reduce_stage2");
+}
+
+int hfx_pair_dots(int dtype, int nb, size_t nk, int np, const void* const* X, const void* const* Y,
+                  const double* const* vx, const double* const* vy, const double* w, unsigned wmask, double* out,
+                  double* st, int op, double* scratch, void* stream) {
+  if (nb < 1 || np < 1 || np > PD_MAX || !X || !Y || !vx || !vy || !scratch || (!out && op == HFX_OP_SUM) ||
+      (op != HFX_OP_SUM && op != HFX_OP_NT0 && op != HFX_OP_NT1 && op != HFX_OP_NT2) || (op != HFX_OP_SUM && !st) ||
+      nk * (size_t)nb * nb >= 0xFFFFFFFFull || (dtype != HFX_DTYPE_C128 && dtype != HFX_DTYPE_C64))
+    return set_error(HFX_EINVAL, "hfx_pair_dots: bad argument");
+  PairDots d{};
+  for (int v = 0; v < np; ++v) {
+    if (!X[v] || !Y[v] || !vx[v] || !vy[v]) return set_error(HFX_EINVAL, "hfx_pair_dots: pair %d is NULL", v);
+    d.x[v] = X[v];
+    d.y[v] = Y[v];
+    d.vx[v] = vx[v];
+    d.vy[v] = vy[v];
+  }
+  if (nk == 0) return HFX_OK;
+  const size_t nm = nk * (size_t)nb * nb;
+  size_t g = (nm + (size_t)RED_BLOCK * 4 - 1) / ((size_t)RED_BLOCK * 4);
+  if (g > (size_t)RED_MAX_BLOCKS) g = RED_MAX_BLOCKS;
+  if (g < 1) g = 1;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL(pair_dots_kernel<double>, dim3((unsigned)g), dim3(RED_BLOCK), 0, s, d, np, nk, nb, w, wmask,
+                       scratch);
+  else
+    hipLaunchKernelGGL(pair_dots_kernel<float>, dim3((unsigned)g), dim3(RED_BLOCK), 0, s, d, np, nk, nb, w, wmask,
+                       scratch);
+  if (int rc = check_launch("pair_dots_kernel")) return rc;
+  PostArgs pa{};
+  pa.op = op;
+  hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(RED_BLOCK), 0, s, scratch, 2 * np, (int)g, out, st, pa);
   return check_launch("reduce_stage2");
 }
 

@@ -144,6 +144,47 @@ __global__ void lincomb_dev_kernel(size_t n1, int kind1, const void* X1, const v
   }
 }
 
+// Up to four device-coefficient updates in one pass, applied per element in
+// order (so an op may read what an earlier op of the list reads and a later
+// op writes, at the same index): the truncated-CG updates of one inner step.
+constexpr int LCD_MAX = 4;
+struct LcdOps {
+  const void* x1[LCD_MAX];
+  const void* y1[LCD_MAX];
+  void* o1[LCD_MAX];
+  const void* x2[LCD_MAX];
+  const void* y2[LCD_MAX];
+  void* o2[LCD_MAX];
+  double a[LCD_MAX], b[LCD_MAX];
+  int ai[LCD_MAX], bi[LCD_MAX];
+};
+
+__global__ void lincomb_dev_batch_kernel(int nops, size_t n1, int kind1, size_t n2, int kind2, LcdOps op,
+                                         const double* sc) {
+  double ca[LCD_MAX], cb[LCD_MAX];
+#pragma unroll
+  for (int j = 0; j < LCD_MAX; ++j) {
+    ca[j] = (j < nops && op.ai[j] >= 0) ? op.a[j] * sc[op.ai[j]] : op.a[j];
+    cb[j] = (j < nops && op.bi[j] >= 0) ? op.b[j] * sc[op.bi[j]] : op.b[j];
+  }
+  const size_t n = n1 + n2;
+  for (size_t i = (size_t)blockIdx.x * UTIL_BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * UTIL_BLOCK) {
+    const bool first = i < n1;
+    const size_t e = first ? i : i - n1;
+    const int kind = first ? kind1 : kind2;
+#pragma unroll
+    for (int j = 0; j < LCD_MAX; ++j)
+      if (j < nops) {
+        const void* X = first ? op.x1[j] : op.x2[j];
+        const void* Y = first ? op.y1[j] : op.y2[j];
+        double xr, xi, yr = 0.0, yi = 0.0;
+        load_any(X, kind, e, xr, xi);
+        if (Y) load_any(Y, kind, e, yr, yi);
+        store_any(first ? op.o1[j] : op.o2[j], kind, e, ca[j] * xr + cb[j] * yr, ca[j] * xi + cb[j] * yi);
+      }
+  }
+}
+
 // out = sum_v c_v X_v over nmat nb x nb matrices, optionally Hermitised per
 // matrix: out_ij = (Y_ij + conj(Y_ji)) / 2 with Y = sum_v c_v X_v.
 constexpr int LCM_MAX = 8;
@@ -371,6 +412,37 @@ int hfx_lincomb_dev(size_t n1, int kind1, const void* X1, const void* Y1, void* 
   hipLaunchKernelGGL(lincomb_dev_kernel, dim3(grid_for(n)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream, n1, kind1, X1,
                      Y1, out1, n2, kind2, X2, Y2, out2, a, a_dev, b, b_dev);
   return check_launch("lincomb_dev_kernel");
+}
+
+int hfx_lincomb_dev_batch(int nops, size_t n1, int kind1, size_t n2, int kind2, const void* const* X1,
+                          const void* const* Y1, void* const* out1, const void* const* X2, const void* const* Y2,
+                          void* const* out2, const double* a, const int* a_idx, const double* b, const int* b_idx,
+                          const double* dev_scalars, void* stream) {
+  if (nops < 1 || nops > LCD_MAX || !X1 || !Y1 || !out1 || !a || !a_idx || !b || !b_idx || kind1 < 0 || kind1 > 3 ||
+      (n2 && (!X2 || !Y2 || !out2 || kind2 < 0 || kind2 > 3)))
+    return set_error(HFX_EINVAL, "hfx_lincomb_dev_batch: bad argument");
+  LcdOps op{};
+  for (int j = 0; j < nops; ++j) {
+    if (!X1[j] || !out1[j] || (n2 && (!X2[j] || !out2[j])) || ((a_idx[j] >= 0 || b_idx[j] >= 0) && !dev_scalars))
+      return set_error(HFX_EINVAL, "hfx_lincomb_dev_batch: op %d is incomplete", j);
+    op.x1[j] = X1[j];
+    op.y1[j] = Y1[j];
+    op.o1[j] = out1[j];
+    if (n2) {
+      op.x2[j] = X2[j];
+      op.y2[j] = Y2[j];
+      op.o2[j] = out2[j];
+    }
+    op.a[j] = a[j];
+    op.b[j] = b[j];
+    op.ai[j] = a_idx[j];
+    op.bi[j] = b_idx[j];
+  }
+  const size_t n = n1 + n2;
+  if (n == 0) return HFX_OK;
+  hipLaunchKernelGGL(lincomb_dev_batch_kernel, dim3(grid_for(n)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream, nops, n1,
+                     kind1, n2, kind2, op, dev_scalars);
+  return check_launch("lincomb_dev_batch_kernel");
 }
 
 int hfx_lincomb_multi(size_t nmat, int nb, int kind, int nin, const void* const* X, const double* coef, int herm,

@@ -93,19 +93,20 @@ class _Newton:
 
     def jdots(self, pairs, wx=0, phase=None):
         """Reduce the inner products of n pairs of (X, p) vectors into out[0:2n],
-        one reduction: X parts at out[i] (weighted by w2d for the bits of wx),
-        p parts weighted by w2d at out[n + i]; with ``phase`` the reduction's
-        last workgroup also takes that Steihaug scalar step; launches only."""
+        one fused pass and one fixed-order sum: X parts at out[i] (weighted by
+        w2d for the bits of wx), p parts weighted by w2d at out[n + i]; with
+        ``phase`` the sum's workgroup also takes that Steihaug scalar step;
+        launches only."""
+        import ctypes as C
         n = len(pairs)
-        part = self.ws.part.view(-1)
-        K.dots([a[0] for a, _ in pairs], [b[0] for _, b in pairs], part, mode=0)
-        K.vdots([a[1] for a, _ in pairs], [b[1] for _, b in pairs], part[n * self.nk:])
+        keep = [(C.c_void_p * n)(*[t.data_ptr() for t in ts])
+                for ts in ([a[0] for a, _ in pairs], [b[0] for _, b in pairs],
+                           [a[1] for a, _ in pairs], [b[1] for _, b in pairs])]
         wmask = int(wx) | (((1 << n) - 1) << n)
-        if phase is None:
-            self.sums(2 * n, self.w2d, wmask, 0)
-        else:
-            K.reduce(self.ws, self.ws.part, 2 * n, self.nk, w=self.w2d, wmask=wmask, out=self.out[0:2 * n],
-                     st=self.nt, op=K.OP_NT0 + int(phase))
+        op = K.OP_SUM if phase is None else K.OP_NT0 + int(phase)
+        L.call("hfx_pair_dots", self.dt, self.nb, self.nk, n, *[C.cast(a, C.c_void_p) for a in keep],
+               L.ptr(self.w2d), wmask, L.ptr(self.out), None if phase is None else L.ptr(self.nt), op,
+               L.ptr(self.ws.scratch), L.stream(self.dev))
 
     def nt_ptr(self, idx):
         return None if idx is None else self.nt.data_ptr() + 8 * int(idx)
@@ -118,6 +119,21 @@ class _Newton:
                op.numel(), KIND[op.dtype], L.ptr(ap), L.ptr(bp), L.ptr(op),
                float(ca), self.nt_ptr(ca_dev), float(cb), self.nt_ptr(cb_dev), L.stream(self.dev))
 
+    def axpys(self, ops):
+        """Several axpy updates (out, a, b, ca, ca_dev, cb, cb_dev) of (X, p) pairs
+        in one launch, applied in order at each element."""
+        import ctypes as C
+        n = len(ops)
+        keep = [(C.c_void_p * n)(*[o[a][part].data_ptr() for o in ops])
+                for part in (0, 1) for a in (1, 2, 0)]                  # X1, Y1, out1, X2, Y2, out2
+        keep += [(C.c_double * n)(*[float(o[3]) for o in ops]),
+                 (C.c_int * n)(*[-1 if o[4] is None else int(o[4]) for o in ops]),
+                 (C.c_double * n)(*[float(o[5]) for o in ops]),
+                 (C.c_int * n)(*[-1 if o[6] is None else int(o[6]) for o in ops])]
+        oX, op_ = ops[0][0]
+        L.call("hfx_lincomb_dev_batch", n, oX.numel(), KIND[oX.dtype], op_.numel(), KIND[op_.dtype],
+               *[C.cast(a, C.c_void_p) for a in keep], L.ptr(self.nt), L.stream(self.dev))
+
     def tr_step(self, B, Q, p, Ft, eps, pen):
         """One step of the Steihaug truncated CG (solver.py:947-991), launches only:
         the Hessian-vector product, alpha and the boundary step, the trial point
@@ -129,14 +145,15 @@ class _Newton:
         d, Hd = (dX, dp), (HdX, Hdp)
         self.hvp(dX, dp, HdX, Hdp, Q, p, Ft, pen, B["dP"], B["R"])
         self.jdots([(d, Hd), (d, d), (v, d), (v, v)], phase=0)
-        self.axpy(va, v, d, cb_dev=NT_ALPHA)                     # v + alpha d
-        self.axpy(ra, r, Hd, cb_dev=NT_NALPHA)                   # r - alpha H d
+        self.axpys([(va, v, d, 1.0, None, 1.0, NT_ALPHA),        # v + alpha d
+                    (ra, r, Hd, 1.0, None, 1.0, NT_NALPHA)])     # r - alpha H d
         self.prec(ra[0], ra[1], za[0], za[1], eps, p)
         self.jdots([(va, va), (ra, ra), (ra, za)], phase=1)
-        self.axpy(v, v, d, cb_dev=NT_CV)                         # boundary step or v + alpha d
-        self.axpy(Hv, Hv, Hd, cb_dev=NT_CV)
-        self.axpy(r, r, Hd, cb_dev=NT_CR)
-        self.axpy(d, za, d, ca_dev=NT_CDA, cb_dev=NT_CDB)        # d = z + beta d (unchanged once stopped)
+        # in this order at each element: d is read by the first op and written by the last
+        self.axpys([(v, v, d, 1.0, None, 1.0, NT_CV),            # boundary step or v + alpha d
+                    (Hv, Hv, Hd, 1.0, None, 1.0, NT_CV),
+                    (r, r, Hd, 1.0, None, 1.0, NT_CR),
+                    (d, za, d, 1.0, NT_CDA, 1.0, NT_CDB)])       # d = z + beta d (unchanged once stopped)
 
     def project_occ(self, vec):
         """vec -= sum_k w_k sum_b vec / (sum w nb) (solver.py:224-233); part holds
