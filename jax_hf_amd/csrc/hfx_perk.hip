@@ -51,13 +51,26 @@ __device__ __forceinline__ double pk_sum(double v) {
 }
 #define PK_SUM(x) pk_sum<PKB>(x)
 
-#define PK_FOR_ELEM(nb, ...)                                    \
-  _Pragma("unroll") for (int m_ = 0; m_ < PK_EPL; ++m_) {       \
-    const int e = threadIdx.x + PKB * m_;                        \
-    if (e < (nb) * (nb)) {                                      \
-      const int i = e / (nb), j = e - i * (nb);                 \
-      __VA_ARGS__                                               \
-    }                                                           \
+// Element loop of one nb x nb matrix over the workgroup: element e = (i, j).
+// nb equal to the size class: every slot is an element, so the body runs
+// without a per-lane guard and the compiler issues its global loads together
+// (a load under a guard is waited for before the next guard's load); smaller
+// nb: the guarded form.
+#define PK_FOR_ELEM(nb, ...)                                                   \
+  if (PK_EPL * PKB == PK_NBMAX * PK_NBMAX && (nb) == PK_NBMAX) {               \
+    _Pragma("unroll") for (int m_ = 0; m_ < PK_EPL; ++m_) {                    \
+      const int e = threadIdx.x + PKB * m_;                                    \
+      const int i = e / PK_NBMAX, j = e % PK_NBMAX;                            \
+      __VA_ARGS__                                                              \
+    }                                                                          \
+  } else {                                                                     \
+    _Pragma("unroll") for (int m_ = 0; m_ < PK_EPL; ++m_) {                    \
+      const int e = threadIdx.x + PKB * m_;                                    \
+      if (e < (nb) * (nb)) {                                                   \
+        const int i = e / (nb), j = e - i * (nb);                              \
+        __VA_ARGS__                                                            \
+      }                                                                        \
+    }                                                                          \
   }
 
 template <int NB, typename T>
@@ -946,6 +959,23 @@ struct PkInputs {
 
 // coef_dev (optional): per-solve coefficients, coef_dev[s * PK_XMAX + v] for the
 // solve s = k / seg_len of a stack of solves (the batched polish).
+// sX = sum_v c_v X_v[k] with the input count a compile-time constant, so that
+// the loads of every input go out together
+template <int NB, typename T, int NIN>
+__device__ __forceinline__ void extrap_sum(const PkInputs& in, const double* sc, size_t k, int nb, cx<T>* sX) {
+  const size_t nb2 = (size_t)nb * nb;
+  cx<T> acc[PK_EPL];
+#pragma unroll
+  for (int m_ = 0; m_ < PK_EPL; ++m_) acc[m_] = cx<T>{(T)0, (T)0};
+#pragma unroll
+  for (int v = 0; v < NIN; ++v) {
+    const cx<T>* Xv = reinterpret_cast<const cx<T>*>(in.x[v]);
+    const T cv = (T)sc[v];
+    PK_FOR_ELEM(nb, { acc[m_] = acc[m_] + cv * Xv[k * nb2 + e]; })
+  }
+  PK_FOR_ELEM(nb, { sX[i * PK_PITCH + j] = acc[m_]; })
+}
+
 template <int NB, typename T>
 __global__ __launch_bounds__(PKB) void pk_extrap_rotate_kernel(int nb, int nin, PkInputs in,
                                                               const double* __restrict__ coef_dev, int seg_len,
@@ -956,11 +986,16 @@ __global__ __launch_bounds__(PKB) void pk_extrap_rotate_kernel(int nb, int nin, 
   if ((int)threadIdx.x < nin)
     sc[threadIdx.x] = coef_dev ? coef_dev[(k / (size_t)seg_len) * PK_XMAX + threadIdx.x] : in.c[threadIdx.x];
   __syncthreads();
-  PK_FOR_ELEM(nb, {
-    cx<T> a{0, 0};
-    for (int v = 0; v < nin; ++v) a = a + (T)sc[v] * reinterpret_cast<const cx<T>*>(in.x[v])[k * nb2 + e];
-    sX[i * PK_PITCH + j] = a;
-  })
+  switch (nin) {
+    case 1: extrap_sum<NB, T, 1>(in, sc, k, nb, sX); break;
+    case 2: extrap_sum<NB, T, 2>(in, sc, k, nb, sX); break;
+    case 3: extrap_sum<NB, T, 3>(in, sc, k, nb, sX); break;
+    case 4: extrap_sum<NB, T, 4>(in, sc, k, nb, sX); break;
+    case 5: extrap_sum<NB, T, 5>(in, sc, k, nb, sX); break;
+    case 6: extrap_sum<NB, T, 6>(in, sc, k, nb, sX); break;
+    case 7: extrap_sum<NB, T, 7>(in, sc, k, nb, sX); break;
+    default: extrap_sum<NB, T, 8>(in, sc, k, nb, sX); break;
+  }
   m_load<NB>(sU, U + k * nb2, nb);
   __syncthreads();
   PK_FOR_ELEM(nb, { sF[i * PK_PITCH + j] = (T)0.5 * (sX[i * PK_PITCH + j] + conj(sX[j * PK_PITCH + i])); })
@@ -1094,35 +1129,55 @@ __global__ __launch_bounds__(PKB) void pk_scf_resid16_kernel(size_t Nk, const cx
   double c2 = 0.0, d2 = 0.0, bd[PK_XMAX];
 #pragma unroll
   for (int v = 0; v < PK_XMAX; ++v) bd[v] = 0.0;
+  // this lane's four elements; each stream below issues its four loads
+  // together (one memory round trip per stored residual, not four)
+  size_t el[4];
+  cx<T> cv[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int row = Mfma<T>::row(l, r);
-    const size_t e = k * nb2 + (size_t)row * nb + col;
-    const cx<T> c{acc.re[r], acc.im[r]};
-    if (comm) {
-      if (rs.r32) reinterpret_cast<cx<float>*>(comm)[e] = cx<float>{(float)c.re, (float)c.im};
-      else comm[e] = c;
-    }
-    c2 += (double)c.re * c.re + (double)c.im * c.im;
-    if (Pn) {
-      const cx<T> d = Pn[e] - sD[row * PK_PITCH + col];
+    el[r] = k * nb2 + (size_t)Mfma<T>::row(l, r) * nb + col;
+    cv[r] = cx<T>{acc.re[r], acc.im[r]};
+    c2 += (double)cv[r].re * cv[r].re + (double)cv[r].im * cv[r].im;
+  }
+  if (Pn) {
+    cx<T> pn[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) pn[r] = Pn[el[r]];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const cx<T> d = pn[r] - sD[Mfma<T>::row(l, r) * PK_PITCH + col];
       d2 += (double)d.re * d.re + (double)d.im * d.im;
     }
+  }
 #pragma unroll
-    for (int v = 0; v < PK_XMAX; ++v)
-      if (v < rs.nr && v != rs.self) {
-        double qre, qim;
-        if (rs.r32) {
-          const cx<float> q = reinterpret_cast<const cx<float>*>(rs.r[v])[e];
-          qre = q.re;
-          qim = q.im;
-        } else {
-          const cx<T> q = reinterpret_cast<const cx<T>*>(rs.r[v])[e];
-          qre = (double)q.re;
-          qim = (double)q.im;
-        }
-        bd[v] += (double)c.re * qre + (double)c.im * qim;
+  for (int v = 0; v < PK_XMAX; ++v)
+    if (v < rs.nr && v != rs.self) {
+      double qre[4], qim[4];
+      if (rs.r32) {
+        const cx<float>* R = reinterpret_cast<const cx<float>*>(rs.r[v]);
+        cx<float> q[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) q[r] = R[el[r]];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) { qre[r] = q[r].re; qim[r] = q[r].im; }
+      } else {
+        const cx<T>* R = reinterpret_cast<const cx<T>*>(rs.r[v]);
+        cx<T> q[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) q[r] = R[el[r]];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) { qre[r] = (double)q[r].re; qim[r] = (double)q[r].im; }
       }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) bd[v] += (double)cv[r].re * qre[r] + (double)cv[r].im * qim[r];
+    }
+  // the new residual last: the stored history above is read before this slot is rewritten
+  if (comm) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (rs.r32) reinterpret_cast<cx<float>*>(comm)[el[r]] = cx<float>{(float)cv[r].re, (float)cv[r].im};
+      else comm[el[r]] = cv[r];
+    }
   }
   c2 = PK_SUM(c2);
   d2 = PK_SUM(d2);
