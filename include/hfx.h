@@ -253,6 +253,11 @@ int hfx_eigh_batched_tol(int dtype, int nb, size_t nk, const void* A, double* la
 int hfx_eigh_batched_basis(int dtype, int nb, size_t nk, const void* A, double* lam, void* U, int herm_in,
                            double rel_tol, void* stream);
 
+/* P = herm(Q diag(p) Q^dagger) and part[k] = sum |P[k] - Pref[k]|^2: the
+ * density of the orbitals (Q, p) of F[Pref] and its per-k density residual
+ * (reference_scf.py:287-294, the Roothaan step's |P_new - P|). */
+int hfx_density_diff(int dtype, int nb, size_t nk, const void* Q, const double* p, const void* Pref, void* P,
+                     double* part, void* stream);
 /* P = herm(Q diag(p) Q^dagger) (solver.py:42-44 + 462, utils.py:473-474);
  * when p is NULL, p = sigmoid((*mu - eps)/T) is formed (and stored to pout). */
 int hfx_density_from_Qp(int dtype, int nb, size_t nk, const void* Q, const double* p, const double* eps,

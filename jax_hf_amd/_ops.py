@@ -62,6 +62,13 @@ def eigh_basis(A, lam, U, herm_in=False, rel_tol=0.0):
            float(rel_tol), s(A.device))
 
 
+def density_diff(Q, p, Pref, P, part):
+    """P = herm(Q diag(p) Q^H) and part[k] = sum |P - Pref|^2 per k."""
+    nb = Q.shape[-1]
+    L.call("hfx_density_diff", dt_code(Q), nb, Q.numel() // (nb * nb), L.ptr(Q), L.ptr(p), L.ptr(Pref), L.ptr(P),
+           L.ptr(part), s(Q.device))
+
+
 def density(Q, P, p=None, eps=None, mu_ptr=None, T=1.0, pout=None):
     nk = Q.numel() // (Q.shape[-1] ** 2)
     L.call("hfx_density_from_Qp", dt_code(Q), Q.shape[-1], nk, L.ptr(Q), L.ptr(p), L.ptr(eps), mu_ptr,

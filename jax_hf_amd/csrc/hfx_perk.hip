@@ -372,12 +372,15 @@ __global__ __launch_bounds__(PKB) void pk_eigh_kernel(int nb, const cx<T>* __res
 
 // P = herm(Q diag(p) Q^dagger)  (solver.py:42-44, 462; utils.py:473-474)
 // p from pvec, or (when pvec is NULL) p = sigmoid((mu - eps)/T) and written to pout.
+// With Pref: part[k] = sum |P - Pref|^2 as well (the density residual of a
+// Roothaan step from Pref, reference_scf.py:291-294).
 template <int NB, typename T>
 __global__ __launch_bounds__(PKB) void pk_density_kernel(int nb, const cx<T>* __restrict__ Q,
                                                         const double* __restrict__ pvec,
                                                         const double* __restrict__ eps, const double* __restrict__ mu,
                                                         double T_, double* __restrict__ pout,
-                                                        cx<T>* __restrict__ P) {
+                                                        cx<T>* __restrict__ P, const cx<T>* __restrict__ Pref,
+                                                        double* __restrict__ part) {
   __shared__ cx<T> sQ[PK_MAT], sX[PK_MAT], sP[PK_MAT];
   __shared__ double sp[PK_NBMAX];
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
@@ -395,7 +398,19 @@ __global__ __launch_bounds__(PKB) void pk_density_kernel(int nb, const cx<T>* __
   PK_FOR_ELEM(nb, { sX[i * PK_PITCH + j] = (T)sp[j] * sQ[i * PK_PITCH + j]; })
   __syncthreads();
   m_gemm<NB, T, false, true>(sP, sX, sQ, nb);
-  PK_FOR_ELEM(nb, { P[k * nb2 + e] = (T)0.5 * (sP[i * PK_PITCH + j] + conj(sP[j * PK_PITCH + i])); })
+  double d2 = 0.0;
+  PK_FOR_ELEM(nb, {
+    const cx<T> v = (T)0.5 * (sP[i * PK_PITCH + j] + conj(sP[j * PK_PITCH + i]));
+    P[k * nb2 + e] = v;
+    if (Pref) {
+      const cx<T> d = v - Pref[k * nb2 + e];
+      d2 += (double)d.re * d.re + (double)d.im * d.im;
+    }
+  })
+  if (Pref) {
+    d2 = PK_SUM(d2);
+    if (threadIdx.x == 0) part[k] = d2;
+  }
 }
 
 // CG step 1 (solver.py:472-491): Ft = Q^dagger F Q, eps = Re diag Ft,
@@ -1448,10 +1463,27 @@ int hfx_density_from_Qp(int dtype, int nb, size_t nk, const void* Q, const doubl
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
     PK_LAUNCH(pk_density_kernel, double, nb, PKC(double, Q), p, eps, mu, T, pout,
-                       PKM(double, P));
+                       PKM(double, P), (const cx<double>*)nullptr, (double*)nullptr);
   else
     PK_LAUNCH(pk_density_kernel, float, nb, PKC(float, Q), p, eps, mu, T, pout,
-                       PKM(float, P));
+                       PKM(float, P), (const cx<float>*)nullptr, (double*)nullptr);
+  return check_launch("pk_density_kernel");
+}
+
+int hfx_density_diff(int dtype, int nb, size_t nk, const void* Q, const double* p, const void* Pref, void* P,
+                     double* part, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  if (!Q || !p || !Pref || !P || !part || P == Pref) return set_error(HFX_EINVAL, "hfx_density_diff: bad argument");
+  if (nk == 0) return HFX_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == HFX_DTYPE_C128)
+    PK_LAUNCH(pk_density_kernel, double, nb, PKC(double, Q), p, (const double*)nullptr, (const double*)nullptr, 1.0,
+              (double*)nullptr, PKM(double, P),
+              PKC(double, Pref), part);
+  else
+    PK_LAUNCH(pk_density_kernel, float, nb, PKC(float, Q), p, (const double*)nullptr, (const double*)nullptr, 1.0,
+              (double*)nullptr, PKM(float, P),
+              PKC(float, Pref), part);
   return check_launch("pk_density_kernel");
 }
 

@@ -502,6 +502,12 @@ class _DIISPolish:
         self._eigh_tol = 0.0
         self.io_host = torch.empty(self.io.shape, dtype=torch.float64, pin_memory=True)
         self.ev = torch.cuda.Event()
+        # the Roothaan check's own orbitals, occupations and mu state, and its readback
+        self.U_chk = z()
+        self.eps_chk, self.occ_chk = torch.zeros_like(self.eps), torch.zeros_like(self.eps)
+        self.st_chk = torch.zeros(K.ST_LEN, dtype=torch.float64, device=dev)
+        self.chk = torch.zeros(2, dtype=torch.float64, device=dev)
+        self.chk_host = torch.empty(2, dtype=torch.float64, pin_memory=True)
         self.Bd = torch.zeros((nh, nh), dtype=torch.float64, device=dev)   # DIIS B matrix
         self.coef = torch.zeros(8, dtype=torch.float64, device=dev)        # DIIS coefficients
         import ctypes as C
@@ -525,6 +531,36 @@ class _DIISPolish:
     # the previous mu one more converges; a solve that needs more is completed
     # on the next readback
     MU_GRAPH_STEPS = 3
+
+    # Once the commutator residual is below the tolerance while the step residual
+    # |P_k - P_{k-1}| (which lags one iteration) is not, test the reference's own
+    # density residual |density(F_k) - P_k| (reference_scf.py:287-294) directly:
+    # one warm eigh of F_k, whose orbitals are the returned state's when it passes,
+    # instead of a further Fock build, update and final eigh.
+    ROOTHAAN_CHECK = True
+
+    def _roothaan_check(self, Fk, Pk, T):
+        """|density(F_k) - P_k| (weighted, as the residuals) with the orbitals,
+        occupations and mu of F_k left in U_chk / occ_chk / st_chk."""
+        self.U_chk.copy_(self.U)
+        K.extrap_rotate(self.Ft, self.U_chk, [Fk], [1.0])
+        K.eigh_basis(self.Ft, self.eps_chk, self.U_chk, herm_in=True)
+        self.st_chk.copy_(self.st)
+        mu_kw = dict(mu0_from_state=True, st=self.st_chk, p_out=self.occ_chk)
+        K.solve_mu_steps(self.wsp, self.eps_chk, self.w, self.nk, self.nb, self.n_e, T,
+                         maxiter=self.MU_GRAPH_STEPS, **mu_kw)
+        self.chk[1:2].copy_(self.st_chk[K.ST_MUDONE:K.ST_MUDONE + 1])
+        self.chk_host.copy_(self.chk, non_blocking=True)
+        self.ev.record()
+        self.ev.synchronize()
+        if self.chk_host[1].item() == 0.0:        # mu needs more steps than the first pass
+            K.solve_mu_steps(self.wsp, self.eps_chk, self.w, self.nk, self.nb, self.n_e, T, maxiter=54, **mu_kw)
+        K.density_diff(self.U_chk, self.occ_chk, Pk, self.tmp, self.wsp.part)
+        K.reduce(self.wsp, self.wsp.part, 1, self.nk, w=self.w, wmask=1, out=self.chk[0:1])
+        self.chk_host.copy_(self.chk, non_blocking=True)
+        self.ev.record()
+        self.ev.synchronize()
+        return float(np.sqrt(self.chk_host[0].item() / self.ws_sum))
 
     def _mu(self, maxiter=54):
         K.solve_mu_steps(self.wsp, self.eps, self.w, self.nk, self.nb, self.n_e, self.kernel.T,
@@ -625,7 +661,7 @@ class _DIISPolish:
             steps = 2 if self.cdt == torch.complex128 and Qc.dtype != torch.complex128 else 1
             K.orthonormalize(self.U, steps)
         hE, hC, hD = [], [], []
-        k, converged, E, idx, redo = 0, False, float("nan"), 0, -1
+        k, converged, E, idx, redo, checked = 0, False, float("nan"), 0, -1, False
         P_res = self.P_cur
         self._eval(0)
         while True:
@@ -670,6 +706,13 @@ class _DIISPolish:
                 converged = True
                 P_res = Pk
                 break
+            if c_res <= tol and k > 0 and self.ROOTHAAN_CHECK:
+                d_ref = self._roothaan_check(self.F_hist[idx], Pk, self.kernel.T)
+                if d_ref <= tol:
+                    hD[-1] = d_ref
+                    converged, checked = True, True
+                    P_res = Pk
+                    break
             if k + 1 >= max_iter:
                 break
             if stall > 0.0 and len(hC) >= 4 and min(hC[-2:]) > stall * min(hC[:-2]):
@@ -682,6 +725,9 @@ class _DIISPolish:
         Ffin = self.F_hist[idx]           # F of the returned iterate (converged, or the last one)
         if not converged:
             P_res = self.P_cur
+        if checked:                       # the check's orbitals are those of F_k
+            return (P_res.clone(), Ffin.clone(), self.U_chk.clone(), self.occ_chk.clone(),
+                    self.st_chk[K.ST_MU].clone(), E, n_it, converged, hE, hC)
         # orbitals / occupations / mu of the returned state (solver.py:606-638 form)
         warm = k > 0 or warm0 or spec
         if warm:

@@ -689,6 +689,30 @@ def test_mixed_solve_mu_completion_paths(dev, monkeypatch):
     assert abs(float(np.sum(w[..., None] * _np(r.p))) - c["n_electrons"] / c["weights"].sum()) < 1e-10
 
 
+@pytest.mark.parametrize("tol", [1e-6, 1e-8])
+def test_mixed_solve_meets_reference_residuals(dev, tol):
+    """The converged mixed solve meets the reference SCF's own stopping test
+    at the returned state (reference_scf.py:287-294): the commutator residual
+    |F P - P F| and the density residual |density(F) - P|, both weighted as the
+    reference, within polish_tol -- recomputed here on the host from the
+    returned density, Fock matrix and orbitals (density(F) = Q diag(p) Q^H,
+    the eigen-decomposition of F, which the check also confirms)."""
+    import jax_hf_amd as J
+    c = syn.config1(nk=32, nb=16, seed=5)
+    G = J.HartreeFockKernel(c["weights"], c["hamiltonian"], c["coulomb_q"], 0.1)
+    P0 = np.zeros(c["hamiltonian"].shape, dtype=np.complex128)
+    r = J.solve(G, P0, 8.0, config=J.SolverConfig(max_iter=400, precision="mixed", polish_tol=tol))
+    assert r.converged
+    P, F, Q, p = _np(r.density), _np(r.fock), _np(r.Q), _np(r.p)
+    w = c["weights"] / c["weights"].sum()
+    wn = lambda X: float(np.sqrt(np.sum(w * np.sum(np.abs(X) ** 2, axis=(-2, -1)))))  # noqa: E731
+    eps = np.real(np.einsum("...ji,...jk,...ki->...i", np.conj(Q), F, Q))
+    assert np.abs(F @ Q - Q * eps[..., None, :]).max() < 1e-9 * np.abs(F).max()
+    Pn = np.einsum("...in,...n,...jn->...ij", Q, p, np.conj(Q))
+    assert wn(F @ P - P @ F) <= tol
+    assert wn(Pn - P) <= tol * (1 + 1e-6)
+
+
 def test_polish_budget_spent_returns_last_iterate(dev):
     """With a polish budget too small to converge, the mixed solve returns its
     last evaluated iterate: not converged, and energy / Fock matrix those of
