@@ -457,8 +457,10 @@ class _DIISPolish:
     One eigh per iteration: F_i = F[P_i] and the DIIS error e_i = F_i P_i -
     P_i F_i (reference_scf.py:292-314) give P_{k+1} = density(herm(sum_i c_i
     F_i)).  The reference loop also forms density(F_k) for its density
-    residual, a second eigh per iteration; here the density residual is the
-    step |P_k - P_{k-1}| the iteration itself takes.  The density update
+    residual, a second eigh per iteration; here the step |P_k - P_{k-1}| the
+    iteration itself takes stands in for it, and once the commutator residual
+    is within tolerance the reference's |density(F_k) - P_k| is formed directly
+    (_roothaan_check), so the stop is the reference's own test.  The density update
     (eigh of the Hermitised extrapolated Fock matrix, Newton mu warm-started
     at the previous mu, occupations, density) is one replayed HIP graph, and
     the object (buffers + graph) is kept on the kernel between solves.
