@@ -677,7 +677,9 @@ class _DIISPolish:
             if spec and len(hC) >= 2 and hC[-2] > 0 and 0 < hD[-2] < float("inf"):
                 pc = hC[-1] * hC[-1] / hC[-2]
                 pd = hD[-1] * hD[-1] / hD[-2]
-                spec = not (pc <= 2.0 * tol and pd <= 2.0 * tol)
+                # with the density-residual check, a predicted commutator stop
+                # suffices: the check then decides before any further update
+                spec = not (pc <= 2.0 * tol and (pd <= 2.0 * tol or self.ROOTHAAN_CHECK))
             if spec:
                 self._advance(k, warm=k > 0 or warm0)
             self.ev.synchronize()
