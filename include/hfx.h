@@ -248,10 +248,11 @@ int hfx_eigh_batched_tol(int dtype, int nb, size_t nk, const void* A, double* la
 /* The eigendecomposition of A = U^dagger F U started from the basis U, as
  * the reference's warm eigh of the rotated Fock matrix followed by the change
  * of basis (solver.py:622-625, reference_scf.py:87-96): (lam, W) = eigh(A)
- * to rel_tol (as hfx_eigh_batched_tol) and U <- U W in place, the rotations
- * accumulated onto U's rows (no W is formed).  A must not alias U. */
-int hfx_eigh_batched_basis(int dtype, int nb, size_t nk, const void* A, double* lam, void* U, int herm_in,
-                           double rel_tol, void* stream);
+ * to rel_tol (as hfx_eigh_batched_tol) and V = U W, the rotations
+ * accumulated onto U's rows (no W is formed); V may be U (in place).  A must
+ * not alias U or V. */
+int hfx_eigh_batched_basis(int dtype, int nb, size_t nk, const void* A, double* lam, const void* U, void* V,
+                           int herm_in, double rel_tol, void* stream);
 
 /* P = herm(Q diag(p) Q^dagger) and part[k] = sum |P[k] - Pref[k]|^2: the
  * density of the orbitals (Q, p) of F[Pref] and its per-k density residual

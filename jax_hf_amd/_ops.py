@@ -55,11 +55,12 @@ def eigh(A, lam, V, herm_in=False, rel_tol=0.0):
            float(rel_tol), s(A.device))
 
 
-def eigh_basis(A, lam, U, herm_in=False, rel_tol=0.0):
-    """Eigendecomposition of A = U^H F U from the basis U: U <- U W in place, lam ascending."""
+def eigh_basis(A, lam, U, herm_in=False, rel_tol=0.0, out=None):
+    """Eigendecomposition of A = U^H F U from the basis U: out = U W (default:
+    U in place), lam ascending."""
     nk = A.numel() // (A.shape[-1] * A.shape[-1])
-    L.call("hfx_eigh_batched_basis", dt_code(A), A.shape[-1], nk, L.ptr(A), L.ptr(lam), L.ptr(U), int(herm_in),
-           float(rel_tol), s(A.device))
+    L.call("hfx_eigh_batched_basis", dt_code(A), A.shape[-1], nk, L.ptr(A), L.ptr(lam), L.ptr(U),
+           L.ptr(U if out is None else out), int(herm_in), float(rel_tol), s(A.device))
 
 
 def density_diff(Q, p, Pref, P, part):

@@ -1447,12 +1447,12 @@ int hfx_eigh_batched_tol(int dtype, int nb, size_t nk, const void* A, double* la
   return eigh_any(dtype, nb, nk, A, lam, V, nullptr, herm_in, rel_tol, stream);
 }
 
-int hfx_eigh_batched_basis(int dtype, int nb, size_t nk, const void* A, double* lam, void* U, int herm_in,
-                           double rel_tol, void* stream) {
+int hfx_eigh_batched_basis(int dtype, int nb, size_t nk, const void* A, double* lam, const void* U, void* V,
+                           int herm_in, double rel_tol, void* stream) {
   if (int rc = pk_check_nb(nb)) return rc;
-  if (!A || !lam || !U || A == U || !(rel_tol >= 0.0) || rel_tol > 1e-3)
+  if (!A || !lam || !U || !V || A == U || A == V || !(rel_tol >= 0.0) || rel_tol > 1e-3)
     return set_error(HFX_EINVAL, "hfx_eigh_batched_basis: bad argument (rel_tol=%g)", rel_tol);
-  return eigh_any(dtype, nb, nk, A, lam, U, U, herm_in, rel_tol, stream);
+  return eigh_any(dtype, nb, nk, A, lam, V, U, herm_in, rel_tol, stream);
 }
 
 int hfx_density_from_Qp(int dtype, int nb, size_t nk, const void* Q, const double* p, const double* eps,

@@ -543,28 +543,25 @@ class _DIISPolish:
     # one warm eigh of F_k, whose orbitals are the returned state's when it passes,
     # instead of a further Fock build, update and final eigh.
     ROOTHAAN_CHECK = True
+    MU_CHECK_STEPS = 6                # mu steps of the check before its one readback
 
     def _roothaan_check(self, Fk, Pk, T):
         """|density(F_k) - P_k| (weighted, as the residuals) with the orbitals,
         occupations and mu of F_k left in U_chk / occ_chk / st_chk."""
-        self.U_chk.copy_(self.U)
-        K.extrap_rotate(self.Ft, self.U_chk, [Fk], [1.0])
-        K.eigh_basis(self.Ft, self.eps_chk, self.U_chk, herm_in=True)
+        K.extrap_rotate(self.Ft, self.U, [Fk], [1.0])
+        K.eigh_basis(self.Ft, self.eps_chk, self.U, herm_in=True, out=self.U_chk)
         self.st_chk.copy_(self.st)
         mu_kw = dict(mu0_from_state=True, st=self.st_chk, p_out=self.occ_chk)
-        K.solve_mu_steps(self.wsp, self.eps_chk, self.w, self.nk, self.nb, self.n_e, T,
-                         maxiter=self.MU_GRAPH_STEPS, **mu_kw)
-        self.chk[1:2].copy_(self.st_chk[K.ST_MUDONE:K.ST_MUDONE + 1])
-        self.chk_host.copy_(self.chk, non_blocking=True)
-        self.ev.record()
-        self.ev.synchronize()
-        if self.chk_host[1].item() == 0.0:        # mu needs more steps than the first pass
-            K.solve_mu_steps(self.wsp, self.eps_chk, self.w, self.nk, self.nb, self.n_e, T, maxiter=54, **mu_kw)
-        K.density_diff(self.U_chk, self.occ_chk, Pk, self.tmp, self.wsp.part)
-        K.reduce(self.wsp, self.wsp.part, 1, self.nk, w=self.w, wmask=1, out=self.chk[0:1])
-        self.chk_host.copy_(self.chk, non_blocking=True)
-        self.ev.record()
-        self.ev.synchronize()
+        for steps in (self.MU_CHECK_STEPS, 54):
+            K.solve_mu_steps(self.wsp, self.eps_chk, self.w, self.nk, self.nb, self.n_e, T, maxiter=steps, **mu_kw)
+            K.density_diff(self.U_chk, self.occ_chk, Pk, self.tmp, self.wsp.part)
+            K.reduce(self.wsp, self.wsp.part, 1, self.nk, w=self.w, wmask=1, out=self.chk[0:1])
+            self.chk[1:2].copy_(self.st_chk[K.ST_MUDONE:K.ST_MUDONE + 1])
+            self.chk_host.copy_(self.chk, non_blocking=True)
+            self.ev.record()
+            self.ev.synchronize()
+            if self.chk_host[1].item() != 0.0:    # else mu needed more steps: finish it, once
+                break
         return float(np.sqrt(self.chk_host[0].item() / self.ws_sum))
 
     def _mu(self, maxiter=54):

@@ -629,7 +629,11 @@ def test_eigh_basis(dev, dt, nb, warm):
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev).to(cdt)   # noqa: E731
     At, Ut = t(A), t(U)
     lam = torch.empty((nk, nb), dtype=torch.float64, device=dev)
-    K.eigh_basis(At, lam, Ut, herm_in=True)
+    Vo = torch.empty_like(Ut)
+    K.eigh_basis(At, lam, Ut, herm_in=True, out=Vo)             # out of place: U unchanged
+    assert torch.equal(Ut, t(U))
+    K.eigh_basis(At, lam, Ut, herm_in=True)                     # in place: the same V
+    assert torch.equal(Ut, Vo)
     Un, ln = _np(Ut).astype(np.complex128), _np(lam)
     lref = np.linalg.eigvalsh(F)
     sc = np.abs(lref).max()
