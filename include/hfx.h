@@ -197,6 +197,10 @@ int hfx_scale_k_ex(size_t nk, int nb, int kind, const void* X, const double* w, 
 
 /* Y = 0.5 (X + X^dagger) over nmat nb x nb matrices (fock.py:13-15, 95-96). */
 int hfx_hermitize(const void* X, void* Y, size_t nmat, int nb, int dtype, void* stream);
+/* hfx_hermitize from X of x_dtype into Y of y_dtype (HFX_C128 / HFX_C64): the
+ * Hermitised start density of solver.py:380-394 in the loop's precision, one
+ * pass.  Y must not alias X. */
+int hfx_hermitize_convert(const void* X, int x_dtype, void* Y, int y_dtype, size_t nmat, int nb, void* stream);
 
 /* out[0] = max|X|, out[1] = max|X| over channels with mask[c] != 0; the
  * off-block check of the block-spec exchange (utils.py:263-266, 282, 295).
@@ -373,6 +377,9 @@ int hfx_superlattice_fock(int dtype, int nkx, int nky, int n_G, int dim_orb, int
  * [1, 4] times U <- U (3 I - U^dagger U) / 2 (each step squares the deviation
  * from unitarity; the CG orbitals handed to the FP64 polish, solver.py:42-44). */
 int hfx_orthonormalize(int dtype, int nb, size_t nk, void* U, int steps, void* stream);
+/* hfx_orthonormalize of complex64 orbitals Q32 into complex128 U (converted on
+ * the way in): the FP32 CG orbitals that start the FP64 polish basis. */
+int hfx_orthonormalize_from(int nb, size_t nk, const void* Q32, void* U, int steps, void* stream);
 
 /* SCF residuals (reference_scf.py:291-294): comm = F D - D F (stored if comm),
  * part[0][k] = sum |comm|^2, part[1][k] = sum |Pn - D|^2 (Pn may be NULL). */

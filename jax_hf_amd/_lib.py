@@ -21,7 +21,7 @@ LIB_PATH = _HERE / "libhfx.so"
 
 HFX_C128 = 0
 HFX_C64 = 1
-ABI_VERSION = 21
+ABI_VERSION = 23
 # Largest band count of the per-k solver kernels (nb <= 16: one wavefront per
 # k-point, nb <= 32: a 256-thread workgroup per k-point).
 PERK_NB_MAX = 32
@@ -57,6 +57,7 @@ _SIGS = {
     "hfx_hf_energy": [vp, vp, vp, vp, vp, i32, vp, vp, vp, vp],
     "hfx_convert": [vp, i32, vp, i32, sz, vp],
     "hfx_hermitize": [vp, vp, sz, i32, i32, vp],
+    "hfx_hermitize_convert": [vp, i32, vp, i32, sz, i32, vp],
     "hfx_lincomb": [sz, i32, f64, f64, vp, f64, f64, vp, vp, vp],
     "hfx_fermi": [vp, vp, f64, sz, vp, vp],
     "hfx_lincomb_multi": [sz, i32, i32, i32, vp, vp, i32, vp, vp],
@@ -92,6 +93,7 @@ _SIGS = {
     "hfx_scf_resid": [i32, i32, sz, vp, vp, vp, vp, vp, vp],
     "hfx_scf_resid_dots": [i32, i32, sz, vp, vp, vp, vp, vp, i32, i32, i32, vp, vp],
     "hfx_orthonormalize": [i32, i32, sz, vp, i32, vp],
+    "hfx_orthonormalize_from": [i32, sz, vp, vp, i32, vp],
     "hfx_symmetry_average": [i32, i32, sz, i32, vp, vp, vp, C.c_ulonglong, vp, f64, vp, vp],
     "hfx_resample_kgrid": [i32, i32, i32, sz, i32, vp, vp, vp],
     "hfx_superlattice_fock": [i32, i32, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp,

@@ -1047,11 +1047,21 @@ __global__ __launch_bounds__(PKB) void pk_rotate_density_kernel(int nb, cx<T>* _
 // Newton-Schulz re-orthonormalisation in place, `steps` times:
 // U <- U (3 I - U^dagger U) / 2.  Each step squares the deviation of U^dagger U
 // from I: an orbital basis carried in FP32 (~1e-7) reaches FP64 rounding in two.
-template <int NB, typename T>
-__global__ __launch_bounds__(PKB) void pk_orthonormalize_kernel(int nb, cx<T>* __restrict__ U, int steps) {
+// src (optional): start from these orbitals in precision TS instead of U's own
+// (the FP32 CG orbitals converted on the way in).
+template <int NB, typename T, typename TS = T>
+__global__ __launch_bounds__(PKB) void pk_orthonormalize_kernel(int nb, cx<T>* __restrict__ U, int steps,
+                                                               const cx<TS>* __restrict__ src) {
   __shared__ cx<T> sU[PK_MAT], sS[PK_MAT], sX[PK_MAT];
   const size_t k = blockIdx.x, nb2 = (size_t)nb * nb;
-  m_load<NB>(sU, U + k * nb2, nb);
+  if (src) {
+    PK_FOR_ELEM(nb, {
+      const cx<TS> v = src[k * nb2 + e];
+      sU[i * PK_PITCH + j] = cx<T>{(T)v.re, (T)v.im};
+    })
+  } else {
+    m_load<NB>(sU, U + k * nb2, nb);
+  }
   __syncthreads();
   for (int it = 0; it < steps; ++it) {
     m_gemm<NB, T, true, false>(sS, sU, sU, nb);     // S = U^H U
@@ -1733,9 +1743,24 @@ int hfx_orthonormalize(int dtype, int nb, size_t nk, void* U, int steps, void* s
   if (!U || steps < 1 || steps > 4) return set_error(HFX_EINVAL, "hfx_orthonormalize: bad argument (steps=%d)", steps);
   hipStream_t s = (hipStream_t)stream;
   if (dtype == HFX_DTYPE_C128)
-    PK_LAUNCH(pk_orthonormalize_kernel, double, nb, PKM(double, U), steps);
+    PK_LAUNCH(pk_orthonormalize_kernel, double, nb, PKM(double, U), steps, (const cx<double>*)nullptr);
   else
-    PK_LAUNCH(pk_orthonormalize_kernel, float, nb, PKM(float, U), steps);
+    PK_LAUNCH(pk_orthonormalize_kernel, float, nb, PKM(float, U), steps, (const cx<float>*)nullptr);
+  return check_launch("pk_orthonormalize_kernel");
+}
+
+int hfx_orthonormalize_from(int nb, size_t nk, const void* Q32, void* U, int steps, void* stream) {
+  if (int rc = pk_check_nb(nb)) return rc;
+  if (!Q32 || !U || steps < 1 || steps > 4)
+    return set_error(HFX_EINVAL, "hfx_orthonormalize_from: bad argument (steps=%d)", steps);
+  if (nk == 0) return HFX_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (nb <= 16)
+    hipLaunchKernelGGL((pk_orthonormalize_kernel<16, double, float>), dim3(nk), dim3(64), 0, s, nb, PKM(double, U),
+                       steps, PKC(float, Q32));
+  else
+    hipLaunchKernelGGL((pk_orthonormalize_kernel<32, double, float>), dim3(nk), dim3(256), 0, s, nb, PKM(double, U),
+                       steps, PKC(float, Q32));
   return check_launch("pk_orthonormalize_kernel");
 }
 

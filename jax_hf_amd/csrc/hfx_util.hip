@@ -38,14 +38,15 @@ __global__ void convert_kernel(const void* src, int sk, void* dst, int dk, size_
   }
 }
 
-template <typename T>
-__global__ void hermitize_kernel(const cx<T>* X, cx<T>* Y, size_t nmat, int nb) {
+template <typename T, typename TY = T>
+__global__ void hermitize_kernel(const cx<T>* X, cx<TY>* Y, size_t nmat, int nb) {
   const size_t nb2 = (size_t)nb * nb, total = nmat * nb2;
   for (size_t e = (size_t)blockIdx.x * UTIL_BLOCK + threadIdx.x; e < total; e += (size_t)gridDim.x * UTIL_BLOCK) {
     const size_t k = e / nb2;
     const int c = (int)(e - k * nb2), i = c / nb, j = c - i * nb;
     const cx<T> a = X[e], b = X[k * nb2 + (size_t)j * nb + i];
-    Y[e] = (T)0.5 * (a + conj(b));
+    const cx<T> h = (T)0.5 * (a + conj(b));
+    Y[e] = cx<TY>{(TY)h.re, (TY)h.im};
   }
 }
 
@@ -540,6 +541,25 @@ int hfx_hermitize(const void* X, void* Y, size_t nmat, int nb, int dtype, void* 
   else
     hipLaunchKernelGGL(hermitize_kernel<float>, dim3(grid_for(n)), dim3(UTIL_BLOCK), 0, (hipStream_t)stream,
                        (const cx<float>*)X, (cx<float>*)Y, nmat, nb);
+  return check_launch("hermitize_kernel");
+}
+
+int hfx_hermitize_convert(const void* X, int x_dtype, void* Y, int y_dtype, size_t nmat, int nb, void* stream) {
+  if (!X || !Y || X == Y || nb <= 0 || (x_dtype != HFX_DTYPE_C128 && x_dtype != HFX_DTYPE_C64) ||
+      (y_dtype != HFX_DTYPE_C128 && y_dtype != HFX_DTYPE_C64))
+    return set_error(HFX_EINVAL, "hfx_hermitize_convert: bad argument");
+  const size_t n = nmat * nb * nb;
+  if (n == 0) return HFX_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 g(grid_for(n)), b(UTIL_BLOCK);
+  if (x_dtype == HFX_DTYPE_C128 && y_dtype == HFX_DTYPE_C64)
+    hipLaunchKernelGGL((hermitize_kernel<double, float>), g, b, 0, s, (const cx<double>*)X, (cx<float>*)Y, nmat, nb);
+  else if (x_dtype == HFX_DTYPE_C64 && y_dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL((hermitize_kernel<float, double>), g, b, 0, s, (const cx<float>*)X, (cx<double>*)Y, nmat, nb);
+  else if (x_dtype == HFX_DTYPE_C128)
+    hipLaunchKernelGGL((hermitize_kernel<double>), g, b, 0, s, (const cx<double>*)X, (cx<double>*)Y, nmat, nb);
+  else
+    hipLaunchKernelGGL((hermitize_kernel<float>), g, b, 0, s, (const cx<float>*)X, (cx<float>*)Y, nmat, nb);
   return check_launch("hermitize_kernel");
 }
 

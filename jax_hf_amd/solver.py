@@ -259,7 +259,13 @@ class _CG:
     def init_from(self, P0):
         """solver.py:380-394: eigh(F[herm(P0)]) -> (Q, eps), mu, p."""
         P0h = torch.empty(self.shape4, dtype=self.cdt, device=self.dev)
-        K.hermitize(to_tensor(P0, self.dev, self.cdt), P0h)
+        if isinstance(P0, torch.Tensor) and P0.device == self.dev and P0.dtype in (torch.complex128, torch.complex64) \
+                and P0.dtype != self.cdt and tuple(P0.shape) == self.shape4:
+            P0c = P0.contiguous()      # hermitised and converted in one pass
+            L.call("hfx_hermitize_convert", L.ptr(P0c), K.dt_code(P0c), L.ptr(P0h), K.dt_code(P0h),
+                   P0c.numel() // (self.nb * self.nb), self.nb, L.stream(self.dev))
+        else:
+            K.hermitize(to_tensor(P0, self.dev, self.cdt), P0h)
         self.fock.build(P0h, self.F, project_fn=self.cfg.project_fn)
         K.eigh(self.F, self.eps, self.Q)
         self.mu(self.eps, False, 0.0)
@@ -631,10 +637,11 @@ class _DIISPolish:
             self._update(self.P_new, warm=False)
         self.P_prev, self.P_cur, self.P_new = self.P_cur, self.P_new, self.P_prev
 
-    def run(self, P_lo, mu0, tol, max_iter, Q_lo=None, stall=0.0):
+    def run(self, P_lo, mu0, tol, max_iter, Q_lo=None, stall=0.0, P_herm=False):
         """Q_lo (optional): the orbitals P_lo was built from.  Re-orthonormalised in
         FP64 (two Newton-Schulz steps), they are the first basis, so that the
-        first update is a warm eigh as well.
+        first update is a warm eigh as well.  P_herm: P_lo is exactly Hermitian
+        already (a density of orbitals), so it is only converted.
 
         Pipelined: the update from iterate k is launched before iterate k's
         residuals are back on the host (the DIIS coefficients are computed on
@@ -651,17 +658,24 @@ class _DIISPolish:
         et = self.EIGH_TOL_PER_TOL * float(tol)
         if et != self._eigh_tol:          # the captured updates hold the eigh tolerance
             self._eigh_tol, self.graphs = et, {}
-        K.hermitize(convert(P_lo, self.cdt) if P_lo.dtype != self.cdt else P_lo, self.P_cur)
+        if P_herm and P_lo.dtype != self.cdt:
+            Pc = P_lo.contiguous()
+            L.call("hfx_convert", L.ptr(Pc), KIND[Pc.dtype], L.ptr(self.P_cur), KIND[self.cdt], Pc.numel(),
+                   L.stream(self.dev))
+        else:
+            K.hermitize(convert(P_lo, self.cdt) if P_lo.dtype != self.cdt else P_lo, self.P_cur)
         self.st[K.ST_MU] = mu0
         warm0 = Q_lo is not None
         if warm0:
             Qc = Q_lo.contiguous()
-            L.call("hfx_convert", L.ptr(Qc), KIND[Qc.dtype], L.ptr(self.U), KIND[self.cdt], Qc.numel(),
-                   L.stream(self.dev))
             # Newton-Schulz to this precision's unitarity: two steps from complex64
-            # orbitals in FP64, one in complex64
-            steps = 2 if self.cdt == torch.complex128 and Qc.dtype != torch.complex128 else 1
-            K.orthonormalize(self.U, steps)
+            # orbitals in FP64 (converted inside the first), one in complex64
+            if self.cdt == torch.complex128 and Qc.dtype == torch.complex64:
+                L.call("hfx_orthonormalize_from", self.nb, self.nk, L.ptr(Qc), L.ptr(self.U), 2, L.stream(self.dev))
+            else:
+                L.call("hfx_convert", L.ptr(Qc), KIND[Qc.dtype], L.ptr(self.U), KIND[self.cdt], Qc.numel(),
+                       L.stream(self.dev))
+                K.orthonormalize(self.U, 2 if self.cdt == torch.complex128 and Qc.dtype != torch.complex128 else 1)
         hE, hC, hD = [], [], []
         k, converged, E, idx, redo, checked = 0, False, float("nan"), 0, -1, False
         P_res = self.P_cur
@@ -792,7 +806,7 @@ def _polish_diis(kernel, runner, n_electrons, cfg, fock_build_fn, k_cg, hist_E, 
             hG += list(hC32)
         pol = _polish_state(kernel, n_electrons)
         density, F, Q, p, mu, E, n2, conv, hE2, hC2 = pol.run(P_lo, mu_start, cfg.polish_tol, cfg.polish_max_iter,
-                                                              Q_lo=Q_lo)
+                                                              Q_lo=Q_lo, P_herm=not P_lo_given and cfg.project_fn is None)
         phases.append(n2)
         k = sum(phases)
         hE += list(hE2)
