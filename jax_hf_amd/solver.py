@@ -232,7 +232,8 @@ class _CG:
     # takes the first step; a step after convergence returns at once but still
     # costs a dispatch): the fourth-order steps converge from the previous mu in
     # one or two, and a solve that needs more is finished on the readback
-    MU_LOOP_STEPS = 4
+    # (96^2 x 16: no solve needed the readback completion with 1-4 steps here)
+    MU_LOOP_STEPS = 3
 
     # mu steps of an eager (not captured) solve before its done flag is read:
     # the remaining steps of mu_maxiter are launched only when it is not set
@@ -541,7 +542,7 @@ class _DIISPolish:
     # mu steps per mu solve inside the update graph after the bracket pass: from
     # the previous mu one more converges; a solve that needs more is completed
     # on the next readback
-    MU_GRAPH_STEPS = 3
+    MU_GRAPH_STEPS = 2
 
     # Once the commutator residual is below the tolerance while the step residual
     # |P_k - P_{k-1}| (which lags one iteration) is not, test the reference's own
@@ -549,7 +550,7 @@ class _DIISPolish:
     # one warm eigh of F_k, whose orbitals are the returned state's when it passes,
     # instead of a further Fock build, update and final eigh.
     ROOTHAAN_CHECK = True
-    MU_CHECK_STEPS = 6                # mu steps of the check before its one readback
+    MU_CHECK_STEPS = 4                # mu steps of the check before its one readback
 
     def _roothaan_check(self, Fk, Pk, T):
         """|density(F_k) - P_k| (weighted, as the residuals) with the orbitals,
