@@ -330,6 +330,7 @@ class _BatchCG:
 # state-vector slots the batched polish reduces into (unused by the CG layout)
 _SL_RES = 20    # [20] sum w |FP - PF|^2, [21] sum w |P - P_prev|^2
 _SL_B = 22      # [22 ...] the new row of the DIIS B matrix (= _SL_RES + 2: one reduction fills both)
+_SL_CHK = 28    # the density residual |density(F_k) - P_k|^2 of the check (in the check's state copy)
 
 
 class _BatchPolish:
@@ -344,6 +345,12 @@ class _BatchPolish:
 
     MU_STEPS = 4     # mu steps per mu solve in an update after the bracket pass; more are run on the readback
     EIGH_TOL_PER_TOL = 1e-5   # update eigh tolerance per unit of the polish tolerance (as in solve)
+    # the single polish's stopping test: once a solve's commutator residual is
+    # within the tolerance, its density residual |density(F_k) - P_k| (the
+    # reference's, reference_scf.py:287-294) is formed directly instead of
+    # waiting one iteration for the step residual
+    ROOTHAAN_CHECK = True
+    MU_CHECK_STEPS = 6
 
     def __init__(self, kernels, n_electrons, nh=6):
         k0 = kernels[0]
@@ -380,18 +387,41 @@ class _BatchPolish:
         self.Bd = torch.zeros((S, nh, nh), **r)
         self.st_host = torch.empty((S, K.ST_LEN), dtype=torch.float64, pin_memory=True)
         self.ev = torch.cuda.Event()
+        # the density-residual check's own orbitals, eigenvalues, occupations and state
+        self.U_chk = e()
+        self.eps_chk, self.occ_chk = torch.empty_like(self.eps), torch.empty_like(self.eps)
+        self.st_chk = torch.zeros_like(self.st)
+        self.chk_host = torch.empty((S, K.ST_LEN), dtype=torch.float64, pin_memory=True)
         import ctypes as C
         self._xa = (C.c_void_p * nh)(*[f.data_ptr() for f in self.F_hist])
 
-    def _reduce(self, nval, slot):
+    def _reduce(self, nval, slot, st=None):
         L.call("hfx_reduce_batched", L.ptr(self.part), int(nval), self.nk_tot, self.Nk, L.ptr(self.w_norm),
-               (1 << nval) - 1, L.ptr(self.st), K.OP_SUM, None, int(slot), L.ptr(self.scratch),
+               (1 << nval) - 1, L.ptr(self.st if st is None else st), K.OP_SUM, None, int(slot),
+               L.ptr(self.scratch), self.scratch.numel(), L.stream(self.dev))
+
+    def _mu(self, maxiter=54, eps=None, st=None, occ=None):
+        L.call("hfx_solve_mu_batched", L.ptr(self.eps if eps is None else eps), L.ptr(self.w_norm),
+               L.ptr(self.targets), self.nk_tot, self.Nk, self.nb, self.T, int(maxiter),
+               L.ptr(self.st if st is None else st), L.ptr(self.occ if occ is None else occ), L.ptr(self.scratch),
                self.scratch.numel(), L.stream(self.dev))
 
-    def _mu(self, maxiter=54):
-        L.call("hfx_solve_mu_batched", L.ptr(self.eps), L.ptr(self.w_norm), L.ptr(self.targets), self.nk_tot,
-               self.Nk, self.nb, self.T, int(maxiter), L.ptr(self.st), L.ptr(self.occ), L.ptr(self.scratch),
-               self.scratch.numel(), L.stream(self.dev))
+    def _check(self, idx, Pk):
+        """Per solve |density(F_k) - P_k| (w-weighted as the residuals), one
+        readback: the eigh of every solve's F_k in the running basis into U_chk,
+        mu and occupations in a copy of the state (frozen solves skip), the
+        density with its distance to P_k.  -> (squared residuals, mu done)."""
+        K.extrap_rotate(self.Ft, self.U, [self.F_hist[idx]], [1.0])
+        K.eigh_basis(self.Ft, self.eps_chk, self.U, herm_in=True, out=self.U_chk)
+        self.st_chk.copy_(self.st)
+        self._mu(self.MU_CHECK_STEPS, eps=self.eps_chk, st=self.st_chk, occ=self.occ_chk)
+        K.density_diff(self.U_chk, self.occ_chk, Pk, self.tmp, self.part)
+        self._reduce(1, _SL_CHK, st=self.st_chk)
+        self.chk_host.copy_(self.st_chk, non_blocking=True)
+        self.ev.record()
+        self.ev.synchronize()
+        sv = self.chk_host.numpy()
+        return sv[:, _SL_CHK].copy(), sv[:, K.ST_MUDONE] != 0.0
 
     def _eval(self, k):
         """F, E, residuals and B row of every solve's iterate k; asynchronous readback."""
@@ -466,7 +496,9 @@ class _BatchPolish:
                         continue
                     c, d = hC[s], hD[s]
                     if len(c) >= 2 and c[-2] > 0 and 0 < d[-2] < float("inf"):
-                        preds.append(c[-1] * c[-1] / c[-2] <= 2.0 * tol and d[-1] * d[-1] / d[-2] <= 2.0 * tol)
+                        # with the density-residual check a predicted commutator stop suffices
+                        preds.append(c[-1] * c[-1] / c[-2] <= 2.0 * tol
+                                     and (self.ROOTHAAN_CHECK or d[-1] * d[-1] / d[-2] <= 2.0 * tol))
                     else:
                         preds.append(False)
                 spec = not (preds and all(preds))
@@ -500,6 +532,16 @@ class _BatchPolish:
                     done[s] = True
                     self.Pfin[s].copy_(Pk[s])
                     self.Ffin[s].copy_(self.F_hist[idx][s])
+            ask = [s for s in range(S) if not done[s] and k > 0 and hC[s] and hC[s][-1] <= tol]
+            if self.ROOTHAAN_CHECK and ask and not bad_mu:
+                d2, mu_ok = self._check(idx, Pk)
+                for s in ask:
+                    d_ref = float(np.sqrt(max(d2[s], 0.0)))
+                    if mu_ok[s] and d_ref <= tol:
+                        hD[s][-1] = d_ref
+                        done[s] = True
+                        self.Pfin[s].copy_(Pk[s])
+                        self.Ffin[s].copy_(self.F_hist[idx][s])
             if all(done) or k + 1 >= max_iter:
                 break
             if not spec:
