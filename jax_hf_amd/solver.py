@@ -745,8 +745,14 @@ class _DIISPolish:
         if not converged:
             P_res = self.P_cur
         if checked:                       # the check's orbitals are those of F_k
-            return (P_res.clone(), Ffin.clone(), self.U_chk.clone(), self.occ_chk.clone(),
-                    self.st_chk[K.ST_MU].clone(), E, n_it, converged, hE, hC)
+            # the check's buffers and F_k's history slot are in no captured graph:
+            # hand them out and take fresh ones instead of copying
+            Q, p = self.U_chk, self.occ_chk
+            self.U_chk, self.occ_chk = torch.empty_like(Q), torch.empty_like(p)
+            self.F_hist[idx] = torch.empty_like(Ffin)
+            import ctypes as C
+            self._xa = (C.c_void_p * self.nh)(*[f.data_ptr() for f in self.F_hist])
+            return (P_res.clone(), Ffin, Q, p, self.st_chk[K.ST_MU].clone(), E, n_it, converged, hE, hC)
         # orbitals / occupations / mu of the returned state (solver.py:606-638 form)
         warm = k > 0 or warm0 or spec
         if warm:
