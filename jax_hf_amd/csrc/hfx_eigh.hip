@@ -212,6 +212,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void ei
   const T tol = rel_tol > (double)eps ? (T)rel_tol : eps;   // rotate while |A_pq| > tol ||A||_F
   const T thr2 = tol * tol * rs;
   for (int sweep = 0; sweep < 30; ++sweep) {
+    if (sweep > 0) {
+      // A sweep rotates exactly when some off-diagonal element is above the
+      // threshold at its start (a pair's element changes only through a
+      // rotation, and the first such pair in step order is reached unchanged):
+      // test that directly instead of running a closing sweep that rotates
+      // nothing.
+      T big = (T)0;
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        const T a2 = b[s].re * b[s].re + b[s].im * b[s].im;
+        big = (s == 0 && h == 0) ? big : fmax(big, a2);   // slot 0 of the first half is the diagonal
+      }
+      if (!__any(act && big > thr2)) break;
+    }
     int rot = 0;
     j16_sweep<T, P>(b, w, m, h, i, nb, act, thr2, rot, sh);
     if (!__any(rot)) break;
