@@ -472,6 +472,7 @@ int hfx_newton_prec(int dtype, int nb, size_t nk, const void* RX, const double* 
 #define HFX_OP_NT0 5
 #define HFX_OP_NT1 6
 #define HFX_OP_NT2 7
+#define HFX_OP_PARTIALS 8   /* hfx_reduce: the block partials only, summed by hfx_diis_coefficients_reduce */
 /* Scratch (float64) for hfx_reduce / hfx_solve_mu. */
 size_t hfx_solver_scratch_doubles(void);
 /* out[v] = sum_k (w[k] if bit v of wmask) part[v][k] in a fixed order; then the
@@ -573,6 +574,14 @@ int hfx_fock_epilogue_planar_batched(const hfx_plan* plan, int nsolve, const voi
  * coef[0..7]: the solution when count >= 2 and the solve is regular and
  * finite, else the plain step e_idx.  nh <= 8. */
 int hfx_diis_coefficients(const double* brow, double* B, int nh, int idx, int count, double* coef, void* stream);
+/* hfx_reduce's sum and hfx_diis_coefficients in one launch: the block partials
+ * that hfx_reduce(..., HFX_OP_PARTIALS, ..., scratch) left for nval values over
+ * nk per-k sums are summed in hfx_reduce's own order into out[0..nval)
+ * (bit-identical to the two-launch sum), and out[2 ..] is the new B row
+ * (the DIIS polish's residual pass: |FP - PF|^2, |P - P_prev|^2, then the
+ * row, reference_scf.py:110-121). */
+int hfx_diis_coefficients_reduce(const double* scratch, size_t nk, int nval, double* out, double* B, int nh, int idx,
+                                 int count, double* coef, void* stream);
 /* The same for nsolve stacked solves: solve s reads brow + s * brow_stride and
  * owns B + s * nh * nh and coef + s * 8 (the batched polish of a scan). */
 int hfx_diis_coefficients_batched(const double* brow, size_t brow_stride, double* B, int nsolve, int nh, int idx,

@@ -21,7 +21,7 @@ LIB_PATH = _HERE / "libhfx.so"
 
 HFX_C128 = 0
 HFX_C64 = 1
-ABI_VERSION = 23
+ABI_VERSION = 24
 # Largest band count of the per-k solver kernels (nb <= 16: one wavefront per
 # k-point, nb <= 32: a 256-thread workgroup per k-point).
 PERK_NB_MAX = 32
@@ -66,6 +66,7 @@ _SIGS = {
     "hfx_eigh_batched": [i32, i32, sz, vp, vp, vp, i32, vp],
     "hfx_eigh_batched_tol": [i32, i32, sz, vp, vp, vp, i32, f64, vp],
     "hfx_eigh_batched_basis": [i32, i32, sz, vp, vp, vp, vp, i32, f64, vp],
+    "hfx_diis_coefficients_reduce": [vp, sz, i32, vp, vp, i32, i32, i32, vp, vp],
     "hfx_density_diff": [i32, i32, sz, vp, vp, vp, vp, vp, vp],
     "hfx_density_from_Qp": [i32, i32, sz, vp, vp, vp, vp, f64, vp, vp, vp],
     "hfx_cg_grad": [i32, i32, sz, vp, vp, vp, vp, vp, vp, vp, vp],

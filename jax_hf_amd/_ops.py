@@ -13,6 +13,7 @@ ST_FROZEN = 16
 ST_LEN = 32
 OP_SUM, OP_GRAD, OP_BETA, OP_TAU0, OP_TRIAL = 0, 1, 2, 3, 4
 OP_NT0, OP_NT1, OP_NT2 = 5, 6, 7
+OP_PARTIALS = 8
 MU_NEWTON_BRACKET, MU_BISECTION, MU_NEWTON = 0, 1, 2
 
 

@@ -75,7 +75,7 @@ using namespace hfx;
 
 extern "C" {
 
-int hfx_abi_version(void) { return 23; }
+int hfx_abi_version(void) { return 24; }
 
 int hfx_last_error(char* buf, size_t n) {
   if (!buf || n == 0) return HFX_EINVAL;

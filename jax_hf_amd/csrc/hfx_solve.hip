@@ -862,9 +862,14 @@ __global__ __launch_bounds__(MUS_BLOCK) void mu_occ_seg_kernel(MuSegArgs a) {
 // coef[nh..7] = 0.
 constexpr int DIIS_MAX = 8;
 constexpr int DIIS_M = DIIS_MAX + 1;   // augmented size; rows past nh + 1 are identity padding
+// s1 (optional): first sum hfx_reduce's block partials of nval values over
+// nblk blocks in reduce_stage2's order (four 64-lane wave sums, then those in
+// order), write the totals to out, and take the B row from out + 2.
 __global__ __launch_bounds__(64) void diis_coef_kernel(const double* __restrict__ brow_all, size_t brow_stride,
                                                        double* __restrict__ B_all, int nsolve, int nh, int idx,
-                                                       int count, double* __restrict__ coef_all) {
+                                                       int count, double* __restrict__ coef_all,
+                                                       const double* __restrict__ s1, int nblk, int nval,
+                                                       double* __restrict__ out) {
   const int sidx = blockIdx.x;
   if (sidx >= nsolve) return;
   const double* brow = brow_all + (size_t)sidx * brow_stride;
@@ -872,7 +877,26 @@ __global__ __launch_bounds__(64) void diis_coef_kernel(const double* __restrict_
   double* coef = coef_all + (size_t)sidx * DIIS_MAX;
   const int r = threadIdx.x;
   const int N = nh;
-  const double br = r < nh ? brow[r] : 0.0;
+  double br;
+  if (s1) {
+    double mine = 0.0;   // lane v keeps total v
+    for (int v = 0; v < nval; ++v) {
+      double tot = 0.0;
+#pragma unroll
+      for (int wv = 0; wv < RED_BLOCK / 64; ++wv) {
+        const int b = wv * 64 + r;
+        double x = 0.0;
+        if (b < nblk) x += s1[(size_t)v * nblk + b];
+        tot += wave_sum(x);
+      }
+      if (r == v) mine = tot;
+      if (r == 0) out[v] = tot;
+    }
+    br = __shfl(mine, r + 2 < 64 ? r + 2 : 63);
+    if (r >= nh) br = 0.0;
+  } else {
+    br = r < nh ? brow[r] : 0.0;
+  }
   double a[DIIS_M];
 #pragma unroll
   for (int j = 0; j < DIIS_M; ++j) {
@@ -1231,7 +1255,8 @@ size_t hfx_solver_scratch_doubles(void) {
 int hfx_reduce(const double* part, int nval, size_t nk, const double* w, unsigned wmask, double* out, double* st,
                int op, const double* params, double* scratch, void* stream) {
   if (!part || !scratch || nval < 1 || nval > RED_MAX_VALS) return set_error(HFX_EINVAL, "hfx_reduce: bad argument");
-  if (op != HFX_OP_SUM && !st) return set_error(HFX_EINVAL, "hfx_reduce: op needs the state vector");
+  if (op != HFX_OP_SUM && op != HFX_OP_PARTIALS && !st)
+    return set_error(HFX_EINVAL, "hfx_reduce: op needs the state vector");
   hipStream_t s = (hipStream_t)stream;
   const int g = reduce_blocks(nk);
   PostArgs pa{};
@@ -1248,12 +1273,13 @@ int hfx_reduce(const double* part, int nval, size_t nk, const double* w, unsigne
     pa.omega0 = (int)params[8];
     pa.restart_every = (int)params[9];
   }
-  if (nk <= RED1_MAX_NK) {
+  if (nk <= RED1_MAX_NK && op != HFX_OP_PARTIALS) {
     hipLaunchKernelGGL(reduce_single, dim3(1), dim3(RED1_BLOCK), 0, s, part, nval, nk, w, wmask, out, st, pa);
     return check_launch("reduce_single");
   }
   hipLaunchKernelGGL(reduce_stage1, dim3(g), dim3(RED_BLOCK), 0, s, part, nval, nk, w, wmask, scratch, st, op);
   if (int rc = check_launch("reduce_stage1")) return rc;
+  if (op == HFX_OP_PARTIALS) return HFX_OK;
   hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(RED_BLOCK), 0, s, scratch, nval, g, out, st, pa);
   return check_launch("reduce_stage2");
 }
@@ -1325,7 +1351,18 @@ int hfx_diis_coefficients_batched(const double* brow, size_t brow_stride, double
       count > nh)
     return set_error(HFX_EINVAL, "hfx_diis_coefficients: bad argument (nh=%d idx=%d count=%d)", nh, idx, count);
   hipLaunchKernelGGL(diis_coef_kernel, dim3((unsigned)nsolve), dim3(64), 0, (hipStream_t)stream, brow, brow_stride, B,
-                     nsolve, nh, idx, count, coef);
+                     nsolve, nh, idx, count, coef, (const double*)nullptr, 0, 0, (double*)nullptr);
+  return check_launch("diis_coef_kernel");
+}
+
+int hfx_diis_coefficients_reduce(const double* scratch, size_t nk, int nval, double* out, double* B, int nh, int idx,
+                                 int count, double* coef, void* stream) {
+  if (!scratch || !out || !B || !coef || nk == 0 || nh < 1 || nh > DIIS_MAX || nval != nh + 2 || nval > 64 ||
+      idx < 0 || idx >= nh || count < 1 || count > nh)
+    return set_error(HFX_EINVAL, "hfx_diis_coefficients_reduce: bad argument (nval=%d nh=%d idx=%d count=%d)", nval,
+                     nh, idx, count);
+  hipLaunchKernelGGL(diis_coef_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, (const double*)nullptr, (size_t)0, B,
+                     1, nh, idx, count, coef, scratch, reduce_blocks(nk), nval, out);
   return check_launch("diis_coef_kernel");
 }
 

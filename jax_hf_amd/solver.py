@@ -606,21 +606,25 @@ class _DIISPolish:
         idx = k % nh
         Fk, Rk = self.F_hist[idx], self.R_hist[idx]
         self.fock.build(self.P_cur, Fk, E_ptr=self.outs.data_ptr() + 8 * 16)
-        # residuals and the B row (reference_scf.py:110-111) in one pass, one reduction
+        # residuals and the B row (reference_scf.py:110-111) in one pass; their
+        # block partials are summed by the device DIIS solve (reference_scf.py:298-314)
+        # of the update from iterate k, which writes the totals for the readback
         K.scf_resid_dots(Fk, self.P_cur, self.P_prev if k > 0 else None, Rk, self.R_hist, idx, part)
-        K.reduce(self.wsp, part, 2 + nh, self.nk, w=self.w, wmask=(1 << (2 + nh)) - 1, out=self.outs[0:2 + nh])
+        K.reduce(self.wsp, part, 2 + nh, self.nk, w=self.w, wmask=(1 << (2 + nh)) - 1, op=K.OP_PARTIALS)
+        L.call("hfx_diis_coefficients_reduce", L.ptr(self.wsp.scratch), self.nk, 2 + nh, L.ptr(self.outs), L.ptr(self.Bd),
+               nh, idx, min(k + 1, nh), L.ptr(self.coef), L.stream(self.dev))
         self.io_host.copy_(self.io, non_blocking=True)
         self.ev.record()
 
     def _advance(self, k, warm):
-        """Update from iterate k, launches only: DIIS coefficients on the device
-        (reference_scf.py:298-314), extrapolation in the basis U, eigh, mu,
-        density; then P_prev <- P_k, P_cur <- P_{k+1}."""
+        """Update from iterate k, launches only, with the DIIS coefficients
+        (reference_scf.py:298-314) that _eval(k) solved on the device:
+        extrapolation in the basis U, eigh, mu, density; then P_prev <- P_k,
+        P_cur <- P_{k+1}."""
         nh = self.nh
         idx = k % nh
         if warm:
-            count = min(k + 1, nh)           # active history slots 0 .. count-1
-            K.diis_coefficients(self.outs[2:2 + nh], self.Bd, idx, count, self.coef)
+            count = min(k + 1, nh)           # active history slots 0 .. count-1 (coefficients: _eval)
             import ctypes as C
             L.call("hfx_extrap_rotate_batched", K.dt_code(self.U), self.nb, self.nk, self.nk, count,
                    C.cast(self._xa, C.c_void_p), L.ptr(self.coef), L.ptr(self.U), L.ptr(self.Ft),
@@ -633,7 +637,6 @@ class _DIISPolish:
             else:
                 g.replay()
         else:                             # first update of a cold run: the plain step, eigh of herm(F_0)
-            K.diis_coefficients(self.outs[2:2 + nh], self.Bd, idx, 1, self.coef)
             self._extrapolate([self.F_hist[idx]], [1.0], False)
             self._update(self.P_new, warm=False)
         self.P_prev, self.P_cur, self.P_new = self.P_cur, self.P_new, self.P_prev

@@ -79,3 +79,42 @@ def test_orthonormalize_from_complex64(dev, nb):
     Un = _np(U1)
     assert np.abs(np.conj(np.swapaxes(Un, -1, -2)) @ Un - np.eye(nb)).max() < 1e-14
     assert np.abs(Un - Q).max() < 1e-6                                   # the polar factor of the rounded Q
+
+
+@pytest.mark.parametrize("nk", [300, 9216])
+def test_diis_coefficients_reduce_matches_two_launches(dev, nk):
+    """hfx_diis_coefficients_reduce (the residual pass's block partials summed
+    inside the device DIIS solve) gives the totals of hfx_reduce and the
+    coefficients of hfx_diis_coefficients on them -- bit for bit above 2048
+    k-points, where hfx_reduce itself takes the two-launch form (below it sums
+    in one workgroup, in another order)."""
+    import torch
+    from jax_hf_amd import _lib as L
+    from jax_hf_amd import _ops as K
+    rng = np.random.default_rng(nk)
+    nh = 6
+    nval = 2 + nh
+    part = torch.from_numpy(rng.uniform(0.1, 1.0, (nval, nk))).to(dev)
+    w = torch.from_numpy(rng.uniform(0.5, 1.0, nk)).to(dev)
+    ws = K.Workspace.get(dev, nk)
+    out1 = torch.zeros(nval, dtype=torch.float64, device=dev)
+    K.reduce(ws, part, nval, nk, w=w, wmask=(1 << nval) - 1, out=out1)
+    B1 = torch.from_numpy(rng.uniform(0.1, 1.0, (nh, nh))).to(dev)
+    B1 = 0.5 * (B1 + B1.T) + 5 * torch.eye(nh, dtype=torch.float64, device=dev)
+    B2 = B1.clone()
+    c1 = torch.zeros(8, dtype=torch.float64, device=dev)
+    c2 = torch.zeros(8, dtype=torch.float64, device=dev)
+    K.diis_coefficients(out1[2:], B1, 3, 5, c1)
+    K.reduce(ws, part, nval, nk, w=w, wmask=(1 << nval) - 1, op=K.OP_PARTIALS)
+    out2 = torch.zeros(nval, dtype=torch.float64, device=dev)
+    L.call("hfx_diis_coefficients_reduce", L.ptr(ws.scratch), nk, nval, L.ptr(out2), L.ptr(B2), nh, 3, 5, L.ptr(c2),
+           L.stream(dev))
+    if nk > 2048:
+        assert torch.equal(out1, out2)
+        assert torch.equal(B1, B2)
+        assert torch.equal(c1, c2)
+    else:
+        assert torch.allclose(out1, out2, rtol=1e-14, atol=0)
+        assert torch.allclose(c1, c2, rtol=1e-10, atol=1e-12)
+    ref = (_np(part) * _np(w)[None, :]).sum(axis=1)
+    assert np.abs(_np(out2) - ref).max() < 1e-12 * ref.max()
