@@ -29,6 +29,8 @@ projection) run through ``solve`` one by one.
 
 from __future__ import annotations
 
+from dataclasses import replace
+
 import numpy as np
 import torch
 
@@ -636,7 +638,8 @@ def _group_mixed(kernels, members, P0s, n_electrons, cfg, results, caller):
     """FP32 stacked CG then the FP64 stacked polish of one mesh group (steps)."""
     ks = [kernels[i] for i in members]
     ne = [n_electrons[i] for i in members]
-    b = _BatchCG(ks, torch.complex64, cfg)
+    # the FP32 phase's line search as in solve: at most mixed_bt_max trials
+    b = _BatchCG(ks, torch.complex64, replace(cfg, bt_max=min(cfg.bt_max, max(int(cfg.mixed_bt_max), 1))))
     b.set_targets(ne)
     b.init_from([P0s[i] for i in members])
     k_of, grad, hE, hG = yield from b.run_steps(cfg.mixed_tol_E, 0.0, max(cfg.max_iter // 2, 1), rel_tol=True)

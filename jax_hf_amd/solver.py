@@ -22,7 +22,7 @@ from __future__ import annotations
 
 import warnings
 import weakref
-from dataclasses import dataclass
+from dataclasses import dataclass, replace
 from typing import Any, NamedTuple
 
 import numpy as np
@@ -77,6 +77,12 @@ class SolverConfig:
     # to residuals <= this (or until its rounding floor stops it), so that the
     # FP64 part is a few refinement builds; 0 skips the stage
     polish32_tol: float = 0.0
+    # mixed form: line-search trials of the FP32 CG phase (at most bt_max; when
+    # none is accepted the fallback step is the next shorter one, as after bt_max
+    # trials).  The captured iteration launches every trial, and the ones after
+    # an accepted trial are ~10 us no-ops each (the trial, its two reduction
+    # launches); the FP32 phase's first trial is the accepted one as a rule
+    mixed_bt_max: int = 2
 
 
 class SolveResult(NamedTuple):
@@ -444,7 +450,8 @@ def _solve_cg(kernel, P0, n_electrons, cfg: SolverConfig, fock_build_fn):
     if polish not in (None, "diis"):
         raise ValueError(f"polish must be None or 'diis'; got {cfg.polish!r}")
     cdt = torch.complex128 if prec == "fp64" else torch.complex64
-    runner = _cg_runner(kernel, cdt, cfg, fock_build_fn, n_electrons)
+    cfg_cg = replace(cfg, bt_max=min(cfg.bt_max, max(int(cfg.mixed_bt_max), 1))) if prec == "mixed" else cfg
+    runner = _cg_runner(kernel, cdt, cfg_cg, fock_build_fn, n_electrons)
     runner.set_target(n_electrons)
     runner.init_from(P0)
     if prec == "mixed":
