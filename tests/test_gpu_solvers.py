@@ -348,6 +348,25 @@ def test_mixed_complex64_diis_stage(dev, nk, nb):
     assert np.abs(_np(r.density) - np.asarray(r0.density_matrix)).max() < 1e-6
 
 
+@pytest.mark.parametrize("bt", [1, 8])
+def test_mixed_fp32_line_search_cap(dev, bt):
+    """The mixed form's FP32 phase with at most `mixed_bt_max` line-search
+    trials (1: the blind fallback step whenever the first trial is rejected;
+    8: the reference's bt_max) reaches the oracle's FP64 fixed point either
+    way (|dE| < 1e-8, density 1e-6)."""
+    import jax_hf_amd as J
+    c = syn.config0(nk=24, nb=4, seed=1)
+    G = J.HartreeFockKernel(c["weights"], c["hamiltonian"], c["coulomb_q"], c["T"])
+    K0 = O.OracleKernel(c["weights"], c["hamiltonian"], c["coulomb_q"], c["T"])
+    r0 = O.solve_scf(K0, c["P0"], c["n_electrons"],
+                     config=O.SCFConfig(max_iter=300, acceleration="diis", density_tol=1e-11, comm_tol=1e-11))
+    r = J.solve(G, c["P0"], c["n_electrons"], config=J.SolverConfig(max_iter=400, precision="mixed",
+                                                                       polish_tol=1e-9, mixed_bt_max=bt))
+    assert r.converged
+    assert abs(float(r.energy) - r0.energy) < 1e-8
+    assert np.abs(_np(r.density) - np.asarray(r0.density_matrix)).max() < 1e-6
+
+
 def _contact_kernels(nk=2, g=0.3, T=0.05):
     import jax_hf_amd as J
     h = np.zeros((nk, nk, 2, 2), np.complex128)
